@@ -1,0 +1,172 @@
+/*
+ * mpdugks.h - C ABI of the MI355X DUGKS time-step library (libmpdugks_hip.so).
+ *
+ * The library replaces the body of the reference's `DUGKS2DSolver()` time loop
+ * (Ark/Cpp/DUGKS2DSolver.cpp:164-328) for a uniform Cartesian mesh: the host
+ * driver keeps mesh set-up, initial conditions and output, uploads the state
+ * once, advances it on the GPU and downloads it at output cadence.
+ *
+ * Conventions
+ *  - Structured cell index  n = j*nx + i  (i along x, j along y), i.e. the
+ *    reference's CarCellArray[i+1][j+1] (Ark/Cpp/Mesh/MeshCartesian.cpp:369-408).
+ *  - Populations are planes: tilde[q*ncell + n], q = 0..Q-1 in the reference's
+ *    velocity order (D2Q9: Ark/Lib/D2Q9.h:24-26; D2V16: Ark/Lib/D2V16.cpp:54-67).
+ *  - Cell macros are planes macro[m*ncell + n], m in mpd_macro_field below; the
+ *    fields are the reference's MacroQuantity members (Ark/Header/Macroscopic.h:32-44)
+ *    plus the per-cell relaxation time f.tau (Ark/Header/Cell_2D.h:114).
+ *  - Every entry point returns 0 on success or a negative mpd_status; nothing
+ *    blocks, aborts or throws across the ABI.  mpd_last_error() gives the text.
+ *  - A handle is not thread-safe; calls on one handle must come from one thread.
+ */
+#ifndef MPDUGKS_H
+#define MPDUGKS_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MPDUGKS_ABI_VERSION 1
+
+typedef enum mpd_status {
+    MPD_OK = 0,
+    MPD_ERR_INVALID = -1,     /* bad argument or unsupported combination */
+    MPD_ERR_DEVICE = -2,      /* HIP runtime error */
+    MPD_ERR_NONFINITE = -3,   /* NaN/Inf or non-positive density detected */
+    MPD_ERR_NOMEM = -4,       /* device allocation failed / over HBM budget */
+    MPD_ERR_STATE = -5        /* call not valid in the handle's current state */
+} mpd_status;
+
+/* Velocity model: the reference's link-time `QModel` choice (Ark/Makefile:5). */
+typedef enum mpd_model {
+    MPD_MODEL_AC_P = 0,      /* Ark/Lib/D2Q9AC.cpp  : Allen-Cahn + pressure form, U == 0 */
+    MPD_MODEL_AC_HCZ = 1,    /* Ark/Lib/D2Q9HCZ.cpp : Allen-Cahn + HCZ momentum (default) */
+    MPD_MODEL_PP_D2Q9 = 2,   /* Ark/Lib/D2Q9PP.cpp  : pseudopotential, Carnahan-Starling */
+    MPD_MODEL_PP_D2V16 = 3   /* Ark/Lib/D2V16.cpp   : 16-velocity set with pseudopotential */
+} mpd_model;
+
+/* Face reconstruction: Ark/Constant/ZeroFlip.h:27-35. */
+typedef enum mpd_flux { MPD_FLUX_CD = 0, MPD_FLUX_UW = 1 } mpd_flux;
+
+/* Boundary in y (x is always periodic): Ark/Constant/ZeroFlip.h:59-77. */
+typedef enum mpd_bc { MPD_BC_PERIODIC = 0, MPD_BC_WALL_BB = 1 } mpd_bc;
+
+/* Arithmetic of the bulk update. */
+typedef enum mpd_precision {
+    MPD_FP64 = 0,            /* plain FP64 drop-in */
+    MPD_FP32_BULK = 1        /* FP32 populations/arithmetic, FP64 EOS and global sums */
+} mpd_precision;
+
+/* Run-time replacement for the compile-time constants of
+ * Ark/Constant/ZeroReference.h and Ark/Constant/ZeroNamespace.h. */
+typedef struct mpd_config {
+    int32_t nx, ny;
+    int32_t model;           /* mpd_model */
+    int32_t flux;            /* mpd_flux */
+    int32_t bc_y;            /* mpd_bc */
+    int32_t precision;       /* mpd_precision */
+    double dx;               /* MinL (cell side)                        ZeroReference.h:20 */
+    double dt;               /* CFL*MinL/MaxU                           ZeroNamespace.h:10-14 */
+    double rt;               /* R0*T0                                   ZeroReference.h:70-74 */
+    double phi_l, phi_v;     /* PhiL, PhiV                              ZeroNamespace.h:79 */
+    double rho_l, rho_v;     /* RhoL, RhoV                              ZeroNamespace.h:81 */
+    double nu_l, nu_v;       /* NuL, NuV (kinematic)                    ZeroNamespace.h:89 */
+    double mobility;         /* M_Phi; tau_h = M/RT                     ZeroNamespace.h:63,97 */
+    double width;            /* interface width wI                      ZeroNamespace.h:69 */
+    double sigma;            /* surface tension                         ZeroNamespace.h:71 */
+    double tau0;             /* Tau0 used by the AC-p viscosity         ZeroReference.h:100 */
+    double gx, gy, rho_ref;  /* body force g*(rho - rho_ref); 0 in the reference (SourceM.cpp:32-33) */
+    double pp_tr;            /* T/Tc of the Carnahan-Starling EOS       ZeroNamespace.h:142 */
+    double pp_k;             /* EOS prefactor K                         ZeroNamespace.h:140 */
+    double pp_tau;           /* constant relaxation time of the PP model (Preprocess.cpp:376,397) */
+} mpd_config;
+
+/* Cell macro planes (order of the `macro` arrays). */
+typedef enum mpd_macro_field {
+    MPD_M_PHI = 0,   /* Phi */
+    MPD_M_RHO,       /* Rho */
+    MPD_M_U,         /* U */
+    MPD_M_V,         /* V */
+    MPD_M_P,         /* p */
+    MPD_M_PHIX,      /* Phi_x: grad(phi) inside MacroSource, then the AC source vector */
+    MPD_M_PHIY,      /* Phi_y */
+    MPD_M_RHOX,      /* Rho_x */
+    MPD_M_RHOY,      /* Rho_y */
+    MPD_M_FX,        /* Fx */
+    MPD_M_FY,        /* Fy */
+    MPD_M_LAP,       /* laplacianPhi */
+    MPD_M_PPU,       /* prevPhiU */
+    MPD_M_PPV,       /* prevPhiV */
+    MPD_M_PSI,       /* Psi (pseudopotential) */
+    MPD_M_SCG,       /* scG = sign(p - rho*RT) */
+    MPD_M_MU,        /* Mu */
+    MPD_M_TAU,       /* f.tau of the cell */
+    MPD_M_COUNT
+} mpd_macro_field;
+
+/* Global moments returned by mpd_reductions (FP64, compensated sums). */
+typedef struct mpd_sums {
+    double sum_rho;          /* sum of rho over cells */
+    double sum_rho_u;        /* sum of rho*u */
+    double sum_rho_v;        /* sum of rho*v */
+    double sum_phi;          /* sum of phi (AC models) */
+    double kinetic_energy;   /* 0.5 * sum rho*(u^2+v^2) */
+    double max_speed;        /* max |u| */
+    double residual;         /* sqrt(sum (rho-rho_1k)^2 / (sum rho^2 + 1e-30)); updates rho_1k
+                                (Ark/Cpp/DUGKS2DSolver.cpp:807-820) when update_residual != 0 */
+    int64_t nonfinite;       /* cells with NaN/Inf or rho <= 0 */
+} mpd_sums;
+
+typedef struct mpd_handle mpd_handle;
+
+/* Library ABI version (MPDUGKS_ABI_VERSION). */
+int mpd_abi_version(void);
+
+/* Fill `cfg` with the reference's shipped constants for `model`
+ * (128x128, dt = 0.5, RT = 1/3, sigma = 1e-2, W = 5, ...). */
+int mpd_default_config(int32_t model, mpd_config *cfg);
+
+/* Create a solver on the current HIP device.  Replaces the allocation in
+ * AllocateResource (Ark/Cpp/main.cpp:150-186) for the step's state. */
+int mpd_create(const mpd_config *cfg, mpd_handle **out);
+int mpd_destroy(mpd_handle *h);
+
+/* Upload host state (FP64).  tilde_h may be NULL for the PP models;
+ * face_fx/face_fy (PP only, may be NULL otherwise) are the persistent face
+ * forces of the x-faces (face i,j between cells (i-1,j),(i,j)) followed by the
+ * y-faces (face i,j between (i,j-1),(i,j)), 2*ncell each.
+ * Mirrors the reference state consumed by the loop: Cell_2D::{h,f}.Tilde and
+ * MacroQuantity (Ark/Header/Cell_2D.h:106-149). */
+int mpd_upload_state(mpd_handle *h, const double *tilde_h, const double *tilde_f,
+                     const double *macro, const double *face_fx, const double *face_fy);
+
+/* Download the state in the same layout (any pointer may be NULL to skip). */
+int mpd_download_state(mpd_handle *h, double *tilde_h, double *tilde_f, double *macro,
+                       double *face_fx, double *face_fy);
+
+/* Advance nsteps time steps (the `while` body, DUGKS2DSolver.cpp:164-328).
+ * Asynchronous on the handle's stream. */
+int mpd_step(mpd_handle *h, int64_t nsteps);
+
+/* Global sums (SumRho / Residual, DUGKS2DSolver.cpp:793-879).  Synchronises. */
+int mpd_reductions(mpd_handle *h, int update_residual, mpd_sums *out);
+
+/* Wait for all work queued on the handle. */
+int mpd_synchronize(mpd_handle *h);
+
+/* Bytes of device memory the handle owns. */
+int mpd_device_bytes(const mpd_handle *h, int64_t *bytes);
+
+/* Average duration (ms) of the step kernel launches since the last call,
+ * measured with HIP events on the handle's stream; resets the counters. */
+int mpd_kernel_timing(mpd_handle *h, double *avg_ms, int64_t *launches);
+
+/* Text of the last error on this thread. */
+const char *mpd_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* MPDUGKS_H */

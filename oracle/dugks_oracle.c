@@ -1,0 +1,813 @@
+/*
+ * CPU FP64 restatement of the reference's 2D DUGKS time step - THE ORACLE.
+ *
+ * Test infrastructure only.  Nothing in the product links, loads or calls
+ * this file; it is the checker used by tests/, by __graft_entry__.smoke() and
+ * by bench.py's CPU-baseline leg.
+ *
+ * It restates `DUGKS2DSolver()`'s loop body (Ark/Cpp/DUGKS2DSolver.cpp:164-328)
+ * on a structured nx*ny grid, keeping the reference's phase order and its
+ * floating-point operation order so that it tracks the reference to round-off:
+ *
+ *   phase 1  Update_phi_Eq / Update_DVDF_Source / Update_phi_BP      (:175-183)
+ *   phase 2  WallShadowC_fBP (walls only)                            (:185-189)
+ *   phase 3  Grad_VS_6points (UW only)                               (:197-206)
+ *   phase 4  Flux_2D on every face (+ Wall_3_BB on wall faces)       (:219-250)
+ *   phase 6  Update_phiFlux_h                                        (:269-273)
+ *   phase 8  Update_phi_T                                            (:280-282)
+ *   phase 9  Update_MacroVar                                         (:284-288)
+ *   phase 10 MacroSource (Allen-Cahn)                                (:289-295)
+ *   phase 11 Update_PseudoForce(cell) (pseudopotential)              (:297-303)
+ *
+ * Face convention: x-face (i,j) is the west face of cell (i,j) with normal +x,
+ * owner (i-1,j), neighbour (i,j); y-face (i,j) is the south face of cell (i,j)
+ * with normal +y, owner (i,j-1).  The reference stores each periodic boundary
+ * face twice (zones 12 and 8) with opposite normals; both copies evaluate to
+ * the same bits as the single structured face here because every operation
+ * that differs is an exact sign flip (see face_recon()).
+ *
+ * One deliberate difference: the pseudopotential face force.  The reference
+ * writes each face's Psi and reads its neighbour faces' Psi inside the same
+ * OpenMP loop (Ark/Cpp/DUGKS2DSolver.cpp:219-230, 733-746), so its result
+ * depends on the face visiting order.  Here all face Psi are computed first
+ * and the face forces second (Jacobi order), the same order the GPU uses.
+ */
+#include "dugks_oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+#define Q9 9
+
+/* D2Q9 lattice, Ark/Lib/D2Q9.h:24-28 and weights Ark/Lib/D2Q9HCZ.cpp:15-19 */
+static const double E9X[Q9] = {0, 1, 1, 0, -1, -1, -1, 0, 1};
+static const double E9Y[Q9] = {0, 0, 1, 1, 1, 0, -1, -1, -1};
+static const int BB9[Q9] = {0, 5, 6, 7, 8, 1, 2, 3, 4};
+static const double W9[Q9] = {4.0 / 9.0, 1.0 / 9.0, 1.0 / 36.0, 1.0 / 9.0, 1.0 / 36.0,
+                              1.0 / 9.0, 1.0 / 36.0, 1.0 / 9.0, 1.0 / 36.0};
+
+struct ora {
+    mpd_config c;
+    int nx, ny, N, walls;
+    /* derived constants (formulas as in Ark/Constant/ZeroNamespace.h) */
+    double hdt, lambda0, aphi, bphi, beta, kappa, kp, taum, xi[2][Q9], dtv;
+    double ah_h, bh_h, ch_h, abp_h, bbp_h, cbp_h; /* h factors: constant tau_h */
+    double ah_pp, bh_pp, ch_pp;                    /* PP face factors (never updated) */
+    /* state */
+    double *h, *f, *m;        /* [Q][N], [Q][N], [MPD_M_COUNT][N] */
+    double *ffx, *ffy, *fpsi; /* PP face force / Psi: x-faces [N] then y-faces [N] */
+    /* scratch */
+    double *hbp, *fbp, *hbpx, *hbpy, *fbpx, *fbpy;
+    double *hfl, *ffl;        /* fluxes: x-faces [Q][N] then y-faces [Q][NY1] */
+    double *gm;               /* wall ghosts [MPD_M_COUNT][2][nx] */
+    double *ghbp, *gfbp;      /* wall ghost bar-plus [Q][2][nx] */
+    int nyf;                  /* rows of y-faces: ny (periodic) or ny+1 (walls) */
+};
+
+/* ---------------- small helpers ---------------- */
+
+static inline int wrap(int i, int n) { return (i % n + n) % n; }
+
+static inline void set_zero(double *d)
+{ /* SetZero, Ark/Header/Mesh_2D.h:26-30 */
+    if (*d < 1.0E-14 && *d > -1.0E-14) *d = 0.0;
+}
+
+/* value of macro field `fld` at cell (i,j); j = -1 / ny address wall ghosts */
+static inline double M(const ora *o, int fld, int i, int j)
+{
+    i = wrap(i, o->nx);
+    if (j < 0 || j >= o->ny) {
+        if (!o->walls) j = wrap(j, o->ny);
+        else return o->gm[((size_t)fld * 2 + (j < 0 ? 0 : 1)) * o->nx + i];
+    }
+    return o->m[(size_t)fld * o->N + (size_t)j * o->nx + i];
+}
+
+static inline double BP(const ora *o, const double *bp, const double *gbp, int q, int i, int j)
+{
+    i = wrap(i, o->nx);
+    if (j < 0 || j >= o->ny) {
+        if (!o->walls) j = wrap(j, o->ny);
+        else return gbp[((size_t)q * 2 + (j < 0 ? 0 : 1)) * o->nx + i];
+    }
+    return bp[(size_t)q * o->N + (size_t)j * o->nx + i];
+}
+
+static inline double *MP(ora *o, int fld) { return o->m + (size_t)fld * o->N; }
+
+/* Carnahan-Starling EOS with a = 1, b = 4, T = Tr*0.0943 (Ark/Lib/nonidealEoS.h:59-73) */
+static double carnahan_starling(const ora *o, double rho)
+{
+    double t_cs = o->c.pp_tr * 0.0943;
+    return rho * 1.0 * t_cs * (1.0 + rho + rho * rho - rho * rho * rho) /
+               ((1.0 - rho) * (1.0 - rho) * (1.0 - rho)) -
+           1.0 * rho * rho;
+}
+
+/* MacroQuantity::calcPsi, Ark/Header/Macroscopic.h:64-68 */
+static void calc_psi(double p, double rho, double rt, double *scg, double *psi)
+{
+    double s = (double)((p > rho * rt) - (p < rho * rt));
+    *scg = s;
+    *psi = sqrt(2 * (p - rho * rt) / s);
+}
+
+/* ---------------- construction ---------------- */
+
+ora *ora_create(const mpd_config *cfg)
+{
+    if (!cfg || cfg->nx < 3 || cfg->ny < 3) return NULL;
+    if (cfg->model != MPD_MODEL_AC_P && cfg->model != MPD_MODEL_AC_HCZ &&
+        cfg->model != MPD_MODEL_PP_D2Q9)
+        return NULL;
+    if (cfg->bc_y == MPD_BC_WALL_BB && (cfg->flux == MPD_FLUX_UW || cfg->model == MPD_MODEL_PP_D2Q9))
+        return NULL; /* the reference has no working wall path for these */
+    ora *o = (ora *)calloc(1, sizeof(ora));
+    o->c = *cfg;
+    o->nx = cfg->nx;
+    o->ny = cfg->ny;
+    o->N = cfg->nx * cfg->ny;
+    o->walls = cfg->bc_y == MPD_BC_WALL_BB;
+    o->nyf = o->walls ? o->ny + 1 : o->ny;
+    const mpd_config *c = cfg;
+    o->hdt = 0.5 * c->dt;
+    o->lambda0 = 1 / (2.0 * c->rt);
+    o->aphi = (c->rho_l - c->rho_v) / (c->phi_l - c->phi_v);
+    o->bphi = c->rho_v - o->aphi * c->phi_v;
+    o->beta = 12.0 * c->sigma / c->width;
+    o->kappa = 3.0 * c->sigma * c->width / 2;
+    o->kp = c->rt * 9 / 5.0;
+    o->taum = c->mobility / c->rt;
+    double maspan = sqrt(3.0 * c->rt);
+    for (int q = 0; q < Q9; ++q) {
+        o->xi[0][q] = maspan * E9X[q];
+        o->xi[1][q] = maspan * E9Y[q];
+    }
+    o->dtv = c->dt / (c->dx * c->dx);
+    /* Face_2D::DVDF::setxh / Cell_2D::DVDF::setxBP with tau_h = TauMass,
+     * set once at initialisation (Ark/Cpp/Preprocess.cpp:727-734,751-758). */
+    o->ah_h = 2.0 * o->taum / (2.0 * o->taum + o->hdt);
+    o->bh_h = 1.0 - o->ah_h;
+    o->ch_h = o->taum * o->bh_h;
+    o->abp_h = (2.0 * o->taum - o->hdt) / (2.0 * o->taum + c->dt);
+    o->bbp_h = 1.0 - o->abp_h;
+    o->cbp_h = o->taum * o->bbp_h;
+    o->ah_pp = 2.0 * c->pp_tau / (2.0 * c->pp_tau + o->hdt);
+    o->bh_pp = 1.0 - o->ah_pp;
+    o->ch_pp = c->pp_tau * o->bh_pp;
+    size_t N = o->N, QN = (size_t)Q9 * N;
+    o->h = calloc(QN, sizeof(double));
+    o->f = calloc(QN, sizeof(double));
+    o->m = calloc((size_t)MPD_M_COUNT * N, sizeof(double));
+    o->ffx = calloc(2 * N, sizeof(double));
+    o->ffy = calloc(2 * N, sizeof(double));
+    o->fpsi = calloc(2 * N, sizeof(double));
+    o->hbp = calloc(QN, sizeof(double));
+    o->fbp = calloc(QN, sizeof(double));
+    o->hbpx = calloc(QN, sizeof(double));
+    o->hbpy = calloc(QN, sizeof(double));
+    o->fbpx = calloc(QN, sizeof(double));
+    o->fbpy = calloc(QN, sizeof(double));
+    o->hfl = calloc(QN + (size_t)Q9 * o->nyf * o->nx, sizeof(double));
+    o->ffl = calloc(QN + (size_t)Q9 * o->nyf * o->nx, sizeof(double));
+    o->gm = calloc((size_t)MPD_M_COUNT * 2 * o->nx, sizeof(double));
+    o->ghbp = calloc((size_t)Q9 * 2 * o->nx, sizeof(double));
+    o->gfbp = calloc((size_t)Q9 * 2 * o->nx, sizeof(double));
+    return o;
+}
+
+void ora_destroy(ora *o)
+{
+    if (!o) return;
+    double *bufs[] = {o->h, o->f, o->m, o->ffx, o->ffy, o->fpsi, o->hbp, o->fbp, o->hbpx,
+                      o->hbpy, o->fbpx, o->fbpy, o->hfl, o->ffl, o->gm, o->ghbp, o->gfbp};
+    for (size_t k = 0; k < sizeof(bufs) / sizeof(bufs[0]); ++k) free(bufs[k]);
+    free(o);
+}
+
+int ora_set_state(ora *o, const double *h, const double *f, const double *m,
+                  const double *face_fx, const double *face_fy)
+{
+    size_t N = o->N, QN = (size_t)Q9 * N;
+    if (h) memcpy(o->h, h, QN * sizeof(double));
+    if (f) memcpy(o->f, f, QN * sizeof(double));
+    if (m) memcpy(o->m, m, (size_t)MPD_M_COUNT * N * sizeof(double));
+    if (face_fx) memcpy(o->ffx, face_fx, 2 * N * sizeof(double));
+    if (face_fy) memcpy(o->ffy, face_fy, 2 * N * sizeof(double));
+    return 0;
+}
+
+int ora_get_state(const ora *o, double *h, double *f, double *m, double *face_fx,
+                  double *face_fy)
+{
+    size_t N = o->N, QN = (size_t)Q9 * N;
+    if (h) memcpy(h, o->h, QN * sizeof(double));
+    if (f) memcpy(f, o->f, QN * sizeof(double));
+    if (m) memcpy(m, o->m, (size_t)MPD_M_COUNT * N * sizeof(double));
+    if (face_fx) memcpy(face_fx, o->ffx, 2 * N * sizeof(double));
+    if (face_fy) memcpy(face_fy, o->ffy, 2 * N * sizeof(double));
+    return 0;
+}
+
+/* ---------------- phase 1: cell equilibria, sources, bar-plus ---------------- */
+
+/* Update_phi_Eq + Update_DVDF_Source + Update_phi_BP for one cell.
+ * HCZ: Ark/Lib/D2Q9HCZ.cpp:54-92,244-257; AC-p: Ark/Lib/D2Q9AC.cpp:69-105,331-344;
+ * PP: Ark/Lib/D2Q9PP.cpp:51-62,137-145; bar-plus: Ark/Cpp/DUGKS2DSolver.cpp:379-401;
+ * factors: Ark/Cpp/Mesh/Cell_2D.cpp:120-125. */
+static void phase1_cell(ora *o, size_t n)
+{
+    const double rt = o->c.rt;
+    double *m = o->m;
+    size_t N = o->N;
+    double phi = m[MPD_M_PHI * N + n], rho = m[MPD_M_RHO * N + n];
+    double U = m[MPD_M_U * N + n], V = m[MPD_M_V * N + n], p = m[MPD_M_P * N + n];
+    double Fx = m[MPD_M_FX * N + n], Fy = m[MPD_M_FY * N + n];
+    double rx = m[MPD_M_RHOX * N + n], ry = m[MPD_M_RHOY * N + n];
+    double px = m[MPD_M_PHIX * N + n], py = m[MPD_M_PHIY * N + n];
+    double tau = m[MPD_M_TAU * N + n];
+    double abp = (2.0 * tau - o->hdt) / (2.0 * tau + o->c.dt);
+    double bbp = 1.0 - abp;
+    double cbp = tau * bbp;
+    int model = o->c.model;
+    double uu = U * U + V * V;
+    for (int q = 0; q < Q9; ++q) {
+        double ex = o->xi[0][q], ey = o->xi[1][q], w = W9[q];
+        size_t k = (size_t)q * N + n;
+        double feq, fso;
+        if (model == MPD_MODEL_PP_D2Q9) {
+            double u1 = (ex * U + ey * V) / rt;
+            feq = w * rho * (1.0 + u1 + 0.5 * u1 * u1 - uu * o->lambda0);
+            fso = (Fx * (ex - U) + Fy * (ey - V)) * feq / (rho * rt);
+        } else {
+            double u1 = (ex * U + ey * V) / rt;
+            double G = u1 + 0.5 * u1 * u1 - uu * o->lambda0;
+            double heq = w * (1.0 + u1) * phi;
+            double hso = (px * ex + py * ey) * w;
+            if (model == MPD_MODEL_AC_HCZ) {
+                feq = w * (p + rho * G * rt);
+                fso = w * ((1 + G) * ((ex - U) * Fx + (ey - V) * Fy) +
+                           G * rt * ((ex - U) * rx + (ey - V) * ry));
+            } else {
+                feq = w * (p / rt + rho * G) - (double)(q == 0) * p / rt;
+                fso = w * ((1 + G) * ((ex - U) * Fx + (ey - V) * Fy) / rt +
+                           G * ((ex - U) * rx + (ey - V) * ry));
+            }
+            o->hbp[k] = o->abp_h * o->h[k] + o->bbp_h * heq + o->cbp_h * hso;
+        }
+        double b = abp * o->f[k] + bbp * feq;
+        b += cbp * fso;
+        o->fbp[k] = b;
+    }
+}
+
+/* ---------------- phase 2: wall ghosts ---------------- */
+
+/* WallShadowC_fBP / ExtrapolationfBP, Ark/Cpp/Boundary.cpp:38-85: ghost macros are
+ * 2*near - next (MacroQuantity arithmetic, Ark/Cpp/Mesh/Macroscopic.cpp:20-76, which
+ * leaves scG = next.scG), with U, V, Mu replaced by -near; bar-plus extrapolated. */
+static void phase2_walls(ora *o)
+{
+    int nx = o->nx, ny = o->ny;
+    for (int side = 0; side < 2; ++side) {
+        int jn = side == 0 ? 0 : ny - 1;
+        int jnn = side == 0 ? 1 : ny - 2;
+        for (int i = 0; i < nx; ++i) {
+            for (int fld = 0; fld < MPD_M_COUNT; ++fld) {
+                double near = M(o, fld, i, jn), next = M(o, fld, i, jnn), g;
+                if (fld == MPD_M_U || fld == MPD_M_V || fld == MPD_M_MU) g = -near;
+                else if (fld == MPD_M_SCG) g = next;
+                else g = 2.0 * near - next;
+                o->gm[((size_t)fld * 2 + side) * nx + i] = g;
+            }
+            for (int q = 0; q < Q9; ++q) {
+                size_t a = (size_t)q * o->N + (size_t)jn * nx + i;
+                size_t b = (size_t)q * o->N + (size_t)jnn * nx + i;
+                o->ghbp[((size_t)q * 2 + side) * nx + i] = 2 * o->hbp[a] - o->hbp[b];
+                o->gfbp[((size_t)q * 2 + side) * nx + i] = 2 * o->fbp[a] - o->fbp[b];
+            }
+        }
+    }
+}
+
+/* ---------------- phase 3: UW gradients ---------------- */
+
+/* update_DVDF_Grad6points, Ark/Lib/GradSchemeBasic.h:95-126 (apsi = 4, bpsi = 1) */
+static void phase3_grad(ora *o, const double *bp, double *gx, double *gy)
+{
+    int nx = o->nx, ny = o->ny;
+    double d2x = 2.0 * o->c.dx;
+    for (int q = 0; q < Q9; ++q)
+        for (int j = 0; j < ny; ++j)
+            for (int i = 0; i < nx; ++i) {
+                double C0 = BP(o, bp, NULL, q, i + 1, j), C1 = BP(o, bp, NULL, q, i, j + 1);
+                double C2 = BP(o, bp, NULL, q, i - 1, j), C3 = BP(o, bp, NULL, q, i, j - 1);
+                double D0 = BP(o, bp, NULL, q, i + 1, j + 1), D1 = BP(o, bp, NULL, q, i - 1, j + 1);
+                double D2 = BP(o, bp, NULL, q, i - 1, j - 1), D3 = BP(o, bp, NULL, q, i + 1, j - 1);
+                size_t k = (size_t)q * o->N + (size_t)j * nx + i;
+                gx[k] = (4 * (C0 - C2) + 1 * (D0 - D1 + D3 - D2)) / (d2x * 6);
+                gy[k] = (4 * (C1 - C3) + 1 * (D0 - D3 + D1 - D2)) / (d2x * 6);
+            }
+}
+
+/* ---------------- phase 4: faces ---------------- */
+
+typedef struct {
+    int dir;        /* 0: x-face (normal +x), 1: y-face (normal +y) */
+    int oi, oj;     /* owner cell */
+    int ni, nj;     /* neighbour cell */
+    double xf, yf;  /* face centre */
+} face_t;
+
+/* f-bar at the face, t + h: CD_Interior_phi_Bh (Ark/Cpp/DUGKS2DSolver.cpp:604-687)
+ * or UW_Interior_phi_Bh (:560-603, selection :688-706).
+ *
+ * CD, x-face: the reference's normal derivative (owner - neigh)/_dx with
+ * _dx = xc_owner - xc_neigh (MeshCartesian.cpp:419-427) equals (n - o)/dx here,
+ * and its tangential derivative (0.5*(fc3-fc1) + 0.5*(fc2-fc0))/_dy with
+ * _dy = +-2dx (MeshCartesian.cpp:145,203) equals the two half-differences of
+ * the owner and neighbour columns over 2dx; all sign changes are exact. */
+static double face_recon(const ora *o, const double *bp, const double *gbp, const double *gx,
+                         const double *gy, const face_t *F, int q)
+{
+    double ex = o->xi[0][q], ey = o->xi[1][q], dx = o->c.dx;
+    double bo = BP(o, bp, gbp, q, F->oi, F->oj), bn = BP(o, bp, gbp, q, F->ni, F->nj);
+    if (o->c.flux == MPD_FLUX_CD) {
+        double dn = (bn - bo) / dx, dt;
+        if (F->dir == 0) {
+            double No = BP(o, bp, gbp, q, F->oi, F->oj + 1), So = BP(o, bp, gbp, q, F->oi, F->oj - 1);
+            double Nn = BP(o, bp, gbp, q, F->ni, F->nj + 1), Sn = BP(o, bp, gbp, q, F->ni, F->nj - 1);
+            dt = (0.5 * (Nn - Sn) + 0.5 * (No - So)) / (2 * dx);
+            return 0.5 * (bo + bn) - o->hdt * (dn * ex + dt * ey);
+        } else {
+            double Eo = BP(o, bp, gbp, q, F->oi + 1, F->oj), Wo = BP(o, bp, gbp, q, F->oi - 1, F->oj);
+            double En = BP(o, bp, gbp, q, F->ni + 1, F->nj), Wn = BP(o, bp, gbp, q, F->ni - 1, F->nj);
+            dt = (0.5 * (En - Wn) + 0.5 * (Eo - Wo)) / (2 * dx);
+            return 0.5 * (bo + bn) - o->hdt * (dt * ex + dn * ey);
+        }
+    }
+    /* UW: upwind along xi.n of the face's own normal */
+    double xin = F->dir == 0 ? ex : ey;
+    double xo = (F->oi + 0.5) * dx, yo = (F->oj + 0.5) * dx;
+    double xn = (F->ni + 0.5) * dx, yn = (F->nj + 0.5) * dx;
+    size_t ko = (size_t)q * o->N + (size_t)wrap(F->oj, o->ny) * o->nx + wrap(F->oi, o->nx);
+    size_t kn = (size_t)q * o->N + (size_t)wrap(F->nj, o->ny) * o->nx + wrap(F->ni, o->nx);
+    double dxo = F->xf - o->hdt * ex - xo, dyo = F->yf - o->hdt * ey - yo;
+    double dxn = F->xf - o->hdt * ex - xn, dyn = F->yf - o->hdt * ey - yn;
+    if (xin > 0) return bo + (dxo * gx[ko] + dyo * gy[ko]);
+    if (xin < 0) return bn + (dxn * gx[kn] + dyn * gy[kn]);
+    return (bo + (dxo * gx[ko] + dyo * gy[ko]) + bn + (dxn * gx[kn] + dyn * gy[kn])) / 2;
+}
+
+typedef struct {
+    double phi, rho, U, V, p, tau, px, py, rx, ry, Fx, Fy, scg, psi;
+} face_macro;
+
+/* Update_MacroVar_h, HCZ Ark/Lib/D2Q9HCZ.cpp:183-242, AC-p Ark/Lib/D2Q9AC.cpp:232-329,
+ * PP Ark/Lib/D2Q9PP.cpp:110-136 (+ Update_PseudoPsi(face), Ark/Cpp/PseudoP.cpp:23-30);
+ * moment sums in sequential order as IntegralGH (Ark/Lib/GaussHermite.h:1-14). */
+static void face_macros(const ora *o, const face_t *F, const double *hb, const double *fb,
+                        double fx_old, double fy_old, face_macro *fm)
+{
+    const double rt = o->c.rt, hdt = o->hdt;
+    int model = o->c.model;
+    if (model == MPD_MODEL_PP_D2Q9) {
+        double s = 0, sx = 0, sy = 0;
+        for (int q = 0; q < Q9; ++q) s += fb[q];
+        for (int q = 0; q < Q9; ++q) sx += fb[q] * o->xi[0][q];
+        for (int q = 0; q < Q9; ++q) sy += fb[q] * o->xi[1][q];
+        fm->rho = s;
+        fm->U = sx / fm->rho;
+        fm->V = sy / fm->rho;
+        fm->U += 0.5 * hdt * fx_old / fm->rho;
+        fm->V += 0.5 * hdt * fy_old / fm->rho;
+        fm->p = o->c.pp_k * carnahan_starling(o, fm->rho);
+        calc_psi(fm->p, fm->rho, rt, &fm->scg, &fm->psi);
+        fm->tau = o->c.pp_tau;
+        return;
+    }
+    fm->px = 0.5 * (M(o, MPD_M_PHIX, F->oi, F->oj) + M(o, MPD_M_PHIX, F->ni, F->nj));
+    fm->py = 0.5 * (M(o, MPD_M_PHIY, F->oi, F->oj) + M(o, MPD_M_PHIY, F->ni, F->nj));
+    fm->rx = 0.5 * (M(o, MPD_M_RHOX, F->oi, F->oj) + M(o, MPD_M_RHOX, F->ni, F->nj));
+    fm->ry = 0.5 * (M(o, MPD_M_RHOY, F->oi, F->oj) + M(o, MPD_M_RHOY, F->ni, F->nj));
+    fm->Fx = 0.5 * (M(o, MPD_M_FX, F->oi, F->oj) + M(o, MPD_M_FX, F->ni, F->nj));
+    fm->Fy = 0.5 * (M(o, MPD_M_FY, F->oi, F->oj) + M(o, MPD_M_FY, F->ni, F->nj));
+    double sh = 0;
+    for (int q = 0; q < Q9; ++q) sh += hb[q];
+    fm->phi = sh;
+    fm->rho = o->aphi * fm->phi + o->bphi;
+    double s = 0, sx = 0, sy = 0;
+    for (int q = 0; q < Q9; ++q) s += fb[q];
+    double mu;
+    if (model == MPD_MODEL_AC_HCZ) {
+        for (int q = 0; q < Q9; ++q) sx += fb[q] * o->xi[0][q];
+        for (int q = 0; q < Q9; ++q) sy += fb[q] * o->xi[1][q];
+        fm->U = sx / (fm->rho * rt);
+        fm->V = sy / (fm->rho * rt);
+        fm->U += 0.5 * hdt * fm->Fx / (fm->rho * rt);
+        fm->V += 0.5 * hdt * fm->Fy / (fm->rho * rt);
+        fm->p = s + 0.5 * hdt * rt * (fm->rx * fm->U + fm->ry * fm->V);
+        mu = o->c.nu_v + (fm->phi - o->c.phi_v) * (o->c.nu_l - o->c.nu_v);
+        mu *= fm->rho;
+    } else {
+        fm->U = 0.0;
+        fm->V = 0.0;
+        fm->p = s - fb[0];
+        fm->p += (0.5 * hdt * (fm->rx * fm->U + fm->ry * fm->V) -
+                  W9[0] * fm->rho * (fm->U * fm->U + fm->V * fm->V) * o->lambda0);
+        fm->p *= o->kp;
+        mu = fm->rho * rt * o->c.tau0;
+    }
+    fm->tau = mu / (fm->rho * rt);
+}
+
+/* Update_phi_Eqh + Update_DVDF_Source_h + Update_phi_h for one population.
+ * HCZ Ark/Lib/D2Q9HCZ.cpp:93-131,258-271; AC-p Ark/Lib/D2Q9AC.cpp:106-144,345-358;
+ * PP Ark/Lib/D2Q9PP.cpp:63-74,146-154; BGK half step Ark/Cpp/DUGKS2DSolver.cpp:402-423;
+ * factors Ark/Cpp/Mesh/Face_2D.cpp:123-128. */
+static void face_bgk(const ora *o, const face_macro *fm, int q, double hb, double fb,
+                     double ah, double bh, double ch, double *hout, double *fout)
+{
+    const double rt = o->c.rt;
+    double ex = o->xi[0][q], ey = o->xi[1][q], w = W9[q];
+    double U = fm->U, V = fm->V, uu = U * U + V * V;
+    double u1 = (ex * U + ey * V) / rt;
+    double feq, fso;
+    if (o->c.model == MPD_MODEL_PP_D2Q9) {
+        feq = w * fm->rho * (1.0 + u1 + 0.5 * u1 * u1 - uu * o->lambda0);
+        fso = (fm->Fx * (ex - U) + fm->Fy * (ey - V)) * feq / (fm->rho * rt);
+        *hout = 0;
+    } else {
+        double G = u1 + 0.5 * u1 * u1 - uu * o->lambda0;
+        double heq = w * fm->phi * (1.0 + u1);
+        double hso = (fm->px * ex + fm->py * ey) * w;
+        *hout = o->ah_h * hb + o->bh_h * heq + o->ch_h * hso;
+        if (o->c.model == MPD_MODEL_AC_HCZ) {
+            feq = w * (fm->p + fm->rho * rt * G);
+            fso = w * ((1 + G) * ((ex - U) * fm->Fx + (ey - V) * fm->Fy) +
+                       G * rt * ((ex - U) * fm->rx + (ey - V) * fm->ry));
+        } else {
+            feq = w * (fm->p / rt + fm->rho * G) - (double)(q == 0) * fm->p / rt;
+            fso = w * ((1 + G) * ((ex - U) * fm->Fx + (ey - V) * fm->Fy) / rt +
+                       G * ((ex - U) * fm->rx + (ey - V) * fm->ry));
+        }
+    }
+    double r = ah * fb + bh * feq;
+    r += ch * fso;
+    *fout = r;
+}
+
+static void make_face(const ora *o, int dir, int i, int j, face_t *F)
+{
+    F->dir = dir;
+    if (dir == 0) {
+        F->oi = i - 1; F->oj = j; F->ni = i; F->nj = j;
+        F->xf = i * o->c.dx; F->yf = (j + 0.5) * o->c.dx;
+    } else {
+        F->oi = i; F->oj = j - 1; F->ni = i; F->nj = j;
+        F->xf = (i + 0.5) * o->c.dx; F->yf = j * o->c.dx;
+    }
+}
+
+/* psi of the face lattice of direction `dir` at (i,j), periodic in both directions */
+static inline double face_psi(const ora *o, int dir, int i, int j)
+{
+    return o->fpsi[(size_t)dir * o->N + (size_t)wrap(j, o->ny) * o->nx + wrap(i, o->nx)];
+}
+
+static void phase4_faces(ora *o)
+{
+    int nx = o->nx, ny = o->ny, N = o->N;
+    int pp = o->c.model == MPD_MODEL_PP_D2Q9;
+    double hb[Q9], fb[Q9];
+    face_macro *fms = NULL;
+    if (pp) fms = (face_macro *)malloc(sizeof(face_macro) * 2 * (size_t)N);
+    /* PP, first pass: face macros and Psi of every face (Jacobi order). */
+    if (pp) {
+        for (int dir = 0; dir < 2; ++dir)
+            for (int j = 0; j < ny; ++j)
+                for (int i = 0; i < nx; ++i) {
+                    face_t F;
+                    make_face(o, dir, i, j, &F);
+                    for (int q = 0; q < Q9; ++q)
+                        fb[q] = face_recon(o, o->fbp, o->gfbp, o->fbpx, o->fbpy, &F, q);
+                    size_t fi = (size_t)dir * N + (size_t)j * nx + i;
+                    face_macro *fm = &fms[fi];
+                    face_macros(o, &F, hb, fb, o->ffx[fi], o->ffy[fi], fm);
+                    o->fpsi[fi] = fm->psi;
+                }
+        /* second pass: face force from the face-lattice gradient of Psi,
+         * update_MQ_x/y(Face_2D*) (Ark/Lib/GradSchemeBasic.h:181-208) and
+         * Update_PseudoForce(face) (Ark/Cpp/PseudoP.cpp:39-46) */
+        double d2x = 2.0 * o->c.dx;
+        for (int dir = 0; dir < 2; ++dir)
+            for (int j = 0; j < ny; ++j)
+                for (int i = 0; i < nx; ++i) {
+                    size_t fi = (size_t)dir * N + (size_t)j * nx + i;
+                    double F1 = face_psi(o, dir, i + 1, j), F5 = face_psi(o, dir, i - 1, j);
+                    double F3 = face_psi(o, dir, i, j + 1), F7 = face_psi(o, dir, i, j - 1);
+                    double F2 = face_psi(o, dir, i + 1, j + 1), F4 = face_psi(o, dir, i - 1, j + 1);
+                    double F6 = face_psi(o, dir, i - 1, j - 1), F8 = face_psi(o, dir, i + 1, j - 1);
+                    double gxp = (4 * (F1 - F5) + 1 * (F2 - F4 + F8 - F6)) / (d2x * 6);
+                    double gyp = (4 * (F3 - F7) + 1 * (F2 - F8 + F4 - F6)) / (d2x * 6);
+                    face_macro *fm = &fms[fi];
+                    gxp *= -fm->scg * fm->psi;
+                    gyp *= -fm->scg * fm->psi;
+                    fm->Fx = gxp;
+                    fm->Fy = gyp;
+                    o->ffx[fi] = gxp;
+                    o->ffy[fi] = gyp;
+                }
+    }
+    for (int dir = 0; dir < 2; ++dir) {
+        int rows = dir == 0 ? ny : o->nyf;
+        double *hfl = o->hfl + (dir ? (size_t)Q9 * N : 0);
+        double *ffl = o->ffl + (dir ? (size_t)Q9 * N : 0);
+        size_t plane = dir == 0 ? (size_t)N : (size_t)o->nyf * nx;
+        for (int j = 0; j < rows; ++j)
+            for (int i = 0; i < nx; ++i) {
+                face_t F;
+                make_face(o, dir, i, j, &F);
+                for (int q = 0; q < Q9; ++q) {
+                    if (!pp) hb[q] = face_recon(o, o->hbp, o->ghbp, o->hbpx, o->hbpy, &F, q);
+                    fb[q] = face_recon(o, o->fbp, o->gfbp, o->fbpx, o->fbpy, &F, q);
+                }
+                face_macro fml, *fm;
+                double ah, bh, ch;
+                if (pp) {
+                    fm = &fms[(size_t)dir * N + (size_t)j * nx + i];
+                    ah = o->ah_pp; bh = o->bh_pp; ch = o->ch_pp;
+                } else {
+                    fm = &fml;
+                    face_macros(o, &F, hb, fb, 0, 0, fm);
+                    ah = 2.0 * fm->tau / (2.0 * fm->tau + o->hdt);
+                    bh = 1.0 - ah;
+                    ch = fm->tau * bh;
+                }
+                double hh[Q9], ff[Q9];
+                for (int q = 0; q < Q9; ++q)
+                    face_bgk(o, fm, q, pp ? 0 : hb[q], fb[q], ah, bh, ch, &hh[q], &ff[q]);
+                /* Wall_3_BB (Ark/Cpp/Boundary.cpp:118-139): populations entering the
+                 * fluid take the value of their bounce-back partner. */
+                int wall = o->walls && dir == 1 && (j == 0 || j == ny);
+                if (wall) {
+                    double h0[Q9], f0[Q9];
+                    memcpy(h0, hh, sizeof(h0));
+                    memcpy(f0, ff, sizeof(f0));
+                    for (int q = 0; q < Q9; ++q) {
+                        double ey = o->xi[1][q];
+                        int incoming = (j == 0) ? (ey > 0) : (ey < 0);
+                        if (incoming) { hh[q] = h0[BB9[q]]; ff[q] = f0[BB9[q]]; }
+                    }
+                }
+                /* Update_phiFlux_h (:424-441): times xi.n dS (setXiDotdS,
+                 * Ark/Cpp/Mesh/MeshConstruct.cpp:289-297), unit normal +x / +y */
+                for (int q = 0; q < Q9; ++q) {
+                    double xin = o->c.dx * (dir == 0 ? o->xi[0][q] : o->xi[1][q]);
+                    size_t k = (size_t)q * plane + (size_t)j * nx + i;
+                    hfl[k] = hh[q] * xin;
+                    ffl[k] = ff[q] * xin;
+                }
+            }
+    }
+    free(fms);
+}
+
+/* ---------------- phase 8: cell update ---------------- */
+
+/* Update_phi_T, Ark/Cpp/DUGKS2DSolver.cpp:748-792, faces summed in the
+ * reference's Face_C order E, N, W, S (sortFacesInThisCell,
+ * Ark/Cpp/Mesh/MeshCartesian.cpp:28-60) with signFlux -1 for the owned E and N
+ * faces and +1 for W and S (Ark/Cpp/Mesh/MeshConstruct.cpp:283-285). */
+static void phase8_update(ora *o)
+{
+    int nx = o->nx, ny = o->ny, N = o->N;
+    const double aTP = 4.0 / 3.0, bTP = 1.0 / 3.0;
+    size_t yplane = (size_t)o->nyf * nx;
+    int pp = o->c.model == MPD_MODEL_PP_D2Q9;
+    for (int q = 0; q < Q9; ++q)
+        for (int j = 0; j < ny; ++j)
+            for (int i = 0; i < nx; ++i) {
+                size_t k = (size_t)q * N + (size_t)j * nx + i;
+                size_t kE = (size_t)q * N + (size_t)j * nx + wrap(i + 1, nx);
+                size_t kW = k;
+                int jn = o->walls ? j + 1 : wrap(j + 1, ny);
+                size_t kN = (size_t)q * yplane + (size_t)jn * nx + i;
+                size_t kS = (size_t)q * yplane + (size_t)j * nx + i;
+                const double *hx = o->hfl, *hy = o->hfl + (size_t)Q9 * N;
+                const double *fx = o->ffl, *fy = o->ffl + (size_t)Q9 * N;
+                if (!pp) {
+                    double s = 0.0;
+                    s += -1 * hx[kE];
+                    s += -1 * hy[kN];
+                    s += 1 * hx[kW];
+                    s += 1 * hy[kS];
+                    o->h[k] = aTP * o->hbp[k] - bTP * o->h[k] + o->dtv * s;
+                }
+                double s = 0.0;
+                s += -1 * fx[kE];
+                s += -1 * fy[kN];
+                s += 1 * fx[kW];
+                s += 1 * fy[kS];
+                o->f[k] = aTP * o->fbp[k] - bTP * o->f[k] + o->dtv * s;
+            }
+}
+
+/* ---------------- phase 9: cell macros ---------------- */
+
+/* Update_MacroVar: HCZ Ark/Lib/D2Q9HCZ.cpp:133-182 (calcMu :24-40), AC-p
+ * Ark/Lib/D2Q9AC.cpp:146-231 (calcMu :34-56), PP Ark/Lib/D2Q9PP.cpp:89-109 with
+ * Update_PseudoPsi (Ark/Cpp/PseudoP.cpp:15-22). */
+static void phase9_macros(ora *o)
+{
+    size_t N = o->N;
+    const double rt = o->c.rt, hdt = o->hdt;
+    double *m = o->m;
+    for (size_t n = 0; n < N; ++n) {
+        double s = 0, sx = 0, sy = 0;
+        if (o->c.model == MPD_MODEL_PP_D2Q9) {
+            for (int q = 0; q < Q9; ++q) s += o->f[q * N + n];
+            for (int q = 0; q < Q9; ++q) sx += o->f[q * N + n] * o->xi[0][q];
+            for (int q = 0; q < Q9; ++q) sy += o->f[q * N + n] * o->xi[1][q];
+            double rho = s;
+            double U = sx / rho, V = sy / rho;
+            U += hdt * m[MPD_M_FX * N + n] / rho;
+            V += hdt * m[MPD_M_FY * N + n] / rho;
+            m[MPD_M_RHO * N + n] = rho;
+            m[MPD_M_U * N + n] = U;
+            m[MPD_M_V * N + n] = V;
+            double p = o->c.pp_k * carnahan_starling(o, rho);
+            m[MPD_M_P * N + n] = p;
+            calc_psi(p, rho, rt, &m[MPD_M_SCG * N + n], &m[MPD_M_PSI * N + n]);
+            continue;
+        }
+        double sh = 0;
+        for (int q = 0; q < Q9; ++q) sh += o->h[q * N + n];
+        double phi = sh;
+        double rho = o->aphi * phi + o->bphi;
+        for (int q = 0; q < Q9; ++q) s += o->f[q * N + n];
+        double U, V, p, mu;
+        double rx = m[MPD_M_RHOX * N + n], ry = m[MPD_M_RHOY * N + n];
+        if (o->c.model == MPD_MODEL_AC_HCZ) {
+            for (int q = 0; q < Q9; ++q) sx += o->f[q * N + n] * o->xi[0][q];
+            for (int q = 0; q < Q9; ++q) sy += o->f[q * N + n] * o->xi[1][q];
+            U = sx / (rho * rt);
+            V = sy / (rho * rt);
+            U += hdt * m[MPD_M_FX * N + n] / (rho * rt);
+            V += hdt * m[MPD_M_FY * N + n] / (rho * rt);
+            p = s + hdt * (rx * U + ry * V) * rt;
+            mu = o->c.nu_v + (phi - o->c.phi_v) * (o->c.nu_l - o->c.nu_v);
+            mu *= rho;
+        } else {
+            U = 0.0;
+            V = 0.0;
+            p = s - o->f[0 * N + n];
+            p += (hdt * (rx * U + ry * V) - rho * W9[0] * (U * U + V * V) * o->lambda0);
+            p *= o->kp;
+            mu = rho * rt * o->c.tau0;
+        }
+        m[MPD_M_PHI * N + n] = phi;
+        m[MPD_M_RHO * N + n] = rho;
+        m[MPD_M_U * N + n] = U;
+        m[MPD_M_V * N + n] = V;
+        m[MPD_M_P * N + n] = p;
+        m[MPD_M_MU * N + n] = mu;
+        m[MPD_M_TAU * N + n] = mu / (rho * rt);
+    }
+}
+
+/* ---------------- phase 10: Allen-Cahn sources ---------------- */
+
+/* MacroSource (Ark/Cpp/SourceM.cpp:15-62): Grad_Phi_6points (GradScheme.cpp:347-376,
+ * 9-point gradient GradSchemeBasic.h:153-180), SourceMomentum, and the
+ * conservative AC source vector written over Phi_x/Phi_y. */
+static void phase10_source(ora *o)
+{
+    int nx = o->nx, ny = o->ny;
+    size_t N = o->N;
+    double d2x = 2.0 * o->c.dx, minl = o->c.dx;
+    const double rt = o->c.rt;
+    const mpd_config *c = &o->c;
+    /* gradients first, from phi of this step only (phi is not written here) */
+    for (int j = 0; j < ny; ++j)
+        for (int i = 0; i < nx; ++i) {
+            size_t n = (size_t)j * nx + i;
+            double C0 = M(o, MPD_M_PHI, i + 1, j), C1 = M(o, MPD_M_PHI, i, j + 1);
+            double C2 = M(o, MPD_M_PHI, i - 1, j), C3 = M(o, MPD_M_PHI, i, j - 1);
+            double D0 = M(o, MPD_M_PHI, i + 1, j + 1), D1 = M(o, MPD_M_PHI, i - 1, j + 1);
+            double D2 = M(o, MPD_M_PHI, i - 1, j - 1), D3 = M(o, MPD_M_PHI, i + 1, j - 1);
+            double phi = o->m[MPD_M_PHI * N + n];
+            double gx = (4 * (C0 - C2) + 1 * (D0 - D1 + D3 - D2)) / (d2x * 6);
+            double gy = (4 * (C1 - C3) + 1 * (D0 - D3 + D1 - D2)) / (d2x * 6);
+            double lap = (4 * (C0 + C1 + C2 + C3) + (D0 + D1 + D2 + D3) - 20 * phi) /
+                         (6 * minl * minl);
+            set_zero(&gx);
+            set_zero(&gy);
+            /* SourceMomentum: chemical potential, F = mu_c grad(phi), grad(rho) */
+            double mu_c = 2 * o->beta * (phi - c->phi_l) * (phi - c->phi_v) *
+                              (2 * phi - c->phi_l - c->phi_v) -
+                          o->kappa * lap;
+            double rx = o->aphi * gx, ry = o->aphi * gy;
+            double Fx = mu_c * gx, Fy = mu_c * gy;
+            if (c->gx != 0.0) Fx -= c->gx * (o->m[MPD_M_RHO * N + n] - c->rho_ref);
+            if (c->gy != 0.0) Fy -= c->gy * (o->m[MPD_M_RHO * N + n] - c->rho_ref);
+            double U = o->m[MPD_M_U * N + n], V = o->m[MPD_M_V * N + n];
+            double dpu = phi * U - o->m[MPD_M_PPU * N + n];
+            double dpv = phi * V - o->m[MPD_M_PPV * N + n];
+            double L = sqrt(gx * gx + gy * gy);
+            double sx, sy;
+            if (L != 0) {
+                double tmp = -4 * (phi - c->phi_l) * (phi - c->phi_v) / (c->width * (c->phi_l - c->phi_v));
+                sx = gx * tmp / L + dpu / (rt * c->dt);
+                sy = gy * tmp / L + dpv / (rt * c->dt);
+            } else {
+                sx = dpu / (rt * c->dt);
+                sy = dpv / (rt * c->dt);
+            }
+            o->m[MPD_M_LAP * N + n] = lap;
+            o->m[MPD_M_RHOX * N + n] = rx;
+            o->m[MPD_M_RHOY * N + n] = ry;
+            o->m[MPD_M_FX * N + n] = Fx;
+            o->m[MPD_M_FY * N + n] = Fy;
+            o->m[MPD_M_PHIX * N + n] = sx;
+            o->m[MPD_M_PHIY * N + n] = sy;
+            o->m[MPD_M_PPU * N + n] = phi * U;
+            o->m[MPD_M_PPV * N + n] = phi * V;
+        }
+}
+
+/* ---------------- phase 11: pseudopotential cell force ---------------- */
+
+/* Update_PseudoForce(Cell_2D&), Ark/Cpp/PseudoP.cpp:31-38 with the 9-point
+ * gradient of Psi (Ark/Cpp/GradScheme.cpp:234-241). */
+static void phase11_ppforce(ora *o)
+{
+    int nx = o->nx, ny = o->ny;
+    size_t N = o->N;
+    double d2x = 2.0 * o->c.dx;
+    double *fxn = malloc(N * sizeof(double)), *fyn = malloc(N * sizeof(double));
+    for (int j = 0; j < ny; ++j)
+        for (int i = 0; i < nx; ++i) {
+            size_t n = (size_t)j * nx + i;
+            double C0 = M(o, MPD_M_PSI, i + 1, j), C1 = M(o, MPD_M_PSI, i, j + 1);
+            double C2 = M(o, MPD_M_PSI, i - 1, j), C3 = M(o, MPD_M_PSI, i, j - 1);
+            double D0 = M(o, MPD_M_PSI, i + 1, j + 1), D1 = M(o, MPD_M_PSI, i - 1, j + 1);
+            double D2 = M(o, MPD_M_PSI, i - 1, j - 1), D3 = M(o, MPD_M_PSI, i + 1, j - 1);
+            double gx = (4 * (C0 - C2) + 1 * (D0 - D1 + D3 - D2)) / (d2x * 6);
+            double gy = (4 * (C1 - C3) + 1 * (D0 - D3 + D1 - D2)) / (d2x * 6);
+            double sp = -o->m[MPD_M_SCG * N + n] * o->m[MPD_M_PSI * N + n];
+            gx *= sp;
+            gy *= sp;
+            fxn[n] = gx;
+            fyn[n] = gy;
+        }
+    memcpy(o->m + MPD_M_FX * N, fxn, N * sizeof(double));
+    memcpy(o->m + MPD_M_FY * N, fyn, N * sizeof(double));
+    free(fxn);
+    free(fyn);
+}
+
+/* ---------------- one time step ---------------- */
+
+int ora_step(ora *o, int nsteps)
+{
+    for (int s = 0; s < nsteps; ++s) {
+        for (size_t n = 0; n < (size_t)o->N; ++n) phase1_cell(o, n);
+        if (o->walls) phase2_walls(o);
+        if (o->c.flux == MPD_FLUX_UW) {
+            if (o->c.model != MPD_MODEL_PP_D2Q9) phase3_grad(o, o->hbp, o->hbpx, o->hbpy);
+            phase3_grad(o, o->fbp, o->fbpx, o->fbpy);
+        }
+        phase4_faces(o);
+        phase8_update(o);
+        phase9_macros(o);
+        if (o->c.model == MPD_MODEL_PP_D2Q9) phase11_ppforce(o);
+        else phase10_source(o);
+    }
+    return 0;
+}
+
+/* Wall-face net flux (fluxCheck, Ark/Cpp/Boundary.cpp:185-222): the largest
+ * |sum_q h(x_f) xi.n dS| and |sum_q f(...)| over wall faces of the last step. */
+double ora_wall_flux_residual(const ora *o)
+{
+    if (!o->walls) return 0.0;
+    double worst = 0.0;
+    size_t plane = (size_t)o->nyf * o->nx;
+    const double *hy = o->hfl + (size_t)Q9 * o->N, *fy = o->ffl + (size_t)Q9 * o->N;
+    for (int side = 0; side < 2; ++side) {
+        int j = side == 0 ? 0 : o->ny;
+        for (int i = 0; i < o->nx; ++i) {
+            double sh = 0, sf = 0;
+            for (int q = 0; q < Q9; ++q) {
+                sh += hy[(size_t)q * plane + (size_t)j * o->nx + i];
+                sf += fy[(size_t)q * plane + (size_t)j * o->nx + i];
+            }
+            if (fabs(sh) > worst) worst = fabs(sh);
+            if (fabs(sf) > worst) worst = fabs(sf);
+        }
+    }
+    return worst;
+}

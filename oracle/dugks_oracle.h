@@ -1,0 +1,21 @@
+/* CPU FP64 oracle of the reference DUGKS step (test infrastructure only).
+ * See dugks_oracle.c for the restatement and its reference citations. */
+#ifndef DUGKS_ORACLE_H
+#define DUGKS_ORACLE_H
+#include "../include/mpdugks.h"
+#ifdef __cplusplus
+extern "C" {
+#endif
+typedef struct ora ora;
+ora *ora_create(const mpd_config *cfg);
+void ora_destroy(ora *o);
+int ora_set_state(ora *o, const double *h, const double *f, const double *m,
+                  const double *face_fx, const double *face_fy);
+int ora_get_state(const ora *o, double *h, double *f, double *m, double *face_fx,
+                  double *face_fy);
+int ora_step(ora *o, int nsteps);
+double ora_wall_flux_residual(const ora *o);
+#ifdef __cplusplus
+}
+#endif
+#endif

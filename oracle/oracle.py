@@ -1,0 +1,77 @@
+"""ctypes front end of the CPU oracle (oracle/_build/liboracle.so).
+
+Test infrastructure only: imported by tests/, __graft_entry__.smoke() and the
+CPU-baseline leg of bench.py, always as the checker.
+"""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(HERE, "_build", "liboracle.so")
+_lib = None
+
+
+def build():
+    subprocess.run(["make", "-s", "-f", os.path.join(HERE, "Makefile")], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB):
+            build()
+        _lib = C.CDLL(LIB)
+        _lib.ora_create.restype = C.c_void_p
+        _lib.ora_create.argtypes = [C.c_void_p]
+        _lib.ora_destroy.argtypes = [C.c_void_p]
+        for fn in (_lib.ora_set_state, _lib.ora_get_state):
+            fn.argtypes = [C.c_void_p] + [C.c_void_p] * 5
+            fn.restype = C.c_int
+        _lib.ora_step.argtypes = [C.c_void_p, C.c_int]
+        _lib.ora_step.restype = C.c_int
+        _lib.ora_wall_flux_residual.argtypes = [C.c_void_p]
+        _lib.ora_wall_flux_residual.restype = C.c_double
+    return _lib
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+class Oracle:
+    """FP64 CPU restatement of the reference step on an nx*ny grid."""
+
+    def __init__(self, cfg):
+        from mpdugks_amd.abi import NM
+        self.cfg = cfg
+        self.N = cfg.nx * cfg.ny
+        self.nm = NM
+        self._h = lib().ora_create(C.byref(cfg))
+        if not self._h:
+            raise ValueError("oracle: unsupported configuration")
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            lib().ora_destroy(self._h)
+            self._h = None
+
+    def set_state(self, h, f, m, ffx=None, ffy=None):
+        arrs = [None if a is None else np.ascontiguousarray(a, dtype=np.float64)
+                for a in (h, f, m, ffx, ffy)]
+        lib().ora_set_state(self._h, *[_ptr(a) for a in arrs])
+
+    def get_state(self):
+        N = self.N
+        h = np.zeros((9, N)); f = np.zeros((9, N)); m = np.zeros((self.nm, N))
+        ffx = np.zeros(2 * N); ffy = np.zeros(2 * N)
+        lib().ora_get_state(self._h, _ptr(h), _ptr(f), _ptr(m), _ptr(ffx), _ptr(ffy))
+        return h, f, m, ffx, ffy
+
+    def step(self, n=1):
+        lib().ora_step(self._h, int(n))
+
+    def wall_flux_residual(self):
+        return lib().ora_wall_flux_residual(self._h)

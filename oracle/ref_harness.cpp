@@ -1,0 +1,148 @@
+// State-dump hook linked into the reference solver built from its own sources.
+//
+// Test infrastructure only (builds the checker, never shipped): the recipe in
+// oracle/Makefile.ref compiles the reference's unmodified translation units
+// and links this file with `--wrap` on the mangled symbols of `Update_phi_Eq` and `Output_Flowfield`.
+//
+// * `Output_Flowfield` is the first call made by `DUGKS2DSolver()`
+//   (Ark/Cpp/DUGKS2DSolver.cpp:150); the wrapper marks the start of the time
+//   loop and forwards to the real writer only when MPD_REF_FLOWFIELD=1.
+// * `Update_phi_Eq(CellArray[0])` is the first call of every time step
+//   (Ark/Cpp/DUGKS2DSolver.cpp:175-183).  When it fires for cell 0 inside the
+//   loop, `k` complete steps have been taken, and the whole cell state is
+//   written out before the step proceeds.
+//
+// Environment: MPD_REF_DUMP_STEPS="0,1,2,10" (steps to dump), MPD_REF_MAX_STEP
+// (exit(0) when reached), MPD_REF_DUMP_DIR (output directory).
+//
+// Dump layout (little-endian): int32 nx, ny, nfields, then nfields planes of
+// nx*ny float64 in structured order idx = j*nx + i, with (i,j) taken from the
+// reference's own CarCellArray[i+1][j+1] (Ark/Cpp/Mesh/MeshCartesian.cpp:369-408).
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <set>
+#include <string>
+#include <vector>
+#include "DUGKSDeclaration.h"
+
+// --wrap acts on linker symbols, so the wrappers use the Itanium-mangled names
+// of `void Update_phi_Eq(Cell_2D&)` and `void Output_Flowfield(double const&, int)`.
+extern "C" void __real__Z13Update_phi_EqR7Cell_2D(Cell_2D &cell);
+extern "C" void __real__Z16Output_FlowfieldRKdi(double const &t, int step);
+
+namespace {
+bool g_started = false;
+int g_steps_seen = -1;
+std::set<int> g_dump;
+int g_max_step = -1;
+std::string g_dir = ".";
+
+void parse_env()
+{
+    static bool done = false;
+    if (done) return;
+    done = true;
+    if (const char *s = std::getenv("MPD_REF_DUMP_STEPS")) {
+        std::string v(s);
+        size_t p = 0;
+        while (p < v.size()) {
+            size_t q = v.find(',', p);
+            if (q == std::string::npos) q = v.size();
+            if (q > p) g_dump.insert(std::atoi(v.substr(p, q - p).c_str()));
+            p = q + 1;
+        }
+    }
+    if (const char *s = std::getenv("MPD_REF_MAX_STEP")) g_max_step = std::atoi(s);
+    if (const char *s = std::getenv("MPD_REF_DUMP_DIR")) g_dir = s;
+}
+
+void dump_cells(int k)
+{
+    const int nx = Nx, ny = Ny;
+    std::vector<std::vector<double>> planes;
+    std::vector<std::string> names;
+    auto add = [&](const char *name) {
+        names.push_back(name);
+        planes.emplace_back(static_cast<size_t>(nx) * ny, 0.0);
+        return planes.size() - 1;
+    };
+#ifdef _ARK_ALLENCAHN_FLIP
+    size_t ph = add("h");
+    for (int q = 1; q < DV_Qv; ++q) add("h");
+#endif
+    size_t pf = add("f");
+    for (int q = 1; q < DV_Qv; ++q) add("f");
+    const char *scal[] = {"Phi", "Rho", "U", "V", "p", "Phi_x", "Phi_y", "Rho_x", "Rho_y",
+                          "Fx", "Fy", "laplacianPhi", "prevPhiU", "prevPhiV", "Psi", "scG",
+                          "Mu", "ftau", "faBP", "fbBP", "fcBP", "haBP", "hbBP", "hcBP",
+                          "xc", "yc"};
+    size_t ps = planes.size();
+    for (const char *n : scal) add(n);
+    for (int j = 0; j < ny; ++j)
+        for (int i = 0; i < nx; ++i) {
+            Cell_2D const &c = *CarCellArray[i + 1][j + 1];
+            size_t idx = static_cast<size_t>(j) * nx + i;
+            for (int q = 0; q < DV_Qv; ++q) {
+#ifdef _ARK_ALLENCAHN_FLIP
+                planes[ph + q][idx] = c.h.Tilde[0][q];
+#endif
+                planes[pf + q][idx] = c.f.Tilde[0][q];
+            }
+            MacroQuantity const &m = c.MsQ();
+            double v[] = {m.Phi, m.Rho, m.U, m.V, m.p, m.Phi_x, m.Phi_y, m.Rho_x, m.Rho_y,
+                          m.Fx, m.Fy, m.laplacianPhi, m.prevPhiU, m.prevPhiV, m.Psi, m.scG,
+                          m.Mu, c.f.tau, c.f.aBP, c.f.bBP, c.f.cBP,
+#ifdef _ARK_ALLENCAHN_FLIP
+                          c.h.aBP, c.h.bBP, c.h.cBP,
+#else
+                          0.0, 0.0, 0.0,
+#endif
+                          c.xc, c.yc};
+            for (size_t s = 0; s < sizeof(v) / sizeof(v[0]); ++s) planes[ps + s][idx] = v[s];
+        }
+    char path[4096];
+    std::snprintf(path, sizeof(path), "%s/cells_step%d.bin", g_dir.c_str(), k);
+    FILE *fp = std::fopen(path, "wb");
+    if (!fp) { std::perror(path); std::exit(3); }
+    int hdr[3] = {nx, ny, static_cast<int>(planes.size())};
+    std::fwrite(hdr, sizeof(int), 3, fp);
+    for (auto &p : planes) std::fwrite(p.data(), sizeof(double), p.size(), fp);
+    std::fclose(fp);
+
+    // Face state (one record per face in FaceArray order): geometry plus the
+    // persistent face macros the pseudopotential model carries between steps.
+    std::snprintf(path, sizeof(path), "%s/faces_step%d.bin", g_dir.c_str(), k);
+    fp = std::fopen(path, "wb");
+    if (!fp) { std::perror(path); std::exit(3); }
+    int fh[2] = {Faces, 12};
+    std::fwrite(fh, sizeof(int), 2, fp);
+    for (int n = 0; n < Faces; ++n) {
+        Face_2D const &f = FaceArray[n];
+        MacroQuantity const &m = f.MsQh();
+        double v[12] = {f.xf, f.yf, f.Vx, f.Vy, static_cast<double>(f.bc_type),
+                        m.Rho, m.U, m.V, m.p, m.Psi, m.Fx, m.Fy};
+        std::fwrite(v, sizeof(double), 12, fp);
+    }
+    std::fclose(fp);
+    std::fprintf(stderr, "[ref_harness] dumped step %d\n", k);
+}
+}  // namespace
+
+extern "C" void __wrap__Z16Output_FlowfieldRKdi(double const &t, int step)
+{
+    parse_env();
+    g_started = true;
+    if (const char *s = std::getenv("MPD_REF_FLOWFIELD"))
+        if (s[0] == '1') __real__Z16Output_FlowfieldRKdi(t, step);
+}
+
+extern "C" void __wrap__Z13Update_phi_EqR7Cell_2D(Cell_2D &cell)
+{
+    if (g_started && &cell == &CellArray[0]) {
+        ++g_steps_seen;
+        if (g_dump.count(g_steps_seen)) dump_cells(g_steps_seen);
+        if (g_max_step >= 0 && g_steps_seen >= g_max_step) std::exit(0);
+    }
+    __real__Z13Update_phi_EqR7Cell_2D(cell);
+}

@@ -1,0 +1,95 @@
+"""Build and run the reference solver (oracle/_ref) and read its state dumps.
+
+Test infrastructure only: used in this container to pin the oracle and to
+write golden fixtures; never imported by the product, and never needed on a
+machine without the reference sources.
+
+A run directory is laid out the way the reference expects
+(Ark/Cpp/Mesh/MeshConstruct.cpp:74-77 reads $HOME/Mesh/<name>.cas; outputs go
+to ../FlowField/{global,Convergence} relative to the working directory,
+Ark/Header/ZeroMacros.h:24).  stdin is /dev/null so the reference's
+getchar() pauses return at once.
+"""
+import os
+import subprocess
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+REF_ROOT = "/root/reference/Ark"
+MESH_NAME = "128_128_Car_Periodic_Square_LBM"  # Ark/Constant/ZeroFlip.h:47-49
+
+# name -> (model translation unit, extra compile flags, walls in y)
+CONFIGS = {
+    "hcz": ("D2Q9HCZ", "", False),
+    "acp": ("D2Q9AC", "", False),
+    "hcz_wall": ("D2Q9HCZ", "-D_Wall_3_BCs_FLIP=3 -D_Wall_3_BCs_BB", True),
+}
+
+CELL_SCALARS = ["Phi", "Rho", "U", "V", "p", "Phi_x", "Phi_y", "Rho_x", "Rho_y",
+                "Fx", "Fy", "laplacianPhi", "prevPhiU", "prevPhiV", "Psi", "scG",
+                "Mu", "ftau", "faBP", "fbBP", "fcBP", "haBP", "hbBP", "hcBP", "xc", "yc"]
+
+
+def available():
+    return os.path.isdir(REF_ROOT)
+
+
+def build(cfg):
+    model, defs, _ = CONFIGS[cfg]
+    cmd = ["make", "-s", "-f", os.path.join(HERE, "Makefile.ref"), "-j8",
+           "CFG=" + cfg, "MODEL=" + model, "DEFS=" + defs]
+    subprocess.run(cmd, check=True, cwd=os.path.dirname(HERE))
+    return os.path.join(HERE, "_ref", cfg, "ark")
+
+
+def run(cfg, steps, max_step, outdir=None, flowfield=False, timeout=3600):
+    """Run config `cfg`, dumping the cell state after each step in `steps`."""
+    exe = build(cfg)
+    base = os.path.join(HERE, "_ref", cfg, "run")
+    home = os.path.join(base, "home")
+    work = os.path.join(base, "work")
+    for d in (os.path.join(home, "Mesh"), work, os.path.join(base, "FlowField", "global"),
+              os.path.join(base, "FlowField", "Convergence")):
+        os.makedirs(d, exist_ok=True)
+    sys.path.insert(0, HERE)
+    from casmesh import write_cas
+    write_cas(os.path.join(home, "Mesh", MESH_NAME + ".cas"), 128, 128,
+              walls_y=CONFIGS[cfg][2])
+    outdir = outdir or os.path.join(base, "dump")
+    os.makedirs(outdir, exist_ok=True)
+    env = dict(os.environ)
+    env.update({"HOME": home, "OMP_NUM_THREADS": "1",
+                "MPD_REF_DUMP_STEPS": ",".join(str(s) for s in steps),
+                "MPD_REF_MAX_STEP": str(max_step), "MPD_REF_DUMP_DIR": outdir,
+                "MPD_REF_FLOWFIELD": "1" if flowfield else "0"})
+    with open(os.devnull) as dn, open(os.path.join(base, "stdout.log"), "w") as log:
+        subprocess.run([exe], cwd=work, env=env, stdin=dn, stdout=log,
+                       stderr=subprocess.STDOUT, check=True, timeout=timeout)
+    return outdir
+
+
+def load_cells(path):
+    """Return dict of (ny, nx) float64 arrays from a cells_step*.bin dump."""
+    with open(path, "rb") as fh:
+        nx, ny, nf = np.frombuffer(fh.read(12), dtype=np.int32)
+        data = np.frombuffer(fh.read(), dtype=np.float64).reshape(nf, ny, nx)
+    out = {}
+    k = 0
+    nq = 9
+    if nf == 2 * nq + len(CELL_SCALARS):
+        out["h"] = data[0:nq].copy()
+        k = nq
+    out["f"] = data[k:k + nq].copy()
+    k += nq
+    for name in CELL_SCALARS:
+        out[name] = data[k].copy()
+        k += 1
+    return out
+
+
+if __name__ == "__main__":
+    cfg = sys.argv[1]
+    steps = [int(s) for s in sys.argv[2].split(",")]
+    print(run(cfg, steps, max(steps)))
