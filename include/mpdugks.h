@@ -106,6 +106,13 @@ typedef enum mpd_macro_field {
     MPD_M_COUNT
 } mpd_macro_field;
 
+/* Kernel path used by mpd_step. */
+typedef enum mpd_path {
+    MPD_PATH_AUTO = 0,       /* fused step kernel where available, else phases */
+    MPD_PATH_PHASES = 1,     /* one kernel per reference phase (plain FP64 path) */
+    MPD_PATH_FUSED = 2       /* one fused kernel per step */
+} mpd_path;
+
 /* Global moments returned by mpd_reductions (FP64, compensated sums). */
 typedef struct mpd_sums {
     double sum_rho;          /* sum of rho over cells */
@@ -132,6 +139,9 @@ int mpd_default_config(int32_t model, mpd_config *cfg);
  * AllocateResource (Ark/Cpp/main.cpp:150-186) for the step's state. */
 int mpd_create(const mpd_config *cfg, mpd_handle **out);
 int mpd_destroy(mpd_handle *h);
+
+/* Select the kernel path (mpd_path); valid before the first mpd_step. */
+int mpd_set_path(mpd_handle *h, int32_t path);
 
 /* Upload host state (FP64).  tilde_h may be NULL for the PP models;
  * face_fx/face_fy (PP only, may be NULL otherwise) are the persistent face
@@ -162,6 +172,25 @@ int mpd_device_bytes(const mpd_handle *h, int64_t *bytes);
 /* Average duration (ms) of the step kernel launches since the last call,
  * measured with HIP events on the handle's stream; resets the counters. */
 int mpd_kernel_timing(mpd_handle *h, double *avg_ms, int64_t *launches);
+
+/* ---- host-side initial states (upload layout; no device work) ---- */
+
+/* AC_Drop (Ark/Cpp/Preprocess.cpp:705-760, AnalyticalPhiAC_Drop :402-419) on the
+ * mesh of `cfg`: phi = (PhiL+PhiV)/2 + sign*(PhiL-PhiV)/2*tanh(2(r-R)/W).
+ * sign = +1 is the reference's case (light phase inside); -1 gives a droplet.
+ * Macros start from p = p0, U = V = 0, Mu = mu0, f.tau = mu0/(rho RT). */
+int mpd_init_ac_drop(const mpd_config *cfg, double cx, double cy, double radius, int32_t sign,
+                     double p0, double mu0, double *tilde_h, double *tilde_f, double *macro);
+
+/* SCMP_Drop (Ark/Cpp/Preprocess.cpp:337-401) for the pseudopotential model. */
+int mpd_init_pp_drop(const mpd_config *cfg, double cx, double cy, double radius, double rho_l,
+                     double rho_v, double width, double mu0, double *tilde_f, double *macro,
+                     double *face_fx, double *face_fy);
+
+/* Pseudopotential liquid-vapour start: rho = rho0*(1 + U(-amp, amp)) per cell,
+ * std::mt19937_64(seed), u = 0, tilde = equilibrium. */
+int mpd_init_pp_noise(const mpd_config *cfg, double rho0, double amp, uint64_t seed, double mu0,
+                      double *tilde_f, double *macro, double *face_fx, double *face_fy);
 
 /* Text of the last error on this thread. */
 const char *mpd_last_error(void);

@@ -47,3 +47,5 @@ def num_q(model):
 
 def has_h(model):
     return model in (MPD_MODEL_AC_P, MPD_MODEL_AC_HCZ)
+
+MPD_PATH_AUTO, MPD_PATH_PHASES, MPD_PATH_FUSED = 0, 1, 2

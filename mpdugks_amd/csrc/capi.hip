@@ -1,0 +1,487 @@
+// C ABI of the DUGKS step library (include/mpdugks.h): handle life cycle,
+// state upload/download, stepping and global reductions.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/mpdugks.h"
+#include "device_state.h"
+#include "fused.h"
+#include "physics.h"
+
+using namespace mpd;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const char *fmt, ...)
+{
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    std::vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    g_err = buf;
+    return code;
+}
+
+#define HIPCHK(call)                                                                          \
+    do {                                                                                      \
+        hipError_t e_ = (call);                                                               \
+        if (e_ != hipSuccess) return fail(MPD_ERR_DEVICE, "%s: %s", #call, hipGetErrorString(e_)); \
+    } while (0)
+
+}  // namespace
+
+struct mpd_handle {
+    mpd_config cfg;
+    Consts C;
+    DevState S;
+    FusedState F;
+    hipStream_t stream = nullptr;
+    int device = 0;
+    int path = MPD_PATH_AUTO;
+    int effective_path = MPD_PATH_PHASES;
+    bool uploaded = false;
+    bool fused_primed = false;   // fused state derived from the explicit macros
+    std::vector<void *> allocs;
+    int64_t bytes = 0;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    double acc_ms = 0.0;
+    int64_t launches = 0;
+    double *d_part = nullptr;
+    int red_blocks = 0;
+};
+
+namespace {
+
+int dalloc(mpd_handle *h, double **p, size_t n)
+{
+    *p = nullptr;
+    if (n == 0) return 0;
+    void *q = nullptr;
+    hipError_t e = hipMalloc(&q, n * sizeof(double));
+    if (e != hipSuccess) return fail(MPD_ERR_NOMEM, "hipMalloc(%zu doubles): %s", n, hipGetErrorString(e));
+    e = hipMemsetAsync(q, 0, n * sizeof(double), h->stream);
+    if (e != hipSuccess) return fail(MPD_ERR_DEVICE, "hipMemsetAsync: %s", hipGetErrorString(e));
+    h->allocs.push_back(q);
+    h->bytes += (int64_t)(n * sizeof(double));
+    *p = (double *)q;
+    return 0;
+}
+
+void make_consts(const mpd_config &c, Consts &C)
+{
+    std::memset(&C, 0, sizeof(C));
+    C.nx = c.nx;
+    C.ny = c.ny;
+    C.model = c.model;
+    C.flux = c.flux;
+    C.walls = c.bc_y == MPD_BC_WALL_BB;
+    C.nyf = C.walls ? c.ny + 1 : c.ny;
+    C.dx = c.dx;
+    C.dt = c.dt;
+    C.hdt = 0.5 * c.dt;
+    C.rt = c.rt;
+    C.lambda0 = 1 / (2.0 * c.rt);
+    C.aphi = (c.rho_l - c.rho_v) / (c.phi_l - c.phi_v);
+    C.bphi = c.rho_v - C.aphi * c.phi_v;
+    C.beta = 12.0 * c.sigma / c.width;
+    C.kappa = 3.0 * c.sigma * c.width / 2;
+    C.kp = c.rt * 9 / 5.0;
+    C.taum = c.mobility / c.rt;
+    C.tau0 = c.tau0;
+    C.phi_l = c.phi_l;
+    C.phi_v = c.phi_v;
+    C.nu_l = c.nu_l;
+    C.nu_v = c.nu_v;
+    C.gx = c.gx;
+    C.gy = c.gy;
+    C.rho_ref = c.rho_ref;
+    C.pp_k = c.pp_k;
+    C.t_cs = c.pp_tr * 0.0943;
+    C.pp_tau = c.pp_tau;
+    C.dtv = c.dt / (c.dx * c.dx);
+    C.d2x6 = (2.0 * c.dx) * 6;
+    C.lapden = 6 * c.dx * c.dx;
+    C.rtdt = c.rt * c.dt;
+    C.wdphi = c.width * (c.phi_l - c.phi_v);
+    C.twodx = 2 * c.dx;
+    C.ah_h = 2.0 * C.taum / (2.0 * C.taum + C.hdt);
+    C.bh_h = 1.0 - C.ah_h;
+    C.ch_h = C.taum * C.bh_h;
+    C.abp_h = (2.0 * C.taum - C.hdt) / (2.0 * C.taum + c.dt);
+    C.bbp_h = 1.0 - C.abp_h;
+    C.cbp_h = C.taum * C.bbp_h;
+    C.ah_pp = 2.0 * c.pp_tau / (2.0 * c.pp_tau + C.hdt);
+    C.bh_pp = 1.0 - C.ah_pp;
+    C.ch_pp = c.pp_tau * C.bh_pp;
+    double maspan = std::sqrt(3.0 * c.rt);
+    const double w9[Q9] = {4.0 / 9.0, 1.0 / 9.0, 1.0 / 36.0, 1.0 / 9.0, 1.0 / 36.0,
+                           1.0 / 9.0, 1.0 / 36.0, 1.0 / 9.0, 1.0 / 36.0};
+    for (int q = 0; q < Q9; ++q) {
+        C.xi[0][q] = maspan * ex9(q);
+        C.xi[1][q] = maspan * ey9(q);
+        C.w[q] = w9[q];
+    }
+}
+
+// ---------------- reductions ----------------
+
+constexpr int RED_THREADS = 256;
+constexpr int RED_FIELDS = 9;  // rho, rho u, rho v, phi, KE, max|u|, dRho^2, rho^2, nonfinite
+
+__device__ inline void neumaier(double &s, double &c, double x)
+{
+    double t = s + x;
+    if (fabs(s) >= fabs(x)) c += (s - t) + x;
+    else c += (x - t) + s;
+    s = t;
+}
+
+__global__ void __launch_bounds__(RED_THREADS) k_reduce(const double *m, double *rho1k, int64_t N,
+                                                        int update, double *part)
+{
+    double s[RED_FIELDS - 1] = {0}, c[RED_FIELDS - 1] = {0};
+    double vmax = 0.0, bad = 0.0;
+    for (int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; n < N;
+         n += (int64_t)gridDim.x * blockDim.x) {
+        double rho = m[MPD_M_RHO * N + n], U = m[MPD_M_U * N + n], V = m[MPD_M_V * N + n];
+        double phi = m[MPD_M_PHI * N + n];
+        bool ok = isfinite(rho) && isfinite(U) && isfinite(V) && isfinite(phi) && rho > 0.0;
+        if (!ok) { bad += 1.0; continue; }
+        double d = rho - rho1k[n];
+        neumaier(s[0], c[0], rho);
+        neumaier(s[1], c[1], rho * U);
+        neumaier(s[2], c[2], rho * V);
+        neumaier(s[3], c[3], phi);
+        neumaier(s[4], c[4], 0.5 * rho * (U * U + V * V));
+        neumaier(s[5], c[5], d * d);
+        neumaier(s[6], c[6], rho * rho);
+        vmax = fmax(vmax, sqrt(U * U + V * V));
+        if (update) rho1k[n] = rho;
+    }
+    __shared__ double sh[RED_FIELDS][RED_THREADS];
+    for (int k = 0; k < 7; ++k) sh[k][threadIdx.x] = s[k] + c[k];
+    sh[7][threadIdx.x] = vmax;
+    sh[8][threadIdx.x] = bad;
+    __syncthreads();
+    for (int w = RED_THREADS / 2; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) {
+            for (int k = 0; k < 7; ++k) sh[k][threadIdx.x] += sh[k][threadIdx.x + w];
+            sh[7][threadIdx.x] = fmax(sh[7][threadIdx.x], sh[7][threadIdx.x + w]);
+            sh[8][threadIdx.x] += sh[8][threadIdx.x + w];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x < RED_FIELDS) part[(size_t)blockIdx.x * RED_FIELDS + threadIdx.x] = sh[threadIdx.x][0];
+}
+
+int check_cfg(const mpd_config *c)
+{
+    if (!c) return fail(MPD_ERR_INVALID, "null config");
+    if (c->nx < 4 || c->ny < 4) return fail(MPD_ERR_INVALID, "mesh %dx%d too small (need >= 4x4)", c->nx, c->ny);
+    if ((int64_t)c->nx * c->ny > (int64_t)1 << 30) return fail(MPD_ERR_INVALID, "mesh too large");
+    if (c->model < MPD_MODEL_AC_P || c->model > MPD_MODEL_PP_D2V16)
+        return fail(MPD_ERR_INVALID, "unknown model %d", c->model);
+    if (c->model == MPD_MODEL_PP_D2V16) return fail(MPD_ERR_INVALID, "D2V16 is served by mpd_d2v16_* entry points");
+    if (c->flux != MPD_FLUX_CD && c->flux != MPD_FLUX_UW) return fail(MPD_ERR_INVALID, "unknown flux %d", c->flux);
+    if (c->bc_y != MPD_BC_PERIODIC && c->bc_y != MPD_BC_WALL_BB) return fail(MPD_ERR_INVALID, "unknown bc %d", c->bc_y);
+    if (c->bc_y == MPD_BC_WALL_BB && (c->flux == MPD_FLUX_UW || c->model == MPD_MODEL_PP_D2Q9))
+        return fail(MPD_ERR_INVALID, "walls are supported with CD and the Allen-Cahn models only");
+    if (c->precision != MPD_FP64 && c->precision != MPD_FP32_BULK)
+        return fail(MPD_ERR_INVALID, "unknown precision %d", c->precision);
+    if (!(c->dx > 0) || !(c->dt > 0) || !(c->rt > 0)) return fail(MPD_ERR_INVALID, "dx, dt, rt must be > 0");
+    if (c->phi_l == c->phi_v || !(c->width > 0)) return fail(MPD_ERR_INVALID, "phi_l == phi_v or width <= 0");
+    return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int mpd_abi_version(void) { return MPDUGKS_ABI_VERSION; }
+
+const char *mpd_last_error(void) { return g_err.c_str(); }
+
+int mpd_default_config(int32_t model, mpd_config *c)
+{
+    if (!c) return fail(MPD_ERR_INVALID, "null config");
+    std::memset(c, 0, sizeof(*c));
+    const double ch_length = 128.0, min_l = ch_length / 128;
+    const double t0 = 1.0 / 3, r0 = 1.0, rt = r0 * t0;
+    const double mu0 = 0.1, rho0 = 1.0, nu0 = mu0 / rho0, tau0 = nu0 / rt;
+    const double maxu = std::sqrt(3.0 * rt), cfl = 0.5;
+    const double cn = 5 / ch_length;
+    c->nx = 128;
+    c->ny = 128;
+    c->model = model;
+    c->flux = MPD_FLUX_CD;
+    c->bc_y = MPD_BC_PERIODIC;
+    c->precision = MPD_FP64;
+    c->dx = min_l;
+    c->dt = cfl > 0.0 ? cfl * min_l / maxu : 1.0e-4;
+    c->rt = rt;
+    c->phi_l = 1.0;
+    c->phi_v = 0.0;
+    c->rho_l = 10.0;
+    c->rho_v = 1.0;
+    c->nu_l = nu0;
+    c->nu_v = nu0;
+    c->mobility = 0.1;
+    c->width = ch_length * cn;
+    c->sigma = 1e-2;
+    c->tau0 = tau0;
+    c->pp_tr = 0.85;
+    c->pp_k = 1.0;
+    c->pp_tau = tau0;
+    if (model == MPD_MODEL_PP_D2V16) c->dt = cfl * min_l / (2 * 1.2);
+    return 0;
+}
+
+int mpd_create(const mpd_config *cfg, mpd_handle **out)
+{
+    if (!out) return fail(MPD_ERR_INVALID, "null output pointer");
+    *out = nullptr;
+    if (int r = check_cfg(cfg)) return r;
+    mpd_handle *h = new (std::nothrow) mpd_handle();
+    if (!h) return fail(MPD_ERR_NOMEM, "host allocation failed");
+    h->cfg = *cfg;
+    make_consts(*cfg, h->C);
+    int r = 0;
+    hipError_t e = hipGetDevice(&h->device);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipEventCreate(&h->ev0);
+    if (e == hipSuccess) e = hipEventCreate(&h->ev1);
+    if (e != hipSuccess) {
+        r = fail(MPD_ERR_DEVICE, "HIP set-up failed: %s", hipGetErrorString(e));
+        mpd_destroy(h);
+        return r;
+    }
+    const size_t N = (size_t)cfg->nx * cfg->ny, QN = (size_t)Q9 * N;
+    const size_t yfn = (size_t)h->C.nyf * cfg->nx;
+    DevState &S = h->S;
+    S.N = (int64_t)N;
+    bool ac = cfg->model != MPD_MODEL_PP_D2Q9, pp = !ac, uw = cfg->flux == MPD_FLUX_UW;
+    if ((r = dalloc(h, &S.h, ac ? QN : 0)) || (r = dalloc(h, &S.f, QN)) ||
+        (r = dalloc(h, &S.m, (size_t)MPD_M_COUNT * N)) || (r = dalloc(h, &S.ffx, pp ? 2 * N : 0)) ||
+        (r = dalloc(h, &S.ffy, pp ? 2 * N : 0)) || (r = dalloc(h, &S.rho1k, N))) {
+        mpd_destroy(h);
+        return r;
+    }
+    // phase-path scratch is allocated lazily in mpd_step (fused path does not need it)
+    (void)QN; (void)yfn; (void)uw;
+    h->red_blocks = 512;
+    if ((r = dalloc(h, &h->d_part, (size_t)h->red_blocks * RED_FIELDS))) {
+        mpd_destroy(h);
+        return r;
+    }
+    *out = h;
+    return 0;
+}
+
+int mpd_destroy(mpd_handle *h)
+{
+    if (!h) return 0;
+    if (h->stream) (void)hipStreamSynchronize(h->stream);
+    for (void *p : h->allocs) (void)hipFree(p);
+    fused_free(h->F);
+    if (h->ev0) (void)hipEventDestroy(h->ev0);
+    if (h->ev1) (void)hipEventDestroy(h->ev1);
+    if (h->stream) (void)hipStreamDestroy(h->stream);
+    delete h;
+    return 0;
+}
+
+int mpd_set_path(mpd_handle *h, int32_t path)
+{
+    if (!h) return fail(MPD_ERR_INVALID, "null handle");
+    if (path < MPD_PATH_AUTO || path > MPD_PATH_FUSED) return fail(MPD_ERR_INVALID, "unknown path %d", path);
+    if (path == MPD_PATH_FUSED && !fused_supported(h->cfg))
+        return fail(MPD_ERR_INVALID, "no fused kernel for this configuration");
+    if (h->cfg.precision == MPD_FP32_BULK && path == MPD_PATH_PHASES)
+        return fail(MPD_ERR_INVALID, "the phase path is FP64 only");
+    h->path = path;
+    return 0;
+}
+
+static int alloc_phase_scratch(mpd_handle *h)
+{
+    DevState &S = h->S;
+    if (S.fbp) return 0;
+    const size_t N = S.N, QN = (size_t)Q9 * N, yfn = (size_t)h->C.nyf * h->cfg.nx;
+    bool ac = h->cfg.model != MPD_MODEL_PP_D2Q9, pp = !ac, uw = h->cfg.flux == MPD_FLUX_UW;
+    int r;
+    if ((r = dalloc(h, &S.hbp, ac ? QN : 0)) || (r = dalloc(h, &S.fbp, QN)) ||
+        (r = dalloc(h, &S.hbpx, ac && uw ? QN : 0)) || (r = dalloc(h, &S.hbpy, ac && uw ? QN : 0)) ||
+        (r = dalloc(h, &S.fbpx, uw ? QN : 0)) || (r = dalloc(h, &S.fbpy, uw ? QN : 0)) ||
+        (r = dalloc(h, &S.hfl, ac ? QN + (size_t)Q9 * yfn : 0)) ||
+        (r = dalloc(h, &S.ffl, QN + (size_t)Q9 * yfn)) ||
+        (r = dalloc(h, &S.gm, h->C.walls ? (size_t)MPD_M_COUNT * 2 * h->cfg.nx : 0)) ||
+        (r = dalloc(h, &S.ghbp, h->C.walls ? (size_t)Q9 * 2 * h->cfg.nx : 0)) ||
+        (r = dalloc(h, &S.gfbp, h->C.walls ? (size_t)Q9 * 2 * h->cfg.nx : 0)) ||
+        (r = dalloc(h, &S.fpsi, pp ? 2 * N : 0)) || (r = dalloc(h, &S.fscr, pp ? 8 * N : 0)))
+        return r;
+    return 0;
+}
+
+int mpd_upload_state(mpd_handle *h, const double *tilde_h, const double *tilde_f, const double *macro,
+                     const double *face_fx, const double *face_fy)
+{
+    if (!h) return fail(MPD_ERR_INVALID, "null handle");
+    if (!tilde_f || !macro) return fail(MPD_ERR_INVALID, "tilde_f and macro are required");
+    bool ac = h->cfg.model != MPD_MODEL_PP_D2Q9;
+    if (ac && !tilde_h) return fail(MPD_ERR_INVALID, "tilde_h is required for the Allen-Cahn models");
+    const size_t N = h->S.N, QN = (size_t)Q9 * N;
+    if (ac) HIPCHK(hipMemcpyAsync(h->S.h, tilde_h, QN * sizeof(double), hipMemcpyHostToDevice, h->stream));
+    HIPCHK(hipMemcpyAsync(h->S.f, tilde_f, QN * sizeof(double), hipMemcpyHostToDevice, h->stream));
+    HIPCHK(hipMemcpyAsync(h->S.m, macro, (size_t)MPD_M_COUNT * N * sizeof(double), hipMemcpyHostToDevice,
+                          h->stream));
+    if (!ac) {
+        if (face_fx) HIPCHK(hipMemcpyAsync(h->S.ffx, face_fx, 2 * N * sizeof(double), hipMemcpyHostToDevice, h->stream));
+        else HIPCHK(hipMemsetAsync(h->S.ffx, 0, 2 * N * sizeof(double), h->stream));
+        if (face_fy) HIPCHK(hipMemcpyAsync(h->S.ffy, face_fy, 2 * N * sizeof(double), hipMemcpyHostToDevice, h->stream));
+        else HIPCHK(hipMemsetAsync(h->S.ffy, 0, 2 * N * sizeof(double), h->stream));
+    }
+    // Rho_1k starts at 0 in the reference (Ark/Header/Macroscopic.h:44)
+    HIPCHK(hipMemsetAsync(h->S.rho1k, 0, N * sizeof(double), h->stream));
+    HIPCHK(hipStreamSynchronize(h->stream));
+    h->uploaded = true;
+    h->fused_primed = false;
+    return 0;
+}
+
+// Bring the explicit macro arrays up to date (fused path keeps a derived state).
+static int sync_macros(mpd_handle *h)
+{
+    if (h->effective_path == MPD_PATH_FUSED && h->fused_primed) {
+        int r = fused_materialize(h->F, h->S, h->C, h->cfg, h->stream);
+        if (r) return fail(MPD_ERR_DEVICE, "materialize: %s", hipGetErrorString(hipGetLastError()));
+    }
+    return 0;
+}
+
+int mpd_download_state(mpd_handle *h, double *tilde_h, double *tilde_f, double *macro, double *face_fx,
+                       double *face_fy)
+{
+    if (!h) return fail(MPD_ERR_INVALID, "null handle");
+    if (!h->uploaded) return fail(MPD_ERR_STATE, "no state uploaded");
+    if (int r = sync_macros(h)) return r;
+    const size_t N = h->S.N, QN = (size_t)Q9 * N;
+    bool ac = h->cfg.model != MPD_MODEL_PP_D2Q9;
+    if (tilde_h && ac) HIPCHK(hipMemcpyAsync(tilde_h, h->S.h, QN * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+    if (tilde_f) HIPCHK(hipMemcpyAsync(tilde_f, h->S.f, QN * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+    if (macro)
+        HIPCHK(hipMemcpyAsync(macro, h->S.m, (size_t)MPD_M_COUNT * N * sizeof(double), hipMemcpyDeviceToHost,
+                              h->stream));
+    if (!ac && face_fx) HIPCHK(hipMemcpyAsync(face_fx, h->S.ffx, 2 * N * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+    if (!ac && face_fy) HIPCHK(hipMemcpyAsync(face_fy, h->S.ffy, 2 * N * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(hipStreamSynchronize(h->stream));
+    return 0;
+}
+
+int mpd_step(mpd_handle *h, int64_t nsteps)
+{
+    if (!h) return fail(MPD_ERR_INVALID, "null handle");
+    if (!h->uploaded) return fail(MPD_ERR_STATE, "no state uploaded");
+    if (nsteps < 0) return fail(MPD_ERR_INVALID, "nsteps < 0");
+    if (nsteps == 0) return 0;
+    int path = h->path;
+    if (path == MPD_PATH_AUTO) path = fused_supported(h->cfg) ? MPD_PATH_FUSED : MPD_PATH_PHASES;
+    if (h->cfg.precision == MPD_FP32_BULK && path != MPD_PATH_FUSED)
+        return fail(MPD_ERR_INVALID, "FP32-bulk needs the fused path");
+    if (path != h->effective_path && h->fused_primed) {
+        // leaving the fused path: write its derived state back as explicit macros
+        if (int r = sync_macros(h)) return r;
+        h->fused_primed = false;
+    }
+    h->effective_path = path;
+    HIPCHK(hipEventRecord(h->ev0, h->stream));
+    if (path == MPD_PATH_PHASES) {
+        if (int r = alloc_phase_scratch(h)) return r;
+        for (int64_t s = 0; s < nsteps; ++s) phase_step(h->S, h->C, h->stream);
+    } else {
+        if (!h->F.ready) {
+            int r = fused_alloc(h->F, h->S, h->C, h->cfg);
+            if (r) return fail(MPD_ERR_NOMEM, "fused path allocation failed");
+        }
+        for (int64_t s = 0; s < nsteps; ++s) {
+            fused_step(h->F, h->S, h->C, h->cfg, !h->fused_primed, h->stream);
+            h->fused_primed = true;
+        }
+    }
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipEventRecord(h->ev1, h->stream));
+    HIPCHK(hipEventSynchronize(h->ev1));
+    float ms = 0.f;
+    HIPCHK(hipEventElapsedTime(&ms, h->ev0, h->ev1));
+    h->acc_ms += ms;
+    h->launches += nsteps;
+    return 0;
+}
+
+int mpd_kernel_timing(mpd_handle *h, double *avg_ms, int64_t *launches)
+{
+    if (!h) return fail(MPD_ERR_INVALID, "null handle");
+    if (avg_ms) *avg_ms = h->launches ? h->acc_ms / (double)h->launches : 0.0;
+    if (launches) *launches = h->launches;
+    h->acc_ms = 0.0;
+    h->launches = 0;
+    return 0;
+}
+
+int mpd_synchronize(mpd_handle *h)
+{
+    if (!h) return fail(MPD_ERR_INVALID, "null handle");
+    HIPCHK(hipStreamSynchronize(h->stream));
+    return 0;
+}
+
+int mpd_device_bytes(const mpd_handle *h, int64_t *bytes)
+{
+    if (!h || !bytes) return fail(MPD_ERR_INVALID, "null argument");
+    *bytes = h->bytes + fused_bytes(h->F);
+    return 0;
+}
+
+int mpd_reductions(mpd_handle *h, int update_residual, mpd_sums *out)
+{
+    if (!h || !out) return fail(MPD_ERR_INVALID, "null argument");
+    if (!h->uploaded) return fail(MPD_ERR_STATE, "no state uploaded");
+    if (int r = sync_macros(h)) return r;
+    hipLaunchKernelGGL(k_reduce, dim3(h->red_blocks), dim3(RED_THREADS), 0, h->stream, h->S.m, h->S.rho1k,
+                       h->S.N, update_residual, h->d_part);
+    HIPCHK(hipGetLastError());
+    std::vector<double> part((size_t)h->red_blocks * RED_FIELDS);
+    HIPCHK(hipMemcpyAsync(part.data(), h->d_part, part.size() * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(hipStreamSynchronize(h->stream));
+    double s[7] = {0}, c[7] = {0}, vmax = 0.0, bad = 0.0;
+    for (int b = 0; b < h->red_blocks; ++b) {
+        for (int k = 0; k < 7; ++k) {
+            double x = part[(size_t)b * RED_FIELDS + k], t = s[k] + x;
+            if (std::fabs(s[k]) >= std::fabs(x)) c[k] += (s[k] - t) + x;
+            else c[k] += (x - t) + s[k];
+            s[k] = t;
+        }
+        vmax = std::fmax(vmax, part[(size_t)b * RED_FIELDS + 7]);
+        bad += part[(size_t)b * RED_FIELDS + 8];
+    }
+    out->sum_rho = s[0] + c[0];
+    out->sum_rho_u = s[1] + c[1];
+    out->sum_rho_v = s[2] + c[2];
+    out->sum_phi = s[3] + c[3];
+    out->kinetic_energy = s[4] + c[4];
+    out->max_speed = vmax;
+    out->residual = std::sqrt((s[5] + c[5]) / ((s[6] + c[6]) + 1.0E-30));
+    out->nonfinite = (int64_t)bad;
+    if (bad > 0) return fail(MPD_ERR_NONFINITE, "%lld cells with NaN/Inf or rho <= 0", (long long)bad);
+    return 0;
+}
+
+}  // extern "C"

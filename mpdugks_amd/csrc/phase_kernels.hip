@@ -1,0 +1,436 @@
+// Plain FP64 GPU path: one HIP kernel per phase of the reference time step
+// (Ark/Cpp/DUGKS2DSolver.cpp:164-328), structure-of-arrays state, explicit
+// cell macros exactly as the reference keeps them.  Built with
+// -ffp-contract=off so every phase reproduces the reference's FP64 results.
+//
+// This is the plain drop-in and the correctness anchor for the fused path.
+#include "device_state.h"
+#include "physics.h"
+
+namespace mpd {
+
+__device__ inline int wrapi(int i, int n) { return i < 0 ? i + n : (i >= n ? i - n : i); }
+
+// macro field `fld` at (i,j); rows -1 and ny are the wall ghosts
+__device__ inline double cellm(const DevState &S, const Consts &C, int fld, int i, int j)
+{
+    i = wrapi(i, C.nx);
+    if (j < 0 || j >= C.ny) {
+        if (!C.walls) j = wrapi(j, C.ny);
+        else return S.gm[((size_t)fld * 2 + (j < 0 ? 0 : 1)) * C.nx + i];
+    }
+    return S.m[(size_t)fld * S.N + (size_t)j * C.nx + i];
+}
+
+__device__ inline double cellbp(const DevState &S, const Consts &C, const double *bp, const double *gbp,
+                                int q, int i, int j)
+{
+    i = wrapi(i, C.nx);
+    if (j < 0 || j >= C.ny) {
+        if (!C.walls) j = wrapi(j, C.ny);
+        else return gbp[((size_t)q * 2 + (j < 0 ? 0 : 1)) * C.nx + i];
+    }
+    return bp[(size_t)q * S.N + (size_t)j * C.nx + i];
+}
+
+// phase 1: equilibria, sources and bar-plus of every cell
+template <int MODEL>
+__global__ void __launch_bounds__(256) k_phase1(DevState S, Consts C)
+{
+    size_t n = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= (size_t)S.N) return;
+    const size_t N = S.N;
+    const double *m = S.m;
+    CellMacro<double> cm;
+    cm.phi = m[MPD_M_PHI * N + n]; cm.rho = m[MPD_M_RHO * N + n];
+    cm.U = m[MPD_M_U * N + n];     cm.V = m[MPD_M_V * N + n];
+    cm.p = m[MPD_M_P * N + n];     cm.tau = m[MPD_M_TAU * N + n];
+    cm.px = m[MPD_M_PHIX * N + n]; cm.py = m[MPD_M_PHIY * N + n];
+    cm.rx = m[MPD_M_RHOX * N + n]; cm.ry = m[MPD_M_RHOY * N + n];
+    cm.Fx = m[MPD_M_FX * N + n];   cm.Fy = m[MPD_M_FY * N + n];
+    double ht[Q9], ft[Q9], hb[Q9], fb[Q9];
+#pragma unroll
+    for (int q = 0; q < Q9; ++q) {
+        ft[q] = S.f[q * N + n];
+        ht[q] = MODEL == MPD_MODEL_PP_D2Q9 ? 0.0 : S.h[q * N + n];
+    }
+    cell_barplus<double, MODEL>(C, cm, ht, ft, hb, fb);
+#pragma unroll
+    for (int q = 0; q < Q9; ++q) {
+        S.fbp[q * N + n] = fb[q];
+        if (MODEL != MPD_MODEL_PP_D2Q9) S.hbp[q * N + n] = hb[q];
+    }
+}
+
+// phase 2: wall ghosts (WallShadowC_fBP / ExtrapolationfBP, Ark/Cpp/Boundary.cpp:38-85)
+__global__ void __launch_bounds__(256) k_ghosts(DevState S, Consts C)
+{
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= 2 * C.nx) return;
+    int side = t / C.nx, i = t % C.nx;
+    int jn = side == 0 ? 0 : C.ny - 1, jnn = side == 0 ? 1 : C.ny - 2;
+    for (int fld = 0; fld < MPD_M_COUNT; ++fld) {
+        double nr = S.m[(size_t)fld * S.N + (size_t)jn * C.nx + i];
+        double nx2 = S.m[(size_t)fld * S.N + (size_t)jnn * C.nx + i];
+        double g;
+        if (fld == MPD_M_U || fld == MPD_M_V || fld == MPD_M_MU) g = -nr;
+        else if (fld == MPD_M_SCG) g = nx2;
+        else g = 2.0 * nr - nx2;
+        S.gm[((size_t)fld * 2 + side) * C.nx + i] = g;
+    }
+    for (int q = 0; q < Q9; ++q) {
+        size_t a = (size_t)q * S.N + (size_t)jn * C.nx + i, b = (size_t)q * S.N + (size_t)jnn * C.nx + i;
+        S.ghbp[((size_t)q * 2 + side) * C.nx + i] = 2 * S.hbp[a] - S.hbp[b];
+        S.gfbp[((size_t)q * 2 + side) * C.nx + i] = 2 * S.fbp[a] - S.fbp[b];
+    }
+}
+
+// phase 3 (UW only): 6-point gradients of bar-plus (Ark/Lib/GradSchemeBasic.h:95-126)
+// (periodic in both directions: the UW scheme has no wall path)
+__global__ void __launch_bounds__(256) k_grad(DevState S, Consts C, const double *bp, double *gx,
+                                              double *gy)
+{
+    size_t n = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= (size_t)S.N) return;
+    int i = (int)(n % C.nx), j = (int)(n / C.nx);
+    auto B = [&](int q, int a, int b) {
+        return bp[(size_t)q * S.N + (size_t)wrapi(b, C.ny) * C.nx + wrapi(a, C.nx)];
+    };
+    for (int q = 0; q < Q9; ++q) {
+        double c[4] = {B(q, i + 1, j), B(q, i, j + 1), B(q, i - 1, j), B(q, i, j - 1)};
+        double d[4] = {B(q, i + 1, j + 1), B(q, i - 1, j + 1), B(q, i - 1, j - 1), B(q, i + 1, j - 1)};
+        double a, b;
+        grad9(C, c, d, a, b);
+        gx[(size_t)q * S.N + n] = a;
+        gy[(size_t)q * S.N + n] = b;
+    }
+}
+
+// f-bar at a face (CD: Ark/Cpp/DUGKS2DSolver.cpp:604-687; UW: :560-603,688-706)
+template <int FLUX>
+__device__ inline double face_value(const DevState &S, const Consts &C, const double *bp, const double *gbp,
+                                    const double *bpx, const double *bpy, int dir, int q, int oi, int oj,
+                                    int ni, int nj, double xf, double yf)
+{
+    double bo = cellbp(S, C, bp, gbp, q, oi, oj), bn = cellbp(S, C, bp, gbp, q, ni, nj);
+    if (FLUX == MPD_FLUX_CD) {
+        if (dir == 0)
+            return cd_face_value<double>(C, 0, q, bo, bn, cellbp(S, C, bp, gbp, q, ni, nj + 1),
+                                         cellbp(S, C, bp, gbp, q, ni, nj - 1),
+                                         cellbp(S, C, bp, gbp, q, oi, oj + 1),
+                                         cellbp(S, C, bp, gbp, q, oi, oj - 1));
+        return cd_face_value<double>(C, 1, q, bo, bn, cellbp(S, C, bp, gbp, q, ni + 1, nj),
+                                     cellbp(S, C, bp, gbp, q, ni - 1, nj),
+                                     cellbp(S, C, bp, gbp, q, oi + 1, oj),
+                                     cellbp(S, C, bp, gbp, q, oi - 1, oj));
+    }
+    double ex = C.xi[0][q], ey = C.xi[1][q];
+    double xin = dir == 0 ? ex : ey;
+    double xo = (oi + 0.5) * C.dx, yo = (oj + 0.5) * C.dx, xn = (ni + 0.5) * C.dx, yn = (nj + 0.5) * C.dx;
+    size_t ko = (size_t)q * S.N + (size_t)wrapi(oj, C.ny) * C.nx + wrapi(oi, C.nx);
+    size_t kn = (size_t)q * S.N + (size_t)wrapi(nj, C.ny) * C.nx + wrapi(ni, C.nx);
+    double dxo = xf - C.hdt * ex - xo, dyo = yf - C.hdt * ey - yo;
+    double dxn = xf - C.hdt * ex - xn, dyn = yf - C.hdt * ey - yn;
+    if (xin > 0) return bo + (dxo * bpx[ko] + dyo * bpy[ko]);
+    if (xin < 0) return bn + (dxn * bpx[kn] + dyn * bpy[kn]);
+    return (bo + (dxo * bpx[ko] + dyo * bpy[ko]) + bn + (dxn * bpx[kn] + dyn * bpy[kn])) / 2;
+}
+
+__device__ inline void face_geom(const Consts &C, int dir, int i, int j, int &oi, int &oj, int &ni, int &nj,
+                                 double &xf, double &yf)
+{
+    if (dir == 0) { oi = i - 1; oj = j; ni = i; nj = j; xf = i * C.dx; yf = (j + 0.5) * C.dx; }
+    else { oi = i; oj = j - 1; ni = i; nj = j; xf = (i + 0.5) * C.dx; yf = j * C.dx; }
+}
+
+// phase 4 + 6 for the Allen-Cahn models: Flux_2D on every face, Wall_3_BB on
+// wall faces, times xi.n dS (Ark/Cpp/DUGKS2DSolver.cpp:219-273)
+template <int MODEL, int FLUX>
+__global__ void __launch_bounds__(256) k_faces_ac(DevState S, Consts C)
+{
+    size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    size_t nxf = S.N, nyf = (size_t)C.nyf * C.nx;
+    if (t >= nxf + nyf) return;
+    int dir = t < nxf ? 0 : 1;
+    size_t fidx = dir == 0 ? t : t - nxf;
+    int i = (int)(fidx % C.nx), j = (int)(fidx / C.nx);
+    int oi, oj, ni, nj;
+    double xf, yf;
+    face_geom(C, dir, i, j, oi, oj, ni, nj, xf, yf);
+    double hb[Q9], fb[Q9];
+#pragma unroll
+    for (int q = 0; q < Q9; ++q) {
+        hb[q] = face_value<FLUX>(S, C, S.hbp, S.ghbp, S.hbpx, S.hbpy, dir, q, oi, oj, ni, nj, xf, yf);
+        fb[q] = face_value<FLUX>(S, C, S.fbp, S.gfbp, S.fbpx, S.fbpy, dir, q, oi, oj, ni, nj, xf, yf);
+    }
+    FaceMacro<double> fm;
+    fm.px = 0.5 * (cellm(S, C, MPD_M_PHIX, oi, oj) + cellm(S, C, MPD_M_PHIX, ni, nj));
+    fm.py = 0.5 * (cellm(S, C, MPD_M_PHIY, oi, oj) + cellm(S, C, MPD_M_PHIY, ni, nj));
+    fm.rx = 0.5 * (cellm(S, C, MPD_M_RHOX, oi, oj) + cellm(S, C, MPD_M_RHOX, ni, nj));
+    fm.ry = 0.5 * (cellm(S, C, MPD_M_RHOY, oi, oj) + cellm(S, C, MPD_M_RHOY, ni, nj));
+    fm.Fx = 0.5 * (cellm(S, C, MPD_M_FX, oi, oj) + cellm(S, C, MPD_M_FX, ni, nj));
+    fm.Fy = 0.5 * (cellm(S, C, MPD_M_FY, oi, oj) + cellm(S, C, MPD_M_FY, ni, nj));
+    face_macros_ac<double, MODEL>(C, hb, fb, fm);
+    double ah = 2.0 * fm.tau / (2.0 * fm.tau + C.hdt);
+    double bh = 1.0 - ah;
+    double ch = fm.tau * bh;
+    double hh[Q9], ff[Q9];
+#pragma unroll
+    for (int q = 0; q < Q9; ++q) face_bgk<double, MODEL>(C, fm, q, hb[q], fb[q], ah, bh, ch, hh[q], ff[q]);
+    if (C.walls && dir == 1 && (j == 0 || j == C.ny)) {
+        // Wall_3_BB (Ark/Cpp/Boundary.cpp:118-139)
+        double h0[Q9], f0[Q9];
+#pragma unroll
+        for (int q = 0; q < Q9; ++q) { h0[q] = hh[q]; f0[q] = ff[q]; }
+#pragma unroll
+        for (int q = 0; q < Q9; ++q) {
+            bool in = j == 0 ? (ey9(q) > 0) : (ey9(q) < 0);
+            if (in) { hh[q] = h0[bb9(q)]; ff[q] = f0[bb9(q)]; }
+        }
+    }
+    size_t plane = dir == 0 ? nxf : nyf;
+    double *hfl = S.hfl + (dir ? (size_t)Q9 * nxf : 0), *ffl = S.ffl + (dir ? (size_t)Q9 * nxf : 0);
+#pragma unroll
+    for (int q = 0; q < Q9; ++q) {
+        double xin = C.dx * (dir == 0 ? C.xi[0][q] : C.xi[1][q]);
+        hfl[(size_t)q * plane + fidx] = hh[q] * xin;
+        ffl[(size_t)q * plane + fidx] = ff[q] * xin;
+    }
+}
+
+// Pseudopotential faces, pass 1: face macros and Psi of every face
+// (Update_MacroVar_h, Ark/Lib/D2Q9PP.cpp:110-136; Update_PseudoPsi(face),
+// Ark/Cpp/PseudoP.cpp:23-30).  All Psi are written before any face force reads them.
+template <int FLUX>
+__global__ void __launch_bounds__(256) k_pp_face_psi(DevState S, Consts C)
+{
+    size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= 2 * (size_t)S.N) return;
+    int dir = t < (size_t)S.N ? 0 : 1;
+    size_t fidx = dir == 0 ? t : t - S.N;
+    int i = (int)(fidx % C.nx), j = (int)(fidx / C.nx);
+    int oi, oj, ni, nj;
+    double xf, yf;
+    face_geom(C, dir, i, j, oi, oj, ni, nj, xf, yf);
+    double s = 0, sx = 0, sy = 0, fb[Q9];
+#pragma unroll
+    for (int q = 0; q < Q9; ++q)
+        fb[q] = face_value<FLUX>(S, C, S.fbp, S.gfbp, S.fbpx, S.fbpy, dir, q, oi, oj, ni, nj, xf, yf);
+#pragma unroll
+    for (int q = 0; q < Q9; ++q) s += fb[q];
+#pragma unroll
+    for (int q = 0; q < Q9; ++q) sx += fb[q] * C.xi[0][q];
+#pragma unroll
+    for (int q = 0; q < Q9; ++q) sy += fb[q] * C.xi[1][q];
+    double rho = s;
+    double U = sx / rho, V = sy / rho;
+    U += 0.5 * C.hdt * S.ffx[t] / rho;
+    V += 0.5 * C.hdt * S.ffy[t] / rho;
+    double p = C.pp_k * carnahan_starling(rho, C.t_cs);
+    double scg, psi;
+    calc_psi(p, rho, C.rt, scg, psi);
+    S.fpsi[t] = psi;
+    S.fscr[0 * 2 * (size_t)S.N + t] = rho;
+    S.fscr[1 * 2 * (size_t)S.N + t] = U;
+    S.fscr[2 * 2 * (size_t)S.N + t] = V;
+    S.fscr[3 * 2 * (size_t)S.N + t] = scg;
+}
+
+// Pseudopotential faces, pass 2: face force from the face-lattice gradient of
+// Psi (update_MQ_x/y(Face_2D*), Ark/Lib/GradSchemeBasic.h:181-208;
+// Update_PseudoForce(face), Ark/Cpp/PseudoP.cpp:39-46), then the BGK half step.
+template <int FLUX>
+__global__ void __launch_bounds__(256) k_pp_face_flux(DevState S, Consts C)
+{
+    size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= 2 * (size_t)S.N) return;
+    int dir = t < (size_t)S.N ? 0 : 1;
+    size_t fidx = dir == 0 ? t : t - S.N;
+    int i = (int)(fidx % C.nx), j = (int)(fidx / C.nx);
+    const double *ps = S.fpsi + (size_t)dir * S.N;
+    auto P = [&](int a, int b) { return ps[(size_t)wrapi(b, C.ny) * C.nx + wrapi(a, C.nx)]; };
+    double c[4] = {P(i + 1, j), P(i, j + 1), P(i - 1, j), P(i, j - 1)};
+    double d[4] = {P(i + 1, j + 1), P(i - 1, j + 1), P(i - 1, j - 1), P(i + 1, j - 1)};
+    double gx, gy;
+    grad9(C, c, d, gx, gy);
+    const size_t F2 = 2 * (size_t)S.N;
+    FaceMacro<double> fm;
+    fm.rho = S.fscr[0 * F2 + t];
+    fm.U = S.fscr[1 * F2 + t];
+    fm.V = S.fscr[2 * F2 + t];
+    double scg = S.fscr[3 * F2 + t], psi = S.fpsi[t];
+    gx *= -scg * psi;
+    gy *= -scg * psi;
+    fm.Fx = gx;
+    fm.Fy = gy;
+    S.ffx[t] = gx;
+    S.ffy[t] = gy;
+    int oi, oj, ni, nj;
+    double xf, yf;
+    face_geom(C, dir, i, j, oi, oj, ni, nj, xf, yf);
+    double *ffl = S.ffl + (dir ? (size_t)Q9 * S.N : 0);
+#pragma unroll
+    for (int q = 0; q < Q9; ++q) {
+        double fb = face_value<FLUX>(S, C, S.fbp, S.gfbp, S.fbpx, S.fbpy, dir, q, oi, oj, ni, nj, xf, yf);
+        double hh, ff;
+        face_bgk<double, MPD_MODEL_PP_D2Q9>(C, fm, q, 0.0, fb, C.ah_pp, C.bh_pp, C.ch_pp, hh, ff);
+        double xin = C.dx * (dir == 0 ? C.xi[0][q] : C.xi[1][q]);
+        ffl[(size_t)q * S.N + fidx] = ff * xin;
+    }
+}
+
+// phase 8 + 9: Update_phi_T (Ark/Cpp/DUGKS2DSolver.cpp:748-792, faces E,N,W,S)
+// and Update_MacroVar of the cell.
+template <int MODEL>
+__global__ void __launch_bounds__(256) k_update(DevState S, Consts C)
+{
+    size_t n = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= (size_t)S.N) return;
+    const size_t N = S.N;
+    int i = (int)(n % C.nx), j = (int)(n / C.nx);
+    size_t yplane = (size_t)C.nyf * C.nx;
+    size_t kE = (size_t)j * C.nx + wrapi(i + 1, C.nx), kW = n;
+    int jn = C.walls ? j + 1 : wrapi(j + 1, C.ny);
+    size_t kN = (size_t)jn * C.nx + i, kS = n;
+    const double aTP = 4.0 / 3.0, bTP = 1.0 / 3.0;
+    const double *hx = S.hfl, *hy = S.hfl + (size_t)Q9 * N;
+    const double *fx = S.ffl, *fy = S.ffl + (size_t)Q9 * N;
+    double ht[Q9], ft[Q9];
+#pragma unroll
+    for (int q = 0; q < Q9; ++q) {
+        if (MODEL != MPD_MODEL_PP_D2Q9) {
+            double s = 0.0;
+            s += -1 * hx[q * N + kE];
+            s += -1 * hy[q * yplane + kN];
+            s += 1 * hx[q * N + kW];
+            s += 1 * hy[q * yplane + kS];
+            ht[q] = aTP * S.hbp[q * N + n] - bTP * S.h[q * N + n] + C.dtv * s;
+            S.h[q * N + n] = ht[q];
+        }
+        double s = 0.0;
+        s += -1 * fx[q * N + kE];
+        s += -1 * fy[q * yplane + kN];
+        s += 1 * fx[q * N + kW];
+        s += 1 * fy[q * yplane + kS];
+        ft[q] = aTP * S.fbp[q * N + n] - bTP * S.f[q * N + n] + C.dtv * s;
+        S.f[q * N + n] = ft[q];
+    }
+    double *m = S.m;
+    if (MODEL == MPD_MODEL_PP_D2Q9) {
+        // Update_MacroVar, Ark/Lib/D2Q9PP.cpp:89-109 + Update_PseudoPsi(cell)
+        double s = 0, sx = 0, sy = 0;
+#pragma unroll
+        for (int q = 0; q < Q9; ++q) s += ft[q];
+#pragma unroll
+        for (int q = 0; q < Q9; ++q) sx += ft[q] * C.xi[0][q];
+#pragma unroll
+        for (int q = 0; q < Q9; ++q) sy += ft[q] * C.xi[1][q];
+        double rho = s, U = sx / rho, V = sy / rho;
+        U += C.hdt * m[MPD_M_FX * N + n] / rho;
+        V += C.hdt * m[MPD_M_FY * N + n] / rho;
+        double p = C.pp_k * carnahan_starling(rho, C.t_cs), scg, psi;
+        calc_psi(p, rho, C.rt, scg, psi);
+        m[MPD_M_RHO * N + n] = rho;
+        m[MPD_M_U * N + n] = U;
+        m[MPD_M_V * N + n] = V;
+        m[MPD_M_P * N + n] = p;
+        m[MPD_M_SCG * N + n] = scg;
+        m[MPD_M_PSI * N + n] = psi;
+        return;
+    }
+    double phi, rho, U, V, p, mu, tau;
+    cell_macrovar_ac<double, MODEL>(C, ht, ft, m[MPD_M_FX * N + n], m[MPD_M_FY * N + n],
+                                    m[MPD_M_RHOX * N + n], m[MPD_M_RHOY * N + n], phi, rho, U, V, p, mu, tau);
+    m[MPD_M_PHI * N + n] = phi;
+    m[MPD_M_RHO * N + n] = rho;
+    m[MPD_M_U * N + n] = U;
+    m[MPD_M_V * N + n] = V;
+    m[MPD_M_P * N + n] = p;
+    m[MPD_M_MU * N + n] = mu;
+    m[MPD_M_TAU * N + n] = tau;
+}
+
+// phase 10: MacroSource (Ark/Cpp/SourceM.cpp:35-62)
+__global__ void __launch_bounds__(256) k_source(DevState S, Consts C)
+{
+    size_t n = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= (size_t)S.N) return;
+    const size_t N = S.N;
+    int i = (int)(n % C.nx), j = (int)(n / C.nx);
+    double c[4] = {cellm(S, C, MPD_M_PHI, i + 1, j), cellm(S, C, MPD_M_PHI, i, j + 1),
+                   cellm(S, C, MPD_M_PHI, i - 1, j), cellm(S, C, MPD_M_PHI, i, j - 1)};
+    double d[4] = {cellm(S, C, MPD_M_PHI, i + 1, j + 1), cellm(S, C, MPD_M_PHI, i - 1, j + 1),
+                   cellm(S, C, MPD_M_PHI, i - 1, j - 1), cellm(S, C, MPD_M_PHI, i + 1, j - 1)};
+    double *m = S.m;
+    AcSource<double> s;
+    ac_source<double>(C, m[MPD_M_PHI * N + n], m[MPD_M_RHO * N + n], m[MPD_M_U * N + n], m[MPD_M_V * N + n],
+                      m[MPD_M_PPU * N + n], m[MPD_M_PPV * N + n], c, d, s);
+    m[MPD_M_LAP * N + n] = s.lap;
+    m[MPD_M_RHOX * N + n] = s.rx;
+    m[MPD_M_RHOY * N + n] = s.ry;
+    m[MPD_M_FX * N + n] = s.Fx;
+    m[MPD_M_FY * N + n] = s.Fy;
+    m[MPD_M_PHIX * N + n] = s.sx;
+    m[MPD_M_PHIY * N + n] = s.sy;
+    m[MPD_M_PPU * N + n] = s.ppu;
+    m[MPD_M_PPV * N + n] = s.ppv;
+}
+
+// phase 11: Update_PseudoForce(cell) (Ark/Cpp/PseudoP.cpp:31-38); reads Psi only
+__global__ void __launch_bounds__(256) k_ppforce(DevState S, Consts C)
+{
+    size_t n = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= (size_t)S.N) return;
+    const size_t N = S.N;
+    int i = (int)(n % C.nx), j = (int)(n / C.nx);
+    double c[4] = {cellm(S, C, MPD_M_PSI, i + 1, j), cellm(S, C, MPD_M_PSI, i, j + 1),
+                   cellm(S, C, MPD_M_PSI, i - 1, j), cellm(S, C, MPD_M_PSI, i, j - 1)};
+    double d[4] = {cellm(S, C, MPD_M_PSI, i + 1, j + 1), cellm(S, C, MPD_M_PSI, i - 1, j + 1),
+                   cellm(S, C, MPD_M_PSI, i - 1, j - 1), cellm(S, C, MPD_M_PSI, i + 1, j - 1)};
+    double gx, gy;
+    grad9(C, c, d, gx, gy);
+    double sp = -S.m[MPD_M_SCG * N + n] * S.m[MPD_M_PSI * N + n];
+    gx *= sp;
+    gy *= sp;
+    S.m[MPD_M_FX * N + n] = gx;
+    S.m[MPD_M_FY * N + n] = gy;
+}
+
+static inline unsigned nblk(size_t n) { return (unsigned)((n + 255) / 256); }
+
+template <int MODEL, int FLUX>
+static void phase_step_t(const DevState &S, const Consts &C, hipStream_t st)
+{
+    hipLaunchKernelGGL((k_phase1<MODEL>), dim3(nblk(S.N)), dim3(256), 0, st, S, C);
+    if (C.walls) hipLaunchKernelGGL(k_ghosts, dim3(nblk(2 * C.nx)), dim3(256), 0, st, S, C);
+    if (FLUX == MPD_FLUX_UW) {
+        if (MODEL != MPD_MODEL_PP_D2Q9)
+            hipLaunchKernelGGL(k_grad, dim3(nblk(S.N)), dim3(256), 0, st, S, C, (const double *)S.hbp, S.hbpx, S.hbpy);
+        hipLaunchKernelGGL(k_grad, dim3(nblk(S.N)), dim3(256), 0, st, S, C, (const double *)S.fbp, S.fbpx, S.fbpy);
+    }
+    if (MODEL == MPD_MODEL_PP_D2Q9) {
+        hipLaunchKernelGGL((k_pp_face_psi<FLUX>), dim3(nblk(2 * S.N)), dim3(256), 0, st, S, C);
+        hipLaunchKernelGGL((k_pp_face_flux<FLUX>), dim3(nblk(2 * S.N)), dim3(256), 0, st, S, C);
+    } else {
+        size_t nf = S.N + (size_t)C.nyf * C.nx;
+        hipLaunchKernelGGL((k_faces_ac<MODEL, FLUX>), dim3(nblk(nf)), dim3(256), 0, st, S, C);
+    }
+    hipLaunchKernelGGL((k_update<MODEL>), dim3(nblk(S.N)), dim3(256), 0, st, S, C);
+    if (MODEL == MPD_MODEL_PP_D2Q9) hipLaunchKernelGGL(k_ppforce, dim3(nblk(S.N)), dim3(256), 0, st, S, C);
+    else hipLaunchKernelGGL(k_source, dim3(nblk(S.N)), dim3(256), 0, st, S, C);
+}
+
+void phase_step(const DevState &S, const Consts &C, hipStream_t st)
+{
+    switch (C.model * 2 + C.flux) {
+    case MPD_MODEL_AC_P * 2 + MPD_FLUX_CD: phase_step_t<MPD_MODEL_AC_P, MPD_FLUX_CD>(S, C, st); break;
+    case MPD_MODEL_AC_P * 2 + MPD_FLUX_UW: phase_step_t<MPD_MODEL_AC_P, MPD_FLUX_UW>(S, C, st); break;
+    case MPD_MODEL_AC_HCZ * 2 + MPD_FLUX_CD: phase_step_t<MPD_MODEL_AC_HCZ, MPD_FLUX_CD>(S, C, st); break;
+    case MPD_MODEL_AC_HCZ * 2 + MPD_FLUX_UW: phase_step_t<MPD_MODEL_AC_HCZ, MPD_FLUX_UW>(S, C, st); break;
+    case MPD_MODEL_PP_D2Q9 * 2 + MPD_FLUX_CD: phase_step_t<MPD_MODEL_PP_D2Q9, MPD_FLUX_CD>(S, C, st); break;
+    case MPD_MODEL_PP_D2Q9 * 2 + MPD_FLUX_UW: phase_step_t<MPD_MODEL_PP_D2Q9, MPD_FLUX_UW>(S, C, st); break;
+    default: break;
+    }
+}
+
+}  // namespace mpd

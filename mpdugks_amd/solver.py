@@ -1,0 +1,195 @@
+"""Python host front end of the step library, mirroring the reference's interface.
+
+The reference exposes one entry point for the hot path, ``DUGKS2DSolver()``
+(Ark/Cpp/DUGKS2DSolver.cpp:148), run after an initialiser such as
+``AC_Drop()`` (Ark/Cpp/Preprocess.cpp:705).  ``Solver`` keeps those names
+(``ac_drop``, ``scmp_drop``, ``step``) over the C ABI of include/mpdugks.h.
+Every call goes to libmpdugks_hip.so; there is no CPU fallback.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+from . import abi
+from .abi import MpdConfig, MpdSums, NM, M
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "_lib", "libmpdugks_hip.so")
+_lib = None
+
+_SYMBOLS = {
+    "mpd_abi_version": ([], C.c_int),
+    "mpd_default_config": ([C.c_int32, C.c_void_p], C.c_int),
+    "mpd_create": ([C.c_void_p, C.c_void_p], C.c_int),
+    "mpd_destroy": ([C.c_void_p], C.c_int),
+    "mpd_set_path": ([C.c_void_p, C.c_int32], C.c_int),
+    "mpd_upload_state": ([C.c_void_p] + [C.c_void_p] * 5, C.c_int),
+    "mpd_download_state": ([C.c_void_p] + [C.c_void_p] * 5, C.c_int),
+    "mpd_step": ([C.c_void_p, C.c_int64], C.c_int),
+    "mpd_reductions": ([C.c_void_p, C.c_int, C.c_void_p], C.c_int),
+    "mpd_synchronize": ([C.c_void_p], C.c_int),
+    "mpd_device_bytes": ([C.c_void_p, C.c_void_p], C.c_int),
+    "mpd_kernel_timing": ([C.c_void_p, C.c_void_p, C.c_void_p], C.c_int),
+    "mpd_last_error": ([], C.c_char_p),
+    "mpd_init_ac_drop": ([C.c_void_p, C.c_double, C.c_double, C.c_double, C.c_int32, C.c_double,
+                          C.c_double, C.c_void_p, C.c_void_p, C.c_void_p], C.c_int),
+    "mpd_init_pp_drop": ([C.c_void_p, C.c_double, C.c_double, C.c_double, C.c_double, C.c_double,
+                          C.c_double, C.c_double, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p],
+                         C.c_int),
+    "mpd_init_pp_noise": ([C.c_void_p, C.c_double, C.c_double, C.c_uint64, C.c_double, C.c_void_p,
+                           C.c_void_p, C.c_void_p, C.c_void_p], C.c_int),
+}
+
+
+class MpdError(RuntimeError):
+    pass
+
+
+def lib():
+    """Load libmpdugks_hip.so (built by ``mpdugks_amd.build``); raise if absent."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise MpdError("libmpdugks_hip.so not built: run `python -m mpdugks_amd.build` "
+                           "(or __graft_entry__.build())")
+        L = C.CDLL(LIB_PATH)
+        for name, (args, res) in _SYMBOLS.items():
+            fn = getattr(L, name)
+            fn.argtypes = args
+            fn.restype = res
+        _lib = L
+    return _lib
+
+
+def _check(rc, what):
+    if rc != 0:
+        msg = lib().mpd_last_error().decode(errors="replace")
+        raise MpdError("%s failed (%s): %s" % (what, abi.STATUS.get(rc, rc), msg))
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+def _f64(a):
+    return None if a is None else np.ascontiguousarray(a, dtype=np.float64)
+
+
+class State:
+    """Host copy of the step state in the ABI layout."""
+
+    def __init__(self, cfg, h=None, f=None, m=None, ffx=None, ffy=None):
+        N = cfg.nx * cfg.ny
+        self.cfg = cfg
+        self.h = h if h is not None else (np.zeros((9, N)) if abi.has_h(cfg.model) else None)
+        self.f = f if f is not None else np.zeros((abi.num_q(cfg.model), N))
+        self.m = m if m is not None else np.zeros((NM, N))
+        self.ffx = ffx if ffx is not None else np.zeros(2 * N)
+        self.ffy = ffy if ffy is not None else np.zeros(2 * N)
+
+    def field(self, name):
+        return self.m[M[name]].reshape(self.cfg.ny, self.cfg.nx)
+
+    def copy(self):
+        c = lambda a: None if a is None else a.copy()
+        return State(self.cfg, c(self.h), c(self.f), c(self.m), c(self.ffx), c(self.ffy))
+
+
+def ac_drop(cfg, cx=None, cy=None, radius=None, sign=+1, p0=1.0, mu0=0.1):
+    """Initial state of AC_Drop (Ark/Cpp/Preprocess.cpp:705-760); defaults are the
+    reference's centre (64, 64) and radius 0.36*ChLength scaled to the mesh."""
+    L = cfg.nx * cfg.dx
+    cx = 0.5 * L if cx is None else cx
+    cy = 0.5 * cfg.ny * cfg.dx if cy is None else cy
+    radius = 0.36 * L if radius is None else radius
+    s = State(cfg)
+    _check(lib().mpd_init_ac_drop(C.byref(cfg), cx, cy, radius, int(sign), p0, mu0, _p(s.h), _p(s.f),
+                                  _p(s.m)), "mpd_init_ac_drop")
+    return s
+
+
+def scmp_drop(cfg, cx=None, cy=None, radius=None, rho_l=2.4806E-1, rho_v=4.5435E-2, width=5.0, mu0=0.1):
+    """Initial state of SCMP_Drop (Ark/Cpp/Preprocess.cpp:337-401)."""
+    L = cfg.nx * cfg.dx
+    cx = 0.5 * L if cx is None else cx
+    cy = 0.5 * cfg.ny * cfg.dx if cy is None else cy
+    radius = 0.25 * L if radius is None else radius
+    s = State(cfg)
+    _check(lib().mpd_init_pp_drop(C.byref(cfg), cx, cy, radius, rho_l, rho_v, width, mu0, _p(s.f),
+                                  _p(s.m), _p(s.ffx), _p(s.ffy)), "mpd_init_pp_drop")
+    return s
+
+
+def pp_noise(cfg, rho0, amp=0.01, seed=0, mu0=0.1):
+    s = State(cfg)
+    _check(lib().mpd_init_pp_noise(C.byref(cfg), rho0, amp, int(seed), mu0, _p(s.f), _p(s.m),
+                                   _p(s.ffx), _p(s.ffy)), "mpd_init_pp_noise")
+    return s
+
+
+class Solver:
+    """One case resident on the current GPU: the drop-in for DUGKS2DSolver()'s loop."""
+
+    def __init__(self, cfg, path=None):
+        self.cfg = cfg
+        self._h = C.c_void_p()
+        _check(lib().mpd_create(C.byref(cfg), C.byref(self._h)), "mpd_create")
+        if path is not None:
+            self.set_path(path)
+
+    def close(self):
+        if getattr(self, "_h", None) and self._h.value:
+            lib().mpd_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_path(self, path):
+        _check(lib().mpd_set_path(self._h, int(path)), "mpd_set_path")
+
+    def upload(self, s):
+        arrs = [_f64(a) for a in (s.h, s.f, s.m, s.ffx, s.ffy)]
+        _check(lib().mpd_upload_state(self._h, *[_p(a) for a in arrs]), "mpd_upload_state")
+
+    def download(self):
+        s = State(self.cfg)
+        _check(lib().mpd_download_state(self._h, _p(s.h), _p(s.f), _p(s.m), _p(s.ffx), _p(s.ffy)),
+               "mpd_download_state")
+        return s
+
+    def step(self, n=1):
+        _check(lib().mpd_step(self._h, int(n)), "mpd_step")
+
+    def synchronize(self):
+        _check(lib().mpd_synchronize(self._h), "mpd_synchronize")
+
+    def reductions(self, update_residual=False):
+        out = MpdSums()
+        rc = lib().mpd_reductions(self._h, int(bool(update_residual)), C.byref(out))
+        if rc not in (0, -3):
+            _check(rc, "mpd_reductions")
+        d = {k: getattr(out, k) for k, _ in MpdSums._fields_}
+        d["status"] = rc
+        return d
+
+    def device_bytes(self):
+        b = C.c_int64()
+        _check(lib().mpd_device_bytes(self._h, C.byref(b)), "mpd_device_bytes")
+        return b.value
+
+    def kernel_timing(self):
+        ms, n = C.c_double(), C.c_int64()
+        _check(lib().mpd_kernel_timing(self._h, C.byref(ms), C.byref(n)), "mpd_kernel_timing")
+        return ms.value, n.value
+
+
+def default_config(model=abi.MPD_MODEL_AC_HCZ):
+    """The library's own defaults (mpd_default_config)."""
+    cfg = MpdConfig()
+    _check(lib().mpd_default_config(int(model), C.byref(cfg)), "mpd_default_config")
+    return cfg

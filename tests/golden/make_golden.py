@@ -1,0 +1,68 @@
+"""Write the golden fixtures of the reference solver (run in the container that
+has /root/reference; the fixtures then travel with the repository).
+
+For each configuration the compiled reference (oracle/Makefile.ref, unmodified
+sources + state-dump hook) is run from its own initialiser and the full cell
+state after steps 0, 1, 2, 10 and 100 is recorded as SHA-256 digests of the
+float64 planes in the library's upload layout (tilde_h, tilde_f, each macro
+field), plus a few raw values for diagnostics.  A bit-exact restatement must
+reproduce every digest.
+
+    python tests/golden/make_golden.py
+"""
+import hashlib
+import json
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, ROOT)
+sys.path.insert(0, os.path.join(ROOT, "oracle"))
+
+from mpdugks_amd.abi import MACRO_FIELDS  # noqa: E402
+
+REF_KEYS = {"phi": "Phi", "rho": "Rho", "u": "U", "v": "V", "p": "p", "phix": "Phi_x",
+            "phiy": "Phi_y", "rhox": "Rho_x", "rhoy": "Rho_y", "fx": "Fx", "fy": "Fy",
+            "lap": "laplacianPhi", "ppu": "prevPhiU", "ppv": "prevPhiV", "psi": "Psi",
+            "scg": "scG", "mu": "Mu", "tau": "ftau"}
+STEPS = [0, 1, 2, 10, 100]
+CONFIGS = {"hcz": "ac_hcz", "acp": "ac_p", "hcz_wall": "ac_hcz"}
+
+
+def digest(a):
+    return hashlib.sha256(np.ascontiguousarray(a, dtype="<f8").tobytes()).hexdigest()
+
+
+def state_digests(d):
+    n = d["f"].shape[1] * d["f"].shape[2]
+    out = {"f": digest(d["f"].reshape(9, n))}
+    if "h" in d:
+        out["h"] = digest(d["h"].reshape(9, n))
+    out["m"] = {k: digest(d[REF_KEYS[k]].reshape(n)) for k in MACRO_FIELDS}
+    return out
+
+
+def main():
+    from ref_run import load_cells, run
+    for cfg, model in CONFIGS.items():
+        out = run(cfg, STEPS, max(STEPS))
+        rec = {"model": model, "walls_y": cfg.endswith("_wall"), "nx": 128, "ny": 128,
+               "source": "reference Ark, unmodified sources, OMP_NUM_THREADS=1, AC_Drop",
+               "steps": {}}
+        for k in STEPS:
+            d = load_cells(os.path.join(out, "cells_step%d.bin" % k))
+            r = state_digests(d)
+            r["sample"] = {name: [float(x) for x in d[REF_KEYS[name]][64, 10:30]]
+                           for name in ("phi", "p", "u", "v")}
+            rec["steps"][str(k)] = r
+        path = os.path.join(HERE, "ref_%s.json" % cfg)
+        with open(path, "w") as fh:
+            json.dump(rec, fh, indent=1)
+        print("wrote", path)
+
+
+if __name__ == "__main__":
+    main()
