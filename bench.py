@@ -1,0 +1,159 @@
+"""Benchmark of the DUGKS step on one MI355X.
+
+Workload: the Rayleigh-Taylor case, 256 x 1024 cells, periodic in x and
+bounce-back walls in y, Allen-Cahn + D2Q9 (HCZ momentum closure), CD faces,
+gravity, Atwood 0.5, Re 3000, seeded interface perturbation (synthetic input,
+built on the host and uploaded once; the timed region starts with the state
+resident in HBM).
+
+A step is one full DUGKS time step of the case (phases 1-12 of the reference
+loop).  The headline value is the FP32-bulk fused kernel's MLUPS; the plain
+FP64 path (one kernel per phase) and the FP64 fused kernel are timed on the
+same workload and reported beside it.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "MLUPS (FP64 plain vs FP32 fused) + %HBM peak; ensemble solves/s, p99 latency"
+HBM_PEAK_GBS = 8000.0
+# Algorithmic bytes per lattice update: the state a cell must carry between
+# steps, read once and written once.  AC+HCZ: 9 h~ + 9 f~ + prevPhiU/V + Phi of
+# the previous step = 21 scalars.
+S_ALG_HCZ = 21
+
+
+def run_gpu(cfg, st, steps, warmup, path):
+    from mpdugks_amd import solver
+    s = solver.Solver(cfg, path=path)
+    s.upload(st)
+    s.step(warmup)
+    s.synchronize()
+    # per-launch HIP-event timing of the step kernel on the solver's stream
+    s.set_kernel_timing(True)
+    s.kernel_timing()
+    s.synchronize()
+    t0 = time.perf_counter()
+    s.step(steps)
+    s.synchronize()
+    wall = time.perf_counter() - t0
+    avg_ms, nlaunch = s.kernel_timing()
+    s.set_kernel_timing(False)
+    # untimed wall-clock rate without per-launch events
+    s.synchronize()
+    t0 = time.perf_counter()
+    s.step(steps)
+    s.synchronize()
+    wall_free = time.perf_counter() - t0
+    red = s.reductions()
+    out = s.download()
+    s.close()
+    return {"wall_s": wall, "wall_s_untimed": wall_free, "kernel_ms": avg_ms, "launches": nlaunch,
+            "state": out, "sums": red}
+
+
+def cpu_baseline(cfg, st, budget_s):
+    """The CPU oracle (single thread, FP64) on the same case for ~budget_s."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle
+    o = oracle.Oracle(cfg)
+    o.set_state(st.h, st.f, st.m)
+    n = 0
+    t0 = time.perf_counter()
+    while time.perf_counter() - t0 < budget_s:
+        o.step(1)
+        n += 1
+    dt = time.perf_counter() - t0
+    return cfg.nx * cfg.ny * n / dt / 1e6, n, dt
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2000)
+    ap.add_argument("--warmup", type=int, default=100)
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--nx", type=int, default=256)
+    ap.add_argument("--ny", type=int, default=1024)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py: this build measures one GPU only (the DUGKS case runs on a single "
+              "MI355X); run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    from mpdugks_amd import abi, cases
+    nx, ny = args.nx, args.ny
+    N = nx * ny
+    results = {}
+    for tag, prec, path in (("fp32_fused", abi.MPD_FP32_BULK, abi.MPD_PATH_FUSED),
+                            ("fp64_fused", abi.MPD_FP64, abi.MPD_PATH_FUSED),
+                            ("fp64_plain", abi.MPD_FP64, abi.MPD_PATH_PHASES)):
+        cfg, st = cases.rayleigh_taylor(nx=nx, ny=ny, precision=prec)
+        steps = args.steps if tag != "fp64_plain" else max(1, args.steps // 4)
+        r = run_gpu(cfg, st, steps, args.warmup, path)
+        r["steps"] = steps
+        r["mlups"] = N * steps / r["wall_s_untimed"] / 1e6
+        r["kernel_mlups"] = N / (r["kernel_ms"] * 1e-3) / 1e6 if r["kernel_ms"] > 0 else 0.0
+        results[tag] = r
+        print("[bench] %s: %d steps, %.4f ms/step wall, kernel %.4f ms, %.1f MLUPS"
+              % (tag, steps, r["wall_s_untimed"] / steps * 1e3, r["kernel_ms"], r["mlups"]), file=sys.stderr,
+              flush=True)
+
+    head = results["fp32_fused"]
+    ref64 = results["fp64_fused"]["state"]
+    g32 = head["state"]
+    # FP32-bulk error against the FP64 fused run after the same steps (both
+    # ran warmup + 2*steps)
+    def rel(a, b):
+        return float(np.sqrt(((a - b) ** 2).sum()) / max(np.sqrt((b ** 2).sum()), 1e-300))
+    err = {k: rel(g32.m[abi.M[k]], ref64.m[abi.M[k]]) for k in ("rho", "phi", "p")}
+
+    bytes_alg = 2 * S_ALG_HCZ * 4  # FP32-bulk
+    achieved = bytes_alg * N / (head["kernel_ms"] * 1e-3) / 1e9
+    line = {
+        "metric": METRIC,
+        "value": round(head["mlups"], 2),
+        "unit": "MLUPS",
+        "n_gpus": 1,
+        "steps": head["steps"],
+        "warmup": args.warmup,
+        "ms_per_step": head["wall_s_untimed"] / head["steps"] * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32",
+        "data": "synthetic",
+        "config": {"workload": "rayleigh_taylor_%dx%d" % (nx, ny), "cells": N,
+                   "physics": "Allen-Cahn + D2Q9 HCZ, CD faces, periodic x, bounce-back walls y, gravity",
+                   "atwood": 0.5, "reynolds": 3000, "precision": "fp32-bulk (fused kernel)",
+                   "parallelism": "single GPU"},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "kernel": "k_fused_step<float,AC_HCZ,walls>",
+                     "alg_bytes_per_cell": bytes_alg, "kernel_ms": head["kernel_ms"]},
+        "fp64_plain_mlups": round(results["fp64_plain"]["mlups"], 2),
+        "fp64_fused_mlups": round(results["fp64_fused"]["mlups"], 2),
+        "fp32_fused_kernel_mlups": round(head["kernel_mlups"], 2),
+        "fp32_vs_fp64_rel_l2": err,
+    }
+    if not args.no_cpu:
+        cfg, st = cases.rayleigh_taylor(nx=nx, ny=ny, precision=abi.MPD_FP64)
+        mlups, nsteps, dt = cpu_baseline(cfg, st, args.cpu_seconds)
+        line["cpu_baseline"] = {"value": round(mlups, 3), "unit": "MLUPS", "cores": 1, "kind": "port",
+                                "sample": "CPU FP64 oracle, same RT %dx%d case, first %d steps (%.1f s)"
+                                % (nx, ny, nsteps, dt)}
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()

@@ -169,8 +169,14 @@ int mpd_synchronize(mpd_handle *h);
 /* Bytes of device memory the handle owns. */
 int mpd_device_bytes(const mpd_handle *h, int64_t *bytes);
 
-/* Average duration (ms) of the step kernel launches since the last call,
- * measured with HIP events on the handle's stream; resets the counters. */
+/* Enable (1) or disable (0) per-launch timing: each step's kernels are then
+ * bracketed by HIP events on the handle's stream (fused path: exactly the one
+ * step kernel).  Off by default. */
+int mpd_set_kernel_timing(mpd_handle *h, int enable);
+
+/* Average duration (ms) per step of the step kernels since the last call:
+ * with per-launch timing on, the mean of the bracketed launches; otherwise the
+ * batch wall time on the stream divided by the steps.  Resets the counters. */
 int mpd_kernel_timing(mpd_handle *h, double *avg_ms, int64_t *launches);
 
 /* ---- host-side initial states (upload layout; no device work) ---- */
@@ -181,6 +187,12 @@ int mpd_kernel_timing(mpd_handle *h, double *avg_ms, int64_t *launches);
  * Macros start from p = p0, U = V = 0, Mu = mu0, f.tau = mu0/(rho RT). */
 int mpd_init_ac_drop(const mpd_config *cfg, double cx, double cy, double radius, int32_t sign,
                      double p0, double mu0, double *tilde_h, double *tilde_f, double *macro);
+
+/* Interface along x at per-column heights y0[nx] (no counterpart in the
+ * reference; used for the Rayleigh-Taylor case): phi = tanh profile of width W,
+ * sign = +1 puts phase L above; Mu from the HCZ viscosity law. */
+int mpd_init_ac_interface(const mpd_config *cfg, const double *y0, int32_t sign, double p0,
+                          double *tilde_h, double *tilde_f, double *macro);
 
 /* SCMP_Drop (Ark/Cpp/Preprocess.cpp:337-401) for the pseudopotential model. */
 int mpd_init_pp_drop(const mpd_config *cfg, double cx, double cy, double radius, double rho_l,

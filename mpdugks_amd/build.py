@@ -20,6 +20,7 @@ SOURCES = {
     "capi.hip": ["-ffp-contract=off"],
     "phase_kernels.hip": ["-ffp-contract=off"],
     "fused_kernels.hip": ["-ffp-contract=off"],
+    "fused_fp32.hip": ["-ffp-contract=fast"],
     "hostinit.cpp": ["-ffp-contract=off"],
 }
 

@@ -50,11 +50,14 @@ struct mpd_handle {
     int effective_path = MPD_PATH_PHASES;
     bool uploaded = false;
     bool fused_primed = false;   // fused state derived from the explicit macros
+    int prime_stage = 0;         // explicit steps taken towards priming
     std::vector<void *> allocs;
     int64_t bytes = 0;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     double acc_ms = 0.0;
     int64_t launches = 0;
+    bool per_launch = false;
+    std::vector<hipEvent_t> evs;  // per-launch timing pairs
     double *d_part = nullptr;
     int red_blocks = 0;
 };
@@ -291,6 +294,7 @@ int mpd_destroy(mpd_handle *h)
     if (!h) return 0;
     if (h->stream) (void)hipStreamSynchronize(h->stream);
     for (void *p : h->allocs) (void)hipFree(p);
+    for (auto e : h->evs) (void)hipEventDestroy(e);
     fused_free(h->F);
     if (h->ev0) (void)hipEventDestroy(h->ev0);
     if (h->ev1) (void)hipEventDestroy(h->ev1);
@@ -354,6 +358,7 @@ int mpd_upload_state(mpd_handle *h, const double *tilde_h, const double *tilde_f
     HIPCHK(hipStreamSynchronize(h->stream));
     h->uploaded = true;
     h->fused_primed = false;
+    h->prime_stage = 0;
     return 0;
 }
 
@@ -396,10 +401,12 @@ int mpd_step(mpd_handle *h, int64_t nsteps)
     if (path == MPD_PATH_AUTO) path = fused_supported(h->cfg) ? MPD_PATH_FUSED : MPD_PATH_PHASES;
     if (h->cfg.precision == MPD_FP32_BULK && path != MPD_PATH_FUSED)
         return fail(MPD_ERR_INVALID, "FP32-bulk needs the fused path");
-    if (path != h->effective_path && h->fused_primed) {
+    if (path != h->effective_path) {
         // leaving the fused path: write its derived state back as explicit macros
-        if (int r = sync_macros(h)) return r;
+        if (h->fused_primed)
+            if (int r = sync_macros(h)) return r;
         h->fused_primed = false;
+        h->prime_stage = 0;
     }
     h->effective_path = path;
     HIPCHK(hipEventRecord(h->ev0, h->stream));
@@ -411,10 +418,50 @@ int mpd_step(mpd_handle *h, int64_t nsteps)
             int r = fused_alloc(h->F, h->S, h->C, h->cfg);
             if (r) return fail(MPD_ERR_NOMEM, "fused path allocation failed");
         }
-        for (int64_t s = 0; s < nsteps; ++s) {
-            fused_step(h->F, h->S, h->C, h->cfg, !h->fused_primed, h->stream);
-            h->fused_primed = true;
+        int64_t s = 0;
+        if (!h->fused_primed) {
+            // The derived state needs two explicit steps: the first step after an
+            // upload uses the uploaded sources (not derivable from phi), and the
+            // momentum update of a step uses the sources of the step before.
+            if (int r = alloc_phase_scratch(h)) return r;
+            for (; s < nsteps && h->prime_stage < 2; ++s) {
+                if (h->prime_stage == 0) fused_convert(h->F, h->S, h->C, 0, h->stream);
+                phase_step(h->S, h->C, h->stream);
+                fused_convert(h->F, h->S, h->C, h->prime_stage + 1, h->stream);
+                ++h->prime_stage;
+            }
+            if (h->prime_stage == 2) h->fused_primed = true;
         }
+        if (h->per_launch) {
+            // bracket each step kernel with events, in chunks of the pool size
+            const int64_t pool = 512;
+            if (h->evs.empty()) {
+                h->evs.resize(2 * pool);
+                for (auto &e : h->evs) HIPCHK(hipEventCreate(&e));
+            }
+            int64_t timed = 0;
+            double ms_sum = 0.0;
+            while (s < nsteps) {
+                int64_t k = 0;
+                for (; k < pool && s < nsteps; ++k, ++s) {
+                    HIPCHK(hipEventRecord(h->evs[2 * k], h->stream));
+                    fused_step(h->F, h->S, h->C, h->cfg, 0, h->stream);
+                    HIPCHK(hipEventRecord(h->evs[2 * k + 1], h->stream));
+                }
+                HIPCHK(hipEventSynchronize(h->evs[2 * k - 1]));
+                for (int64_t a = 0; a < k; ++a) {
+                    float ms = 0.f;
+                    HIPCHK(hipEventElapsedTime(&ms, h->evs[2 * a], h->evs[2 * a + 1]));
+                    ms_sum += ms;
+                }
+                timed += k;
+            }
+            h->acc_ms += ms_sum;
+            h->launches += timed;
+            HIPCHK(hipGetLastError());
+            return 0;
+        }
+        for (; s < nsteps; ++s) fused_step(h->F, h->S, h->C, h->cfg, 0, h->stream);
     }
     HIPCHK(hipGetLastError());
     HIPCHK(hipEventRecord(h->ev1, h->stream));
@@ -431,6 +478,15 @@ int mpd_kernel_timing(mpd_handle *h, double *avg_ms, int64_t *launches)
     if (!h) return fail(MPD_ERR_INVALID, "null handle");
     if (avg_ms) *avg_ms = h->launches ? h->acc_ms / (double)h->launches : 0.0;
     if (launches) *launches = h->launches;
+    h->acc_ms = 0.0;
+    h->launches = 0;
+    return 0;
+}
+
+int mpd_set_kernel_timing(mpd_handle *h, int enable)
+{
+    if (!h) return fail(MPD_ERR_INVALID, "null handle");
+    h->per_launch = enable != 0;
     h->acc_ms = 0.0;
     h->launches = 0;
     return 0;

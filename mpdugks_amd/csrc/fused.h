@@ -17,9 +17,20 @@ struct FusedState {
     int nmem = 0;
     // ping-pong state planes (element type depends on precision)
     void *st[2] = {nullptr, nullptr};
+    void *gl[2] = {nullptr, nullptr};  // walls: lagged ghost phi rows [2][nx]
     int cur = 0;
     int nplanes = 0;
+    int64_t ncell = 0;
+    bool fp32 = false;
+    double p_ref = 0.0;  // FP32: f~ stored minus the rest equilibrium at p_ref
 };
+
+// FP32 planes of the derived state (see fused_kernels.hip)
+struct FusedPlanes32 {
+    float *h, *f, *phi, *phl, *ppu, *ppv, *gl;
+};
+// one FP32-bulk step (fused_fp32.hip)
+void fused32_step(const FusedPlanes32 &in, const FusedPlanes32 &out, const Consts &C, int64_t N, hipStream_t st);
 
 bool fused_supported(const mpd_config &cfg);
 int fused_alloc(FusedState &F, const DevState &S, const Consts &C, const mpd_config &cfg);
@@ -28,6 +39,10 @@ int64_t fused_bytes(const FusedState &F);
 // first != 0: take the explicit macros of S as the state of this step
 void fused_step(FusedState &F, const DevState &S, const Consts &C, const mpd_config &cfg, int first,
                 hipStream_t st);
+// priming from the explicit state S: which = 0 before the first phase step
+// (walls: lagged ghost phi), 1 after it (phi_lag, prevPhiU/V), 2 after the
+// second (populations, phi)
+void fused_convert(FusedState &F, const DevState &S, const Consts &C, int which, hipStream_t st);
 // write the full macro set (and tilde planes) of the current fused state into S
 int fused_materialize(FusedState &F, const DevState &S, const Consts &C, const mpd_config &cfg,
                       hipStream_t st);

@@ -82,6 +82,44 @@ int mpd_init_ac_drop(const mpd_config *cfg, double cx, double cy, double radius,
     return 0;
 }
 
+// Flat interface along x with per-column height y0[i] (Rayleigh-Taylor start;
+// the reference has no such initialiser): phi = (PhiL+PhiV)/2 +
+// sign*(PhiL-PhiV)/2*tanh(2(y-y0)/W) (sign = +1: phase L above), u = 0, p = p0,
+// Mu = rho*(NuV + (phi-PhiV)(NuL-NuV)) (the HCZ viscosity, D2Q9HCZ.cpp:24-40),
+// f.tau = Mu/(rho RT), tilde = equilibrium.
+int mpd_init_ac_interface(const mpd_config *cfg, const double *y0, int32_t sign, double p0, double *tilde_h,
+                          double *tilde_f, double *macro)
+{
+    if (!valid(cfg) || cfg->model == MPD_MODEL_PP_D2Q9 || !y0 || !tilde_h || !tilde_f || !macro)
+        return MPD_ERR_INVALID;
+    const mpd_config &c = *cfg;
+    const size_t N = (size_t)c.nx * c.ny;
+    const double aphi = (c.rho_l - c.rho_v) / (c.phi_l - c.phi_v), bphi = c.rho_v - aphi * c.phi_v;
+    std::memset(macro, 0, sizeof(double) * MPD_M_COUNT * N);
+    double heq[9], feq[9];
+    for (int j = 0; j < c.ny; ++j)
+        for (int i = 0; i < c.nx; ++i) {
+            size_t n = (size_t)j * c.nx + i;
+            double y = (j + 0.5) * c.dx;
+            double t = std::tanh(2.0 * (y - y0[i]) / c.width);
+            if (sign < 0) t = -t;
+            double phi = 0.5 * (c.phi_l + c.phi_v) + 0.5 * (c.phi_l - c.phi_v) * t;
+            double rho = aphi * phi + bphi;
+            double mu = (c.nu_v + (phi - c.phi_v) * (c.nu_l - c.nu_v)) * rho;
+            macro[MPD_M_PHI * N + n] = phi;
+            macro[MPD_M_RHO * N + n] = rho;
+            macro[MPD_M_P * N + n] = p0;
+            macro[MPD_M_MU * N + n] = mu;
+            macro[MPD_M_TAU * N + n] = mu / (rho * c.rt);
+            cell_eq(c, phi, rho, 0.0, 0.0, p0, heq, feq);
+            for (int q = 0; q < 9; ++q) {
+                tilde_h[q * N + n] = heq[q];
+                tilde_f[q * N + n] = feq[q];
+            }
+        }
+    return 0;
+}
+
 // SCMP_Drop (Ark/Cpp/Preprocess.cpp:337-401): rho = (RhoL+RhoV)/2 - (RhoV-RhoL)/2 *
 // tanh(2(r-R)/width), p = rho RT, F = 0, tau = pp_tau, tilde = equilibrium; face
 // forces are the owner/neighbour averages of the (zero) cell forces.

@@ -32,8 +32,11 @@ _SYMBOLS = {
     "mpd_device_bytes": ([C.c_void_p, C.c_void_p], C.c_int),
     "mpd_kernel_timing": ([C.c_void_p, C.c_void_p, C.c_void_p], C.c_int),
     "mpd_last_error": ([], C.c_char_p),
+    "mpd_set_kernel_timing": ([C.c_void_p, C.c_int], C.c_int),
     "mpd_init_ac_drop": ([C.c_void_p, C.c_double, C.c_double, C.c_double, C.c_int32, C.c_double,
                           C.c_double, C.c_void_p, C.c_void_p, C.c_void_p], C.c_int),
+    "mpd_init_ac_interface": ([C.c_void_p, C.c_void_p, C.c_int32, C.c_double, C.c_void_p, C.c_void_p,
+                               C.c_void_p], C.c_int),
     "mpd_init_pp_drop": ([C.c_void_p, C.c_double, C.c_double, C.c_double, C.c_double, C.c_double,
                           C.c_double, C.c_double, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p],
                          C.c_int),
@@ -109,6 +112,16 @@ def ac_drop(cfg, cx=None, cy=None, radius=None, sign=+1, p0=1.0, mu0=0.1):
     return s
 
 
+def ac_interface(cfg, y0, sign=+1, p0=1.0):
+    """Interface along x at per-column heights y0 (nx values)."""
+    y0 = np.ascontiguousarray(y0, dtype=np.float64)
+    assert y0.shape == (cfg.nx,)
+    s = State(cfg)
+    _check(lib().mpd_init_ac_interface(C.byref(cfg), _p(y0), int(sign), p0, _p(s.h), _p(s.f), _p(s.m)),
+           "mpd_init_ac_interface")
+    return s
+
+
 def scmp_drop(cfg, cx=None, cy=None, radius=None, rho_l=2.4806E-1, rho_v=4.5435E-2, width=5.0, mu0=0.1):
     """Initial state of SCMP_Drop (Ark/Cpp/Preprocess.cpp:337-401)."""
     L = cfg.nx * cfg.dx
@@ -181,6 +194,9 @@ class Solver:
         b = C.c_int64()
         _check(lib().mpd_device_bytes(self._h, C.byref(b)), "mpd_device_bytes")
         return b.value
+
+    def set_kernel_timing(self, enable):
+        _check(lib().mpd_set_kernel_timing(self._h, int(bool(enable))), "mpd_set_kernel_timing")
 
     def kernel_timing(self):
         ms, n = C.c_double(), C.c_int64()
