@@ -61,6 +61,20 @@ def run_gpu(cfg, st, steps, warmup, path):
             "state": out, "sums": red}
 
 
+def copy_calibration(mib):
+    """A plain device-to-device copy of `mib` MiB (reads and writes exactly that
+    many bytes), run 5 times; its FETCH_SIZE / WRITE_SIZE per dispatch give the
+    scale of the counters for the traffic figure."""
+    import torch
+    n = int(mib * (1 << 20)) // 4
+    a = torch.ones(n, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(5):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print("[bench] copy calibration: %d bytes per copy" % (4 * n), file=sys.stderr, flush=True)
+
+
 def cpu_baseline(cfg, st, budget_s):
     """The CPU oracle (single thread, FP64) on the same case for ~budget_s."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
@@ -85,6 +99,10 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--nx", type=int, default=256)
     ap.add_argument("--ny", type=int, default=1024)
+    ap.add_argument("--only", choices=["fp32_fused", "fp64_fused", "fp64_plain"], default=None,
+                    help="run one variant only (for profiling); prints its numbers, not the headline line")
+    ap.add_argument("--copy-mb", type=float, default=0.0,
+                    help="also run a plain device-to-device copy of this many MiB (counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build measures one GPU only (the DUGKS case runs on a single "
@@ -94,10 +112,15 @@ def main():
     from mpdugks_amd import abi, cases
     nx, ny = args.nx, args.ny
     N = nx * ny
+    if args.copy_mb > 0:
+        copy_calibration(args.copy_mb)
     results = {}
-    for tag, prec, path in (("fp32_fused", abi.MPD_FP32_BULK, abi.MPD_PATH_FUSED),
-                            ("fp64_fused", abi.MPD_FP64, abi.MPD_PATH_FUSED),
-                            ("fp64_plain", abi.MPD_FP64, abi.MPD_PATH_PHASES)):
+    variants = (("fp32_fused", abi.MPD_FP32_BULK, abi.MPD_PATH_FUSED),
+                ("fp64_fused", abi.MPD_FP64, abi.MPD_PATH_FUSED),
+                ("fp64_plain", abi.MPD_FP64, abi.MPD_PATH_PHASES))
+    for tag, prec, path in variants:
+        if args.only and tag != args.only:
+            continue
         cfg, st = cases.rayleigh_taylor(nx=nx, ny=ny, precision=prec)
         steps = args.steps if tag != "fp64_plain" else max(1, args.steps // 4)
         r = run_gpu(cfg, st, steps, args.warmup, path)
@@ -109,6 +132,11 @@ def main():
               % (tag, steps, r["wall_s_untimed"] / steps * 1e3, r["kernel_ms"], r["mlups"]), file=sys.stderr,
               flush=True)
 
+    if args.only:
+        r = results[args.only]
+        print(json.dumps({"variant": args.only, "steps": r["steps"], "mlups": r["mlups"],
+                          "kernel_ms": r["kernel_ms"], "ms_per_step": r["wall_s_untimed"] / r["steps"] * 1e3}))
+        return
     head = results["fp32_fused"]
     ref64 = results["fp64_fused"]["state"]
     g32 = head["state"]
@@ -119,6 +147,13 @@ def main():
     err = {k: rel(g32.m[abi.M[k]], ref64.m[abi.M[k]]) for k in ("rho", "phi", "p")}
 
     bytes_alg = 2 * S_ALG_HCZ * 4  # FP32-bulk
+    traffic = None
+    try:  # HBM bytes per launch from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE runs
+        with open(os.path.join(ROOT, "profiles", "traffic.json")) as fh:
+            rec = json.load(fh).get("k_fused32<AC_HCZ,walls> rayleigh_taylor_%dx%d" % (nx, ny))
+        traffic = rec["traffic_bytes"] if rec else None
+    except (OSError, ValueError):
+        traffic = None
     achieved = bytes_alg * N / (head["kernel_ms"] * 1e-3) / 1e9
     line = {
         "metric": METRIC,
@@ -138,7 +173,7 @@ def main():
                    "atwood": 0.5, "reynolds": 3000, "precision": "fp32-bulk (fused kernel)",
                    "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "kernel": "k_fused_step<float,AC_HCZ,walls>",
                      "alg_bytes_per_cell": bytes_alg, "kernel_ms": head["kernel_ms"]},
         "fp64_plain_mlups": round(results["fp64_plain"]["mlups"], 2),
