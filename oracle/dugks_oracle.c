@@ -811,3 +811,14 @@ double ora_wall_flux_residual(const ora *o)
     }
     return worst;
 }
+
+/* Phases 9-11 only (cell macros and sources from the current populations):
+ * lets a test feed the reference's post-update populations and compare the
+ * macros the oracle derives from them with the reference's. */
+int ora_cell_phases(ora *o)
+{
+    phase9_macros(o);
+    if (o->c.model == MPD_MODEL_PP_D2Q9) phase11_ppforce(o);
+    else phase10_source(o);
+    return 0;
+}

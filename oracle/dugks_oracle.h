@@ -15,6 +15,7 @@ int ora_get_state(const ora *o, double *h, double *f, double *m, double *face_fx
                   double *face_fy);
 int ora_step(ora *o, int nsteps);
 double ora_wall_flux_residual(const ora *o);
+int ora_cell_phases(ora *o);
 #ifdef __cplusplus
 }
 #endif

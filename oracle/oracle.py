@@ -34,6 +34,8 @@ def lib():
         _lib.ora_step.restype = C.c_int
         _lib.ora_wall_flux_residual.argtypes = [C.c_void_p]
         _lib.ora_wall_flux_residual.restype = C.c_double
+        _lib.ora_cell_phases.argtypes = [C.c_void_p]
+        _lib.ora_cell_phases.restype = C.c_int
     return _lib
 
 
@@ -72,6 +74,10 @@ class Oracle:
 
     def step(self, n=1):
         lib().ora_step(self._h, int(n))
+
+    def cell_phases(self):
+        """Phases 9-11 (macros and sources) on the current populations."""
+        lib().ora_cell_phases(self._h)
 
     def wall_flux_residual(self):
         return lib().ora_wall_flux_residual(self._h)

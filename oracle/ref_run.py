@@ -20,11 +20,16 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 REF_ROOT = "/root/reference/Ark"
 MESH_NAME = "128_128_Car_Periodic_Square_LBM"  # Ark/Constant/ZeroFlip.h:47-49
 
-# name -> (model translation unit, extra compile flags, walls in y)
+# name -> (model translation unit, extra compile flags, walls in y, extra -I dir)
+CFGDIR = os.path.join(HERE, "refcfg")
 CONFIGS = {
-    "hcz": ("D2Q9HCZ", "", False),
-    "acp": ("D2Q9AC", "", False),
-    "hcz_wall": ("D2Q9HCZ", "-D_Wall_3_BCs_FLIP=3 -D_Wall_3_BCs_BB", True),
+    "hcz": ("D2Q9HCZ", "", False, ""),
+    "acp": ("D2Q9AC", "", False, ""),
+    "hcz_wall": ("D2Q9HCZ", "-D_Wall_3_BCs_FLIP=3 -D_Wall_3_BCs_BB", True, ""),
+    "hcz_uw": ("D2Q9HCZ", "-include " + os.path.join(CFGDIR, "uw", "flags.h"), False, ""),
+    "acp_uw": ("D2Q9AC", "-include " + os.path.join(CFGDIR, "uw", "flags.h"), False, ""),
+    "pp": ("D2Q9PP", "-include " + os.path.join(CFGDIR, "pp", "flags.h"), False,
+           os.path.join(CFGDIR, "pp")),
 }
 
 CELL_SCALARS = ["Phi", "Rho", "U", "V", "p", "Phi_x", "Phi_y", "Rho_x", "Rho_y",
@@ -37,9 +42,9 @@ def available():
 
 
 def build(cfg):
-    model, defs, _ = CONFIGS[cfg]
+    model, defs, _, idir = CONFIGS[cfg]
     cmd = ["make", "-s", "-f", os.path.join(HERE, "Makefile.ref"), "-j8",
-           "CFG=" + cfg, "MODEL=" + model, "DEFS=" + defs]
+           "CFG=" + cfg, "MODEL=" + model, "DEFS=" + defs, "IQUOTE=" + ("-I" + idir if idir else "")]
     subprocess.run(cmd, check=True, cwd=os.path.dirname(HERE))
     return os.path.join(HERE, "_ref", cfg, "ark")
 

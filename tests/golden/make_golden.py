@@ -29,7 +29,13 @@ REF_KEYS = {"phi": "Phi", "rho": "Rho", "u": "U", "v": "V", "p": "p", "phix": "P
             "lap": "laplacianPhi", "ppu": "prevPhiU", "ppv": "prevPhiV", "psi": "Psi",
             "scg": "scG", "mu": "Mu", "tau": "ftau"}
 STEPS = [0, 1, 2, 10, 100]
-CONFIGS = {"hcz": "ac_hcz", "acp": "ac_p", "hcz_wall": "ac_hcz"}
+# name -> (model, steps recorded)
+CONFIGS = {"hcz": ("ac_hcz", STEPS), "acp": ("ac_p", STEPS), "hcz_wall": ("ac_hcz", STEPS),
+           # UW face scheme: bit-exact at step 1; afterwards the reference's two copies
+           # of each periodic face differ in the last bit (see tests/test_oracle_ref_live.py)
+           "hcz_uw": ("ac_hcz", [0, 1]), "acp_uw": ("ac_p", [0, 1]),
+           # pseudopotential: the initial state only (its face force is order dependent)
+           "pp": ("pp_d2q9", [0])}
 
 
 def digest(a):
@@ -47,16 +53,18 @@ def state_digests(d):
 
 def main():
     from ref_run import load_cells, run
-    for cfg, model in CONFIGS.items():
-        out = run(cfg, STEPS, max(STEPS))
-        rec = {"model": model, "walls_y": cfg.endswith("_wall"), "nx": 128, "ny": 128,
-               "source": "reference Ark, unmodified sources, OMP_NUM_THREADS=1, AC_Drop",
+    for cfg, (model, steps) in CONFIGS.items():
+        out = run(cfg, steps, max(steps))
+        rec = {"model": model, "walls_y": cfg.endswith("_wall"), "uw": cfg.endswith("_uw"),
+               "nx": 128, "ny": 128,
+               "source": "reference Ark, unmodified sources, OMP_NUM_THREADS=1, %s"
+                         % ("SCMP_Drop" if cfg == "pp" else "AC_Drop"),
                "steps": {}}
-        for k in STEPS:
+        for k in steps:
             d = load_cells(os.path.join(out, "cells_step%d.bin" % k))
             r = state_digests(d)
             r["sample"] = {name: [float(x) for x in d[REF_KEYS[name]][64, 10:30]]
-                           for name in ("phi", "p", "u", "v")}
+                           for name in ("phi", "rho", "p", "u", "v")}
             rec["steps"][str(k)] = r
         path = os.path.join(HERE, "ref_%s.json" % cfg)
         with open(path, "w") as fh:

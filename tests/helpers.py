@@ -19,9 +19,9 @@ def make_case(model="ac_hcz", nx=128, ny=128, flux=abi.MPD_FLUX_CD, walls=False,
     cfg.bc_y = abi.MPD_BC_WALL_BB if walls else abi.MPD_BC_PERIODIC
     cfg.precision = precision
     if m == abi.MPD_MODEL_PP_D2Q9:
-        st = solver.scmp_drop(cfg, radius=radius if radius is not None else 0.25 * min(nx, ny))
+        st = solver.scmp_drop(cfg, radius=radius if radius is not None else 0.25 * nx)
     else:
-        st = solver.ac_drop(cfg, radius=radius if radius is not None else 0.36 * min(nx, ny), sign=sign)
+        st = solver.ac_drop(cfg, radius=radius if radius is not None else 0.36 * nx, sign=sign)
     return cfg, st
 
 

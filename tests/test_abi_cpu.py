@@ -1,0 +1,60 @@
+"""C ABI checks that need no GPU: the library loads, exports every function
+include/mpdugks.h declares, and its host-only entry points agree with the
+Python mirror (defaults) and with the reference (initial state)."""
+import ctypes as C
+import os
+import re
+
+import numpy as np
+
+from mpdugks_amd import abi, solver
+from mpdugks_amd.config import default_config
+
+HEADER = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "include", "mpdugks.h")
+
+
+def declared_functions():
+    text = open(HEADER).read()
+    return sorted(set(re.findall(r"^\s*(?:int|const char \*|void)\s*\*?\s*(mpd_\w+)\s*\(", text, re.M)))
+
+
+def test_library_exports_every_declared_symbol():
+    L = solver.lib()
+    names = declared_functions()
+    assert len(names) >= 15
+    missing = [n for n in names if not hasattr(L, n)]
+    assert missing == []
+
+
+def test_python_mirror_binds_every_symbol():
+    assert set(declared_functions()) <= set(solver._SYMBOLS)
+
+
+def test_abi_version():
+    assert solver.lib().mpd_abi_version() == 1
+
+
+def test_default_config_matches_python_mirror():
+    for m in (abi.MPD_MODEL_AC_P, abi.MPD_MODEL_AC_HCZ, abi.MPD_MODEL_PP_D2Q9):
+        assert bytes(solver.default_config(m)) == bytes(default_config(m))
+
+
+def test_config_struct_layout():
+    # 6 int32 then 19 doubles, no padding surprises
+    assert C.sizeof(abi.MpdConfig) == 6 * 4 + 19 * 8
+    assert abi.MpdConfig.dx.offset == 24
+
+
+def test_invalid_init_arguments_are_rejected():
+    cfg = default_config(abi.MPD_MODEL_PP_D2Q9)
+    rc = solver.lib().mpd_init_ac_drop(C.byref(cfg), 1.0, 1.0, 1.0, 1, 1.0, 0.1, None, None, None)
+    assert rc == -1
+
+
+def test_interface_initialiser_profile():
+    cfg = default_config(abi.MPD_MODEL_AC_HCZ, nx=8, ny=40)
+    y0 = np.full(8, 20.0)
+    st = solver.ac_interface(cfg, y0, sign=+1)
+    phi = st.field("phi")
+    assert phi[0, 0] < 1e-3 and phi[-1, 0] > 1 - 1e-3           # phase L above
+    assert np.allclose(st.h.sum(axis=0), st.m[abi.M["phi"]], rtol=0, atol=1e-15)

@@ -17,6 +17,7 @@ from helpers import digest, make_case, oracle_run
 
 GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
 CASES = ["hcz", "acp", "hcz_wall"]
+UW_CASES = ["hcz_uw", "acp_uw"]
 
 
 def load(name):
@@ -36,11 +37,19 @@ def check_state(st, rec):
     return bad
 
 
-@pytest.mark.parametrize("name", CASES)
+@pytest.mark.parametrize("name", CASES + UW_CASES + ["pp"])
 def test_initial_state_matches_reference(name):
     g = load(name)
     cfg, st = make_case(g["model"], walls=g["walls_y"])
     assert check_state(st, g["steps"]["0"]) == []
+
+
+@pytest.mark.parametrize("name", UW_CASES)
+def test_oracle_uw_first_step_bit_exact(name, oracle_mod):
+    g = load(name)
+    cfg, st = make_case(g["model"], flux=abi.MPD_FLUX_UW)
+    out = oracle_run(cfg, st, [1])
+    assert check_state(out[1], g["steps"]["1"]) == []
 
 
 @pytest.mark.parametrize("name", CASES)
