@@ -61,6 +61,61 @@ def run_gpu(cfg, st, steps, warmup, path):
             "state": out, "sums": red}
 
 
+def ensemble_bench(ncases, steps, seed=0, solve_steps=20000):
+    """Batched ensemble (one launch per step over all cases, HIP-graph replay)
+    against the plain FP64 path run case by case; solves/s extrapolated to
+    `solve_steps` steps per case from the measured per-step times."""
+    from mpdugks_amd import abi, cases, solver
+    specs = cases.ensemble_specs(ncases, seed=seed)
+    members = []
+    t0 = time.perf_counter()
+    batch = solver.Batch()
+    for sp in specs:
+        cfg, st = cases.ensemble_case(sp)
+        s = solver.Solver(cfg, path=abi.MPD_PATH_FUSED)
+        s.upload(st)
+        s.step(2)                      # two explicit steps prime the derived state
+        batch.add(s)
+        members.append(s)
+    setup = time.perf_counter() - t0
+    ncells = batch.size()[1]
+    batch.step(20)
+    batch.timing()
+    t0 = time.perf_counter()
+    batch.step(steps)
+    wall = time.perf_counter() - t0
+    ms_batch, _ = batch.timing()
+    red = [m.reductions() for m in members[:8]]
+    batch.close()
+    for m in members:
+        m.close()
+    t_solve_batch = ms_batch * 1e-3 * solve_steps
+    # plain FP64 path, one case at a time: per-step time for each mesh side
+    per_side = {}
+    for side in sorted(set(sp["side"] for sp in specs)):
+        sp = dict(next(x for x in specs if x["side"] == side))
+        cfg, st = cases.ensemble_case(sp, precision=abi.MPD_FP64)
+        s = solver.Solver(cfg, path=abi.MPD_PATH_PHASES)
+        s.upload(st)
+        s.step(10)
+        s.synchronize()
+        n = max(20, steps // 10)
+        t1 = time.perf_counter()
+        s.step(n)
+        s.synchronize()
+        per_side[side] = (time.perf_counter() - t1) / n
+        s.close()
+    t_solve_plain = sum(per_side[sp["side"]] for sp in specs) * solve_steps
+    return {"cases": ncases, "cells": ncells, "solve_steps": solve_steps,
+            "batch_ms_per_step": ms_batch, "batch_mlups": ncells / (ms_batch * 1e-3) / 1e6,
+            "solves_per_s_fp32_batched": ncases / t_solve_batch,
+            "solves_per_s_fp64_plain": ncases / t_solve_plain,
+            "speedup_vs_fp64_plain": t_solve_plain / t_solve_batch,
+            "fp64_plain_ms_per_step_by_side": {str(k): v * 1e3 for k, v in per_side.items()},
+            "setup_s": setup, "timed_steps": steps,
+            "nonfinite_in_first_8": int(sum(r["nonfinite"] for r in red))}
+
+
 def copy_calibration(mib):
     """A plain device-to-device copy of `mib` MiB (reads and writes exactly that
     many bytes), run 5 times; its FETCH_SIZE / WRITE_SIZE per dispatch give the
@@ -101,6 +156,9 @@ def main():
     ap.add_argument("--ny", type=int, default=1024)
     ap.add_argument("--only", choices=["fp32_fused", "fp64_fused", "fp64_plain"], default=None,
                     help="run one variant only (for profiling); prints its numbers, not the headline line")
+    ap.add_argument("--ensemble-cases", type=int, default=256)
+    ap.add_argument("--ensemble-steps", type=int, default=400)
+    ap.add_argument("--no-ensemble", action="store_true")
     ap.add_argument("--copy-mb", type=float, default=0.0,
                     help="also run a plain device-to-device copy of this many MiB (counter calibration)")
     args = ap.parse_args()
@@ -181,6 +239,13 @@ def main():
         "fp32_fused_kernel_mlups": round(head["kernel_mlups"], 2),
         "fp32_vs_fp64_rel_l2": err,
     }
+    if not args.no_ensemble:
+        ens = ensemble_bench(args.ensemble_cases, args.ensemble_steps)
+        line["ensemble"] = ens
+        print("[bench] ensemble: %d cases, %d cells, batch %.3f ms/step, %.2f solves/s (FP32 batched) vs "
+              "%.3f (FP64 plain), x%.1f" % (ens["cases"], ens["cells"], ens["batch_ms_per_step"],
+                                             ens["solves_per_s_fp32_batched"], ens["solves_per_s_fp64_plain"],
+                                             ens["speedup_vs_fp64_plain"]), file=sys.stderr, flush=True)
     if not args.no_cpu:
         cfg, st = cases.rayleigh_taylor(nx=nx, ny=ny, precision=abi.MPD_FP64)
         mlups, nsteps, dt = cpu_baseline(cfg, st, args.cpu_seconds)

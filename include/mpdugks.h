@@ -179,6 +179,25 @@ int mpd_set_kernel_timing(mpd_handle *h, int enable);
  * batch wall time on the stream divided by the steps.  Resets the counters. */
 int mpd_kernel_timing(mpd_handle *h, double *avg_ms, int64_t *launches);
 
+/* ---- batched ensembles ----
+ * A batch advances many FP32-bulk cases (Allen-Cahn, CD; same model and y
+ * boundary, any sizes and constants) with one kernel launch per step: every
+ * workgroup takes one 16x15 tile of one case from a work list, and the step
+ * pair is captured once in a HIP graph and replayed.  Members keep their own
+ * state; mpd_download_state / mpd_reductions on a member synchronise the batch.
+ * No counterpart in the reference, which runs one case per process. */
+typedef struct mpd_batch mpd_batch;
+int mpd_batch_create(mpd_batch **out);
+int mpd_batch_destroy(mpd_batch *b);
+/* h must have taken >= 2 steps on the fused path (MPD_ERR_STATE otherwise). */
+int mpd_batch_add(mpd_batch *b, mpd_handle *h);
+int mpd_batch_remove(mpd_batch *b, mpd_handle *h);
+int mpd_batch_size(const mpd_batch *b, int32_t *ncases, int64_t *ncells);
+/* Advance every member by nsteps (synchronises at the end). */
+int mpd_batch_step(mpd_batch *b, int64_t nsteps);
+/* Mean wall time per batch step on the batch stream since the last call. */
+int mpd_batch_timing(mpd_batch *b, double *avg_ms, int64_t *steps);
+
 /* ---- host-side initial states (upload layout; no device work) ---- */
 
 /* AC_Drop (Ark/Cpp/Preprocess.cpp:705-760, AnalyticalPhiAC_Drop :402-419) on the

@@ -39,6 +39,8 @@ int fail(int code, const char *fmt, ...)
 
 }  // namespace
 
+struct mpd_batch;
+
 struct mpd_handle {
     mpd_config cfg;
     Consts C;
@@ -60,6 +62,19 @@ struct mpd_handle {
     std::vector<hipEvent_t> evs;  // per-launch timing pairs
     double *d_part = nullptr;
     int red_blocks = 0;
+    mpd_batch *batch = nullptr;  // set while the handle is a member of a batch
+};
+
+struct mpd_batch {
+    hipStream_t stream = nullptr;
+    std::vector<mpd_handle *> members;
+    Batch32Plan *plan = nullptr;
+    bool dirty = true;            // tables / graph need rebuilding
+    hipGraphExec_t exec = nullptr;
+    hipGraph_t graph = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    double acc_ms = 0.0;
+    int64_t steps = 0;
 };
 
 namespace {
@@ -289,9 +304,12 @@ int mpd_create(const mpd_config *cfg, mpd_handle **out)
     return 0;
 }
 
+int mpd_batch_remove(mpd_batch *b, mpd_handle *h);
+
 int mpd_destroy(mpd_handle *h)
 {
     if (!h) return 0;
+    if (h->batch) mpd_batch_remove(h->batch, h);
     if (h->stream) (void)hipStreamSynchronize(h->stream);
     for (void *p : h->allocs) (void)hipFree(p);
     for (auto e : h->evs) (void)hipEventDestroy(e);
@@ -376,6 +394,7 @@ int mpd_download_state(mpd_handle *h, double *tilde_h, double *tilde_f, double *
                        double *face_fy)
 {
     if (!h) return fail(MPD_ERR_INVALID, "null handle");
+    if (h->batch) HIPCHK(hipStreamSynchronize(h->batch->stream));
     if (!h->uploaded) return fail(MPD_ERR_STATE, "no state uploaded");
     if (int r = sync_macros(h)) return r;
     const size_t N = h->S.N, QN = (size_t)Q9 * N;
@@ -394,6 +413,7 @@ int mpd_download_state(mpd_handle *h, double *tilde_h, double *tilde_f, double *
 int mpd_step(mpd_handle *h, int64_t nsteps)
 {
     if (!h) return fail(MPD_ERR_INVALID, "null handle");
+    if (h->batch) return fail(MPD_ERR_STATE, "handle is a member of a batch: step the batch");
     if (!h->uploaded) return fail(MPD_ERR_STATE, "no state uploaded");
     if (nsteps < 0) return fail(MPD_ERR_INVALID, "nsteps < 0");
     if (nsteps == 0) return 0;
@@ -509,6 +529,7 @@ int mpd_device_bytes(const mpd_handle *h, int64_t *bytes)
 int mpd_reductions(mpd_handle *h, int update_residual, mpd_sums *out)
 {
     if (!h || !out) return fail(MPD_ERR_INVALID, "null argument");
+    if (h->batch) HIPCHK(hipStreamSynchronize(h->batch->stream));
     if (!h->uploaded) return fail(MPD_ERR_STATE, "no state uploaded");
     if (int r = sync_macros(h)) return r;
     hipLaunchKernelGGL(k_reduce, dim3(h->red_blocks), dim3(RED_THREADS), 0, h->stream, h->S.m, h->S.rho1k,
@@ -537,6 +558,152 @@ int mpd_reductions(mpd_handle *h, int update_residual, mpd_sums *out)
     out->residual = std::sqrt((s[5] + c[5]) / ((s[6] + c[6]) + 1.0E-30));
     out->nonfinite = (int64_t)bad;
     if (bad > 0) return fail(MPD_ERR_NONFINITE, "%lld cells with NaN/Inf or rho <= 0", (long long)bad);
+    return 0;
+}
+
+// ---------------- batched ensembles ----------------
+
+static void batch_drop_graph(mpd_batch *b)
+{
+    if (b->exec) (void)hipGraphExecDestroy(b->exec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    b->exec = nullptr;
+    b->graph = nullptr;
+}
+
+int mpd_batch_create(mpd_batch **out)
+{
+    if (!out) return fail(MPD_ERR_INVALID, "null output pointer");
+    *out = nullptr;
+    mpd_batch *b = new (std::nothrow) mpd_batch();
+    if (!b) return fail(MPD_ERR_NOMEM, "host allocation failed");
+    hipError_t e = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipEventCreate(&b->ev0);
+    if (e == hipSuccess) e = hipEventCreate(&b->ev1);
+    b->plan = batch32_new();
+    if (e != hipSuccess || !b->plan) {
+        mpd_batch_destroy(b);
+        return fail(MPD_ERR_DEVICE, "batch set-up failed: %s", hipGetErrorString(e));
+    }
+    *out = b;
+    return 0;
+}
+
+int mpd_batch_destroy(mpd_batch *b)
+{
+    if (!b) return 0;
+    if (b->stream) (void)hipStreamSynchronize(b->stream);
+    for (mpd_handle *h : b->members) h->batch = nullptr;
+    batch_drop_graph(b);
+    batch32_free(b->plan);
+    if (b->ev0) (void)hipEventDestroy(b->ev0);
+    if (b->ev1) (void)hipEventDestroy(b->ev1);
+    if (b->stream) (void)hipStreamDestroy(b->stream);
+    delete b;
+    return 0;
+}
+
+int mpd_batch_add(mpd_batch *b, mpd_handle *h)
+{
+    if (!b || !h) return fail(MPD_ERR_INVALID, "null argument");
+    if (h->batch) return fail(MPD_ERR_STATE, "handle already belongs to a batch");
+    if (h->cfg.precision != MPD_FP32_BULK || !fused_supported(h->cfg))
+        return fail(MPD_ERR_INVALID, "batches take FP32-bulk cases of the fused path (Allen-Cahn, CD)");
+    if (!h->fused_primed || h->effective_path != MPD_PATH_FUSED)
+        return fail(MPD_ERR_STATE, "step the case at least twice on the fused path before adding it");
+    if (!b->members.empty()) {
+        const mpd_config &c0 = b->members[0]->cfg;
+        if (c0.model != h->cfg.model || c0.bc_y != h->cfg.bc_y)
+            return fail(MPD_ERR_INVALID, "all cases of a batch share the model and the y boundary");
+    }
+    HIPCHK(hipStreamSynchronize(h->stream));
+    HIPCHK(hipStreamSynchronize(b->stream));
+    b->members.push_back(h);
+    h->batch = b;
+    b->dirty = true;
+    return 0;
+}
+
+int mpd_batch_remove(mpd_batch *b, mpd_handle *h)
+{
+    if (!b || !h) return fail(MPD_ERR_INVALID, "null argument");
+    if (h->batch != b) return fail(MPD_ERR_STATE, "handle is not a member of this batch");
+    HIPCHK(hipStreamSynchronize(b->stream));
+    for (size_t k = 0; k < b->members.size(); ++k)
+        if (b->members[k] == h) {
+            b->members.erase(b->members.begin() + (long)k);
+            break;
+        }
+    h->batch = nullptr;
+    b->dirty = true;
+    return 0;
+}
+
+int mpd_batch_size(const mpd_batch *b, int32_t *ncases, int64_t *ncells)
+{
+    if (!b) return fail(MPD_ERR_INVALID, "null batch");
+    int64_t cells = 0;
+    for (mpd_handle *h : b->members) cells += h->S.N;
+    if (ncases) *ncases = (int32_t)b->members.size();
+    if (ncells) *ncells = cells;
+    return 0;
+}
+
+static int batch_rebuild(mpd_batch *b)
+{
+    batch_drop_graph(b);
+    std::vector<BatchMember> m(b->members.size());
+    for (size_t k = 0; k < b->members.size(); ++k) {
+        mpd_handle *h = b->members[k];
+        m[k].C = &h->C;
+        m[k].N = h->S.N;
+        m[k].pl[0] = fused32_planes(h->F, 0);
+        m[k].pl[1] = fused32_planes(h->F, 1);
+        m[k].cur = h->F.cur;
+    }
+    if (batch32_plan(b->plan, m.data(), (int)m.size(), b->stream))
+        return fail(MPD_ERR_DEVICE, "batch tables: %s", hipGetErrorString(hipGetLastError()));
+    // one graph = two steps (in->out, out->in), so the planes are back in place after each replay
+    HIPCHK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+    batch32_step(b->plan, 0, b->stream);
+    batch32_step(b->plan, 1, b->stream);
+    HIPCHK(hipStreamEndCapture(b->stream, &b->graph));
+    HIPCHK(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
+    b->dirty = false;
+    return 0;
+}
+
+int mpd_batch_step(mpd_batch *b, int64_t nsteps)
+{
+    if (!b) return fail(MPD_ERR_INVALID, "null batch");
+    if (nsteps < 0) return fail(MPD_ERR_INVALID, "nsteps < 0");
+    if (nsteps == 0 || b->members.empty()) return 0;
+    if (b->dirty)
+        if (int r = batch_rebuild(b)) return r;
+    HIPCHK(hipEventRecord(b->ev0, b->stream));
+    for (int64_t s = 0; s + 1 < nsteps; s += 2) HIPCHK(hipGraphLaunch(b->exec, b->stream));
+    if (nsteps % 2) {
+        batch32_step(b->plan, 0, b->stream);
+        HIPCHK(hipGetLastError());
+        for (mpd_handle *h : b->members) h->F.cur = 1 - h->F.cur;
+        b->dirty = true;
+    }
+    HIPCHK(hipEventRecord(b->ev1, b->stream));
+    HIPCHK(hipEventSynchronize(b->ev1));
+    float ms = 0.f;
+    HIPCHK(hipEventElapsedTime(&ms, b->ev0, b->ev1));
+    b->acc_ms += ms;
+    b->steps += nsteps;
+    return 0;
+}
+
+int mpd_batch_timing(mpd_batch *b, double *avg_ms, int64_t *steps)
+{
+    if (!b) return fail(MPD_ERR_INVALID, "null batch");
+    if (avg_ms) *avg_ms = b->steps ? b->acc_ms / (double)b->steps : 0.0;
+    if (steps) *steps = b->steps;
+    b->acc_ms = 0.0;
+    b->steps = 0;
     return 0;
 }
 

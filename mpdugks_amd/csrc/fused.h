@@ -32,6 +32,22 @@ struct FusedPlanes32 {
 // one FP32-bulk step (fused_fp32.hip)
 void fused32_step(const FusedPlanes32 &in, const FusedPlanes32 &out, const Consts &C, int64_t N, hipStream_t st);
 
+// batched FP32-bulk step over many cases (fused_fp32.hip)
+struct BatchMember {
+    const Consts *C;
+    int64_t N;
+    FusedPlanes32 pl[2];
+    int cur;
+};
+struct Batch32Plan;
+Batch32Plan *batch32_new();
+void batch32_free(Batch32Plan *P);
+int batch32_plan(Batch32Plan *P, const BatchMember *m, int n, hipStream_t st);
+// one step of every case; parity 0/1 selects in/out relative to each case's cur
+void batch32_step(const Batch32Plan *P, int parity, hipStream_t st);
+// FP32 planes of a fused state (set 0/1)
+FusedPlanes32 fused32_planes(FusedState &F, int which);
+
 bool fused_supported(const mpd_config &cfg);
 int fused_alloc(FusedState &F, const DevState &S, const Consts &C, const mpd_config &cfg);
 void fused_free(FusedState &F);

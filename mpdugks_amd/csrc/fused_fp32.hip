@@ -16,6 +16,8 @@
 // Compiled with FP contraction enabled.
 #include <hip/hip_runtime.h>
 
+#include <vector>
+
 #include "fused.h"
 #include "physics.h"
 
@@ -47,6 +49,15 @@ struct K32 {
     float bbp0, cbp0, bh0, ch0;       // AC-p constant factors
     float flx;                        // MaSpan * face area
     float w[9];
+};
+
+// one case of a batched launch (device table entry)
+struct BatchCase {
+    K32 k;
+    int nx, ny;
+    int64_t N;
+    FusedPlanes32 pl[2];
+    int cur;  // plane set holding the current state
 };
 
 __device__ __forceinline__ int wrapc(int i, int n)
@@ -197,9 +208,10 @@ __device__ __forceinline__ void face_flux(const K32 &k, const float *bp, const f
     }
 }
 
+// One 16x15 tile of one case: stages A-D of the step.
 template <int MODEL, int WALLS>
-__global__ void __launch_bounds__(NT) k_fused32(K32 k, int nx, int ny, FusedPlanes32 in, FusedPlanes32 out, int64_t N,
-                                               int ntx, int ntiles)
+__device__ __forceinline__ void tile32(const K32 &k, int nx, int ny, const FusedPlanes32 &in, const FusedPlanes32 &out,
+                                       int64_t N, int i0, int j0)
 {
     constexpr bool HCZ = MODEL == MPD_MODEL_AC_HCZ;
     __shared__ float s_bp[18][R1N];        // bar-plus h(0..8) f(9..17); later y-face fluxes
@@ -208,10 +220,6 @@ __global__ void __launch_bounds__(NT) k_fused32(K32 k, int nx, int ny, FusedPlan
     float *s_phi = s_u, *s_phl = s_u + R2N;
     static_assert(2 * R2N <= 2 * NZ * NXF, "union too small");
     static_assert((2 * NZ + 1) * NYF <= 18 * R1N, "y-face fluxes do not fit");
-
-    const int tile = tile_of_block(blockIdx.x, ntiles);
-    if (tile < 0) return;
-    const int i0 = (tile % ntx) * TX, j0 = (tile / ntx) * TY;
     const int tid = threadIdx.x;
 
     // ---- stage A ----
@@ -446,6 +454,30 @@ __global__ void __launch_bounds__(NT) k_fused32(K32 k, int nx, int ny, FusedPlan
     out.phi[n] = phi;
 }
 
+template <int MODEL, int WALLS>
+__global__ void __launch_bounds__(NT) k_fused32(K32 k, int nx, int ny, FusedPlanes32 in, FusedPlanes32 out, int64_t N,
+                                               int ntx, int ntiles)
+{
+    const int tile = tile_of_block(blockIdx.x, ntiles);
+    if (tile < 0) return;
+    tile32<MODEL, WALLS>(k, nx, ny, in, out, N, (tile % ntx) * TX, (tile / ntx) * TY);
+}
+
+// Batched step: every workgroup takes one (case, tile) entry of a work list
+// built on the host; the cases share the model and the y boundary.
+template <int MODEL, int WALLS>
+__global__ void __launch_bounds__(NT) k_fused32_batch(const BatchCase *cases, const int2 *work, int nwork, int parity)
+{
+    const int b = blockIdx.x;
+    if (b >= nwork) return;
+    const int2 w = work[b];
+    if (w.x < 0) return;
+    const BatchCase &c = cases[w.x];
+    const int sel = c.cur ^ parity;
+    const int ntx = (c.nx + TX - 1) / TX;
+    tile32<MODEL, WALLS>(c.k, c.nx, c.ny, c.pl[sel], c.pl[sel ^ 1], c.N, (w.y % ntx) * TX, (w.y / ntx) * TY);
+}
+
 K32 make_k32(const Consts &C)
 {
     K32 k;
@@ -495,6 +527,84 @@ K32 make_k32(const Consts &C)
 }
 
 }  // namespace
+
+struct Batch32Plan {
+    BatchCase *d_cases = nullptr;
+    int2 *d_work = nullptr;
+    int ncases = 0, nwork = 0, model = 0, walls = 0;
+    size_t cap_cases = 0, cap_work = 0;
+};
+
+Batch32Plan *batch32_new() { return new Batch32Plan(); }
+
+void batch32_free(Batch32Plan *P)
+{
+    if (!P) return;
+    if (P->d_cases) (void)hipFree(P->d_cases);
+    if (P->d_work) (void)hipFree(P->d_work);
+    delete P;
+}
+
+// (Re)build the device tables: cases in order, tiles row-major per case, then
+// dealt so that XCD b%8 receives a contiguous run of the list.
+int batch32_plan(Batch32Plan *P, const BatchMember *m, int n, hipStream_t st)
+{
+    std::vector<BatchCase> cs((size_t)n);
+    std::vector<int2> list;
+    for (int c = 0; c < n; ++c) {
+        const Consts &C = *m[c].C;
+        BatchCase &b = cs[(size_t)c];
+        b.k = make_k32(C);
+        b.nx = C.nx;
+        b.ny = C.ny;
+        b.N = m[c].N;
+        b.pl[0] = m[c].pl[0];
+        b.pl[1] = m[c].pl[1];
+        b.cur = m[c].cur;
+        const int ntiles = ((C.nx + TX - 1) / TX) * ((C.ny + TY - 1) / TY);
+        for (int t = 0; t < ntiles; ++t) list.push_back(make_int2(c, t));
+    }
+    const int per = ((int)list.size() + 7) / 8;
+    std::vector<int2> work((size_t)per * 8, make_int2(-1, 0));
+    for (int b = 0; b < per * 8; ++b) {
+        const int p = (b % 8) * per + b / 8;
+        if (p < (int)list.size()) work[(size_t)b] = list[(size_t)p];
+    }
+    if ((size_t)n > P->cap_cases) {
+        if (P->d_cases) (void)hipFree(P->d_cases);
+        if (hipMalloc(&P->d_cases, sizeof(BatchCase) * (size_t)n) != hipSuccess) return -1;
+        P->cap_cases = (size_t)n;
+    }
+    if (work.size() > P->cap_work) {
+        if (P->d_work) (void)hipFree(P->d_work);
+        if (hipMalloc(&P->d_work, sizeof(int2) * work.size()) != hipSuccess) return -1;
+        P->cap_work = work.size();
+    }
+    if (n && hipMemcpyAsync(P->d_cases, cs.data(), sizeof(BatchCase) * (size_t)n, hipMemcpyHostToDevice, st) != hipSuccess)
+        return -1;
+    if (!work.empty() &&
+        hipMemcpyAsync(P->d_work, work.data(), sizeof(int2) * work.size(), hipMemcpyHostToDevice, st) != hipSuccess)
+        return -1;
+    if (hipStreamSynchronize(st) != hipSuccess) return -1;
+    P->ncases = n;
+    P->nwork = (int)work.size();
+    P->model = n ? m[0].C->model : 0;
+    P->walls = n ? m[0].C->walls : 0;
+    return 0;
+}
+
+void batch32_step(const Batch32Plan *P, int parity, hipStream_t st)
+{
+    if (!P->nwork) return;
+    dim3 g((unsigned)P->nwork), b(NT);
+    if (P->model == MPD_MODEL_AC_HCZ) {
+        if (P->walls) hipLaunchKernelGGL((k_fused32_batch<MPD_MODEL_AC_HCZ, 1>), g, b, 0, st, P->d_cases, P->d_work, P->nwork, parity);
+        else hipLaunchKernelGGL((k_fused32_batch<MPD_MODEL_AC_HCZ, 0>), g, b, 0, st, P->d_cases, P->d_work, P->nwork, parity);
+    } else {
+        if (P->walls) hipLaunchKernelGGL((k_fused32_batch<MPD_MODEL_AC_P, 1>), g, b, 0, st, P->d_cases, P->d_work, P->nwork, parity);
+        else hipLaunchKernelGGL((k_fused32_batch<MPD_MODEL_AC_P, 0>), g, b, 0, st, P->d_cases, P->d_work, P->nwork, parity);
+    }
+}
 
 void fused32_step(const FusedPlanes32 &in, const FusedPlanes32 &out, const Consts &C, int64_t N, hipStream_t st)
 {

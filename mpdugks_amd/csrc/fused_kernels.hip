@@ -532,6 +532,12 @@ void convert_t(FusedState &F, const DevState &S, const Consts &C, int which, hip
 
 }  // namespace
 
+FusedPlanes32 fused32_planes(FusedState &F, int which)
+{
+    Planes<float> a = planes_of<float>(F, which);
+    return FusedPlanes32{a.h, a.f, a.phi, a.phl, a.ppu, a.ppv, a.gl};
+}
+
 bool fused_supported(const mpd_config &cfg)
 {
     return (cfg.model == MPD_MODEL_AC_HCZ || cfg.model == MPD_MODEL_AC_P) && cfg.flux == MPD_FLUX_CD &&

@@ -33,6 +33,13 @@ _SYMBOLS = {
     "mpd_kernel_timing": ([C.c_void_p, C.c_void_p, C.c_void_p], C.c_int),
     "mpd_last_error": ([], C.c_char_p),
     "mpd_set_kernel_timing": ([C.c_void_p, C.c_int], C.c_int),
+    "mpd_batch_create": ([C.c_void_p], C.c_int),
+    "mpd_batch_destroy": ([C.c_void_p], C.c_int),
+    "mpd_batch_add": ([C.c_void_p, C.c_void_p], C.c_int),
+    "mpd_batch_remove": ([C.c_void_p, C.c_void_p], C.c_int),
+    "mpd_batch_size": ([C.c_void_p, C.c_void_p, C.c_void_p], C.c_int),
+    "mpd_batch_step": ([C.c_void_p, C.c_int64], C.c_int),
+    "mpd_batch_timing": ([C.c_void_p, C.c_void_p, C.c_void_p], C.c_int),
     "mpd_init_ac_drop": ([C.c_void_p, C.c_double, C.c_double, C.c_double, C.c_int32, C.c_double,
                           C.c_double, C.c_void_p, C.c_void_p, C.c_void_p], C.c_int),
     "mpd_init_ac_interface": ([C.c_void_p, C.c_void_p, C.c_int32, C.c_double, C.c_void_p, C.c_void_p,
@@ -202,6 +209,48 @@ class Solver:
         ms, n = C.c_double(), C.c_int64()
         _check(lib().mpd_kernel_timing(self._h, C.byref(ms), C.byref(n)), "mpd_kernel_timing")
         return ms.value, n.value
+
+
+class Batch:
+    """Many FP32-bulk cases advanced by one kernel launch per step (mpd_batch_*)."""
+
+    def __init__(self):
+        self._b = C.c_void_p()
+        _check(lib().mpd_batch_create(C.byref(self._b)), "mpd_batch_create")
+        self.members = []
+
+    def add(self, solver_):
+        _check(lib().mpd_batch_add(self._b, solver_._h), "mpd_batch_add")
+        self.members.append(solver_)
+
+    def remove(self, solver_):
+        _check(lib().mpd_batch_remove(self._b, solver_._h), "mpd_batch_remove")
+        self.members.remove(solver_)
+
+    def step(self, n):
+        _check(lib().mpd_batch_step(self._b, int(n)), "mpd_batch_step")
+
+    def size(self):
+        nc, ncell = C.c_int32(), C.c_int64()
+        _check(lib().mpd_batch_size(self._b, C.byref(nc), C.byref(ncell)), "mpd_batch_size")
+        return nc.value, ncell.value
+
+    def timing(self):
+        ms, n = C.c_double(), C.c_int64()
+        _check(lib().mpd_batch_timing(self._b, C.byref(ms), C.byref(n)), "mpd_batch_timing")
+        return ms.value, n.value
+
+    def close(self):
+        if getattr(self, "_b", None) and self._b.value:
+            lib().mpd_batch_destroy(self._b)
+            self._b = C.c_void_p()
+            self.members = []
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def default_config(model=abi.MPD_MODEL_AC_HCZ):
