@@ -116,6 +116,27 @@ def ensemble_bench(ncases, steps, seed=0, solve_steps=20000):
             "nonfinite_in_first_8": int(sum(r["nonfinite"] for r in red))}
 
 
+def serving_bench(nreq, max_steps, budget_gb, glups, seed=0):
+    """Poisson stream of two-droplet coalescence requests (radii U(10,60), gap
+    U{2..8}, mesh side 64-1024), FP32-bulk, KE-residual stop 1e-8 checked every
+    1000 steps with a cap of `max_steps`; the mean arrival rate puts the
+    offered work at ~80% of `glups` (the batched throughput)."""
+    from mpdugks_amd import serve
+    probe = serve.coalescence_requests(nreq, rate=1.0, seed=seed)
+    mean_work = np.mean([r.cells for r in probe]) * max_steps
+    rate = 0.8 * glups * 1e9 / mean_work
+    reqs = serve.coalescence_requests(nreq, rate=rate, seed=seed)
+    eng = serve.GpuEngine()
+    srv = serve.Server(eng, hbm_budget=int(budget_gb * 2 ** 30), check_every=1000, ke_tol=1e-8,
+                       max_steps=max_steps)
+    stats = srv.run(reqs)
+    eng.close()
+    stats.update({"requests": len(reqs), "arrival_rate_per_s": rate, "max_steps": max_steps,
+                  "hbm_budget_gb": budget_gb, "ke_tol": 1e-8,
+                  "achieved_glups": stats["cell_steps"] / stats["wall_s"] / 1e9})
+    return stats
+
+
 def copy_calibration(mib):
     """A plain device-to-device copy of `mib` MiB (reads and writes exactly that
     many bytes), run 5 times; its FETCH_SIZE / WRITE_SIZE per dispatch give the
@@ -159,6 +180,10 @@ def main():
     ap.add_argument("--ensemble-cases", type=int, default=256)
     ap.add_argument("--ensemble-steps", type=int, default=400)
     ap.add_argument("--no-ensemble", action="store_true")
+    ap.add_argument("--serve-requests", type=int, default=32)
+    ap.add_argument("--serve-max-steps", type=int, default=20000)
+    ap.add_argument("--serve-budget-gb", type=float, default=2.0)
+    ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--copy-mb", type=float, default=0.0,
                     help="also run a plain device-to-device copy of this many MiB (counter calibration)")
     args = ap.parse_args()
@@ -246,6 +271,13 @@ def main():
               "%.3f (FP64 plain), x%.1f" % (ens["cases"], ens["cells"], ens["batch_ms_per_step"],
                                              ens["solves_per_s_fp32_batched"], ens["solves_per_s_fp64_plain"],
                                              ens["speedup_vs_fp64_plain"]), file=sys.stderr, flush=True)
+    if not args.no_serve:
+        glups = line.get("ensemble", {}).get("batch_mlups", 10000.0) / 1e3
+        sv = serving_bench(args.serve_requests, args.serve_max_steps, args.serve_budget_gb, glups)
+        line["serving"] = sv
+        print("[bench] serving: %d done, %.2f solves/s, p50 %.3f s, p99 %.3f s, busy %.2f"
+              % (sv["completed"], sv["solves_per_s"], sv["p50_latency_s"], sv["p99_latency_s"],
+                 sv["gpu_busy_frac"]), file=sys.stderr, flush=True)
     if not args.no_cpu:
         cfg, st = cases.rayleigh_taylor(nx=nx, ny=ny, precision=abi.MPD_FP64)
         mlups, nsteps, dt = cpu_baseline(cfg, st, args.cpu_seconds)

@@ -1,0 +1,163 @@
+"""Serving loop: simulation requests kept resident on the GPU in one batch.
+
+Requests arrive over time (seeded Poisson process); each is admitted when its
+arrival time has passed and its device memory fits the HBM budget, primed with
+the two explicit steps the derived state needs, and added to the batch
+(libmpdugks_hip's mpd_batch_*: one kernel launch per step over all resident
+cases).  Every `check_every` steps each case's kinetic-energy residual
+|KE^n - KE^(n-check_every)| / KE^n is evaluated from FP64 device sums; a case
+retires when it falls below `ke_tol` or reaches `max_steps`.  Latency is
+completion time minus arrival time (wall clock).
+
+The reference runs one case per process with a density residual stop
+(Ark/Cpp/DUGKS2DSolver.cpp:807-820); this loop and its KE residual are new.
+"""
+import collections
+import time
+
+import numpy as np
+
+
+class Request:
+    __slots__ = ("rid", "arrival", "make", "bytes_estimate", "admitted", "done", "steps", "converged",
+                 "ke_prev", "handle", "cells")
+
+    def __init__(self, rid, arrival, make, bytes_estimate, cells):
+        self.rid = rid
+        self.arrival = arrival
+        self.make = make              # () -> (cfg, state)
+        self.bytes_estimate = bytes_estimate
+        self.cells = cells
+        self.admitted = None
+        self.done = None
+        self.steps = 0
+        self.converged = False
+        self.ke_prev = None
+        self.handle = None
+
+
+class GpuEngine:
+    """Runs cases through the library's batch API."""
+
+    def __init__(self):
+        from . import solver
+        self._solver = solver
+        self.batch = solver.Batch()
+
+    def admit(self, req):
+        from . import abi
+        cfg, st = req.make()
+        s = self._solver.Solver(cfg, path=abi.MPD_PATH_FUSED)
+        s.upload(st)
+        s.step(2)
+        self.batch.add(s)
+        req.handle = s
+        req.steps = 2
+        return s.device_bytes()
+
+    def step(self, n):
+        self.batch.step(n)
+
+    def kinetic_energy(self, req):
+        return req.handle.reductions()["kinetic_energy"]
+
+    def retire(self, req):
+        self.batch.remove(req.handle)
+        req.handle.close()
+        req.handle = None
+
+    def close(self):
+        self.batch.close()
+
+
+class Server:
+    def __init__(self, engine, hbm_budget, check_every=1000, ke_tol=1e-8, max_steps=10 ** 6,
+                 clock=time.perf_counter, sleep=time.sleep):
+        self.engine = engine
+        self.budget = hbm_budget
+        self.check_every = check_every
+        self.ke_tol = ke_tol
+        self.max_steps = max_steps
+        self.clock = clock
+        self.sleep = sleep
+        self.used = 0
+        self.rejected = []
+
+    def run(self, requests):
+        pending = collections.deque(sorted(requests, key=lambda r: r.arrival))
+        active, finished = [], []
+        busy = 0.0
+        t0 = self.clock()
+        while pending or active:
+            now = self.clock() - t0
+            # admission in arrival order while the HBM budget allows
+            while pending and pending[0].arrival <= now:
+                r = pending[0]
+                if r.bytes_estimate > self.budget:
+                    self.rejected.append(pending.popleft())   # can never fit
+                    continue
+                if self.used + r.bytes_estimate > self.budget:
+                    break
+                pending.popleft()
+                r.admitted = now
+                r.bytes_estimate = max(r.bytes_estimate, self.engine.admit(r))
+                self.used += r.bytes_estimate
+                active.append(r)
+            if not active:
+                if pending:
+                    self.sleep(max(0.0, min(pending[0].arrival - now, 0.01)))
+                continue
+            t1 = self.clock()
+            self.engine.step(self.check_every)
+            busy += self.clock() - t1
+            now = self.clock() - t0
+            still = []
+            for r in active:
+                r.steps += self.check_every
+                ke = self.engine.kinetic_energy(r)
+                res = abs(ke - r.ke_prev) / ke if (r.ke_prev is not None and ke > 0) else np.inf
+                r.ke_prev = ke
+                if res < self.ke_tol or r.steps >= self.max_steps:
+                    r.converged = res < self.ke_tol
+                    r.done = now
+                    self.engine.retire(r)
+                    self.used -= r.bytes_estimate
+                    finished.append(r)
+                else:
+                    still.append(r)
+            active = still
+        wall = self.clock() - t0
+        lat = np.array([r.done - r.arrival for r in finished]) if finished else np.array([0.0])
+        return {
+            "completed": len(finished),
+            "rejected": len(self.rejected),
+            "converged": int(sum(r.converged for r in finished)),
+            "wall_s": wall,
+            "solves_per_s": len(finished) / wall if wall > 0 else 0.0,
+            "p50_latency_s": float(np.percentile(lat, 50)),
+            "p99_latency_s": float(np.percentile(lat, 99)),
+            "gpu_busy_frac": busy / wall if wall > 0 else 0.0,
+            "cell_steps": int(sum(r.cells * r.steps for r in finished)),
+        }
+
+
+def coalescence_requests(n, rate, seed=0, max_side=1024):
+    """n two-droplet requests with Poisson arrivals of mean rate `rate` (1/s):
+    radii U(10, 60) cells, gap U{2..8}; mesh from cases.coalescence_case."""
+    from . import cases
+    rng = np.random.default_rng(seed)
+    t = 0.0
+    reqs = []
+    for k in range(n):
+        t += rng.exponential(1.0 / rate)
+        r1, r2 = rng.uniform(10, 60, 2)
+        gap = int(rng.integers(2, 9))
+        need = 2 * (r1 + r2) + gap + 16
+        side = next(s for s in (64, 96, 128, 192, 256, 384, 512, 768, 1024) if s >= need)
+        if side > max_side:
+            continue
+        make = (lambda a=r1, b=r2, g=gap: cases.coalescence_case(a, b, g))
+        # device bytes: FP64 explicit state + phase scratch + two FP32 derived copies
+        cells = side * side
+        reqs.append(Request(k, t, make, bytes_estimate=cells * 1100, cells=cells))
+    return reqs
