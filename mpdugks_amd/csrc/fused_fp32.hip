@@ -26,29 +26,37 @@ namespace mpd {
 namespace {
 
 constexpr int TX = 16, TY = 15;
-constexpr int R1X = TX + 2, R1Y = TY + 2;
-constexpr int R2X = TX + 4, R2Y = TY + 4;
+constexpr int R1X = TX + 2, R1Y = TY + 2;  // bar-plus region (cells with a one-cell halo)
+constexpr int R2X = TX + 4, R2Y = TY + 4;  // phi region (two-cell halo)
 constexpr int R1N = R1X * R1Y, R2N = R2X * R2Y;
-constexpr int NXF = (TX + 1) * TY;  // 255
-constexpr int NYF = TX * (TY + 1);  // 256
+constexpr int NIN = TX * TY;               // interior cells of a tile
+constexpr int NRING = R1N - NIN;           // halo ring of the bar-plus region
+constexpr int NXF = (TX + 1) * TY;         // 255 x-faces
+constexpr int NYF = TX * (TY + 1);         // 256 y-faces
 constexpr int NT = 256;
-constexpr int NZ = 6;               // populations with non-zero normal velocity per face direction
-
-// populations with e_x != 0 (x-face fluxes) and e_y != 0 (y-face fluxes)
-__host__ __device__ constexpr int xq(int z) { return z == 0 ? 1 : z == 1 ? 2 : z == 2 ? 4 : z == 3 ? 5 : z == 4 ? 6 : 8; }
-__host__ __device__ constexpr int yq(int z) { return z == 0 ? 2 : z == 1 ? 3 : z == 2 ? 4 : z == 3 ? 6 : z == 4 ? 7 : 8; }
+constexpr int PS = 320;                    // LDS plane stride in floats: a multiple of 64 dwords, so every
+                                           // plane of one site is an immediate offset from one address
+constexpr int FS = 256;                    // LDS stride of the face-flux planes
+constexpr int NBP = 24;                    // 9 h + 9 f bar-plus planes, then 6 half-macro planes
+constexpr int NFL = 13;                    // per face direction: 6 h fluxes, 6 f fluxes, h mass flux
+static_assert(R1N <= PS && NXF <= FS && NYF <= FS, "plane strides");
+static_assert(NIN <= NT && NRING - (NT - NIN) <= 64, "stage-B mapping: one partial wave for the ring");
+static_assert(2 * NFL * FS <= NBP * PS, "face fluxes must fit over the bar-plus planes");
 
 struct K32 {
     float ms;        // MaSpan
     float ms_rt;     // MaSpan / RT
-    float rt, inv_rt, lambda0, hdt, dt, aphi, bphi, beta, kappa, phi_l, phi_v, nu_v, dnu;
-    float inv_d2x6, inv_lapden, inv_rtdt, inv_wdphi, inv_dx, inv_2dx, kp, dtv;
+    float rt, inv_rt, mlambda0, hdt, hhdt, hhdt_rt, dt, bdt, aphi, bphi, beta, kappa, phi_l, phi_v;
+    float tau_a, tau_b;               // tau = tau_a phi + tau_b (HCZ)
+    float inv_d2x6, inv_lapden, inv_rtdt, inv_wdphi, kp, dtv;
+    float ra, rb, rt4;                // face reconstruction: (1/2 + A, 1/2 - A), tangential factor
     float gx, gy, rho_ref;
-    float bbp_h, cbp_h, bh_h, ch_h;   // Allen-Cahn factors (constant tau_h)
-    float tau0;                       // AC-p constant tau
-    float bbp0, cbp0, bh0, ch0;       // AC-p constant factors
+    float bbp_h, bh_h;                // Allen-Cahn factors (constant tau_h)
+    float cbpw_h[3], chw_h[3];        // c * w_q for the weight classes 0 / axis / diagonal
+    float bbp0, bh0;                  // AC-p constant factors
+    float cbpw0[3], chw0[3];
     float flx;                        // MaSpan * face area
-    float w[9];
+    float w[3];                       // weights rest / axis / diagonal, w0 = 1 - 4 w1 - 4 w2
 };
 
 // one case of a batched launch (device table entry)
@@ -60,10 +68,16 @@ struct BatchCase {
     int cur;  // plane set holding the current state
 };
 
-__device__ __forceinline__ int wrapc(int i, int n)
+// periodic wrap; `small` (uniform) selects the general form for meshes narrower
+// than the halo span, otherwise one conditional add or subtract suffices
+__device__ __forceinline__ int wrap(int i, int n, bool small)
 {
-    int r = i % n;
-    return r < 0 ? r + n : r;
+    if (small) {
+        int r = i % n;
+        return r < 0 ? r + n : r;
+    }
+    i += i < 0 ? n : 0;
+    return i >= n ? i - n : i;
 }
 
 __device__ __forceinline__ int tile_of_block(int b, int ntiles)
@@ -76,382 +90,567 @@ __device__ __forceinline__ int tile_of_block(int b, int ntiles)
 
 __device__ __forceinline__ float rcp(float x) { return __builtin_amdgcn_rcpf(x); }
 
-// grad(phi), laplacian, SetZero (GradScheme.cpp:347-376); c: E N W S, d: NE NW SW SE
-__device__ __forceinline__ void stencil(const K32 &k, float phi, const float *c, const float *d, float &gx, float &gy,
-                                        float &lap)
+// Buffer resources over the planes of one case: loads and stores are a
+// per-lane 32-bit byte offset plus a uniform plane offset, with no 64-bit
+// address arithmetic (planes of one array span less than 4 GiB).
+using rsrc_t = __amdgpu_buffer_rsrc_t;
+__device__ __forceinline__ rsrc_t make_rsrc(const float *p, uint32_t bytes)
 {
-    gx = (4.f * (c[0] - c[2]) + (d[0] - d[1] + d[3] - d[2])) * k.inv_d2x6;
-    gy = (4.f * (c[1] - c[3]) + (d[0] - d[3] + d[1] - d[2])) * k.inv_d2x6;
-    lap = (4.f * (c[0] + c[1] + c[2] + c[3]) + (d[0] + d[1] + d[2] + d[3]) - 20.f * phi) * k.inv_lapden;
+    return __builtin_amdgcn_make_buffer_rsrc((void *)p, (short)0, (int)bytes, 0x00020000);
+}
+__device__ __forceinline__ float ld(rsrc_t r, uint32_t n, uint32_t plane_bytes = 0)
+{
+    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, n * 4u, plane_bytes, 0));
+}
+__device__ __forceinline__ void st(rsrc_t r, uint32_t n, float v, uint32_t plane_bytes = 0)
+{
+    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(uint32_t, v), r, n * 4u, plane_bytes, 0);
+}
+
+// The planes of one state set are consecutive in one allocation (h~ 0-8,
+// f~ 9-17, phi 18, phi_lag 19, prevPhiU 20, prevPhiV 21; fused_kernels.hip
+// planes_of), so one resource covers them and a plane is a uniform offset.
+// The lagged ghost-phi rows (walls) are a separate array.
+enum : uint32_t { P_H = 0, P_F = 9, P_PHI = 18, P_PHL = 19, P_PPU = 20, P_PPV = 21 };
+struct Rs {
+    rsrc_t a, gl;
+    uint32_t pb;  // bytes per plane
+};
+__device__ __forceinline__ Rs make_rs(const FusedPlanes32 &p, int64_t N, int nx, int nplanes)
+{
+    Rs r;
+    r.pb = (uint32_t)(N * 4);
+    r.a = make_rsrc(p.h, (uint32_t)nplanes * r.pb);
+    r.gl = make_rsrc(p.gl ? p.gl : p.h, p.gl ? (uint32_t)(2 * nx * 4) : 0u);
+    return r;
+}
+
+// e_a x + e_b y for compile-time e in {-1, 0, 1} (no multiplications by 0 or 1)
+template <int A, int B>
+__device__ __forceinline__ float edot(float x, float y)
+{
+    if (A == 0 && B == 0) return 0.f;
+    if (A == 0) return B > 0 ? y : -y;
+    if (B == 0) return A > 0 ? x : -x;
+    if (A > 0) return B > 0 ? x + y : x - y;
+    return B > 0 ? y - x : -(x + y);
+}
+
+// grad(phi), laplacian, SetZero (GradScheme.cpp:347-376) from the phi region s
+// around index c (row stride R2X)
+__device__ __forceinline__ void stencil(const K32 &k, const float *s, int c, float &gx, float &gy, float &lap)
+{
+    const float e = s[c + 1], nn = s[c + R2X], w = s[c - 1], so = s[c - R2X];
+    const float ne = s[c + R2X + 1], nw = s[c + R2X - 1], sw = s[c - R2X - 1], se = s[c - R2X + 1];
+    const float phi = s[c];
+    gx = (4.f * (e - w) + ((ne - nw) + (se - sw))) * k.inv_d2x6;
+    gy = (4.f * (nn - so) + ((ne - se) + (nw - sw))) * k.inv_d2x6;
+    lap = (4.f * ((e + nn) + (w + so)) + ((ne + nw) + (sw + se)) - 20.f * phi) * k.inv_lapden;
     if (fabsf(gx) < 1.0E-14f) gx = 0.f;
     if (fabsf(gy) < 1.0E-14f) gy = 0.f;
 }
 
-// SourceMomentum (SourceM.cpp:19-34)
+// SourceMomentum (SourceM.cpp:19-34) plus the gravity body force
 __device__ __forceinline__ void momentum_source(const K32 &k, float phi, float gx, float gy, float lap, float &Fx,
                                                 float &Fy, float &rx, float &ry)
 {
-    float mu_c = 2.f * k.beta * (phi - k.phi_l) * (phi - k.phi_v) * (2.f * phi - k.phi_l - k.phi_v) - k.kappa * lap;
+    const float mu_c =
+        2.f * k.beta * (phi - k.phi_l) * (phi - k.phi_v) * (2.f * phi - k.phi_l - k.phi_v) - k.kappa * lap;
     rx = k.aphi * gx;
     ry = k.aphi * gy;
-    Fx = mu_c * gx;
-    Fy = mu_c * gy;
-    float rho = k.aphi * phi + k.bphi;
-    Fx -= k.gx * (rho - k.rho_ref);
-    Fy -= k.gy * (rho - k.rho_ref);
+    const float drho = k.aphi * phi + (k.bphi - k.rho_ref);
+    Fx = mu_c * gx - k.gx * drho;
+    Fy = mu_c * gy - k.gy * drho;
 }
 
-// One face: reconstruct (CD_Interior_phi_Bh, DUGKS2DSolver.cpp:604-687), face
-// macros (Update_MacroVar_h), BGK half step (Update_phi_h) and flux for the six
-// populations with non-zero normal velocity.  bp: 18 planes of R1N, mac: 6 planes.
-template <int DIR, int MODEL, int WALLS>
-__device__ __forceinline__ void face_flux(const K32 &k, const float *bp, const float *mac, int ko, int kn, int step_t,
-                                          float *hfl, float *ffl, int wall)
+// Quantities of one site (cell centre or face) shared by its populations.
+struct Site {
+    float U, V, G0, UF, UR;            // velocity, -|u|^2 Lambda0, u.F, u.grad rho
+    float mFx, mFy, mrx, mry, mAx, mAy;  // MaSpan * (F, grad rho, AC source); AC-p: F / RT
+    float wphi[3], wp[3], wr[3];       // w phi, w p (AC-p: w p / RT), w rho RT per weight class
+};
+
+// Equilibria and sources of the opposite pair (q, q') with e_q = (EX, EY),
+// e_q' = -e_q (Update_phi_Eq + Update_DVDF_Source: D2Q9HCZ.cpp:54-92,244-257,
+// D2Q9AC.cpp:69-105,331-344).  fs, fsp and hs are the sources without the
+// weight; the caller's factor carries it.
+template <int MODEL, int EX, int EY>
+__device__ __forceinline__ void pair_eq(const K32 &k, const Site &m, float &heq, float &heqp, float &feq, float &feqp,
+                                        float &fs, float &fsp, float &hs)
+{
+    constexpr int W = (EX == 0 || EY == 0) ? 1 : 2;
+    hs = edot<EX, EY>(m.mAx, m.mAy);
+    if (MODEL == MPD_MODEL_AC_HCZ) {
+        const float u1 = k.ms_rt * edot<EX, EY>(m.U, m.V);
+        const float S = fmaf(0.5f * u1, u1, m.G0);
+        const float G = S + u1, Gp = S - u1;
+        heq = fmaf(m.wphi[W], u1, m.wphi[W]);
+        heqp = fmaf(-m.wphi[W], u1, m.wphi[W]);
+        feq = fmaf(m.wr[W], G, m.wp[W]);
+        feqp = fmaf(m.wr[W], Gp, m.wp[W]);
+        const float a = edot<EX, EY>(m.mFx, m.mFy), c = edot<EX, EY>(m.mrx, m.mry);
+        const float xF = a - m.UF, xFp = -a - m.UF;
+        const float xR = c - m.UR, xRp = -c - m.UR;
+        fs = fmaf(G, fmaf(k.rt, xR, xF), xF);      // (1 + G) xF + G RT xR
+        fsp = fmaf(Gp, fmaf(k.rt, xRp, xFp), xFp);
+    } else {
+        heq = heqp = m.wphi[W];
+        feq = feqp = m.wp[W];
+        fs = edot<EX, EY>(m.mFx, m.mFy);
+        fsp = -fs;
+    }
+}
+
+// Bar-plus of one cell (Update_MacroVar with the lagged sources, MacroSource,
+// Update_phi_Eq/Source, Update_phi_BP: D2Q9HCZ.cpp:133-182, D2Q9AC.cpp:146-231,
+// SourceM.cpp:35-62, DUGKS2DSolver.cpp:379-401) written to LDS at c.  base
+// receives b + (b - x~)/3 of every population (h: q >= 1), the part of
+// Update_phi_T (DUGKS2DSolver.cpp:748-792) that does not involve the fluxes.
+template <int MODEL, int WALLS>
+__device__ __forceinline__ void cell_bp(const K32 &k, const Rs &in, const Rs &out,
+                                        int nx, int ny, bool small, int i0, int j0, int rx, int ry, float *S,
+                                        const float *s_phi, const float *s_phl, float *base, bool interior)
 {
     constexpr bool HCZ = MODEL == MPD_MODEL_AC_HCZ;
-    float hb[9], fb[9];
-    const float hdt = k.hdt;
+    const int gi = wrap(i0 - 1 + rx, nx, small);
+    int gj = j0 - 1 + ry;
+    if (WALLS && (gj < 0 || gj >= ny)) return;  // wall ghost rows: extrapolated afterwards
+    gj = WALLS ? gj : wrap(gj, ny, small);
+    const uint32_t n = (uint32_t)(gj * nx + gi);
+    float h[9], f[9];
 #pragma unroll
     for (int q = 0; q < 9; ++q) {
-        const int en = DIR == 0 ? ex9(q) : ey9(q);
-        const int et = DIR == 0 ? ey9(q) : ex9(q);
-#pragma unroll
-        for (int s = 0; s < 2; ++s) {
-            const float *b = bp + (s * 9 + q) * R1N;
-            const float bo = b[ko], bn = b[kn];
-            float v = 0.5f * (bo + bn);
-            if (en != 0) v -= (hdt * k.ms * k.inv_dx * (float)en) * (bn - bo);
-            if (et != 0)
-                v -= (hdt * k.ms * k.inv_2dx * 0.5f * (float)et) *
-                     ((b[kn + step_t] - b[kn - step_t]) + (b[ko + step_t] - b[ko - step_t]));
-            if (s == 0) hb[q] = v; else fb[q] = v;
-        }
+        h[q] = ld(in.a, n, (P_H + q) * in.pb);
+        f[q] = ld(in.a, n, (P_F + q) * in.pb);
     }
-    const float ax = 0.5f * (mac[0 * R1N + ko] + mac[0 * R1N + kn]);
-    const float ay = 0.5f * (mac[1 * R1N + ko] + mac[1 * R1N + kn]);
-    const float rgx = 0.5f * (mac[2 * R1N + ko] + mac[2 * R1N + kn]);
-    const float rgy = 0.5f * (mac[3 * R1N + ko] + mac[3 * R1N + kn]);
-    const float Fx = 0.5f * (mac[4 * R1N + ko] + mac[4 * R1N + kn]);
-    const float Fy = 0.5f * (mac[5 * R1N + ko] + mac[5 * R1N + kn]);
-    float phi = 0.f;
-#pragma unroll
-    for (int q = 0; q < 9; ++q) phi += hb[q];
+    const int c = ry * R1X + rx;
+    const int c2 = (ry + 1) * R2X + (rx + 1);
+    const float phi = s_phi[c2];
     const float rho = k.aphi * phi + k.bphi;
-    float U = 0.f, V = 0.f, p, bh, ch;
+    Site m;
+    float p, bbp, tau = 0.f;
+    m.U = m.V = 0.f;
     if (HCZ) {
+        float lgx, lgy, llap, lFx, lFy, lrx, lry;
+        stencil(k, s_phl, c2, lgx, lgy, llap);
+        momentum_source(k, s_phl[c2], lgx, lgy, llap, lFx, lFy, lrx, lry);
         const float irr = rcp(rho * k.rt);
-        const float sx = k.ms * ((fb[1] + fb[2] + fb[8]) - (fb[4] + fb[5] + fb[6]));
-        const float sy = k.ms * ((fb[2] + fb[3] + fb[4]) - (fb[6] + fb[7] + fb[8]));
-        U = (sx + 0.5f * hdt * Fx) * irr;
-        V = (sy + 0.5f * hdt * Fy) * irr;
-        float s = 0.f;
-#pragma unroll
-        for (int q = 0; q < 9; ++q) s += fb[q];
-        p = s + 0.5f * hdt * k.rt * (rgx * U + rgy * V);
-        const float tau = (k.nu_v + (phi - k.phi_v) * k.dnu) * k.inv_rt;
-        bh = hdt * rcp(2.f * tau + hdt);
-        ch = tau * bh;
+        const float sx = (f[1] + f[2] + f[8]) - (f[4] + f[5] + f[6]);
+        const float sy = (f[2] + f[3] + f[4]) - (f[6] + f[7] + f[8]);
+        m.U = fmaf(k.ms, sx, k.hdt * lFx) * irr;
+        m.V = fmaf(k.ms, sy, k.hdt * lFy) * irr;
+        const float s = ((f[0] + f[1]) + (f[2] + f[3])) + ((f[4] + f[5]) + (f[6] + f[7])) + f[8];
+        p = fmaf(k.hdt * k.rt, lrx * m.U + lry * m.V, s);
+        tau = fmaf(phi, k.tau_a, k.tau_b);
+        bbp = k.bdt * rcp(2.f * tau + k.dt);   // 1 - (2 tau - dt/2)/(2 tau + dt)
     } else {
-        float s = 0.f;
-#pragma unroll
-        for (int q = 1; q < 9; ++q) s += fb[q];
-        p = k.kp * s;
-        bh = k.bh0;
-        ch = k.ch0;
+        p = k.kp * (((f[1] + f[2]) + (f[3] + f[4])) + ((f[5] + f[6]) + (f[7] + f[8])));
+        bbp = k.bbp0;
     }
-    const float uu = U * U + V * V;
-    const float UF = U * Fx + V * Fy, UR = U * rgx + V * rgy;
-    float ho[NZ], fo[NZ];
-#pragma unroll
-    for (int z = 0; z < NZ; ++z) {
-        const int q = DIR == 0 ? (z == 0 ? 1 : z == 1 ? 2 : z == 2 ? 4 : z == 3 ? 5 : z == 4 ? 6 : 8)
-                               : (z == 0 ? 2 : z == 1 ? 3 : z == 2 ? 4 : z == 3 ? 6 : z == 4 ? 7 : 8);
-        const float ex = (float)ex9(q), ey = (float)ey9(q), w = k.w[q];
-        float heq, feq, fso;
+    float gx, gy, lap, Fx, Fy, rgx, rgy;
+    stencil(k, s_phi, c2, gx, gy, lap);
+    momentum_source(k, phi, gx, gy, lap, Fx, Fy, rgx, rgy);
+    float ax, ay, npu = 0.f, npv = 0.f;
+    {
+        const float L2 = gx * gx + gy * gy;
+        float dpu = 0.f, dpv = 0.f;
         if (HCZ) {
-            const float u1 = k.ms_rt * (ex * U + ey * V);
-            const float G = u1 * (1.f + 0.5f * u1) - uu * k.lambda0;
-            heq = w * phi * (1.f + u1);
-            feq = w * (p + rho * k.rt * G);
-            const float xF = k.ms * (ex * Fx + ey * Fy) - UF;
-            const float xR = k.ms * (ex * rgx + ey * rgy) - UR;
-            fso = w * ((1.f + G) * xF + G * k.rt * xR);
+            npu = phi * m.U;
+            npv = phi * m.V;
+            dpu = (npu - ld(in.a, n, P_PPU * in.pb)) * k.inv_rtdt;
+            dpv = (npv - ld(in.a, n, P_PPV * in.pb)) * k.inv_rtdt;
+        }
+        if (L2 != 0.f) {
+            const float t = -4.f * (phi - k.phi_l) * (phi - k.phi_v) * k.inv_wdphi * rsqrtf(L2);
+            ax = fmaf(gx, t, dpu);
+            ay = fmaf(gy, t, dpv);
         } else {
-            heq = w * phi;
-            feq = w * p * k.inv_rt;
-            fso = w * k.ms * (ex * Fx + ey * Fy) * k.inv_rt;
-        }
-        const float hso = w * k.ms * (ex * ax + ey * ay);
-        ho[z] = hb[q] + k.bh_h * (heq - hb[q]) + k.ch_h * hso;
-        fo[z] = fb[q] + bh * (feq - fb[q]) + ch * fso;
-    }
-    if (WALLS && DIR == 1 && wall != 0) {
-        // Wall_3_BB (Boundary.cpp:118-139): the y-face populations entering the
-        // fluid (e_y > 0 at the bottom wall, < 0 at the top) take their partner's
-        // value; order {2,3,4,6,7,8} pairs z with z+3
-        float h0[NZ], f0[NZ];
-#pragma unroll
-        for (int z = 0; z < NZ; ++z) { h0[z] = ho[z]; f0[z] = fo[z]; }
-#pragma unroll
-        for (int z = 0; z < NZ; ++z) {
-            const bool entering = wall < 0 ? (z < 3) : (z >= 3);
-            if (entering) { ho[z] = h0[(z + 3) % NZ]; fo[z] = f0[(z + 3) % NZ]; }
+            ax = dpu;
+            ay = dpv;
         }
     }
+    m.mAx = k.ms * ax;
+    m.mAy = k.ms * ay;
+    float cw[3], chw[3];
+    if (HCZ) {
+        m.G0 = k.mlambda0 * (m.U * m.U + m.V * m.V);
+        m.UF = m.U * Fx + m.V * Fy;
+        m.UR = m.U * rgx + m.V * rgy;
+        m.mFx = k.ms * Fx;
+        m.mFy = k.ms * Fy;
+        m.mrx = k.ms * rgx;
+        m.mry = k.ms * rgy;
+        const float r = rho * k.rt;
+        const float cbp = tau * bbp;
 #pragma unroll
-    for (int z = 0; z < NZ; ++z) {
-        const int q = DIR == 0 ? (z == 0 ? 1 : z == 1 ? 2 : z == 2 ? 4 : z == 3 ? 5 : z == 4 ? 6 : 8)
-                               : (z == 0 ? 2 : z == 1 ? 3 : z == 2 ? 4 : z == 3 ? 6 : z == 4 ? 7 : 8);
-        const float en = DIR == 0 ? (float)ex9(q) : (float)ey9(q);
-        hfl[z] = ho[z] * (k.flx * en);
-        ffl[z] = fo[z] * (k.flx * en);
+        for (int t = 0; t < 3; ++t) {
+            m.wphi[t] = k.w[t] * phi;
+            m.wp[t] = k.w[t] * p;
+            m.wr[t] = k.w[t] * r;
+            cw[t] = cbp * k.w[t];
+            chw[t] = k.cbpw_h[t];
+        }
+    } else {
+        m.G0 = m.UF = m.UR = m.mrx = m.mry = 0.f;
+        m.mFx = k.ms * Fx * k.inv_rt;
+        m.mFy = k.ms * Fy * k.inv_rt;
+        const float pr = p * k.inv_rt;
+#pragma unroll
+        for (int t = 0; t < 3; ++t) {
+            m.wphi[t] = k.w[t] * phi;
+            m.wp[t] = k.w[t] * pr;
+            m.wr[t] = 0.f;
+            cw[t] = k.cbpw0[t];
+            chw[t] = k.cbpw_h[t];
+        }
+    }
+    float bh[9], bf[9];
+    {
+        // rest population: u1 = 0, G = G0, no Allen-Cahn source
+        const float heq = m.wphi[0];
+        float feq, fs;
+        if (HCZ) {
+            feq = fmaf(m.wr[0], m.G0, m.wp[0]);
+            fs = -fmaf(m.G0, fmaf(k.rt, m.UR, m.UF), m.UF);
+        } else {
+            feq = (k.w[0] - 1.f) * p * k.inv_rt;
+            fs = 0.f;
+        }
+        bh[0] = fmaf(k.bbp_h, heq - h[0], h[0]);
+        bf[0] = fmaf(bbp, feq - f[0], f[0]) + cw[0] * fs;
+    }
+#define MPD_CELL_PAIR(Q, EX, EY)                                                         \
+    {                                                                                    \
+        constexpr int QP = Q + 4, W = (EX == 0 || EY == 0) ? 1 : 2;                      \
+        float heq, heqp, feq, feqp, fs, fsp, hs;                                         \
+        pair_eq<MODEL, EX, EY>(k, m, heq, heqp, feq, feqp, fs, fsp, hs);                 \
+        bh[Q] = fmaf(k.bbp_h, heq - h[Q], h[Q]) + chw[W] * hs;                           \
+        bh[QP] = fmaf(k.bbp_h, heqp - h[QP], h[QP]) - chw[W] * hs;                       \
+        bf[Q] = fmaf(bbp, feq - f[Q], f[Q]) + cw[W] * fs;                                \
+        bf[QP] = fmaf(bbp, feqp - f[QP], f[QP]) + cw[W] * fsp;                           \
+    }
+    MPD_CELL_PAIR(1, 1, 0)
+    MPD_CELL_PAIR(2, 1, 1)
+    MPD_CELL_PAIR(3, 0, 1)
+    MPD_CELL_PAIR(4, -1, 1)
+#undef MPD_CELL_PAIR
+    const float third = 1.f / 3.f;
+#pragma unroll
+    for (int q = 0; q < 9; ++q) {
+        S[q * PS + c] = bh[q];
+        S[(9 + q) * PS + c] = bf[q];
+        if (q > 0) base[q - 1] = fmaf(third, bh[q] - h[q], bh[q]);
+        base[8 + q] = fmaf(third, bf[q] - f[q], bf[q]);
+    }
+    // face averages need (o + n) / 2: store the halves
+    S[18 * PS + c] = 0.5f * ax;
+    S[19 * PS + c] = 0.5f * ay;
+    S[20 * PS + c] = 0.5f * rgx;
+    S[21 * PS + c] = 0.5f * rgy;
+    S[22 * PS + c] = 0.5f * Fx;
+    S[23 * PS + c] = 0.5f * Fy;
+    if (interior) {
+        if (HCZ || WALLS) st(out.a, n, phi, P_PHL * out.pb);
+        if (HCZ) {
+            st(out.a, n, npu, P_PPU * out.pb);
+            st(out.a, n, npv, P_PPV * out.pb);
+            if (WALLS && (gj == 0 || gj == ny - 1)) {
+                const int nxt = gj == 0 ? c2 + R2X : c2 - R2X;
+                st(out.gl, (uint32_t)((gj == 0 ? 0 : nx) + gi), 2.f * s_phl[c2] - s_phl[nxt]);
+            }
+        }
     }
 }
 
-// One 16x15 tile of one case: stages A-D of the step.
-template <int MODEL, int WALLS>
-__device__ __forceinline__ void tile32(const K32 &k, int nx, int ny, const FusedPlanes32 &in, const FusedPlanes32 &out,
-                                       int64_t N, int i0, int j0)
+// Face reconstruction of one population (CD_Interior_phi_Bh,
+// DUGKS2DSolver.cpp:604-687): 1/2 (o + n) - h e_n (n - o)/dx - h e_t (dt o + dt n)/2
+template <int DIR, int Q>
+__device__ __forceinline__ float recon(const K32 &k, const float *b, int ko)
+{
+    constexpr int DN = DIR == 0 ? 1 : R1X, DT = DIR == 0 ? R1X : 1;
+    constexpr int EN = DIR == 0 ? ex9(Q) : ey9(Q), ET = DIR == 0 ? ey9(Q) : ex9(Q);
+    const float bo = b[ko], bn = b[ko + DN];
+    float v;
+    if (EN > 0) v = fmaf(k.rb, bn, k.ra * bo);
+    else if (EN < 0) v = fmaf(k.ra, bn, k.rb * bo);
+    else v = 0.5f * (bo + bn);
+    if (ET != 0) {
+        const float t = (b[ko + DN + DT] - b[ko + DN - DT]) + (b[ko + DT] - b[ko - DT]);
+        v = ET > 0 ? fmaf(-k.rt4, t, v) : fmaf(k.rt4, t, v);
+    }
+    return v;
+}
+
+// One face: reconstruction, face macros (Update_MacroVar_h, D2Q9HCZ.cpp:183-242,
+// D2Q9AC.cpp:232-329), BGK half step (Update_phi_h, DUGKS2DSolver.cpp:402-423)
+// and micro-flux (Update_phiFlux_h, :424-441) of the six populations with a
+// non-zero normal velocity, in pair order (first member e_n = +1).
+template <int DIR, int MODEL, int WALLS>
+__device__ __forceinline__ void face(const K32 &k, const float *S, int ko, int wall, float *hfl, float *ffl,
+                                     float &mass)
 {
     constexpr bool HCZ = MODEL == MPD_MODEL_AC_HCZ;
-    __shared__ float s_bp[18][R1N];        // bar-plus h(0..8) f(9..17); later y-face fluxes
-    __shared__ float s_mac[6][R1N];        // AC source (2), grad rho (2), F (2)
-    __shared__ float s_u[(2 * NZ + 1) * NXF];  // phi, phi_lag, then x-face fluxes (+ h mass flux)
-    float *s_phi = s_u, *s_phl = s_u + R2N;
-    static_assert(2 * R2N <= 2 * NZ * NXF, "union too small");
-    static_assert((2 * NZ + 1) * NYF <= 18 * R1N, "y-face fluxes do not fit");
+    constexpr int DN = DIR == 0 ? 1 : R1X;
+    float hv[9], fv[9];
+#define MPD_RECON(Q)                                    \
+    hv[Q] = recon<DIR, Q>(k, S + Q * PS, ko);           \
+    fv[Q] = recon<DIR, Q>(k, S + (9 + Q) * PS, ko);
+    MPD_RECON(0) MPD_RECON(1) MPD_RECON(2) MPD_RECON(3) MPD_RECON(4)
+    MPD_RECON(5) MPD_RECON(6) MPD_RECON(7) MPD_RECON(8)
+#undef MPD_RECON
+    const int kn = ko + DN;
+    const float ax = S[18 * PS + ko] + S[18 * PS + kn];
+    const float ay = S[19 * PS + ko] + S[19 * PS + kn];
+    const float rgx = S[20 * PS + ko] + S[20 * PS + kn];
+    const float rgy = S[21 * PS + ko] + S[21 * PS + kn];
+    const float Fx = S[22 * PS + ko] + S[22 * PS + kn];
+    const float Fy = S[23 * PS + ko] + S[23 * PS + kn];
+    const float phi = ((hv[0] + hv[1]) + (hv[2] + hv[3])) + ((hv[4] + hv[5]) + (hv[6] + hv[7])) + hv[8];
+    Site m;
+    m.mAx = k.ms * ax;
+    m.mAy = k.ms * ay;
+    float bh, cw[3];
+    if (HCZ) {
+        const float rho = k.aphi * phi + k.bphi;
+        const float irr = rcp(rho * k.rt);
+        const float sx = (fv[1] + fv[2] + fv[8]) - (fv[4] + fv[5] + fv[6]);
+        const float sy = (fv[2] + fv[3] + fv[4]) - (fv[6] + fv[7] + fv[8]);
+        m.U = fmaf(k.ms, sx, k.hhdt * Fx) * irr;
+        m.V = fmaf(k.ms, sy, k.hhdt * Fy) * irr;
+        const float s = ((fv[0] + fv[1]) + (fv[2] + fv[3])) + ((fv[4] + fv[5]) + (fv[6] + fv[7])) + fv[8];
+        const float p = fmaf(k.hhdt_rt, rgx * m.U + rgy * m.V, s);
+        const float tau = fmaf(phi, k.tau_a, k.tau_b);
+        bh = k.hdt * rcp(2.f * tau + k.hdt);
+        const float ch = tau * bh;
+        m.G0 = k.mlambda0 * (m.U * m.U + m.V * m.V);
+        m.UF = m.U * Fx + m.V * Fy;
+        m.UR = m.U * rgx + m.V * rgy;
+        m.mFx = k.ms * Fx;
+        m.mFy = k.ms * Fy;
+        m.mrx = k.ms * rgx;
+        m.mry = k.ms * rgy;
+        const float r = rho * k.rt;
+#pragma unroll
+        for (int t = 1; t < 3; ++t) {
+            m.wphi[t] = k.w[t] * phi;
+            m.wp[t] = k.w[t] * p;
+            m.wr[t] = k.w[t] * r;
+            cw[t] = ch * k.w[t];
+        }
+    } else {
+        const float p = k.kp * (((fv[1] + fv[2]) + (fv[3] + fv[4])) + ((fv[5] + fv[6]) + (fv[7] + fv[8])));
+        bh = k.bh0;
+        m.U = m.V = m.G0 = m.UF = m.UR = m.mrx = m.mry = 0.f;
+        m.mFx = k.ms * Fx * k.inv_rt;
+        m.mFy = k.ms * Fy * k.inv_rt;
+        const float pr = p * k.inv_rt;
+#pragma unroll
+        for (int t = 1; t < 3; ++t) {
+            m.wphi[t] = k.w[t] * phi;
+            m.wp[t] = k.w[t] * pr;
+            m.wr[t] = 0.f;
+            cw[t] = k.chw0[t];
+        }
+    }
+    // pairs: x-faces (1,5) (2,6) (8,4); y-faces (3,7) (2,6) (4,8)
+    float ho[6], fo[6];
+#define MPD_FACE_PAIR(Z, Q, QP, EX, EY)                                                  \
+    {                                                                                    \
+        constexpr int W = (EX == 0 || EY == 0) ? 1 : 2;                                  \
+        float heq, heqp, feq, feqp, fs, fsp, hs;                                         \
+        pair_eq<MODEL, EX, EY>(k, m, heq, heqp, feq, feqp, fs, fsp, hs);                 \
+        ho[Z] = fmaf(k.bh_h, heq - hv[Q], hv[Q]) + k.chw_h[W] * hs;                      \
+        ho[Z + 1] = fmaf(k.bh_h, heqp - hv[QP], hv[QP]) - k.chw_h[W] * hs;               \
+        fo[Z] = fmaf(bh, feq - fv[Q], fv[Q]) + cw[W] * fs;                               \
+        fo[Z + 1] = fmaf(bh, feqp - fv[QP], fv[QP]) + cw[W] * fsp;                       \
+    }
+    if (DIR == 0) {
+        MPD_FACE_PAIR(0, 1, 5, 1, 0)
+        MPD_FACE_PAIR(2, 2, 6, 1, 1)
+        MPD_FACE_PAIR(4, 8, 4, 1, -1)
+    } else {
+        MPD_FACE_PAIR(0, 3, 7, 0, 1)
+        MPD_FACE_PAIR(2, 2, 6, 1, 1)
+        MPD_FACE_PAIR(4, 4, 8, -1, 1)
+    }
+#undef MPD_FACE_PAIR
+    if (WALLS && DIR == 1 && wall != 0) {
+        // Wall_3_BB (Boundary.cpp:118-139): the populations entering the fluid
+        // (e_y > 0 at the bottom wall, < 0 at the top) take their partner's value
+#pragma unroll
+        for (int z = 0; z < 6; z += 2) {
+            if (wall < 0) { ho[z] = ho[z + 1]; fo[z] = fo[z + 1]; }
+            else { ho[z + 1] = ho[z]; fo[z + 1] = fo[z]; }
+        }
+    }
+    mass = 0.f;
+#pragma unroll
+    for (int z = 0; z < 6; ++z) {
+        const float s = (z & 1) ? -k.flx : k.flx;   // flux = value * MaSpan * area * e_n
+        hfl[z] = ho[z] * s;
+        ffl[z] = fo[z] * s;
+        mass += hfl[z];
+    }
+}
+
+// One 16x15 tile of one case.
+//  A: phi (and lagged phi) with a two-cell halo into LDS;
+//  B: bar-plus and half macros of the 18x17 region (interior cells first, so
+//     each interior thread keeps its own b + (b - x~)/3 in registers);
+//  C: every x- and y-face of the tile (one of each per thread);
+//  D: Update_phi_T, phi moved by the antisymmetric face mass fluxes.
+template <int MODEL, int WALLS>
+__device__ __forceinline__ void tile32(const K32 &k, int nx, int ny, const FusedPlanes32 &pin,
+                                       const FusedPlanes32 &pout, int64_t N, int i0, int j0)
+{
+    constexpr bool HCZ = MODEL == MPD_MODEL_AC_HCZ;
+    __shared__ float s_big[NBP * PS];
+    __shared__ float s_phi[R2N];
+    __shared__ float s_phl[HCZ ? R2N : 1];
     const int tid = threadIdx.x;
+    const bool small = nx < TX + 4 || ny < TY + 4;
+    constexpr int NPL = HCZ ? 22 : (WALLS ? 20 : 19);
+    const Rs in = make_rs(pin, N, nx, NPL), out = make_rs(pout, N, nx, NPL);
 
     // ---- stage A ----
     for (int q = tid; q < R2N; q += NT) {
-        int gi = wrapc(i0 - 2 + q % R2X, nx), gj = j0 - 2 + q / R2X;
+        const int rx = q % R2X, ry = q / R2X;
+        const int gi = wrap(i0 - 2 + rx, nx, small);
+        int gj = j0 - 2 + ry;
         float p = 0.f, pl = 0.f;
         if (WALLS && (gj < 0 || gj >= ny)) {
             if (gj == -1 || gj == ny) {
-                int jn = gj < 0 ? 0 : ny - 1, jnn = gj < 0 ? 1 : ny - 2;
-                p = 2.f * in.phl[(size_t)jn * nx + gi] - in.phl[(size_t)jnn * nx + gi];
-                if (HCZ) pl = in.gl[(gj < 0 ? 0 : nx) + gi];
+                const int jn = gj < 0 ? 0 : ny - 1, jnn = gj < 0 ? 1 : ny - 2;
+                p = 2.f * ld(in.a, (uint32_t)(jn * nx + gi), P_PHL * in.pb) -
+                    ld(in.a, (uint32_t)(jnn * nx + gi), P_PHL * in.pb);
+                if (HCZ) pl = ld(in.gl, (uint32_t)((gj < 0 ? 0 : nx) + gi));
             }
         } else {
-            gj = wrapc(gj, ny);
-            p = in.phi[(size_t)gj * nx + gi];
-            if (HCZ) pl = in.phl[(size_t)gj * nx + gi];
+            gj = WALLS ? gj : wrap(gj, ny, small);
+            const uint32_t n = (uint32_t)(gj * nx + gi);
+            p = ld(in.a, n, P_PHI * in.pb);
+            if (HCZ) pl = ld(in.a, n, P_PHL * in.pb);
         }
         s_phi[q] = p;
         if (HCZ) s_phl[q] = pl;
     }
     __syncthreads();
 
-    // ---- stage B: macros and bar-plus ----
-    for (int c = tid; c < R1N; c += NT) {
-        const int rx = c % R1X, ry = c / R1X;
-        const int gi = wrapc(i0 - 1 + rx, nx);
-        int gj = j0 - 1 + ry;
-        if (WALLS && (gj < 0 || gj >= ny)) continue;
-        gj = wrapc(gj, ny);
-        const size_t n = (size_t)gj * nx + gi;
-        float h[9], f[9];
-#pragma unroll
-        for (int q = 0; q < 9; ++q) {
-            h[q] = in.h[(size_t)q * N + n];
-            f[q] = in.f[(size_t)q * N + n];
-        }
-        const int c2 = (ry + 1) * R2X + (rx + 1);
-        const float phi = s_phi[c2];
-        const float pc[4] = {s_phi[c2 + 1], s_phi[c2 + R2X], s_phi[c2 - 1], s_phi[c2 - R2X]};
-        const float pd[4] = {s_phi[c2 + R2X + 1], s_phi[c2 + R2X - 1], s_phi[c2 - R2X - 1], s_phi[c2 - R2X + 1]};
-        const float rho = k.aphi * phi + k.bphi;
-        float U = 0.f, V = 0.f, p, bbp, cbp;
-        if (HCZ) {
-            // Update_MacroVar with the previous step's sources (D2Q9HCZ.cpp:133-182)
-            const float lc[4] = {s_phl[c2 + 1], s_phl[c2 + R2X], s_phl[c2 - 1], s_phl[c2 - R2X]};
-            const float ld[4] = {s_phl[c2 + R2X + 1], s_phl[c2 + R2X - 1], s_phl[c2 - R2X - 1], s_phl[c2 - R2X + 1]};
-            float lgx, lgy, llap, lFx, lFy, lrx, lry;
-            stencil(k, s_phl[c2], lc, ld, lgx, lgy, llap);
-            momentum_source(k, s_phl[c2], lgx, lgy, llap, lFx, lFy, lrx, lry);
-            const float irr = rcp(rho * k.rt);
-            const float sx = k.ms * ((f[1] + f[2] + f[8]) - (f[4] + f[5] + f[6]));
-            const float sy = k.ms * ((f[2] + f[3] + f[4]) - (f[6] + f[7] + f[8]));
-            U = (sx + k.hdt * lFx) * irr;
-            V = (sy + k.hdt * lFy) * irr;
-            float s = 0.f;
-#pragma unroll
-            for (int q = 0; q < 9; ++q) s += f[q];
-            p = s + k.hdt * k.rt * (lrx * U + lry * V);
-            const float tau = (k.nu_v + (phi - k.phi_v) * k.dnu) * k.inv_rt;
-            bbp = 1.5f * k.dt * rcp(2.f * tau + k.dt);   // 1 - (2 tau - dt/2)/(2 tau + dt)
-            cbp = tau * bbp;
+    // ---- stage B ----
+    const int ci = tid % TX, cj = tid / TX;
+    const bool cell = tid < NIN;
+    const bool inside = cell && i0 + ci < nx && j0 + cj < ny;
+    float base[17];
+    {
+        int rx, ry;
+        if (cell) {
+            rx = ci + 1;
+            ry = cj + 1;
         } else {
-            // AC-p: U = V = 0, p = Kp * sum_{q != 0} f (D2Q9AC.cpp:197-203)
-            float s = 0.f;
-#pragma unroll
-            for (int q = 1; q < 9; ++q) s += f[q];
-            p = k.kp * s;
-            bbp = k.bbp0;
-            cbp = k.cbp0;
+            const int r = tid - NIN;   // ring cells 0 .. NT - NIN - 1 (bottom row)
+            rx = r;
+            ry = 0;
         }
-        // MacroSource (SourceM.cpp:35-62) from the current phi stencil
-        float gx, gy, lap, Fx, Fy, rgx, rgy;
-        stencil(k, phi, pc, pd, gx, gy, lap);
-        momentum_source(k, phi, gx, gy, lap, Fx, Fy, rgx, rgy);
-        float ax, ay, npu = 0.f, npv = 0.f;
-        {
-            const float L2 = gx * gx + gy * gy;
-            float dpu = 0.f, dpv = 0.f;
-            if (HCZ) {
-                npu = phi * U;
-                npv = phi * V;
-                dpu = (npu - in.ppu[n]) * k.inv_rtdt;
-                dpv = (npv - in.ppv[n]) * k.inv_rtdt;
-            }
-            if (L2 != 0.f) {
-                const float t = -4.f * (phi - k.phi_l) * (phi - k.phi_v) * k.inv_wdphi * rsqrtf(L2);
-                ax = gx * t + dpu;
-                ay = gy * t + dpv;
-            } else {
-                ax = dpu;
-                ay = dpv;
-            }
-        }
-        // Update_phi_Eq / Update_DVDF_Source / Update_phi_BP (D2Q9HCZ.cpp:54-92,244-257;
-        // D2Q9AC.cpp:69-105; DUGKS2DSolver.cpp:379-401)
-        const float uu = U * U + V * V;
-        const float UF = U * Fx + V * Fy, UR = U * rgx + V * rgy;
-#pragma unroll
-        for (int q = 0; q < 9; ++q) {
-            const float ex = (float)ex9(q), ey = (float)ey9(q), w = k.w[q];
-            const float eu = ex * U + ey * V;            // e . u (compile-time e)
-            const float eA = k.ms * (ex * ax + ey * ay);  // xi . AC source
-            float heq, feq, fso;
-            if (HCZ) {
-                const float u1 = k.ms_rt * eu;
-                const float G = u1 * (1.f + 0.5f * u1) - uu * k.lambda0;
-                heq = w * (1.f + u1) * phi;
-                feq = w * (p + rho * k.rt * G);
-                const float xF = k.ms * (ex * Fx + ey * Fy) - UF;
-                const float xR = k.ms * (ex * rgx + ey * rgy) - UR;
-                fso = w * ((1.f + G) * xF + G * k.rt * xR);
-            } else {
-                heq = w * phi;
-                feq = (w - (q == 0 ? 1.f : 0.f)) * p * k.inv_rt;
-                fso = w * k.ms * (ex * Fx + ey * Fy) * k.inv_rt;
-            }
-            const float hso = w * eA;
-            s_bp[q][c] = h[q] + k.bbp_h * (heq - h[q]) + k.cbp_h * hso;
-            s_bp[9 + q][c] = f[q] + bbp * (feq - f[q]) + cbp * fso;
-        }
-        s_mac[0][c] = ax; s_mac[1][c] = ay; s_mac[2][c] = rgx;
-        s_mac[3][c] = rgy; s_mac[4][c] = Fx; s_mac[5][c] = Fy;
-        const bool interior = rx >= 1 && rx <= TX && ry >= 1 && ry <= TY && i0 - 1 + rx < nx && j0 - 1 + ry < ny;
-        if (interior && (HCZ || WALLS)) out.phl[n] = phi;
-        if (interior && HCZ) {
-            out.ppu[n] = npu;
-            out.ppv[n] = npv;
-            if (WALLS && (gj == 0 || gj == ny - 1)) {
-                const int nxt = gj == 0 ? c2 + R2X : c2 - R2X;
-                out.gl[(gj == 0 ? 0 : nx) + gi] = 2.f * s_phl[c2] - s_phl[nxt];
-            }
-        }
+        cell_bp<MODEL, WALLS>(k, in, out, nx, ny, small, i0, j0, rx, ry, s_big, s_phi, s_phl, base, inside);
+    }
+    if (tid < NRING - (NT - NIN)) {
+        // remaining ring cells: rest of the bottom row, top row, left and right columns
+        const int r = tid + (NT - NIN);
+        int rx, ry;
+        if (r < R1X) { rx = r; ry = 0; }
+        else if (r < 2 * R1X) { rx = r - R1X; ry = R1Y - 1; }
+        else if (r < 2 * R1X + TY) { rx = 0; ry = r - 2 * R1X + 1; }
+        else { rx = R1X - 1; ry = r - 2 * R1X - TY + 1; }
+        float dummy[17];
+        cell_bp<MODEL, WALLS>(k, in, out, nx, ny, small, i0, j0, rx, ry, s_big, s_phi, s_phl, dummy, false);
     }
     __syncthreads();
 
     if (WALLS) {
-        for (int c = tid; c < R1N; c += NT) {
-            const int rx = c % R1X, ry = c / R1X;
-            const int gj = j0 - 1 + ry;
-            if (gj != -1 && gj != ny) continue;
-            const int dn = gj < 0 ? 1 : -1;
-            const int kn = (ry + dn) * R1X + rx, knn = (ry + 2 * dn) * R1X + rx;
+        // ghost rows (j = -1 below the bottom wall, j = ny above the top wall):
+        // WallShadowC_fBP / ExtrapolationfBP (Boundary.cpp:38-117), 2 near - next
+        const int gy0 = j0 == 0 ? 0 : -1;                       // R1 row of j = -1
+        const int gy1 = ny - j0 + 1 < R1Y ? ny - j0 + 1 : -1;   // R1 row of j = ny
+        const int row = tid < R1X ? gy0 : (tid < 2 * R1X ? gy1 : -1);
+        if (row >= 0) {
+            const int rx = tid < R1X ? tid : tid - R1X;
+            const int dn = row == gy0 && tid < R1X ? 1 : -1;
+            const int c = row * R1X + rx, kn = c + dn * R1X, knn = c + 2 * dn * R1X;
 #pragma unroll
-            for (int q = 0; q < 18; ++q) s_bp[q][c] = 2.f * s_bp[q][kn] - s_bp[q][knn];
-#pragma unroll
-            for (int a = 0; a < 6; ++a) s_mac[a][c] = 2.f * s_mac[a][kn] - s_mac[a][knn];
+            for (int q = 0; q < NBP; ++q) s_big[q * PS + c] = 2.f * s_big[q * PS + kn] - s_big[q * PS + knn];
         }
         __syncthreads();
     }
 
-    // ---- stage C: faces ----
+    // ---- stage C ----
+    float xh[6], xf[6], xm = 0.f, yh[6], yf[6], ym;
     if (tid < NXF) {
         const int fi = tid % (TX + 1), fj = tid / (TX + 1);
-        const int ko = (fj + 1) * R1X + fi;
-        float hfl[NZ], ffl[NZ];
-        face_flux<0, MODEL, WALLS>(k, &s_bp[0][0], &s_mac[0][0], ko, ko + 1, R1X, hfl, ffl, 0);
-        float mass = 0.f;
-#pragma unroll
-        for (int z = 0; z < NZ; ++z) {
-            s_u[z * NXF + tid] = hfl[z];
-            s_u[(NZ + z) * NXF + tid] = ffl[z];
-            mass += hfl[z];
-        }
-        s_u[2 * NZ * NXF + tid] = mass;
+        face<0, MODEL, WALLS>(k, s_big, (fj + 1) * R1X + fi, 0, xh, xf, xm);
     }
-    float yh[NZ], yf[NZ];
     {
         const int fi = tid % TX, fj = tid / TX;
-        const int ko = fj * R1X + fi + 1;
         const int gj = j0 + fj;
         const int wall = WALLS ? (gj == 0 ? -1 : (gj == ny ? 1 : 0)) : 0;
-        face_flux<1, MODEL, WALLS>(k, &s_bp[0][0], &s_mac[0][0], ko, ko + R1X, 1, yh, yf, wall);
-    }
-    const int ci = tid % TX, cj = tid / TX;
-    const bool cell = tid < TX * TY;
-    const int kc = (cj + 1) * R1X + ci + 1;
-    float hbp[9], fbp[9];
-    if (cell) {
-#pragma unroll
-        for (int q = 0; q < 9; ++q) { hbp[q] = s_bp[q][kc]; fbp[q] = s_bp[9 + q][kc]; }
+        face<1, MODEL, WALLS>(k, s_big, fj * R1X + fi + 1, wall, yh, yf, ym);
     }
     __syncthreads();
-    {
-        float mass = 0.f;
+    float *s_fx = s_big, *s_fy = s_big + NFL * FS;
+    if (tid < NXF) {
 #pragma unroll
-        for (int z = 0; z < NZ; ++z) {
-            s_bp[z][tid] = yh[z];
-            s_bp[NZ + z][tid] = yf[z];
-            mass += yh[z];
+        for (int z = 0; z < 6; ++z) {
+            s_fx[z * FS + tid] = xh[z];
+            s_fx[(6 + z) * FS + tid] = xf[z];
         }
-        s_bp[2 * NZ][tid] = mass;
+        s_fx[12 * FS + tid] = xm;
     }
+#pragma unroll
+    for (int z = 0; z < 6; ++z) {
+        s_fy[z * FS + tid] = yh[z];
+        s_fy[(6 + z) * FS + tid] = yf[z];
+    }
+    s_fy[12 * FS + tid] = ym;
     __syncthreads();
 
-    // ---- stage D: Update_phi_T as b + (b - x~)/3 + dt/V * (W - E + S - N) ----
-    const int gi = i0 + ci, gj = j0 + cj;
-    if (!cell || gi >= nx || gj >= ny) return;
-    const size_t n = (size_t)gj * nx + gi;
+    // ---- stage D: x~' = b + (b - x~)/3 + dt/V (W - E + S - N) ----
+    if (!inside) return;
+    const uint32_t n = (uint32_t)((j0 + cj) * nx + (i0 + ci));
     const int kW = cj * (TX + 1) + ci, kE = kW + 1;
     const int kS = cj * TX + ci, kN = kS + TX;
-    float fluxh[9], fluxf[9];
+    // plane z of the x- (y-) face fluxes holds population xp[z] (yp[z])
+    constexpr int xp[6] = {1, 5, 2, 6, 8, 4};
+    constexpr int yp[6] = {3, 7, 2, 6, 4, 8};
+    float dh[9], df[9];
 #pragma unroll
-    for (int q = 0; q < 9; ++q) { fluxh[q] = 0.f; fluxf[q] = 0.f; }
+    for (int q = 0; q < 9; ++q) { dh[q] = 0.f; df[q] = 0.f; }
 #pragma unroll
-    for (int z = 0; z < NZ; ++z) {
-        const int qx = xq(z), qy = yq(z);
-        fluxh[qx] += s_u[z * NXF + kW] - s_u[z * NXF + kE];
-        fluxf[qx] += s_u[(NZ + z) * NXF + kW] - s_u[(NZ + z) * NXF + kE];
-        fluxh[qy] += s_bp[z][kS] - s_bp[z][kN];
-        fluxf[qy] += s_bp[NZ + z][kS] - s_bp[NZ + z][kN];
+    for (int z = 0; z < 6; ++z) {
+        dh[xp[z]] += s_fx[z * FS + kW] - s_fx[z * FS + kE];
+        df[xp[z]] += s_fx[(6 + z) * FS + kW] - s_fx[(6 + z) * FS + kE];
+    }
+#pragma unroll
+    for (int z = 0; z < 6; ++z) {
+        dh[yp[z]] += s_fy[z * FS + kS] - s_fy[z * FS + kN];
+        df[yp[z]] += s_fy[(6 + z) * FS + kS] - s_fy[(6 + z) * FS + kN];
     }
     // phi is the conserved variable: it moves only by the antisymmetric face
     // mass fluxes (each face's sum over populations, shared by both cells), and
     // h~_0 takes the remainder so that sum_q h~ = phi.  In exact arithmetic this
     // is the same update; in FP32 it keeps the rounding of the per-population
     // products from drifting the phase-field mass.
-    const float third = 1.f / 3.f;
-    const float mW = s_u[2 * NZ * NXF + kW], mE = s_u[2 * NZ * NXF + kE];
-    const float mS = s_bp[2 * NZ][kS], mN = s_bp[2 * NZ][kN];
-    const float phi = in.phi[n] + k.dtv * ((mW - mE) + (mS - mN));
+    const float dm = (s_fx[12 * FS + kW] - s_fx[12 * FS + kE]) + (s_fy[12 * FS + kS] - s_fy[12 * FS + kN]);
+    const float phi = fmaf(k.dtv, dm, s_phi[(cj + 2) * R2X + ci + 2]);
     float rest = 0.f;
 #pragma unroll
     for (int q = 1; q < 9; ++q) {
-        const float hn = hbp[q] + third * (hbp[q] - in.h[(size_t)q * N + n]) + k.dtv * fluxh[q];
-        out.h[(size_t)q * N + n] = hn;
+        const float hn = fmaf(k.dtv, dh[q], base[q - 1]);
+        st(out.a, n, hn, (P_H + q) * out.pb);
         rest += hn;
     }
-    out.h[n] = phi - rest;
+    st(out.a, n, phi - rest, P_H * out.pb);
 #pragma unroll
-    for (int q = 0; q < 9; ++q)
-        out.f[(size_t)q * N + n] = fbp[q] + third * (fbp[q] - in.f[(size_t)q * N + n]) + k.dtv * fluxf[q];
-    out.phi[n] = phi;
+    for (int q = 0; q < 9; ++q) st(out.a, n, fmaf(k.dtv, df[q], base[8 + q]), (P_F + q) * out.pb);
+    st(out.a, n, phi, P_PHI * out.pb);
 }
 
 template <int MODEL, int WALLS>
@@ -482,47 +681,61 @@ K32 make_k32(const Consts &C)
 {
     K32 k;
     const double ms = C.xi[0][1];
+    const double dnu = C.nu_l - C.nu_v;
     k.ms = (float)ms;
     k.ms_rt = (float)(ms / C.rt);
     k.rt = (float)C.rt;
     k.inv_rt = (float)(1.0 / C.rt);
-    k.lambda0 = (float)C.lambda0;
+    k.mlambda0 = (float)(-C.lambda0);
     k.hdt = (float)C.hdt;
+    k.hhdt = (float)(0.5 * C.hdt);
+    k.hhdt_rt = (float)(0.5 * C.hdt * C.rt);
     k.dt = (float)C.dt;
+    k.bdt = (float)(1.5 * C.dt);
     k.aphi = (float)C.aphi;
     k.bphi = (float)C.bphi;
     k.beta = (float)C.beta;
     k.kappa = (float)C.kappa;
     k.phi_l = (float)C.phi_l;
     k.phi_v = (float)C.phi_v;
-    k.nu_v = (float)C.nu_v;
-    k.dnu = (float)(C.nu_l - C.nu_v);
+    // tau = (nu_V + (phi - phi_V)(nu_L - nu_V)) / RT (calcMu/calcTau, D2Q9HCZ.cpp:24-44)
+    k.tau_a = (float)(dnu / C.rt);
+    k.tau_b = (float)((C.nu_v - C.phi_v * dnu) / C.rt);
     k.inv_d2x6 = (float)(1.0 / C.d2x6);
     k.inv_lapden = (float)(1.0 / C.lapden);
     k.inv_rtdt = (float)(1.0 / C.rtdt);
     k.inv_wdphi = (float)(1.0 / C.wdphi);
-    k.inv_dx = (float)(1.0 / C.dx);
-    k.inv_2dx = (float)(1.0 / C.twodx);
     k.kp = (float)C.kp;
     k.dtv = (float)C.dtv;
+    const double A = C.hdt * ms / C.dx;  // normal difference factor h e_n / dx
+    k.ra = (float)(0.5 + A);
+    k.rb = (float)(0.5 - A);
+    k.rt4 = (float)(C.hdt * ms / (2.0 * C.twodx));
     k.gx = (float)C.gx;
     k.gy = (float)C.gy;
     k.rho_ref = (float)C.rho_ref;
-    k.bbp_h = (float)(1.5 * C.dt / (2.0 * C.taum + C.dt));
-    k.cbp_h = (float)(C.taum * 1.5 * C.dt / (2.0 * C.taum + C.dt));
-    k.bh_h = (float)(C.hdt / (2.0 * C.taum + C.hdt));
-    k.ch_h = (float)(C.taum * C.hdt / (2.0 * C.taum + C.hdt));
-    k.tau0 = (float)C.tau0;
-    // AC-p: tau = Mu/(rho RT) with Mu = rho RT Tau0 (D2Q9AC.cpp:34-60) -> Tau0
-    k.bbp0 = (float)(1.5 * C.dt / (2.0 * C.tau0 + C.dt));
-    k.cbp0 = (float)(C.tau0 * 1.5 * C.dt / (2.0 * C.tau0 + C.dt));
-    k.bh0 = (float)(C.hdt / (2.0 * C.tau0 + C.hdt));
-    k.ch0 = (float)(C.tau0 * C.hdt / (2.0 * C.tau0 + C.hdt));
-    k.flx = (float)(ms * C.dx);
     // weights rounded to FP32 with w0 = 1 - 4 w1 - 4 w2 so that they sum to 1 exactly
     const float w1 = 1.f / 9.f, w2 = 1.f / 36.f;
-    const float w0 = 1.f - 4.f * w1 - 4.f * w2;
-    for (int q = 0; q < 9; ++q) k.w[q] = q == 0 ? w0 : (q % 2 ? w1 : w2);
+    k.w[0] = 1.f - 4.f * w1 - 4.f * w2;
+    k.w[1] = w1;
+    k.w[2] = w2;
+    const double wd[3] = {4.0 / 9.0, 1.0 / 9.0, 1.0 / 36.0};
+    const double bbp_h = 1.5 * C.dt / (2.0 * C.taum + C.dt), cbp_h = C.taum * bbp_h;
+    const double bh_h = C.hdt / (2.0 * C.taum + C.hdt), ch_h = C.taum * bh_h;
+    // AC-p: tau = Mu/(rho RT) with Mu = rho RT Tau0 (D2Q9AC.cpp:34-60) -> Tau0
+    const double bbp0 = 1.5 * C.dt / (2.0 * C.tau0 + C.dt), cbp0 = C.tau0 * bbp0;
+    const double bh0 = C.hdt / (2.0 * C.tau0 + C.hdt), ch0 = C.tau0 * bh0;
+    k.bbp_h = (float)bbp_h;
+    k.bh_h = (float)bh_h;
+    k.bbp0 = (float)bbp0;
+    k.bh0 = (float)bh0;
+    for (int t = 0; t < 3; ++t) {
+        k.cbpw_h[t] = (float)(cbp_h * wd[t]);
+        k.chw_h[t] = (float)(ch_h * wd[t]);
+        k.cbpw0[t] = (float)(cbp0 * wd[t]);
+        k.chw0[t] = (float)(ch0 * wd[t]);
+    }
+    k.flx = (float)(ms * C.dx);
     return k;
 }
 
