@@ -37,7 +37,8 @@ constexpr int NT = 256;
 constexpr int PS = 320;                    // LDS plane stride in floats: a multiple of 64 dwords, so every
                                            // plane of one site is an immediate offset from one address
 constexpr int FS = 256;                    // LDS stride of the face-flux planes
-constexpr int NBP = 24;                    // 9 h + 9 f bar-plus planes, then 6 half-macro planes
+constexpr int NBP = 24;
+constexpr bool PRELOAD_RING = false;  // also hoist the second-pass loads above the phi staging                    // 9 h + 9 f bar-plus planes, then 6 half-macro planes
 constexpr int NFL = 13;                    // per face direction: 6 h fluxes, 6 f fluxes, h mass flux
 static_assert(R1N <= PS && NXF <= FS && NYF <= FS, "plane strides");
 static_assert(NIN <= NT && NRING - (NT - NIN) <= 64, "stage-B mapping: one partial wave for the ring");
@@ -48,14 +49,13 @@ struct K32 {
     float ms_rt;     // MaSpan / RT
     float rt, inv_rt, mlambda0, hdt, hhdt, hhdt_rt, dt, bdt, aphi, bphi, beta, kappa, phi_l, phi_v;
     float tau_a, tau_b;               // tau = tau_a phi + tau_b (HCZ)
-    float inv_d2x6, inv_lapden, inv_rtdt, inv_wdphi, kp, dtv;
+    float inv_d2x6, inv_lapden, inv_rtdt, inv_wdphi, kp, dtvf;  // dtvf = dt/V * MaSpan * area
     float ra, rb, rt4;                // face reconstruction: (1/2 + A, 1/2 - A), tangential factor
     float gx, gy, rho_ref;
     float bbp_h, bh_h;                // Allen-Cahn factors (constant tau_h)
     float cbpw_h[3], chw_h[3];        // c * w_q for the weight classes 0 / axis / diagonal
     float bbp0, bh0;                  // AC-p constant factors
     float cbpw0[3], chw0[3];
-    float flx;                        // MaSpan * face area
     float w[3];                       // weights rest / axis / diagonal, w0 = 1 - 4 w1 - 4 w2
 };
 
@@ -89,6 +89,16 @@ __device__ __forceinline__ int tile_of_block(int b, int ntiles)
 }
 
 __device__ __forceinline__ float rcp(float x) { return __builtin_amdgcn_rcpf(x); }
+
+// An LDS index the compiler cannot relate to other indices: loads of the 18
+// planes at one site then pair as two-plane reads from one base register
+// (plane stride a multiple of 64 dwords) instead of pairing neighbouring sites
+// of one plane, which needs a base register per plane.
+__device__ __forceinline__ int opaque(int x)
+{
+    asm("" : "+v"(x));
+    return x;
+}
 
 // Buffer resources over the planes of one case: loads and stores are a
 // per-lane 32-bit byte offset plus a uniform plane offset, with no 64-bit
@@ -206,23 +216,44 @@ __device__ __forceinline__ void pair_eq(const K32 &k, const Site &m, float &heq,
 // SourceM.cpp:35-62, DUGKS2DSolver.cpp:379-401) written to LDS at c.  base
 // receives b + (b - x~)/3 of every population (h: q >= 1), the part of
 // Update_phi_T (DUGKS2DSolver.cpp:748-792) that does not involve the fluxes.
+// Populations (and HCZ prevPhiU/V) of one cell of the bar-plus region, loaded
+// ahead of the phi staging so that their latency overlaps it.
+struct CellIn {
+    float h[9], f[9], pu, pv;
+    uint32_t n;
+    int gi, gj;
+    bool valid;
+};
 template <int MODEL, int WALLS>
-__device__ __forceinline__ void cell_bp(const K32 &k, const Rs &in, const Rs &out,
-                                        int nx, int ny, bool small, int i0, int j0, int rx, int ry, float *S,
-                                        const float *s_phi, const float *s_phl, float *base, bool interior)
+__device__ __forceinline__ void cell_load(const Rs &in, int nx, int ny, bool small, int i0, int j0, int rx, int ry,
+                                          CellIn &x)
 {
-    constexpr bool HCZ = MODEL == MPD_MODEL_AC_HCZ;
-    const int gi = wrap(i0 - 1 + rx, nx, small);
+    x.gi = wrap(i0 - 1 + rx, nx, small);
     int gj = j0 - 1 + ry;
-    if (WALLS && (gj < 0 || gj >= ny)) return;  // wall ghost rows: extrapolated afterwards
-    gj = WALLS ? gj : wrap(gj, ny, small);
-    const uint32_t n = (uint32_t)(gj * nx + gi);
-    float h[9], f[9];
+    x.valid = !(WALLS && (gj < 0 || gj >= ny));  // wall ghost rows: extrapolated afterwards
+    x.gj = gj = WALLS ? gj : wrap(gj, ny, small);
+    x.n = (uint32_t)(gj * nx + x.gi);
+    if (!x.valid) return;
 #pragma unroll
     for (int q = 0; q < 9; ++q) {
-        h[q] = ld(in.a, n, (P_H + q) * in.pb);
-        f[q] = ld(in.a, n, (P_F + q) * in.pb);
+        x.h[q] = ld(in.a, x.n, (P_H + q) * in.pb);
+        x.f[q] = ld(in.a, x.n, (P_F + q) * in.pb);
     }
+    if (MODEL == MPD_MODEL_AC_HCZ) {
+        x.pu = ld(in.a, x.n, P_PPU * in.pb);
+        x.pv = ld(in.a, x.n, P_PPV * in.pb);
+    }
+}
+
+template <int MODEL, int WALLS>
+__device__ __forceinline__ void cell_bp(const K32 &k, const Rs &out, int nx, int ny, int rx, int ry, const CellIn &x,
+                                        float *S, const float *s_phi, const float *s_phl, float *base, bool interior)
+{
+    constexpr bool HCZ = MODEL == MPD_MODEL_AC_HCZ;
+    if (!x.valid) return;
+    const float *h = x.h, *f = x.f;
+    const uint32_t n = x.n;
+    const int gi = x.gi, gj = x.gj;
     const int c = ry * R1X + rx;
     const int c2 = (ry + 1) * R2X + (rx + 1);
     const float phi = s_phi[c2];
@@ -257,8 +288,8 @@ __device__ __forceinline__ void cell_bp(const K32 &k, const Rs &in, const Rs &ou
         if (HCZ) {
             npu = phi * m.U;
             npv = phi * m.V;
-            dpu = (npu - ld(in.a, n, P_PPU * in.pb)) * k.inv_rtdt;
-            dpv = (npv - ld(in.a, n, P_PPV * in.pb)) * k.inv_rtdt;
+            dpu = (npu - x.pu) * k.inv_rtdt;
+            dpv = (npv - x.pv) * k.inv_rtdt;
         }
         if (L2 != 0.f) {
             const float t = -4.f * (phi - k.phi_l) * (phi - k.phi_v) * k.inv_wdphi * rsqrtf(L2);
@@ -364,18 +395,20 @@ __device__ __forceinline__ void cell_bp(const K32 &k, const Rs &in, const Rs &ou
 
 // Face reconstruction of one population (CD_Interior_phi_Bh,
 // DUGKS2DSolver.cpp:604-687): 1/2 (o + n) - h e_n (n - o)/dx - h e_t (dt o + dt n)/2
+struct FaceIdx {
+    int o, n, op, om, np, nm;  // owner, neighbour, and their tangential neighbours (+/-)
+};
 template <int DIR, int Q>
-__device__ __forceinline__ float recon(const K32 &k, const float *b, int ko)
+__device__ __forceinline__ float recon(const K32 &k, const float *b, const FaceIdx &x)
 {
-    constexpr int DN = DIR == 0 ? 1 : R1X, DT = DIR == 0 ? R1X : 1;
     constexpr int EN = DIR == 0 ? ex9(Q) : ey9(Q), ET = DIR == 0 ? ey9(Q) : ex9(Q);
-    const float bo = b[ko], bn = b[ko + DN];
+    const float bo = b[x.o], bn = b[x.n];
     float v;
     if (EN > 0) v = fmaf(k.rb, bn, k.ra * bo);
     else if (EN < 0) v = fmaf(k.ra, bn, k.rb * bo);
     else v = 0.5f * (bo + bn);
     if (ET != 0) {
-        const float t = (b[ko + DN + DT] - b[ko + DN - DT]) + (b[ko + DT] - b[ko - DT]);
+        const float t = (b[x.np] - b[x.nm]) + (b[x.op] - b[x.om]);
         v = ET > 0 ? fmaf(-k.rt4, t, v) : fmaf(k.rt4, t, v);
     }
     return v;
@@ -390,15 +423,22 @@ __device__ __forceinline__ void face(const K32 &k, const float *S, int ko, int w
                                      float &mass)
 {
     constexpr bool HCZ = MODEL == MPD_MODEL_AC_HCZ;
-    constexpr int DN = DIR == 0 ? 1 : R1X;
+    constexpr int DN = DIR == 0 ? 1 : R1X, DT = DIR == 0 ? R1X : 1;
+    FaceIdx x;
+    x.o = ko;
+    x.n = opaque(ko + DN);
+    x.op = opaque(ko + DT);
+    x.om = opaque(ko - DT);
+    x.np = opaque(ko + DN + DT);
+    x.nm = opaque(ko + DN - DT);
     float hv[9], fv[9];
 #define MPD_RECON(Q)                                    \
-    hv[Q] = recon<DIR, Q>(k, S + Q * PS, ko);           \
-    fv[Q] = recon<DIR, Q>(k, S + (9 + Q) * PS, ko);
+    hv[Q] = recon<DIR, Q>(k, S + Q * PS, x);            \
+    fv[Q] = recon<DIR, Q>(k, S + (9 + Q) * PS, x);
     MPD_RECON(0) MPD_RECON(1) MPD_RECON(2) MPD_RECON(3) MPD_RECON(4)
     MPD_RECON(5) MPD_RECON(6) MPD_RECON(7) MPD_RECON(8)
 #undef MPD_RECON
-    const int kn = ko + DN;
+    const int kn = x.n;
     const float ax = S[18 * PS + ko] + S[18 * PS + kn];
     const float ay = S[19 * PS + ko] + S[19 * PS + kn];
     const float rgx = S[20 * PS + ko] + S[20 * PS + kn];
@@ -483,13 +523,13 @@ __device__ __forceinline__ void face(const K32 &k, const float *S, int ko, int w
             else { ho[z + 1] = ho[z]; fo[z + 1] = fo[z]; }
         }
     }
-    mass = 0.f;
+    // the flux is value * MaSpan * area * e_n: the factor MaSpan * area is
+    // applied with dt/V in the cell update, the sign e_n there and here
+    mass = ((ho[0] - ho[1]) + (ho[2] - ho[3])) + (ho[4] - ho[5]);
 #pragma unroll
     for (int z = 0; z < 6; ++z) {
-        const float s = (z & 1) ? -k.flx : k.flx;   // flux = value * MaSpan * area * e_n
-        hfl[z] = ho[z] * s;
-        ffl[z] = fo[z] * s;
-        mass += hfl[z];
+        hfl[z] = ho[z];
+        ffl[z] = fo[z];
     }
 }
 
@@ -511,6 +551,26 @@ __device__ __forceinline__ void tile32(const K32 &k, int nx, int ny, const Fused
     const bool small = nx < TX + 4 || ny < TY + 4;
     constexpr int NPL = HCZ ? 22 : (WALLS ? 20 : 19);
     const Rs in = make_rs(pin, N, nx, NPL), out = make_rs(pout, N, nx, NPL);
+
+    // stage-B cells: interior cells first (thread = cell of stage D), then the
+    // ring: rest of the bottom row, top row, left and right columns
+    const int ci = tid % TX, cj = tid / TX;
+    const bool cell = tid < NIN;
+    const bool inside = cell && i0 + ci < nx && j0 + cj < ny;
+    constexpr int NR2 = NRING - (NT - NIN);   // ring cells of the second pass
+    CellIn x1, x2;
+    int rx1, ry1, rx2 = 0, ry2 = 0;
+    if (cell) { rx1 = ci + 1; ry1 = cj + 1; }
+    else { rx1 = tid - NIN; ry1 = 0; }
+    cell_load<MODEL, WALLS>(in, nx, ny, small, i0, j0, rx1, ry1, x1);
+    if (tid < NR2) {
+        const int r = tid + (NT - NIN);
+        if (r < R1X) { rx2 = r; ry2 = 0; }
+        else if (r < 2 * R1X) { rx2 = r - R1X; ry2 = R1Y - 1; }
+        else if (r < 2 * R1X + TY) { rx2 = 0; ry2 = r - 2 * R1X + 1; }
+        else { rx2 = R1X - 1; ry2 = r - 2 * R1X - TY + 1; }
+        if (PRELOAD_RING) cell_load<MODEL, WALLS>(in, nx, ny, small, i0, j0, rx2, ry2, x2);
+    }
 
     // ---- stage A ----
     for (int q = tid; q < R2N; q += NT) {
@@ -537,32 +597,12 @@ __device__ __forceinline__ void tile32(const K32 &k, int nx, int ny, const Fused
     __syncthreads();
 
     // ---- stage B ----
-    const int ci = tid % TX, cj = tid / TX;
-    const bool cell = tid < NIN;
-    const bool inside = cell && i0 + ci < nx && j0 + cj < ny;
     float base[17];
-    {
-        int rx, ry;
-        if (cell) {
-            rx = ci + 1;
-            ry = cj + 1;
-        } else {
-            const int r = tid - NIN;   // ring cells 0 .. NT - NIN - 1 (bottom row)
-            rx = r;
-            ry = 0;
-        }
-        cell_bp<MODEL, WALLS>(k, in, out, nx, ny, small, i0, j0, rx, ry, s_big, s_phi, s_phl, base, inside);
-    }
-    if (tid < NRING - (NT - NIN)) {
-        // remaining ring cells: rest of the bottom row, top row, left and right columns
-        const int r = tid + (NT - NIN);
-        int rx, ry;
-        if (r < R1X) { rx = r; ry = 0; }
-        else if (r < 2 * R1X) { rx = r - R1X; ry = R1Y - 1; }
-        else if (r < 2 * R1X + TY) { rx = 0; ry = r - 2 * R1X + 1; }
-        else { rx = R1X - 1; ry = r - 2 * R1X - TY + 1; }
+    cell_bp<MODEL, WALLS>(k, out, nx, ny, rx1, ry1, x1, s_big, s_phi, s_phl, base, inside);
+    if (tid < NR2) {
+        if (!PRELOAD_RING) cell_load<MODEL, WALLS>(in, nx, ny, small, i0, j0, rx2, ry2, x2);
         float dummy[17];
-        cell_bp<MODEL, WALLS>(k, in, out, nx, ny, small, i0, j0, rx, ry, s_big, s_phi, s_phl, dummy, false);
+        cell_bp<MODEL, WALLS>(k, out, nx, ny, rx2, ry2, x2, s_big, s_phi, s_phl, dummy, false);
     }
     __syncthreads();
 
@@ -615,9 +655,10 @@ __device__ __forceinline__ void tile32(const K32 &k, int nx, int ny, const Fused
     // ---- stage D: x~' = b + (b - x~)/3 + dt/V (W - E + S - N) ----
     if (!inside) return;
     const uint32_t n = (uint32_t)((j0 + cj) * nx + (i0 + ci));
-    const int kW = cj * (TX + 1) + ci, kE = kW + 1;
-    const int kS = cj * TX + ci, kN = kS + TX;
-    // plane z of the x- (y-) face fluxes holds population xp[z] (yp[z])
+    const int kW = cj * (TX + 1) + ci, kE = opaque(kW + 1);
+    const int kS = cj * TX + ci, kN = opaque(kS + TX);
+    // plane z of the x- (y-) face values holds population xp[z] (yp[z]); even
+    // z have e_n = +1, odd z e_n = -1
     constexpr int xp[6] = {1, 5, 2, 6, 8, 4};
     constexpr int yp[6] = {3, 7, 2, 6, 4, 8};
     float dh[9], df[9];
@@ -625,13 +666,15 @@ __device__ __forceinline__ void tile32(const K32 &k, int nx, int ny, const Fused
     for (int q = 0; q < 9; ++q) { dh[q] = 0.f; df[q] = 0.f; }
 #pragma unroll
     for (int z = 0; z < 6; ++z) {
-        dh[xp[z]] += s_fx[z * FS + kW] - s_fx[z * FS + kE];
-        df[xp[z]] += s_fx[(6 + z) * FS + kW] - s_fx[(6 + z) * FS + kE];
+        const int a = (z & 1) ? kE : kW, b = (z & 1) ? kW : kE;
+        dh[xp[z]] += s_fx[z * FS + a] - s_fx[z * FS + b];
+        df[xp[z]] += s_fx[(6 + z) * FS + a] - s_fx[(6 + z) * FS + b];
     }
 #pragma unroll
     for (int z = 0; z < 6; ++z) {
-        dh[yp[z]] += s_fy[z * FS + kS] - s_fy[z * FS + kN];
-        df[yp[z]] += s_fy[(6 + z) * FS + kS] - s_fy[(6 + z) * FS + kN];
+        const int a = (z & 1) ? kN : kS, b = (z & 1) ? kS : kN;
+        dh[yp[z]] += s_fy[z * FS + a] - s_fy[z * FS + b];
+        df[yp[z]] += s_fy[(6 + z) * FS + a] - s_fy[(6 + z) * FS + b];
     }
     // phi is the conserved variable: it moves only by the antisymmetric face
     // mass fluxes (each face's sum over populations, shared by both cells), and
@@ -639,17 +682,17 @@ __device__ __forceinline__ void tile32(const K32 &k, int nx, int ny, const Fused
     // is the same update; in FP32 it keeps the rounding of the per-population
     // products from drifting the phase-field mass.
     const float dm = (s_fx[12 * FS + kW] - s_fx[12 * FS + kE]) + (s_fy[12 * FS + kS] - s_fy[12 * FS + kN]);
-    const float phi = fmaf(k.dtv, dm, s_phi[(cj + 2) * R2X + ci + 2]);
+    const float phi = fmaf(k.dtvf, dm, s_phi[(cj + 2) * R2X + ci + 2]);
     float rest = 0.f;
 #pragma unroll
     for (int q = 1; q < 9; ++q) {
-        const float hn = fmaf(k.dtv, dh[q], base[q - 1]);
+        const float hn = fmaf(k.dtvf, dh[q], base[q - 1]);
         st(out.a, n, hn, (P_H + q) * out.pb);
         rest += hn;
     }
     st(out.a, n, phi - rest, P_H * out.pb);
 #pragma unroll
-    for (int q = 0; q < 9; ++q) st(out.a, n, fmaf(k.dtv, df[q], base[8 + q]), (P_F + q) * out.pb);
+    for (int q = 0; q < 9; ++q) st(out.a, n, fmaf(k.dtvf, df[q], base[8 + q]), (P_F + q) * out.pb);
     st(out.a, n, phi, P_PHI * out.pb);
 }
 
@@ -706,7 +749,7 @@ K32 make_k32(const Consts &C)
     k.inv_rtdt = (float)(1.0 / C.rtdt);
     k.inv_wdphi = (float)(1.0 / C.wdphi);
     k.kp = (float)C.kp;
-    k.dtv = (float)C.dtv;
+    k.dtvf = (float)(C.dtv * ms * C.dx);
     const double A = C.hdt * ms / C.dx;  // normal difference factor h e_n / dx
     k.ra = (float)(0.5 + A);
     k.rb = (float)(0.5 - A);
@@ -735,7 +778,6 @@ K32 make_k32(const Consts &C)
         k.cbpw0[t] = (float)(cbp0 * wd[t]);
         k.chw0[t] = (float)(ch0 * wd[t]);
     }
-    k.flx = (float)(ms * C.dx);
     return k;
 }
 
