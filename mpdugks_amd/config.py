@@ -45,8 +45,8 @@ def default_config(model=MPD_MODEL_AC_HCZ, nx=128, ny=128, **overrides):
     cfg.pp_k = 1.0                          # ZeroNamespace.h:140
     cfg.pp_tau = tau0                       # Preprocess.cpp:376,397
     if model == MPD_MODEL_PP_D2V16:
-        # D2V16 spans c = MaSpan = 1.2 (Ark/Lib/D2V16.h) so MaxU = 2c
-        cfg.dt = cfl * min_l / (2 * 1.2)
+        # MaxU = 2 MaSpan sqrt(2) with MaSpan = 1.2 (Ark/Lib/D2V16.h:16-20)
+        cfg.dt = cfl * min_l / (2 * 1.2 * math.sqrt(2.0))
     for k, v in overrides.items():
         setattr(cfg, k, v)
     return cfg

@@ -11,6 +11,7 @@
 
 #include "../../include/mpdugks.h"
 #include "device_state.h"
+#include "d2v16.h"
 #include "fused.h"
 #include "physics.h"
 
@@ -201,6 +202,9 @@ __global__ void __launch_bounds__(RED_THREADS) k_reduce(const double *m, double 
     if (threadIdx.x < RED_FIELDS) part[(size_t)blockIdx.x * RED_FIELDS + threadIdx.x] = sh[threadIdx.x][0];
 }
 
+int num_q(const mpd_config &c) { return c.model == MPD_MODEL_PP_D2V16 ? d2v16::Q : Q9; }
+bool is_pp(const mpd_config &c) { return c.model == MPD_MODEL_PP_D2Q9 || c.model == MPD_MODEL_PP_D2V16; }
+
 int check_cfg(const mpd_config *c)
 {
     if (!c) return fail(MPD_ERR_INVALID, "null config");
@@ -208,7 +212,8 @@ int check_cfg(const mpd_config *c)
     if ((int64_t)c->nx * c->ny > (int64_t)1 << 30) return fail(MPD_ERR_INVALID, "mesh too large");
     if (c->model < MPD_MODEL_AC_P || c->model > MPD_MODEL_PP_D2V16)
         return fail(MPD_ERR_INVALID, "unknown model %d", c->model);
-    if (c->model == MPD_MODEL_PP_D2V16) return fail(MPD_ERR_INVALID, "D2V16 is served by mpd_d2v16_* entry points");
+    if (c->model == MPD_MODEL_PP_D2V16 && (c->flux != MPD_FLUX_CD || c->bc_y != MPD_BC_PERIODIC))
+        return fail(MPD_ERR_INVALID, "D2V16 runs with CD faces on a periodic mesh");
     if (c->flux != MPD_FLUX_CD && c->flux != MPD_FLUX_UW) return fail(MPD_ERR_INVALID, "unknown flux %d", c->flux);
     if (c->bc_y != MPD_BC_PERIODIC && c->bc_y != MPD_BC_WALL_BB) return fail(MPD_ERR_INVALID, "unknown bc %d", c->bc_y);
     if (c->bc_y == MPD_BC_WALL_BB && (c->flux == MPD_FLUX_UW || c->model == MPD_MODEL_PP_D2Q9))
@@ -259,7 +264,7 @@ int mpd_default_config(int32_t model, mpd_config *c)
     c->pp_tr = 0.85;
     c->pp_k = 1.0;
     c->pp_tau = tau0;
-    if (model == MPD_MODEL_PP_D2V16) c->dt = cfl * min_l / (2 * 1.2);
+    if (model == MPD_MODEL_PP_D2V16) c->dt = cfl * min_l / d2v16::MAXU;  // MaxU = 2 MaSpan sqrt(2), D2V16.h
     return 0;
 }
 
@@ -282,11 +287,11 @@ int mpd_create(const mpd_config *cfg, mpd_handle **out)
         mpd_destroy(h);
         return r;
     }
-    const size_t N = (size_t)cfg->nx * cfg->ny, QN = (size_t)Q9 * N;
+    const size_t N = (size_t)cfg->nx * cfg->ny, QN = (size_t)num_q(*cfg) * N;
     const size_t yfn = (size_t)h->C.nyf * cfg->nx;
     DevState &S = h->S;
     S.N = (int64_t)N;
-    bool ac = cfg->model != MPD_MODEL_PP_D2Q9, pp = !ac, uw = cfg->flux == MPD_FLUX_UW;
+    bool pp = is_pp(*cfg), ac = !pp, uw = cfg->flux == MPD_FLUX_UW;
     if ((r = dalloc(h, &S.h, ac ? QN : 0)) || (r = dalloc(h, &S.f, QN)) ||
         (r = dalloc(h, &S.m, (size_t)MPD_M_COUNT * N)) || (r = dalloc(h, &S.ffx, pp ? 2 * N : 0)) ||
         (r = dalloc(h, &S.ffy, pp ? 2 * N : 0)) || (r = dalloc(h, &S.rho1k, N))) {
@@ -337,14 +342,14 @@ static int alloc_phase_scratch(mpd_handle *h)
 {
     DevState &S = h->S;
     if (S.fbp) return 0;
-    const size_t N = S.N, QN = (size_t)Q9 * N, yfn = (size_t)h->C.nyf * h->cfg.nx;
-    bool ac = h->cfg.model != MPD_MODEL_PP_D2Q9, pp = !ac, uw = h->cfg.flux == MPD_FLUX_UW;
+    const size_t Q = (size_t)num_q(h->cfg), N = S.N, QN = Q * N, yfn = (size_t)h->C.nyf * h->cfg.nx;
+    bool pp = is_pp(h->cfg), ac = !pp, uw = h->cfg.flux == MPD_FLUX_UW;
     int r;
     if ((r = dalloc(h, &S.hbp, ac ? QN : 0)) || (r = dalloc(h, &S.fbp, QN)) ||
         (r = dalloc(h, &S.hbpx, ac && uw ? QN : 0)) || (r = dalloc(h, &S.hbpy, ac && uw ? QN : 0)) ||
         (r = dalloc(h, &S.fbpx, uw ? QN : 0)) || (r = dalloc(h, &S.fbpy, uw ? QN : 0)) ||
-        (r = dalloc(h, &S.hfl, ac ? QN + (size_t)Q9 * yfn : 0)) ||
-        (r = dalloc(h, &S.ffl, QN + (size_t)Q9 * yfn)) ||
+        (r = dalloc(h, &S.hfl, ac ? QN + Q * yfn : 0)) ||
+        (r = dalloc(h, &S.ffl, QN + Q * yfn)) ||
         (r = dalloc(h, &S.gm, h->C.walls ? (size_t)MPD_M_COUNT * 2 * h->cfg.nx : 0)) ||
         (r = dalloc(h, &S.ghbp, h->C.walls ? (size_t)Q9 * 2 * h->cfg.nx : 0)) ||
         (r = dalloc(h, &S.gfbp, h->C.walls ? (size_t)Q9 * 2 * h->cfg.nx : 0)) ||
@@ -358,9 +363,9 @@ int mpd_upload_state(mpd_handle *h, const double *tilde_h, const double *tilde_f
 {
     if (!h) return fail(MPD_ERR_INVALID, "null handle");
     if (!tilde_f || !macro) return fail(MPD_ERR_INVALID, "tilde_f and macro are required");
-    bool ac = h->cfg.model != MPD_MODEL_PP_D2Q9;
+    bool ac = !is_pp(h->cfg);
     if (ac && !tilde_h) return fail(MPD_ERR_INVALID, "tilde_h is required for the Allen-Cahn models");
-    const size_t N = h->S.N, QN = (size_t)Q9 * N;
+    const size_t N = h->S.N, QN = (size_t)num_q(h->cfg) * N;
     if (ac) HIPCHK(hipMemcpyAsync(h->S.h, tilde_h, QN * sizeof(double), hipMemcpyHostToDevice, h->stream));
     HIPCHK(hipMemcpyAsync(h->S.f, tilde_f, QN * sizeof(double), hipMemcpyHostToDevice, h->stream));
     HIPCHK(hipMemcpyAsync(h->S.m, macro, (size_t)MPD_M_COUNT * N * sizeof(double), hipMemcpyHostToDevice,
@@ -397,8 +402,8 @@ int mpd_download_state(mpd_handle *h, double *tilde_h, double *tilde_f, double *
     if (h->batch) HIPCHK(hipStreamSynchronize(h->batch->stream));
     if (!h->uploaded) return fail(MPD_ERR_STATE, "no state uploaded");
     if (int r = sync_macros(h)) return r;
-    const size_t N = h->S.N, QN = (size_t)Q9 * N;
-    bool ac = h->cfg.model != MPD_MODEL_PP_D2Q9;
+    const size_t N = h->S.N, QN = (size_t)num_q(h->cfg) * N;
+    bool ac = !is_pp(h->cfg);
     if (tilde_h && ac) HIPCHK(hipMemcpyAsync(tilde_h, h->S.h, QN * sizeof(double), hipMemcpyDeviceToHost, h->stream));
     if (tilde_f) HIPCHK(hipMemcpyAsync(tilde_f, h->S.f, QN * sizeof(double), hipMemcpyDeviceToHost, h->stream));
     if (macro)

@@ -7,6 +7,7 @@
 #include <random>
 
 #include "../../include/mpdugks.h"
+#include "d2v16.h"
 
 namespace {
 
@@ -15,11 +16,26 @@ const double W9[9] = {4.0 / 9.0, 1.0 / 9.0, 1.0 / 36.0, 1.0 / 9.0, 1.0 / 36.0,
 const int EX[9] = {0, 1, 1, 0, -1, -1, -1, 0, 1};
 const int EY[9] = {0, 0, 1, 1, 1, 0, -1, -1, -1};
 
+constexpr mpd::d2v16::Table kV16 = mpd::d2v16::make_table();
+
+int num_q(const mpd_config &c) { return c.model == MPD_MODEL_PP_D2V16 ? mpd::d2v16::Q : 9; }
+
 // Update_phi_Eq with zero sources for one cell (HCZ Ark/Lib/D2Q9HCZ.cpp:54-92,
-// AC-p Ark/Lib/D2Q9AC.cpp:69-105, PP Ark/Lib/D2Q9PP.cpp:51-62).
+// AC-p Ark/Lib/D2Q9AC.cpp:69-105, PP Ark/Lib/D2Q9PP.cpp:51-62; D2V16
+// Ark/Lib/D2V16.cpp:157-166 with Macro16 at the reference temperature).
 void cell_eq(const mpd_config &c, double phi, double rho, double U, double V, double p, double *heq,
              double *feq)
 {
+    if (c.model == MPD_MODEL_PP_D2V16) {
+        double M[16];
+        mpd::d2v16::macro16<double>(M, rho, U, V, c.rt, rho * c.rt);
+        for (int j = 0; j < 16; ++j) {
+            double s = 0;
+            for (int k = 0; k < 16; ++k) s += kV16.a[j][k] * M[k];
+            feq[j] = s;
+        }
+        return;
+    }
     const double rt = c.rt, lambda0 = 1 / (2.0 * rt), maspan = std::sqrt(3.0 * rt);
     double uu = U * U + V * V;
     for (int q = 0; q < 9; ++q) {
@@ -39,7 +55,7 @@ void cell_eq(const mpd_config &c, double phi, double rho, double U, double V, do
 int valid(const mpd_config *c)
 {
     return c && c->nx > 0 && c->ny > 0 && c->rt > 0 && c->model >= MPD_MODEL_AC_P &&
-           c->model <= MPD_MODEL_PP_D2Q9;
+           c->model <= MPD_MODEL_PP_D2V16;
 }
 
 }  // namespace
@@ -54,7 +70,7 @@ extern "C" {
 int mpd_init_ac_drop(const mpd_config *cfg, double cx, double cy, double radius, int32_t sign,
                      double p0, double mu0, double *tilde_h, double *tilde_f, double *macro)
 {
-    if (!valid(cfg) || cfg->model == MPD_MODEL_PP_D2Q9 || !tilde_h || !tilde_f || !macro) return MPD_ERR_INVALID;
+    if (!valid(cfg) || cfg->model >= MPD_MODEL_PP_D2Q9 || !tilde_h || !tilde_f || !macro) return MPD_ERR_INVALID;
     const mpd_config &c = *cfg;
     const size_t N = (size_t)c.nx * c.ny;
     const double aphi = (c.rho_l - c.rho_v) / (c.phi_l - c.phi_v), bphi = c.rho_v - aphi * c.phi_v;
@@ -90,7 +106,7 @@ int mpd_init_ac_drop(const mpd_config *cfg, double cx, double cy, double radius,
 int mpd_init_ac_interface(const mpd_config *cfg, const double *y0, int32_t sign, double p0, double *tilde_h,
                           double *tilde_f, double *macro)
 {
-    if (!valid(cfg) || cfg->model == MPD_MODEL_PP_D2Q9 || !y0 || !tilde_h || !tilde_f || !macro)
+    if (!valid(cfg) || cfg->model >= MPD_MODEL_PP_D2Q9 || !y0 || !tilde_h || !tilde_f || !macro)
         return MPD_ERR_INVALID;
     const mpd_config &c = *cfg;
     const size_t N = (size_t)c.nx * c.ny;
@@ -126,11 +142,12 @@ int mpd_init_ac_interface(const mpd_config *cfg, const double *y0, int32_t sign,
 int mpd_init_pp_drop(const mpd_config *cfg, double cx, double cy, double radius, double rho_l, double rho_v,
                      double width, double mu0, double *tilde_f, double *macro, double *face_fx, double *face_fy)
 {
-    if (!valid(cfg) || cfg->model != MPD_MODEL_PP_D2Q9 || !tilde_f || !macro) return MPD_ERR_INVALID;
+    if (!valid(cfg) || cfg->model < MPD_MODEL_PP_D2Q9 || !tilde_f || !macro) return MPD_ERR_INVALID;
     const mpd_config &c = *cfg;
     const size_t N = (size_t)c.nx * c.ny;
     std::memset(macro, 0, sizeof(double) * MPD_M_COUNT * N);
-    double feq[9], heq[9];
+    double feq[16], heq[16];
+    const int nq = num_q(c);
     for (int j = 0; j < c.ny; ++j)
         for (int i = 0; i < c.nx; ++i) {
             size_t n = (size_t)j * c.nx + i;
@@ -142,7 +159,7 @@ int mpd_init_pp_drop(const mpd_config *cfg, double cx, double cy, double radius,
             macro[MPD_M_MU * N + n] = mu0;
             macro[MPD_M_TAU * N + n] = c.pp_tau;
             cell_eq(c, 0.0, rho, 0.0, 0.0, rho * c.rt, heq, feq);
-            for (int q = 0; q < 9; ++q) tilde_f[q * N + n] = feq[q];
+            for (int q = 0; q < nq; ++q) tilde_f[q * N + n] = feq[q];
         }
     if (face_fx) std::memset(face_fx, 0, sizeof(double) * 2 * N);
     if (face_fy) std::memset(face_fy, 0, sizeof(double) * 2 * N);
@@ -154,13 +171,14 @@ int mpd_init_pp_drop(const mpd_config *cfg, double cx, double cy, double radius,
 int mpd_init_pp_noise(const mpd_config *cfg, double rho0, double amp, uint64_t seed, double mu0,
                       double *tilde_f, double *macro, double *face_fx, double *face_fy)
 {
-    if (!valid(cfg) || cfg->model != MPD_MODEL_PP_D2Q9 || !tilde_f || !macro) return MPD_ERR_INVALID;
+    if (!valid(cfg) || cfg->model < MPD_MODEL_PP_D2Q9 || !tilde_f || !macro) return MPD_ERR_INVALID;
     const mpd_config &c = *cfg;
     const size_t N = (size_t)c.nx * c.ny;
     std::memset(macro, 0, sizeof(double) * MPD_M_COUNT * N);
     std::mt19937_64 gen(seed);
     std::uniform_real_distribution<double> U(-amp, amp);
-    double feq[9], heq[9];
+    double feq[16], heq[16];
+    const int nq = num_q(c);
     for (size_t n = 0; n < N; ++n) {
         double rho = rho0 * (1.0 + U(gen));
         macro[MPD_M_RHO * N + n] = rho;
@@ -168,7 +186,7 @@ int mpd_init_pp_noise(const mpd_config *cfg, double rho0, double amp, uint64_t s
         macro[MPD_M_MU * N + n] = mu0;
         macro[MPD_M_TAU * N + n] = c.pp_tau;
         cell_eq(c, 0.0, rho, 0.0, 0.0, rho * c.rt, heq, feq);
-        for (int q = 0; q < 9; ++q) tilde_f[q * N + n] = feq[q];
+        for (int q = 0; q < nq; ++q) tilde_f[q * N + n] = feq[q];
     }
     if (face_fx) std::memset(face_fx, 0, sizeof(double) * 2 * N);
     if (face_fy) std::memset(face_fy, 0, sizeof(double) * 2 * N);

@@ -4,6 +4,7 @@
 // -ffp-contract=off so every phase reproduces the reference's FP64 results.
 //
 // This is the plain drop-in and the correctness anchor for the fused path.
+#include "d2v16.h"
 #include "device_state.h"
 #include "physics.h"
 
@@ -396,6 +397,188 @@ __global__ void __launch_bounds__(256) k_ppforce(DevState S, Consts C)
     S.m[MPD_M_FY * N + n] = gy;
 }
 
+// ---------------- D2V16 pseudopotential (FP64, CD faces, periodic) ----------------
+// The D2V16 set (Ark/Lib/D2V16.cpp:11-75) with the pseudopotential force of the
+// D2Q9PP model: the phases of the D2Q9PP kernels above with sixteen populations
+// and the moment-matrix equilibrium; isothermal at the reference temperature
+// (p = rho R0 T0 in Macro16).
+
+constexpr d2v16::Table kV16 = d2v16::make_table();
+
+// f_eq_j = sum_k iArkC[j][k] M16_k (Update_phi_Eq, Ark/Lib/D2V16.cpp:157-166)
+__device__ inline void feq_v16(const Consts &C, double rho, double U, double V, double *feq)
+{
+    double M[d2v16::Q];
+    d2v16::macro16<double>(M, rho, U, V, C.rt, rho * C.rt);
+#pragma unroll
+    for (int j = 0; j < d2v16::Q; ++j) {
+        double s = 0;
+#pragma unroll
+        for (int k = 0; k < d2v16::Q; ++k) s += kV16.a[j][k] * M[k];
+        feq[j] = s;
+    }
+}
+
+// phase 1: equilibria, He-Shan-Doolen source, bar-plus (DUGKS2DSolver.cpp:379-401)
+__global__ void __launch_bounds__(256) k_v16_phase1(DevState S, Consts C)
+{
+    size_t n = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= (size_t)S.N) return;
+    const size_t N = S.N;
+    const double *m = S.m;
+    const double rho = m[MPD_M_RHO * N + n], U = m[MPD_M_U * N + n], V = m[MPD_M_V * N + n];
+    const double Fx = m[MPD_M_FX * N + n], Fy = m[MPD_M_FY * N + n], tau = m[MPD_M_TAU * N + n];
+    const double abp = (2.0 * tau - C.hdt) / (2.0 * tau + C.dt);
+    const double bbp = 1.0 - abp;
+    const double cbp = tau * bbp;
+    double feq[d2v16::Q];
+    feq_v16(C, rho, U, V, feq);
+#pragma unroll
+    for (int q = 0; q < d2v16::Q; ++q) {
+        const double ex = kV16.x[q], ey = kV16.y[q];
+        double fso = (Fx * (ex - U) + Fy * (ey - V)) * feq[q] / (rho * C.rt);
+        double b = abp * S.f[q * N + n] + bbp * feq[q];
+        b += cbp * fso;
+        S.fbp[q * N + n] = b;
+    }
+}
+
+// CD face value (CD_Interior_phi_Bh, DUGKS2DSolver.cpp:604-687) of population q
+__device__ inline double face_value_v16(const DevState &S, const Consts &C, int dir, int q, int oi, int oj, int ni,
+                                        int nj)
+{
+    const double *bp = S.fbp + (size_t)q * S.N;
+    auto B = [&](int a, int b) { return bp[(size_t)wrapi(b, C.ny) * C.nx + wrapi(a, C.nx)]; };
+    const double bo = B(oi, oj), bn = B(ni, nj);
+    const double dn = (bn - bo) / C.dx;
+    const double ex = kV16.x[q], ey = kV16.y[q];
+    if (dir == 0) {
+        double dt = (0.5 * (B(ni, nj + 1) - B(ni, nj - 1)) + 0.5 * (B(oi, oj + 1) - B(oi, oj - 1))) / C.twodx;
+        return 0.5 * (bo + bn) - C.hdt * (dn * ex + dt * ey);
+    }
+    double dt = (0.5 * (B(ni + 1, nj) - B(ni - 1, nj)) + 0.5 * (B(oi + 1, oj) - B(oi - 1, oj))) / C.twodx;
+    return 0.5 * (bo + bn) - C.hdt * (dt * ex + dn * ey);
+}
+
+// faces, pass 1: face macros and Psi of every face (as k_pp_face_psi)
+__global__ void __launch_bounds__(256) k_v16_face_psi(DevState S, Consts C)
+{
+    size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= 2 * (size_t)S.N) return;
+    int dir = t < (size_t)S.N ? 0 : 1;
+    size_t fidx = dir == 0 ? t : t - S.N;
+    int i = (int)(fidx % C.nx), j = (int)(fidx / C.nx);
+    int oi, oj, ni, nj;
+    double xf, yf;
+    face_geom(C, dir, i, j, oi, oj, ni, nj, xf, yf);
+    double fb[d2v16::Q], s = 0, sx = 0, sy = 0;
+#pragma unroll
+    for (int q = 0; q < d2v16::Q; ++q) fb[q] = face_value_v16(S, C, dir, q, oi, oj, ni, nj);
+#pragma unroll
+    for (int q = 0; q < d2v16::Q; ++q) s += fb[q];
+#pragma unroll
+    for (int q = 0; q < d2v16::Q; ++q) sx += fb[q] * kV16.x[q];
+#pragma unroll
+    for (int q = 0; q < d2v16::Q; ++q) sy += fb[q] * kV16.y[q];
+    double rho = s;
+    double U = sx / rho, V = sy / rho;
+    U += 0.5 * C.hdt * S.ffx[t] / rho;
+    V += 0.5 * C.hdt * S.ffy[t] / rho;
+    double p = C.pp_k * carnahan_starling(rho, C.t_cs);
+    double scg, psi;
+    calc_psi(p, rho, C.rt, scg, psi);
+    const size_t F2 = 2 * (size_t)S.N;
+    S.fpsi[t] = psi;
+    S.fscr[0 * F2 + t] = rho;
+    S.fscr[1 * F2 + t] = U;
+    S.fscr[2 * F2 + t] = V;
+    S.fscr[3 * F2 + t] = scg;
+}
+
+// faces, pass 2: face force from the face-lattice gradient of Psi, BGK half
+// step (Update_phi_h, DUGKS2DSolver.cpp:402-423) and flux (:424-441)
+__global__ void __launch_bounds__(256) k_v16_face_flux(DevState S, Consts C)
+{
+    size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= 2 * (size_t)S.N) return;
+    int dir = t < (size_t)S.N ? 0 : 1;
+    size_t fidx = dir == 0 ? t : t - S.N;
+    int i = (int)(fidx % C.nx), j = (int)(fidx / C.nx);
+    const double *ps = S.fpsi + (size_t)dir * S.N;
+    auto P = [&](int a, int b) { return ps[(size_t)wrapi(b, C.ny) * C.nx + wrapi(a, C.nx)]; };
+    double c[4] = {P(i + 1, j), P(i, j + 1), P(i - 1, j), P(i, j - 1)};
+    double d[4] = {P(i + 1, j + 1), P(i - 1, j + 1), P(i - 1, j - 1), P(i + 1, j - 1)};
+    double gx, gy;
+    grad9(C, c, d, gx, gy);
+    const size_t F2 = 2 * (size_t)S.N;
+    const double rho = S.fscr[0 * F2 + t], U = S.fscr[1 * F2 + t], V = S.fscr[2 * F2 + t];
+    const double scg = S.fscr[3 * F2 + t], psi = S.fpsi[t];
+    gx *= -scg * psi;
+    gy *= -scg * psi;
+    S.ffx[t] = gx;
+    S.ffy[t] = gy;
+    int oi, oj, ni, nj;
+    double xf, yf;
+    face_geom(C, dir, i, j, oi, oj, ni, nj, xf, yf);
+    double feq[d2v16::Q];
+    feq_v16(C, rho, U, V, feq);
+    double *ffl = S.ffl + (dir ? (size_t)d2v16::Q * S.N : 0);
+#pragma unroll
+    for (int q = 0; q < d2v16::Q; ++q) {
+        const double ex = kV16.x[q], ey = kV16.y[q];
+        double fb = face_value_v16(S, C, dir, q, oi, oj, ni, nj);
+        double fso = (gx * (ex - U) + gy * (ey - V)) * feq[q] / (rho * C.rt);
+        double r = C.ah_pp * fb + C.bh_pp * feq[q];
+        r += C.ch_pp * fso;
+        double xin = C.dx * (dir == 0 ? ex : ey);
+        ffl[(size_t)q * S.N + fidx] = r * xin;
+    }
+}
+
+// Update_phi_T (faces E, N, W, S) and the pseudopotential cell macros
+// (Ark/Lib/D2Q9PP.cpp:89-109 with sixteen populations; Update_PseudoPsi)
+__global__ void __launch_bounds__(256) k_v16_update(DevState S, Consts C)
+{
+    size_t n = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= (size_t)S.N) return;
+    const size_t N = S.N;
+    int i = (int)(n % C.nx), j = (int)(n / C.nx);
+    size_t kE = (size_t)j * C.nx + wrapi(i + 1, C.nx), kW = n;
+    size_t kN = (size_t)wrapi(j + 1, C.ny) * C.nx + i, kS = n;
+    const double aTP = 4.0 / 3.0, bTP = 1.0 / 3.0;
+    const double *fx = S.ffl, *fy = S.ffl + (size_t)d2v16::Q * N;
+    double ft[d2v16::Q];
+#pragma unroll
+    for (int q = 0; q < d2v16::Q; ++q) {
+        double s = 0.0;
+        s += -1 * fx[q * N + kE];
+        s += -1 * fy[q * N + kN];
+        s += 1 * fx[q * N + kW];
+        s += 1 * fy[q * N + kS];
+        ft[q] = aTP * S.fbp[q * N + n] - bTP * S.f[q * N + n] + C.dtv * s;
+        S.f[q * N + n] = ft[q];
+    }
+    double s = 0, sx = 0, sy = 0;
+#pragma unroll
+    for (int q = 0; q < d2v16::Q; ++q) s += ft[q];
+#pragma unroll
+    for (int q = 0; q < d2v16::Q; ++q) sx += ft[q] * kV16.x[q];
+#pragma unroll
+    for (int q = 0; q < d2v16::Q; ++q) sy += ft[q] * kV16.y[q];
+    double *m = S.m;
+    double rho = s, U = sx / rho, V = sy / rho;
+    U += C.hdt * m[MPD_M_FX * N + n] / rho;
+    V += C.hdt * m[MPD_M_FY * N + n] / rho;
+    double p = C.pp_k * carnahan_starling(rho, C.t_cs), scg, psi;
+    calc_psi(p, rho, C.rt, scg, psi);
+    m[MPD_M_RHO * N + n] = rho;
+    m[MPD_M_U * N + n] = U;
+    m[MPD_M_V * N + n] = V;
+    m[MPD_M_P * N + n] = p;
+    m[MPD_M_SCG * N + n] = scg;
+    m[MPD_M_PSI * N + n] = psi;
+}
+
 static inline unsigned nblk(size_t n) { return (unsigned)((n + 255) / 256); }
 
 template <int MODEL, int FLUX>
@@ -420,8 +603,21 @@ static void phase_step_t(const DevState &S, const Consts &C, hipStream_t st)
     else hipLaunchKernelGGL(k_source, dim3(nblk(S.N)), dim3(256), 0, st, S, C);
 }
 
+static void phase_step_v16(const DevState &S, const Consts &C, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_v16_phase1, dim3(nblk(S.N)), dim3(256), 0, st, S, C);
+    hipLaunchKernelGGL(k_v16_face_psi, dim3(nblk(2 * S.N)), dim3(256), 0, st, S, C);
+    hipLaunchKernelGGL(k_v16_face_flux, dim3(nblk(2 * S.N)), dim3(256), 0, st, S, C);
+    hipLaunchKernelGGL(k_v16_update, dim3(nblk(S.N)), dim3(256), 0, st, S, C);
+    hipLaunchKernelGGL(k_ppforce, dim3(nblk(S.N)), dim3(256), 0, st, S, C);
+}
+
 void phase_step(const DevState &S, const Consts &C, hipStream_t st)
 {
+    if (C.model == MPD_MODEL_PP_D2V16) {
+        phase_step_v16(S, C, st);
+        return;
+    }
     switch (C.model * 2 + C.flux) {
     case MPD_MODEL_AC_P * 2 + MPD_FLUX_CD: phase_step_t<MPD_MODEL_AC_P, MPD_FLUX_CD>(S, C, st); break;
     case MPD_MODEL_AC_P * 2 + MPD_FLUX_UW: phase_step_t<MPD_MODEL_AC_P, MPD_FLUX_UW>(S, C, st); break;

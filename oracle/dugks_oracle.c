@@ -39,6 +39,7 @@
 #include <string.h>
 
 #define Q9 9
+#define QMAX 16
 
 /* D2Q9 lattice, Ark/Lib/D2Q9.h:24-28 and weights Ark/Lib/D2Q9HCZ.cpp:15-19 */
 static const double E9X[Q9] = {0, 1, 1, 0, -1, -1, -1, 0, 1};
@@ -51,7 +52,9 @@ struct ora {
     mpd_config c;
     int nx, ny, N, walls;
     /* derived constants (formulas as in Ark/Constant/ZeroNamespace.h) */
-    double hdt, lambda0, aphi, bphi, beta, kappa, kp, taum, xi[2][Q9], dtv;
+    int Q, pp;                /* populations per cell; pseudopotential model (D2Q9 or D2V16) */
+    double hdt, lambda0, aphi, bphi, beta, kappa, kp, taum, xi[2][QMAX], dtv;
+    double v16a[QMAX][QMAX];  /* D2V16 equilibrium matrix (iArkC) */
     double ah_h, bh_h, ch_h, abp_h, bbp_h, cbp_h; /* h factors: constant tau_h */
     double ah_pp, bh_pp, ch_pp;                    /* PP face factors (never updated) */
     /* state */
@@ -114,14 +117,126 @@ static void calc_psi(double p, double rho, double rt, double *scg, double *psi)
     *psi = sqrt(2 * (p - rho * rt) / s);
 }
 
+/* D2V16 (Ark/Lib/D2V16.h:14-20, Ark/Lib/D2V16.cpp:11-75): c = MaSpan = 1.2,
+ * eta = (3 + nK) c, nK = 0.  Velocities in four shells of four (axis c,
+ * diagonal c, axis 2c, diagonal 2c); row j of the equilibrium matrix iArkC from
+ * the closed-form coefficients of D2V16.cpp:17-35, written per shell with the
+ * direction signs (sx, sy) of velocity j.  e[j] = |xi|^2 (+ eta^2, first shell). */
+static void d2v16_tables(double a[QMAX][QMAX], double *xi_x, double *xi_y, double *e)
+{
+    const double c = 1.2, et = (3.0 + 0) * c;
+    const double c2 = c * c, e2 = et * et, c2e2 = c2 * e2;
+    const double A = 1.0 / (4.0 * e2), B = (-21.0 * c2 - e2) / (48.0 * c2e2);
+    const double D = (7.0 * c2 + e2) / (32.0 * c2e2), E = (-3.0 * c2 - e2) / (96.0 * c2e2);
+    const double F = 1.0 / (3.0 * c2);
+    const double G = (3.0 * c2 - 5.0 * e2) / (4.0 * e2 * (e2 - 3.0 * c2));
+    const double H = 3.0 * (c2 + e2) / (4.0 * e2 * (e2 - 3.0 * c2));
+    const double I = (21.0 * c2 - 11.0 * e2) / (48.0 * c2e2);
+    const double J = (21.0 * c2 * c2 - 32.0 * c2e2 + 11.0 * e2 * e2) / (32.0 * c2e2 * (e2 - 3.0 * c2));
+    const double K = (21.0 * c2 * c2 - 36.0 * c2e2 + 7.0 * e2 * e2) / (32.0 * c2e2 * (e2 - 3.0 * c2));
+    const double L = (3.0 * c2 - 5.0 * e2) / (96.0 * c2e2);
+    const double Nn = 1.0 / (2.0 * c * e2), P = (3.0 * c2 - e2) / (12.0 * c2e2 * c);
+    const double Qq = (c2 + e2) / (16.0 * c2e2 * c), R = (c2 - e2) / (16.0 * c2e2 * c);
+    const double S = 1.0 / (24.0 * c2 * c2), T = 1.0 / (4.0 * c2 * (e2 - 3.0 * c2));
+    const double U = (c2 - e2) / (32.0 * c2 * c2 * (e2 - 3.0 * c2));
+    const double V = (5.0 * c2 - e2) / (32.0 * c2 * c2 * (e2 - 3.0 * c2));
+    /* direction signs: axis shells E N W S, diagonal shells NE NW SW SE */
+    static const int AX[4] = {1, 0, -1, 0}, AY[4] = {0, 1, 0, -1};
+    static const int DX[4] = {1, -1, -1, 1}, DY[4] = {1, 1, -1, -1};
+    for (int j = 0; j < 16; ++j) {
+        int sh = j / 4, r = j % 4;
+        double x = sh % 2 == 0 ? AX[r] : DX[r], y = sh % 2 == 0 ? AY[r] : DY[r];
+        double xx = x * x, yy = y * y, sp = sh < 2 ? c : 2 * c;
+        double *w = a[j];
+        if (xi_x) xi_x[j] = sp * x;
+        if (xi_y) xi_y[j] = sp * y;
+        if (e) e[j] = sh == 0 ? c2 + e2 : sh == 1 ? 2 * c2 : sh == 2 ? 4 * c2 : 8 * c2;
+        for (int m = 0; m < 16; ++m) w[m] = 0.0;
+        if (sh == 0) {
+            w[3] = A; w[5] = xx * G + yy * H; w[6] = xx * H + yy * G;
+            w[7] = x * Nn; w[8] = y * Nn; w[9] = -x * Nn; w[10] = -y * Nn; w[11] = -y * Nn; w[12] = -x * Nn;
+            w[14] = (xx - yy) * T; w[15] = (yy - xx) * T;
+        } else if (sh == 1) {
+            w[0] = 2.0 / 3.0; w[1] = x * (1.0 / (3.0 * c)); w[2] = y * (1.0 / (3.0 * c)); w[3] = B;
+            w[4] = x * y * F; w[5] = I; w[6] = I; w[7] = -x * (Nn / 2); w[8] = -y * (Nn / 2);
+            w[9] = x * P; w[10] = y * P; w[11] = y * (Nn / 2); w[12] = x * (Nn / 2);
+            w[13] = -x * y * S; w[14] = S / 2; w[15] = S / 2;
+        } else if (sh == 2) {
+            w[0] = -0.5; w[3] = D; w[5] = xx * J + yy * K; w[6] = xx * K + yy * J;
+            w[7] = -x * (Nn / 8); w[8] = -y * (Nn / 8); w[9] = x * Qq; w[10] = y * Qq;
+            w[11] = y * R; w[12] = x * R; w[14] = xx * U + yy * V; w[15] = xx * V + yy * U;
+        } else {
+            w[0] = 1.0 / 12; w[1] = -x * (1.0 / (24 * c)); w[2] = -y * (1.0 / (24 * c)); w[3] = E;
+            w[4] = -x * y * (F / 16); w[5] = L; w[6] = L; w[7] = x * (Nn / 16); w[8] = y * (Nn / 16);
+            w[9] = -x * (P / 8); w[10] = -y * (P / 8); w[11] = -y * (R / 2); w[12] = -x * (R / 2);
+            w[13] = x * y * (S / 4); w[14] = S / 4; w[15] = S / 4;
+        }
+    }
+}
+
+/* test access to the D2V16 tables */
+void ora_d2v16_tables(double *a, double *xi_x, double *xi_y, double *e)
+{
+    double t[QMAX][QMAX];
+    d2v16_tables(t, xi_x, xi_y, e);
+    if (a) memcpy(a, t, sizeof(t));
+}
+
+/* Pseudopotential equilibrium of population q.  D2Q9: Ark/Lib/D2Q9PP.cpp:51-62.
+ * D2V16: f_eq_q = sum_k iArkC[q][k] M16_k (Update_phi_Eq, D2V16.cpp:157-166)
+ * with Macro16 (:125-145) at the reference temperature, p = rho R0 T0. */
+static double pp_feq(const ora *o, int q, double rho, double U, double V)
+{
+    const double rt = o->c.rt;
+    if (o->Q == Q9) {
+        double ex = o->xi[0][q], ey = o->xi[1][q], uu = U * U + V * V;
+        double u1 = (ex * U + ey * V) / rt;
+        return W9[q] * rho * (1.0 + u1 + 0.5 * u1 * u1 - uu * o->lambda0);
+    }
+    double M[16], u2 = U * U, v2 = V * V, p = rho * rt;
+    const double deg = 3.0 + 0, degp2 = deg + 2.0;
+    M[0] = rho;
+    M[1] = rho * U;
+    M[2] = rho * V;
+    M[3] = rho * (deg * rt + u2 + v2);
+    M[4] = M[1] * V;
+    M[5] = rho * u2 + p;
+    M[6] = rho * v2 + p;
+    M[7] = M[1] * (degp2 * rt + u2 + v2);
+    M[8] = M[2] * (degp2 * rt + u2 + v2);
+    M[9] = M[1] * (3.0 * rt + u2);
+    M[10] = M[2] * (3.0 * rt + v2);
+    M[11] = M[2] * (rt + u2);
+    M[12] = M[1] * (rt + v2);
+    M[13] = ((deg + 4.0) * rt + u2 + v2) * M[4];
+    M[14] = degp2 * rho * rt * rt + p * ((deg + 5.0) * u2 + v2) + rho * u2 * (u2 + v2);
+    M[15] = degp2 * rho * rt * rt + p * ((deg + 5.0) * v2 + u2) + rho * v2 * (u2 + v2);
+    double sum = 0;
+    for (int k = 0; k < 16; ++k) sum += o->v16a[q][k] * M[k];
+    return sum;
+}
+
+/* test access: D2V16 equilibrium at (rho, U, V) and temperature rt */
+void ora_d2v16_feq(double rho, double U, double V, double rt, double *feq)
+{
+    ora o;
+    memset(&o, 0, sizeof(o));
+    o.Q = 16;
+    o.c.rt = rt;
+    d2v16_tables(o.v16a, o.xi[0], o.xi[1], NULL);
+    for (int q = 0; q < 16; ++q) feq[q] = pp_feq(&o, q, rho, U, V);
+}
+
 /* ---------------- construction ---------------- */
 
 ora *ora_create(const mpd_config *cfg)
 {
     if (!cfg || cfg->nx < 3 || cfg->ny < 3) return NULL;
     if (cfg->model != MPD_MODEL_AC_P && cfg->model != MPD_MODEL_AC_HCZ &&
-        cfg->model != MPD_MODEL_PP_D2Q9)
+        cfg->model != MPD_MODEL_PP_D2Q9 && cfg->model != MPD_MODEL_PP_D2V16)
         return NULL;
+    if (cfg->model == MPD_MODEL_PP_D2V16 && (cfg->flux != MPD_FLUX_CD || cfg->bc_y != MPD_BC_PERIODIC))
+        return NULL; /* D2V16: CD faces on a periodic mesh */
     if (cfg->bc_y == MPD_BC_WALL_BB && (cfg->flux == MPD_FLUX_UW || cfg->model == MPD_MODEL_PP_D2Q9))
         return NULL; /* the reference has no working wall path for these */
     ora *o = (ora *)calloc(1, sizeof(ora));
@@ -140,10 +255,17 @@ ora *ora_create(const mpd_config *cfg)
     o->kappa = 3.0 * c->sigma * c->width / 2;
     o->kp = c->rt * 9 / 5.0;
     o->taum = c->mobility / c->rt;
-    double maspan = sqrt(3.0 * c->rt);
-    for (int q = 0; q < Q9; ++q) {
-        o->xi[0][q] = maspan * E9X[q];
-        o->xi[1][q] = maspan * E9Y[q];
+    o->pp = c->model == MPD_MODEL_PP_D2Q9 || c->model == MPD_MODEL_PP_D2V16;
+    if (c->model == MPD_MODEL_PP_D2V16) {
+        o->Q = 16;
+        d2v16_tables(o->v16a, o->xi[0], o->xi[1], NULL);
+    } else {
+        o->Q = Q9;
+        double maspan = sqrt(3.0 * c->rt);
+        for (int q = 0; q < Q9; ++q) {
+            o->xi[0][q] = maspan * E9X[q];
+            o->xi[1][q] = maspan * E9Y[q];
+        }
     }
     o->dtv = c->dt / (c->dx * c->dx);
     /* Face_2D::DVDF::setxh / Cell_2D::DVDF::setxBP with tau_h = TauMass,
@@ -157,7 +279,7 @@ ora *ora_create(const mpd_config *cfg)
     o->ah_pp = 2.0 * c->pp_tau / (2.0 * c->pp_tau + o->hdt);
     o->bh_pp = 1.0 - o->ah_pp;
     o->ch_pp = c->pp_tau * o->bh_pp;
-    size_t N = o->N, QN = (size_t)Q9 * N;
+    size_t N = o->N, QN = (size_t)o->Q * N;
     o->h = calloc(QN, sizeof(double));
     o->f = calloc(QN, sizeof(double));
     o->m = calloc((size_t)MPD_M_COUNT * N, sizeof(double));
@@ -170,8 +292,8 @@ ora *ora_create(const mpd_config *cfg)
     o->hbpy = calloc(QN, sizeof(double));
     o->fbpx = calloc(QN, sizeof(double));
     o->fbpy = calloc(QN, sizeof(double));
-    o->hfl = calloc(QN + (size_t)Q9 * o->nyf * o->nx, sizeof(double));
-    o->ffl = calloc(QN + (size_t)Q9 * o->nyf * o->nx, sizeof(double));
+    o->hfl = calloc(QN + (size_t)o->Q * o->nyf * o->nx, sizeof(double));
+    o->ffl = calloc(QN + (size_t)o->Q * o->nyf * o->nx, sizeof(double));
     o->gm = calloc((size_t)MPD_M_COUNT * 2 * o->nx, sizeof(double));
     o->ghbp = calloc((size_t)Q9 * 2 * o->nx, sizeof(double));
     o->gfbp = calloc((size_t)Q9 * 2 * o->nx, sizeof(double));
@@ -190,7 +312,7 @@ void ora_destroy(ora *o)
 int ora_set_state(ora *o, const double *h, const double *f, const double *m,
                   const double *face_fx, const double *face_fy)
 {
-    size_t N = o->N, QN = (size_t)Q9 * N;
+    size_t N = o->N, QN = (size_t)o->Q * N;
     if (h) memcpy(o->h, h, QN * sizeof(double));
     if (f) memcpy(o->f, f, QN * sizeof(double));
     if (m) memcpy(o->m, m, (size_t)MPD_M_COUNT * N * sizeof(double));
@@ -202,7 +324,7 @@ int ora_set_state(ora *o, const double *h, const double *f, const double *m,
 int ora_get_state(const ora *o, double *h, double *f, double *m, double *face_fx,
                   double *face_fy)
 {
-    size_t N = o->N, QN = (size_t)Q9 * N;
+    size_t N = o->N, QN = (size_t)o->Q * N;
     if (h) memcpy(h, o->h, QN * sizeof(double));
     if (f) memcpy(f, o->f, QN * sizeof(double));
     if (m) memcpy(m, o->m, (size_t)MPD_M_COUNT * N * sizeof(double));
@@ -233,13 +355,12 @@ static void phase1_cell(ora *o, size_t n)
     double cbp = tau * bbp;
     int model = o->c.model;
     double uu = U * U + V * V;
-    for (int q = 0; q < Q9; ++q) {
-        double ex = o->xi[0][q], ey = o->xi[1][q], w = W9[q];
+    for (int q = 0; q < o->Q; ++q) {
+        double ex = o->xi[0][q], ey = o->xi[1][q], w = o->pp ? 0.0 : W9[q];
         size_t k = (size_t)q * N + n;
         double feq, fso;
-        if (model == MPD_MODEL_PP_D2Q9) {
-            double u1 = (ex * U + ey * V) / rt;
-            feq = w * rho * (1.0 + u1 + 0.5 * u1 * u1 - uu * o->lambda0);
+        if (o->pp) {
+            feq = pp_feq(o, q, rho, U, V);
             fso = (Fx * (ex - U) + Fy * (ey - V)) * feq / (rho * rt);
         } else {
             double u1 = (ex * U + ey * V) / rt;
@@ -373,11 +494,12 @@ static void face_macros(const ora *o, const face_t *F, const double *hb, const d
 {
     const double rt = o->c.rt, hdt = o->hdt;
     int model = o->c.model;
-    if (model == MPD_MODEL_PP_D2Q9) {
+    (void)model;
+    if (o->pp) {
         double s = 0, sx = 0, sy = 0;
-        for (int q = 0; q < Q9; ++q) s += fb[q];
-        for (int q = 0; q < Q9; ++q) sx += fb[q] * o->xi[0][q];
-        for (int q = 0; q < Q9; ++q) sy += fb[q] * o->xi[1][q];
+        for (int q = 0; q < o->Q; ++q) s += fb[q];
+        for (int q = 0; q < o->Q; ++q) sx += fb[q] * o->xi[0][q];
+        for (int q = 0; q < o->Q; ++q) sy += fb[q] * o->xi[1][q];
         fm->rho = s;
         fm->U = sx / fm->rho;
         fm->V = sy / fm->rho;
@@ -431,12 +553,12 @@ static void face_bgk(const ora *o, const face_macro *fm, int q, double hb, doubl
                      double ah, double bh, double ch, double *hout, double *fout)
 {
     const double rt = o->c.rt;
-    double ex = o->xi[0][q], ey = o->xi[1][q], w = W9[q];
+    double ex = o->xi[0][q], ey = o->xi[1][q], w = o->pp ? 0.0 : W9[q];
     double U = fm->U, V = fm->V, uu = U * U + V * V;
     double u1 = (ex * U + ey * V) / rt;
     double feq, fso;
-    if (o->c.model == MPD_MODEL_PP_D2Q9) {
-        feq = w * fm->rho * (1.0 + u1 + 0.5 * u1 * u1 - uu * o->lambda0);
+    if (o->pp) {
+        feq = pp_feq(o, q, fm->rho, U, V);
         fso = (fm->Fx * (ex - U) + fm->Fy * (ey - V)) * feq / (fm->rho * rt);
         *hout = 0;
     } else {
@@ -480,8 +602,8 @@ static inline double face_psi(const ora *o, int dir, int i, int j)
 static void phase4_faces(ora *o)
 {
     int nx = o->nx, ny = o->ny, N = o->N;
-    int pp = o->c.model == MPD_MODEL_PP_D2Q9;
-    double hb[Q9], fb[Q9];
+    int pp = o->pp, Q = o->Q;
+    double hb[QMAX], fb[QMAX];
     face_macro *fms = NULL;
     if (pp) fms = (face_macro *)malloc(sizeof(face_macro) * 2 * (size_t)N);
     /* PP, first pass: face macros and Psi of every face (Jacobi order). */
@@ -491,7 +613,7 @@ static void phase4_faces(ora *o)
                 for (int i = 0; i < nx; ++i) {
                     face_t F;
                     make_face(o, dir, i, j, &F);
-                    for (int q = 0; q < Q9; ++q)
+                    for (int q = 0; q < Q; ++q)
                         fb[q] = face_recon(o, o->fbp, o->gfbp, o->fbpx, o->fbpy, &F, q);
                     size_t fi = (size_t)dir * N + (size_t)j * nx + i;
                     face_macro *fm = &fms[fi];
@@ -523,14 +645,14 @@ static void phase4_faces(ora *o)
     }
     for (int dir = 0; dir < 2; ++dir) {
         int rows = dir == 0 ? ny : o->nyf;
-        double *hfl = o->hfl + (dir ? (size_t)Q9 * N : 0);
-        double *ffl = o->ffl + (dir ? (size_t)Q9 * N : 0);
+        double *hfl = o->hfl + (dir ? (size_t)Q * N : 0);
+        double *ffl = o->ffl + (dir ? (size_t)Q * N : 0);
         size_t plane = dir == 0 ? (size_t)N : (size_t)o->nyf * nx;
         for (int j = 0; j < rows; ++j)
             for (int i = 0; i < nx; ++i) {
                 face_t F;
                 make_face(o, dir, i, j, &F);
-                for (int q = 0; q < Q9; ++q) {
+                for (int q = 0; q < Q; ++q) {
                     if (!pp) hb[q] = face_recon(o, o->hbp, o->ghbp, o->hbpx, o->hbpy, &F, q);
                     fb[q] = face_recon(o, o->fbp, o->gfbp, o->fbpx, o->fbpy, &F, q);
                 }
@@ -546,8 +668,8 @@ static void phase4_faces(ora *o)
                     bh = 1.0 - ah;
                     ch = fm->tau * bh;
                 }
-                double hh[Q9], ff[Q9];
-                for (int q = 0; q < Q9; ++q)
+                double hh[QMAX], ff[QMAX];
+                for (int q = 0; q < Q; ++q)
                     face_bgk(o, fm, q, pp ? 0 : hb[q], fb[q], ah, bh, ch, &hh[q], &ff[q]);
                 /* Wall_3_BB (Ark/Cpp/Boundary.cpp:118-139): populations entering the
                  * fluid take the value of their bounce-back partner. */
@@ -564,7 +686,7 @@ static void phase4_faces(ora *o)
                 }
                 /* Update_phiFlux_h (:424-441): times xi.n dS (setXiDotdS,
                  * Ark/Cpp/Mesh/MeshConstruct.cpp:289-297), unit normal +x / +y */
-                for (int q = 0; q < Q9; ++q) {
+                for (int q = 0; q < Q; ++q) {
                     double xin = o->c.dx * (dir == 0 ? o->xi[0][q] : o->xi[1][q]);
                     size_t k = (size_t)q * plane + (size_t)j * nx + i;
                     hfl[k] = hh[q] * xin;
@@ -586,8 +708,8 @@ static void phase8_update(ora *o)
     int nx = o->nx, ny = o->ny, N = o->N;
     const double aTP = 4.0 / 3.0, bTP = 1.0 / 3.0;
     size_t yplane = (size_t)o->nyf * nx;
-    int pp = o->c.model == MPD_MODEL_PP_D2Q9;
-    for (int q = 0; q < Q9; ++q)
+    int pp = o->pp, Q = o->Q;
+    for (int q = 0; q < Q; ++q)
         for (int j = 0; j < ny; ++j)
             for (int i = 0; i < nx; ++i) {
                 size_t k = (size_t)q * N + (size_t)j * nx + i;
@@ -596,8 +718,8 @@ static void phase8_update(ora *o)
                 int jn = o->walls ? j + 1 : wrap(j + 1, ny);
                 size_t kN = (size_t)q * yplane + (size_t)jn * nx + i;
                 size_t kS = (size_t)q * yplane + (size_t)j * nx + i;
-                const double *hx = o->hfl, *hy = o->hfl + (size_t)Q9 * N;
-                const double *fx = o->ffl, *fy = o->ffl + (size_t)Q9 * N;
+                const double *hx = o->hfl, *hy = o->hfl + (size_t)Q * N;
+                const double *fx = o->ffl, *fy = o->ffl + (size_t)Q * N;
                 if (!pp) {
                     double s = 0.0;
                     s += -1 * hx[kE];
@@ -627,10 +749,10 @@ static void phase9_macros(ora *o)
     double *m = o->m;
     for (size_t n = 0; n < N; ++n) {
         double s = 0, sx = 0, sy = 0;
-        if (o->c.model == MPD_MODEL_PP_D2Q9) {
-            for (int q = 0; q < Q9; ++q) s += o->f[q * N + n];
-            for (int q = 0; q < Q9; ++q) sx += o->f[q * N + n] * o->xi[0][q];
-            for (int q = 0; q < Q9; ++q) sy += o->f[q * N + n] * o->xi[1][q];
+        if (o->pp) {
+            for (int q = 0; q < o->Q; ++q) s += o->f[q * N + n];
+            for (int q = 0; q < o->Q; ++q) sx += o->f[q * N + n] * o->xi[0][q];
+            for (int q = 0; q < o->Q; ++q) sy += o->f[q * N + n] * o->xi[1][q];
             double rho = s;
             double U = sx / rho, V = sy / rho;
             U += hdt * m[MPD_M_FX * N + n] / rho;
@@ -777,13 +899,13 @@ int ora_step(ora *o, int nsteps)
         for (size_t n = 0; n < (size_t)o->N; ++n) phase1_cell(o, n);
         if (o->walls) phase2_walls(o);
         if (o->c.flux == MPD_FLUX_UW) {
-            if (o->c.model != MPD_MODEL_PP_D2Q9) phase3_grad(o, o->hbp, o->hbpx, o->hbpy);
+            if (!o->pp) phase3_grad(o, o->hbp, o->hbpx, o->hbpy);
             phase3_grad(o, o->fbp, o->fbpx, o->fbpy);
         }
         phase4_faces(o);
         phase8_update(o);
         phase9_macros(o);
-        if (o->c.model == MPD_MODEL_PP_D2Q9) phase11_ppforce(o);
+        if (o->pp) phase11_ppforce(o);
         else phase10_source(o);
     }
     return 0;
@@ -818,7 +940,7 @@ double ora_wall_flux_residual(const ora *o)
 int ora_cell_phases(ora *o)
 {
     phase9_macros(o);
-    if (o->c.model == MPD_MODEL_PP_D2Q9) phase11_ppforce(o);
+    if (o->pp) phase11_ppforce(o);
     else phase10_source(o);
     return 0;
 }

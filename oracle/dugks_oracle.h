@@ -16,6 +16,8 @@ int ora_get_state(const ora *o, double *h, double *f, double *m, double *face_fx
 int ora_step(ora *o, int nsteps);
 double ora_wall_flux_residual(const ora *o);
 int ora_cell_phases(ora *o);
+void ora_d2v16_tables(double *a, double *xi_x, double *xi_y, double *e);
+void ora_d2v16_feq(double rho, double U, double V, double rt, double *feq);
 #ifdef __cplusplus
 }
 #endif

@@ -36,6 +36,10 @@ def lib():
         _lib.ora_wall_flux_residual.restype = C.c_double
         _lib.ora_cell_phases.argtypes = [C.c_void_p]
         _lib.ora_cell_phases.restype = C.c_int
+        _lib.ora_d2v16_tables.argtypes = [C.c_void_p] * 4
+        _lib.ora_d2v16_tables.restype = None
+        _lib.ora_d2v16_feq.argtypes = [C.c_double] * 4 + [C.c_void_p]
+        _lib.ora_d2v16_feq.restype = None
     return _lib
 
 
@@ -66,8 +70,9 @@ class Oracle:
         lib().ora_set_state(self._h, *[_ptr(a) for a in arrs])
 
     def get_state(self):
-        N = self.N
-        h = np.zeros((9, N)); f = np.zeros((9, N)); m = np.zeros((self.nm, N))
+        from mpdugks_amd.abi import num_q
+        N, Q = self.N, num_q(self.cfg.model)
+        h = np.zeros((Q, N)); f = np.zeros((Q, N)); m = np.zeros((self.nm, N))
         ffx = np.zeros(2 * N); ffy = np.zeros(2 * N)
         lib().ora_get_state(self._h, _ptr(h), _ptr(f), _ptr(m), _ptr(ffx), _ptr(ffy))
         return h, f, m, ffx, ffy
@@ -81,3 +86,18 @@ class Oracle:
 
     def wall_flux_residual(self):
         return lib().ora_wall_flux_residual(self._h)
+
+
+def d2v16_tables():
+    """(A, xi_x, xi_y, E) of the oracle's D2V16 set: A[j][k] maps the 16
+    equilibrium moments to populations, E_j = |xi_j|^2 (+ eta^2 on shell 0)."""
+    a = np.zeros((16, 16)); x = np.zeros(16); y = np.zeros(16); e = np.zeros(16)
+    lib().ora_d2v16_tables(_ptr(a), _ptr(x), _ptr(y), _ptr(e))
+    return a, x, y, e
+
+
+def d2v16_feq(rho, u, v, rt):
+    """The oracle's D2V16 equilibrium populations at (rho, u, v), temperature rt."""
+    f = np.zeros(16)
+    lib().ora_d2v16_feq(rho, u, v, rt, _ptr(f))
+    return f
