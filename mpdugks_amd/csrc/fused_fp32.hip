@@ -146,13 +146,50 @@ __device__ __forceinline__ float edot(float x, float y)
     return B > 0 ? y - x : -(x + y);
 }
 
-// grad(phi), laplacian, SetZero (GradScheme.cpp:347-376) from the phi region s
-// around index c (row stride R2X)
-__device__ __forceinline__ void stencil(const K32 &k, const float *s, int c, float &gx, float &gy, float &lap)
+// LDS layout of the bar-plus region (R1X x R1Y cells): interior columns
+// 1..TX at row pitch TX, so that the two 16-lane rows of a half-wave fall on
+// disjoint banks, then the halo columns 0 and TX+1 (two per row).  The phi
+// region (R2X x R2Y) likewise: columns 2..TX+1 at pitch TX, then the four halo
+// columns of each row.  Column info of one column: index = row * pitch + off.
+struct Col {
+    int pitch, off;
+};
+__device__ __forceinline__ Col col1(int c)
 {
-    const float e = s[c + 1], nn = s[c + R2X], w = s[c - 1], so = s[c - R2X];
-    const float ne = s[c + R2X + 1], nw = s[c + R2X - 1], sw = s[c - R2X - 1], se = s[c - R2X + 1];
-    const float phi = s[c];
+    const bool in = c >= 1 && c <= TX;
+    return Col{in ? TX : 2, in ? c - 1 : TX * R1Y + (c > TX ? 1 : 0)};
+}
+__device__ __forceinline__ Col col2(int c)
+{
+    const bool in = c >= 2 && c <= TX + 1;
+    return Col{in ? TX : 4, in ? c - 2 : TX * R2Y + (c < 2 ? c : c - TX)};
+}
+__device__ __forceinline__ int at(Col c, int row) { return row * c.pitch + c.off; }
+__device__ __forceinline__ int r1(int col, int row) { return at(col1(col), row); }
+
+// grad(phi), laplacian, SetZero (GradScheme.cpp:347-376) from the phi region s
+// at the nine indices of a 3x3 stencil (ix: -1, 0, +1 columns; rows likewise)
+struct Stencil {
+    int w[3], c[3], e[3];  // rows row-1, row, row+1
+};
+__device__ __forceinline__ Stencil stencil_idx(int col, int row)
+{
+    const Col a = col2(col - 1), b = col2(col), d = col2(col + 1);
+    Stencil t;
+#pragma unroll
+    for (int r = 0; r < 3; ++r) {
+        t.w[r] = at(a, row - 1 + r);
+        t.c[r] = at(b, row - 1 + r);
+        t.e[r] = at(d, row - 1 + r);
+    }
+    return t;
+}
+__device__ __forceinline__ void stencil(const K32 &k, const float *s, const Stencil &t, float &gx, float &gy,
+                                        float &lap)
+{
+    const float e = s[t.e[1]], nn = s[t.c[2]], w = s[t.w[1]], so = s[t.c[0]];
+    const float ne = s[t.e[2]], nw = s[t.w[2]], sw = s[t.w[0]], se = s[t.e[0]];
+    const float phi = s[t.c[1]];
     gx = (4.f * (e - w) + ((ne - nw) + (se - sw))) * k.inv_d2x6;
     gy = (4.f * (nn - so) + ((ne - se) + (nw - sw))) * k.inv_d2x6;
     lap = (4.f * ((e + nn) + (w + so)) + ((ne + nw) + (sw + se)) - 20.f * phi) * k.inv_lapden;
@@ -247,24 +284,26 @@ __device__ __forceinline__ void cell_load(const Rs &in, int nx, int ny, bool sma
 
 template <int MODEL, int WALLS>
 __device__ __forceinline__ void cell_bp(const K32 &k, const Rs &out, int nx, int ny, int rx, int ry, const CellIn &x,
-                                        float *S, const float *s_phi, const float *s_phl, float *base, bool interior)
+                                        float *S, const float *s_phi, const float *s_phl, float *base, float &phi_out,
+                                        bool interior)
 {
     constexpr bool HCZ = MODEL == MPD_MODEL_AC_HCZ;
     if (!x.valid) return;
     const float *h = x.h, *f = x.f;
     const uint32_t n = x.n;
     const int gi = x.gi, gj = x.gj;
-    const int c = ry * R1X + rx;
-    const int c2 = (ry + 1) * R2X + (rx + 1);
-    const float phi = s_phi[c2];
+    const int c = r1(rx, ry);
+    const Stencil sti = stencil_idx(rx + 1, ry + 1);
+    const float phi = s_phi[sti.c[1]];
+    phi_out = phi;
     const float rho = k.aphi * phi + k.bphi;
     Site m;
     float p, bbp, tau = 0.f;
     m.U = m.V = 0.f;
     if (HCZ) {
         float lgx, lgy, llap, lFx, lFy, lrx, lry;
-        stencil(k, s_phl, c2, lgx, lgy, llap);
-        momentum_source(k, s_phl[c2], lgx, lgy, llap, lFx, lFy, lrx, lry);
+        stencil(k, s_phl, sti, lgx, lgy, llap);
+        momentum_source(k, s_phl[sti.c[1]], lgx, lgy, llap, lFx, lFy, lrx, lry);
         const float irr = rcp(rho * k.rt);
         const float sx = (f[1] + f[2] + f[8]) - (f[4] + f[5] + f[6]);
         const float sy = (f[2] + f[3] + f[4]) - (f[6] + f[7] + f[8]);
@@ -279,7 +318,7 @@ __device__ __forceinline__ void cell_bp(const K32 &k, const Rs &out, int nx, int
         bbp = k.bbp0;
     }
     float gx, gy, lap, Fx, Fy, rgx, rgy;
-    stencil(k, s_phi, c2, gx, gy, lap);
+    stencil(k, s_phi, sti, gx, gy, lap);
     momentum_source(k, phi, gx, gy, lap, Fx, Fy, rgx, rgy);
     float ax, ay, npu = 0.f, npv = 0.f;
     {
@@ -386,8 +425,8 @@ __device__ __forceinline__ void cell_bp(const K32 &k, const Rs &out, int nx, int
             st(out.a, n, npu, P_PPU * out.pb);
             st(out.a, n, npv, P_PPV * out.pb);
             if (WALLS && (gj == 0 || gj == ny - 1)) {
-                const int nxt = gj == 0 ? c2 + R2X : c2 - R2X;
-                st(out.gl, (uint32_t)((gj == 0 ? 0 : nx) + gi), 2.f * s_phl[c2] - s_phl[nxt]);
+                const int nxt = gj == 0 ? sti.c[2] : sti.c[0];
+                st(out.gl, (uint32_t)((gj == 0 ? 0 : nx) + gi), 2.f * s_phl[sti.c[1]] - s_phl[nxt]);
             }
         }
     }
@@ -414,23 +453,42 @@ __device__ __forceinline__ float recon(const K32 &k, const float *b, const FaceI
     return v;
 }
 
+// Indices of the face between R1 cells (oc, orow) and its neighbour in
+// direction DIR, with the tangential neighbours of both (LDS layout above).
+template <int DIR>
+__device__ __forceinline__ FaceIdx face_idx(int oc, int orow)
+{
+    FaceIdx x;
+    if (DIR == 0) {
+        const Col a = col1(oc), b = col1(oc + 1);
+        x.o = at(a, orow);
+        x.n = opaque(at(b, orow));
+        x.op = opaque(at(a, orow + 1));
+        x.om = opaque(at(a, orow - 1));
+        x.np = opaque(at(b, orow + 1));
+        x.nm = opaque(at(b, orow - 1));
+    } else {
+        const Col a = col1(oc - 1), b = col1(oc), d = col1(oc + 1);
+        x.o = at(b, orow);
+        x.n = opaque(at(b, orow + 1));
+        x.op = opaque(at(d, orow));
+        x.om = opaque(at(a, orow));
+        x.np = opaque(at(d, orow + 1));
+        x.nm = opaque(at(a, orow + 1));
+    }
+    return x;
+}
+
 // One face: reconstruction, face macros (Update_MacroVar_h, D2Q9HCZ.cpp:183-242,
 // D2Q9AC.cpp:232-329), BGK half step (Update_phi_h, DUGKS2DSolver.cpp:402-423)
 // and micro-flux (Update_phiFlux_h, :424-441) of the six populations with a
 // non-zero normal velocity, in pair order (first member e_n = +1).
 template <int DIR, int MODEL, int WALLS>
-__device__ __forceinline__ void face(const K32 &k, const float *S, int ko, int wall, float *hfl, float *ffl,
+__device__ __forceinline__ void face(const K32 &k, const float *S, const FaceIdx &x, int wall, float *hfl, float *ffl,
                                      float &mass)
 {
     constexpr bool HCZ = MODEL == MPD_MODEL_AC_HCZ;
-    constexpr int DN = DIR == 0 ? 1 : R1X, DT = DIR == 0 ? R1X : 1;
-    FaceIdx x;
-    x.o = ko;
-    x.n = opaque(ko + DN);
-    x.op = opaque(ko + DT);
-    x.om = opaque(ko - DT);
-    x.np = opaque(ko + DN + DT);
-    x.nm = opaque(ko + DN - DT);
+    const int ko = x.o;
     float hv[9], fv[9];
 #define MPD_RECON(Q)                                    \
     hv[Q] = recon<DIR, Q>(k, S + Q * PS, x);            \
@@ -574,7 +632,16 @@ __device__ __forceinline__ void tile32(const K32 &k, int nx, int ny, const Fused
 
     // ---- stage A ----
     for (int q = tid; q < R2N; q += NT) {
-        const int rx = q % R2X, ry = q / R2X;
+        // LDS index q -> phi-region column and row (layout of col2)
+        int rx, ry;
+        if (q < TX * R2Y) {
+            ry = q / TX;
+            rx = q % TX + 2;
+        } else {
+            const int t = q - TX * R2Y, hc = t % 4;
+            ry = t / 4;
+            rx = hc < 2 ? hc : hc + TX;
+        }
         const int gi = wrap(i0 - 2 + rx, nx, small);
         int gj = j0 - 2 + ry;
         float p = 0.f, pl = 0.f;
@@ -597,12 +664,12 @@ __device__ __forceinline__ void tile32(const K32 &k, int nx, int ny, const Fused
     __syncthreads();
 
     // ---- stage B ----
-    float base[17];
-    cell_bp<MODEL, WALLS>(k, out, nx, ny, rx1, ry1, x1, s_big, s_phi, s_phl, base, inside);
+    float base[17], phi_old = 0.f;
+    cell_bp<MODEL, WALLS>(k, out, nx, ny, rx1, ry1, x1, s_big, s_phi, s_phl, base, phi_old, inside);
     if (tid < NR2) {
         if (!PRELOAD_RING) cell_load<MODEL, WALLS>(in, nx, ny, small, i0, j0, rx2, ry2, x2);
-        float dummy[17];
-        cell_bp<MODEL, WALLS>(k, out, nx, ny, rx2, ry2, x2, s_big, s_phi, s_phl, dummy, false);
+        float dummy[17], dphi;
+        cell_bp<MODEL, WALLS>(k, out, nx, ny, rx2, ry2, x2, s_big, s_phi, s_phl, dummy, dphi, false);
     }
     __syncthreads();
 
@@ -615,7 +682,8 @@ __device__ __forceinline__ void tile32(const K32 &k, int nx, int ny, const Fused
         if (row >= 0) {
             const int rx = tid < R1X ? tid : tid - R1X;
             const int dn = row == gy0 && tid < R1X ? 1 : -1;
-            const int c = row * R1X + rx, kn = c + dn * R1X, knn = c + 2 * dn * R1X;
+            const Col cc = col1(rx);
+            const int c = at(cc, row), kn = at(cc, row + dn), knn = at(cc, row + 2 * dn);
 #pragma unroll
             for (int q = 0; q < NBP; ++q) s_big[q * PS + c] = 2.f * s_big[q * PS + kn] - s_big[q * PS + knn];
         }
@@ -623,16 +691,18 @@ __device__ __forceinline__ void tile32(const K32 &k, int nx, int ny, const Fused
     }
 
     // ---- stage C ----
+    // x-face t: column t % TX, row t / TX for t < TX*TY, then the last column
+    // (face index TX*TY + row); y-face t: column t % TX, row t / TX
     float xh[6], xf[6], xm = 0.f, yh[6], yf[6], ym;
     if (tid < NXF) {
-        const int fi = tid % (TX + 1), fj = tid / (TX + 1);
-        face<0, MODEL, WALLS>(k, s_big, (fj + 1) * R1X + fi, 0, xh, xf, xm);
+        const int fi = tid < NIN ? tid % TX : TX, fj = tid < NIN ? tid / TX : tid - NIN;
+        face<0, MODEL, WALLS>(k, s_big, face_idx<0>(fi, fj + 1), 0, xh, xf, xm);
     }
     {
         const int fi = tid % TX, fj = tid / TX;
         const int gj = j0 + fj;
         const int wall = WALLS ? (gj == 0 ? -1 : (gj == ny ? 1 : 0)) : 0;
-        face<1, MODEL, WALLS>(k, s_big, fj * R1X + fi + 1, wall, yh, yf, ym);
+        face<1, MODEL, WALLS>(k, s_big, face_idx<1>(fi + 1, fj), wall, yh, yf, ym);
     }
     __syncthreads();
     float *s_fx = s_big, *s_fy = s_big + NFL * FS;
@@ -655,7 +725,7 @@ __device__ __forceinline__ void tile32(const K32 &k, int nx, int ny, const Fused
     // ---- stage D: x~' = b + (b - x~)/3 + dt/V (W - E + S - N) ----
     if (!inside) return;
     const uint32_t n = (uint32_t)((j0 + cj) * nx + (i0 + ci));
-    const int kW = cj * (TX + 1) + ci, kE = opaque(kW + 1);
+    const int kW = cj * TX + ci, kE = opaque(ci < TX - 1 ? kW + 1 : NIN + cj);
     const int kS = cj * TX + ci, kN = opaque(kS + TX);
     // plane z of the x- (y-) face values holds population xp[z] (yp[z]); even
     // z have e_n = +1, odd z e_n = -1
@@ -682,7 +752,7 @@ __device__ __forceinline__ void tile32(const K32 &k, int nx, int ny, const Fused
     // is the same update; in FP32 it keeps the rounding of the per-population
     // products from drifting the phase-field mass.
     const float dm = (s_fx[12 * FS + kW] - s_fx[12 * FS + kE]) + (s_fy[12 * FS + kS] - s_fy[12 * FS + kN]);
-    const float phi = fmaf(k.dtvf, dm, s_phi[(cj + 2) * R2X + ci + 2]);
+    const float phi = fmaf(k.dtvf, dm, phi_old);
     float rest = 0.f;
 #pragma unroll
     for (int q = 1; q < 9; ++q) {
