@@ -34,15 +34,18 @@ constexpr int NRING = R1N - NIN;           // halo ring of the bar-plus region
 constexpr int NXF = (TX + 1) * TY;         // 255 x-faces
 constexpr int NYF = TX * (TY + 1);         // 256 y-faces
 constexpr int NT = 256;
-constexpr int PS = 320;                    // LDS plane stride in floats: a multiple of 64 dwords, so every
-                                           // plane of one site is an immediate offset from one address
-constexpr int FS = 256;                    // LDS stride of the face-flux planes
-constexpr int NBP = 24;
-constexpr bool PRELOAD_RING = false;  // also hoist the second-pass loads above the phi staging                    // 9 h + 9 f bar-plus planes, then 6 half-macro planes
-constexpr int NFL = 13;                    // per face direction: 6 h fluxes, 6 f fluxes, h mass flux
+// LDS plane strides in float2 elements.  Neither is a multiple of 32 float2
+// (64 dwords): the compiler would otherwise pair reads of two planes at one
+// site into one two-address instruction whose two addresses share a bank.
+constexpr int PS = 324;                    // bar-plus planes
+constexpr int FS = 260;                    // face-value planes
+constexpr int NBP = 12;                    // float2 planes: (h, f) bar-plus of q = 0..8, then the
+                                           // half macros (ax, ay), (rgx, rgy), (Fx, Fy)
+constexpr int NFL = 7;                     // float2 planes per face direction: 6 (h, f) values, (h mass, 0)
+constexpr bool PRELOAD_RING = false;       // also hoist the second-pass loads above the phi staging
 static_assert(R1N <= PS && NXF <= FS && NYF <= FS, "plane strides");
 static_assert(NIN <= NT && NRING - (NT - NIN) <= 64, "stage-B mapping: one partial wave for the ring");
-static_assert(2 * NFL * FS <= NBP * PS, "face fluxes must fit over the bar-plus planes");
+static_assert(2 * NFL * FS <= NBP * PS, "face values must fit over the bar-plus planes");
 
 struct K32 {
     float ms;        // MaSpan
@@ -284,7 +287,7 @@ __device__ __forceinline__ void cell_load(const Rs &in, int nx, int ny, bool sma
 
 template <int MODEL, int WALLS>
 __device__ __forceinline__ void cell_bp(const K32 &k, const Rs &out, int nx, int ny, int rx, int ry, const CellIn &x,
-                                        float *S, const float *s_phi, const float *s_phl, float *base, float &phi_out,
+                                        float2 *S, const float *s_phi, const float *s_phl, float *base, float &phi_out,
                                         bool interior)
 {
     constexpr bool HCZ = MODEL == MPD_MODEL_AC_HCZ;
@@ -407,18 +410,14 @@ __device__ __forceinline__ void cell_bp(const K32 &k, const Rs &out, int nx, int
     const float third = 1.f / 3.f;
 #pragma unroll
     for (int q = 0; q < 9; ++q) {
-        S[q * PS + c] = bh[q];
-        S[(9 + q) * PS + c] = bf[q];
+        S[q * PS + c] = make_float2(bh[q], bf[q]);
         if (q > 0) base[q - 1] = fmaf(third, bh[q] - h[q], bh[q]);
         base[8 + q] = fmaf(third, bf[q] - f[q], bf[q]);
     }
     // face averages need (o + n) / 2: store the halves
-    S[18 * PS + c] = 0.5f * ax;
-    S[19 * PS + c] = 0.5f * ay;
-    S[20 * PS + c] = 0.5f * rgx;
-    S[21 * PS + c] = 0.5f * rgy;
-    S[22 * PS + c] = 0.5f * Fx;
-    S[23 * PS + c] = 0.5f * Fy;
+    S[9 * PS + c] = make_float2(0.5f * ax, 0.5f * ay);
+    S[10 * PS + c] = make_float2(0.5f * rgx, 0.5f * rgy);
+    S[11 * PS + c] = make_float2(0.5f * Fx, 0.5f * Fy);
     if (interior) {
         if (HCZ || WALLS) st(out.a, n, phi, P_PHL * out.pb);
         if (HCZ) {
@@ -438,17 +437,27 @@ struct FaceIdx {
     int o, n, op, om, np, nm;  // owner, neighbour, and their tangential neighbours (+/-)
 };
 template <int DIR, int Q>
-__device__ __forceinline__ float recon(const K32 &k, const float *b, const FaceIdx &x)
+__device__ __forceinline__ float2 recon(const K32 &k, const float2 *b, const FaceIdx &x)
 {
     constexpr int EN = DIR == 0 ? ex9(Q) : ey9(Q), ET = DIR == 0 ? ey9(Q) : ex9(Q);
-    const float bo = b[x.o], bn = b[x.n];
-    float v;
-    if (EN > 0) v = fmaf(k.rb, bn, k.ra * bo);
-    else if (EN < 0) v = fmaf(k.ra, bn, k.rb * bo);
-    else v = 0.5f * (bo + bn);
+    const float2 bo = b[x.o], bn = b[x.n];
+    float2 v;
+    if (EN > 0) {
+        v.x = fmaf(k.rb, bn.x, k.ra * bo.x);
+        v.y = fmaf(k.rb, bn.y, k.ra * bo.y);
+    } else if (EN < 0) {
+        v.x = fmaf(k.ra, bn.x, k.rb * bo.x);
+        v.y = fmaf(k.ra, bn.y, k.rb * bo.y);
+    } else {
+        v.x = 0.5f * (bo.x + bn.x);
+        v.y = 0.5f * (bo.y + bn.y);
+    }
     if (ET != 0) {
-        const float t = (b[x.np] - b[x.nm]) + (b[x.op] - b[x.om]);
-        v = ET > 0 ? fmaf(-k.rt4, t, v) : fmaf(k.rt4, t, v);
+        const float2 np = b[x.np], nm = b[x.nm], op = b[x.op], om = b[x.om];
+        const float tx = (np.x - nm.x) + (op.x - om.x), ty = (np.y - nm.y) + (op.y - om.y);
+        const float c = ET > 0 ? -k.rt4 : k.rt4;
+        v.x = fmaf(c, tx, v.x);
+        v.y = fmaf(c, ty, v.y);
     }
     return v;
 }
@@ -484,25 +493,28 @@ __device__ __forceinline__ FaceIdx face_idx(int oc, int orow)
 // and micro-flux (Update_phiFlux_h, :424-441) of the six populations with a
 // non-zero normal velocity, in pair order (first member e_n = +1).
 template <int DIR, int MODEL, int WALLS>
-__device__ __forceinline__ void face(const K32 &k, const float *S, const FaceIdx &x, int wall, float *hfl, float *ffl,
-                                     float &mass)
+__device__ __forceinline__ void face(const K32 &k, const float2 *S, const FaceIdx &x, int wall, float *hfl,
+                                     float *ffl, float &mass)
 {
     constexpr bool HCZ = MODEL == MPD_MODEL_AC_HCZ;
     const int ko = x.o;
     float hv[9], fv[9];
-#define MPD_RECON(Q)                                    \
-    hv[Q] = recon<DIR, Q>(k, S + Q * PS, x);            \
-    fv[Q] = recon<DIR, Q>(k, S + (9 + Q) * PS, x);
+#define MPD_RECON(Q)                                      \
+    {                                                     \
+        const float2 v = recon<DIR, Q>(k, S + Q * PS, x); \
+        hv[Q] = v.x;                                      \
+        fv[Q] = v.y;                                      \
+    }
     MPD_RECON(0) MPD_RECON(1) MPD_RECON(2) MPD_RECON(3) MPD_RECON(4)
     MPD_RECON(5) MPD_RECON(6) MPD_RECON(7) MPD_RECON(8)
 #undef MPD_RECON
     const int kn = x.n;
-    const float ax = S[18 * PS + ko] + S[18 * PS + kn];
-    const float ay = S[19 * PS + ko] + S[19 * PS + kn];
-    const float rgx = S[20 * PS + ko] + S[20 * PS + kn];
-    const float rgy = S[21 * PS + ko] + S[21 * PS + kn];
-    const float Fx = S[22 * PS + ko] + S[22 * PS + kn];
-    const float Fy = S[23 * PS + ko] + S[23 * PS + kn];
+    const float2 A0 = S[9 * PS + ko], A1 = S[9 * PS + kn];
+    const float2 R0 = S[10 * PS + ko], R1 = S[10 * PS + kn];
+    const float2 F0 = S[11 * PS + ko], F1 = S[11 * PS + kn];
+    const float ax = A0.x + A1.x, ay = A0.y + A1.y;
+    const float rgx = R0.x + R1.x, rgy = R0.y + R1.y;
+    const float Fx = F0.x + F1.x, Fy = F0.y + F1.y;
     const float phi = ((hv[0] + hv[1]) + (hv[2] + hv[3])) + ((hv[4] + hv[5]) + (hv[6] + hv[7])) + hv[8];
     Site m;
     m.mAx = k.ms * ax;
@@ -602,7 +614,7 @@ __device__ __forceinline__ void tile32(const K32 &k, int nx, int ny, const Fused
                                        const FusedPlanes32 &pout, int64_t N, int i0, int j0)
 {
     constexpr bool HCZ = MODEL == MPD_MODEL_AC_HCZ;
-    __shared__ float s_big[NBP * PS];
+    __shared__ float2 s_big[NBP * PS];
     __shared__ float s_phi[R2N];
     __shared__ float s_phl[HCZ ? R2N : 1];
     const int tid = threadIdx.x;
@@ -685,7 +697,10 @@ __device__ __forceinline__ void tile32(const K32 &k, int nx, int ny, const Fused
             const Col cc = col1(rx);
             const int c = at(cc, row), kn = at(cc, row + dn), knn = at(cc, row + 2 * dn);
 #pragma unroll
-            for (int q = 0; q < NBP; ++q) s_big[q * PS + c] = 2.f * s_big[q * PS + kn] - s_big[q * PS + knn];
+            for (int q = 0; q < NBP; ++q) {
+                const float2 a = s_big[q * PS + kn], b = s_big[q * PS + knn];
+                s_big[q * PS + c] = make_float2(2.f * a.x - b.x, 2.f * a.y - b.y);
+            }
         }
         __syncthreads();
     }
@@ -705,21 +720,15 @@ __device__ __forceinline__ void tile32(const K32 &k, int nx, int ny, const Fused
         face<1, MODEL, WALLS>(k, s_big, face_idx<1>(fi + 1, fj), wall, yh, yf, ym);
     }
     __syncthreads();
-    float *s_fx = s_big, *s_fy = s_big + NFL * FS;
+    float2 *s_fx = s_big, *s_fy = s_big + NFL * FS;
     if (tid < NXF) {
 #pragma unroll
-        for (int z = 0; z < 6; ++z) {
-            s_fx[z * FS + tid] = xh[z];
-            s_fx[(6 + z) * FS + tid] = xf[z];
-        }
-        s_fx[12 * FS + tid] = xm;
+        for (int z = 0; z < 6; ++z) s_fx[z * FS + tid] = make_float2(xh[z], xf[z]);
+        s_fx[6 * FS + tid] = make_float2(xm, 0.f);
     }
 #pragma unroll
-    for (int z = 0; z < 6; ++z) {
-        s_fy[z * FS + tid] = yh[z];
-        s_fy[(6 + z) * FS + tid] = yf[z];
-    }
-    s_fy[12 * FS + tid] = ym;
+    for (int z = 0; z < 6; ++z) s_fy[z * FS + tid] = make_float2(yh[z], yf[z]);
+    s_fy[6 * FS + tid] = make_float2(ym, 0.f);
     __syncthreads();
 
     // ---- stage D: x~' = b + (b - x~)/3 + dt/V (W - E + S - N) ----
@@ -737,21 +746,23 @@ __device__ __forceinline__ void tile32(const K32 &k, int nx, int ny, const Fused
 #pragma unroll
     for (int z = 0; z < 6; ++z) {
         const int a = (z & 1) ? kE : kW, b = (z & 1) ? kW : kE;
-        dh[xp[z]] += s_fx[z * FS + a] - s_fx[z * FS + b];
-        df[xp[z]] += s_fx[(6 + z) * FS + a] - s_fx[(6 + z) * FS + b];
+        const float2 va = s_fx[z * FS + a], vb = s_fx[z * FS + b];
+        dh[xp[z]] += va.x - vb.x;
+        df[xp[z]] += va.y - vb.y;
     }
 #pragma unroll
     for (int z = 0; z < 6; ++z) {
         const int a = (z & 1) ? kN : kS, b = (z & 1) ? kS : kN;
-        dh[yp[z]] += s_fy[z * FS + a] - s_fy[z * FS + b];
-        df[yp[z]] += s_fy[(6 + z) * FS + a] - s_fy[(6 + z) * FS + b];
+        const float2 va = s_fy[z * FS + a], vb = s_fy[z * FS + b];
+        dh[yp[z]] += va.x - vb.x;
+        df[yp[z]] += va.y - vb.y;
     }
     // phi is the conserved variable: it moves only by the antisymmetric face
     // mass fluxes (each face's sum over populations, shared by both cells), and
     // h~_0 takes the remainder so that sum_q h~ = phi.  In exact arithmetic this
     // is the same update; in FP32 it keeps the rounding of the per-population
     // products from drifting the phase-field mass.
-    const float dm = (s_fx[12 * FS + kW] - s_fx[12 * FS + kE]) + (s_fy[12 * FS + kS] - s_fy[12 * FS + kN]);
+    const float dm = (s_fx[6 * FS + kW].x - s_fx[6 * FS + kE].x) + (s_fy[6 * FS + kS].x - s_fy[6 * FS + kN].x);
     const float phi = fmaf(k.dtvf, dm, phi_old);
     float rest = 0.f;
 #pragma unroll
