@@ -223,6 +223,72 @@ int mpd_init_pp_drop(const mpd_config *cfg, double cx, double cy, double radius,
 int mpd_init_pp_noise(const mpd_config *cfg, double rho0, double amp, uint64_t seed, double mu0,
                       double *tilde_f, double *macro, double *face_fx, double *face_fy);
 
+/* ---------------- output files of the reference driver ----------------
+ * Byte-identical writers of the files the reference's driver writes
+ * (Ark/Cpp/Output.cpp), fed from downloaded macro planes.  Host code: no GPU
+ * is needed.  Paths are used as given; mpd_output_name gives the reference's
+ * file name relative to its FlowField directory. */
+typedef enum mpd_output_kind {
+    MPD_OUT_FLOWFIELD = 0,  /* global/step..   Output_Flowfield, Output.cpp:484-555 */
+    MPD_OUT_SUMRHO = 1,     /* Convergence/SumRho_..   Output_SumRho, :258-271 */
+    MPD_OUT_RESIDUAL = 2,   /* Convergence/Residual_.. Output_Residual, :272-287 */
+    MPD_OUT_MIDX = 3,       /* Convergence/MidX_..     Output_MidX, :309-390 */
+    MPD_OUT_MIDY = 4,       /* Convergence/MidY_..     Output_MidY, :404-458 */
+    MPD_OUT_DIAG = 5        /* Convergence/Diag_..     Output_Diag, :459-515 */
+} mpd_output_kind;
+
+/* Names and constants the writers print (the reference's compile-time values). */
+typedef struct mpd_output_info {
+    double ma, cfl, mu0, re;  /* Ma, CFL, Mu0, Re (ZeroReference.h, ZeroNamespace.h) */
+    double m_phi;             /* PhaseFieldAC::M_Phi, printed in profile zone titles */
+    double rho_l, rho_v;      /* density ratio in profile file names */
+    double temperature;       /* the T zone of the flow field (MacroQuantity::T = T0) */
+    int32_t nl;               /* NL, the mesh-size tag of file names */
+    char mesh_type[32];       /* _MESHTYPE_ARK */
+    char mesh_file[128];      /* _MESHFILE_NAME_ARK */
+} mpd_output_info;
+
+/* The reference's defaults for `cfg` (ZeroFlip.h, ZeroReference.h). */
+int mpd_default_output_info(const mpd_config *cfg, mpd_output_info *info);
+
+/* File name of `kind` for `step` at time `t` (NUL-terminated, at most `len` bytes). */
+int mpd_output_name(int32_t kind, const mpd_output_info *info, int64_t step, double t, char *buf,
+                    int32_t len);
+
+/* Output_Flowfield: zones Rho, U V, p, T, Phi, f.tau, Fx, Fy over the cells in
+ * the driver's CellArray order (`order[k]` = structured index of cell k; NULL for
+ * structured order), scientific with 12 digits. */
+int mpd_write_flowfield(const char *path, int32_t nx, int32_t ny, const double *macro, const int64_t *order,
+                        const mpd_output_info *info);
+
+/* Output_MidX / Output_MidY / Output_Diag (kind MPD_OUT_MIDX/MIDY/DIAG): Rho, p,
+ * thermodynamic pressure (thermoPressure, Output.cpp:146-162), U, V along the
+ * profile; cfg gives the mesh and the phase-field constants. */
+int mpd_write_profile(const char *path, int32_t kind, const mpd_config *cfg, const double *macro,
+                      const mpd_output_info *info);
+
+/* Output_SumRho / Output_Residual: append one "t    value" record. */
+int mpd_append_convergence(const char *path, double t, double value);
+
+/* Case.ark (OutputCase, Output.cpp:65-144). */
+typedef struct mpd_case_info {
+    char case_name[64], mesh_file[128], mesh_type[32], flux_scheme[16], bc[32];
+    char velocity_model[16], force_model[32], eos_model[32];
+    int32_t zone_left, zone_right, zone_top, zone_bottom;
+    double omega0, prandtl, nk, cv, gamma;
+    int32_t qu, qv, nl;
+    double ch_length, min_l, kn, ma, re, u0, r0, t0, mu0, nu0, tau0, lambda0, cfl, dt, residual;
+    int32_t end_step;
+    double maspan, eta;
+    int32_t phase_field;      /* print the phase-field block */
+    double cn, pe, eo, mo, re_mp, mobility, tau_mass, width, sigma, beta, kappa;
+    double rho_l, rho_v, mu_l, mu_v, nu_l, nu_v, radius;
+} mpd_case_info;
+
+/* The reference's Case.ark values for `cfg` (AC_Drop case, shipped flags). */
+int mpd_default_case_info(const mpd_config *cfg, mpd_case_info *info);
+int mpd_write_case(const char *path, const mpd_case_info *info);
+
 /* Text of the last error on this thread. */
 const char *mpd_last_error(void);
 

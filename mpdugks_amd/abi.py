@@ -49,3 +49,29 @@ def has_h(model):
     return model in (MPD_MODEL_AC_P, MPD_MODEL_AC_HCZ)
 
 MPD_PATH_AUTO, MPD_PATH_PHASES, MPD_PATH_FUSED = 0, 1, 2
+
+MPD_OUT_FLOWFIELD, MPD_OUT_SUMRHO, MPD_OUT_RESIDUAL, MPD_OUT_MIDX, MPD_OUT_MIDY, MPD_OUT_DIAG = range(6)
+
+
+class MpdOutputInfo(C.Structure):
+    _fields_ = [("ma", C.c_double), ("cfl", C.c_double), ("mu0", C.c_double), ("re", C.c_double),
+                ("m_phi", C.c_double), ("rho_l", C.c_double), ("rho_v", C.c_double),
+                ("temperature", C.c_double), ("nl", C.c_int32), ("mesh_type", C.c_char * 32),
+                ("mesh_file", C.c_char * 128)]
+
+
+class MpdCaseInfo(C.Structure):
+    _fields_ = ([(k, C.c_char * n) for k, n in (("case_name", 64), ("mesh_file", 128), ("mesh_type", 32),
+                                                ("flux_scheme", 16), ("bc", 32), ("velocity_model", 16),
+                                                ("force_model", 32), ("eos_model", 32))]
+                + [(k, C.c_int32) for k in ("zone_left", "zone_right", "zone_top", "zone_bottom")]
+                + [(k, C.c_double) for k in ("omega0", "prandtl", "nk", "cv", "gamma")]
+                + [(k, C.c_int32) for k in ("qu", "qv", "nl")]
+                + [(k, C.c_double) for k in ("ch_length", "min_l", "kn", "ma", "re", "u0", "r0", "t0", "mu0",
+                                             "nu0", "tau0", "lambda0", "cfl", "dt", "residual")]
+                + [("end_step", C.c_int32)]
+                + [(k, C.c_double) for k in ("maspan", "eta")]
+                + [("phase_field", C.c_int32)]
+                + [(k, C.c_double) for k in ("cn", "pe", "eo", "mo", "re_mp", "mobility", "tau_mass", "width",
+                                             "sigma", "beta", "kappa", "rho_l", "rho_v", "mu_l", "mu_v", "nu_l",
+                                             "nu_v", "radius")])

@@ -22,6 +22,7 @@ SOURCES = {
     "fused_kernels.hip": ["-ffp-contract=off"],
     "fused_fp32.hip": ["-ffp-contract=fast"],
     "hostinit.cpp": ["-ffp-contract=off"],
+    "writers.cpp": ["-ffp-contract=off"],
 }
 
 

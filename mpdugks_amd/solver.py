@@ -49,6 +49,13 @@ _SYMBOLS = {
                          C.c_int),
     "mpd_init_pp_noise": ([C.c_void_p, C.c_double, C.c_double, C.c_uint64, C.c_double, C.c_void_p,
                            C.c_void_p, C.c_void_p, C.c_void_p], C.c_int),
+    "mpd_default_output_info": ([C.c_void_p, C.c_void_p], C.c_int),
+    "mpd_output_name": ([C.c_int32, C.c_void_p, C.c_int64, C.c_double, C.c_char_p, C.c_int32], C.c_int),
+    "mpd_write_flowfield": ([C.c_char_p, C.c_int32, C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p], C.c_int),
+    "mpd_write_profile": ([C.c_char_p, C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p], C.c_int),
+    "mpd_append_convergence": ([C.c_char_p, C.c_double, C.c_double], C.c_int),
+    "mpd_default_case_info": ([C.c_void_p, C.c_void_p], C.c_int),
+    "mpd_write_case": ([C.c_char_p, C.c_void_p], C.c_int),
 }
 
 
@@ -258,3 +265,45 @@ def default_config(model=abi.MPD_MODEL_AC_HCZ):
     cfg = MpdConfig()
     _check(lib().mpd_default_config(int(model), C.byref(cfg)), "mpd_default_config")
     return cfg
+
+
+# ---------------- output files (Output.cpp formats; host only) ----------------
+
+def output_info(cfg):
+    """The reference's naming constants for cfg (mpd_default_output_info)."""
+    o = abi.MpdOutputInfo()
+    _check(lib().mpd_default_output_info(C.byref(cfg), C.byref(o)), "mpd_default_output_info")
+    return o
+
+
+def output_name(kind, info, step, t):
+    buf = C.create_string_buffer(512)
+    _check(lib().mpd_output_name(int(kind), C.byref(info), int(step), float(t), buf, 512), "mpd_output_name")
+    return buf.value.decode()
+
+
+def write_flowfield(path, cfg, state, info, order=None):
+    m = _f64(state.m)
+    o = None if order is None else np.ascontiguousarray(order, dtype=np.int64)
+    _check(lib().mpd_write_flowfield(os.fsencode(path), cfg.nx, cfg.ny, _p(m), _p(o), C.byref(info)),
+           "mpd_write_flowfield")
+
+
+def write_profile(path, kind, cfg, state, info):
+    m = _f64(state.m)
+    _check(lib().mpd_write_profile(os.fsencode(path), int(kind), C.byref(cfg), _p(m), C.byref(info)),
+           "mpd_write_profile")
+
+
+def append_convergence(path, t, value):
+    _check(lib().mpd_append_convergence(os.fsencode(path), float(t), float(value)), "mpd_append_convergence")
+
+
+def case_info(cfg):
+    c = abi.MpdCaseInfo()
+    _check(lib().mpd_default_case_info(C.byref(cfg), C.byref(c)), "mpd_default_case_info")
+    return c
+
+
+def write_case(path, info):
+    _check(lib().mpd_write_case(os.fsencode(path), C.byref(info)), "mpd_write_case")

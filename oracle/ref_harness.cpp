@@ -13,7 +13,10 @@
 //   written out before the step proceeds.
 //
 // Environment: MPD_REF_DUMP_STEPS="0,1,2,10" (steps to dump), MPD_REF_MAX_STEP
-// (exit(0) when reached), MPD_REF_DUMP_DIR (output directory).
+// (exit(0) when reached), MPD_REF_DUMP_DIR (output directory),
+// MPD_REF_WRITERS=1 (at each dumped step also call the reference's own
+// Output_Flowfield, Output_Diag, Output_MidX and Output_MidY, Ark/Cpp/Output.cpp,
+// so their files can serve as format fixtures).
 //
 // Dump layout (little-endian): int32 nx, ny, nfields, then nfields planes of
 // nx*ny float64 in structured order idx = j*nx + i, with (i,j) taken from the
@@ -30,6 +33,9 @@
 // of `void Update_phi_Eq(Cell_2D&)` and `void Output_Flowfield(double const&, int)`.
 extern "C" void __real__Z13Update_phi_EqR7Cell_2D(Cell_2D &cell);
 extern "C" void __real__Z16Output_FlowfieldRKdi(double const &t, int step);
+void Output_Diag(int step);
+void Output_MidX(int step);
+void Output_MidY(int step);
 
 namespace {
 bool g_started = false;
@@ -125,6 +131,14 @@ void dump_cells(int k)
         std::fwrite(v, sizeof(double), 12, fp);
     }
     std::fclose(fp);
+    if (const char *s = std::getenv("MPD_REF_WRITERS"))
+        if (s[0] == '1') {
+            const double t = k * dt;
+            __real__Z16Output_FlowfieldRKdi(t, k);
+            Output_Diag(k);
+            Output_MidX(k);
+            Output_MidY(k);
+        }
     std::fprintf(stderr, "[ref_harness] dumped step %d\n", k);
 }
 }  // namespace

@@ -49,7 +49,7 @@ def build(cfg):
     return os.path.join(HERE, "_ref", cfg, "ark")
 
 
-def run(cfg, steps, max_step, outdir=None, flowfield=False, timeout=3600):
+def run(cfg, steps, max_step, outdir=None, flowfield=False, timeout=3600, writers=False):
     """Run config `cfg`, dumping the cell state after each step in `steps`."""
     exe = build(cfg)
     base = os.path.join(HERE, "_ref", cfg, "run")
@@ -63,12 +63,18 @@ def run(cfg, steps, max_step, outdir=None, flowfield=False, timeout=3600):
     write_cas(os.path.join(home, "Mesh", MESH_NAME + ".cas"), 128, 128,
               walls_y=CONFIGS[cfg][2])
     outdir = outdir or os.path.join(base, "dump")
+    if writers:  # fresh output tree: the convergence files are appended to
+        import shutil
+        shutil.rmtree(os.path.join(base, "FlowField"), ignore_errors=True)
+        for d in ("global", "Convergence"):
+            os.makedirs(os.path.join(base, "FlowField", d), exist_ok=True)
     os.makedirs(outdir, exist_ok=True)
     env = dict(os.environ)
     env.update({"HOME": home, "OMP_NUM_THREADS": "1",
                 "MPD_REF_DUMP_STEPS": ",".join(str(s) for s in steps),
                 "MPD_REF_MAX_STEP": str(max_step), "MPD_REF_DUMP_DIR": outdir,
-                "MPD_REF_FLOWFIELD": "1" if flowfield else "0"})
+                "MPD_REF_FLOWFIELD": "1" if flowfield else "0",
+                "MPD_REF_WRITERS": "1" if writers else "0"})
     with open(os.devnull) as dn, open(os.path.join(base, "stdout.log"), "w") as log:
         subprocess.run([exe], cwd=work, env=env, stdin=dn, stdout=log,
                        stderr=subprocess.STDOUT, check=True, timeout=timeout)
