@@ -166,12 +166,34 @@ def cpu_baseline(cfg, st, budget_s):
     return cfg.nx * cfg.ny * n / dt / 1e6, n, dt
 
 
+def cpu_reference(steps, threads):
+    """The reference solver binary (oracle/_ref/hcz/ark, built here from the
+    reference's sources) timed by its own per-1000-step print on its shipped
+    case; None when the binary is absent."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import ref_run
+    try:
+        rows = ref_run.time_prebuilt("hcz", steps=steps, threads=threads, timeout=180)
+    except Exception as e:  # a missing or failing checker binary only drops this field
+        print("[bench] reference CPU solver not timed: %s" % e, file=sys.stderr, flush=True)
+        return None
+    if not rows:
+        return None
+    ms = rows[-1][1]  # the last 1000-step block (start-up excluded)
+    return {"value": round(128 * 128 * 1000 / (ms * 1e3), 3), "unit": "MLUPS", "cores": threads,
+            "kind": "reference",
+            "sample": "reference Ark OpenMP solver, its own 128x128 AC+HCZ drop, steps %d-%d (%.2f s)"
+                      % (rows[-1][0] - 1000, rows[-1][0], ms / 1e3)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=2000)
     ap.add_argument("--warmup", type=int, default=100)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--cpu-ref-steps", type=int, default=2000)
+    ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--nx", type=int, default=256)
     ap.add_argument("--ny", type=int, default=1024)
@@ -284,6 +306,14 @@ def main():
         line["cpu_baseline"] = {"value": round(mlups, 3), "unit": "MLUPS", "cores": 1, "kind": "port",
                                 "sample": "CPU FP64 oracle, same RT %dx%d case, first %d steps (%.1f s)"
                                 % (nx, ny, nsteps, dt)}
+        # the reference's own OpenMP solver (built from its sources into
+        # oracle/_ref) on its shipped 128x128 AC+HCZ drop: it cannot run the RT
+        # case (mesh size and walls are compile-time constants there)
+        ref = cpu_reference(args.cpu_ref_steps, args.cpu_threads)
+        if ref:
+            line["cpu_reference"] = ref
+            print("[bench] reference CPU solver: %.3f MLUPS on %d threads" % (ref["value"], ref["cores"]),
+                  file=sys.stderr, flush=True)
     print(json.dumps(line))
 
 

@@ -104,3 +104,39 @@ if __name__ == "__main__":
     cfg = sys.argv[1]
     steps = [int(s) for s in sys.argv[2].split(",")]
     print(run(cfg, steps, max(steps)))
+
+
+def time_prebuilt(cfg="hcz", steps=2000, threads=1, timeout=120):
+    """Time the already-built reference binary of `cfg` on its own case (no
+    rebuild, so it also runs where the reference sources are absent): returns
+    [(step, ms for the preceding 1000 steps)] from its own progress lines
+    (DUGKS2DSolver.cpp:872-876)."""
+    import re
+    import tempfile
+    exe = os.path.join(HERE, "_ref", cfg, "ark")
+    if not os.path.exists(exe):
+        return None
+    base = tempfile.mkdtemp(prefix="mpd_refrun_")
+    home = os.path.join(base, "home")
+    work = os.path.join(base, "work")
+    for d in (os.path.join(home, "Mesh"), work, os.path.join(base, "FlowField", "global"),
+              os.path.join(base, "FlowField", "Convergence")):
+        os.makedirs(d, exist_ok=True)
+    sys.path.insert(0, HERE)
+    from casmesh import write_cas
+    write_cas(os.path.join(home, "Mesh", MESH_NAME + ".cas"), 128, 128, walls_y=CONFIGS[cfg][2])
+    env = dict(os.environ)
+    env.update({"HOME": home, "OMP_NUM_THREADS": str(threads), "MPD_REF_DUMP_STEPS": "",
+                "MPD_REF_MAX_STEP": str(steps + 1), "MPD_REF_DUMP_DIR": base, "MPD_REF_FLOWFIELD": "0",
+                "MPD_REF_WRITERS": "0"})
+    with open(os.devnull) as dn:
+        r = subprocess.run([exe], cwd=work, env=env, stdin=dn, stdout=subprocess.PIPE,
+                           stderr=subprocess.STDOUT, timeout=timeout)
+    import shutil
+    shutil.rmtree(base, ignore_errors=True)
+    out = []
+    for line in r.stdout.decode(errors="replace").split("\n"):
+        m = re.match(r"^(\d+)\s+\S+\s+\S+\s+\S+\s+\S+\s+(\d+)\s*$", line)
+        if m:
+            out.append((int(m.group(1)), int(m.group(2))))
+    return out
