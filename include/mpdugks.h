@@ -51,7 +51,11 @@ typedef enum mpd_model {
 typedef enum mpd_flux { MPD_FLUX_CD = 0, MPD_FLUX_UW = 1 } mpd_flux;
 
 /* Boundary in y (x is always periodic): Ark/Constant/ZeroFlip.h:59-77. */
-typedef enum mpd_bc { MPD_BC_PERIODIC = 0, MPD_BC_WALL_BB = 1 } mpd_bc;
+typedef enum mpd_bc {
+    MPD_BC_PERIODIC = 0,  /* periodic pair 12/8 (GhostCells.cpp:20-28) */
+    MPD_BC_WALL_BB = 1,   /* no-slip wall, bounce-back (Wall_3_BB, Boundary.cpp:118-139) */
+    MPD_BC_WALL_NEE = 2   /* no-slip wall, non-equilibrium extrapolation (Wall_3_NEE, Boundary.cpp:140-184) */
+} mpd_bc;
 
 /* Arithmetic of the bulk update. */
 typedef enum mpd_precision {

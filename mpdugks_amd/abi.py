@@ -3,7 +3,7 @@ import ctypes as C
 
 MPD_MODEL_AC_P, MPD_MODEL_AC_HCZ, MPD_MODEL_PP_D2Q9, MPD_MODEL_PP_D2V16 = 0, 1, 2, 3
 MPD_FLUX_CD, MPD_FLUX_UW = 0, 1
-MPD_BC_PERIODIC, MPD_BC_WALL_BB = 0, 1
+MPD_BC_PERIODIC, MPD_BC_WALL_BB, MPD_BC_WALL_NEE = 0, 1, 2
 MPD_FP64, MPD_FP32_BULK = 0, 1
 
 MACRO_FIELDS = ["phi", "rho", "u", "v", "p", "phix", "phiy", "rhox", "rhoy", "fx", "fy",

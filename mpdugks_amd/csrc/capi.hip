@@ -102,7 +102,8 @@ void make_consts(const mpd_config &c, Consts &C)
     C.ny = c.ny;
     C.model = c.model;
     C.flux = c.flux;
-    C.walls = c.bc_y == MPD_BC_WALL_BB;
+    C.walls = c.bc_y == MPD_BC_WALL_BB || c.bc_y == MPD_BC_WALL_NEE;
+    C.nee = c.bc_y == MPD_BC_WALL_NEE;
     C.nyf = C.walls ? c.ny + 1 : c.ny;
     C.dx = c.dx;
     C.dt = c.dt;
@@ -215,8 +216,9 @@ int check_cfg(const mpd_config *c)
     if (c->model == MPD_MODEL_PP_D2V16 && (c->flux != MPD_FLUX_CD || c->bc_y != MPD_BC_PERIODIC))
         return fail(MPD_ERR_INVALID, "D2V16 runs with CD faces on a periodic mesh");
     if (c->flux != MPD_FLUX_CD && c->flux != MPD_FLUX_UW) return fail(MPD_ERR_INVALID, "unknown flux %d", c->flux);
-    if (c->bc_y != MPD_BC_PERIODIC && c->bc_y != MPD_BC_WALL_BB) return fail(MPD_ERR_INVALID, "unknown bc %d", c->bc_y);
-    if (c->bc_y == MPD_BC_WALL_BB && (c->flux == MPD_FLUX_UW || c->model == MPD_MODEL_PP_D2Q9))
+    if (c->bc_y != MPD_BC_PERIODIC && c->bc_y != MPD_BC_WALL_BB && c->bc_y != MPD_BC_WALL_NEE)
+        return fail(MPD_ERR_INVALID, "unknown bc %d", c->bc_y);
+    if (c->bc_y != MPD_BC_PERIODIC && (c->flux == MPD_FLUX_UW || c->model == MPD_MODEL_PP_D2Q9))
         return fail(MPD_ERR_INVALID, "walls are supported with CD and the Allen-Cahn models only");
     if (c->precision != MPD_FP64 && c->precision != MPD_FP32_BULK)
         return fail(MPD_ERR_INVALID, "unknown precision %d", c->precision);

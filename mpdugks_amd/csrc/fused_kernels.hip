@@ -540,8 +540,10 @@ FusedPlanes32 fused32_planes(FusedState &F, int which)
 
 bool fused_supported(const mpd_config &cfg)
 {
+    // walls: bounce-back only (the non-equilibrium extrapolation needs the
+    // cell equilibria at the wall row, which the tile does not keep)
     return (cfg.model == MPD_MODEL_AC_HCZ || cfg.model == MPD_MODEL_AC_P) && cfg.flux == MPD_FLUX_CD &&
-           cfg.nx >= 4 && cfg.ny >= 4;
+           cfg.bc_y != MPD_BC_WALL_NEE && cfg.nx >= 4 && cfg.ny >= 4;
 }
 
 int fused_alloc(FusedState &F, const DevState &S, const Consts &C, const mpd_config &cfg)

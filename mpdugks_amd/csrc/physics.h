@@ -22,6 +22,7 @@ __host__ __device__ constexpr int bb9(int q) { return q == 0 ? 0 : (q <= 4 ? q +
 // expressions as Ark/Constant/ZeroNamespace.h / ZeroReference.h.
 struct Consts {
     int32_t nx, ny, model, flux, walls, nyf;
+    int32_t nee;      // walls use non-equilibrium extrapolation (Wall_3_NEE) instead of bounce-back
     double dx, dt, hdt, rt, lambda0, aphi, bphi, beta, kappa, kp, taum, tau0;
     double phi_l, phi_v, nu_l, nu_v, gx, gy, rho_ref, pp_k, t_cs, pp_tau;
     double dtv;       // dt / volume                    (Cell_2D.cpp:182)
@@ -57,6 +58,29 @@ template <typename T>
 struct CellMacro {
     T phi, rho, U, V, p, tau, px, py, rx, ry, Fx, Fy;
 };
+
+// Equilibria and sources of population q of one cell for the Allen-Cahn
+// models (the expressions of cell_barplus below).
+template <typename T, int MODEL>
+__device__ inline void cell_eq_so(const Consts &C, const CellMacro<T> &m, int q, T &heq, T &feq, T &hso, T &fso)
+{
+    const T rt = (T)C.rt;
+    const T uu = m.U * m.U + m.V * m.V;
+    const T ex = (T)C.xi[0][q], ey = (T)C.xi[1][q], w = (T)C.w[q];
+    T u1 = (ex * m.U + ey * m.V) / rt;
+    T G = u1 + (T)0.5 * u1 * u1 - uu * (T)C.lambda0;
+    heq = w * ((T)1.0 + u1) * m.phi;
+    hso = (m.px * ex + m.py * ey) * w;
+    if (MODEL == MPD_MODEL_AC_HCZ) {
+        feq = w * (m.p + m.rho * G * rt);
+        fso = w * (((T)1 + G) * ((ex - m.U) * m.Fx + (ey - m.V) * m.Fy) +
+                   G * rt * ((ex - m.U) * m.rx + (ey - m.V) * m.ry));
+    } else {
+        feq = w * (m.p / rt + m.rho * G) - (T)(q == 0) * m.p / rt;
+        fso = w * (((T)1 + G) * ((ex - m.U) * m.Fx + (ey - m.V) * m.Fy) / rt +
+                   G * ((ex - m.U) * m.rx + (ey - m.V) * m.ry));
+    }
+}
 
 // Update_phi_Eq + Update_DVDF_Source + Update_phi_BP for one cell
 // (HCZ Ark/Lib/D2Q9HCZ.cpp:54-92,244-257; AC-p Ark/Lib/D2Q9AC.cpp:69-105,331-344;

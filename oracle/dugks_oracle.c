@@ -237,14 +237,14 @@ ora *ora_create(const mpd_config *cfg)
         return NULL;
     if (cfg->model == MPD_MODEL_PP_D2V16 && (cfg->flux != MPD_FLUX_CD || cfg->bc_y != MPD_BC_PERIODIC))
         return NULL; /* D2V16: CD faces on a periodic mesh */
-    if (cfg->bc_y == MPD_BC_WALL_BB && (cfg->flux == MPD_FLUX_UW || cfg->model == MPD_MODEL_PP_D2Q9))
+    if (cfg->bc_y != MPD_BC_PERIODIC && (cfg->flux == MPD_FLUX_UW || cfg->model == MPD_MODEL_PP_D2Q9))
         return NULL; /* the reference has no working wall path for these */
     ora *o = (ora *)calloc(1, sizeof(ora));
     o->c = *cfg;
     o->nx = cfg->nx;
     o->ny = cfg->ny;
     o->N = cfg->nx * cfg->ny;
-    o->walls = cfg->bc_y == MPD_BC_WALL_BB;
+    o->walls = cfg->bc_y == MPD_BC_WALL_BB || cfg->bc_y == MPD_BC_WALL_NEE;
     o->nyf = o->walls ? o->ny + 1 : o->ny;
     const mpd_config *c = cfg;
     o->hdt = 0.5 * c->dt;
@@ -599,6 +599,67 @@ static inline double face_psi(const ora *o, int dir, int i, int j)
     return o->fpsi[(size_t)dir * o->N + (size_t)wrap(j, o->ny) * o->nx + wrap(i, o->nx)];
 }
 
+/* Equilibria and sources of population q of cell n (the AC part of
+ * phase1_cell, same expressions). */
+static void cell_eq_so(const ora *o, size_t n, int q, double *heq, double *feq, double *hso, double *fso)
+{
+    const double rt = o->c.rt;
+    const double *m = o->m;
+    size_t N = o->N;
+    double phi = m[MPD_M_PHI * N + n], rho = m[MPD_M_RHO * N + n];
+    double U = m[MPD_M_U * N + n], V = m[MPD_M_V * N + n], p = m[MPD_M_P * N + n];
+    double Fx = m[MPD_M_FX * N + n], Fy = m[MPD_M_FY * N + n];
+    double rx = m[MPD_M_RHOX * N + n], ry = m[MPD_M_RHOY * N + n];
+    double px = m[MPD_M_PHIX * N + n], py = m[MPD_M_PHIY * N + n];
+    double uu = U * U + V * V;
+    double ex = o->xi[0][q], ey = o->xi[1][q], w = W9[q];
+    double u1 = (ex * U + ey * V) / rt;
+    double G = u1 + 0.5 * u1 * u1 - uu * o->lambda0;
+    *heq = w * (1.0 + u1) * phi;
+    *hso = (px * ex + py * ey) * w;
+    if (o->c.model == MPD_MODEL_AC_HCZ) {
+        *feq = w * (p + rho * G * rt);
+        *fso = w * ((1 + G) * ((ex - U) * Fx + (ey - V) * Fy) + G * rt * ((ex - U) * rx + (ey - V) * ry));
+    } else {
+        *feq = w * (p / rt + rho * G) - (double)(q == 0) * p / rt;
+        *fso = w * ((1 + G) * ((ex - U) * Fx + (ey - V) * Fy) / rt + G * ((ex - U) * rx + (ey - V) * ry));
+    }
+}
+
+/* Wall_3_NEE (Ark/Cpp/Boundary.cpp:140-184): the wall face takes the
+ * equilibrium of face macros extrapolated from its fluid cell (Phi and Rho
+ * along their cell gradients -- Phi_x/Phi_y hold the AC source vector there --,
+ * p of the cell, U = V = 0: the face macros are never set otherwise) plus the
+ * cell's non-equilibrium part 2 tau / (2 tau + dt) (x~ - x_eq + h x_so); the
+ * caller then applies Wall_3_BB. */
+static void wall_nee(const ora *o, const face_t *F, int j, double *hh, double *ff)
+{
+    const double rt = o->c.rt, dx = o->c.dx;
+    int jc = j == 0 ? 0 : o->ny - 1, i = F->oi;
+    size_t N = o->N, n = (size_t)jc * o->nx + i;
+    const double *m = o->m;
+    double xc = (i + 0.5) * dx, yc = (jc + 0.5) * dx;
+    double phi = m[MPD_M_PHI * N + n] + m[MPD_M_PHIX * N + n] * (F->xf - xc) + m[MPD_M_PHIY * N + n] * (F->yf - yc);
+    double rho = m[MPD_M_RHO * N + n] + m[MPD_M_RHOX * N + n] * (F->xf - xc) + m[MPD_M_RHOY * N + n] * (F->yf - yc);
+    double p = m[MPD_M_P * N + n], U = 0.0, V = 0.0, uu = U * U + V * V;
+    double tau = m[MPD_M_TAU * N + n];
+    double hNEq = 2.0 * o->taum / (2.0 * o->taum + o->c.dt);
+    double fNEq = 2.0 * tau / (2.0 * tau + o->c.dt);
+    for (int q = 0; q < Q9; ++q) {
+        double heq, feq, hso, fso;
+        cell_eq_so(o, n, q, &heq, &feq, &hso, &fso);
+        /* Update_phi_Eqh with the face macros (expressions of face_bgk) */
+        double ex = o->xi[0][q], ey = o->xi[1][q], w = W9[q];
+        double u1 = (ex * U + ey * V) / rt;
+        double G = u1 + 0.5 * u1 * u1 - uu * o->lambda0;
+        double heqf = w * phi * (1.0 + u1), feqf;
+        if (o->c.model == MPD_MODEL_AC_HCZ) feqf = w * (p + rho * rt * G);
+        else feqf = w * (p / rt + rho * G) - (double)(q == 0) * p / rt;
+        hh[q] = heqf + hNEq * (o->h[q * N + n] - heq + o->hdt * hso);
+        ff[q] = feqf + fNEq * (o->f[q * N + n] - feq + o->hdt * fso);
+    }
+}
+
 static void phase4_faces(ora *o)
 {
     int nx = o->nx, ny = o->ny, N = o->N;
@@ -674,6 +735,7 @@ static void phase4_faces(ora *o)
                 /* Wall_3_BB (Ark/Cpp/Boundary.cpp:118-139): populations entering the
                  * fluid take the value of their bounce-back partner. */
                 int wall = o->walls && dir == 1 && (j == 0 || j == ny);
+                if (wall && o->c.bc_y == MPD_BC_WALL_NEE) wall_nee(o, &F, j, hh, ff);
                 if (wall) {
                     double h0[Q9], f0[Q9];
                     memcpy(h0, hh, sizeof(h0));

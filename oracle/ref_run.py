@@ -26,6 +26,8 @@ CONFIGS = {
     "hcz": ("D2Q9HCZ", "", False, ""),
     "acp": ("D2Q9AC", "", False, ""),
     "hcz_wall": ("D2Q9HCZ", "-D_Wall_3_BCs_FLIP=3 -D_Wall_3_BCs_BB", True, ""),
+    "hcz_nee": ("D2Q9HCZ", "-D_Wall_3_BCs_FLIP=3 -D_Wall_3_BCs_NEE", True, ""),
+    "acp_nee": ("D2Q9AC", "-D_Wall_3_BCs_FLIP=3 -D_Wall_3_BCs_NEE", True, ""),
     "hcz_uw": ("D2Q9HCZ", "-include " + os.path.join(CFGDIR, "uw", "flags.h"), False, ""),
     "acp_uw": ("D2Q9AC", "-include " + os.path.join(CFGDIR, "uw", "flags.h"), False, ""),
     "pp": ("D2Q9PP", "-include " + os.path.join(CFGDIR, "pp", "flags.h"), False,

@@ -31,6 +31,8 @@ REF_KEYS = {"phi": "Phi", "rho": "Rho", "u": "U", "v": "V", "p": "p", "phix": "P
 STEPS = [0, 1, 2, 10, 100]
 # name -> (model, steps recorded)
 CONFIGS = {"hcz": ("ac_hcz", STEPS), "acp": ("ac_p", STEPS), "hcz_wall": ("ac_hcz", STEPS),
+           # walls with non-equilibrium extrapolation (Wall_3_NEE)
+           "hcz_nee": ("ac_hcz", STEPS), "acp_nee": ("ac_p", STEPS),
            # UW face scheme: bit-exact at step 1; afterwards the reference's two copies
            # of each periodic face differ in the last bit (see tests/test_oracle_ref_live.py)
            "hcz_uw": ("ac_hcz", [0, 1]), "acp_uw": ("ac_p", [0, 1]),
@@ -51,11 +53,14 @@ def state_digests(d):
     return out
 
 
-def main():
+def main(names=None):
     from ref_run import load_cells, run
     for cfg, (model, steps) in CONFIGS.items():
+        if names and cfg not in names:
+            continue
         out = run(cfg, steps, max(steps))
-        rec = {"model": model, "walls_y": cfg.endswith("_wall"), "uw": cfg.endswith("_uw"),
+        rec = {"model": model, "walls_y": cfg.endswith(("_wall", "_nee")), "uw": cfg.endswith("_uw"),
+               "bc": "nee" if cfg.endswith("_nee") else ("bb" if cfg.endswith("_wall") else "periodic"),
                "nx": 128, "ny": 128,
                "source": "reference Ark, unmodified sources, OMP_NUM_THREADS=1, %s"
                          % ("SCMP_Drop" if cfg == "pp" else "AC_Drop"),
@@ -73,4 +78,4 @@ def main():
 
 
 if __name__ == "__main__":
-    main()
+    main(sys.argv[1:] or None)

@@ -12,11 +12,11 @@ def digest(a):
 
 
 def make_case(model="ac_hcz", nx=128, ny=128, flux=abi.MPD_FLUX_CD, walls=False, sign=+1,
-              radius=None, precision=abi.MPD_FP64, **over):
+              radius=None, precision=abi.MPD_FP64, nee=False, **over):
     m = abi.MODEL_NAMES[model]
     cfg = default_config(m, nx=nx, ny=ny, **over)
     cfg.flux = flux
-    cfg.bc_y = abi.MPD_BC_WALL_BB if walls else abi.MPD_BC_PERIODIC
+    cfg.bc_y = (abi.MPD_BC_WALL_NEE if nee else abi.MPD_BC_WALL_BB) if walls else abi.MPD_BC_PERIODIC
     cfg.precision = precision
     if m == abi.MPD_MODEL_PP_D2Q9:
         st = solver.scmp_drop(cfg, radius=radius if radius is not None else 0.25 * nx)

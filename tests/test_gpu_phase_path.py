@@ -19,6 +19,9 @@ CASES = [
     ("acp_cd_periodic_128", dict(model="ac_p")),
     ("hcz_cd_walls_64x96_droplet", dict(model="ac_hcz", nx=64, ny=96, walls=True, sign=-1, radius=20.0)),
     ("acp_cd_walls_96x64", dict(model="ac_p", nx=96, ny=64, walls=True, radius=20.0)),
+    # non-equilibrium-extrapolation walls, with the interface reaching the wall rows
+    ("hcz_cd_nee_64x48_droplet", dict(model="ac_hcz", nx=64, ny=48, walls=True, nee=True, sign=-1, radius=21.0)),
+    ("acp_cd_nee_64x48", dict(model="ac_p", nx=64, ny=48, walls=True, nee=True, radius=21.0)),
     ("hcz_uw_periodic_96x64", dict(model="ac_hcz", nx=96, ny=64, flux=abi.MPD_FLUX_UW, radius=20.0)),
     ("acp_uw_periodic_64", dict(model="ac_p", nx=64, ny=64, flux=abi.MPD_FLUX_UW, radius=20.0)),
     ("pp_cd_periodic_64", dict(model="pp_d2q9", nx=64, ny=64)),

@@ -2,8 +2,8 @@
 
 tests/golden/ref_*.json hold SHA-256 digests of the full cell state of the
 compiled reference (Ark, unmodified sources) after 0, 1, 2, 10 and 100 steps of
-its AC_Drop case on its 128x128 mesh (D2Q9HCZ, D2Q9AC, and D2Q9HCZ with
-bounce-back walls in y).  The host initialiser and the oracle must reproduce
+its AC_Drop case on its 128x128 mesh (D2Q9HCZ, D2Q9AC, and both with walls in
+y: bounce-back for HCZ, non-equilibrium extrapolation for HCZ and AC-p).  The host initialiser and the oracle must reproduce
 every digest bit for bit.
 """
 import json
@@ -16,7 +16,7 @@ from mpdugks_amd import abi, solver
 from helpers import digest, make_case, oracle_run
 
 GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
-CASES = ["hcz", "acp", "hcz_wall"]
+CASES = ["hcz", "acp", "hcz_wall", "hcz_nee", "acp_nee"]
 UW_CASES = ["hcz_uw", "acp_uw"]
 
 
@@ -40,7 +40,7 @@ def check_state(st, rec):
 @pytest.mark.parametrize("name", CASES + UW_CASES + ["pp"])
 def test_initial_state_matches_reference(name):
     g = load(name)
-    cfg, st = make_case(g["model"], walls=g["walls_y"])
+    cfg, st = make_case(g["model"], walls=g["walls_y"], nee=g.get("bc") == "nee")
     assert check_state(st, g["steps"]["0"]) == []
 
 
@@ -55,7 +55,7 @@ def test_oracle_uw_first_step_bit_exact(name, oracle_mod):
 @pytest.mark.parametrize("name", CASES)
 def test_oracle_bit_exact_vs_reference(name, oracle_mod):
     g = load(name)
-    cfg, st = make_case(g["model"], walls=g["walls_y"])
+    cfg, st = make_case(g["model"], walls=g["walls_y"], nee=g.get("bc") == "nee")
     steps = sorted(int(k) for k in g["steps"] if int(k) > 0)
     out = oracle_run(cfg, st, steps)
     for k in steps:
