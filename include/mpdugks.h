@@ -227,6 +227,27 @@ int mpd_init_pp_drop(const mpd_config *cfg, double cx, double cy, double radius,
 int mpd_init_pp_noise(const mpd_config *cfg, double rho0, double amp, uint64_t seed, double mu0,
                       double *tilde_f, double *macro, double *face_fx, double *face_fy);
 
+/* ---------------- start states built on the device ----------------
+ * The handle's state is computed on its GPU (no host arrays, no upload), as the
+ * host initialisers below compute it, with the device's tanh/sqrt (agreement to
+ * a few units in the last place).  Afterwards the handle is as after
+ * mpd_upload_state.  For drivers that create many cases (Preprocess.cpp:705-823
+ * builds them on the host cell by cell). */
+
+/* AC_Drop for 1..8 drops (Ark/Cpp/Preprocess.cpp:705-760): each drop's tanh
+ * profile of mpd_init_ac_drop, joined as the larger phi for droplets
+ * (sign = -1) and the smaller for bubbles (sign = +1). */
+int mpd_device_init_drops(mpd_handle *h, int32_t ndrops, const double *cx, const double *cy, const double *radius,
+                          int32_t sign, double p0, double mu0);
+
+/* mpd_init_ac_interface on the device; y0[nx] is a host array. */
+int mpd_device_init_interface(mpd_handle *h, const double *y0, int32_t sign, double p0);
+
+/* Pseudopotential noisy start: rho = rho0 (1 + U(-amp, amp)) per cell with a
+ * counter-based splitmix64 stream of (seed, cell) (not the host's mt19937_64
+ * sequence), u = 0, tilde = equilibrium, face forces 0. */
+int mpd_device_init_pp_noise(mpd_handle *h, double rho0, double amp, uint64_t seed, double mu0);
+
 /* ---------------- output files of the reference driver ----------------
  * Byte-identical writers of the files the reference's driver writes
  * (Ark/Cpp/Output.cpp), fed from downloaded macro planes.  Host code: no GPU

@@ -26,11 +26,17 @@ def rayleigh_taylor(nx=256, ny=1024, precision=abi.MPD_FP32_BULK, seed=0, width=
     cfg.sigma = sigma
     cfg.gy = g
     cfg.rho_ref = 0.5 * (rho_heavy + rho_light)
-    rng = np.random.default_rng(seed)
-    x = (np.arange(nx) + 0.5) * cfg.dx
-    y0 = 0.5 * ny * cfg.dx + 0.1 * W * np.cos(2 * np.pi * x / W) + rng.uniform(-1, 1, nx) * 1e-3 * W
-    st = solver.ac_interface(cfg, y0, sign=+1)
+    st = solver.ac_interface(cfg, rayleigh_taylor_y0(cfg, seed), sign=+1)
     return cfg, st
+
+
+def rayleigh_taylor_y0(cfg, seed=0):
+    """Interface heights of the Rayleigh-Taylor start: mid-height + 0.1 W cos(2 pi x / W)
+    + U(-1, 1) * 1e-3 W per column (W = nx, seeded)."""
+    W = float(cfg.nx)
+    rng = np.random.default_rng(seed)
+    x = (np.arange(cfg.nx) + 0.5) * cfg.dx
+    return 0.5 * cfg.ny * cfg.dx + 0.1 * W * np.cos(2 * np.pi * x / W) + rng.uniform(-1, 1, cfg.nx) * 1e-3 * W
 
 
 ENSEMBLE_SIDES = (64, 96, 128, 192, 256)
@@ -79,9 +85,7 @@ def coalescence_case(r1, r2, gap, precision=abi.MPD_FP32_BULK, width=4.0, sides=
     cfg.rho_v, cfg.rho_l = 1.0, ratio
     cfg.sigma = sigma
     cfg.width = width
-    c = 0.5 * side * cfg.dx
-    x1 = c - 0.5 * (2 * r1 + 2 * r2 + gap) + r1
-    x2 = x1 + r1 + gap + r2
+    (x1, c, _), (x2, _, _) = coalescence_drops(r1, r2, gap, cfg)
     a = solver.ac_drop(cfg, cx=x1, cy=c, radius=r1, sign=-1)
     b = solver.ac_drop(cfg, cx=x2, cy=c, radius=r2, sign=-1)
     # union of the two droplets: phi = max of the two profiles, then equilibrium
@@ -92,3 +96,26 @@ def coalescence_case(r1, r2, gap, precision=abi.MPD_FP32_BULK, width=4.0, sides=
         arr_a[:, sel] = arr_b[:, sel]
     assert np.allclose(a.m[abi.M["phi"]], phi)
     return cfg, a
+
+
+def coalescence_config(r1, r2, gap, precision=abi.MPD_FP32_BULK, width=4.0,
+                       sides=(64, 96, 128, 192, 256, 384, 512, 768, 1024), ratio=10.0, sigma=1e-3):
+    """The configuration of coalescence_case without building the state (for a
+    start built on the device): (cfg, drops)."""
+    need = 2 * (r1 + r2) + gap + 4 * width
+    side = next(s for s in sides if s >= need)
+    cfg = default_config(abi.MPD_MODEL_AC_HCZ, nx=side, ny=side)
+    cfg.precision = precision
+    cfg.rho_v, cfg.rho_l = 1.0, ratio
+    cfg.sigma = sigma
+    cfg.width = width
+    return cfg, coalescence_drops(r1, r2, gap, cfg)
+
+
+def coalescence_drops(r1, r2, gap, cfg):
+    """Centres and radii of the two droplets, side by side at mid-height with
+    `gap` cells between their interfaces: [(cx, cy, r), (cx, cy, r)]."""
+    c = 0.5 * cfg.nx * cfg.dx
+    x1 = c - 0.5 * (2 * r1 + 2 * r2 + gap) + r1
+    x2 = x1 + r1 + gap + r2
+    return [(x1, c, r1), (x2, c, r2)]

@@ -397,6 +397,60 @@ static int sync_macros(mpd_handle *h)
     return 0;
 }
 
+static int device_state_ready(mpd_handle *h)
+{
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(h->stream));
+    h->uploaded = true;
+    h->fused_primed = false;
+    h->prime_stage = 0;
+    return 0;
+}
+
+int mpd_device_init_drops(mpd_handle *h, int32_t ndrops, const double *cx, const double *cy, const double *radius,
+                          int32_t sign, double p0, double mu0)
+{
+    if (!h) return fail(MPD_ERR_INVALID, "null handle");
+    if (h->batch) return fail(MPD_ERR_STATE, "handle is a member of a batch");
+    if (is_pp(h->cfg)) return fail(MPD_ERR_INVALID, "drops start the Allen-Cahn models");
+    if (ndrops < 1 || ndrops > MAX_DROPS || !cx || !cy || !radius)
+        return fail(MPD_ERR_INVALID, "need 1..%d drops", MAX_DROPS);
+    DropSet D;
+    D.n = ndrops;
+    for (int d = 0; d < ndrops; ++d) {
+        D.cx[d] = cx[d];
+        D.cy[d] = cy[d];
+        D.r[d] = radius[d];
+    }
+    D.width = h->cfg.width;
+    devinit_drops(h->S, h->C, D, sign, p0, mu0, h->stream);
+    return device_state_ready(h);
+}
+
+int mpd_device_init_interface(mpd_handle *h, const double *y0, int32_t sign, double p0)
+{
+    if (!h) return fail(MPD_ERR_INVALID, "null handle");
+    if (h->batch) return fail(MPD_ERR_STATE, "handle is a member of a batch");
+    if (is_pp(h->cfg) || !y0) return fail(MPD_ERR_INVALID, "interface needs an Allen-Cahn model and y0[nx]");
+    double *d_y0 = nullptr;
+    HIPCHK(hipMalloc(&d_y0, sizeof(double) * (size_t)h->cfg.nx));
+    hipError_t e = hipMemcpyAsync(d_y0, y0, sizeof(double) * (size_t)h->cfg.nx, hipMemcpyHostToDevice, h->stream);
+    if (e == hipSuccess) devinit_interface(h->S, h->C, d_y0, sign, p0, h->cfg.width, h->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+    (void)hipFree(d_y0);
+    if (e != hipSuccess) return fail(MPD_ERR_DEVICE, "interface start: %s", hipGetErrorString(e));
+    return device_state_ready(h);
+}
+
+int mpd_device_init_pp_noise(mpd_handle *h, double rho0, double amp, uint64_t seed, double mu0)
+{
+    if (!h) return fail(MPD_ERR_INVALID, "null handle");
+    if (h->batch) return fail(MPD_ERR_STATE, "handle is a member of a batch");
+    if (!is_pp(h->cfg)) return fail(MPD_ERR_INVALID, "noise start is for the pseudopotential models");
+    devinit_noise(h->S, h->C, rho0, amp, seed, mu0, h->stream);
+    return device_state_ready(h);
+}
+
 int mpd_download_state(mpd_handle *h, double *tilde_h, double *tilde_f, double *macro, double *face_fx,
                        double *face_fy)
 {

@@ -24,4 +24,18 @@ struct DevState {
 struct Consts;
 void phase_step(const DevState &S, const Consts &C, hipStream_t st);
 
+// start states built on the device (devinit.hip)
+constexpr int MAX_DROPS = 8;
+struct DropSet {
+    int n;
+    double cx[MAX_DROPS], cy[MAX_DROPS], r[MAX_DROPS];
+    double width;
+};
+void devinit_drops(const DevState &S, const Consts &C, const DropSet &D, int sign, double p0, double mu0,
+                   hipStream_t st);
+void devinit_interface(const DevState &S, const Consts &C, const double *d_y0, int sign, double p0, double width,
+                       hipStream_t st);
+void devinit_noise(const DevState &S, const Consts &C, double rho0, double amp, uint64_t seed, double mu0,
+                   hipStream_t st);
+
 }  // namespace mpd

@@ -25,7 +25,7 @@ class Request:
     def __init__(self, rid, arrival, make, bytes_estimate, cells):
         self.rid = rid
         self.arrival = arrival
-        self.make = make              # () -> (cfg, state)
+        self.make = make              # () -> (cfg, state), or (cfg, drops) for a start built on the GPU
         self.bytes_estimate = bytes_estimate
         self.cells = cells
         self.admitted = None
@@ -48,7 +48,10 @@ class GpuEngine:
         from . import abi
         cfg, st = req.make()
         s = self._solver.Solver(cfg, path=abi.MPD_PATH_FUSED)
-        s.upload(st)
+        if isinstance(st, list):
+            s.init_drops(st, sign=-1)   # droplets built on the device: no host state, no upload
+        else:
+            s.upload(st)
         s.step(2)
         self.batch.add(s)
         req.handle = s
@@ -143,7 +146,8 @@ class Server:
 
 def coalescence_requests(n, rate, seed=0, max_side=1024):
     """n two-droplet requests with Poisson arrivals of mean rate `rate` (1/s):
-    radii U(10, 60) cells, gap U{2..8}; mesh from cases.coalescence_case."""
+    radii U(10, 60) cells, gap U{2..8}; mesh from cases.coalescence_config, the
+    droplets built on the GPU."""
     from . import cases
     rng = np.random.default_rng(seed)
     t = 0.0
@@ -156,7 +160,7 @@ def coalescence_requests(n, rate, seed=0, max_side=1024):
         side = next(s for s in (64, 96, 128, 192, 256, 384, 512, 768, 1024) if s >= need)
         if side > max_side:
             continue
-        make = (lambda a=r1, b=r2, g=gap: cases.coalescence_case(a, b, g))
+        make = (lambda a=r1, b=r2, g=gap: cases.coalescence_config(a, b, g))
         # device bytes: FP64 explicit state + phase scratch + two FP32 derived copies
         cells = side * side
         reqs.append(Request(k, t, make, bytes_estimate=cells * 1100, cells=cells))

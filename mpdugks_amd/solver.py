@@ -49,6 +49,10 @@ _SYMBOLS = {
                          C.c_int),
     "mpd_init_pp_noise": ([C.c_void_p, C.c_double, C.c_double, C.c_uint64, C.c_double, C.c_void_p,
                            C.c_void_p, C.c_void_p, C.c_void_p], C.c_int),
+    "mpd_device_init_drops": ([C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int32, C.c_double,
+                               C.c_double], C.c_int),
+    "mpd_device_init_interface": ([C.c_void_p, C.c_void_p, C.c_int32, C.c_double], C.c_int),
+    "mpd_device_init_pp_noise": ([C.c_void_p, C.c_double, C.c_double, C.c_uint64, C.c_double], C.c_int),
     "mpd_default_output_info": ([C.c_void_p, C.c_void_p], C.c_int),
     "mpd_output_name": ([C.c_int32, C.c_void_p, C.c_int64, C.c_double, C.c_char_p, C.c_int32], C.c_int),
     "mpd_write_flowfield": ([C.c_char_p, C.c_int32, C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p], C.c_int),
@@ -178,6 +182,20 @@ class Solver:
 
     def set_path(self, path):
         _check(lib().mpd_set_path(self._h, int(path)), "mpd_set_path")
+
+    def init_drops(self, drops, sign=-1, p0=1.0, mu0=0.1):
+        """Start state built on the GPU (mpd_device_init_drops): drops = [(cx, cy, r), ...]."""
+        d = np.ascontiguousarray(drops, dtype=np.float64).reshape(-1, 3)
+        cx, cy, r = (np.ascontiguousarray(d[:, k]) for k in range(3))
+        _check(lib().mpd_device_init_drops(self._h, len(d), _p(cx), _p(cy), _p(r), int(sign), p0, mu0),
+               "mpd_device_init_drops")
+
+    def init_interface(self, y0, sign=+1, p0=1.0):
+        y = np.ascontiguousarray(y0, dtype=np.float64)
+        _check(lib().mpd_device_init_interface(self._h, _p(y), int(sign), p0), "mpd_device_init_interface")
+
+    def init_pp_noise(self, rho0, amp=0.01, seed=0, mu0=0.1):
+        _check(lib().mpd_device_init_pp_noise(self._h, rho0, amp, int(seed), mu0), "mpd_device_init_pp_noise")
 
     def upload(self, s):
         arrs = [_f64(a) for a in (s.h, s.f, s.m, s.ffx, s.ffy)]
