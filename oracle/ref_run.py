@@ -51,7 +51,7 @@ def build(cfg):
     return os.path.join(HERE, "_ref", cfg, "ark")
 
 
-def run(cfg, steps, max_step, outdir=None, flowfield=False, timeout=3600, writers=False):
+def run(cfg, steps, max_step, outdir=None, flowfield=False, timeout=3600, writers=False, cell_order=None):
     """Run config `cfg`, dumping the cell state after each step in `steps`."""
     exe = build(cfg)
     base = os.path.join(HERE, "_ref", cfg, "run")
@@ -63,7 +63,7 @@ def run(cfg, steps, max_step, outdir=None, flowfield=False, timeout=3600, writer
     sys.path.insert(0, HERE)
     from casmesh import write_cas
     write_cas(os.path.join(home, "Mesh", MESH_NAME + ".cas"), 128, 128,
-              walls_y=CONFIGS[cfg][2])
+              walls_y=CONFIGS[cfg][2], cell_order=cell_order)
     outdir = outdir or os.path.join(base, "dump")
     if writers:  # fresh output tree: the convergence files are appended to
         import shutil
