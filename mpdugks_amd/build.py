@@ -21,6 +21,7 @@ SOURCES = {
     "phase_kernels.hip": ["-ffp-contract=off"],
     "fused_kernels.hip": ["-ffp-contract=off"],
     "fused_fp32.hip": ["-ffp-contract=fast"],
+    "pp32.hip": ["-ffp-contract=fast"],
     "devinit.hip": ["-ffp-contract=off"],
     "hostinit.cpp": ["-ffp-contract=off"],
     "writers.cpp": ["-ffp-contract=off"],

@@ -222,6 +222,10 @@ int check_cfg(const mpd_config *c)
         return fail(MPD_ERR_INVALID, "walls are supported with CD and the Allen-Cahn models only");
     if (c->precision != MPD_FP64 && c->precision != MPD_FP32_BULK)
         return fail(MPD_ERR_INVALID, "unknown precision %d", c->precision);
+    if (c->precision == MPD_FP32_BULK && !fused_supported(*c))
+        return fail(MPD_ERR_INVALID, "no FP32-bulk kernel for this model / flux / boundary (FP32-bulk: Allen-Cahn "
+                                     "with CD faces and periodic or bounce-back y; pseudopotential D2Q9 with CD "
+                                     "faces, periodic)");
     if (!(c->dx > 0) || !(c->dt > 0) || !(c->rt > 0)) return fail(MPD_ERR_INVALID, "dx, dt, rt must be > 0");
     if (c->phi_l == c->phi_v || !(c->width > 0)) return fail(MPD_ERR_INVALID, "phi_l == phi_v or width <= 0");
     return 0;
@@ -500,6 +504,11 @@ int mpd_step(mpd_handle *h, int64_t nsteps)
             if (r) return fail(MPD_ERR_NOMEM, "fused path allocation failed");
         }
         int64_t s = 0;
+        if (!h->fused_primed && !fused_needs_priming(h->cfg)) {
+            // explicit state carried as is (FP32-bulk pseudopotential)
+            fused_convert(h->F, h->S, h->C, 0, h->stream);
+            h->fused_primed = true;
+        }
         if (!h->fused_primed) {
             // The derived state needs two explicit steps: the first step after an
             // upload uses the uploaded sources (not derivable from phi), and the
@@ -668,7 +677,7 @@ int mpd_batch_add(mpd_batch *b, mpd_handle *h)
 {
     if (!b || !h) return fail(MPD_ERR_INVALID, "null argument");
     if (h->batch) return fail(MPD_ERR_STATE, "handle already belongs to a batch");
-    if (h->cfg.precision != MPD_FP32_BULK || !fused_supported(h->cfg))
+    if (h->cfg.precision != MPD_FP32_BULK || !fused_supported(h->cfg) || is_pp(h->cfg))
         return fail(MPD_ERR_INVALID, "batches take FP32-bulk cases of the fused path (Allen-Cahn, CD)");
     if (!h->fused_primed || h->effective_path != MPD_PATH_FUSED)
         return fail(MPD_ERR_STATE, "step the case at least twice on the fused path before adding it");

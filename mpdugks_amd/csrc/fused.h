@@ -10,6 +10,22 @@ namespace mpd {
 
 struct Consts;
 
+// FP32-bulk pseudopotential D2Q9 planes (pp32.hip)
+struct PP32Planes {
+    int nx, ny;
+    int64_t N;
+    double *rho;                                  // [N] conserved density (FP64)
+    float *f, *fbp;                               // [9][N]
+    float *U, *V, *tau, *Fx, *Fy, *psi, *scg;     // [N]
+    float *ffx, *ffy;                             // face force [2N]
+    float *fpsi, *frho, *fU, *fV, *fscg, *fmass;  // face scratch [2N]
+    float *ffl;                                   // face fluxes [9][2N]
+};
+int pp32_alloc(PP32Planes &P, void **mem, int64_t *bytes, const Consts &C, int64_t N);
+void pp32_from64(const PP32Planes &P, const DevState &S, hipStream_t st);
+void pp32_to64(const PP32Planes &P, const DevState &S, const Consts &C, hipStream_t st);
+void pp32_step(const PP32Planes &P, const Consts &C, hipStream_t st);
+
 struct FusedState {
     bool ready = false;
     int64_t bytes = 0;
@@ -23,6 +39,8 @@ struct FusedState {
     int64_t ncell = 0;
     bool fp32 = false;
     double p_ref = 0.0;  // FP32: f~ stored minus the rest equilibrium at p_ref
+    bool pp = false;     // FP32-bulk pseudopotential: explicit FP32 planes, no derived state
+    PP32Planes pp32 = {};
 };
 
 // FP32 planes of the derived state (see fused_kernels.hip)
@@ -49,6 +67,8 @@ void batch32_step(const Batch32Plan *P, int parity, hipStream_t st);
 FusedPlanes32 fused32_planes(FusedState &F, int which);
 
 bool fused_supported(const mpd_config &cfg);
+// the path starts from two explicit FP64 steps (the Allen-Cahn derived state)
+bool fused_needs_priming(const mpd_config &cfg);
 int fused_alloc(FusedState &F, const DevState &S, const Consts &C, const mpd_config &cfg);
 void fused_free(FusedState &F);
 int64_t fused_bytes(const FusedState &F);

@@ -16,6 +16,7 @@
 // Compiled with FP contraction enabled.
 #include <hip/hip_runtime.h>
 
+#include <cmath>
 #include <vector>
 
 #include "fused.h"
@@ -93,10 +94,9 @@ __device__ __forceinline__ int tile_of_block(int b, int ntiles)
 
 __device__ __forceinline__ float rcp(float x) { return __builtin_amdgcn_rcpf(x); }
 
-// An LDS index the compiler cannot relate to other indices: loads of the 18
-// planes at one site then pair as two-plane reads from one base register
-// (plane stride a multiple of 64 dwords) instead of pairing neighbouring sites
-// of one plane, which needs a base register per plane.
+// An LDS index the compiler cannot relate to other indices, so that it does
+// not pair the reads of neighbouring sites of one plane into two-address
+// instructions (each such pair keeps a base register per plane live).
 __device__ __forceinline__ int opaque(int x)
 {
     asm("" : "+v"(x));
@@ -838,11 +838,14 @@ K32 make_k32(const Consts &C)
     k.gx = (float)C.gx;
     k.gy = (float)C.gy;
     k.rho_ref = (float)C.rho_ref;
-    // weights rounded to FP32 with w0 = 1 - 4 w1 - 4 w2 so that they sum to 1 exactly
-    const float w1 = 1.f / 9.f, w2 = 1.f / 36.f;
-    k.w[0] = 1.f - 4.f * w1 - 4.f * w2;
-    k.w[1] = w1;
-    k.w[2] = w2;
+    // FP32 weights that sum to 1 exactly: w1 = fl(1/9) (a multiple of 2^-27),
+    // w2 = 1/36 rounded to a multiple of 2^-27 (one ulp below fl(1/36)), so
+    // that w0 = 1 - 4 w1 - 4 w2 is a multiple of 2^-25 in [1/4, 1/2): exact in
+    // FP32 (computed in FP64, where every step is exact)
+    const double w1d = (double)(1.f / 9.f), w2d = std::ldexp(std::nearbyint(std::ldexp(1.0 / 36.0, 27)), -27);
+    k.w[0] = (float)(1.0 - 4.0 * w1d - 4.0 * w2d);
+    k.w[1] = (float)w1d;
+    k.w[2] = (float)w2d;
     const double wd[3] = {4.0 / 9.0, 1.0 / 9.0, 1.0 / 36.0};
     const double bbp_h = 1.5 * C.dt / (2.0 * C.taum + C.dt), cbp_h = C.taum * bbp_h;
     const double bh_h = C.hdt / (2.0 * C.taum + C.hdt), ch_h = C.taum * bh_h;

@@ -540,17 +540,32 @@ FusedPlanes32 fused32_planes(FusedState &F, int which)
 
 bool fused_supported(const mpd_config &cfg)
 {
+    // FP32-bulk pseudopotential D2Q9: the FP32 phase kernels of pp32.hip
+    if (cfg.model == MPD_MODEL_PP_D2Q9)
+        return cfg.precision == MPD_FP32_BULK && cfg.flux == MPD_FLUX_CD && cfg.bc_y == MPD_BC_PERIODIC;
     // walls: bounce-back only (the non-equilibrium extrapolation needs the
     // cell equilibria at the wall row, which the tile does not keep)
     return (cfg.model == MPD_MODEL_AC_HCZ || cfg.model == MPD_MODEL_AC_P) && cfg.flux == MPD_FLUX_CD &&
            cfg.bc_y != MPD_BC_WALL_NEE && cfg.nx >= 4 && cfg.ny >= 4;
 }
 
+bool fused_needs_priming(const mpd_config &cfg) { return cfg.model != MPD_MODEL_PP_D2Q9; }
+
 int fused_alloc(FusedState &F, const DevState &S, const Consts &C, const mpd_config &cfg)
 {
     fused_free(F);
     F.ncell = S.N;
     F.fp32 = cfg.precision == MPD_FP32_BULK;
+    F.pp = cfg.model == MPD_MODEL_PP_D2Q9;
+    if (F.pp) {
+        void *p = nullptr;
+        int64_t b = 0;
+        if (pp32_alloc(F.pp32, &p, &b, C, S.N)) return -1;
+        F.mem[F.nmem++] = p;
+        F.bytes = b;
+        F.ready = true;
+        return 0;
+    }
     // h~(9) f~(9) phi, + phi_lag (HCZ, or walls for the ghost rows), + ppu ppv (HCZ)
     F.nplanes = cfg.model == MPD_MODEL_AC_HCZ ? 22 : (C.walls ? 20 : 19);
     // FP32 keeps f as a deviation from the rest equilibrium at p_ref = 1 (p0 of
@@ -590,6 +605,10 @@ int64_t fused_bytes(const FusedState &F) { return F.bytes; }
 
 void fused_convert(FusedState &F, const DevState &S, const Consts &C, int which, hipStream_t st)
 {
+    if (F.pp) {
+        pp32_from64(F.pp32, S, st);
+        return;
+    }
     if (F.fp32) convert_t<float>(F, S, C, which, st);
     else convert_t<double>(F, S, C, which, st);
 }
@@ -597,14 +616,16 @@ void fused_convert(FusedState &F, const DevState &S, const Consts &C, int which,
 void fused_step(FusedState &F, const DevState &S, const Consts &C, const mpd_config &cfg, int first, hipStream_t st)
 {
     (void)S; (void)cfg; (void)first;
-    if (F.fp32) step_t<float>(F, C, st);
+    if (F.pp) pp32_step(F.pp32, C, st);
+    else if (F.fp32) step_t<float>(F, C, st);
     else step_t<double>(F, C, st);
 }
 
 int fused_materialize(FusedState &F, const DevState &S, const Consts &C, const mpd_config &cfg, hipStream_t st)
 {
     (void)cfg;
-    if (F.fp32) materialize_t<float>(F, S, C, st);
+    if (F.pp) pp32_to64(F.pp32, S, C, st);
+    else if (F.fp32) materialize_t<float>(F, S, C, st);
     else materialize_t<double>(F, S, C, st);
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }

@@ -58,3 +58,22 @@ def test_interface_initialiser_profile():
     phi = st.field("phi")
     assert phi[0, 0] < 1e-3 and phi[-1, 0] > 1 - 1e-3           # phase L above
     assert np.allclose(st.h.sum(axis=0), st.m[abi.M["phi"]], rtol=0, atol=1e-15)
+
+
+def test_fp32_bulk_without_a_kernel_is_refused_at_create():
+    """FP32-bulk exists for the Allen-Cahn models (CD; periodic or bounce-back y)
+    and the pseudopotential D2Q9 model (CD, periodic); other combinations are
+    refused by mpd_create before any device call."""
+    L = solver.lib()
+    bad = [
+        (abi.MPD_MODEL_AC_HCZ, dict(flux=abi.MPD_FLUX_UW)),
+        (abi.MPD_MODEL_AC_HCZ, dict(bc_y=abi.MPD_BC_WALL_NEE)),
+        (abi.MPD_MODEL_PP_D2Q9, dict(flux=abi.MPD_FLUX_UW)),
+        (abi.MPD_MODEL_PP_D2V16, dict()),
+    ]
+    for model, over in bad:
+        cfg = default_config(model, nx=32, ny=32, **over)
+        cfg.precision = abi.MPD_FP32_BULK
+        h = C.c_void_p()
+        assert L.mpd_create(C.byref(cfg), C.byref(h)) == -1, (model, over)  # MPD_ERR_INVALID
+        assert b"FP32-bulk" in L.mpd_last_error()
