@@ -34,6 +34,7 @@ struct FusedState {
     // ping-pong state planes (element type depends on precision)
     void *st[2] = {nullptr, nullptr};
     void *gl[2] = {nullptr, nullptr};  // walls: lagged ghost phi rows [2][nx]
+    double *phd[2] = {nullptr, nullptr};  // FP32-bulk: the conserved phi in FP64 [N]
     int cur = 0;
     int nplanes = 0;
     int64_t ncell = 0;
@@ -43,9 +44,11 @@ struct FusedState {
     PP32Planes pp32 = {};
 };
 
-// FP32 planes of the derived state (see fused_kernels.hip)
+// FP32 planes of the derived state (see fused_kernels.hip); phd is the
+// conserved phase field in FP64, phi its FP32 copy for the stencils
 struct FusedPlanes32 {
     float *h, *f, *phi, *phl, *ppu, *ppv, *gl;
+    double *phd;
 };
 // one FP32-bulk step (fused_fp32.hip)
 void fused32_step(const FusedPlanes32 &in, const FusedPlanes32 &out, const Consts &C, int64_t N, hipStream_t st);

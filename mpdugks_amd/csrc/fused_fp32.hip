@@ -119,14 +119,23 @@ __device__ __forceinline__ void st(rsrc_t r, uint32_t n, float v, uint32_t plane
 {
     __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(uint32_t, v), r, n * 4u, plane_bytes, 0);
 }
+__device__ __forceinline__ double ld64(rsrc_t r, uint32_t n)
+{
+    return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, n * 8u, 0, 0));
+}
+__device__ __forceinline__ void st64(rsrc_t r, uint32_t n, double v)
+{
+    using u2 = unsigned int __attribute__((ext_vector_type(2)));
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u2, v), r, n * 8u, 0, 0);
+}
 
 // The planes of one state set are consecutive in one allocation (h~ 0-8,
 // f~ 9-17, phi 18, phi_lag 19, prevPhiU 20, prevPhiV 21; fused_kernels.hip
 // planes_of), so one resource covers them and a plane is a uniform offset.
-// The lagged ghost-phi rows (walls) are a separate array.
+// The lagged ghost-phi rows (walls) and the FP64 phase field are separate arrays.
 enum : uint32_t { P_H = 0, P_F = 9, P_PHI = 18, P_PHL = 19, P_PPU = 20, P_PPV = 21 };
 struct Rs {
-    rsrc_t a, gl;
+    rsrc_t a, gl, d;
     uint32_t pb;  // bytes per plane
 };
 __device__ __forceinline__ Rs make_rs(const FusedPlanes32 &p, int64_t N, int nx, int nplanes)
@@ -135,6 +144,7 @@ __device__ __forceinline__ Rs make_rs(const FusedPlanes32 &p, int64_t N, int nx,
     r.pb = (uint32_t)(N * 4);
     r.a = make_rsrc(p.h, (uint32_t)nplanes * r.pb);
     r.gl = make_rsrc(p.gl ? p.gl : p.h, p.gl ? (uint32_t)(2 * nx * 4) : 0u);
+    r.d = make_rsrc((const float *)p.phd, (uint32_t)(N * 8));
     return r;
 }
 
@@ -287,8 +297,7 @@ __device__ __forceinline__ void cell_load(const Rs &in, int nx, int ny, bool sma
 
 template <int MODEL, int WALLS>
 __device__ __forceinline__ void cell_bp(const K32 &k, const Rs &out, int nx, int ny, int rx, int ry, const CellIn &x,
-                                        float2 *S, const float *s_phi, const float *s_phl, float *base, float &phi_out,
-                                        bool interior)
+                                        float2 *S, const float *s_phi, const float *s_phl, float *base, bool interior)
 {
     constexpr bool HCZ = MODEL == MPD_MODEL_AC_HCZ;
     if (!x.valid) return;
@@ -298,7 +307,6 @@ __device__ __forceinline__ void cell_bp(const K32 &k, const Rs &out, int nx, int
     const int c = r1(rx, ry);
     const Stencil sti = stencil_idx(rx + 1, ry + 1);
     const float phi = s_phi[sti.c[1]];
-    phi_out = phi;
     const float rho = k.aphi * phi + k.bphi;
     Site m;
     float p, bbp, tau = 0.f;
@@ -676,12 +684,12 @@ __device__ __forceinline__ void tile32(const K32 &k, int nx, int ny, const Fused
     __syncthreads();
 
     // ---- stage B ----
-    float base[17], phi_old = 0.f;
-    cell_bp<MODEL, WALLS>(k, out, nx, ny, rx1, ry1, x1, s_big, s_phi, s_phl, base, phi_old, inside);
+    float base[17];
+    cell_bp<MODEL, WALLS>(k, out, nx, ny, rx1, ry1, x1, s_big, s_phi, s_phl, base, inside);
     if (tid < NR2) {
         if (!PRELOAD_RING) cell_load<MODEL, WALLS>(in, nx, ny, small, i0, j0, rx2, ry2, x2);
-        float dummy[17], dphi;
-        cell_bp<MODEL, WALLS>(k, out, nx, ny, rx2, ry2, x2, s_big, s_phi, s_phl, dummy, dphi, false);
+        float dummy[17];
+        cell_bp<MODEL, WALLS>(k, out, nx, ny, rx2, ry2, x2, s_big, s_phi, s_phl, dummy, false);
     }
     __syncthreads();
 
@@ -757,13 +765,16 @@ __device__ __forceinline__ void tile32(const K32 &k, int nx, int ny, const Fused
         dh[yp[z]] += va.x - vb.x;
         df[yp[z]] += va.y - vb.y;
     }
-    // phi is the conserved variable: it moves only by the antisymmetric face
-    // mass fluxes (each face's sum over populations, shared by both cells), and
-    // h~_0 takes the remainder so that sum_q h~ = phi.  In exact arithmetic this
-    // is the same update; in FP32 it keeps the rounding of the per-population
-    // products from drifting the phase-field mass.
-    const float dm = (s_fx[6 * FS + kW].x - s_fx[6 * FS + kE].x) + (s_fy[6 * FS + kS].x - s_fy[6 * FS + kN].x);
-    const float phi = fmaf(k.dtvf, dm, phi_old);
+    // phi is the conserved variable, kept in FP64: it moves only by the
+    // antisymmetric face mass fluxes (each face's sum over populations, shared
+    // by both cells, differenced in FP64), and h~_0 takes the remainder so that
+    // sum_q h~ = phi.  In exact arithmetic this is the same update; here it
+    // keeps the rounding of the per-population products, and of an FP32 phi,
+    // from drifting the phase-field mass.  The FP32 copy feeds the stencils.
+    const double dm = ((double)s_fx[6 * FS + kW].x - (double)s_fx[6 * FS + kE].x) +
+                      ((double)s_fy[6 * FS + kS].x - (double)s_fy[6 * FS + kN].x);
+    const double phd = ld64(in.d, n) + (double)k.dtvf * dm;
+    const float phi = (float)phd;
     float rest = 0.f;
 #pragma unroll
     for (int q = 1; q < 9; ++q) {
@@ -771,7 +782,8 @@ __device__ __forceinline__ void tile32(const K32 &k, int nx, int ny, const Fused
         st(out.a, n, hn, (P_H + q) * out.pb);
         rest += hn;
     }
-    st(out.a, n, phi - rest, P_H * out.pb);
+    st(out.a, n, (float)(phd - (double)rest), P_H * out.pb);
+    st64(out.d, n, phd);
 #pragma unroll
     for (int q = 0; q < 9; ++q) st(out.a, n, fmaf(k.dtvf, df[q], base[8 + q]), (P_F + q) * out.pb);
     st(out.a, n, phi, P_PHI * out.pb);

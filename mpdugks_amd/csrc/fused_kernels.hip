@@ -46,6 +46,7 @@ static_assert(NXF <= NTHREADS && NYF <= NTHREADS && TX * TY <= NTHREADS, "tile t
 template <typename T>
 struct Planes {
     T *h, *f, *phi, *phl, *ppu, *ppv, *gl;
+    double *phd;  // FP32-bulk: the conserved phi in FP64 (nullptr for the FP64 state)
 };
 
 // periodic index for any offset (small meshes may wrap more than once per tile)
@@ -370,6 +371,7 @@ __global__ void __launch_bounds__(256) k_to_derived(Consts C, DevState S, Planes
             P.f[q * N + n] = (T)(S.f[q * N + n] - base);
         }
         P.phi[n] = (T)S.m[MPD_M_PHI * N + n];
+        if (P.phd) P.phd[n] = S.m[MPD_M_PHI * N + n];
     }
 }
 
@@ -428,8 +430,9 @@ __global__ void __launch_bounds__(256) k_materialize(Consts C, DevState S, Plane
     T mu;
     derive_cell<T, MODEL>(C, ht, ft, pl, lc, ld, ppu, ppv, pc, pd, m, s, mu);
     double *M = S.m;
-    M[MPD_M_PHI * N + n] = (double)m.phi;
-    M[MPD_M_RHO * N + n] = (double)m.rho;
+    // FP32-bulk: phi and rho from the conserved FP64 phase field
+    M[MPD_M_PHI * N + n] = P.phd ? P.phd[n] : (double)m.phi;
+    M[MPD_M_RHO * N + n] = P.phd ? C.aphi * P.phd[n] + C.bphi : (double)m.rho;
     M[MPD_M_U * N + n] = (double)m.U;
     M[MPD_M_V * N + n] = (double)m.V;
     M[MPD_M_P * N + n] = (double)m.p + p_ref;
@@ -465,6 +468,7 @@ Planes<T> planes_of(FusedState &F, int which)
     P.ppu = F.nplanes > 20 ? b + 20 * N : nullptr;
     P.ppv = F.nplanes > 21 ? b + 21 * N : nullptr;
     P.gl = F.gl[which] ? (T *)F.gl[which] : nullptr;
+    P.phd = F.phd[which];
     return P;
 }
 
@@ -487,7 +491,8 @@ template <>
 void step_t<float>(FusedState &F, const Consts &C, hipStream_t st)
 {
     Planes<float> a = planes_of<float>(F, F.cur), b = planes_of<float>(F, 1 - F.cur);
-    FusedPlanes32 in{a.h, a.f, a.phi, a.phl, a.ppu, a.ppv, a.gl}, out{b.h, b.f, b.phi, b.phl, b.ppu, b.ppv, b.gl};
+    FusedPlanes32 in{a.h, a.f, a.phi, a.phl, a.ppu, a.ppv, a.gl, a.phd},
+        out{b.h, b.f, b.phi, b.phl, b.ppu, b.ppv, b.gl, b.phd};
     fused32_step(in, out, C, F.ncell, st);
     F.cur = 1 - F.cur;
 }
@@ -535,7 +540,7 @@ void convert_t(FusedState &F, const DevState &S, const Consts &C, int which, hip
 FusedPlanes32 fused32_planes(FusedState &F, int which)
 {
     Planes<float> a = planes_of<float>(F, which);
-    return FusedPlanes32{a.h, a.f, a.phi, a.phl, a.ppu, a.ppv, a.gl};
+    return FusedPlanes32{a.h, a.f, a.phi, a.phl, a.ppu, a.ppv, a.gl, a.phd};
 }
 
 bool fused_supported(const mpd_config &cfg)
@@ -579,6 +584,13 @@ int fused_alloc(FusedState &F, const DevState &S, const Consts &C, const mpd_con
         F.st[w] = p;
         F.bytes += (int64_t)(es * (size_t)F.nplanes * S.N);
         F.gl[w] = nullptr;
+        F.phd[w] = nullptr;
+        if (F.fp32) {
+            if (hipMalloc(&p, sizeof(double) * (size_t)S.N) != hipSuccess) { fused_free(F); return -1; }
+            F.mem[F.nmem++] = p;
+            F.phd[w] = (double *)p;
+            F.bytes += (int64_t)(sizeof(double) * (size_t)S.N);
+        }
         if (C.walls && cfg.model == MPD_MODEL_AC_HCZ) {
             if (hipMalloc(&p, es * 2 * (size_t)C.nx) != hipSuccess) { fused_free(F); return -1; }
             F.mem[F.nmem++] = p;
@@ -599,6 +611,7 @@ void fused_free(FusedState &F)
     F.ready = false;
     F.st[0] = F.st[1] = nullptr;
     F.gl[0] = F.gl[1] = nullptr;
+    F.phd[0] = F.phd[1] = nullptr;
 }
 
 int64_t fused_bytes(const FusedState &F) { return F.bytes; }

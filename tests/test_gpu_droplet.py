@@ -11,7 +11,8 @@ FP64 oracle's state after 10^3 and 10^4 steps (tests/golden/make_droplet_golden.
   the thermodynamic one and the droplet is still ringing at 10^4 steps; the
   record pins the value, not Laplace's law.
 * FP32-bulk (stated tolerances): Delta p within 1e-3 and the largest spurious
-  speed within 1e-2 of the FP64 values; sum(phi) within 1e-5.  Pointwise,
+  speed within 1e-2 of the FP64 values; sum(phi) within 1e-9 (the phase field
+  is carried in FP64 and moved only by antisymmetric face fluxes).  Pointwise,
   rho, phi and p within 1e-5 relative L2 after 10^3 steps; after 10^4 steps
   the problem's own conditioning sets the floor: the FP64 oracle started from
   a state perturbed at FP32 round-off size (relative 6e-8) ends 2.3e-5 (rho)
@@ -98,7 +99,7 @@ def test_fp32_bulk_laplace_and_spurious_currents(runs, k):
           % (k, o["dp"], g["dp"], o["umax"], g["umax"], o["sum_phi"], g["sum_phi"]))
     assert abs(o["dp"] / g["dp"] - 1) < 1e-3
     assert abs(o["umax"] / g["umax"] - 1) < 1e-2
-    assert abs(o["sum_phi"] / g["sum_phi"] - 1) < 1e-5
+    assert abs(o["sum_phi"] / g["sum_phi"] - 1) < 1e-9
 
 
 def test_fp32_bulk_fields_vs_fp64(runs):
