@@ -285,6 +285,20 @@ def main():
     def rel(a, b):
         return float(np.sqrt(((a - b) ** 2).sum()) / max(np.sqrt((b ** 2).sum()), 1e-300))
     err = {k: rel(g32.m[abi.M[k]], ref64.m[abi.M[k]]) for k in ("rho", "phi", "p")}
+    # the case's own sensitivity: the FP64 fused run from the start state with
+    # every population multiplied by 1 + 6e-8 U(-1, 1) (FP32 round-off size),
+    # after the same number of steps
+    from mpdugks_amd import solver
+    cfg64, st64 = cases.rayleigh_taylor(nx=nx, ny=ny, precision=abi.MPD_FP64)
+    rng = np.random.default_rng(1)
+    st64.h = st64.h * (1 + 6e-8 * rng.uniform(-1, 1, st64.h.shape))
+    st64.f = st64.f * (1 + 6e-8 * rng.uniform(-1, 1, st64.f.shape))
+    sp = solver.Solver(cfg64, path=abi.MPD_PATH_FUSED)
+    sp.upload(st64)
+    sp.step(args.warmup + results["fp64_fused"]["steps"])
+    pert = sp.download()
+    sp.close()
+    err_floor = {k: rel(pert.m[abi.M[k]], ref64.m[abi.M[k]]) for k in ("rho", "phi", "p")}
 
     bytes_alg = 2 * S_ALG_HCZ * 4  # FP32-bulk
     traffic = None
@@ -320,6 +334,7 @@ def main():
         "fp64_fused_mlups": round(results["fp64_fused"]["mlups"], 2),
         "fp32_fused_kernel_mlups": round(head["kernel_mlups"], 2),
         "fp32_vs_fp64_rel_l2": err,
+        "fp64_perturbed_vs_fp64_rel_l2": err_floor,
     }
     if not args.no_ensemble:
         ens = ensemble_bench(args.ensemble_cases, args.ensemble_steps)

@@ -23,8 +23,10 @@ struct PP32Planes {
 };
 int pp32_alloc(PP32Planes &P, void **mem, int64_t *bytes, const Consts &C, int64_t N);
 void pp32_from64(const PP32Planes &P, const DevState &S, hipStream_t st);
-void pp32_to64(const PP32Planes &P, const DevState &S, const Consts &C, hipStream_t st);
-void pp32_step(const PP32Planes &P, const Consts &C, hipStream_t st);
+// force_stale: the stored cell force is the previous step's (every step but the
+// first after a conversion); the step or the conversion recomputes it from Psi
+void pp32_to64(const PP32Planes &P, const DevState &S, const Consts &C, bool force_stale, hipStream_t st);
+void pp32_step(const PP32Planes &P, const Consts &C, bool force_stale, hipStream_t st);
 
 struct FusedState {
     bool ready = false;
@@ -42,6 +44,7 @@ struct FusedState {
     double p_ref = 0.0;  // FP32: f~ stored minus the rest equilibrium at p_ref
     bool pp = false;     // FP32-bulk pseudopotential: explicit FP32 planes, no derived state
     PP32Planes pp32 = {};
+    bool pp_force_stale = false;  // the stored cell force predates the last update
 };
 
 // FP32 planes of the derived state (see fused_kernels.hip); phd is the

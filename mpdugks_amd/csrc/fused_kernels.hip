@@ -620,6 +620,7 @@ void fused_convert(FusedState &F, const DevState &S, const Consts &C, int which,
 {
     if (F.pp) {
         pp32_from64(F.pp32, S, st);
+        F.pp_force_stale = false;
         return;
     }
     if (F.fp32) convert_t<float>(F, S, C, which, st);
@@ -629,7 +630,10 @@ void fused_convert(FusedState &F, const DevState &S, const Consts &C, int which,
 void fused_step(FusedState &F, const DevState &S, const Consts &C, const mpd_config &cfg, int first, hipStream_t st)
 {
     (void)S; (void)cfg; (void)first;
-    if (F.pp) pp32_step(F.pp32, C, st);
+    if (F.pp) {
+        pp32_step(F.pp32, C, F.pp_force_stale, st);
+        F.pp_force_stale = true;
+    }
     else if (F.fp32) step_t<float>(F, C, st);
     else step_t<double>(F, C, st);
 }
@@ -637,7 +641,10 @@ void fused_step(FusedState &F, const DevState &S, const Consts &C, const mpd_con
 int fused_materialize(FusedState &F, const DevState &S, const Consts &C, const mpd_config &cfg, hipStream_t st)
 {
     (void)cfg;
-    if (F.pp) pp32_to64(F.pp32, S, C, st);
+    if (F.pp) {
+        pp32_to64(F.pp32, S, C, F.pp_force_stale, st);
+        F.pp_force_stale = false;
+    }
     else if (F.fp32) materialize_t<float>(F, S, C, st);
     else materialize_t<double>(F, S, C, st);
     return hipGetLastError() == hipSuccess ? 0 : -1;

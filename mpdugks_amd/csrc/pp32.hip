@@ -35,9 +35,29 @@ __device__ __forceinline__ int wrapp(int i, int n) { return i < 0 ? i + n : (i >
 
 inline unsigned blocks(size_t n) { return (unsigned)((n + 255) / 256); }
 
+// Update_PseudoForce(cell) (PseudoP.cpp:31-38): F = -scG Psi grad(Psi), from
+// the Psi that the last update left
+__device__ __forceinline__ void pp32_cell_force(const PP32Planes &P, const Consts &C, size_t n, float &Fx, float &Fy)
+{
+    const int nx = P.nx, ny = P.ny;
+    const int i = (int)(n % nx), j = (int)(n / nx);
+    auto S = [&](int a, int b) { return P.psi[(size_t)wrapp(b, ny) * nx + wrapp(a, nx)]; };
+    const float c[4] = {S(i + 1, j), S(i, j + 1), S(i - 1, j), S(i, j - 1)};
+    const float d[4] = {S(i + 1, j + 1), S(i - 1, j + 1), S(i - 1, j - 1), S(i + 1, j - 1)};
+    float gx, gy;
+    grad9<float>(C, c, d, gx, gy);
+    const float sp = -P.scg[n] * P.psi[n];
+    Fx = gx * sp;
+    Fy = gy * sp;
+}
+
 // phase 1: equilibria, He-Shan-Doolen source and bar-plus of every cell
 // (Update_phi_Eq / Update_DVDF_Source / Update_phi_BP: D2Q9PP.cpp:51-62,137-145,
-// DUGKS2DSolver.cpp:379-401)
+// DUGKS2DSolver.cpp:379-401).  FORCE: first the cell force that the previous
+// step's last phase (Update_PseudoForce) produced, from Psi, and store it for
+// the update; without FORCE the stored force is used (the first step after a
+// conversion from the explicit state, whose force may not derive from Psi).
+template <bool FORCE>
 __global__ void __launch_bounds__(256) k_pp32_phase1(PP32Planes P, Consts C)
 {
     const size_t n = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -48,8 +68,14 @@ __global__ void __launch_bounds__(256) k_pp32_phase1(PP32Planes P, Consts C)
     cm.U = P.U[n];
     cm.V = P.V[n];
     cm.tau = P.tau[n];
-    cm.Fx = P.Fx[n];
-    cm.Fy = P.Fy[n];
+    if (FORCE) {
+        pp32_cell_force(P, C, n, cm.Fx, cm.Fy);
+        P.Fx[n] = cm.Fx;
+        P.Fy[n] = cm.Fy;
+    } else {
+        cm.Fx = P.Fx[n];
+        cm.Fy = P.Fy[n];
+    }
     float ft[Q9], fb[Q9], ht[Q9] = {}, hb[Q9];
 #pragma unroll
     for (int q = 0; q < Q9; ++q) ft[q] = P.f[q * N + n];
@@ -85,8 +111,12 @@ __global__ void __launch_bounds__(256) k_pp32_face_psi(PP32Planes P, Consts C)
     const size_t fi = dir ? t - N : t;
     const int i = (int)(fi % P.nx), j = (int)(fi / P.nx);
     float fb[Q9];
+    float *fv = P.ffl + (dir ? (size_t)Q9 * N : 0);
 #pragma unroll
-    for (int q = 0; q < Q9; ++q) fb[q] = pp32_face_value(P, C, dir, q, i, j);
+    for (int q = 0; q < Q9; ++q) {
+        fb[q] = pp32_face_value(P, C, dir, q, i, j);
+        fv[(size_t)q * N + fi] = fb[q];  // the flux pass reads it back and overwrites it with the flux
+    }
     float s = 0.f, sx = 0.f, sy = 0.f;
 #pragma unroll
     for (int q = 0; q < Q9; ++q) s += fb[q];
@@ -111,7 +141,7 @@ __global__ void __launch_bounds__(256) k_pp32_face_psi(PP32Planes P, Consts C)
 // faces, pass 2: face force from the face-lattice gradient of Psi
 // (update_MQ_x/y(Face_2D*), GradSchemeBasic.h:181-208; Update_PseudoForce(face),
 // PseudoP.cpp:39-46), half-step BGK (DUGKS2DSolver.cpp:402-423) and micro-flux
-// (:424-441)
+// (:424-441) of the face values that pass 1 left in the flux planes
 __global__ void __launch_bounds__(256) k_pp32_face_flux(PP32Planes P, Consts C)
 {
     const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -143,7 +173,7 @@ __global__ void __launch_bounds__(256) k_pp32_face_flux(PP32Planes P, Consts C)
     float mass = 0.f;
 #pragma unroll
     for (int q = 0; q < Q9; ++q) {
-        const float fb = pp32_face_value(P, C, dir, q, i, j);
+        const float fb = fl[(size_t)q * N + fi];
         float hh, ff;
         face_bgk<float, MPD_MODEL_PP_D2Q9>(C, fm, q, 0.f, fb, ah, bh, ch, hh, ff);
         const float xin = (float)(C.dx * C.xi[dir][q]);
@@ -204,21 +234,16 @@ __global__ void __launch_bounds__(256) k_pp32_update(PP32Planes P, Consts C)
     P.scg[n] = (float)scg;
 }
 
-// Update_PseudoForce(cell) (PseudoP.cpp:31-38): F = -scG Psi grad(Psi)
+// the cell force of the current state (the last phase of the reference step),
+// for downloads: the steps themselves compute it at the start of the next step
 __global__ void __launch_bounds__(256) k_pp32_force(PP32Planes P, Consts C)
 {
     const size_t n = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (n >= (size_t)P.N) return;
-    const int nx = P.nx, ny = P.ny;
-    const int i = (int)(n % nx), j = (int)(n / nx);
-    auto S = [&](int a, int b) { return P.psi[(size_t)wrapp(b, ny) * nx + wrapp(a, nx)]; };
-    const float c[4] = {S(i + 1, j), S(i, j + 1), S(i - 1, j), S(i, j - 1)};
-    const float d[4] = {S(i + 1, j + 1), S(i - 1, j + 1), S(i - 1, j - 1), S(i + 1, j - 1)};
-    float gx, gy;
-    grad9<float>(C, c, d, gx, gy);
-    const float sp = -P.scg[n] * P.psi[n];
-    P.Fx[n] = gx * sp;
-    P.Fy[n] = gy * sp;
+    float Fx, Fy;
+    pp32_cell_force(P, C, n, Fx, Fy);
+    P.Fx[n] = Fx;
+    P.Fy[n] = Fy;
 }
 
 // explicit FP64 state -> FP32 planes
@@ -313,19 +338,20 @@ void pp32_from64(const PP32Planes &P, const DevState &S, hipStream_t st)
     hipLaunchKernelGGL(k_pp32_from64, dim3(blocks(P.N)), dim3(256), 0, st, P, S);
 }
 
-void pp32_to64(const PP32Planes &P, const DevState &S, const Consts &C, hipStream_t st)
+void pp32_to64(const PP32Planes &P, const DevState &S, const Consts &C, bool force_stale, hipStream_t st)
 {
+    if (force_stale) hipLaunchKernelGGL(k_pp32_force, dim3(blocks(P.N)), dim3(256), 0, st, P, C);
     hipLaunchKernelGGL(k_pp32_to64, dim3(blocks(P.N)), dim3(256), 0, st, P, S, C);
 }
 
-void pp32_step(const PP32Planes &P, const Consts &C, hipStream_t st)
+void pp32_step(const PP32Planes &P, const Consts &C, bool force_stale, hipStream_t st)
 {
     const size_t N = P.N;
-    hipLaunchKernelGGL(k_pp32_phase1, dim3(blocks(N)), dim3(256), 0, st, P, C);
+    if (force_stale) hipLaunchKernelGGL(k_pp32_phase1<true>, dim3(blocks(N)), dim3(256), 0, st, P, C);
+    else hipLaunchKernelGGL(k_pp32_phase1<false>, dim3(blocks(N)), dim3(256), 0, st, P, C);
     hipLaunchKernelGGL(k_pp32_face_psi, dim3(blocks(2 * N)), dim3(256), 0, st, P, C);
     hipLaunchKernelGGL(k_pp32_face_flux, dim3(blocks(2 * N)), dim3(256), 0, st, P, C);
     hipLaunchKernelGGL(k_pp32_update, dim3(blocks(N)), dim3(256), 0, st, P, C);
-    hipLaunchKernelGGL(k_pp32_force, dim3(blocks(N)), dim3(256), 0, st, P, C);
 }
 
 }  // namespace mpd
