@@ -187,20 +187,36 @@ __global__ void __launch_bounds__(RED_THREADS) k_reduce(const double *m, double 
         vmax = fmax(vmax, sqrt(U * U + V * V));
         if (update) rho1k[n] = rho;
     }
-    __shared__ double sh[RED_FIELDS][RED_THREADS];
-    for (int k = 0; k < 7; ++k) sh[k][threadIdx.x] = s[k] + c[k];
-    sh[7][threadIdx.x] = vmax;
-    sh[8][threadIdx.x] = bad;
-    __syncthreads();
-    for (int w = RED_THREADS / 2; w > 0; w >>= 1) {
-        if ((int)threadIdx.x < w) {
-            for (int k = 0; k < 7; ++k) sh[k][threadIdx.x] += sh[k][threadIdx.x + w];
-            sh[7][threadIdx.x] = fmax(sh[7][threadIdx.x], sh[7][threadIdx.x + w]);
-            sh[8][threadIdx.x] += sh[8][threadIdx.x + w];
+    // across the 64 lanes of each wavefront: butterfly shuffles, each level
+    // adding the partner's (sum, compensation) pair with an exact two-sum
+    for (int off = 32; off > 0; off >>= 1) {
+        for (int k = 0; k < 7; ++k) {
+            const double so = __shfl_xor(s[k], off, 64), co = __shfl_xor(c[k], off, 64);
+            neumaier(s[k], c[k], so);
+            c[k] += co;
         }
-        __syncthreads();
+        vmax = fmax(vmax, __shfl_xor(vmax, off, 64));
+        bad += __shfl_xor(bad, off, 64);
     }
-    if (threadIdx.x < RED_FIELDS) part[(size_t)blockIdx.x * RED_FIELDS + threadIdx.x] = sh[threadIdx.x][0];
+    // across the wavefronts of the workgroup
+    constexpr int NW = RED_THREADS / 64;
+    __shared__ double sh[RED_FIELDS][NW];
+    const int lane = threadIdx.x % 64, wave = threadIdx.x / 64;
+    if (lane == 0) {
+        for (int k = 0; k < 7; ++k) sh[k][wave] = s[k] + c[k];
+        sh[7][wave] = vmax;
+        sh[8][wave] = bad;
+    }
+    __syncthreads();
+    if (threadIdx.x < RED_FIELDS) {
+        const int k = threadIdx.x;
+        double t = sh[k][0], e = 0.0;
+        for (int w = 1; w < NW; ++w) {
+            if (k == 7) t = fmax(t, sh[k][w]);
+            else neumaier(t, e, sh[k][w]);
+        }
+        part[(size_t)blockIdx.x * RED_FIELDS + k] = t + e;
+    }
 }
 
 int num_q(const mpd_config &c) { return c.model == MPD_MODEL_PP_D2V16 ? d2v16::Q : Q9; }
