@@ -354,8 +354,8 @@ int mpd_set_path(mpd_handle *h, int32_t path)
     if (path < MPD_PATH_AUTO || path > MPD_PATH_FUSED) return fail(MPD_ERR_INVALID, "unknown path %d", path);
     if (path == MPD_PATH_FUSED && !fused_supported(h->cfg))
         return fail(MPD_ERR_INVALID, "no fused kernel for this configuration");
-    if (h->cfg.precision == MPD_FP32_BULK && path == MPD_PATH_PHASES)
-        return fail(MPD_ERR_INVALID, "the phase path is FP64 only");
+    if (h->cfg.precision == MPD_FP32_BULK && path == MPD_PATH_PHASES && !is_pp(h->cfg))
+        return fail(MPD_ERR_INVALID, "the phase path is FP64 only (and FP32 for the pseudopotential model)");
     h->path = path;
     return 0;
 }
@@ -500,6 +500,10 @@ int mpd_step(mpd_handle *h, int64_t nsteps)
     if (nsteps == 0) return 0;
     int path = h->path;
     if (path == MPD_PATH_AUTO) path = fused_supported(h->cfg) ? MPD_PATH_FUSED : MPD_PATH_PHASES;
+    // FP32-bulk pseudopotential: the phase path is the same FP32 state stepped
+    // with one launch per phase instead of one per step
+    const bool pp32_phases = h->cfg.precision == MPD_FP32_BULK && is_pp(h->cfg) && path == MPD_PATH_PHASES;
+    if (pp32_phases) path = MPD_PATH_FUSED;
     if (h->cfg.precision == MPD_FP32_BULK && path != MPD_PATH_FUSED)
         return fail(MPD_ERR_INVALID, "FP32-bulk needs the fused path");
     if (path != h->effective_path) {
@@ -519,6 +523,7 @@ int mpd_step(mpd_handle *h, int64_t nsteps)
             int r = fused_alloc(h->F, h->S, h->C, h->cfg);
             if (r) return fail(MPD_ERR_NOMEM, "fused path allocation failed");
         }
+        h->F.pp32.phases = pp32_phases;
         int64_t s = 0;
         if (!h->fused_primed && !fused_needs_priming(h->cfg)) {
             // explicit state carried as is (FP32-bulk pseudopotential)

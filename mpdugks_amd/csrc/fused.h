@@ -21,12 +21,23 @@ struct PP32Planes {
     float *fpsi, *frho, *fU, *fV, *fscg, *fmass;  // face scratch [2N]
     float *ffl;                                   // face fluxes [9][2N]
 };
-int pp32_alloc(PP32Planes &P, void **mem, int64_t *bytes, const Consts &C, int64_t N);
-void pp32_from64(const PP32Planes &P, const DevState &S, hipStream_t st);
+// the carried planes of one state set: the one-launch step reads the current
+// set (with its halos) and writes the other
+struct PP32Set {
+    double *rho;
+    float *f, *U, *V, *Fx, *Fy, *psi, *scg, *ffx, *ffy;
+};
+struct PP32State {
+    PP32Planes P;         // the current set, tau, and the scratch of the per-phase kernels
+    PP32Set other;        // the second set
+    bool phases = false;  // step with one launch per phase instead of the one-launch tile kernel
+};
+int pp32_alloc(PP32State &S, void **mem, int64_t *bytes, const Consts &C, int64_t N);
+void pp32_from64(const PP32State &S, const DevState &D, hipStream_t st);
 // force_stale: the stored cell force is the previous step's (every step but the
 // first after a conversion); the step or the conversion recomputes it from Psi
-void pp32_to64(const PP32Planes &P, const DevState &S, const Consts &C, bool force_stale, hipStream_t st);
-void pp32_step(const PP32Planes &P, const Consts &C, bool force_stale, hipStream_t st);
+void pp32_to64(const PP32State &S, const DevState &D, const Consts &C, bool force_stale, hipStream_t st);
+void pp32_step(PP32State &S, const Consts &C, bool force_stale, hipStream_t st);
 
 struct FusedState {
     bool ready = false;
@@ -43,7 +54,7 @@ struct FusedState {
     bool fp32 = false;
     double p_ref = 0.0;  // FP32: f~ stored minus the rest equilibrium at p_ref
     bool pp = false;     // FP32-bulk pseudopotential: explicit FP32 planes, no derived state
-    PP32Planes pp32 = {};
+    PP32State pp32;
     bool pp_force_stale = false;  // the stored cell force predates the last update
 };
 

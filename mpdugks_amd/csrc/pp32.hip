@@ -294,35 +294,308 @@ __global__ void __launch_bounds__(256) k_pp32_to64(PP32Planes P, DevState S, Con
     }
 }
 
+// ---------------- one launch per step ----------------
+//
+// One 16x15 tile per 256-thread workgroup does the four phases above for its
+// cells, reading the current set with halos and writing the other set:
+//  S0: Psi and scG on the tile plus three cells (the cell force of a bar-plus cell);
+//  S1: cell force and bar-plus on the tile plus two cells (the cells a face-Psi face reads);
+//  S2: face macros and Psi on the x- and y-faces within one face of the tile's faces
+//      (the tile's own faces also keep their nine face values);
+//  S3: face force, BGK and flux of the tile's own faces (17x15 x-faces, 16x16 y-faces);
+//  S4: update of the tile's cells.
+// Every value is computed with the same device functions as the per-phase
+// kernels, from the same inputs.
+constexpr int FTX = 16, FTY = 15, FNT = 256;
+constexpr int QX = FTX + 6, QY = FTY + 6, QN = QX * QY;           // Psi region: -3 .. +2
+constexpr int RX = FTX + 4, RY = FTY + 4, RN = RX * RY;           // bar-plus region: -2 .. +1
+constexpr int XPX = FTX + 3, XPY = FTY + 2, XPN = XPX * XPY;      // x-face Psi region: fx -1..TX+1, fy -1..TY
+constexpr int YPX = FTX + 2, YPY = FTY + 3, YPN = YPX * YPY;      // y-face Psi region: fx -1..TX, fy -1..TY+1
+constexpr int XFX = FTX + 1, XFN = XFX * FTY;                     // flux x-faces: fx 0..TX, fy 0..TY-1
+constexpr int YFN = FTX * (FTY + 1);                              // flux y-faces: fx 0..TX-1, fy 0..TY
+constexpr int FN = XFN + YFN;
+static_assert(FTX * FTY <= FNT, "one tile cell per thread in the update");
+// the six populations with a non-zero normal velocity on x- and y-faces (the
+// other three have no flux), in increasing q; slot of q, or -1
+constexpr int MX[6] = {1, 2, 4, 5, 6, 8}, MY[6] = {2, 3, 4, 6, 7, 8};
+__host__ __device__ constexpr int slot_x(int q) { return q == 1 ? 0 : q == 2 ? 1 : q == 4 ? 2 : q == 5 ? 3 : q == 6 ? 4 : q == 8 ? 5 : -1; }
+__host__ __device__ constexpr int slot_y(int q) { return q == 2 ? 0 : q == 3 ? 1 : q == 4 ? 2 : q == 6 ? 3 : q == 7 ? 4 : q == 8 ? 5 : -1; }
+
+__device__ __forceinline__ int wrapn(int i, int n)
+{
+    const int r = i % n;
+    return r < 0 ? r + n : r;
+}
+
+__device__ __forceinline__ int fused_tile_of_block(int b, int ntiles)
+{
+    const int per = (ntiles + 7) / 8;
+    const int t = (b % 8) * per + b / 8;  // XCD k (block b mod 8) takes a contiguous band of tiles
+    return t < ntiles ? t : -1;
+}
+
+// CD face value of population q from the bar-plus region (index of cell (cx, cy))
+__device__ __forceinline__ float fused_face_value(const Consts &C, const float *bp, int dir, int q, int fx, int fy)
+{
+    auto B = [&](int cx, int cy) { return bp[(cy + 2) * RX + (cx + 2)]; };
+    if (dir == 0) {
+        const int ox = fx - 1;
+        return cd_face_value<float>(C, 0, q, B(ox, fy), B(fx, fy), B(fx, fy + 1), B(fx, fy - 1), B(ox, fy + 1),
+                                    B(ox, fy - 1));
+    }
+    const int oy = fy - 1;
+    return cd_face_value<float>(C, 1, q, B(fx, oy), B(fx, fy), B(fx + 1, fy), B(fx - 1, fy), B(fx + 1, oy),
+                                B(fx - 1, oy));
+}
+
+template <bool FORCE>
+__global__ void __launch_bounds__(FNT) k_pp32_tile(PP32Planes in, PP32Set out, Consts C, int ntx, int ntiles)
+{
+    __shared__ float s_bp[Q9 * RN];
+    __shared__ float s_F[2][FTX * FTY];
+    __shared__ float s_fm[5][XPN + YPN];   // face Psi, scG, rho, U, V
+    __shared__ float s_fl[6 * FN];         // face values of the tile's faces, then their fluxes
+    // cell Psi and scG (S0-S1) live in the face-macro array until S2 writes it
+    float *const s_psi = &s_fm[0][0], *const s_scg = s_psi + QN;
+    static_assert(2 * QN <= 5 * (XPN + YPN), "Psi staging must fit in the face-macro array");
+    const int tile = fused_tile_of_block(blockIdx.x, ntiles);
+    if (tile < 0) return;
+    const int nx = in.nx, ny = in.ny;
+    const size_t N = in.N;
+    const int i0 = (tile % ntx) * FTX, j0 = (tile / ntx) * FTY;
+    const int tid = threadIdx.x;
+
+    // S0: Psi, scG
+    for (int q = tid; q < QN; q += FNT) {
+        const int cx = q % QX - 3, cy = q / QX - 3;
+        const size_t n = (size_t)wrapn(j0 + cy, ny) * nx + wrapn(i0 + cx, nx);
+        s_psi[q] = in.psi[n];
+        s_scg[q] = in.scg[n];
+    }
+    __syncthreads();
+
+    // S1: cell force and bar-plus
+    for (int r = tid; r < RN; r += FNT) {
+        const int cx = r % RX - 2, cy = r / RX - 2;
+        const size_t n = (size_t)wrapn(j0 + cy, ny) * nx + wrapn(i0 + cx, nx);
+        CellMacro<float> cm = {};
+        cm.rho = (float)in.rho[n];
+        cm.U = in.U[n];
+        cm.V = in.V[n];
+        cm.tau = in.tau[n];
+        if (FORCE) {
+            auto S = [&](int a, int b) { return s_psi[(b + 3) * QX + (a + 3)]; };
+            const float c[4] = {S(cx + 1, cy), S(cx, cy + 1), S(cx - 1, cy), S(cx, cy - 1)};
+            const float d[4] = {S(cx + 1, cy + 1), S(cx - 1, cy + 1), S(cx - 1, cy - 1), S(cx + 1, cy - 1)};
+            float gx, gy;
+            grad9<float>(C, c, d, gx, gy);
+            const int qi = (cy + 3) * QX + (cx + 3);
+            const float sp = -s_scg[qi] * s_psi[qi];
+            cm.Fx = gx * sp;
+            cm.Fy = gy * sp;
+        } else {
+            cm.Fx = in.Fx[n];
+            cm.Fy = in.Fy[n];
+        }
+        if (cx >= 0 && cx < FTX && cy >= 0 && cy < FTY) {
+            s_F[0][cy * FTX + cx] = cm.Fx;
+            s_F[1][cy * FTX + cx] = cm.Fy;
+        }
+        float ft[Q9], fb[Q9], ht[Q9] = {}, hb[Q9];
+#pragma unroll
+        for (int q = 0; q < Q9; ++q) ft[q] = in.f[q * N + n];
+        cell_barplus<float, MPD_MODEL_PP_D2Q9>(C, cm, ht, ft, hb, fb);
+#pragma unroll
+        for (int q = 0; q < Q9; ++q) s_bp[q * RN + r] = fb[q];
+    }
+    __syncthreads();
+
+    // S2: face macros with the face's previous force, and face Psi (FP64 EOS)
+    for (int k = tid; k < XPN + YPN; k += FNT) {
+        const int dir = k < XPN ? 0 : 1;
+        int fx, fy;
+        if (dir == 0) { fx = k % XPX - 1; fy = k / XPX - 1; }
+        else { fx = (k - XPN) % YPX - 1; fy = (k - XPN) / YPX - 1; }
+        const size_t gf = (dir ? N : 0) + (size_t)wrapn(j0 + fy, ny) * nx + wrapn(i0 + fx, nx);
+        float fb[Q9];
+#pragma unroll
+        for (int q = 0; q < Q9; ++q) fb[q] = fused_face_value(C, s_bp + q * RN, dir, q, fx, fy);
+        // a face of the tile keeps its values for the flux stage
+        int kf = -1;
+        if (dir == 0 && fx >= 0 && fx <= FTX && fy >= 0 && fy < FTY) kf = fy * XFX + fx;
+        if (dir == 1 && fx >= 0 && fx < FTX && fy >= 0 && fy <= FTY) kf = XFN + fy * FTX + fx;
+        if (kf >= 0) {
+#pragma unroll
+            for (int z = 0; z < 6; ++z) s_fl[z * FN + kf] = fb[dir ? MY[z] : MX[z]];
+        }
+        float s = 0.f, sx = 0.f, sy = 0.f;
+#pragma unroll
+        for (int q = 0; q < Q9; ++q) s += fb[q];
+#pragma unroll
+        for (int q = 0; q < Q9; ++q) sx += fb[q] * (float)C.xi[0][q];
+#pragma unroll
+        for (int q = 0; q < Q9; ++q) sy += fb[q] * (float)C.xi[1][q];
+        const float rho = s;
+        float U = sx / rho, V = sy / rho;
+        U += 0.5f * (float)C.hdt * in.ffx[gf] / rho;
+        V += 0.5f * (float)C.hdt * in.ffy[gf] / rho;
+        const double p = C.pp_k * carnahan_starling((double)rho, C.t_cs);
+        double scg, psi;
+        calc_psi(p, (double)rho, C.rt, scg, psi);
+        s_fm[0][k] = (float)psi;
+        s_fm[1][k] = (float)scg;
+        s_fm[2][k] = rho;
+        s_fm[3][k] = U;
+        s_fm[4][k] = V;
+    }
+    __syncthreads();
+
+    // S3: face force, half-step BGK and micro-flux of the tile's faces
+    const float ah = (float)C.ah_pp, bh = (float)C.bh_pp, ch = (float)C.ch_pp;
+    for (int k = tid; k < FN; k += FNT) {
+        const int dir = k < XFN ? 0 : 1;
+        int fx, fy, pk;
+        if (dir == 0) { fx = k % XFX; fy = k / XFX; pk = (fy + 1) * XPX + (fx + 1); }
+        else { fx = (k - XFN) % FTX; fy = (k - XFN) / FTX; pk = XPN + (fy + 1) * YPX + (fx + 1); }
+        const int pw = dir ? YPX : XPX;
+        auto S = [&](int a, int b) { return s_fm[0][pk + b * pw + a]; };
+        const float c[4] = {S(1, 0), S(0, 1), S(-1, 0), S(0, -1)};
+        const float d[4] = {S(1, 1), S(-1, 1), S(-1, -1), S(1, -1)};
+        float gx, gy;
+        grad9<float>(C, c, d, gx, gy);
+        const float sp = -s_fm[1][pk] * s_fm[0][pk];
+        gx *= sp;
+        gy *= sp;
+        const bool own = (dir == 0 ? fx < FTX : fy < FTY) && i0 + fx < nx && j0 + fy < ny;
+        if (own) {
+            const size_t gf = (dir ? N : 0) + (size_t)(j0 + fy) * nx + (i0 + fx);
+            out.ffx[gf] = gx;
+            out.ffy[gf] = gy;
+        }
+        FaceMacro<float> fm = {};
+        fm.rho = s_fm[2][pk];
+        fm.U = s_fm[3][pk];
+        fm.V = s_fm[4][pk];
+        fm.Fx = gx;
+        fm.Fy = gy;
+        // the populations with a zero normal velocity carry no flux
+        if (dir == 0) {
+#pragma unroll
+            for (int z = 0; z < 6; ++z) {
+                float hh, ff;
+                face_bgk<float, MPD_MODEL_PP_D2Q9>(C, fm, MX[z], 0.f, s_fl[z * FN + k], ah, bh, ch, hh, ff);
+                s_fl[z * FN + k] = ff * (float)(C.dx * C.xi[0][MX[z]]);
+            }
+        } else {
+#pragma unroll
+            for (int z = 0; z < 6; ++z) {
+                float hh, ff;
+                face_bgk<float, MPD_MODEL_PP_D2Q9>(C, fm, MY[z], 0.f, s_fl[z * FN + k], ah, bh, ch, hh, ff);
+                s_fl[z * FN + k] = ff * (float)(C.dx * C.xi[1][MY[z]]);
+            }
+        }
+    }
+    __syncthreads();
+
+    // S4: Update_phi_T, the conserved FP64 density, Update_MacroVar, Psi
+    const int cx = tid % FTX, cy = tid / FTX;
+    if (tid >= FTX * FTY || i0 + cx >= nx || j0 + cy >= ny) return;
+    const size_t n = (size_t)(j0 + cy) * nx + (i0 + cx);
+    const int kW = cy * XFX + cx, kE = kW + 1;
+    const int kS = XFN + cy * FTX + cx, kN = kS + FTX;
+    const int rc = (cy + 2) * RX + (cx + 2);
+    const float aTP = 4.f / 3.f, bTP = 1.f / 3.f, dtv = (float)C.dtv;
+    float ft[Q9];
+    double rest = 0.0;
+#pragma unroll
+    for (int q = 0; q < Q9; ++q) {
+        const int zx = slot_x(q), zy = slot_y(q);
+        float s = 0.f;
+        if (zx >= 0) s += -s_fl[zx * FN + kE];
+        if (zy >= 0) s += -s_fl[zy * FN + kN];
+        if (zx >= 0) s += s_fl[zx * FN + kW];
+        if (zy >= 0) s += s_fl[zy * FN + kS];
+        ft[q] = aTP * s_bp[q * RN + rc] - bTP * in.f[q * N + n] + dtv * s;
+        if (q > 0) rest += (double)ft[q];
+    }
+    // each face's mass flux: the sum of its population fluxes in increasing q
+    auto mass = [&](int k) {
+        float m = 0.f;
+#pragma unroll
+        for (int z = 0; z < 6; ++z) m += s_fl[z * FN + k];
+        return m;
+    };
+    const double dm = ((double)mass(kW) - (double)mass(kE)) + ((double)mass(kS) - (double)mass(kN));
+    const double rho_d = in.rho[n] + C.dtv * dm;
+    ft[0] = (float)(rho_d - rest);
+#pragma unroll
+    for (int q = 0; q < Q9; ++q) out.f[q * N + n] = ft[q];
+    float sx = 0.f, sy = 0.f;
+#pragma unroll
+    for (int q = 0; q < Q9; ++q) sx += ft[q] * (float)C.xi[0][q];
+#pragma unroll
+    for (int q = 0; q < Q9; ++q) sy += ft[q] * (float)C.xi[1][q];
+    const float rho = (float)rho_d;
+    float U = sx / rho, V = sy / rho;
+    U += (float)C.hdt * s_F[0][cy * FTX + cx] / rho;
+    V += (float)C.hdt * s_F[1][cy * FTX + cx] / rho;
+    const double p = C.pp_k * carnahan_starling(rho_d, C.t_cs);
+    double scg, psi;
+    calc_psi(p, rho_d, C.rt, scg, psi);
+    out.rho[n] = rho_d;
+    out.U[n] = U;
+    out.V[n] = V;
+    out.psi[n] = (float)psi;
+    out.scg[n] = (float)scg;
+}
+
 }  // namespace
 
-int pp32_alloc(PP32Planes &P, void **mem, int64_t *bytes, const Consts &C, int64_t N)
+int pp32_alloc(PP32State &St, void **mem, int64_t *bytes, const Consts &C, int64_t N)
 {
-    // FP64 density [N]; FP32 cell planes: f 9, bar-plus 9, U V tau Fx Fy Psi
-    // scG 7; FP32 face planes (2N each): force 2, Psi rho U V scG mass 6, fluxes 9
-    const size_t nf = (size_t)(9 + 9 + 7) * N + (size_t)(2 + 6 + 9) * 2 * N;
-    const size_t bytes_total = (size_t)N * sizeof(double) + nf * sizeof(float);
+    // two sets of: FP64 density [N]; FP32 f 9, U V Fx Fy Psi scG 6 [N], face
+    // force 2 [2N]; then tau [N] and the per-phase scratch: bar-plus 9 [N], face
+    // Psi rho U V scG mass 6 [2N], fluxes 9 [2N]
+    const size_t n = (size_t)N;
+    const size_t set_f = 15 * n + 4 * n, scratch_f = n + 9 * n + 12 * n + 18 * n;
+    const size_t bytes_total = 2 * n * sizeof(double) + (2 * set_f + scratch_f) * sizeof(float);
     void *base = nullptr;
     if (hipMalloc(&base, bytes_total) != hipSuccess) return -1;
     *mem = base;
     *bytes = (int64_t)bytes_total;
+    PP32Planes &P = St.P;
     P.nx = C.nx;
     P.ny = C.ny;
     P.N = N;
-    const size_t n = (size_t)N;
-    P.rho = (double *)base;
-    float *p = (float *)(P.rho + n);
-    P.f = p; p += 9 * n;
-    P.fbp = p; p += 9 * n;
-    P.U = p; p += n;
-    P.V = p; p += n;
+    double *d = (double *)base;
+    float *p = (float *)(d + 2 * n);
+    PP32Set sets[2];
+    for (int w = 0; w < 2; ++w) {
+        PP32Set &S = sets[w];
+        S.rho = d + w * n;
+        S.f = p; p += 9 * n;
+        S.U = p; p += n;
+        S.V = p; p += n;
+        S.Fx = p; p += n;
+        S.Fy = p; p += n;
+        S.psi = p; p += n;
+        S.scg = p; p += n;
+        S.ffx = p; p += 2 * n;
+        S.ffy = p; p += 2 * n;
+    }
+    P.rho = sets[0].rho;
+    P.f = sets[0].f;
+    P.U = sets[0].U;
+    P.V = sets[0].V;
+    P.Fx = sets[0].Fx;
+    P.Fy = sets[0].Fy;
+    P.psi = sets[0].psi;
+    P.scg = sets[0].scg;
+    P.ffx = sets[0].ffx;
+    P.ffy = sets[0].ffy;
+    St.other = sets[1];
     P.tau = p; p += n;
-    P.Fx = p; p += n;
-    P.Fy = p; p += n;
-    P.psi = p; p += n;
-    P.scg = p; p += n;
-    P.ffx = p; p += 2 * n;
-    P.ffy = p; p += 2 * n;
+    P.fbp = p; p += 9 * n;
     P.fpsi = p; p += 2 * n;
     P.frho = p; p += 2 * n;
     P.fU = p; p += 2 * n;
@@ -333,25 +606,47 @@ int pp32_alloc(PP32Planes &P, void **mem, int64_t *bytes, const Consts &C, int64
     return 0;
 }
 
-void pp32_from64(const PP32Planes &P, const DevState &S, hipStream_t st)
+void pp32_from64(const PP32State &St, const DevState &S, hipStream_t st)
 {
-    hipLaunchKernelGGL(k_pp32_from64, dim3(blocks(P.N)), dim3(256), 0, st, P, S);
+    hipLaunchKernelGGL(k_pp32_from64, dim3(blocks(St.P.N)), dim3(256), 0, st, St.P, S);
 }
 
-void pp32_to64(const PP32Planes &P, const DevState &S, const Consts &C, bool force_stale, hipStream_t st)
+void pp32_to64(const PP32State &St, const DevState &S, const Consts &C, bool force_stale, hipStream_t st)
 {
-    if (force_stale) hipLaunchKernelGGL(k_pp32_force, dim3(blocks(P.N)), dim3(256), 0, st, P, C);
-    hipLaunchKernelGGL(k_pp32_to64, dim3(blocks(P.N)), dim3(256), 0, st, P, S, C);
+    if (force_stale) hipLaunchKernelGGL(k_pp32_force, dim3(blocks(St.P.N)), dim3(256), 0, st, St.P, C);
+    hipLaunchKernelGGL(k_pp32_to64, dim3(blocks(St.P.N)), dim3(256), 0, st, St.P, S, C);
 }
 
-void pp32_step(const PP32Planes &P, const Consts &C, bool force_stale, hipStream_t st)
+void pp32_step(PP32State &St, const Consts &C, bool force_stale, hipStream_t st)
 {
+    PP32Planes &P = St.P;
     const size_t N = P.N;
-    if (force_stale) hipLaunchKernelGGL(k_pp32_phase1<true>, dim3(blocks(N)), dim3(256), 0, st, P, C);
-    else hipLaunchKernelGGL(k_pp32_phase1<false>, dim3(blocks(N)), dim3(256), 0, st, P, C);
-    hipLaunchKernelGGL(k_pp32_face_psi, dim3(blocks(2 * N)), dim3(256), 0, st, P, C);
-    hipLaunchKernelGGL(k_pp32_face_flux, dim3(blocks(2 * N)), dim3(256), 0, st, P, C);
-    hipLaunchKernelGGL(k_pp32_update, dim3(blocks(N)), dim3(256), 0, st, P, C);
+    if (St.phases) {
+        if (force_stale) hipLaunchKernelGGL(k_pp32_phase1<true>, dim3(blocks(N)), dim3(256), 0, st, P, C);
+        else hipLaunchKernelGGL(k_pp32_phase1<false>, dim3(blocks(N)), dim3(256), 0, st, P, C);
+        hipLaunchKernelGGL(k_pp32_face_psi, dim3(blocks(2 * N)), dim3(256), 0, st, P, C);
+        hipLaunchKernelGGL(k_pp32_face_flux, dim3(blocks(2 * N)), dim3(256), 0, st, P, C);
+        hipLaunchKernelGGL(k_pp32_update, dim3(blocks(N)), dim3(256), 0, st, P, C);
+        return;
+    }
+    const int ntx = (P.nx + FTX - 1) / FTX, nty = (P.ny + FTY - 1) / FTY, ntiles = ntx * nty;
+    const unsigned nblocks = (unsigned)(((ntiles + 7) / 8) * 8);
+    if (force_stale) hipLaunchKernelGGL(k_pp32_tile<true>, dim3(nblocks), dim3(FNT), 0, st, P, St.other, C, ntx, ntiles);
+    else hipLaunchKernelGGL(k_pp32_tile<false>, dim3(nblocks), dim3(FNT), 0, st, P, St.other, C, ntx, ntiles);
+    // the written set becomes the current one
+    PP32Set cur{P.rho, P.f, P.U, P.V, P.Fx, P.Fy, P.psi, P.scg, P.ffx, P.ffy};
+    const PP32Set &o = St.other;
+    P.rho = o.rho;
+    P.f = o.f;
+    P.U = o.U;
+    P.V = o.V;
+    P.Fx = o.Fx;
+    P.Fy = o.Fy;
+    P.psi = o.psi;
+    P.scg = o.scg;
+    P.ffx = o.ffx;
+    P.ffy = o.ffy;
+    St.other = cur;
 }
 
 }  // namespace mpd

@@ -83,3 +83,45 @@ def test_pp32_device_noise_start_runs():
     assert r["status"] == 0 and r["nonfinite"] == 0
     assert abs(r["sum_rho"] - m0) / m0 < 1e-9
     assert rho.max() - rho.min() > 10 * 0.02 * 0.1645  # grown from the +-1 % noise
+
+
+@pytest.mark.parametrize("n", [64, 40])
+def test_pp32_one_launch_equals_per_phase(n):
+    """The one-launch tile kernel against the per-phase FP32 kernels (same
+    device functions, same inputs): equal to FP32 rounding after 300 steps,
+    with the phase path selectable on the same FP32 state."""
+    cfg, st = make_case("pp_d2q9", nx=n, ny=n + 8, radius=0.25 * n, precision=abi.MPD_FP32_BULK)
+    out = {}
+    for path in (abi.MPD_PATH_FUSED, abi.MPD_PATH_PHASES):
+        s = solver.Solver(cfg, path=path)
+        s.upload(st)
+        s.step(300)
+        out[path] = s.download()
+        s.close()
+    a, b = out[abi.MPD_PATH_FUSED], out[abi.MPD_PATH_PHASES]
+    for name in ("f", "ffx", "ffy"):
+        x, y = getattr(a, name), getattr(b, name)
+        err = float(np.abs(x - y).max() / np.abs(y).max())
+        print("pp32 one-launch vs per-phase %dx%d %s: max rel %.3e" % (n, n + 8, name, err))
+        assert err < 1e-5, (name, err)
+    m = rel_l2(a.m[abi.M["rho"]], b.m[abi.M["rho"]])
+    assert m < 1e-6, m
+
+
+def test_pp32_switch_paths_mid_run():
+    cfg, st = make_case("pp_d2q9", nx=48, ny=48, precision=abi.MPD_FP32_BULK)
+    ref = solver.Solver(cfg, path=abi.MPD_PATH_FUSED)
+    ref.upload(st)
+    ref.step(200)
+    r = ref.download()
+    ref.close()
+    s = solver.Solver(cfg, path=abi.MPD_PATH_FUSED)
+    s.upload(st)
+    s.step(70)
+    s.set_path(abi.MPD_PATH_PHASES)
+    s.step(60)
+    s.set_path(abi.MPD_PATH_FUSED)
+    s.step(70)
+    g = s.download()
+    s.close()
+    assert rel_l2(g.f, r.f) < 1e-6
