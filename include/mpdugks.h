@@ -113,7 +113,8 @@ typedef enum mpd_macro_field {
 /* Kernel path used by mpd_step. */
 typedef enum mpd_path {
     MPD_PATH_AUTO = 0,       /* fused step kernel where available, else phases */
-    MPD_PATH_PHASES = 1,     /* one kernel per reference phase (plain FP64 path) */
+    MPD_PATH_PHASES = 1,     /* one kernel per reference phase (the plain FP64 path; for an FP32-bulk
+                                pseudopotential handle, the FP32 kernels of each phase) */
     MPD_PATH_FUSED = 2       /* one fused kernel per step */
 } mpd_path;
 
