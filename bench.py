@@ -204,7 +204,7 @@ def cpu_reference(steps, threads):
     import ref_run
     try:
         rows = ref_run.time_prebuilt("hcz", steps=steps, threads=threads, timeout=180)
-    except Exception as e:  # a missing or failing checker binary only drops this field
+    except Exception as e:  # a missing or failing reference binary only drops this field
         print("[bench] reference CPU solver not timed: %s" % e, file=sys.stderr, flush=True)
         return None
     if not rows:
@@ -242,7 +242,7 @@ def main():
                     help="also run a plain device-to-device copy of this many MiB (counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py: this build measures one GPU only (the DUGKS case runs on a single "
+        print("bench.py: measures one GPU only (the DUGKS case runs on a single "
               "MI355X); run with --gpus 1", file=sys.stderr)
         sys.exit(2)
 

@@ -1,4 +1,4 @@
-"""Fluent ``.cas`` writer used to run the reference as the checker (test
+"""Fluent ``.cas`` writer used to run the compiled reference for comparisons (test
 infrastructure): the writer itself is the library's, mpdugks_amd/mesh.py."""
 import os
 import sys

@@ -2,7 +2,7 @@
  * CPU FP64 restatement of the reference's 2D DUGKS time step - THE ORACLE.
  *
  * Test infrastructure only.  Nothing in the product links, loads or calls
- * this file; it is the checker used by tests/, by __graft_entry__.smoke() and
+ * this file; it is the reference implementation used by tests/, by __graft_entry__.smoke() and
  * by bench.py's CPU-baseline leg.
  *
  * It restates `DUGKS2DSolver()`'s loop body (Ark/Cpp/DUGKS2DSolver.cpp:164-328)

@@ -1,7 +1,8 @@
 """ctypes front end of the CPU oracle (oracle/_build/liboracle.so).
 
 Test infrastructure only: imported by tests/, __graft_entry__.smoke() and the
-CPU-baseline leg of bench.py, always as the checker.
+CPU-baseline leg of bench.py, always as the reference results are compared
+against or timed beside, never as the product.
 """
 import ctypes as C
 import os

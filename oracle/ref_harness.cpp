@@ -1,6 +1,6 @@
 // State-dump hook linked into the reference solver built from its own sources.
 //
-// Test infrastructure only (builds the checker, never shipped): the recipe in
+// Test infrastructure only (a reference build to compare against, never shipped): the recipe in
 // oracle/Makefile.ref compiles the reference's unmodified translation units
 // and links this file with `--wrap` on the mangled symbols of `Update_phi_Eq` and `Output_Flowfield`.
 //

@@ -1,5 +1,5 @@
-"""Write the golden fixtures of the reference solver (run in the container that
-has /root/reference; the fixtures then travel with the repository).
+"""Write the golden fixtures of the reference solver (run where /root/reference
+is present; the fixtures then travel with the repository).
 
 For each configuration the compiled reference (oracle/Makefile.ref, unmodified
 sources + state-dump hook) is run from its own initialiser and the full cell

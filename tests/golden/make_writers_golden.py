@@ -1,5 +1,5 @@
-"""Write the golden fixture of the reference's output files (run in the container
-that has /root/reference; the fixture then travels with the repository).
+"""Write the golden fixture of the reference's output files (run where
+/root/reference is present; the fixture then travels with the repository).
 
 The compiled reference (oracle/Makefile.ref: unmodified sources + state-dump
 hook, which with MPD_REF_WRITERS=1 also calls the reference's own

@@ -1,6 +1,6 @@
 """Oracle against the compiled reference run live (oracle/_ref), for the parts
 that need full reference fields as input.  Skipped where the reference sources
-are absent (the GPU box); runs in the container that builds the checker.
+(and so the compiled reference) are absent.
 
 * UW face scheme (reference built with the UW flag set): bit-exact at step 1
   (also pinned by tests/golden/ref_*_uw.json); later the reference's two stored
