@@ -186,7 +186,7 @@ int mpd_kernel_timing(mpd_handle *h, double *avg_ms, int64_t *launches);
 
 /* ---- batched ensembles ----
  * A batch advances many FP32-bulk cases (Allen-Cahn, CD; same model and y
- * boundary, any sizes and constants) with one kernel launch per step: every
+ * boundary -- periodic, BB or NEE -- any sizes and constants) with one kernel launch per step: every
  * workgroup takes one 16x15 tile of one case from a work list, and the step
  * pair is captured once in a HIP graph and replayed.  Members keep their own
  * state; mpd_download_state / mpd_reductions on a member synchronise the batch.

@@ -1,5 +1,5 @@
-// FP32-bulk fused DUGKS step (Allen-Cahn models, CD faces, periodic x,
-// periodic or bounce-back y).  Same stages and derived state as the FP64
+// FP32-bulk fused DUGKS step (Allen-Cahn models, CD faces, periodic x; y
+// periodic, bounce-back or NEE walls).  Same stages and derived state as the FP64
 // fused kernel (fused_kernels.hip), arithmetic rewritten for throughput:
 //  - D2Q9 directions and weights are compile-time constants, so e_q . u
 //    folds to adds and the populations with zero normal velocity skip the
@@ -61,6 +61,7 @@ struct K32 {
     float bbp0, bh0;                  // AC-p constant factors
     float cbpw0[3], chw0[3];
     float w[3];                       // weights rest / axis / diagonal, w0 = 1 - 4 w1 - 4 w2
+    float hneq, fneq0, hdx;           // NEE walls: 2 tau/(2 tau + dt) of h (and of f, AC-p), dx/2
 };
 
 // one case of a batched launch (device table entry)
@@ -297,7 +298,8 @@ __device__ __forceinline__ void cell_load(const Rs &in, int nx, int ny, bool sma
 
 template <int MODEL, int WALLS>
 __device__ __forceinline__ void cell_bp(const K32 &k, const Rs &out, int nx, int ny, int rx, int ry, const CellIn &x,
-                                        float2 *S, const float *s_phi, const float *s_phl, float *base, bool interior)
+                                        float2 *S, const float *s_phi, const float *s_phl, float *base, bool interior,
+                                        float2 *s_nee)
 {
     constexpr bool HCZ = MODEL == MPD_MODEL_AC_HCZ;
     if (!x.valid) return;
@@ -309,7 +311,7 @@ __device__ __forceinline__ void cell_bp(const K32 &k, const Rs &out, int nx, int
     const float phi = s_phi[sti.c[1]];
     const float rho = k.aphi * phi + k.bphi;
     Site m;
-    float p, bbp, tau = 0.f;
+    float p, bbp, tau = 0.f, rtau = 0.f;
     m.U = m.V = 0.f;
     if (HCZ) {
         float lgx, lgy, llap, lFx, lFy, lrx, lry;
@@ -323,7 +325,8 @@ __device__ __forceinline__ void cell_bp(const K32 &k, const Rs &out, int nx, int
         const float s = ((f[0] + f[1]) + (f[2] + f[3])) + ((f[4] + f[5]) + (f[6] + f[7])) + f[8];
         p = fmaf(k.hdt * k.rt, lrx * m.U + lry * m.V, s);
         tau = fmaf(phi, k.tau_a, k.tau_b);
-        bbp = k.bdt * rcp(2.f * tau + k.dt);   // 1 - (2 tau - dt/2)/(2 tau + dt)
+        rtau = rcp(2.f * tau + k.dt);
+        bbp = k.bdt * rtau;                  // 1 - (2 tau - dt/2)/(2 tau + dt)
     } else {
         p = k.kp * (((f[1] + f[2]) + (f[3] + f[4])) + ((f[5] + f[6]) + (f[7] + f[8])));
         bbp = k.bbp0;
@@ -385,6 +388,19 @@ __device__ __forceinline__ void cell_bp(const K32 &k, const Rs &out, int nx, int
             chw[t] = k.cbpw_h[t];
         }
     }
+    // NEE walls (Wall_3_NEE, Boundary.cpp:140-184; WALLS == 2): the macros of a
+    // wall face are extrapolated from its fluid cell -- phi along the
+    // Allen-Cahn source vector (the reference's Phi_x/y at that point), p of
+    // the cell, U = V = 0 -- so the face equilibria are w phi_f and w p (AC-p
+    // w p / RT).  A population leaving the fluid takes that plus the cell's
+    // non-equilibrium part 2 tau/(2 tau + dt) (x~ - x_eq + dt/2 x_so);
+    // bounce-back gives its partner the same value.  Bottom wall (cell row 0):
+    // the populations with e_y < 0; top wall (row ny - 1): e_y > 0.
+    const bool top = gj == ny - 1;
+    const float phf = fmaf(top ? k.hdx : -k.hdx, ay, phi);
+    const float pe = HCZ ? p : p * k.inv_rt;
+    const float fneq = HCZ ? 2.f * tau * rtau : k.fneq0;
+    float nh[3], nf[3];  // pairs (3,7), (2,6), (4,8)
     float bh[9], bf[9];
     {
         // rest population: u1 = 0, G = G0, no Allen-Cahn source
@@ -409,6 +425,14 @@ __device__ __forceinline__ void cell_bp(const K32 &k, const Rs &out, int nx, int
         bh[QP] = fmaf(k.bbp_h, heqp - h[QP], h[QP]) - chw[W] * hs;                       \
         bf[Q] = fmaf(bbp, feq - f[Q], f[Q]) + cw[W] * fs;                                \
         bf[QP] = fmaf(bbp, feqp - f[QP], f[QP]) + cw[W] * fsp;                           \
+        if (WALLS == 2 && Q >= 2) {                                                      \
+            constexpr int NK = Q == 3 ? 0 : (Q == 2 ? 1 : 2);                            \
+            const float hw = k.hdt * k.w[W];                                             \
+            const float hx = top ? h[Q] - heq : h[QP] - heqp;                            \
+            const float fx = top ? f[Q] - feq : f[QP] - feqp;                            \
+            nh[NK] = fmaf(k.w[W], phf, k.hneq * fmaf(hw, top ? hs : -hs, hx));           \
+            nf[NK] = fmaf(k.w[W], pe, fneq * fmaf(hw, top ? fs : fsp, fx));              \
+        }                                                                                \
     }
     MPD_CELL_PAIR(1, 1, 0)
     MPD_CELL_PAIR(2, 1, 1)
@@ -426,6 +450,10 @@ __device__ __forceinline__ void cell_bp(const K32 &k, const Rs &out, int nx, int
     S[9 * PS + c] = make_float2(0.5f * ax, 0.5f * ay);
     S[10 * PS + c] = make_float2(0.5f * rgx, 0.5f * rgy);
     S[11 * PS + c] = make_float2(0.5f * Fx, 0.5f * Fy);
+    if (WALLS == 2 && interior && (gj == 0 || top)) {
+#pragma unroll
+        for (int z = 0; z < 3; ++z) s_nee[(top ? 3 * TX : 0) + 3 * (rx - 1) + z] = make_float2(nh[z], nf[z]);
+    }
     if (interior) {
         if (HCZ || WALLS) st(out.a, n, phi, P_PHL * out.pb);
         if (HCZ) {
@@ -592,7 +620,7 @@ __device__ __forceinline__ void face(const K32 &k, const float2 *S, const FaceId
         MPD_FACE_PAIR(4, 4, 8, -1, 1)
     }
 #undef MPD_FACE_PAIR
-    if (WALLS && DIR == 1 && wall != 0) {
+    if (WALLS == 1 && DIR == 1 && wall != 0) {
         // Wall_3_BB (Boundary.cpp:118-139): the populations entering the fluid
         // (e_y > 0 at the bottom wall, < 0 at the top) take their partner's value
 #pragma unroll
@@ -625,6 +653,7 @@ __device__ __forceinline__ void tile32(const K32 &k, int nx, int ny, const Fused
     __shared__ float2 s_big[NBP * PS];
     __shared__ float s_phi[R2N];
     __shared__ float s_phl[HCZ ? R2N : 1];
+    __shared__ float2 s_nee[WALLS == 2 ? 6 * TX : 1];  // NEE wall values: [bottom, top][column][pair]
     const int tid = threadIdx.x;
     const bool small = nx < TX + 4 || ny < TY + 4;
     constexpr int NPL = HCZ ? 22 : (WALLS ? 20 : 19);
@@ -685,11 +714,11 @@ __device__ __forceinline__ void tile32(const K32 &k, int nx, int ny, const Fused
 
     // ---- stage B ----
     float base[17];
-    cell_bp<MODEL, WALLS>(k, out, nx, ny, rx1, ry1, x1, s_big, s_phi, s_phl, base, inside);
+    cell_bp<MODEL, WALLS>(k, out, nx, ny, rx1, ry1, x1, s_big, s_phi, s_phl, base, inside, s_nee);
     if (tid < NR2) {
         if (!PRELOAD_RING) cell_load<MODEL, WALLS>(in, nx, ny, small, i0, j0, rx2, ry2, x2);
         float dummy[17];
-        cell_bp<MODEL, WALLS>(k, out, nx, ny, rx2, ry2, x2, s_big, s_phi, s_phl, dummy, false);
+        cell_bp<MODEL, WALLS>(k, out, nx, ny, rx2, ry2, x2, s_big, s_phi, s_phl, dummy, false, s_nee);
     }
     __syncthreads();
 
@@ -726,6 +755,17 @@ __device__ __forceinline__ void tile32(const K32 &k, int nx, int ny, const Fused
         const int gj = j0 + fj;
         const int wall = WALLS ? (gj == 0 ? -1 : (gj == ny ? 1 : 0)) : 0;
         face<1, MODEL, WALLS>(k, s_big, face_idx<1>(fi + 1, fj), wall, yh, yf, ym);
+        if (WALLS == 2 && wall != 0) {
+            // NEE wall face: the values of the wall-row cell (stage B), both
+            // members of each pair alike, so no phase-field mass crosses
+            const float2 *v = s_nee + (wall < 0 ? 0 : 3 * TX) + 3 * fi;
+#pragma unroll
+            for (int z = 0; z < 3; ++z) {
+                yh[2 * z] = yh[2 * z + 1] = v[z].x;
+                yf[2 * z] = yf[2 * z + 1] = v[z].y;
+            }
+            ym = 0.f;
+        }
     }
     __syncthreads();
     float2 *s_fx = s_big, *s_fy = s_big + NFL * FS;
@@ -874,8 +914,14 @@ K32 make_k32(const Consts &C)
         k.cbpw0[t] = (float)(cbp0 * wd[t]);
         k.chw0[t] = (float)(ch0 * wd[t]);
     }
+    k.hneq = (float)(2.0 * C.taum / (2.0 * C.taum + C.dt));
+    k.fneq0 = (float)(2.0 * C.tau0 / (2.0 * C.tau0 + C.dt));
+    k.hdx = (float)(0.5 * C.dx);
     return k;
 }
+
+// WALLS template argument: 0 periodic y, 1 bounce-back, 2 non-equilibrium extrapolation
+int walls_code(const Consts &C) { return C.walls ? (C.nee ? 2 : 1) : 0; }
 
 }  // namespace
 
@@ -940,36 +986,44 @@ int batch32_plan(Batch32Plan *P, const BatchMember *m, int n, hipStream_t st)
     P->ncases = n;
     P->nwork = (int)work.size();
     P->model = n ? m[0].C->model : 0;
-    P->walls = n ? m[0].C->walls : 0;
+    P->walls = n ? walls_code(*m[0].C) : 0;
     return 0;
+}
+
+template <int MODEL>
+void batch32_launch(const Batch32Plan *P, int parity, hipStream_t st)
+{
+    dim3 g((unsigned)P->nwork), b(NT);
+    if (P->walls == 2) hipLaunchKernelGGL((k_fused32_batch<MODEL, 2>), g, b, 0, st, P->d_cases, P->d_work, P->nwork, parity);
+    else if (P->walls) hipLaunchKernelGGL((k_fused32_batch<MODEL, 1>), g, b, 0, st, P->d_cases, P->d_work, P->nwork, parity);
+    else hipLaunchKernelGGL((k_fused32_batch<MODEL, 0>), g, b, 0, st, P->d_cases, P->d_work, P->nwork, parity);
 }
 
 void batch32_step(const Batch32Plan *P, int parity, hipStream_t st)
 {
     if (!P->nwork) return;
-    dim3 g((unsigned)P->nwork), b(NT);
-    if (P->model == MPD_MODEL_AC_HCZ) {
-        if (P->walls) hipLaunchKernelGGL((k_fused32_batch<MPD_MODEL_AC_HCZ, 1>), g, b, 0, st, P->d_cases, P->d_work, P->nwork, parity);
-        else hipLaunchKernelGGL((k_fused32_batch<MPD_MODEL_AC_HCZ, 0>), g, b, 0, st, P->d_cases, P->d_work, P->nwork, parity);
-    } else {
-        if (P->walls) hipLaunchKernelGGL((k_fused32_batch<MPD_MODEL_AC_P, 1>), g, b, 0, st, P->d_cases, P->d_work, P->nwork, parity);
-        else hipLaunchKernelGGL((k_fused32_batch<MPD_MODEL_AC_P, 0>), g, b, 0, st, P->d_cases, P->d_work, P->nwork, parity);
-    }
+    if (P->model == MPD_MODEL_AC_HCZ) batch32_launch<MPD_MODEL_AC_HCZ>(P, parity, st);
+    else batch32_launch<MPD_MODEL_AC_P>(P, parity, st);
+}
+
+template <int MODEL>
+void fused32_launch(const K32 &k, const FusedPlanes32 &in, const FusedPlanes32 &out, const Consts &C, int64_t N,
+                    hipStream_t st)
+{
+    const int ntx = (C.nx + TX - 1) / TX, nty = (C.ny + TY - 1) / TY;
+    const int ntiles = ntx * nty;
+    dim3 g((unsigned)(((ntiles + 7) / 8) * 8)), b(NT);
+    const int w = walls_code(C);
+    if (w == 2) hipLaunchKernelGGL((k_fused32<MODEL, 2>), g, b, 0, st, k, C.nx, C.ny, in, out, N, ntx, ntiles);
+    else if (w) hipLaunchKernelGGL((k_fused32<MODEL, 1>), g, b, 0, st, k, C.nx, C.ny, in, out, N, ntx, ntiles);
+    else hipLaunchKernelGGL((k_fused32<MODEL, 0>), g, b, 0, st, k, C.nx, C.ny, in, out, N, ntx, ntiles);
 }
 
 void fused32_step(const FusedPlanes32 &in, const FusedPlanes32 &out, const Consts &C, int64_t N, hipStream_t st)
 {
     const K32 k = make_k32(C);
-    const int ntx = (C.nx + TX - 1) / TX, nty = (C.ny + TY - 1) / TY;
-    const int ntiles = ntx * nty;
-    const int nblocks = ((ntiles + 7) / 8) * 8;
-    if (C.model == MPD_MODEL_AC_HCZ) {
-        if (C.walls) hipLaunchKernelGGL((k_fused32<MPD_MODEL_AC_HCZ, 1>), dim3(nblocks), dim3(NT), 0, st, k, C.nx, C.ny, in, out, N, ntx, ntiles);
-        else hipLaunchKernelGGL((k_fused32<MPD_MODEL_AC_HCZ, 0>), dim3(nblocks), dim3(NT), 0, st, k, C.nx, C.ny, in, out, N, ntx, ntiles);
-    } else {
-        if (C.walls) hipLaunchKernelGGL((k_fused32<MPD_MODEL_AC_P, 1>), dim3(nblocks), dim3(NT), 0, st, k, C.nx, C.ny, in, out, N, ntx, ntiles);
-        else hipLaunchKernelGGL((k_fused32<MPD_MODEL_AC_P, 0>), dim3(nblocks), dim3(NT), 0, st, k, C.nx, C.ny, in, out, N, ntx, ntiles);
-    }
+    if (C.model == MPD_MODEL_AC_HCZ) fused32_launch<MPD_MODEL_AC_HCZ>(k, in, out, C, N, st);
+    else fused32_launch<MPD_MODEL_AC_P>(k, in, out, C, N, st);
 }
 
 }  // namespace mpd

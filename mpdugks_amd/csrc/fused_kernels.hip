@@ -548,10 +548,10 @@ bool fused_supported(const mpd_config &cfg)
     // FP32-bulk pseudopotential D2Q9: the FP32 phase kernels of pp32.hip
     if (cfg.model == MPD_MODEL_PP_D2Q9)
         return cfg.precision == MPD_FP32_BULK && cfg.flux == MPD_FLUX_CD && cfg.bc_y == MPD_BC_PERIODIC;
-    // walls: bounce-back only (the non-equilibrium extrapolation needs the
-    // cell equilibria at the wall row, which the tile does not keep)
+    // walls: bounce-back in both precisions; non-equilibrium extrapolation in
+    // the FP32 tile kernel only (FP64 NEE runs on the phase path)
     return (cfg.model == MPD_MODEL_AC_HCZ || cfg.model == MPD_MODEL_AC_P) && cfg.flux == MPD_FLUX_CD &&
-           cfg.bc_y != MPD_BC_WALL_NEE && cfg.nx >= 4 && cfg.ny >= 4;
+           (cfg.bc_y != MPD_BC_WALL_NEE || cfg.precision == MPD_FP32_BULK) && cfg.nx >= 4 && cfg.ny >= 4;
 }
 
 bool fused_needs_priming(const mpd_config &cfg) { return cfg.model != MPD_MODEL_PP_D2Q9; }

@@ -61,13 +61,13 @@ def test_interface_initialiser_profile():
 
 
 def test_fp32_bulk_without_a_kernel_is_refused_at_create():
-    """FP32-bulk exists for the Allen-Cahn models (CD; periodic or bounce-back y)
-    and the pseudopotential D2Q9 model (CD, periodic); other combinations are
-    refused by mpd_create before any device call."""
+    """FP32-bulk exists for the Allen-Cahn models (CD; periodic, bounce-back or
+    NEE y) and the pseudopotential D2Q9 model (CD, periodic); other
+    combinations are refused by mpd_create before any device call."""
     L = solver.lib()
     bad = [
         (abi.MPD_MODEL_AC_HCZ, dict(flux=abi.MPD_FLUX_UW)),
-        (abi.MPD_MODEL_AC_HCZ, dict(bc_y=abi.MPD_BC_WALL_NEE)),
+        (abi.MPD_MODEL_AC_P, dict(flux=abi.MPD_FLUX_UW)),
         (abi.MPD_MODEL_PP_D2Q9, dict(flux=abi.MPD_FLUX_UW)),
         (abi.MPD_MODEL_PP_D2V16, dict()),
     ]

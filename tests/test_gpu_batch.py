@@ -5,6 +5,7 @@ import numpy as np
 import pytest
 
 from mpdugks_amd import abi, cases, solver
+from mpdugks_amd.config import default_config
 
 pytestmark = pytest.mark.gpu
 
@@ -60,4 +61,30 @@ def test_unprimed_case_is_rejected():
     b = solver.Batch()
     with pytest.raises(solver.MpdError):
         b.add(s)
+    b.close()
+
+
+def test_batch_nee_walls_matches_individual_cases():
+    """Cases with NEE walls in y batch like periodic ones (same tile code)."""
+    def primed(side, r):
+        cfg = default_config(abi.MPD_MODEL_AC_HCZ, nx=side, ny=side + 5)
+        cfg.precision = abi.MPD_FP32_BULK
+        cfg.bc_y = abi.MPD_BC_WALL_NEE
+        st = solver.ac_drop(cfg, radius=r, sign=-1)
+        s = solver.Solver(cfg, path=abi.MPD_PATH_FUSED)
+        s.upload(st)
+        s.step(2)
+        return s
+    dims = [(32, 9.0), (48, 15.0), (40, 12.0)]
+    single = [primed(*d) for d in dims]
+    batched = [primed(*d) for d in dims]
+    b = solver.Batch()
+    for s in batched:
+        b.add(s)
+    b.step(9)
+    for s in single:
+        s.step(9)
+    for a, c in zip(batched, single):
+        ga, gc = a.download(), c.download()
+        assert np.array_equal(ga.f, gc.f) and np.array_equal(ga.h, gc.h) and np.array_equal(ga.m, gc.m)
     b.close()

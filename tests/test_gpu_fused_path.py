@@ -100,6 +100,63 @@ def test_fp32_bulk_error_vs_fp64_oracle(model, n, steps, oracle_mod):
     assert errs["u"] < 1e-3 and errs["v"] < 1e-3, errs
 
 
+FP32_WALL_CASES = [
+    # model, mesh, wall type, steps (Wall_3_BB / Wall_3_NEE in y, periodic x)
+    ("ac_hcz", (64, 48), "bb", 60),
+    ("ac_hcz", (64, 48), "nee", 60),
+    ("ac_p", (70, 47), "nee", 300),
+    ("ac_p", (70, 47), "bb", 300),
+]
+
+
+@pytest.mark.parametrize("model,mesh,bc,steps", FP32_WALL_CASES,
+                         ids=["%s_%dx%d_%s" % (c[0], c[1][0], c[1][1], c[2]) for c in FP32_WALL_CASES])
+def test_fp32_bulk_walls_error_vs_fp64_oracle(model, mesh, bc, steps, oracle_mod):
+    """FP32-bulk with walls against the FP64 oracle (same bar as periodic):
+    a drop touching neither wall, so the wall faces carry the non-equilibrium
+    part of a fluid at rest plus the gravity-free pressure field."""
+    nx, ny = mesh
+    cfg, st = make_case(model, nx=nx, ny=ny, walls=True, nee=bc == "nee", sign=-1,
+                        radius=0.3 * ny, precision=abi.MPD_FP32_BULK)
+    ref = oracle_run(cfg, st, [steps])[steps]
+    s = solver.Solver(cfg)
+    s.upload(st)
+    s.step(steps)
+    g = s.download()
+    s.close()
+    errs = {name: rel_l2(g.m[abi.M[name]], ref.m[abi.M[name]]) for name in ("rho", "phi", "p")}
+    N = cfg.nx * cfg.ny
+    for name in ("u", "v"):
+        errs[name] = rel_l2(g.m[abi.M[name]], ref.m[abi.M[name]], floor=1e-6 * np.sqrt(N))
+    print("fp32-bulk %s %s walls %d steps errors: %s" % (model, bc, steps, errs))
+    assert errs["rho"] < 1e-5 and errs["phi"] < 1e-5 and errs["p"] < 1e-5, errs
+    assert errs["u"] < 1e-3 and errs["v"] < 1e-3, errs
+
+
+def test_fp32_bulk_nee_differs_from_bb(oracle_mod):
+    """The two wall types give different flows (so the NEE branch is the one
+    taken): a drop reaching to 4 cells from both walls, 40 steps; the
+    velocity fields of NEE and BB differ by ~35% (relative L2), and each FP32
+    run stays within 1e-3 of that distance from the oracle's run of its type."""
+    out = {}
+    for bc in ("bb", "nee"):
+        cfg, st = make_case("ac_hcz", nx=48, ny=48, walls=True, nee=bc == "nee", sign=-1, radius=20.0,
+                            precision=abi.MPD_FP32_BULK)
+        ref = oracle_run(cfg, st, [40])[40]
+        s = solver.Solver(cfg)
+        s.upload(st)
+        s.step(40)
+        out[bc] = (s.download(), ref)
+        s.close()
+    (gb, rb), (gn, rn) = out["bb"], out["nee"]
+    for name in ("u", "v"):
+        d_ref = rel_l2(rn.m[abi.M[name]], rb.m[abi.M[name]])
+        d_bb, d_nee = rel_l2(gb.m[abi.M[name]], rb.m[abi.M[name]]), rel_l2(gn.m[abi.M[name]], rn.m[abi.M[name]])
+        print("%s: oracle NEE vs BB %.3e, FP32 BB vs oracle %.3e, FP32 NEE vs oracle %.3e" % (name, d_ref, d_bb, d_nee))
+        assert d_ref > 0.1, d_ref
+        assert d_bb < 1e-3 * d_ref and d_nee < 1e-3 * d_ref
+
+
 def test_fp32_bulk_walls_conserves_mass():
     """Bounce-back walls and periodic x: sum(phi) is conserved by the scheme;
     FP32-bulk carries phi in FP64 and keeps the relative drift below 1e-11 over
