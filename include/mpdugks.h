@@ -148,6 +148,14 @@ int mpd_destroy(mpd_handle *h);
 /* Select the kernel path (mpd_path); valid before the first mpd_step. */
 int mpd_set_path(mpd_handle *h, int32_t path);
 
+/* Tile height of the FP32-bulk Allen-Cahn step kernel (tiles of 16 x rows
+ * cells): 0 (the default) chooses per mesh, 15 or 16 forces one; the state
+ * does not depend on it.  Takes effect at the next mpd_step; get returns the
+ * height that step uses.  No counterpart in the reference (a launch shape of
+ * this library). */
+int mpd_set_tile_rows(mpd_handle *h, int32_t rows);
+int mpd_get_tile_rows(mpd_handle *h, int32_t *rows);
+
 /* Upload host state (FP64).  tilde_h may be NULL for the PP models;
  * face_fx/face_fy (PP only, may be NULL otherwise) are the persistent face
  * forces of the x-faces (face i,j between cells (i-1,j),(i,j)) followed by the

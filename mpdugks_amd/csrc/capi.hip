@@ -49,6 +49,8 @@ struct mpd_handle {
     FusedState F;
     hipStream_t stream = nullptr;
     int device = 0;
+    int cus = 0;                 // compute units of the device
+    int tile_rows = 0;           // FP32-bulk Allen-Cahn tile height: 0 chosen per mesh, or 15 / 16
     int path = MPD_PATH_AUTO;
     int effective_path = MPD_PATH_PHASES;
     bool uploaded = false;
@@ -301,6 +303,7 @@ int mpd_create(const mpd_config *cfg, mpd_handle **out)
     make_consts(*cfg, h->C);
     int r = 0;
     hipError_t e = hipGetDevice(&h->device);
+    if (e == hipSuccess) e = hipDeviceGetAttribute(&h->cus, hipDeviceAttributeMultiprocessorCount, h->device);
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking);
     if (e == hipSuccess) e = hipEventCreate(&h->ev0);
     if (e == hipSuccess) e = hipEventCreate(&h->ev1);
@@ -357,6 +360,21 @@ int mpd_set_path(mpd_handle *h, int32_t path)
     if (h->cfg.precision == MPD_FP32_BULK && path == MPD_PATH_PHASES && !is_pp(h->cfg))
         return fail(MPD_ERR_INVALID, "the phase path is FP64 only (and FP32 for the pseudopotential model)");
     h->path = path;
+    return 0;
+}
+
+int mpd_set_tile_rows(mpd_handle *h, int32_t rows)
+{
+    if (!h) return fail(MPD_ERR_INVALID, "null handle");
+    if (rows != 0 && rows != 15 && rows != 16) return fail(MPD_ERR_INVALID, "tile rows %d (0, 15 or 16)", rows);
+    h->tile_rows = rows;
+    return 0;
+}
+
+int mpd_get_tile_rows(mpd_handle *h, int32_t *rows)
+{
+    if (!h || !rows) return fail(MPD_ERR_INVALID, "null argument");
+    *rows = h->tile_rows ? h->tile_rows : fused32_tile_rows(h->cfg.nx, h->cfg.ny, h->cus);
     return 0;
 }
 
@@ -524,6 +542,7 @@ int mpd_step(mpd_handle *h, int64_t nsteps)
             if (r) return fail(MPD_ERR_NOMEM, "fused path allocation failed");
         }
         h->F.pp32.phases = pp32_phases;
+        h->F.tile_rows = h->tile_rows ? h->tile_rows : fused32_tile_rows(h->cfg.nx, h->cfg.ny, h->cus);
         int64_t s = 0;
         if (!h->fused_primed && !fused_needs_priming(h->cfg)) {
             // explicit state carried as is (FP32-bulk pseudopotential)

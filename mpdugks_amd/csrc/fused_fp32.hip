@@ -26,27 +26,43 @@ namespace mpd {
 
 namespace {
 
-constexpr int TX = 16, TY = 15;
-constexpr int R1X = TX + 2, R1Y = TY + 2;  // bar-plus region (cells with a one-cell halo)
-constexpr int R2X = TX + 4, R2Y = TY + 4;  // phi region (two-cell halo)
-constexpr int R1N = R1X * R1Y, R2N = R2X * R2Y;
-constexpr int NIN = TX * TY;               // interior cells of a tile
-constexpr int NRING = R1N - NIN;           // halo ring of the bar-plus region
-constexpr int NXF = (TX + 1) * TY;         // 255 x-faces
-constexpr int NYF = TX * (TY + 1);         // 256 y-faces
+constexpr int TX = 16;                     // tile width; the height TY (15 or 16) is a template argument
+constexpr int R1X = TX + 2;                // bar-plus region (cells with a one-cell halo)
+constexpr int R2X = TX + 4;                // phi region (two-cell halo)
 constexpr int NT = 256;
 // LDS plane strides in float2 elements.  Neither is a multiple of 32 float2
 // (64 dwords): the compiler would otherwise pair reads of two planes at one
 // site into one two-address instruction whose two addresses share a bank.
 constexpr int PS = 324;                    // bar-plus planes
-constexpr int FS = 260;                    // face-value planes
 constexpr int NBP = 12;                    // float2 planes: (h, f) bar-plus of q = 0..8, then the
                                            // half macros (ax, ay), (rgx, rgy), (Fx, Fy)
 constexpr int NFL = 7;                     // float2 planes per face direction: 6 (h, f) values, (h mass, 0)
 constexpr bool PRELOAD_RING = false;       // also hoist the second-pass loads above the phi staging
-static_assert(R1N <= PS && NXF <= FS && NYF <= FS, "plane strides");
-static_assert(NIN <= NT && NRING - (NT - NIN) <= 64, "stage-B mapping: one partial wave for the ring");
-static_assert(2 * NFL * FS <= NBP * PS, "face values must fit over the bar-plus planes");
+
+// Tile of TX x TY interior cells, 256 threads (one interior cell each in the
+// update).  TY = 15: 255 x- and 256 y-faces, one of each per thread, and the
+// bar-plus ring takes one partial wave after the interior.  TY = 16: 272
+// faces of each direction (the last 16 of each taken by one wave in a second
+// pass) and a ring of 68 cells (two waves): about 1.2x the instructions of a
+// 16x15 tile for 1.07x the cells, chosen where it removes a partly filled
+// last round of workgroups.
+template <int TY_>
+struct Tile {
+    static constexpr int TY = TY_;
+    static constexpr int R1Y = TY + 2, R2Y = TY + 4;
+    static constexpr int R1N = R1X * R1Y, R2N = R2X * R2Y;
+    static constexpr int NIN = TX * TY;               // interior cells
+    static constexpr int NRING = R1N - NIN;           // halo ring of the bar-plus region
+    static constexpr int NXF = (TX + 1) * TY;         // x-faces
+    static constexpr int NYF = TX * (TY + 1);         // y-faces
+    static constexpr int FS = TY == 15 ? 260 : 276;   // face-value plane stride (float2)
+    static_assert(TY == 15 || TY == 16, "tile heights");
+    static_assert(R1N <= PS && NXF <= FS && NYF <= FS, "plane strides");
+    static_assert(NIN <= NT && NRING - (NT - NIN) <= NT, "stage-B mapping");
+    static_assert(NXF - NT <= 64 && NYF - NT <= 64, "second face pass: one wave per direction");
+    static_assert(2 * NFL * FS <= NBP * PS, "face values must fit over the bar-plus planes");
+};
+constexpr int BATCH_TY = 15;               // tile height of batched launches
 
 struct K32 {
     float ms;        // MaSpan
@@ -168,27 +184,31 @@ __device__ __forceinline__ float edot(float x, float y)
 struct Col {
     int pitch, off;
 };
+template <int TY>
 __device__ __forceinline__ Col col1(int c)
 {
     const bool in = c >= 1 && c <= TX;
-    return Col{in ? TX : 2, in ? c - 1 : TX * R1Y + (c > TX ? 1 : 0)};
+    return Col{in ? TX : 2, in ? c - 1 : TX * Tile<TY>::R1Y + (c > TX ? 1 : 0)};
 }
+template <int TY>
 __device__ __forceinline__ Col col2(int c)
 {
     const bool in = c >= 2 && c <= TX + 1;
-    return Col{in ? TX : 4, in ? c - 2 : TX * R2Y + (c < 2 ? c : c - TX)};
+    return Col{in ? TX : 4, in ? c - 2 : TX * Tile<TY>::R2Y + (c < 2 ? c : c - TX)};
 }
 __device__ __forceinline__ int at(Col c, int row) { return row * c.pitch + c.off; }
-__device__ __forceinline__ int r1(int col, int row) { return at(col1(col), row); }
+template <int TY>
+__device__ __forceinline__ int r1(int col, int row) { return at(col1<TY>(col), row); }
 
 // grad(phi), laplacian, SetZero (GradScheme.cpp:347-376) from the phi region s
 // at the nine indices of a 3x3 stencil (ix: -1, 0, +1 columns; rows likewise)
 struct Stencil {
     int w[3], c[3], e[3];  // rows row-1, row, row+1
 };
+template <int TY>
 __device__ __forceinline__ Stencil stencil_idx(int col, int row)
 {
-    const Col a = col2(col - 1), b = col2(col), d = col2(col + 1);
+    const Col a = col2<TY>(col - 1), b = col2<TY>(col), d = col2<TY>(col + 1);
     Stencil t;
 #pragma unroll
     for (int r = 0; r < 3; ++r) {
@@ -296,7 +316,7 @@ __device__ __forceinline__ void cell_load(const Rs &in, int nx, int ny, bool sma
     }
 }
 
-template <int MODEL, int WALLS>
+template <int MODEL, int WALLS, int TY>
 __device__ __forceinline__ void cell_bp(const K32 &k, const Rs &out, int nx, int ny, int rx, int ry, const CellIn &x,
                                         float2 *S, const float *s_phi, const float *s_phl, float *base, bool interior,
                                         float2 *s_nee)
@@ -306,8 +326,8 @@ __device__ __forceinline__ void cell_bp(const K32 &k, const Rs &out, int nx, int
     const float *h = x.h, *f = x.f;
     const uint32_t n = x.n;
     const int gi = x.gi, gj = x.gj;
-    const int c = r1(rx, ry);
-    const Stencil sti = stencil_idx(rx + 1, ry + 1);
+    const int c = r1<TY>(rx, ry);
+    const Stencil sti = stencil_idx<TY>(rx + 1, ry + 1);
     const float phi = s_phi[sti.c[1]];
     const float rho = k.aphi * phi + k.bphi;
     Site m;
@@ -500,12 +520,12 @@ __device__ __forceinline__ float2 recon(const K32 &k, const float2 *b, const Fac
 
 // Indices of the face between R1 cells (oc, orow) and its neighbour in
 // direction DIR, with the tangential neighbours of both (LDS layout above).
-template <int DIR>
+template <int DIR, int TY>
 __device__ __forceinline__ FaceIdx face_idx(int oc, int orow)
 {
     FaceIdx x;
     if (DIR == 0) {
-        const Col a = col1(oc), b = col1(oc + 1);
+        const Col a = col1<TY>(oc), b = col1<TY>(oc + 1);
         x.o = at(a, orow);
         x.n = opaque(at(b, orow));
         x.op = opaque(at(a, orow + 1));
@@ -513,7 +533,7 @@ __device__ __forceinline__ FaceIdx face_idx(int oc, int orow)
         x.np = opaque(at(b, orow + 1));
         x.nm = opaque(at(b, orow - 1));
     } else {
-        const Col a = col1(oc - 1), b = col1(oc), d = col1(oc + 1);
+        const Col a = col1<TY>(oc - 1), b = col1<TY>(oc), d = col1<TY>(oc + 1);
         x.o = at(b, orow);
         x.n = opaque(at(b, orow + 1));
         x.op = opaque(at(d, orow));
@@ -639,17 +659,53 @@ __device__ __forceinline__ void face(const K32 &k, const float2 *S, const FaceId
     }
 }
 
-// One 16x15 tile of one case.
+// x-face t of a tile: column t % TX, row t / TX for t < TX*TY, then the last
+// column (face index TX*TY + row)
+template <int MODEL, int WALLS, int TY>
+__device__ __forceinline__ void xface(const K32 &k, const float2 *S, int t, float *h, float *f, float &m)
+{
+    constexpr int NIN = Tile<TY>::NIN;
+    const int fi = t < NIN ? t % TX : TX, fj = t < NIN ? t / TX : t - NIN;
+    face<0, MODEL, WALLS>(k, S, face_idx<0, TY>(fi, fj + 1), 0, h, f, m);
+}
+
+// y-face t of a tile: column t % TX, row t / TX
+template <int MODEL, int WALLS, int TY>
+__device__ __forceinline__ void yface(const K32 &k, const float2 *S, const float2 *s_nee, int t, int j0, int ny,
+                                      float *h, float *f, float &m)
+{
+    const int fi = t % TX, fj = t / TX;
+    const int gj = j0 + fj;
+    const int wall = WALLS ? (gj == 0 ? -1 : (gj == ny ? 1 : 0)) : 0;
+    face<1, MODEL, WALLS>(k, S, face_idx<1, TY>(fi + 1, fj), wall, h, f, m);
+    if (WALLS == 2 && wall != 0) {
+        // NEE wall face: the values of the wall-row cell (stage B), both
+        // members of each pair alike, so no phase-field mass crosses
+        const float2 *v = s_nee + (wall < 0 ? 0 : 3 * TX) + 3 * fi;
+#pragma unroll
+        for (int z = 0; z < 3; ++z) {
+            h[2 * z] = h[2 * z + 1] = v[z].x;
+            f[2 * z] = f[2 * z + 1] = v[z].y;
+        }
+        m = 0.f;
+    }
+}
+
+// One 16 x TY tile of one case.
 //  A: phi (and lagged phi) with a two-cell halo into LDS;
-//  B: bar-plus and half macros of the 18x17 region (interior cells first, so
-//     each interior thread keeps its own b + (b - x~)/3 in registers);
-//  C: every x- and y-face of the tile (one of each per thread);
+//  B: bar-plus and half macros of the (TX+2) x (TY+2) region (interior cells
+//     first, so each interior thread keeps its own b + (b - x~)/3 in registers);
+//  C: every x- and y-face of the tile (one of each per thread, then for TY = 16
+//     the last 16 of each direction on waves 2 and 3);
 //  D: Update_phi_T, phi moved by the antisymmetric face mass fluxes.
-template <int MODEL, int WALLS>
+template <int MODEL, int WALLS, int TY>
 __device__ __forceinline__ void tile32(const K32 &k, int nx, int ny, const FusedPlanes32 &pin,
                                        const FusedPlanes32 &pout, int64_t N, int i0, int j0)
 {
     constexpr bool HCZ = MODEL == MPD_MODEL_AC_HCZ;
+    using G = Tile<TY>;
+    constexpr int R1Y = G::R1Y, R2Y = G::R2Y, R2N = G::R2N, NIN = G::NIN, NRING = G::NRING;
+    constexpr int NXF = G::NXF, NYF = G::NYF, FS = G::FS;
     __shared__ float2 s_big[NBP * PS];
     __shared__ float s_phi[R2N];
     __shared__ float s_phl[HCZ ? R2N : 1];
@@ -714,11 +770,11 @@ __device__ __forceinline__ void tile32(const K32 &k, int nx, int ny, const Fused
 
     // ---- stage B ----
     float base[17];
-    cell_bp<MODEL, WALLS>(k, out, nx, ny, rx1, ry1, x1, s_big, s_phi, s_phl, base, inside, s_nee);
+    cell_bp<MODEL, WALLS, TY>(k, out, nx, ny, rx1, ry1, x1, s_big, s_phi, s_phl, base, inside, s_nee);
     if (tid < NR2) {
         if (!PRELOAD_RING) cell_load<MODEL, WALLS>(in, nx, ny, small, i0, j0, rx2, ry2, x2);
         float dummy[17];
-        cell_bp<MODEL, WALLS>(k, out, nx, ny, rx2, ry2, x2, s_big, s_phi, s_phl, dummy, false, s_nee);
+        cell_bp<MODEL, WALLS, TY>(k, out, nx, ny, rx2, ry2, x2, s_big, s_phi, s_phl, dummy, false, s_nee);
     }
     __syncthreads();
 
@@ -731,7 +787,7 @@ __device__ __forceinline__ void tile32(const K32 &k, int nx, int ny, const Fused
         if (row >= 0) {
             const int rx = tid < R1X ? tid : tid - R1X;
             const int dn = row == gy0 && tid < R1X ? 1 : -1;
-            const Col cc = col1(rx);
+            const Col cc = col1<TY>(rx);
             const int c = at(cc, row), kn = at(cc, row + dn), knn = at(cc, row + 2 * dn);
 #pragma unroll
             for (int q = 0; q < NBP; ++q) {
@@ -743,30 +799,36 @@ __device__ __forceinline__ void tile32(const K32 &k, int nx, int ny, const Fused
     }
 
     // ---- stage C ----
-    // x-face t: column t % TX, row t / TX for t < TX*TY, then the last column
-    // (face index TX*TY + row); y-face t: column t % TX, row t / TX
-    float xh[6], xf[6], xm = 0.f, yh[6], yf[6], ym;
-    if (tid < NXF) {
-        const int fi = tid < NIN ? tid % TX : TX, fj = tid < NIN ? tid / TX : tid - NIN;
-        face<0, MODEL, WALLS>(k, s_big, face_idx<0>(fi, fj + 1), 0, xh, xf, xm);
-    }
-    {
-        const int fi = tid % TX, fj = tid / TX;
-        const int gj = j0 + fj;
-        const int wall = WALLS ? (gj == 0 ? -1 : (gj == ny ? 1 : 0)) : 0;
-        face<1, MODEL, WALLS>(k, s_big, face_idx<1>(fi + 1, fj), wall, yh, yf, ym);
-        if (WALLS == 2 && wall != 0) {
-            // NEE wall face: the values of the wall-row cell (stage B), both
-            // members of each pair alike, so no phase-field mass crosses
-            const float2 *v = s_nee + (wall < 0 ? 0 : 3 * TX) + 3 * fi;
+    // TY = 16: the last 16 faces of each direction first, on waves 2 (x) and
+    // 3 (y), straight into LDS of their own (the face planes overlay the
+    // bar-plus planes, which the other faces still read); then one x- and one
+    // y-face per thread, held in registers until every face is computed
+    constexpr int EXF = NXF - NT, EYF = NYF - NT, NE = 16;
+    static_assert(EXF <= NE && EYF <= NE, "second face pass");
+    __shared__ float2 s_ext[EXF > 0 || EYF > 0 ? 2 * NFL * NE : 1];
+    int et = -1;
+    if (EXF > 0 && tid >= 128 && tid < 128 + EXF) {
+        et = tid - 128;
+        float h[6], f[6], m = 0.f;
+        xface<MODEL, WALLS, TY>(k, s_big, NT + et, h, f, m);
 #pragma unroll
-            for (int z = 0; z < 3; ++z) {
-                yh[2 * z] = yh[2 * z + 1] = v[z].x;
-                yf[2 * z] = yf[2 * z + 1] = v[z].y;
-            }
-            ym = 0.f;
-        }
+        for (int z = 0; z < 6; ++z) s_ext[z * NE + et] = make_float2(h[z], f[z]);
+        s_ext[6 * NE + et] = make_float2(m, 0.f);
     }
+    if (EYF > 0 && tid >= 192 && tid < 192 + EYF) {
+        et = tid - 192;
+        float h[6], f[6], m = 0.f;
+        yface<MODEL, WALLS, TY>(k, s_big, s_nee, NT + et, j0, ny, h, f, m);
+#pragma unroll
+        for (int z = 0; z < 6; ++z) s_ext[(NFL + z) * NE + et] = make_float2(h[z], f[z]);
+        s_ext[(NFL + 6) * NE + et] = make_float2(m, 0.f);
+    }
+    float xh[6], xf[6], xm = 0.f, yh[6], yf[6], ym = 0.f;
+    if (tid < NXF) xface<MODEL, WALLS, TY>(k, s_big, tid, xh, xf, xm);
+    // TY = 16: every thread has both faces, and the scheduler would interleave
+    // the two (past 128 VGPRs, below 4 workgroups per CU); keep them apart
+    if (NXF >= NT) __builtin_amdgcn_sched_barrier(0);
+    if (tid < NYF) yface<MODEL, WALLS, TY>(k, s_big, s_nee, tid, j0, ny, yh, yf, ym);
     __syncthreads();
     float2 *s_fx = s_big, *s_fy = s_big + NFL * FS;
     if (tid < NXF) {
@@ -774,9 +836,19 @@ __device__ __forceinline__ void tile32(const K32 &k, int nx, int ny, const Fused
         for (int z = 0; z < 6; ++z) s_fx[z * FS + tid] = make_float2(xh[z], xf[z]);
         s_fx[6 * FS + tid] = make_float2(xm, 0.f);
     }
+    if (tid < NYF) {
 #pragma unroll
-    for (int z = 0; z < 6; ++z) s_fy[z * FS + tid] = make_float2(yh[z], yf[z]);
-    s_fy[6 * FS + tid] = make_float2(ym, 0.f);
+        for (int z = 0; z < 6; ++z) s_fy[z * FS + tid] = make_float2(yh[z], yf[z]);
+        s_fy[6 * FS + tid] = make_float2(ym, 0.f);
+    }
+    if ((EXF > 0 || EYF > 0) && et >= 0) {
+        // this thread's second-pass face into its slot NT + et
+        const bool y = tid >= 192;
+        const float2 *src = s_ext + (y ? NFL * NE : 0) + et;
+        float2 *d = (y ? s_fy : s_fx) + NT + et;
+#pragma unroll
+        for (int z = 0; z < NFL; ++z) d[z * FS] = src[z * NE];
+    }
     __syncthreads();
 
     // ---- stage D: x~' = b + (b - x~)/3 + dt/V (W - E + S - N) ----
@@ -829,13 +901,13 @@ __device__ __forceinline__ void tile32(const K32 &k, int nx, int ny, const Fused
     st(out.a, n, phi, P_PHI * out.pb);
 }
 
-template <int MODEL, int WALLS>
+template <int MODEL, int WALLS, int TY>
 __global__ void __launch_bounds__(NT) k_fused32(K32 k, int nx, int ny, FusedPlanes32 in, FusedPlanes32 out, int64_t N,
                                                int ntx, int ntiles)
 {
     const int tile = tile_of_block(blockIdx.x, ntiles);
     if (tile < 0) return;
-    tile32<MODEL, WALLS>(k, nx, ny, in, out, N, (tile % ntx) * TX, (tile / ntx) * TY);
+    tile32<MODEL, WALLS, TY>(k, nx, ny, in, out, N, (tile % ntx) * TX, (tile / ntx) * TY);
 }
 
 // Batched step: every workgroup takes one (case, tile) entry of a work list
@@ -850,7 +922,8 @@ __global__ void __launch_bounds__(NT) k_fused32_batch(const BatchCase *cases, co
     const BatchCase &c = cases[w.x];
     const int sel = c.cur ^ parity;
     const int ntx = (c.nx + TX - 1) / TX;
-    tile32<MODEL, WALLS>(c.k, c.nx, c.ny, c.pl[sel], c.pl[sel ^ 1], c.N, (w.y % ntx) * TX, (w.y / ntx) * TY);
+    tile32<MODEL, WALLS, BATCH_TY>(c.k, c.nx, c.ny, c.pl[sel], c.pl[sel ^ 1], c.N, (w.y % ntx) * TX,
+                                   (w.y / ntx) * BATCH_TY);
 }
 
 K32 make_k32(const Consts &C)
@@ -958,7 +1031,7 @@ int batch32_plan(Batch32Plan *P, const BatchMember *m, int n, hipStream_t st)
         b.pl[0] = m[c].pl[0];
         b.pl[1] = m[c].pl[1];
         b.cur = m[c].cur;
-        const int ntiles = ((C.nx + TX - 1) / TX) * ((C.ny + TY - 1) / TY);
+        const int ntiles = ((C.nx + TX - 1) / TX) * ((C.ny + BATCH_TY - 1) / BATCH_TY);
         for (int t = 0; t < ntiles; ++t) list.push_back(make_int2(c, t));
     }
     const int per = ((int)list.size() + 7) / 8;
@@ -1006,7 +1079,7 @@ void batch32_step(const Batch32Plan *P, int parity, hipStream_t st)
     else batch32_launch<MPD_MODEL_AC_P>(P, parity, st);
 }
 
-template <int MODEL>
+template <int MODEL, int TY>
 void fused32_launch(const K32 &k, const FusedPlanes32 &in, const FusedPlanes32 &out, const Consts &C, int64_t N,
                     hipStream_t st)
 {
@@ -1014,16 +1087,37 @@ void fused32_launch(const K32 &k, const FusedPlanes32 &in, const FusedPlanes32 &
     const int ntiles = ntx * nty;
     dim3 g((unsigned)(((ntiles + 7) / 8) * 8)), b(NT);
     const int w = walls_code(C);
-    if (w == 2) hipLaunchKernelGGL((k_fused32<MODEL, 2>), g, b, 0, st, k, C.nx, C.ny, in, out, N, ntx, ntiles);
-    else if (w) hipLaunchKernelGGL((k_fused32<MODEL, 1>), g, b, 0, st, k, C.nx, C.ny, in, out, N, ntx, ntiles);
-    else hipLaunchKernelGGL((k_fused32<MODEL, 0>), g, b, 0, st, k, C.nx, C.ny, in, out, N, ntx, ntiles);
+    if (w == 2) hipLaunchKernelGGL((k_fused32<MODEL, 2, TY>), g, b, 0, st, k, C.nx, C.ny, in, out, N, ntx, ntiles);
+    else if (w) hipLaunchKernelGGL((k_fused32<MODEL, 1, TY>), g, b, 0, st, k, C.nx, C.ny, in, out, N, ntx, ntiles);
+    else hipLaunchKernelGGL((k_fused32<MODEL, 0, TY>), g, b, 0, st, k, C.nx, C.ny, in, out, N, ntx, ntiles);
 }
 
-void fused32_step(const FusedPlanes32 &in, const FusedPlanes32 &out, const Consts &C, int64_t N, hipStream_t st)
+int fused32_tile_rows(int nx, int ny, int cus)
+{
+    // 16 rows where that brings the tiles within one round of resident
+    // workgroups (4 per CU) and 15 rows would not; else 15 rows (fewer
+    // instructions per cell).  Measured on MI355X (256 CUs), FP32 HCZ with
+    // walls, us per step 16x15 -> 16x16: 256x1024 18.9 -> 18.2, 512x512
+    // 19.3 -> 17.6, 1024x256 20.4 -> 18.9; on meshes of several rounds
+    // either height wins by up to 5% with no simple rule (1024x1024 51.8 vs
+    // 54.4, 2048x2048 261 vs 249), and one-round meshes favour 15 rows.
+    const int64_t slots = 4 * (int64_t)(cus > 0 ? cus : 256);
+    const int64_t ntx = (nx + TX - 1) / TX;
+    const int64_t t15 = ntx * ((ny + 14) / 15), t16 = ntx * ((ny + 15) / 16);
+    return t16 <= slots && t15 > slots ? 16 : 15;
+}
+
+void fused32_step(const FusedPlanes32 &in, const FusedPlanes32 &out, const Consts &C, int64_t N, int tile_rows,
+                  hipStream_t st)
 {
     const K32 k = make_k32(C);
-    if (C.model == MPD_MODEL_AC_HCZ) fused32_launch<MPD_MODEL_AC_HCZ>(k, in, out, C, N, st);
-    else fused32_launch<MPD_MODEL_AC_P>(k, in, out, C, N, st);
+    if (C.model == MPD_MODEL_AC_HCZ) {
+        if (tile_rows == 16) fused32_launch<MPD_MODEL_AC_HCZ, 16>(k, in, out, C, N, st);
+        else fused32_launch<MPD_MODEL_AC_HCZ, 15>(k, in, out, C, N, st);
+    } else {
+        if (tile_rows == 16) fused32_launch<MPD_MODEL_AC_P, 16>(k, in, out, C, N, st);
+        else fused32_launch<MPD_MODEL_AC_P, 15>(k, in, out, C, N, st);
+    }
 }
 
 }  // namespace mpd

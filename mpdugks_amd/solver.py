@@ -24,6 +24,8 @@ _SYMBOLS = {
     "mpd_create": ([C.c_void_p, C.c_void_p], C.c_int),
     "mpd_destroy": ([C.c_void_p], C.c_int),
     "mpd_set_path": ([C.c_void_p, C.c_int32], C.c_int),
+    "mpd_set_tile_rows": ([C.c_void_p, C.c_int32], C.c_int),
+    "mpd_get_tile_rows": ([C.c_void_p, C.POINTER(C.c_int32)], C.c_int),
     "mpd_upload_state": ([C.c_void_p] + [C.c_void_p] * 5, C.c_int),
     "mpd_download_state": ([C.c_void_p] + [C.c_void_p] * 5, C.c_int),
     "mpd_step": ([C.c_void_p, C.c_int64], C.c_int),
@@ -182,6 +184,15 @@ class Solver:
 
     def set_path(self, path):
         _check(lib().mpd_set_path(self._h, int(path)), "mpd_set_path")
+
+    def set_tile_rows(self, rows):
+        """FP32-bulk Allen-Cahn tile height: 0 chosen per mesh, 15 or 16."""
+        _check(lib().mpd_set_tile_rows(self._h, int(rows)), "mpd_set_tile_rows")
+
+    def tile_rows(self):
+        r = C.c_int32()
+        _check(lib().mpd_get_tile_rows(self._h, C.byref(r)), "mpd_get_tile_rows")
+        return r.value
 
     def init_drops(self, drops, sign=-1, p0=1.0, mu0=0.1):
         """Start state built on the GPU (mpd_device_init_drops): drops = [(cx, cy, r), ...]."""
