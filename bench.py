@@ -48,11 +48,12 @@ def run_gpu(cfg, st, steps, warmup, path):
     wall = time.perf_counter() - t0
     avg_ms, nlaunch = s.kernel_timing()
     wall_free = wall
+    rows = s.tile_rows()
     red = s.reductions()
     out = s.download()
     s.close()
     return {"wall_s": wall, "wall_s_untimed": wall_free, "kernel_ms": avg_ms, "launches": nlaunch,
-            "state": out, "sums": red}
+            "state": out, "sums": red, "tile_rows": rows}
 
 
 def ensemble_bench(ncases, steps, seed=0, solve_steps=20000):
@@ -301,10 +302,11 @@ def main():
     err_floor = {k: rel(pert.m[abi.M[k]], ref64.m[abi.M[k]]) for k in ("rho", "phi", "p")}
 
     bytes_alg = 2 * S_ALG_HCZ * 4  # FP32-bulk
+    kernel = "k_fused32<AC_HCZ,BB walls,16x%d tiles>" % head["tile_rows"]
     traffic = None
     try:  # HBM bytes per launch from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE runs
         with open(os.path.join(ROOT, "profiles", "traffic.json")) as fh:
-            rec = json.load(fh).get("k_fused32<AC_HCZ,walls> rayleigh_taylor_%dx%d" % (nx, ny))
+            rec = json.load(fh).get("%s rayleigh_taylor_%dx%d" % (kernel, nx, ny))
         traffic = rec["traffic_bytes"] if rec else None
     except (OSError, ValueError):
         traffic = None
@@ -328,7 +330,7 @@ def main():
                    "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-                     "kernel": "k_fused32<AC_HCZ,walls>",
+                     "kernel": kernel,
                      "alg_bytes_per_cell": bytes_alg, "kernel_ms": head["kernel_ms"]},
         "fp64_plain_mlups": round(results["fp64_plain"]["mlups"], 2),
         "fp64_fused_mlups": round(results["fp64_fused"]["mlups"], 2),
