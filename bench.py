@@ -74,7 +74,8 @@ def ensemble_bench(ncases, steps, seed=0, solve_steps=20000):
         members.append(s)
     setup = time.perf_counter() - t0
     ncells = batch.size()[1]
-    batch.step(20)
+    batch.step(20)                     # includes the tile-height tuning (8 + 8 timed steps)
+    rows, ms15, ms16 = batch.tile_rows()
     batch.timing()
     t0 = time.perf_counter()
     batch.step(steps)
@@ -107,6 +108,7 @@ def ensemble_bench(ncases, steps, seed=0, solve_steps=20000):
             "solves_per_s_fp64_plain": ncases / t_solve_plain,
             "speedup_vs_fp64_plain": t_solve_plain / t_solve_batch,
             "fp64_plain_ms_per_step_by_side": {str(k): v * 1e3 for k, v in per_side.items()},
+            "tile_rows": rows, "tuning_ms_per_step": {"16x15": ms15, "16x16": ms16},
             "setup_s": setup, "timed_steps": steps,
             "nonfinite_in_first_8": int(sum(r["nonfinite"] for r in red))}
 

@@ -210,6 +210,16 @@ int mpd_batch_size(const mpd_batch *b, int32_t *ncases, int64_t *ncells);
 int mpd_batch_step(mpd_batch *b, int64_t nsteps);
 /* Mean wall time per batch step on the batch stream since the last call. */
 int mpd_batch_timing(mpd_batch *b, double *avg_ms, int64_t *steps);
+/* Tile height of the batched launch (tiles of 16 x rows cells).  0 (the
+ * default) tunes it: the first mpd_batch_step of at least 16 steps after the
+ * members change times 8 steps with 16x15 tiles and 8 with 16x16 (ordinary
+ * steps of the run: with periodic and BB walls both give the same state bit
+ * for bit) and keeps the faster.  15 or 16 forces one.  Batches with NEE walls
+ * use 16x15 unless forced, so that their results do not depend on a timing.
+ * mpd_batch_tile_rows returns the height in use and the last tuning's ms per
+ * step of each (0 where not measured).  No counterpart in the reference. */
+int mpd_batch_set_tile_rows(mpd_batch *b, int32_t rows);
+int mpd_batch_tile_rows(const mpd_batch *b, int32_t *rows, double *ms_rows15, double *ms_rows16);
 
 /* ---- host-side initial states (upload layout; no device work) ---- */
 

@@ -78,6 +78,10 @@ struct mpd_batch {
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     double acc_ms = 0.0;
     int64_t steps = 0;
+    int rows_req = 0;             // tile height: 0 tuned, or 15 / 16
+    int rows = 15;                // height of the current tables
+    bool tune_pending = false;    // members changed since the last tuning
+    double tune_ms[2] = {0.0, 0.0};  // last tuning: ms per step with 15 / 16 rows
 };
 
 namespace {
@@ -731,6 +735,7 @@ int mpd_batch_add(mpd_batch *b, mpd_handle *h)
     b->members.push_back(h);
     h->batch = b;
     b->dirty = true;
+    b->tune_pending = true;
     return 0;
 }
 
@@ -746,6 +751,7 @@ int mpd_batch_remove(mpd_batch *b, mpd_handle *h)
         }
     h->batch = nullptr;
     b->dirty = true;
+    b->tune_pending = true;
     return 0;
 }
 
@@ -759,7 +765,13 @@ int mpd_batch_size(const mpd_batch *b, int32_t *ncases, int64_t *ncells)
     return 0;
 }
 
-static int batch_rebuild(mpd_batch *b)
+static bool batch_nee(const mpd_batch *b)
+{
+    return !b->members.empty() && b->members[0]->cfg.bc_y == MPD_BC_WALL_NEE;
+}
+
+// device tables of the current members with b->rows tile rows (no graph)
+static int batch_tables(mpd_batch *b)
 {
     batch_drop_graph(b);
     std::vector<BatchMember> m(b->members.size());
@@ -771,8 +783,14 @@ static int batch_rebuild(mpd_batch *b)
         m[k].pl[1] = fused32_planes(h->F, 1);
         m[k].cur = h->F.cur;
     }
-    if (batch32_plan(b->plan, m.data(), (int)m.size(), b->stream))
+    if (batch32_plan(b->plan, m.data(), (int)m.size(), b->rows, b->stream))
         return fail(MPD_ERR_DEVICE, "batch tables: %s", hipGetErrorString(hipGetLastError()));
+    return 0;
+}
+
+static int batch_rebuild(mpd_batch *b)
+{
+    if (int r = batch_tables(b)) return r;
     // one graph = two steps (in->out, out->in), so the planes are back in place after each replay
     HIPCHK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
     batch32_step(b->plan, 0, b->stream);
@@ -783,11 +801,82 @@ static int batch_rebuild(mpd_batch *b)
     return 0;
 }
 
+// Tile-height autotuner: TUNE_STEPS steps of the batch with 16x15 tiles, as
+// many with 16x16 tiles; of each, the first TUNE_WARM warm the new tables and
+// code, the rest are timed with events on the batch stream.  The faster
+// height is kept until the members change.  The two heights give the same
+// state bit for bit with periodic and bounce-back walls (each cell and face is
+// computed by the same code), so the tuning steps are ordinary steps of the
+// run.  Even counts leave every member's current plane set in place.
+constexpr int TUNE_STEPS = 8, TUNE_WARM = 2;
+
+static int batch_tune(mpd_batch *b)
+{
+    const int cand[2] = {15, 16};
+    for (int c = 0; c < 2; ++c) {
+        b->rows = cand[c];
+        if (int r = batch_tables(b)) return r;
+        HIPCHK(hipEventRecord(b->ev1, b->stream));
+        for (int s = 0; s < TUNE_STEPS; ++s) {
+            if (s == TUNE_WARM) HIPCHK(hipEventRecord(b->ev0, b->stream));
+            batch32_step(b->plan, s & 1, b->stream);
+        }
+        HIPCHK(hipGetLastError());
+        hipEvent_t e_all = nullptr;
+        HIPCHK(hipEventCreate(&e_all));
+        hipError_t e = hipEventRecord(e_all, b->stream);
+        if (e == hipSuccess) e = hipEventSynchronize(e_all);
+        float ms = 0.f, ms_all = 0.f;
+        if (e == hipSuccess) e = hipEventElapsedTime(&ms, b->ev0, e_all);
+        if (e == hipSuccess) e = hipEventElapsedTime(&ms_all, b->ev1, e_all);
+        (void)hipEventDestroy(e_all);
+        if (e != hipSuccess) return fail(MPD_ERR_DEVICE, "batch tuning: %s", hipGetErrorString(e));
+        b->tune_ms[c] = ms / (TUNE_STEPS - TUNE_WARM);
+        b->acc_ms += ms_all;
+        b->steps += TUNE_STEPS;
+    }
+    b->rows = b->tune_ms[1] < b->tune_ms[0] ? 16 : 15;
+    b->tune_pending = false;
+    b->dirty = true;
+    return 0;
+}
+
+int mpd_batch_set_tile_rows(mpd_batch *b, int32_t rows)
+{
+    if (!b) return fail(MPD_ERR_INVALID, "null batch");
+    if (rows != 0 && rows != 15 && rows != 16) return fail(MPD_ERR_INVALID, "tile rows %d (0, 15 or 16)", rows);
+    HIPCHK(hipStreamSynchronize(b->stream));
+    b->rows_req = rows;
+    b->tune_pending = rows == 0;
+    b->dirty = true;
+    return 0;
+}
+
+int mpd_batch_tile_rows(const mpd_batch *b, int32_t *rows, double *ms_rows15, double *ms_rows16)
+{
+    if (!b) return fail(MPD_ERR_INVALID, "null batch");
+    if (rows) *rows = b->rows_req ? b->rows_req : b->rows;
+    if (ms_rows15) *ms_rows15 = b->tune_ms[0];
+    if (ms_rows16) *ms_rows16 = b->tune_ms[1];
+    return 0;
+}
+
 int mpd_batch_step(mpd_batch *b, int64_t nsteps)
 {
     if (!b) return fail(MPD_ERR_INVALID, "null batch");
     if (nsteps < 0) return fail(MPD_ERR_INVALID, "nsteps < 0");
     if (nsteps == 0 || b->members.empty()) return 0;
+    if (b->rows_req) {
+        if (b->rows != b->rows_req) b->dirty = true;
+        b->rows = b->rows_req;
+    } else if (batch_nee(b)) {
+        if (b->rows != 15) b->dirty = true;
+        b->rows = 15;   // NEE: the heights differ by FP32 rounding; keep results independent of timing
+    } else if (b->tune_pending && nsteps >= 2 * TUNE_STEPS) {
+        if (int r = batch_tune(b)) return r;
+        nsteps -= 2 * TUNE_STEPS;
+        if (nsteps == 0) return 0;
+    }
     if (b->dirty)
         if (int r = batch_rebuild(b)) return r;
     HIPCHK(hipEventRecord(b->ev0, b->stream));

@@ -81,7 +81,7 @@ struct BatchMember {
 struct Batch32Plan;
 Batch32Plan *batch32_new();
 void batch32_free(Batch32Plan *P);
-int batch32_plan(Batch32Plan *P, const BatchMember *m, int n, hipStream_t st);
+int batch32_plan(Batch32Plan *P, const BatchMember *m, int n, int tile_rows, hipStream_t st);
 // one step of every case; parity 0/1 selects in/out relative to each case's cur
 void batch32_step(const Batch32Plan *P, int parity, hipStream_t st);
 // FP32 planes of a fused state (set 0/1)

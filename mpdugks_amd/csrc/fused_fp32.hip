@@ -62,7 +62,6 @@ struct Tile {
     static_assert(NXF - NT <= 64 && NYF - NT <= 64, "second face pass: one wave per direction");
     static_assert(2 * NFL * FS <= NBP * PS, "face values must fit over the bar-plus planes");
 };
-constexpr int BATCH_TY = 15;               // tile height of batched launches
 
 struct K32 {
     float ms;        // MaSpan
@@ -911,8 +910,8 @@ __global__ void __launch_bounds__(NT) k_fused32(K32 k, int nx, int ny, FusedPlan
 }
 
 // Batched step: every workgroup takes one (case, tile) entry of a work list
-// built on the host; the cases share the model and the y boundary.
-template <int MODEL, int WALLS>
+// built on the host; the cases share the model, the y boundary and the tile height.
+template <int MODEL, int WALLS, int TY>
 __global__ void __launch_bounds__(NT) k_fused32_batch(const BatchCase *cases, const int2 *work, int nwork, int parity)
 {
     const int b = blockIdx.x;
@@ -922,8 +921,7 @@ __global__ void __launch_bounds__(NT) k_fused32_batch(const BatchCase *cases, co
     const BatchCase &c = cases[w.x];
     const int sel = c.cur ^ parity;
     const int ntx = (c.nx + TX - 1) / TX;
-    tile32<MODEL, WALLS, BATCH_TY>(c.k, c.nx, c.ny, c.pl[sel], c.pl[sel ^ 1], c.N, (w.y % ntx) * TX,
-                                   (w.y / ntx) * BATCH_TY);
+    tile32<MODEL, WALLS, TY>(c.k, c.nx, c.ny, c.pl[sel], c.pl[sel ^ 1], c.N, (w.y % ntx) * TX, (w.y / ntx) * TY);
 }
 
 K32 make_k32(const Consts &C)
@@ -1001,7 +999,7 @@ int walls_code(const Consts &C) { return C.walls ? (C.nee ? 2 : 1) : 0; }
 struct Batch32Plan {
     BatchCase *d_cases = nullptr;
     int2 *d_work = nullptr;
-    int ncases = 0, nwork = 0, model = 0, walls = 0;
+    int ncases = 0, nwork = 0, model = 0, walls = 0, tile_rows = 15;
     size_t cap_cases = 0, cap_work = 0;
 };
 
@@ -1017,8 +1015,9 @@ void batch32_free(Batch32Plan *P)
 
 // (Re)build the device tables: cases in order, tiles row-major per case, then
 // dealt so that XCD b%8 receives a contiguous run of the list.
-int batch32_plan(Batch32Plan *P, const BatchMember *m, int n, hipStream_t st)
+int batch32_plan(Batch32Plan *P, const BatchMember *m, int n, int tile_rows, hipStream_t st)
 {
+    const int ty = tile_rows == 16 ? 16 : 15;
     std::vector<BatchCase> cs((size_t)n);
     std::vector<int2> list;
     for (int c = 0; c < n; ++c) {
@@ -1031,7 +1030,7 @@ int batch32_plan(Batch32Plan *P, const BatchMember *m, int n, hipStream_t st)
         b.pl[0] = m[c].pl[0];
         b.pl[1] = m[c].pl[1];
         b.cur = m[c].cur;
-        const int ntiles = ((C.nx + TX - 1) / TX) * ((C.ny + BATCH_TY - 1) / BATCH_TY);
+        const int ntiles = ((C.nx + TX - 1) / TX) * ((C.ny + ty - 1) / ty);
         for (int t = 0; t < ntiles; ++t) list.push_back(make_int2(c, t));
     }
     const int per = ((int)list.size() + 7) / 8;
@@ -1060,23 +1059,32 @@ int batch32_plan(Batch32Plan *P, const BatchMember *m, int n, hipStream_t st)
     P->nwork = (int)work.size();
     P->model = n ? m[0].C->model : 0;
     P->walls = n ? walls_code(*m[0].C) : 0;
+    P->tile_rows = ty;
     return 0;
 }
 
-template <int MODEL>
+template <int MODEL, int TY>
 void batch32_launch(const Batch32Plan *P, int parity, hipStream_t st)
 {
     dim3 g((unsigned)P->nwork), b(NT);
-    if (P->walls == 2) hipLaunchKernelGGL((k_fused32_batch<MODEL, 2>), g, b, 0, st, P->d_cases, P->d_work, P->nwork, parity);
-    else if (P->walls) hipLaunchKernelGGL((k_fused32_batch<MODEL, 1>), g, b, 0, st, P->d_cases, P->d_work, P->nwork, parity);
-    else hipLaunchKernelGGL((k_fused32_batch<MODEL, 0>), g, b, 0, st, P->d_cases, P->d_work, P->nwork, parity);
+    if (P->walls == 2)
+        hipLaunchKernelGGL((k_fused32_batch<MODEL, 2, TY>), g, b, 0, st, P->d_cases, P->d_work, P->nwork, parity);
+    else if (P->walls)
+        hipLaunchKernelGGL((k_fused32_batch<MODEL, 1, TY>), g, b, 0, st, P->d_cases, P->d_work, P->nwork, parity);
+    else
+        hipLaunchKernelGGL((k_fused32_batch<MODEL, 0, TY>), g, b, 0, st, P->d_cases, P->d_work, P->nwork, parity);
 }
 
 void batch32_step(const Batch32Plan *P, int parity, hipStream_t st)
 {
     if (!P->nwork) return;
-    if (P->model == MPD_MODEL_AC_HCZ) batch32_launch<MPD_MODEL_AC_HCZ>(P, parity, st);
-    else batch32_launch<MPD_MODEL_AC_P>(P, parity, st);
+    if (P->model == MPD_MODEL_AC_HCZ) {
+        if (P->tile_rows == 16) batch32_launch<MPD_MODEL_AC_HCZ, 16>(P, parity, st);
+        else batch32_launch<MPD_MODEL_AC_HCZ, 15>(P, parity, st);
+    } else {
+        if (P->tile_rows == 16) batch32_launch<MPD_MODEL_AC_P, 16>(P, parity, st);
+        else batch32_launch<MPD_MODEL_AC_P, 15>(P, parity, st);
+    }
 }
 
 template <int MODEL, int TY>

@@ -42,6 +42,8 @@ _SYMBOLS = {
     "mpd_batch_size": ([C.c_void_p, C.c_void_p, C.c_void_p], C.c_int),
     "mpd_batch_step": ([C.c_void_p, C.c_int64], C.c_int),
     "mpd_batch_timing": ([C.c_void_p, C.c_void_p, C.c_void_p], C.c_int),
+    "mpd_batch_set_tile_rows": ([C.c_void_p, C.c_int32], C.c_int),
+    "mpd_batch_tile_rows": ([C.c_void_p, C.POINTER(C.c_int32), C.POINTER(C.c_double), C.POINTER(C.c_double)], C.c_int),
     "mpd_init_ac_drop": ([C.c_void_p, C.c_double, C.c_double, C.c_double, C.c_int32, C.c_double,
                           C.c_double, C.c_void_p, C.c_void_p, C.c_void_p], C.c_int),
     "mpd_init_ac_interface": ([C.c_void_p, C.c_void_p, C.c_int32, C.c_double, C.c_void_p, C.c_void_p,
@@ -275,6 +277,16 @@ class Batch:
         ms, n = C.c_double(), C.c_int64()
         _check(lib().mpd_batch_timing(self._b, C.byref(ms), C.byref(n)), "mpd_batch_timing")
         return ms.value, n.value
+
+    def set_tile_rows(self, rows):
+        """0: tuned per membership (default); 15 or 16: forced."""
+        _check(lib().mpd_batch_set_tile_rows(self._b, int(rows)), "mpd_batch_set_tile_rows")
+
+    def tile_rows(self):
+        """(rows in use, ms/step measured with 15 rows, with 16 rows) of the last tuning."""
+        r, a, b = C.c_int32(), C.c_double(), C.c_double()
+        _check(lib().mpd_batch_tile_rows(self._b, C.byref(r), C.byref(a), C.byref(b)), "mpd_batch_tile_rows")
+        return r.value, a.value, b.value
 
     def close(self):
         if getattr(self, "_b", None) and self._b.value:

@@ -88,3 +88,52 @@ def test_batch_nee_walls_matches_individual_cases():
         ga, gc = a.download(), c.download()
         assert np.array_equal(ga.f, gc.f) and np.array_equal(ga.h, gc.h) and np.array_equal(ga.m, gc.m)
     b.close()
+
+
+def test_batch_tile_heights_and_autotuner():
+    """Forced 16x15 and 16x16 batches give the same states bit for bit; the
+    autotuner's 16 timed steps are ordinary steps (the tuned batch equals the
+    cases stepped alone) and it keeps the faster height."""
+    specs = cases.ensemble_specs(5, seed=11, sides=(64, 96, 128))
+    runs = {}
+    for mode in (15, 16, 0):
+        members = [_primed(sp) for sp in specs]
+        b = solver.Batch()
+        for s in members:
+            b.add(s)
+        b.set_tile_rows(mode)
+        b.step(40)
+        b.step(3)
+        rows, ms15, ms16 = b.tile_rows()
+        if mode:
+            assert rows == mode
+        else:
+            assert ms15 > 0 and ms16 > 0 and rows == (16 if ms16 < ms15 else 15), (rows, ms15, ms16)
+        runs[mode] = [m.download() for m in members]
+        b.close()
+    single = [_primed(sp) for sp in specs]
+    for s in single:
+        s.step(43)
+    ref = [s.download() for s in single]
+    for mode in (15, 16, 0):
+        for g, r in zip(runs[mode], ref):
+            assert np.array_equal(g.f, r.f) and np.array_equal(g.h, r.h) and np.array_equal(g.m, r.m), mode
+
+
+def test_batch_nee_keeps_16x15_unless_forced():
+    cfg = default_config(abi.MPD_MODEL_AC_P, nx=48, ny=40)
+    cfg.precision = abi.MPD_FP32_BULK
+    cfg.bc_y = abi.MPD_BC_WALL_NEE
+    s = solver.Solver(cfg, path=abi.MPD_PATH_FUSED)
+    s.upload(solver.ac_drop(cfg, radius=10.0))
+    s.step(2)
+    b = solver.Batch()
+    b.add(s)
+    b.step(40)
+    assert b.tile_rows() == (15, 0.0, 0.0)
+    b.set_tile_rows(16)
+    b.step(2)
+    assert b.tile_rows()[0] == 16
+    with pytest.raises(solver.MpdError):
+        b.set_tile_rows(17)
+    b.close()
