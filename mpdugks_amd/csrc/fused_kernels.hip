@@ -1,6 +1,6 @@
 // Fused DUGKS step: one kernel launch per time step for the Allen-Cahn models
-// (AC-p, AC+HCZ) with the CD face scheme, periodic in x, periodic or
-// bounce-back walls in y.
+// (AC-p, AC+HCZ) with the CD face scheme, periodic in x; y periodic or walls
+// (bounce-back or non-equilibrium extrapolation).
 //
 // Derived state.  Between steps only what the reference carries and cannot
 // recompute is kept, in structure-of-arrays planes (ping-pong):
@@ -120,6 +120,7 @@ __global__ void __launch_bounds__(NTHREADS) k_fused_step(Consts C, Planes<T> in,
     __shared__ T s_bp[2 * Q9][R1N];     // bar-plus: h in 0..8, f in 9..17; later the y-face fluxes
     __shared__ T s_mac[6][R1N];         // px, py, rx, ry, Fx, Fy
     __shared__ T s_u[2 * Q9 * NXF];     // phi/phi_lag (stage A-B), then the x-face fluxes
+    __shared__ T s_nee[WALLS == 2 ? 2 * TX * 2 * Q9 : 1];  // NEE wall faces: [bottom, top][column][h, f]
     T *s_phi = s_u, *s_phl = s_u + R2N;
     static_assert(2 * R2N <= 2 * Q9 * NXF, "union too small");
 
@@ -195,6 +196,14 @@ __global__ void __launch_bounds__(NTHREADS) k_fused_step(Consts C, Planes<T> in,
         s_mac[0][k] = m.px; s_mac[1][k] = m.py; s_mac[2][k] = m.rx;
         s_mac[3][k] = m.ry; s_mac[4][k] = m.Fx; s_mac[5][k] = m.Fy;
         const bool interior = rx >= 1 && rx <= TX && ry >= 1 && ry <= TY && i0 - 1 + rx < nx && j0 - 1 + ry < ny;
+        if (WALLS == 2 && interior && (gj == 0 || gj == ny - 1)) {
+            // Wall_3_NEE values of this cell's wall face (the phase path's
+            // expressions: same macros, same tilde populations)
+            T *v = s_nee + ((gj == 0 ? 0 : 1) * TX + (rx - 1)) * 2 * Q9;
+            const T xc = ((T)gi + (T)0.5) * (T)C.dx;
+            nee_face<T, MODEL>(C, m, ht, ft, xc, (T)(gj == 0 ? 0 : ny) * (T)C.dx, xc,
+                               ((T)gj + (T)0.5) * (T)C.dx, v, v + Q9);
+        }
         if (interior && (HCZ || WALLS)) out.phl[n] = m.phi;
         if (interior && HCZ) {
             out.ppu[n] = s.ppu;
@@ -287,6 +296,12 @@ __global__ void __launch_bounds__(NTHREADS) k_fused_step(Consts C, Planes<T> in,
 #pragma unroll
         for (int q = 0; q < Q9; ++q) face_bgk<T, MODEL>(C, fm, q, hb[q], fb[q], ah, bh, ch, yh[q], yf[q]);
         const int gj = j0 + fj;
+        if (WALLS == 2 && (gj == 0 || gj == ny)) {
+            // Wall_3_NEE: the wall-row cell's values (stage B), then bounce-back
+            const T *v = s_nee + ((gj == 0 ? 0 : 1) * TX + fi) * 2 * Q9;
+#pragma unroll
+            for (int q = 0; q < Q9; ++q) { yh[q] = v[q]; yf[q] = v[Q9 + q]; }
+        }
         if (WALLS && (gj == 0 || gj == ny)) {
             // Wall_3_BB (Ark/Cpp/Boundary.cpp:118-139): incoming take the partner's value
             T h0[Q9], f0[Q9];
@@ -502,10 +517,12 @@ void step_t<double>(FusedState &F, const Consts &C, hipStream_t st)
 {
     using T = double;
     if (C.model == MPD_MODEL_AC_HCZ) {
-        if (C.walls) launch_step<T, MPD_MODEL_AC_HCZ, 1>(F, C, st);
+        if (C.nee) launch_step<T, MPD_MODEL_AC_HCZ, 2>(F, C, st);
+        else if (C.walls) launch_step<T, MPD_MODEL_AC_HCZ, 1>(F, C, st);
         else launch_step<T, MPD_MODEL_AC_HCZ, 0>(F, C, st);
     } else {
-        if (C.walls) launch_step<T, MPD_MODEL_AC_P, 1>(F, C, st);
+        if (C.nee) launch_step<T, MPD_MODEL_AC_P, 2>(F, C, st);
+        else if (C.walls) launch_step<T, MPD_MODEL_AC_P, 1>(F, C, st);
         else launch_step<T, MPD_MODEL_AC_P, 0>(F, C, st);
     }
 }
@@ -548,10 +565,8 @@ bool fused_supported(const mpd_config &cfg)
     // FP32-bulk pseudopotential D2Q9: the FP32 phase kernels of pp32.hip
     if (cfg.model == MPD_MODEL_PP_D2Q9)
         return cfg.precision == MPD_FP32_BULK && cfg.flux == MPD_FLUX_CD && cfg.bc_y == MPD_BC_PERIODIC;
-    // walls: bounce-back in both precisions; non-equilibrium extrapolation in
-    // the FP32 tile kernel only (FP64 NEE runs on the phase path)
     return (cfg.model == MPD_MODEL_AC_HCZ || cfg.model == MPD_MODEL_AC_P) && cfg.flux == MPD_FLUX_CD &&
-           (cfg.bc_y != MPD_BC_WALL_NEE || cfg.precision == MPD_FP32_BULK) && cfg.nx >= 4 && cfg.ny >= 4;
+           cfg.nx >= 4 && cfg.ny >= 4;
 }
 
 bool fused_needs_priming(const mpd_config &cfg) { return cfg.model != MPD_MODEL_PP_D2Q9; }

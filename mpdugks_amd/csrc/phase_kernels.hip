@@ -179,11 +179,8 @@ __global__ void __launch_bounds__(256) k_faces_ac(DevState S, Consts C)
 #pragma unroll
     for (int q = 0; q < Q9; ++q) face_bgk<double, MODEL>(C, fm, q, hb[q], fb[q], ah, bh, ch, hh[q], ff[q]);
     if (C.walls && dir == 1 && (j == 0 || j == C.ny) && C.nee) {
-        // Wall_3_NEE (Ark/Cpp/Boundary.cpp:140-184): equilibrium of face macros
-        // extrapolated from the fluid cell (Phi, Rho along the cell's Phi_x/y
-        // -- the AC source vector -- and Rho_x/y; p of the cell; U = V = 0, the
-        // face macros are never set otherwise) plus the cell's non-equilibrium
-        // part 2 tau/(2 tau + dt) (x~ - x_eq + h x_so); bounce-back follows
+        // Wall_3_NEE (Ark/Cpp/Boundary.cpp:140-184) from the fluid cell's
+        // macros and tilde populations; bounce-back follows
         const int jc = j == 0 ? 0 : C.ny - 1;
         const size_t N = S.N, n = (size_t)jc * C.nx + i;
         const double *m = S.m;
@@ -194,25 +191,10 @@ __global__ void __launch_bounds__(256) k_faces_ac(DevState S, Consts C)
         cm.px = m[MPD_M_PHIX * N + n]; cm.py = m[MPD_M_PHIY * N + n];
         cm.rx = m[MPD_M_RHOX * N + n]; cm.ry = m[MPD_M_RHOY * N + n];
         cm.Fx = m[MPD_M_FX * N + n];   cm.Fy = m[MPD_M_FY * N + n];
-        const double xc = (i + 0.5) * C.dx, yc = (jc + 0.5) * C.dx;
-        const double phf = cm.phi + cm.px * (xf - xc) + cm.py * (yf - yc);
-        const double rhf = cm.rho + cm.rx * (xf - xc) + cm.ry * (yf - yc);
-        const double pf = cm.p, Uf = 0.0, Vf = 0.0, uu = Uf * Uf + Vf * Vf;
-        const double hneq = 2.0 * C.taum / (2.0 * C.taum + C.dt);
-        const double fneq = 2.0 * cm.tau / (2.0 * cm.tau + C.dt);
+        double ht[Q9], ft[Q9];
 #pragma unroll
-        for (int q = 0; q < Q9; ++q) {
-            double heq, feq, hso, fso;
-            cell_eq_so<double, MODEL>(C, cm, q, heq, feq, hso, fso);
-            const double ex = C.xi[0][q], ey = C.xi[1][q], w = C.w[q];
-            const double u1 = (ex * Uf + ey * Vf) / C.rt;
-            const double G = u1 + 0.5 * u1 * u1 - uu * C.lambda0;
-            const double heqf = w * phf * (1.0 + u1);
-            const double feqf = MODEL == MPD_MODEL_AC_HCZ ? w * (pf + rhf * C.rt * G)
-                                                          : w * (pf / C.rt + rhf * G) - (double)(q == 0) * pf / C.rt;
-            hh[q] = heqf + hneq * (S.h[q * N + n] - heq + C.hdt * hso);
-            ff[q] = feqf + fneq * (S.f[q * N + n] - feq + C.hdt * fso);
-        }
+        for (int q = 0; q < Q9; ++q) { ht[q] = S.h[q * N + n]; ft[q] = S.f[q * N + n]; }
+        nee_face<double, MODEL>(C, cm, ht, ft, xf, yf, (i + 0.5) * C.dx, (jc + 0.5) * C.dx, hh, ff);
     }
     if (C.walls && dir == 1 && (j == 0 || j == C.ny)) {
         // Wall_3_BB (Ark/Cpp/Boundary.cpp:118-139)

@@ -220,6 +220,37 @@ __device__ inline void face_bgk(const Consts &C, const FaceMacro<T> &fm, int q, 
     fout = r;
 }
 
+// Wall_3_NEE (Ark/Cpp/Boundary.cpp:140-184) for the Allen-Cahn models: face
+// values of every population at the wall face (xf, yf) of fluid cell centre
+// (xc, yc) with macros cm and tilde populations ht, ft.  The face macros are
+// extrapolated from the cell -- Phi, Rho along the cell's Phi_x/y (the AC
+// source vector at this point of the step) and Rho_x/y; p of the cell; U = V
+// = 0 (the reference never sets the face velocity otherwise) -- and the
+// cell's non-equilibrium part 2 tau/(2 tau + dt) (x~ - x_eq + dt/2 x_so) is
+// added.  Bounce-back follows, in the caller.
+template <typename T, int MODEL>
+__device__ inline void nee_face(const Consts &C, const CellMacro<T> &cm, const T *ht, const T *ft, T xf, T yf,
+                                T xc, T yc, T *hh, T *ff)
+{
+    const T phf = cm.phi + cm.px * (xf - xc) + cm.py * (yf - yc);
+    const T rhf = cm.rho + cm.rx * (xf - xc) + cm.ry * (yf - yc);
+    const T pf = cm.p, Uf = 0, Vf = 0, uu = Uf * Uf + Vf * Vf;
+    const T hneq = (T)2.0 * (T)C.taum / ((T)2.0 * (T)C.taum + (T)C.dt);
+    const T fneq = (T)2.0 * cm.tau / ((T)2.0 * cm.tau + (T)C.dt);
+    for (int q = 0; q < Q9; ++q) {
+        T heq, feq, hso, fso;
+        cell_eq_so<T, MODEL>(C, cm, q, heq, feq, hso, fso);
+        const T ex = (T)C.xi[0][q], ey = (T)C.xi[1][q], w = (T)C.w[q];
+        const T u1 = (ex * Uf + ey * Vf) / (T)C.rt;
+        const T G = u1 + (T)0.5 * u1 * u1 - uu * (T)C.lambda0;
+        const T heqf = w * phf * ((T)1.0 + u1);
+        const T feqf = MODEL == MPD_MODEL_AC_HCZ ? w * (pf + rhf * (T)C.rt * G)
+                                                 : w * (pf / (T)C.rt + rhf * G) - (T)(q == 0) * pf / (T)C.rt;
+        hh[q] = heqf + hneq * (ht[q] - heq + (T)C.hdt * hso);
+        ff[q] = feqf + fneq * (ft[q] - feq + (T)C.hdt * fso);
+    }
+}
+
 // Update_MacroVar for the Allen-Cahn models (HCZ Ark/Lib/D2Q9HCZ.cpp:133-182,
 // AC-p Ark/Lib/D2Q9AC.cpp:146-231).  Fx/Fy/rx/ry are the sources of the
 // previous MacroSource.  Writes phi, rho, U, V, p, mu, tau.

@@ -23,6 +23,9 @@ CASES = [
     ("hcz_periodic_70x45", dict(model="ac_hcz", nx=70, ny=45, radius=15.0)),
     ("hcz_walls_70x47", dict(model="ac_hcz", nx=70, ny=47, walls=True, radius=15.0)),
     ("hcz_periodic_8x6", dict(model="ac_hcz", nx=8, ny=6, radius=2.5)),
+    ("hcz_nee_64x48_droplet", dict(model="ac_hcz", nx=64, ny=48, walls=True, nee=True, sign=-1, radius=21.0)),
+    ("acp_nee_70x47", dict(model="ac_p", nx=70, ny=47, walls=True, nee=True, radius=20.0)),
+    ("hcz_nee_20x12", dict(model="ac_hcz", nx=20, ny=12, walls=True, nee=True, sign=-1, radius=4.0)),
 ]
 
 
