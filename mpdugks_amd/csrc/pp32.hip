@@ -35,9 +35,122 @@ __device__ __forceinline__ int wrapp(int i, int n) { return i < 0 ? i + n : (i >
 
 inline unsigned blocks(size_t n) { return (unsigned)((n + 255) / 256); }
 
+// FP32 constants of the step, derived on the host in FP64 from Consts: the
+// reference's expressions with every division by a constant replaced by a
+// product with its reciprocal, and the D2Q9 velocities folded at compile time
+// (xi = MaSpan e, e in {-1, 0, 1}^2).
+struct KPP {
+    float ms;          // MaSpan
+    float ms_rt;       // MaSpan / RT
+    float inv_rt, lambda0, hdt, dt, half_hdt;
+    float ra, rb;      // CD face value, e_n = +1: ra o + rb n, ra = 1/2 + A, rb = 1/2 - A, A = hdt MaSpan / dx
+    float rt4;         // tangential factor hdt MaSpan / (2 twodx)
+    float inv_d2x6;    // 9-point gradient
+    float w[3];        // weights: rest, axis, diagonal
+    float ah, bh, ch;  // face BGK factors
+    float dtv;         // dt / V
+    float flux;        // dx MaSpan: micro-flux per unit e_n
+};
+
+KPP make_kpp(const Consts &C)
+{
+    KPP k;
+    const double ms = C.xi[0][1];
+    k.ms = (float)ms;
+    k.ms_rt = (float)(ms / C.rt);
+    k.inv_rt = (float)(1.0 / C.rt);
+    k.lambda0 = (float)C.lambda0;
+    k.hdt = (float)C.hdt;
+    k.dt = (float)C.dt;
+    k.half_hdt = (float)(0.5 * C.hdt);
+    const double A = C.hdt * ms / C.dx;
+    k.ra = (float)(0.5 + A);
+    k.rb = (float)(0.5 - A);
+    k.rt4 = (float)(C.hdt * ms / (2.0 * C.twodx));
+    k.inv_d2x6 = (float)(1.0 / C.d2x6);
+    k.w[0] = (float)C.w[0];
+    k.w[1] = (float)C.w[1];
+    k.w[2] = (float)C.w[2];
+    k.ah = (float)C.ah_pp;
+    k.bh = (float)C.bh_pp;
+    k.ch = (float)C.ch_pp;
+    k.dtv = (float)C.dtv;
+    k.flux = (float)(C.dx * ms);
+    return k;
+}
+
+// weight class of population q: 0 rest, 1 axis (odd q), 2 diagonal (even q)
+__host__ __device__ constexpr int wcls(int q) { return q == 0 ? 0 : ((q & 1) ? 1 : 2); }
+
+// 9-point isotropic gradient (update_MQ_x/y, Ark/Lib/GradSchemeBasic.h:153-180);
+// c = E, N, W, S, d = NE, NW, SW, SE
+__device__ __forceinline__ void pp_grad9(const KPP &k, const float *c, const float *d, float &gx, float &gy)
+{
+    gx = (4.f * (c[0] - c[2]) + (d[0] - d[1] + d[3] - d[2])) * k.inv_d2x6;
+    gy = (4.f * (c[1] - c[3]) + (d[0] - d[3] + d[1] - d[2])) * k.inv_d2x6;
+}
+
+// CD face value of population q (CD_Interior_phi_Bh, DUGKS2DSolver.cpp:604-687)
+// on a face of direction DIR: owner o, neighbour n, the neighbour's and the
+// owner's tangential pairs (high, low):
+//   1/2 (o + n) - hdt xi_n (n - o)/dx - hdt xi_t (1/2 (tn_hi - tn_lo) + 1/2 (to_hi - to_lo))/twodx
+template <int DIR>
+__device__ __forceinline__ float pp_recon(const KPP &k, int q, float bo, float bn, float tn_hi, float tn_lo,
+                                          float to_hi, float to_lo)
+{
+    const int en = DIR == 0 ? ex9(q) : ey9(q), et = DIR == 0 ? ey9(q) : ex9(q);
+    float v = en > 0 ? fmaf(k.rb, bn, k.ra * bo) : (en < 0 ? fmaf(k.ra, bn, k.rb * bo) : 0.5f * (bo + bn));
+    if (et != 0) v = fmaf(et > 0 ? -k.rt4 : k.rt4, (tn_hi - tn_lo) + (to_hi - to_lo), v);
+    return v;
+}
+
+// Equilibrium and He-Shan-Doolen source of population q (Update_phi_Eq /
+// Update_DVDF_Source, D2Q9PP.cpp:51-62,137-145): f_eq = w rho (1 + u1 + u1^2/2
+// - |u|^2 Lambda0), u1 = xi.u/RT; f_So = F.(xi - u) f_eq / (rho RT).  uul =
+// |u|^2 Lambda0, irr = 1/(rho RT).
+__device__ __forceinline__ float pp_feq(const KPP &k, int q, float rho, float U, float V, float uul, float Fx,
+                                        float Fy, float irr, float &fso)
+{
+    const int ex = ex9(q), ey = ey9(q);
+    const float u1 = k.ms_rt * ((float)ex * U + (float)ey * V);
+    const float feq = k.w[wcls(q)] * rho * (1.f + u1 + 0.5f * u1 * u1 - uul);
+    fso = (Fx * ((float)ex * k.ms - U) + Fy * ((float)ey * k.ms - V)) * feq * irr;
+    return feq;
+}
+
+// bar-plus of one cell (Update_phi_BP, DUGKS2DSolver.cpp:379-401; factors
+// Cell_2D.cpp:120-125): aBP f~ + bBP f_eq + cBP f_So
+__device__ __forceinline__ void pp_barplus(const KPP &k, float rho, float U, float V, float tau, float Fx, float Fy,
+                                           const float *ft, float *fb)
+{
+    const float abp = (2.f * tau - k.hdt) / (2.f * tau + k.dt);
+    const float bbp = 1.f - abp, cbp = tau * bbp;
+    const float uul = (U * U + V * V) * k.lambda0;
+    const float irr = k.inv_rt / rho;
+#pragma unroll
+    for (int q = 0; q < Q9; ++q) {
+        float fso;
+        const float feq = pp_feq(k, q, rho, U, V, uul, Fx, Fy, irr, fso);
+        fb[q] = abp * ft[q] + bbp * feq + cbp * fso;
+    }
+}
+
+// Psi and scG of a density (MacroQuantity::calcPsi, Ark/Header/Macroscopic.h:64-68)
+// in FP64 (near the spinodal p - rho RT is a difference of nearly equal
+// numbers): Psi = sqrt(2 (p - rho RT) / scG) = sqrt(|2 (p - rho RT)|) for
+// scG = +-1, NaN for scG = 0 (0/0); rounded to FP32 once.
+__device__ __forceinline__ void pp_psi(const Consts &C, double rho, float &scg, float &psi)
+{
+    const double p = C.pp_k * carnahan_starling(rho, C.t_cs);
+    const double rrt = rho * C.rt;
+    const int s = (p > rrt) - (p < rrt);
+    scg = (float)s;
+    psi = s != 0 ? (float)sqrt(fabs(2.0 * (p - rrt))) : __builtin_nanf("");
+}
+
 // Update_PseudoForce(cell) (PseudoP.cpp:31-38): F = -scG Psi grad(Psi), from
 // the Psi that the last update left
-__device__ __forceinline__ void pp32_cell_force(const PP32Planes &P, const Consts &C, size_t n, float &Fx, float &Fy)
+__device__ __forceinline__ void pp32_cell_force(const PP32Planes &P, const KPP &k, size_t n, float &Fx, float &Fy)
 {
     const int nx = P.nx, ny = P.ny;
     const int i = (int)(n % nx), j = (int)(n / nx);
@@ -45,7 +158,7 @@ __device__ __forceinline__ void pp32_cell_force(const PP32Planes &P, const Const
     const float c[4] = {S(i + 1, j), S(i, j + 1), S(i - 1, j), S(i, j - 1)};
     const float d[4] = {S(i + 1, j + 1), S(i - 1, j + 1), S(i - 1, j - 1), S(i + 1, j - 1)};
     float gx, gy;
-    grad9<float>(C, c, d, gx, gy);
+    pp_grad9(k, c, d, gx, gy);
     const float sp = -P.scg[n] * P.psi[n];
     Fx = gx * sp;
     Fy = gy * sp;
@@ -58,51 +171,90 @@ __device__ __forceinline__ void pp32_cell_force(const PP32Planes &P, const Const
 // the update; without FORCE the stored force is used (the first step after a
 // conversion from the explicit state, whose force may not derive from Psi).
 template <bool FORCE>
-__global__ void __launch_bounds__(256) k_pp32_phase1(PP32Planes P, Consts C)
+__global__ void __launch_bounds__(256) k_pp32_phase1(PP32Planes P, KPP k)
 {
     const size_t n = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     const size_t N = P.N;
     if (n >= N) return;
-    CellMacro<float> cm = {};
-    cm.rho = (float)P.rho[n];
-    cm.U = P.U[n];
-    cm.V = P.V[n];
-    cm.tau = P.tau[n];
+    float Fx, Fy;
     if (FORCE) {
-        pp32_cell_force(P, C, n, cm.Fx, cm.Fy);
-        P.Fx[n] = cm.Fx;
-        P.Fy[n] = cm.Fy;
+        pp32_cell_force(P, k, n, Fx, Fy);
+        P.Fx[n] = Fx;
+        P.Fy[n] = Fy;
     } else {
-        cm.Fx = P.Fx[n];
-        cm.Fy = P.Fy[n];
+        Fx = P.Fx[n];
+        Fy = P.Fy[n];
     }
-    float ft[Q9], fb[Q9], ht[Q9] = {}, hb[Q9];
+    float ft[Q9], fb[Q9];
 #pragma unroll
     for (int q = 0; q < Q9; ++q) ft[q] = P.f[q * N + n];
-    cell_barplus<float, MPD_MODEL_PP_D2Q9>(C, cm, ht, ft, hb, fb);
+    pp_barplus(k, (float)P.rho[n], P.U[n], P.V[n], P.tau[n], Fx, Fy, ft, fb);
 #pragma unroll
     for (int q = 0; q < Q9; ++q) P.fbp[q * N + n] = fb[q];
 }
 
 // CD face value of population q (CD_Interior_phi_Bh, DUGKS2DSolver.cpp:604-687)
 // on face t of direction dir at (i, j): owner west / south, neighbour east / north
-__device__ __forceinline__ float pp32_face_value(const PP32Planes &P, const Consts &C, int dir, int q, int i, int j)
+__device__ __forceinline__ float pp32_face_value(const PP32Planes &P, const KPP &k, int dir, int q, int i, int j)
 {
     const int nx = P.nx, ny = P.ny;
     const float *b = P.fbp + (size_t)q * P.N;
     auto B = [&](int a, int c) { return b[(size_t)wrapp(c, ny) * nx + wrapp(a, nx)]; };
     if (dir == 0) {
         const int oi = i - 1;
-        return cd_face_value<float>(C, 0, q, B(oi, j), B(i, j), B(i, j + 1), B(i, j - 1), B(oi, j + 1), B(oi, j - 1));
+        return pp_recon<0>(k, q, B(oi, j), B(i, j), B(i, j + 1), B(i, j - 1), B(oi, j + 1), B(oi, j - 1));
     }
     const int oj = j - 1;
-    return cd_face_value<float>(C, 1, q, B(i, oj), B(i, j), B(i + 1, j), B(i - 1, j), B(i + 1, oj), B(i - 1, oj));
+    return pp_recon<1>(k, q, B(i, oj), B(i, j), B(i + 1, j), B(i - 1, j), B(i + 1, oj), B(i - 1, oj));
+}
+
+// face macros (Update_MacroVar_h, D2Q9PP.cpp:110-136) from the nine face
+// values and the face's force of the previous step: rho, U, V
+__device__ __forceinline__ void pp_face_macros(const KPP &k, const float *fb, float ffx, float ffy, float &rho,
+                                               float &U, float &V)
+{
+    float s = 0.f, sx = 0.f, sy = 0.f;
+#pragma unroll
+    for (int q = 0; q < Q9; ++q) s += fb[q];
+#pragma unroll
+    for (int q = 0; q < Q9; ++q) sx += (float)ex9(q) * fb[q];
+#pragma unroll
+    for (int q = 0; q < Q9; ++q) sy += (float)ey9(q) * fb[q];
+    rho = s;
+    const float r = 1.f / s;
+    U = (k.ms * sx + k.half_hdt * ffx) * r;
+    V = (k.ms * sy + k.half_hdt * ffy) * r;
+}
+
+// half-step BGK of population q at a face (Update_phi_Eqh / Update_DVDF_Source_h /
+// Update_phi_h, D2Q9PP.cpp:63-74,146-154, DUGKS2DSolver.cpp:402-423)
+__device__ __forceinline__ float pp_face_bgk(const KPP &k, int q, float fb, float rho, float U, float V, float uul,
+                                             float Fx, float Fy, float irr)
+{
+    float fso;
+    const float feq = pp_feq(k, q, rho, U, V, uul, Fx, Fy, irr, fso);
+    return k.ah * fb + k.bh * feq + k.ch * fso;
+}
+
+// cell velocity (Update_MacroVar, D2Q9PP.cpp:89-109) from the new populations,
+// the density and the cell force: u = sum xi f~ / rho + hdt F / rho
+__device__ __forceinline__ void pp_cell_macros(const KPP &k, const float *ft, float rho, float Fx, float Fy, float &U,
+                                               float &V)
+{
+    float sx = 0.f, sy = 0.f;
+#pragma unroll
+    for (int q = 0; q < Q9; ++q) sx += (float)ex9(q) * ft[q];
+#pragma unroll
+    for (int q = 0; q < Q9; ++q) sy += (float)ey9(q) * ft[q];
+    const float r = 1.f / rho;
+    U = (k.ms * sx + k.hdt * Fx) * r;
+    V = (k.ms * sy + k.hdt * Fy) * r;
 }
 
 // faces, pass 1: face macros (Update_MacroVar_h, D2Q9PP.cpp:110-136) with the
 // face force of the previous step, and Psi of the face (Update_PseudoPsi,
 // PseudoP.cpp:23-30) in FP64
-__global__ void __launch_bounds__(256) k_pp32_face_psi(PP32Planes P, Consts C)
+__global__ void __launch_bounds__(256) k_pp32_face_psi(PP32Planes P, Consts C, KPP k)
 {
     const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     const size_t N = P.N;
@@ -114,35 +266,24 @@ __global__ void __launch_bounds__(256) k_pp32_face_psi(PP32Planes P, Consts C)
     float *fv = P.ffl + (dir ? (size_t)Q9 * N : 0);
 #pragma unroll
     for (int q = 0; q < Q9; ++q) {
-        fb[q] = pp32_face_value(P, C, dir, q, i, j);
+        fb[q] = pp32_face_value(P, k, dir, q, i, j);
         fv[(size_t)q * N + fi] = fb[q];  // the flux pass reads it back and overwrites it with the flux
     }
-    float s = 0.f, sx = 0.f, sy = 0.f;
-#pragma unroll
-    for (int q = 0; q < Q9; ++q) s += fb[q];
-#pragma unroll
-    for (int q = 0; q < Q9; ++q) sx += fb[q] * (float)C.xi[0][q];
-#pragma unroll
-    for (int q = 0; q < Q9; ++q) sy += fb[q] * (float)C.xi[1][q];
-    const float rho = s;
-    float U = sx / rho, V = sy / rho;
-    U += 0.5f * (float)C.hdt * P.ffx[t] / rho;
-    V += 0.5f * (float)C.hdt * P.ffy[t] / rho;
-    const double p = C.pp_k * carnahan_starling((double)rho, C.t_cs);
-    double scg, psi;
-    calc_psi(p, (double)rho, C.rt, scg, psi);
-    P.fpsi[t] = (float)psi;
+    float rho, U, V, scg, psi;
+    pp_face_macros(k, fb, P.ffx[t], P.ffy[t], rho, U, V);
+    pp_psi(C, (double)rho, scg, psi);
+    P.fpsi[t] = psi;
     P.frho[t] = rho;
     P.fU[t] = U;
     P.fV[t] = V;
-    P.fscg[t] = (float)scg;
+    P.fscg[t] = scg;
 }
 
 // faces, pass 2: face force from the face-lattice gradient of Psi
 // (update_MQ_x/y(Face_2D*), GradSchemeBasic.h:181-208; Update_PseudoForce(face),
 // PseudoP.cpp:39-46), half-step BGK (DUGKS2DSolver.cpp:402-423) and micro-flux
 // (:424-441) of the face values that pass 1 left in the flux planes
-__global__ void __launch_bounds__(256) k_pp32_face_flux(PP32Planes P, Consts C)
+__global__ void __launch_bounds__(256) k_pp32_face_flux(PP32Planes P, KPP k)
 {
     const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     const size_t N = P.N;
@@ -156,36 +297,34 @@ __global__ void __launch_bounds__(256) k_pp32_face_flux(PP32Planes P, Consts C)
     const float c[4] = {S(i + 1, j), S(i, j + 1), S(i - 1, j), S(i, j - 1)};
     const float d[4] = {S(i + 1, j + 1), S(i - 1, j + 1), S(i - 1, j - 1), S(i + 1, j - 1)};
     float gx, gy;
-    grad9<float>(C, c, d, gx, gy);
+    pp_grad9(k, c, d, gx, gy);
     const float sp = -P.fscg[t] * P.fpsi[t];
     gx *= sp;
     gy *= sp;
     P.ffx[t] = gx;
     P.ffy[t] = gy;
-    FaceMacro<float> fm = {};
-    fm.rho = P.frho[t];
-    fm.U = P.fU[t];
-    fm.V = P.fV[t];
-    fm.Fx = gx;
-    fm.Fy = gy;
-    const float ah = (float)C.ah_pp, bh = (float)C.bh_pp, ch = (float)C.ch_pp;
+    const float rho = P.frho[t], U = P.fU[t], V = P.fV[t];
+    const float uul = (U * U + V * V) * k.lambda0, irr = k.inv_rt / rho;
     float *fl = P.ffl + (dir ? (size_t)Q9 * N : 0);
     float mass = 0.f;
 #pragma unroll
     for (int q = 0; q < Q9; ++q) {
-        const float fb = fl[(size_t)q * N + fi];
-        float hh, ff;
-        face_bgk<float, MPD_MODEL_PP_D2Q9>(C, fm, q, 0.f, fb, ah, bh, ch, hh, ff);
-        const float xin = (float)(C.dx * C.xi[dir][q]);
-        fl[(size_t)q * N + fi] = ff * xin;
-        mass += ff * xin;
+        const int en = dir == 0 ? ex9(q) : ey9(q);
+        if (en == 0) {  // no flux
+            fl[(size_t)q * N + fi] = 0.f;
+            continue;
+        }
+        const float ff = pp_face_bgk(k, q, fl[(size_t)q * N + fi], rho, U, V, uul, gx, gy, irr);
+        const float flx = en > 0 ? ff * k.flux : -(ff * k.flux);
+        fl[(size_t)q * N + fi] = flx;
+        mass += flx;
     }
     P.fmass[t] = mass;
 }
 
 // Update_phi_T (DUGKS2DSolver.cpp:748-792), Update_MacroVar (D2Q9PP.cpp:89-109)
 // and Psi of the cell (FP64 EOS)
-__global__ void __launch_bounds__(256) k_pp32_update(PP32Planes P, Consts C)
+__global__ void __launch_bounds__(256) k_pp32_update(PP32Planes P, Consts C, KPP k)
 {
     const size_t n = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     const size_t N = P.N;
@@ -194,7 +333,7 @@ __global__ void __launch_bounds__(256) k_pp32_update(PP32Planes P, Consts C)
     const int i = (int)(n % nx), j = (int)(n / nx);
     const size_t kE = (size_t)j * nx + wrapp(i + 1, nx), kW = n;
     const size_t kN = (size_t)wrapp(j + 1, ny) * nx + i, kS = n;
-    const float aTP = 4.f / 3.f, bTP = 1.f / 3.f, dtv = (float)C.dtv;
+    const float aTP = 4.f / 3.f, bTP = 1.f / 3.f, dtv = k.dtv;
     const float *fx = P.ffl, *fy = P.ffl + (size_t)Q9 * N;
     float ft[Q9];
     double rest = 0.0;
@@ -215,33 +354,24 @@ __global__ void __launch_bounds__(256) k_pp32_update(PP32Planes P, Consts C)
     ft[0] = (float)(rho_d - rest);
 #pragma unroll
     for (int q = 0; q < Q9; ++q) P.f[q * N + n] = ft[q];
-    float sx = 0.f, sy = 0.f;
-#pragma unroll
-    for (int q = 0; q < Q9; ++q) sx += ft[q] * (float)C.xi[0][q];
-#pragma unroll
-    for (int q = 0; q < Q9; ++q) sy += ft[q] * (float)C.xi[1][q];
-    const float rho = (float)rho_d;
-    float U = sx / rho, V = sy / rho;
-    U += (float)C.hdt * P.Fx[n] / rho;
-    V += (float)C.hdt * P.Fy[n] / rho;
-    const double p = C.pp_k * carnahan_starling(rho_d, C.t_cs);
-    double scg, psi;
-    calc_psi(p, rho_d, C.rt, scg, psi);
+    float U, V, scg, psi;
+    pp_cell_macros(k, ft, (float)rho_d, P.Fx[n], P.Fy[n], U, V);
+    pp_psi(C, rho_d, scg, psi);
     P.rho[n] = rho_d;
     P.U[n] = U;
     P.V[n] = V;
-    P.psi[n] = (float)psi;
-    P.scg[n] = (float)scg;
+    P.psi[n] = psi;
+    P.scg[n] = scg;
 }
 
 // the cell force of the current state (the last phase of the reference step),
 // for downloads: the steps themselves compute it at the start of the next step
-__global__ void __launch_bounds__(256) k_pp32_force(PP32Planes P, Consts C)
+__global__ void __launch_bounds__(256) k_pp32_force(PP32Planes P, KPP k)
 {
     const size_t n = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (n >= (size_t)P.N) return;
     float Fx, Fy;
-    pp32_cell_force(P, C, n, Fx, Fy);
+    pp32_cell_force(P, k, n, Fx, Fy);
     P.Fx[n] = Fx;
     P.Fy[n] = Fy;
 }
@@ -335,21 +465,19 @@ __device__ __forceinline__ int fused_tile_of_block(int b, int ntiles)
 }
 
 // CD face value of population q from the bar-plus region (index of cell (cx, cy))
-__device__ __forceinline__ float fused_face_value(const Consts &C, const float *bp, int dir, int q, int fx, int fy)
+__device__ __forceinline__ float fused_face_value(const KPP &k, const float *bp, int dir, int q, int fx, int fy)
 {
     auto B = [&](int cx, int cy) { return bp[(cy + 2) * RX + (cx + 2)]; };
     if (dir == 0) {
         const int ox = fx - 1;
-        return cd_face_value<float>(C, 0, q, B(ox, fy), B(fx, fy), B(fx, fy + 1), B(fx, fy - 1), B(ox, fy + 1),
-                                    B(ox, fy - 1));
+        return pp_recon<0>(k, q, B(ox, fy), B(fx, fy), B(fx, fy + 1), B(fx, fy - 1), B(ox, fy + 1), B(ox, fy - 1));
     }
     const int oy = fy - 1;
-    return cd_face_value<float>(C, 1, q, B(fx, oy), B(fx, fy), B(fx + 1, fy), B(fx - 1, fy), B(fx + 1, oy),
-                                B(fx - 1, oy));
+    return pp_recon<1>(k, q, B(fx, oy), B(fx, fy), B(fx + 1, fy), B(fx - 1, fy), B(fx + 1, oy), B(fx - 1, oy));
 }
 
 template <bool FORCE>
-__global__ void __launch_bounds__(FNT) k_pp32_tile(PP32Planes in, PP32Set out, Consts C, int ntx, int ntiles)
+__global__ void __launch_bounds__(FNT) k_pp32_tile(PP32Planes in, PP32Set out, Consts C, KPP k, int ntx, int ntiles)
 {
     __shared__ float s_bp[Q9 * RN];
     __shared__ float s_F[2][FTX * FTY];
@@ -378,48 +506,44 @@ __global__ void __launch_bounds__(FNT) k_pp32_tile(PP32Planes in, PP32Set out, C
     for (int r = tid; r < RN; r += FNT) {
         const int cx = r % RX - 2, cy = r / RX - 2;
         const size_t n = (size_t)wrapn(j0 + cy, ny) * nx + wrapn(i0 + cx, nx);
-        CellMacro<float> cm = {};
-        cm.rho = (float)in.rho[n];
-        cm.U = in.U[n];
-        cm.V = in.V[n];
-        cm.tau = in.tau[n];
+        float Fx, Fy;
         if (FORCE) {
             auto S = [&](int a, int b) { return s_psi[(b + 3) * QX + (a + 3)]; };
             const float c[4] = {S(cx + 1, cy), S(cx, cy + 1), S(cx - 1, cy), S(cx, cy - 1)};
             const float d[4] = {S(cx + 1, cy + 1), S(cx - 1, cy + 1), S(cx - 1, cy - 1), S(cx + 1, cy - 1)};
             float gx, gy;
-            grad9<float>(C, c, d, gx, gy);
+            pp_grad9(k, c, d, gx, gy);
             const int qi = (cy + 3) * QX + (cx + 3);
             const float sp = -s_scg[qi] * s_psi[qi];
-            cm.Fx = gx * sp;
-            cm.Fy = gy * sp;
+            Fx = gx * sp;
+            Fy = gy * sp;
         } else {
-            cm.Fx = in.Fx[n];
-            cm.Fy = in.Fy[n];
+            Fx = in.Fx[n];
+            Fy = in.Fy[n];
         }
         if (cx >= 0 && cx < FTX && cy >= 0 && cy < FTY) {
-            s_F[0][cy * FTX + cx] = cm.Fx;
-            s_F[1][cy * FTX + cx] = cm.Fy;
+            s_F[0][cy * FTX + cx] = Fx;
+            s_F[1][cy * FTX + cx] = Fy;
         }
-        float ft[Q9], fb[Q9], ht[Q9] = {}, hb[Q9];
+        float ft[Q9], fb[Q9];
 #pragma unroll
         for (int q = 0; q < Q9; ++q) ft[q] = in.f[q * N + n];
-        cell_barplus<float, MPD_MODEL_PP_D2Q9>(C, cm, ht, ft, hb, fb);
+        pp_barplus(k, (float)in.rho[n], in.U[n], in.V[n], in.tau[n], Fx, Fy, ft, fb);
 #pragma unroll
         for (int q = 0; q < Q9; ++q) s_bp[q * RN + r] = fb[q];
     }
     __syncthreads();
 
     // S2: face macros with the face's previous force, and face Psi (FP64 EOS)
-    for (int k = tid; k < XPN + YPN; k += FNT) {
-        const int dir = k < XPN ? 0 : 1;
+    for (int kk = tid; kk < XPN + YPN; kk += FNT) {
+        const int dir = kk < XPN ? 0 : 1;
         int fx, fy;
-        if (dir == 0) { fx = k % XPX - 1; fy = k / XPX - 1; }
-        else { fx = (k - XPN) % YPX - 1; fy = (k - XPN) / YPX - 1; }
+        if (dir == 0) { fx = kk % XPX - 1; fy = kk / XPX - 1; }
+        else { fx = (kk - XPN) % YPX - 1; fy = (kk - XPN) / YPX - 1; }
         const size_t gf = (dir ? N : 0) + (size_t)wrapn(j0 + fy, ny) * nx + wrapn(i0 + fx, nx);
         float fb[Q9];
 #pragma unroll
-        for (int q = 0; q < Q9; ++q) fb[q] = fused_face_value(C, s_bp + q * RN, dir, q, fx, fy);
+        for (int q = 0; q < Q9; ++q) fb[q] = fused_face_value(k, s_bp + q * RN, dir, q, fx, fy);
         // a face of the tile keeps its values for the flux stage
         int kf = -1;
         if (dir == 0 && fx >= 0 && fx <= FTX && fy >= 0 && fy < FTY) kf = fy * XFX + fx;
@@ -428,41 +552,29 @@ __global__ void __launch_bounds__(FNT) k_pp32_tile(PP32Planes in, PP32Set out, C
 #pragma unroll
             for (int z = 0; z < 6; ++z) s_fl[z * FN + kf] = fb[dir ? MY[z] : MX[z]];
         }
-        float s = 0.f, sx = 0.f, sy = 0.f;
-#pragma unroll
-        for (int q = 0; q < Q9; ++q) s += fb[q];
-#pragma unroll
-        for (int q = 0; q < Q9; ++q) sx += fb[q] * (float)C.xi[0][q];
-#pragma unroll
-        for (int q = 0; q < Q9; ++q) sy += fb[q] * (float)C.xi[1][q];
-        const float rho = s;
-        float U = sx / rho, V = sy / rho;
-        U += 0.5f * (float)C.hdt * in.ffx[gf] / rho;
-        V += 0.5f * (float)C.hdt * in.ffy[gf] / rho;
-        const double p = C.pp_k * carnahan_starling((double)rho, C.t_cs);
-        double scg, psi;
-        calc_psi(p, (double)rho, C.rt, scg, psi);
-        s_fm[0][k] = (float)psi;
-        s_fm[1][k] = (float)scg;
-        s_fm[2][k] = rho;
-        s_fm[3][k] = U;
-        s_fm[4][k] = V;
+        float rho, U, V, scg, psi;
+        pp_face_macros(k, fb, in.ffx[gf], in.ffy[gf], rho, U, V);
+        pp_psi(C, (double)rho, scg, psi);
+        s_fm[0][kk] = psi;
+        s_fm[1][kk] = scg;
+        s_fm[2][kk] = rho;
+        s_fm[3][kk] = U;
+        s_fm[4][kk] = V;
     }
     __syncthreads();
 
     // S3: face force, half-step BGK and micro-flux of the tile's faces
-    const float ah = (float)C.ah_pp, bh = (float)C.bh_pp, ch = (float)C.ch_pp;
-    for (int k = tid; k < FN; k += FNT) {
-        const int dir = k < XFN ? 0 : 1;
+    for (int kk = tid; kk < FN; kk += FNT) {
+        const int dir = kk < XFN ? 0 : 1;
         int fx, fy, pk;
-        if (dir == 0) { fx = k % XFX; fy = k / XFX; pk = (fy + 1) * XPX + (fx + 1); }
-        else { fx = (k - XFN) % FTX; fy = (k - XFN) / FTX; pk = XPN + (fy + 1) * YPX + (fx + 1); }
+        if (dir == 0) { fx = kk % XFX; fy = kk / XFX; pk = (fy + 1) * XPX + (fx + 1); }
+        else { fx = (kk - XFN) % FTX; fy = (kk - XFN) / FTX; pk = XPN + (fy + 1) * YPX + (fx + 1); }
         const int pw = dir ? YPX : XPX;
         auto S = [&](int a, int b) { return s_fm[0][pk + b * pw + a]; };
         const float c[4] = {S(1, 0), S(0, 1), S(-1, 0), S(0, -1)};
         const float d[4] = {S(1, 1), S(-1, 1), S(-1, -1), S(1, -1)};
         float gx, gy;
-        grad9<float>(C, c, d, gx, gy);
+        pp_grad9(k, c, d, gx, gy);
         const float sp = -s_fm[1][pk] * s_fm[0][pk];
         gx *= sp;
         gy *= sp;
@@ -472,26 +584,20 @@ __global__ void __launch_bounds__(FNT) k_pp32_tile(PP32Planes in, PP32Set out, C
             out.ffx[gf] = gx;
             out.ffy[gf] = gy;
         }
-        FaceMacro<float> fm = {};
-        fm.rho = s_fm[2][pk];
-        fm.U = s_fm[3][pk];
-        fm.V = s_fm[4][pk];
-        fm.Fx = gx;
-        fm.Fy = gy;
+        const float rho = s_fm[2][pk], U = s_fm[3][pk], V = s_fm[4][pk];
+        const float uul = (U * U + V * V) * k.lambda0, irr = k.inv_rt / rho;
         // the populations with a zero normal velocity carry no flux
         if (dir == 0) {
 #pragma unroll
             for (int z = 0; z < 6; ++z) {
-                float hh, ff;
-                face_bgk<float, MPD_MODEL_PP_D2Q9>(C, fm, MX[z], 0.f, s_fl[z * FN + k], ah, bh, ch, hh, ff);
-                s_fl[z * FN + k] = ff * (float)(C.dx * C.xi[0][MX[z]]);
+                const float ff = pp_face_bgk(k, MX[z], s_fl[z * FN + kk], rho, U, V, uul, gx, gy, irr);
+                s_fl[z * FN + kk] = ex9(MX[z]) > 0 ? ff * k.flux : -(ff * k.flux);
             }
         } else {
 #pragma unroll
             for (int z = 0; z < 6; ++z) {
-                float hh, ff;
-                face_bgk<float, MPD_MODEL_PP_D2Q9>(C, fm, MY[z], 0.f, s_fl[z * FN + k], ah, bh, ch, hh, ff);
-                s_fl[z * FN + k] = ff * (float)(C.dx * C.xi[1][MY[z]]);
+                const float ff = pp_face_bgk(k, MY[z], s_fl[z * FN + kk], rho, U, V, uul, gx, gy, irr);
+                s_fl[z * FN + kk] = ey9(MY[z]) > 0 ? ff * k.flux : -(ff * k.flux);
             }
         }
     }
@@ -504,7 +610,7 @@ __global__ void __launch_bounds__(FNT) k_pp32_tile(PP32Planes in, PP32Set out, C
     const int kW = cy * XFX + cx, kE = kW + 1;
     const int kS = XFN + cy * FTX + cx, kN = kS + FTX;
     const int rc = (cy + 2) * RX + (cx + 2);
-    const float aTP = 4.f / 3.f, bTP = 1.f / 3.f, dtv = (float)C.dtv;
+    const float aTP = 4.f / 3.f, bTP = 1.f / 3.f, dtv = k.dtv;
     float ft[Q9];
     double rest = 0.0;
 #pragma unroll
@@ -519,10 +625,10 @@ __global__ void __launch_bounds__(FNT) k_pp32_tile(PP32Planes in, PP32Set out, C
         if (q > 0) rest += (double)ft[q];
     }
     // each face's mass flux: the sum of its population fluxes in increasing q
-    auto mass = [&](int k) {
+    auto mass = [&](int kf) {
         float m = 0.f;
 #pragma unroll
-        for (int z = 0; z < 6; ++z) m += s_fl[z * FN + k];
+        for (int z = 0; z < 6; ++z) m += s_fl[z * FN + kf];
         return m;
     };
     const double dm = ((double)mass(kW) - (double)mass(kE)) + ((double)mass(kS) - (double)mass(kN));
@@ -530,23 +636,14 @@ __global__ void __launch_bounds__(FNT) k_pp32_tile(PP32Planes in, PP32Set out, C
     ft[0] = (float)(rho_d - rest);
 #pragma unroll
     for (int q = 0; q < Q9; ++q) out.f[q * N + n] = ft[q];
-    float sx = 0.f, sy = 0.f;
-#pragma unroll
-    for (int q = 0; q < Q9; ++q) sx += ft[q] * (float)C.xi[0][q];
-#pragma unroll
-    for (int q = 0; q < Q9; ++q) sy += ft[q] * (float)C.xi[1][q];
-    const float rho = (float)rho_d;
-    float U = sx / rho, V = sy / rho;
-    U += (float)C.hdt * s_F[0][cy * FTX + cx] / rho;
-    V += (float)C.hdt * s_F[1][cy * FTX + cx] / rho;
-    const double p = C.pp_k * carnahan_starling(rho_d, C.t_cs);
-    double scg, psi;
-    calc_psi(p, rho_d, C.rt, scg, psi);
+    float U, V, scg, psi;
+    pp_cell_macros(k, ft, (float)rho_d, s_F[0][cy * FTX + cx], s_F[1][cy * FTX + cx], U, V);
+    pp_psi(C, rho_d, scg, psi);
     out.rho[n] = rho_d;
     out.U[n] = U;
     out.V[n] = V;
-    out.psi[n] = (float)psi;
-    out.scg[n] = (float)scg;
+    out.psi[n] = psi;
+    out.scg[n] = scg;
 }
 
 }  // namespace
@@ -613,7 +710,7 @@ void pp32_from64(const PP32State &St, const DevState &S, hipStream_t st)
 
 void pp32_to64(const PP32State &St, const DevState &S, const Consts &C, bool force_stale, hipStream_t st)
 {
-    if (force_stale) hipLaunchKernelGGL(k_pp32_force, dim3(blocks(St.P.N)), dim3(256), 0, st, St.P, C);
+    if (force_stale) hipLaunchKernelGGL(k_pp32_force, dim3(blocks(St.P.N)), dim3(256), 0, st, St.P, make_kpp(C));
     hipLaunchKernelGGL(k_pp32_to64, dim3(blocks(St.P.N)), dim3(256), 0, st, St.P, S, C);
 }
 
@@ -621,18 +718,19 @@ void pp32_step(PP32State &St, const Consts &C, bool force_stale, hipStream_t st)
 {
     PP32Planes &P = St.P;
     const size_t N = P.N;
+    const KPP k = make_kpp(C);
     if (St.phases) {
-        if (force_stale) hipLaunchKernelGGL(k_pp32_phase1<true>, dim3(blocks(N)), dim3(256), 0, st, P, C);
-        else hipLaunchKernelGGL(k_pp32_phase1<false>, dim3(blocks(N)), dim3(256), 0, st, P, C);
-        hipLaunchKernelGGL(k_pp32_face_psi, dim3(blocks(2 * N)), dim3(256), 0, st, P, C);
-        hipLaunchKernelGGL(k_pp32_face_flux, dim3(blocks(2 * N)), dim3(256), 0, st, P, C);
-        hipLaunchKernelGGL(k_pp32_update, dim3(blocks(N)), dim3(256), 0, st, P, C);
+        if (force_stale) hipLaunchKernelGGL(k_pp32_phase1<true>, dim3(blocks(N)), dim3(256), 0, st, P, k);
+        else hipLaunchKernelGGL(k_pp32_phase1<false>, dim3(blocks(N)), dim3(256), 0, st, P, k);
+        hipLaunchKernelGGL(k_pp32_face_psi, dim3(blocks(2 * N)), dim3(256), 0, st, P, C, k);
+        hipLaunchKernelGGL(k_pp32_face_flux, dim3(blocks(2 * N)), dim3(256), 0, st, P, k);
+        hipLaunchKernelGGL(k_pp32_update, dim3(blocks(N)), dim3(256), 0, st, P, C, k);
         return;
     }
     const int ntx = (P.nx + FTX - 1) / FTX, nty = (P.ny + FTY - 1) / FTY, ntiles = ntx * nty;
     const unsigned nblocks = (unsigned)(((ntiles + 7) / 8) * 8);
-    if (force_stale) hipLaunchKernelGGL(k_pp32_tile<true>, dim3(nblocks), dim3(FNT), 0, st, P, St.other, C, ntx, ntiles);
-    else hipLaunchKernelGGL(k_pp32_tile<false>, dim3(nblocks), dim3(FNT), 0, st, P, St.other, C, ntx, ntiles);
+    if (force_stale) hipLaunchKernelGGL(k_pp32_tile<true>, dim3(nblocks), dim3(FNT), 0, st, P, St.other, C, k, ntx, ntiles);
+    else hipLaunchKernelGGL(k_pp32_tile<false>, dim3(nblocks), dim3(FNT), 0, st, P, St.other, C, k, ntx, ntiles);
     // the written set becomes the current one
     PP32Set cur{P.rho, P.f, P.U, P.V, P.Fx, P.Fy, P.psi, P.scg, P.ffx, P.ffy};
     const PP32Set &o = St.other;
