@@ -1,4 +1,5 @@
-"""Benchmark and test cases named by BASELINE configs (synthetic, seeded)."""
+"""Benchmark and test cases (synthetic, seeded): the Rayleigh-Taylor instability,
+the static droplet/bubble ensemble and two-droplet coalescence."""
 import numpy as np
 
 from . import abi, solver
