@@ -117,12 +117,17 @@ __global__ void __launch_bounds__(NTHREADS) k_fused_step(Consts C, Planes<T> in,
                                                          int ntiles)
 {
     constexpr bool HCZ = MODEL == MPD_MODEL_AC_HCZ;
-    __shared__ T s_bp[2 * Q9][R1N];     // bar-plus: h in 0..8, f in 9..17; later the y-face fluxes
-    __shared__ T s_mac[6][R1N];         // px, py, rx, ry, Fx, Fy
-    __shared__ T s_u[2 * Q9 * NXF];     // phi/phi_lag (stage A-B), then the x-face fluxes
+    // bar-plus (h in 0..8, f in 9..17) and the six face sources px, py, rx,
+    // ry, Fx, Fy on the one-cell halo; after the faces, the face fluxes of the
+    // six populations with a non-zero normal velocity (x: 12 x NXF, y: 12 x NYF)
+    __shared__ T s_big[(2 * Q9 + 6) * R1N];
+    __shared__ T s_ph[2 * R2N];         // phi, phi_lag on the two-cell halo
     __shared__ T s_nee[WALLS == 2 ? 2 * TX * 2 * Q9 : 1];  // NEE wall faces: [bottom, top][column][h, f]
-    T *s_phi = s_u, *s_phl = s_u + R2N;
-    static_assert(2 * R2N <= 2 * Q9 * NXF, "union too small");
+    T(*s_bp)[R1N] = reinterpret_cast<T(*)[R1N]>(s_big);
+    T(*s_mac)[R1N] = reinterpret_cast<T(*)[R1N]>(s_big + 2 * Q9 * R1N);
+    T *s_phi = s_ph, *s_phl = s_ph + R2N;
+    T *s_fx = s_big, *s_fy = s_big + 12 * NXF;
+    static_assert(12 * (NXF + NYF) <= (2 * Q9 + 6) * R1N, "face fluxes must fit over the bar-plus planes");
 
     const int tile = tile_of_block(blockIdx.x, ntiles);
     if (tile < 0) return;
@@ -236,7 +241,13 @@ __global__ void __launch_bounds__(NTHREADS) k_fused_step(Consts C, Planes<T> in,
     }
 
     // ---- stage C: face fluxes ----
+    // Each thread keeps the fluxes of its x- and y-face (the six populations
+    // with a non-zero normal velocity; the other three carry a zero flux) in
+    // registers until every face has read the bar-plus planes, then writes
+    // them over those planes.
     const T hdt = (T)C.hdt;
+    constexpr int MX[6] = {1, 2, 4, 5, 6, 8}, MY[6] = {2, 3, 4, 6, 7, 8};
+    T xh[6], xf[6];
     // x-faces: thread -> face (fi, fj) between R1 cells (fi, fj+1) and (fi+1, fj+1)
     if (tid < NXF) {
         const int fi = tid % (TX + 1), fj = tid / (TX + 1);
@@ -259,19 +270,18 @@ __global__ void __launch_bounds__(NTHREADS) k_fused_step(Consts C, Planes<T> in,
         T ah = (T)2.0 * fm.tau / ((T)2.0 * fm.tau + hdt);
         T bh = (T)1.0 - ah;
         T ch = fm.tau * bh;
-        T hh[Q9], ff[Q9];
 #pragma unroll
-        for (int q = 0; q < Q9; ++q) face_bgk<T, MODEL>(C, fm, q, hb[q], fb[q], ah, bh, ch, hh[q], ff[q]);
-        // s_u held phi/phi_lag, last read before the barrier that closed stage B
-#pragma unroll
-        for (int q = 0; q < Q9; ++q) {
+        for (int z = 0; z < 6; ++z) {
+            const int q = MX[z];
+            T hh, ff;
+            face_bgk<T, MODEL>(C, fm, q, hb[q], fb[q], ah, bh, ch, hh, ff);
             const T xin = (T)C.dx * (T)C.xi[0][q];
-            s_u[q * NXF + tid] = hh[q] * xin;
-            s_u[(Q9 + q) * NXF + tid] = ff[q] * xin;
+            xh[z] = hh * xin;
+            xf[z] = ff * xin;
         }
     }
     // y-faces: thread -> face (fi, fj) between R1 cells (fi+1, fj) and (fi+1, fj+1)
-    T yh[Q9], yf[Q9];
+    T yh[6], yf[6];
     {
         const int fi = tid % TX, fj = tid / TX;
         const int ko = fj * R1X + fi + 1, kn = ko + R1X;
@@ -293,34 +303,42 @@ __global__ void __launch_bounds__(NTHREADS) k_fused_step(Consts C, Planes<T> in,
         T ah = (T)2.0 * fm.tau / ((T)2.0 * fm.tau + hdt);
         T bh = (T)1.0 - ah;
         T ch = fm.tau * bh;
+        // face values of the populations with e_y != 0 (wall faces: the
+        // incoming ones are replaced below by their partners, also e_y != 0)
+        T vh[Q9], vf[Q9];
 #pragma unroll
-        for (int q = 0; q < Q9; ++q) face_bgk<T, MODEL>(C, fm, q, hb[q], fb[q], ah, bh, ch, yh[q], yf[q]);
+        for (int z = 0; z < 6; ++z) {
+            const int q = MY[z];
+            face_bgk<T, MODEL>(C, fm, q, hb[q], fb[q], ah, bh, ch, vh[q], vf[q]);
+        }
         const int gj = j0 + fj;
         if (WALLS == 2 && (gj == 0 || gj == ny)) {
             // Wall_3_NEE: the wall-row cell's values (stage B), then bounce-back
             const T *v = s_nee + ((gj == 0 ? 0 : 1) * TX + fi) * 2 * Q9;
 #pragma unroll
-            for (int q = 0; q < Q9; ++q) { yh[q] = v[q]; yf[q] = v[Q9 + q]; }
+            for (int z = 0; z < 6; ++z) { vh[MY[z]] = v[MY[z]]; vf[MY[z]] = v[Q9 + MY[z]]; }
         }
         if (WALLS && (gj == 0 || gj == ny)) {
             // Wall_3_BB (Ark/Cpp/Boundary.cpp:118-139): incoming take the partner's value
             T h0[Q9], f0[Q9];
 #pragma unroll
-            for (int q = 0; q < Q9; ++q) { h0[q] = yh[q]; f0[q] = yf[q]; }
+            for (int z = 0; z < 6; ++z) { h0[MY[z]] = vh[MY[z]]; f0[MY[z]] = vf[MY[z]]; }
 #pragma unroll
-            for (int q = 0; q < Q9; ++q) {
+            for (int z = 0; z < 6; ++z) {
+                const int q = MY[z];
                 const bool inc = gj == 0 ? (ey9(q) > 0) : (ey9(q) < 0);
-                if (inc) { yh[q] = h0[bb9(q)]; yf[q] = f0[bb9(q)]; }
+                if (inc) { vh[q] = h0[bb9(q)]; vf[q] = f0[bb9(q)]; }
             }
         }
 #pragma unroll
-        for (int q = 0; q < Q9; ++q) {
+        for (int z = 0; z < 6; ++z) {
+            const int q = MY[z];
             const T xin = (T)C.dx * (T)C.xi[1][q];
-            yh[q] *= xin;
-            yf[q] *= xin;
+            yh[z] = vh[q] * xin;
+            yf[z] = vf[q] * xin;
         }
     }
-    // own bar-plus before the region is reused for the y-face fluxes
+    // own bar-plus before the region is reused for the face fluxes
     const int ci = tid % TX, cj = tid / TX;
     const bool cell = tid < TX * TY;
     const int kc = (cj + 1) * R1X + ci + 1;
@@ -330,14 +348,23 @@ __global__ void __launch_bounds__(NTHREADS) k_fused_step(Consts C, Planes<T> in,
         for (int q = 0; q < Q9; ++q) { hbp[q] = s_bp[q][kc]; fbp[q] = s_bp[Q9 + q][kc]; }
     }
     __syncthreads();
+    if (tid < NXF) {
 #pragma unroll
-    for (int q = 0; q < Q9; ++q) {
-        s_bp[q][tid] = yh[q];
-        s_bp[Q9 + q][tid] = yf[q];
+        for (int z = 0; z < 6; ++z) {
+            s_fx[z * NXF + tid] = xh[z];
+            s_fx[(6 + z) * NXF + tid] = xf[z];
+        }
+    }
+#pragma unroll
+    for (int z = 0; z < 6; ++z) {
+        s_fy[z * NYF + tid] = yh[z];
+        s_fy[(6 + z) * NYF + tid] = yf[z];
     }
     __syncthreads();
 
     // ---- stage D: Update_phi_T (Ark/Cpp/DUGKS2DSolver.cpp:748-792), faces E, N, W, S ----
+    // (a population with a zero normal velocity on a face contributes a zero
+    // flux there; skipping it leaves every sum unchanged)
     const int gi = i0 + ci, gj = j0 + cj;
     if (!cell || gi >= nx || gj >= ny) return;
     const size_t n = (size_t)gj * nx + gi;
@@ -347,19 +374,21 @@ __global__ void __launch_bounds__(NTHREADS) k_fused_step(Consts C, Planes<T> in,
     T phi = 0;
 #pragma unroll
     for (int q = 0; q < Q9; ++q) {
+        const int zx = q == 1 ? 0 : q == 2 ? 1 : q == 4 ? 2 : q == 5 ? 3 : q == 6 ? 4 : q == 8 ? 5 : -1;
+        const int zy = q == 2 ? 0 : q == 3 ? 1 : q == 4 ? 2 : q == 6 ? 3 : q == 7 ? 4 : q == 8 ? 5 : -1;
         T s = 0;
-        s += -1 * s_u[q * NXF + kE];
-        s += -1 * s_bp[q][kN];
-        s += 1 * s_u[q * NXF + kW];
-        s += 1 * s_bp[q][kS];
+        if (zx >= 0) s += -1 * s_fx[zx * NXF + kE];
+        if (zy >= 0) s += -1 * s_fy[zy * NYF + kN];
+        if (zx >= 0) s += 1 * s_fx[zx * NXF + kW];
+        if (zy >= 0) s += 1 * s_fy[zy * NYF + kS];
         T hn = aTP * hbp[q] - bTP * in.h[(size_t)q * N + n] + dtv * s;
         out.h[(size_t)q * N + n] = hn;
         phi += hn;
         T sf = 0;
-        sf += -1 * s_u[(Q9 + q) * NXF + kE];
-        sf += -1 * s_bp[Q9 + q][kN];
-        sf += 1 * s_u[(Q9 + q) * NXF + kW];
-        sf += 1 * s_bp[Q9 + q][kS];
+        if (zx >= 0) sf += -1 * s_fx[(6 + zx) * NXF + kE];
+        if (zy >= 0) sf += -1 * s_fy[(6 + zy) * NYF + kN];
+        if (zx >= 0) sf += 1 * s_fx[(6 + zx) * NXF + kW];
+        if (zy >= 0) sf += 1 * s_fy[(6 + zy) * NYF + kS];
         out.f[(size_t)q * N + n] = aTP * fbp[q] - bTP * in.f[(size_t)q * N + n] + dtv * sf;
     }
     out.phi[n] = phi;
