@@ -117,7 +117,9 @@ def pp_bench(n=512, steps=1000, tr=0.85, seed=3, cmp_steps=200):
     """Pseudopotential liquid-vapour separation (D2Q9, Carnahan-Starling EOS):
     n x n periodic, rho0 = 0.1645 (1 + U(-0.01, 0.01)) per cell built on the
     device.  T/Tc = 0.85, the reference's shipped value: at 0.8 the reference
-    algorithm itself diverges (non-finite by step 1000 on 128^2, FP64).
+    algorithm itself diverges (non-finite by step 1000 on 128^2, FP64); at 0.85
+    this start stays finite to step 2,500 on 512^2 (non-finite by 3,000, FP64 and
+    FP32 alike), so the timed steps stay well inside that.
     FP32-bulk (FP32 populations, FP64 EOS and conserved density) against the
     plain FP64 phase path: MLUPS, mass drift, and the relative L2 distance of
     rho after `cmp_steps` steps from the same start."""
