@@ -48,12 +48,12 @@ def run_gpu(cfg, st, steps, warmup, path):
     wall = time.perf_counter() - t0
     avg_ms, nlaunch = s.kernel_timing()
     wall_free = wall
-    rows = s.tile_rows()
+    tile = s.tile()
     red = s.reductions()
     out = s.download()
     s.close()
     return {"wall_s": wall, "wall_s_untimed": wall_free, "kernel_ms": avg_ms, "launches": nlaunch,
-            "state": out, "sums": red, "tile_rows": rows}
+            "state": out, "sums": red, "tile": tile}
 
 
 def ensemble_bench(ncases, steps, seed=0, solve_steps=20000):
@@ -74,8 +74,8 @@ def ensemble_bench(ncases, steps, seed=0, solve_steps=20000):
         members.append(s)
     setup = time.perf_counter() - t0
     ncells = batch.size()[1]
-    batch.step(20)                     # includes the tile-height tuning (8 + 8 timed steps)
-    rows, ms15, ms16 = batch.tile_rows()
+    batch.step(30)                     # includes the tile-shape tuning (8 steps per shape)
+    tile, tune_ms = batch.tile()
     batch.timing()
     t0 = time.perf_counter()
     batch.step(steps)
@@ -108,7 +108,8 @@ def ensemble_bench(ncases, steps, seed=0, solve_steps=20000):
             "solves_per_s_fp64_plain": ncases / t_solve_plain,
             "speedup_vs_fp64_plain": t_solve_plain / t_solve_batch,
             "fp64_plain_ms_per_step_by_side": {str(k): v * 1e3 for k, v in per_side.items()},
-            "tile_rows": rows, "tuning_ms_per_step": {"16x15": ms15, "16x16": ms16},
+            "tile": abi.TILE_NAMES[tile],
+            "tuning_ms_per_step": {abi.TILE_NAMES[t + 1]: ms for t, ms in enumerate(tune_ms)},
             "setup_s": setup, "timed_steps": steps,
             "nonfinite_in_first_8": int(sum(r["nonfinite"] for r in red))}
 
@@ -306,7 +307,7 @@ def main():
     err_floor = {k: rel(pert.m[abi.M[k]], ref64.m[abi.M[k]]) for k in ("rho", "phi", "p")}
 
     bytes_alg = 2 * S_ALG_HCZ * 4  # FP32-bulk
-    kernel = "k_fused32<AC_HCZ,BB walls,16x%d tiles>" % head["tile_rows"]
+    kernel = "k_fused32<AC_HCZ,BB walls,%s tiles>" % abi.TILE_NAMES[head["tile"]]
     traffic = None
     try:  # HBM bytes per launch from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE runs
         with open(os.path.join(ROOT, "profiles", "traffic.json")) as fh:

@@ -148,13 +148,20 @@ int mpd_destroy(mpd_handle *h);
 /* Select the kernel path (mpd_path); valid before the first mpd_step. */
 int mpd_set_path(mpd_handle *h, int32_t path);
 
-/* Tile height of the FP32-bulk Allen-Cahn step kernel (tiles of 16 x rows
- * cells): 0 (the default) chooses per mesh, 15 or 16 forces one; the state
- * does not depend on it.  Takes effect at the next mpd_step; get returns the
- * height that step uses.  No counterpart in the reference (a launch shape of
- * this library). */
-int mpd_set_tile_rows(mpd_handle *h, int32_t rows);
-int mpd_get_tile_rows(mpd_handle *h, int32_t *rows);
+/* Tile shape of the FP32-bulk Allen-Cahn step kernel (cells per workgroup).
+ * MPD_TILE_AUTO (the default) chooses per mesh; the others force one.  The
+ * state does not depend on it (bit for bit with periodic and BB walls, to
+ * FP32 rounding with NEE walls).  Takes effect at the next mpd_step;
+ * mpd_get_tile returns the shape that step uses.  No counterpart in the
+ * reference (a launch shape of this library). */
+typedef enum mpd_tile {
+    MPD_TILE_AUTO = 0,
+    MPD_TILE_16X15 = 1,      /* 256 threads */
+    MPD_TILE_16X16 = 2,      /* 256 threads */
+    MPD_TILE_32X16 = 3       /* 512 threads */
+} mpd_tile;
+int mpd_set_tile(mpd_handle *h, int32_t tile);
+int mpd_get_tile(mpd_handle *h, int32_t *tile);
 
 /* Upload host state (FP64).  tilde_h may be NULL for the PP models;
  * face_fx/face_fy (PP only, may be NULL otherwise) are the persistent face
@@ -210,16 +217,17 @@ int mpd_batch_size(const mpd_batch *b, int32_t *ncases, int64_t *ncells);
 int mpd_batch_step(mpd_batch *b, int64_t nsteps);
 /* Mean wall time per batch step on the batch stream since the last call. */
 int mpd_batch_timing(mpd_batch *b, double *avg_ms, int64_t *steps);
-/* Tile height of the batched launch (tiles of 16 x rows cells).  0 (the
- * default) tunes it: the first mpd_batch_step of at least 16 steps after the
- * members change times 8 steps with 16x15 tiles and 8 with 16x16 (ordinary
- * steps of the run: with periodic and BB walls both give the same state bit
- * for bit) and keeps the faster.  15 or 16 forces one.  Batches with NEE walls
- * use 16x15 unless forced, so that their results do not depend on a timing.
- * mpd_batch_tile_rows returns the height in use and the last tuning's ms per
- * step of each (0 where not measured).  No counterpart in the reference. */
-int mpd_batch_set_tile_rows(mpd_batch *b, int32_t rows);
-int mpd_batch_tile_rows(const mpd_batch *b, int32_t *rows, double *ms_rows15, double *ms_rows16);
+/* Tile shape of the batched launch (mpd_tile).  MPD_TILE_AUTO (the default)
+ * tunes it: the first mpd_batch_step of at least 24 steps after the members
+ * change runs 8 steps with each of the three shapes (ordinary steps of the
+ * run: with periodic and BB walls all give the same state bit for bit), times
+ * the last 6 of each and keeps the fastest.  Any other value forces a shape.
+ * Batches with NEE walls use 16x15 unless forced, so that their results do
+ * not depend on a timing.  mpd_batch_tile returns the shape in use and the
+ * last tuning's ms per step of each shape, indexed by mpd_tile - 1 (0 where
+ * not measured).  No counterpart in the reference. */
+int mpd_batch_set_tile(mpd_batch *b, int32_t tile);
+int mpd_batch_tile(const mpd_batch *b, int32_t *tile, double ms_per_step[3]);
 
 /* ---- host-side initial states (upload layout; no device work) ---- */
 

@@ -50,7 +50,7 @@ struct mpd_handle {
     hipStream_t stream = nullptr;
     int device = 0;
     int cus = 0;                 // compute units of the device
-    int tile_rows = 0;           // FP32-bulk Allen-Cahn tile height: 0 chosen per mesh, or 15 / 16
+    int tile = MPD_TILE_AUTO;    // FP32-bulk Allen-Cahn tile shape (mpd_tile)
     int path = MPD_PATH_AUTO;
     int effective_path = MPD_PATH_PHASES;
     bool uploaded = false;
@@ -78,10 +78,10 @@ struct mpd_batch {
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     double acc_ms = 0.0;
     int64_t steps = 0;
-    int rows_req = 0;             // tile height: 0 tuned, or 15 / 16
-    int rows = 15;                // height of the current tables
-    bool tune_pending = false;    // members changed since the last tuning
-    double tune_ms[2] = {0.0, 0.0};  // last tuning: ms per step with 15 / 16 rows
+    int tile_req = MPD_TILE_AUTO;   // tile shape: tuned, or forced (mpd_tile)
+    int tile = MPD_TILE_16X15;      // shape of the current tables
+    bool tune_pending = false;      // members changed since the last tuning
+    double tune_ms[3] = {0.0, 0.0, 0.0};  // last tuning: ms per step of each shape
 };
 
 namespace {
@@ -367,18 +367,18 @@ int mpd_set_path(mpd_handle *h, int32_t path)
     return 0;
 }
 
-int mpd_set_tile_rows(mpd_handle *h, int32_t rows)
+int mpd_set_tile(mpd_handle *h, int32_t tile)
 {
     if (!h) return fail(MPD_ERR_INVALID, "null handle");
-    if (rows != 0 && rows != 15 && rows != 16) return fail(MPD_ERR_INVALID, "tile rows %d (0, 15 or 16)", rows);
-    h->tile_rows = rows;
+    if (tile < MPD_TILE_AUTO || tile > MPD_TILE_32X16) return fail(MPD_ERR_INVALID, "unknown tile shape %d", tile);
+    h->tile = tile;
     return 0;
 }
 
-int mpd_get_tile_rows(mpd_handle *h, int32_t *rows)
+int mpd_get_tile(mpd_handle *h, int32_t *tile)
 {
-    if (!h || !rows) return fail(MPD_ERR_INVALID, "null argument");
-    *rows = h->tile_rows ? h->tile_rows : fused32_tile_rows(h->cfg.nx, h->cfg.ny, h->cus);
+    if (!h || !tile) return fail(MPD_ERR_INVALID, "null argument");
+    *tile = h->tile ? h->tile : fused32_tile_auto(h->cfg.nx, h->cfg.ny, h->cus);
     return 0;
 }
 
@@ -546,7 +546,7 @@ int mpd_step(mpd_handle *h, int64_t nsteps)
             if (r) return fail(MPD_ERR_NOMEM, "fused path allocation failed");
         }
         h->F.pp32.phases = pp32_phases;
-        h->F.tile_rows = h->tile_rows ? h->tile_rows : fused32_tile_rows(h->cfg.nx, h->cfg.ny, h->cus);
+        h->F.tile = h->tile ? h->tile : fused32_tile_auto(h->cfg.nx, h->cfg.ny, h->cus);
         int64_t s = 0;
         if (!h->fused_primed && !fused_needs_priming(h->cfg)) {
             // explicit state carried as is (FP32-bulk pseudopotential)
@@ -770,7 +770,7 @@ static bool batch_nee(const mpd_batch *b)
     return !b->members.empty() && b->members[0]->cfg.bc_y == MPD_BC_WALL_NEE;
 }
 
-// device tables of the current members with b->rows tile rows (no graph)
+// device tables of the current members with tiles of shape b->tile (no graph)
 static int batch_tables(mpd_batch *b)
 {
     batch_drop_graph(b);
@@ -783,7 +783,7 @@ static int batch_tables(mpd_batch *b)
         m[k].pl[1] = fused32_planes(h->F, 1);
         m[k].cur = h->F.cur;
     }
-    if (batch32_plan(b->plan, m.data(), (int)m.size(), b->rows, b->stream))
+    if (batch32_plan(b->plan, m.data(), (int)m.size(), b->tile, b->stream))
         return fail(MPD_ERR_DEVICE, "batch tables: %s", hipGetErrorString(hipGetLastError()));
     return 0;
 }
@@ -801,20 +801,19 @@ static int batch_rebuild(mpd_batch *b)
     return 0;
 }
 
-// Tile-height autotuner: TUNE_STEPS steps of the batch with 16x15 tiles, as
-// many with 16x16 tiles; of each, the first TUNE_WARM warm the new tables and
-// code, the rest are timed with events on the batch stream.  The faster
-// height is kept until the members change.  The two heights give the same
-// state bit for bit with periodic and bounce-back walls (each cell and face is
-// computed by the same code), so the tuning steps are ordinary steps of the
-// run.  Even counts leave every member's current plane set in place.
-constexpr int TUNE_STEPS = 8, TUNE_WARM = 2;
+// Tile-shape autotuner: TUNE_STEPS steps of the batch with each tile shape;
+// of each, the first TUNE_WARM warm the new tables and code, the rest are
+// timed with events on the batch stream.  The fastest shape is kept until the
+// members change.  The shapes give the same state bit for bit with periodic
+// and bounce-back walls (each cell and face is computed by the same code), so
+// the tuning steps are ordinary steps of the run.  Even counts leave every
+// member's current plane set in place.
+constexpr int TUNE_STEPS = 8, TUNE_WARM = 2, NSHAPES = 3;
 
 static int batch_tune(mpd_batch *b)
 {
-    const int cand[2] = {15, 16};
-    for (int c = 0; c < 2; ++c) {
-        b->rows = cand[c];
+    for (int c = 0; c < NSHAPES; ++c) {
+        b->tile = MPD_TILE_16X15 + c;
         if (int r = batch_tables(b)) return r;
         HIPCHK(hipEventRecord(b->ev1, b->stream));
         for (int s = 0; s < TUNE_STEPS; ++s) {
@@ -835,29 +834,32 @@ static int batch_tune(mpd_batch *b)
         b->acc_ms += ms_all;
         b->steps += TUNE_STEPS;
     }
-    b->rows = b->tune_ms[1] < b->tune_ms[0] ? 16 : 15;
+    int best = 0;
+    for (int c = 1; c < NSHAPES; ++c)
+        if (b->tune_ms[c] < b->tune_ms[best]) best = c;
+    b->tile = MPD_TILE_16X15 + best;
     b->tune_pending = false;
     b->dirty = true;
     return 0;
 }
 
-int mpd_batch_set_tile_rows(mpd_batch *b, int32_t rows)
+int mpd_batch_set_tile(mpd_batch *b, int32_t tile)
 {
     if (!b) return fail(MPD_ERR_INVALID, "null batch");
-    if (rows != 0 && rows != 15 && rows != 16) return fail(MPD_ERR_INVALID, "tile rows %d (0, 15 or 16)", rows);
+    if (tile < MPD_TILE_AUTO || tile > MPD_TILE_32X16) return fail(MPD_ERR_INVALID, "unknown tile shape %d", tile);
     HIPCHK(hipStreamSynchronize(b->stream));
-    b->rows_req = rows;
-    b->tune_pending = rows == 0;
+    b->tile_req = tile;
+    b->tune_pending = tile == MPD_TILE_AUTO;
     b->dirty = true;
     return 0;
 }
 
-int mpd_batch_tile_rows(const mpd_batch *b, int32_t *rows, double *ms_rows15, double *ms_rows16)
+int mpd_batch_tile(const mpd_batch *b, int32_t *tile, double ms_per_step[3])
 {
     if (!b) return fail(MPD_ERR_INVALID, "null batch");
-    if (rows) *rows = b->rows_req ? b->rows_req : b->rows;
-    if (ms_rows15) *ms_rows15 = b->tune_ms[0];
-    if (ms_rows16) *ms_rows16 = b->tune_ms[1];
+    if (tile) *tile = b->tile_req ? b->tile_req : b->tile;
+    if (ms_per_step)
+        for (int c = 0; c < NSHAPES; ++c) ms_per_step[c] = b->tune_ms[c];
     return 0;
 }
 
@@ -866,15 +868,15 @@ int mpd_batch_step(mpd_batch *b, int64_t nsteps)
     if (!b) return fail(MPD_ERR_INVALID, "null batch");
     if (nsteps < 0) return fail(MPD_ERR_INVALID, "nsteps < 0");
     if (nsteps == 0 || b->members.empty()) return 0;
-    if (b->rows_req) {
-        if (b->rows != b->rows_req) b->dirty = true;
-        b->rows = b->rows_req;
+    if (b->tile_req) {
+        if (b->tile != b->tile_req) b->dirty = true;
+        b->tile = b->tile_req;
     } else if (batch_nee(b)) {
-        if (b->rows != 15) b->dirty = true;
-        b->rows = 15;   // NEE: the heights differ by FP32 rounding; keep results independent of timing
-    } else if (b->tune_pending && nsteps >= 2 * TUNE_STEPS) {
+        if (b->tile != MPD_TILE_16X15) b->dirty = true;
+        b->tile = MPD_TILE_16X15;   // NEE: the shapes differ by FP32 rounding; keep results independent of timing
+    } else if (b->tune_pending && nsteps >= NSHAPES * TUNE_STEPS) {
         if (int r = batch_tune(b)) return r;
-        nsteps -= 2 * TUNE_STEPS;
+        nsteps -= NSHAPES * TUNE_STEPS;
         if (nsteps == 0) return 0;
     }
     if (b->dirty)

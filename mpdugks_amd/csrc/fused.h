@@ -56,7 +56,7 @@ struct FusedState {
     bool pp = false;     // FP32-bulk pseudopotential: explicit FP32 planes, no derived state
     PP32State pp32;
     bool pp_force_stale = false;  // the stored cell force predates the last update
-    int tile_rows = 15;           // FP32-bulk Allen-Cahn: tile height of the step kernel
+    int tile = MPD_TILE_16X15;   // FP32-bulk Allen-Cahn: tile shape of the step kernel (mpd_tile)
 };
 
 // FP32 planes of the derived state (see fused_kernels.hip); phd is the
@@ -65,11 +65,11 @@ struct FusedPlanes32 {
     float *h, *f, *phi, *phl, *ppu, *ppv, *gl;
     double *phd;
 };
-// one FP32-bulk step (fused_fp32.hip) with 16 x tile_rows tiles (15 or 16)
-void fused32_step(const FusedPlanes32 &in, const FusedPlanes32 &out, const Consts &C, int64_t N, int tile_rows,
+// one FP32-bulk step (fused_fp32.hip) with tiles of shape `tile` (mpd_tile)
+void fused32_step(const FusedPlanes32 &in, const FusedPlanes32 &out, const Consts &C, int64_t N, int tile,
                   hipStream_t st);
-// tile height of the FP32-bulk step for an nx x ny mesh on a GPU with `cus` compute units
-int fused32_tile_rows(int nx, int ny, int cus);
+// tile shape (mpd_tile) of the FP32-bulk step for an nx x ny mesh on a GPU with `cus` compute units
+int fused32_tile_auto(int nx, int ny, int cus);
 
 // batched FP32-bulk step over many cases (fused_fp32.hip)
 struct BatchMember {
@@ -81,7 +81,7 @@ struct BatchMember {
 struct Batch32Plan;
 Batch32Plan *batch32_new();
 void batch32_free(Batch32Plan *P);
-int batch32_plan(Batch32Plan *P, const BatchMember *m, int n, int tile_rows, hipStream_t st);
+int batch32_plan(Batch32Plan *P, const BatchMember *m, int n, int tile, hipStream_t st);
 // one step of every case; parity 0/1 selects in/out relative to each case's cur
 void batch32_step(const Batch32Plan *P, int parity, hipStream_t st);
 // FP32 planes of a fused state (set 0/1)

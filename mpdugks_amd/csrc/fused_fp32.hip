@@ -26,41 +26,46 @@ namespace mpd {
 
 namespace {
 
-constexpr int TX = 16;                     // tile width; the height TY (15 or 16) is a template argument
-constexpr int R1X = TX + 2;                // bar-plus region (cells with a one-cell halo)
-constexpr int R2X = TX + 4;                // phi region (two-cell halo)
-constexpr int NT = 256;
-// LDS plane strides in float2 elements.  Neither is a multiple of 32 float2
-// (64 dwords): the compiler would otherwise pair reads of two planes at one
-// site into one two-address instruction whose two addresses share a bank.
-constexpr int PS = 324;                    // bar-plus planes
 constexpr int NBP = 12;                    // float2 planes: (h, f) bar-plus of q = 0..8, then the
                                            // half macros (ax, ay), (rgx, rgy), (Fx, Fy)
 constexpr int NFL = 7;                     // float2 planes per face direction: 6 (h, f) values, (h mass, 0)
 constexpr bool PRELOAD_RING = false;       // also hoist the second-pass loads above the phi staging
 
-// Tile of TX x TY interior cells, 256 threads (one interior cell each in the
-// update).  TY = 15: 255 x- and 256 y-faces, one of each per thread, and the
-// bar-plus ring takes one partial wave after the interior.  TY = 16: 272
-// faces of each direction (the last 16 of each taken by one wave in a second
-// pass) and a ring of 68 cells (two waves): about 1.2x the instructions of a
-// 16x15 tile for 1.07x the cells, chosen where it removes a partly filled
-// last round of workgroups.
-template <int TY_>
-struct Tile {
-    static constexpr int TY = TY_;
-    static constexpr int R1Y = TY + 2, R2Y = TY + 4;
+// Tile of TX x TY interior cells, one interior cell per thread in the update.
+//  16x15, 256 threads: 255 x- and 256 y-faces, one of each per thread; the
+//    bar-plus ring takes one partial wave after the interior.
+//  16x16, 256 threads: 272 faces of each direction (the last 16 of each on one
+//    wave in a second pass) and a ring of 68 cells (two waves): about 1.2x the
+//    instructions of a 16x15 tile for 1.07x the cells.
+//  32x16, 512 threads: 528 x- and 544 y-faces (16 and 32 in the second pass),
+//    a ring of 100 cells (two waves): per cell about the work of 16x15, with
+//    half as many tiles, in workgroups of twice the LDS (two per CU).
+// LDS plane strides in float2 elements: PS (bar-plus) and FS (face values) are
+// not multiples of 32 float2 (64 dwords), or the compiler would pair reads of
+// two planes at one site into one two-address instruction whose two
+// addresses share a bank.
+template <int TX_, int TY_>
+struct TileK {
+    static constexpr int TX = TX_, TY = TY_;
+    static constexpr int NT = TX * TY > 256 ? 512 : 256;   // threads
+    static constexpr int R1X = TX + 2, R1Y = TY + 2;       // bar-plus region (one-cell halo)
+    static constexpr int R2X = TX + 4, R2Y = TY + 4;       // phi region (two-cell halo)
     static constexpr int R1N = R1X * R1Y, R2N = R2X * R2Y;
-    static constexpr int NIN = TX * TY;               // interior cells
-    static constexpr int NRING = R1N - NIN;           // halo ring of the bar-plus region
-    static constexpr int NXF = (TX + 1) * TY;         // x-faces
-    static constexpr int NYF = TX * (TY + 1);         // y-faces
-    static constexpr int FS = TY == 15 ? 260 : 276;   // face-value plane stride (float2)
-    static_assert(TY == 15 || TY == 16, "tile heights");
+    static constexpr int NIN = TX * TY;                    // interior cells
+    static constexpr int NRING = R1N - NIN;                // halo ring of the bar-plus region
+    static constexpr int NXF = (TX + 1) * TY;              // x-faces
+    static constexpr int NYF = TX * (TY + 1);              // y-faces
+    static constexpr int PS = TX == 16 ? 324 : 612;
+    static constexpr int FS = TX == 32 ? 548 : (TY == 15 ? 260 : 276);
+    static constexpr int EXF = NXF - NT, EYF = NYF - NT;   // second-pass faces
+    static constexpr int NE = 32;                          // second-pass slots per direction
+    static constexpr int XE0 = NT - 128, YE0 = NT - 64;    // first thread of the x / y second pass
+    static constexpr int NBIG = NBP * PS > 2 * NFL * FS ? NBP * PS : 2 * NFL * FS;  // s_big, float2
+    static_assert((TX == 16 && (TY == 15 || TY == 16)) || (TX == 32 && TY == 16), "tile shapes");
     static_assert(R1N <= PS && NXF <= FS && NYF <= FS, "plane strides");
+    static_assert(PS % 32 != 0 && FS % 32 != 0, "plane strides");
     static_assert(NIN <= NT && NRING - (NT - NIN) <= NT, "stage-B mapping");
-    static_assert(NXF - NT <= 64 && NYF - NT <= 64, "second face pass: one wave per direction");
-    static_assert(2 * NFL * FS <= NBP * PS, "face values must fit over the bar-plus planes");
+    static_assert(EXF <= NE && EYF <= NE && NE <= 64, "second face pass: one wave per direction");
 };
 
 struct K32 {
@@ -183,31 +188,33 @@ __device__ __forceinline__ float edot(float x, float y)
 struct Col {
     int pitch, off;
 };
-template <int TY>
+template <class G>
 __device__ __forceinline__ Col col1(int c)
 {
+    constexpr int TX = G::TX;
     const bool in = c >= 1 && c <= TX;
-    return Col{in ? TX : 2, in ? c - 1 : TX * Tile<TY>::R1Y + (c > TX ? 1 : 0)};
+    return Col{in ? TX : 2, in ? c - 1 : TX * G::R1Y + (c > TX ? 1 : 0)};
 }
-template <int TY>
+template <class G>
 __device__ __forceinline__ Col col2(int c)
 {
+    constexpr int TX = G::TX;
     const bool in = c >= 2 && c <= TX + 1;
-    return Col{in ? TX : 4, in ? c - 2 : TX * Tile<TY>::R2Y + (c < 2 ? c : c - TX)};
+    return Col{in ? TX : 4, in ? c - 2 : TX * G::R2Y + (c < 2 ? c : c - TX)};
 }
 __device__ __forceinline__ int at(Col c, int row) { return row * c.pitch + c.off; }
-template <int TY>
-__device__ __forceinline__ int r1(int col, int row) { return at(col1<TY>(col), row); }
+template <class G>
+__device__ __forceinline__ int r1(int col, int row) { return at(col1<G>(col), row); }
 
 // grad(phi), laplacian, SetZero (GradScheme.cpp:347-376) from the phi region s
 // at the nine indices of a 3x3 stencil (ix: -1, 0, +1 columns; rows likewise)
 struct Stencil {
     int w[3], c[3], e[3];  // rows row-1, row, row+1
 };
-template <int TY>
+template <class G>
 __device__ __forceinline__ Stencil stencil_idx(int col, int row)
 {
-    const Col a = col2<TY>(col - 1), b = col2<TY>(col), d = col2<TY>(col + 1);
+    const Col a = col2<G>(col - 1), b = col2<G>(col), d = col2<G>(col + 1);
     Stencil t;
 #pragma unroll
     for (int r = 0; r < 3; ++r) {
@@ -315,18 +322,19 @@ __device__ __forceinline__ void cell_load(const Rs &in, int nx, int ny, bool sma
     }
 }
 
-template <int MODEL, int WALLS, int TY>
+template <int MODEL, int WALLS, class G>
 __device__ __forceinline__ void cell_bp(const K32 &k, const Rs &out, int nx, int ny, int rx, int ry, const CellIn &x,
                                         float2 *S, const float *s_phi, const float *s_phl, float *base, bool interior,
                                         float2 *s_nee)
 {
     constexpr bool HCZ = MODEL == MPD_MODEL_AC_HCZ;
+    constexpr int TX = G::TX, PS = G::PS;
     if (!x.valid) return;
     const float *h = x.h, *f = x.f;
     const uint32_t n = x.n;
     const int gi = x.gi, gj = x.gj;
-    const int c = r1<TY>(rx, ry);
-    const Stencil sti = stencil_idx<TY>(rx + 1, ry + 1);
+    const int c = r1<G>(rx, ry);
+    const Stencil sti = stencil_idx<G>(rx + 1, ry + 1);
     const float phi = s_phi[sti.c[1]];
     const float rho = k.aphi * phi + k.bphi;
     Site m;
@@ -519,12 +527,12 @@ __device__ __forceinline__ float2 recon(const K32 &k, const float2 *b, const Fac
 
 // Indices of the face between R1 cells (oc, orow) and its neighbour in
 // direction DIR, with the tangential neighbours of both (LDS layout above).
-template <int DIR, int TY>
+template <int DIR, class G>
 __device__ __forceinline__ FaceIdx face_idx(int oc, int orow)
 {
     FaceIdx x;
     if (DIR == 0) {
-        const Col a = col1<TY>(oc), b = col1<TY>(oc + 1);
+        const Col a = col1<G>(oc), b = col1<G>(oc + 1);
         x.o = at(a, orow);
         x.n = opaque(at(b, orow));
         x.op = opaque(at(a, orow + 1));
@@ -532,7 +540,7 @@ __device__ __forceinline__ FaceIdx face_idx(int oc, int orow)
         x.np = opaque(at(b, orow + 1));
         x.nm = opaque(at(b, orow - 1));
     } else {
-        const Col a = col1<TY>(oc - 1), b = col1<TY>(oc), d = col1<TY>(oc + 1);
+        const Col a = col1<G>(oc - 1), b = col1<G>(oc), d = col1<G>(oc + 1);
         x.o = at(b, orow);
         x.n = opaque(at(b, orow + 1));
         x.op = opaque(at(d, orow));
@@ -547,11 +555,12 @@ __device__ __forceinline__ FaceIdx face_idx(int oc, int orow)
 // D2Q9AC.cpp:232-329), BGK half step (Update_phi_h, DUGKS2DSolver.cpp:402-423)
 // and micro-flux (Update_phiFlux_h, :424-441) of the six populations with a
 // non-zero normal velocity, in pair order (first member e_n = +1).
-template <int DIR, int MODEL, int WALLS>
+template <int DIR, int MODEL, int WALLS, class G>
 __device__ __forceinline__ void face(const K32 &k, const float2 *S, const FaceIdx &x, int wall, float *hfl,
                                      float *ffl, float &mass)
 {
     constexpr bool HCZ = MODEL == MPD_MODEL_AC_HCZ;
+    constexpr int PS = G::PS;
     const int ko = x.o;
     float hv[9], fv[9];
 #define MPD_RECON(Q)                                      \
@@ -660,23 +669,24 @@ __device__ __forceinline__ void face(const K32 &k, const float2 *S, const FaceId
 
 // x-face t of a tile: column t % TX, row t / TX for t < TX*TY, then the last
 // column (face index TX*TY + row)
-template <int MODEL, int WALLS, int TY>
+template <int MODEL, int WALLS, class G>
 __device__ __forceinline__ void xface(const K32 &k, const float2 *S, int t, float *h, float *f, float &m)
 {
-    constexpr int NIN = Tile<TY>::NIN;
+    constexpr int NIN = G::NIN, TX = G::TX;
     const int fi = t < NIN ? t % TX : TX, fj = t < NIN ? t / TX : t - NIN;
-    face<0, MODEL, WALLS>(k, S, face_idx<0, TY>(fi, fj + 1), 0, h, f, m);
+    face<0, MODEL, WALLS, G>(k, S, face_idx<0, G>(fi, fj + 1), 0, h, f, m);
 }
 
 // y-face t of a tile: column t % TX, row t / TX
-template <int MODEL, int WALLS, int TY>
+template <int MODEL, int WALLS, class G>
 __device__ __forceinline__ void yface(const K32 &k, const float2 *S, const float2 *s_nee, int t, int j0, int ny,
                                       float *h, float *f, float &m)
 {
+    constexpr int TX = G::TX;
     const int fi = t % TX, fj = t / TX;
     const int gj = j0 + fj;
     const int wall = WALLS ? (gj == 0 ? -1 : (gj == ny ? 1 : 0)) : 0;
-    face<1, MODEL, WALLS>(k, S, face_idx<1, TY>(fi + 1, fj), wall, h, f, m);
+    face<1, MODEL, WALLS, G>(k, S, face_idx<1, G>(fi + 1, fj), wall, h, f, m);
     if (WALLS == 2 && wall != 0) {
         // NEE wall face: the values of the wall-row cell (stage B), both
         // members of each pair alike, so no phase-field mass crosses
@@ -697,15 +707,14 @@ __device__ __forceinline__ void yface(const K32 &k, const float2 *S, const float
 //  C: every x- and y-face of the tile (one of each per thread, then for TY = 16
 //     the last 16 of each direction on waves 2 and 3);
 //  D: Update_phi_T, phi moved by the antisymmetric face mass fluxes.
-template <int MODEL, int WALLS, int TY>
+template <int MODEL, int WALLS, class G>
 __device__ __forceinline__ void tile32(const K32 &k, int nx, int ny, const FusedPlanes32 &pin,
                                        const FusedPlanes32 &pout, int64_t N, int i0, int j0)
 {
     constexpr bool HCZ = MODEL == MPD_MODEL_AC_HCZ;
-    using G = Tile<TY>;
-    constexpr int R1Y = G::R1Y, R2Y = G::R2Y, R2N = G::R2N, NIN = G::NIN, NRING = G::NRING;
-    constexpr int NXF = G::NXF, NYF = G::NYF, FS = G::FS;
-    __shared__ float2 s_big[NBP * PS];
+    constexpr int TX = G::TX, TY = G::TY, NT = G::NT, R1X = G::R1X, R1Y = G::R1Y, R2Y = G::R2Y, R2N = G::R2N;
+    constexpr int NIN = G::NIN, NRING = G::NRING, NXF = G::NXF, NYF = G::NYF, PS = G::PS, FS = G::FS;
+    __shared__ float2 s_big[G::NBIG];
     __shared__ float s_phi[R2N];
     __shared__ float s_phl[HCZ ? R2N : 1];
     __shared__ float2 s_nee[WALLS == 2 ? 6 * TX : 1];  // NEE wall values: [bottom, top][column][pair]
@@ -769,11 +778,11 @@ __device__ __forceinline__ void tile32(const K32 &k, int nx, int ny, const Fused
 
     // ---- stage B ----
     float base[17];
-    cell_bp<MODEL, WALLS, TY>(k, out, nx, ny, rx1, ry1, x1, s_big, s_phi, s_phl, base, inside, s_nee);
+    cell_bp<MODEL, WALLS, G>(k, out, nx, ny, rx1, ry1, x1, s_big, s_phi, s_phl, base, inside, s_nee);
     if (tid < NR2) {
         if (!PRELOAD_RING) cell_load<MODEL, WALLS>(in, nx, ny, small, i0, j0, rx2, ry2, x2);
         float dummy[17];
-        cell_bp<MODEL, WALLS, TY>(k, out, nx, ny, rx2, ry2, x2, s_big, s_phi, s_phl, dummy, false, s_nee);
+        cell_bp<MODEL, WALLS, G>(k, out, nx, ny, rx2, ry2, x2, s_big, s_phi, s_phl, dummy, false, s_nee);
     }
     __syncthreads();
 
@@ -786,7 +795,7 @@ __device__ __forceinline__ void tile32(const K32 &k, int nx, int ny, const Fused
         if (row >= 0) {
             const int rx = tid < R1X ? tid : tid - R1X;
             const int dn = row == gy0 && tid < R1X ? 1 : -1;
-            const Col cc = col1<TY>(rx);
+            const Col cc = col1<G>(rx);
             const int c = at(cc, row), kn = at(cc, row + dn), knn = at(cc, row + 2 * dn);
 #pragma unroll
             for (int q = 0; q < NBP; ++q) {
@@ -798,36 +807,35 @@ __device__ __forceinline__ void tile32(const K32 &k, int nx, int ny, const Fused
     }
 
     // ---- stage C ----
-    // TY = 16: the last 16 faces of each direction first, on waves 2 (x) and
-    // 3 (y), straight into LDS of their own (the face planes overlay the
-    // bar-plus planes, which the other faces still read); then one x- and one
-    // y-face per thread, held in registers until every face is computed
-    constexpr int EXF = NXF - NT, EYF = NYF - NT, NE = 16;
-    static_assert(EXF <= NE && EYF <= NE, "second face pass");
+    // faces beyond NT (16x16, 32x16) first, on the second-to-last (x) and
+    // last (y) wave, straight into LDS of their own (the face planes overlay
+    // the bar-plus planes, which the other faces still read); then one x- and
+    // one y-face per thread, held in registers until every face is computed
+    constexpr int EXF = G::EXF, EYF = G::EYF, NE = G::NE, XE0 = G::XE0, YE0 = G::YE0;
     __shared__ float2 s_ext[EXF > 0 || EYF > 0 ? 2 * NFL * NE : 1];
     int et = -1;
-    if (EXF > 0 && tid >= 128 && tid < 128 + EXF) {
-        et = tid - 128;
+    if (EXF > 0 && tid >= XE0 && tid < XE0 + EXF) {
+        et = tid - XE0;
         float h[6], f[6], m = 0.f;
-        xface<MODEL, WALLS, TY>(k, s_big, NT + et, h, f, m);
+        xface<MODEL, WALLS, G>(k, s_big, NT + et, h, f, m);
 #pragma unroll
         for (int z = 0; z < 6; ++z) s_ext[z * NE + et] = make_float2(h[z], f[z]);
         s_ext[6 * NE + et] = make_float2(m, 0.f);
     }
-    if (EYF > 0 && tid >= 192 && tid < 192 + EYF) {
-        et = tid - 192;
+    if (EYF > 0 && tid >= YE0 && tid < YE0 + EYF) {
+        et = tid - YE0;
         float h[6], f[6], m = 0.f;
-        yface<MODEL, WALLS, TY>(k, s_big, s_nee, NT + et, j0, ny, h, f, m);
+        yface<MODEL, WALLS, G>(k, s_big, s_nee, NT + et, j0, ny, h, f, m);
 #pragma unroll
         for (int z = 0; z < 6; ++z) s_ext[(NFL + z) * NE + et] = make_float2(h[z], f[z]);
         s_ext[(NFL + 6) * NE + et] = make_float2(m, 0.f);
     }
     float xh[6], xf[6], xm = 0.f, yh[6], yf[6], ym = 0.f;
-    if (tid < NXF) xface<MODEL, WALLS, TY>(k, s_big, tid, xh, xf, xm);
-    // TY = 16: every thread has both faces, and the scheduler would interleave
-    // the two (past 128 VGPRs, below 4 workgroups per CU); keep them apart
+    if (tid < NXF) xface<MODEL, WALLS, G>(k, s_big, tid, xh, xf, xm);
+    // NXF >= NT: every thread has both faces, and the scheduler would
+    // interleave the two (past 128 VGPRs, below 4 waves per SIMD); keep them apart
     if (NXF >= NT) __builtin_amdgcn_sched_barrier(0);
-    if (tid < NYF) yface<MODEL, WALLS, TY>(k, s_big, s_nee, tid, j0, ny, yh, yf, ym);
+    if (tid < NYF) yface<MODEL, WALLS, G>(k, s_big, s_nee, tid, j0, ny, yh, yf, ym);
     __syncthreads();
     float2 *s_fx = s_big, *s_fy = s_big + NFL * FS;
     if (tid < NXF) {
@@ -842,7 +850,7 @@ __device__ __forceinline__ void tile32(const K32 &k, int nx, int ny, const Fused
     }
     if ((EXF > 0 || EYF > 0) && et >= 0) {
         // this thread's second-pass face into its slot NT + et
-        const bool y = tid >= 192;
+        const bool y = tid >= YE0;
         const float2 *src = s_ext + (y ? NFL * NE : 0) + et;
         float2 *d = (y ? s_fy : s_fx) + NT + et;
 #pragma unroll
@@ -900,19 +908,22 @@ __device__ __forceinline__ void tile32(const K32 &k, int nx, int ny, const Fused
     st(out.a, n, phi, P_PHI * out.pb);
 }
 
-template <int MODEL, int WALLS, int TY>
-__global__ void __launch_bounds__(NT) k_fused32(K32 k, int nx, int ny, FusedPlanes32 in, FusedPlanes32 out, int64_t N,
-                                               int ntx, int ntiles)
+template <int MODEL, int WALLS, int TX, int TY>
+__global__ void __launch_bounds__(TX * TY > 256 ? 512 : 256) k_fused32(K32 k, int nx, int ny, FusedPlanes32 in,
+                                                                        FusedPlanes32 out, int64_t N, int ntx,
+                                                                        int ntiles)
 {
+    static_assert((TX * TY > 256 ? 512 : 256) == TileK<TX, TY>::NT, "launch bounds");
     const int tile = tile_of_block(blockIdx.x, ntiles);
     if (tile < 0) return;
-    tile32<MODEL, WALLS, TY>(k, nx, ny, in, out, N, (tile % ntx) * TX, (tile / ntx) * TY);
+    tile32<MODEL, WALLS, TileK<TX, TY>>(k, nx, ny, in, out, N, (tile % ntx) * TX, (tile / ntx) * TY);
 }
 
 // Batched step: every workgroup takes one (case, tile) entry of a work list
 // built on the host; the cases share the model, the y boundary and the tile height.
-template <int MODEL, int WALLS, int TY>
-__global__ void __launch_bounds__(NT) k_fused32_batch(const BatchCase *cases, const int2 *work, int nwork, int parity)
+template <int MODEL, int WALLS, int TX, int TY>
+__global__ void __launch_bounds__(TX * TY > 256 ? 512 : 256) k_fused32_batch(const BatchCase *cases, const int2 *work,
+                                                                            int nwork, int parity)
 {
     const int b = blockIdx.x;
     if (b >= nwork) return;
@@ -921,7 +932,8 @@ __global__ void __launch_bounds__(NT) k_fused32_batch(const BatchCase *cases, co
     const BatchCase &c = cases[w.x];
     const int sel = c.cur ^ parity;
     const int ntx = (c.nx + TX - 1) / TX;
-    tile32<MODEL, WALLS, TY>(c.k, c.nx, c.ny, c.pl[sel], c.pl[sel ^ 1], c.N, (w.y % ntx) * TX, (w.y / ntx) * TY);
+    tile32<MODEL, WALLS, TileK<TX, TY>>(c.k, c.nx, c.ny, c.pl[sel], c.pl[sel ^ 1], c.N, (w.y % ntx) * TX,
+                                        (w.y / ntx) * TY);
 }
 
 K32 make_k32(const Consts &C)
@@ -999,7 +1011,7 @@ int walls_code(const Consts &C) { return C.walls ? (C.nee ? 2 : 1) : 0; }
 struct Batch32Plan {
     BatchCase *d_cases = nullptr;
     int2 *d_work = nullptr;
-    int ncases = 0, nwork = 0, model = 0, walls = 0, tile_rows = 15;
+    int ncases = 0, nwork = 0, model = 0, walls = 0, tile = MPD_TILE_16X15;
     size_t cap_cases = 0, cap_work = 0;
 };
 
@@ -1015,9 +1027,9 @@ void batch32_free(Batch32Plan *P)
 
 // (Re)build the device tables: cases in order, tiles row-major per case, then
 // dealt so that XCD b%8 receives a contiguous run of the list.
-int batch32_plan(Batch32Plan *P, const BatchMember *m, int n, int tile_rows, hipStream_t st)
+int batch32_plan(Batch32Plan *P, const BatchMember *m, int n, int tile, hipStream_t st)
 {
-    const int ty = tile_rows == 16 ? 16 : 15;
+    const int tx = tile == MPD_TILE_32X16 ? 32 : 16, ty = tile == MPD_TILE_16X15 ? 15 : 16;
     std::vector<BatchCase> cs((size_t)n);
     std::vector<int2> list;
     for (int c = 0; c < n; ++c) {
@@ -1030,7 +1042,7 @@ int batch32_plan(Batch32Plan *P, const BatchMember *m, int n, int tile_rows, hip
         b.pl[0] = m[c].pl[0];
         b.pl[1] = m[c].pl[1];
         b.cur = m[c].cur;
-        const int ntiles = ((C.nx + TX - 1) / TX) * ((C.ny + ty - 1) / ty);
+        const int ntiles = ((C.nx + tx - 1) / tx) * ((C.ny + ty - 1) / ty);
         for (int t = 0; t < ntiles; ++t) list.push_back(make_int2(c, t));
     }
     const int per = ((int)list.size() + 7) / 8;
@@ -1059,73 +1071,87 @@ int batch32_plan(Batch32Plan *P, const BatchMember *m, int n, int tile_rows, hip
     P->nwork = (int)work.size();
     P->model = n ? m[0].C->model : 0;
     P->walls = n ? walls_code(*m[0].C) : 0;
-    P->tile_rows = ty;
+    P->tile = tile == MPD_TILE_32X16 || tile == MPD_TILE_16X16 ? tile : MPD_TILE_16X15;
     return 0;
 }
 
-template <int MODEL, int TY>
+template <int MODEL, int TX, int TY>
 void batch32_launch(const Batch32Plan *P, int parity, hipStream_t st)
 {
-    dim3 g((unsigned)P->nwork), b(NT);
+    dim3 g((unsigned)P->nwork), b(TileK<TX, TY>::NT);
     if (P->walls == 2)
-        hipLaunchKernelGGL((k_fused32_batch<MODEL, 2, TY>), g, b, 0, st, P->d_cases, P->d_work, P->nwork, parity);
+        hipLaunchKernelGGL((k_fused32_batch<MODEL, 2, TX, TY>), g, b, 0, st, P->d_cases, P->d_work, P->nwork, parity);
     else if (P->walls)
-        hipLaunchKernelGGL((k_fused32_batch<MODEL, 1, TY>), g, b, 0, st, P->d_cases, P->d_work, P->nwork, parity);
+        hipLaunchKernelGGL((k_fused32_batch<MODEL, 1, TX, TY>), g, b, 0, st, P->d_cases, P->d_work, P->nwork, parity);
     else
-        hipLaunchKernelGGL((k_fused32_batch<MODEL, 0, TY>), g, b, 0, st, P->d_cases, P->d_work, P->nwork, parity);
+        hipLaunchKernelGGL((k_fused32_batch<MODEL, 0, TX, TY>), g, b, 0, st, P->d_cases, P->d_work, P->nwork, parity);
+}
+
+template <int MODEL>
+void batch32_launch_shape(const Batch32Plan *P, int parity, hipStream_t st)
+{
+    if (P->tile == MPD_TILE_32X16) batch32_launch<MODEL, 32, 16>(P, parity, st);
+    else if (P->tile == MPD_TILE_16X16) batch32_launch<MODEL, 16, 16>(P, parity, st);
+    else batch32_launch<MODEL, 16, 15>(P, parity, st);
 }
 
 void batch32_step(const Batch32Plan *P, int parity, hipStream_t st)
 {
     if (!P->nwork) return;
-    if (P->model == MPD_MODEL_AC_HCZ) {
-        if (P->tile_rows == 16) batch32_launch<MPD_MODEL_AC_HCZ, 16>(P, parity, st);
-        else batch32_launch<MPD_MODEL_AC_HCZ, 15>(P, parity, st);
-    } else {
-        if (P->tile_rows == 16) batch32_launch<MPD_MODEL_AC_P, 16>(P, parity, st);
-        else batch32_launch<MPD_MODEL_AC_P, 15>(P, parity, st);
-    }
+    if (P->model == MPD_MODEL_AC_HCZ) batch32_launch_shape<MPD_MODEL_AC_HCZ>(P, parity, st);
+    else batch32_launch_shape<MPD_MODEL_AC_P>(P, parity, st);
 }
 
-template <int MODEL, int TY>
+template <int MODEL, int TX, int TY>
 void fused32_launch(const K32 &k, const FusedPlanes32 &in, const FusedPlanes32 &out, const Consts &C, int64_t N,
                     hipStream_t st)
 {
     const int ntx = (C.nx + TX - 1) / TX, nty = (C.ny + TY - 1) / TY;
     const int ntiles = ntx * nty;
-    dim3 g((unsigned)(((ntiles + 7) / 8) * 8)), b(NT);
+    dim3 g((unsigned)(((ntiles + 7) / 8) * 8)), b(TileK<TX, TY>::NT);
     const int w = walls_code(C);
-    if (w == 2) hipLaunchKernelGGL((k_fused32<MODEL, 2, TY>), g, b, 0, st, k, C.nx, C.ny, in, out, N, ntx, ntiles);
-    else if (w) hipLaunchKernelGGL((k_fused32<MODEL, 1, TY>), g, b, 0, st, k, C.nx, C.ny, in, out, N, ntx, ntiles);
-    else hipLaunchKernelGGL((k_fused32<MODEL, 0, TY>), g, b, 0, st, k, C.nx, C.ny, in, out, N, ntx, ntiles);
+    if (w == 2) hipLaunchKernelGGL((k_fused32<MODEL, 2, TX, TY>), g, b, 0, st, k, C.nx, C.ny, in, out, N, ntx, ntiles);
+    else if (w) hipLaunchKernelGGL((k_fused32<MODEL, 1, TX, TY>), g, b, 0, st, k, C.nx, C.ny, in, out, N, ntx, ntiles);
+    else hipLaunchKernelGGL((k_fused32<MODEL, 0, TX, TY>), g, b, 0, st, k, C.nx, C.ny, in, out, N, ntx, ntiles);
 }
 
-int fused32_tile_rows(int nx, int ny, int cus)
+template <int MODEL>
+void fused32_launch_shape(const K32 &k, const FusedPlanes32 &in, const FusedPlanes32 &out, const Consts &C,
+                          int64_t N, int tile, hipStream_t st)
 {
-    // 16 rows where that brings the tiles within one round of resident
-    // workgroups (4 per CU) and 15 rows would not; else 15 rows (fewer
-    // instructions per cell).  Measured on MI355X (256 CUs), FP32 HCZ with
-    // walls, us per step 16x15 -> 16x16: 256x1024 18.9 -> 18.2, 512x512
-    // 19.3 -> 17.6, 1024x256 20.4 -> 18.9; on meshes of several rounds
-    // either height wins by up to 5% with no simple rule (1024x1024 51.8 vs
-    // 54.4, 2048x2048 261 vs 249), and one-round meshes favour 15 rows.
-    const int64_t slots = 4 * (int64_t)(cus > 0 ? cus : 256);
-    const int64_t ntx = (nx + TX - 1) / TX;
-    const int64_t t15 = ntx * ((ny + 14) / 15), t16 = ntx * ((ny + 15) / 16);
-    return t16 <= slots && t15 > slots ? 16 : 15;
+    if (tile == MPD_TILE_32X16) fused32_launch<MODEL, 32, 16>(k, in, out, C, N, st);
+    else if (tile == MPD_TILE_16X16) fused32_launch<MODEL, 16, 16>(k, in, out, C, N, st);
+    else fused32_launch<MODEL, 16, 15>(k, in, out, C, N, st);
 }
 
-void fused32_step(const FusedPlanes32 &in, const FusedPlanes32 &out, const Consts &C, int64_t N, int tile_rows,
+int fused32_tile_auto(int nx, int ny, int cus)
+{
+    // A launch whose tiles all fit in one round of resident workgroups (4 per
+    // CU of 256 threads, 2 of 512) runs without a partly filled last round.
+    // 32x16 where its tiles fit one round and 16x15 tiles would not; else
+    // 16x16 under the same condition; else 16x15 (the fewest instructions per
+    // cell on meshes of several rounds).  Measured on MI355X (256 CUs), FP32
+    // HCZ with walls, us per step 16x15 / 16x16 / 32x16: 256x1024 19.2 / 18.1
+    // / 17.5, 1024x256 20.5 / 19.0 / 17.2, 512x512 19.5 / 18.2 / 18.0,
+    // 1024x1024 51.9 / 54.2 / 54.7, 768x768 32.8 / 35.5 / 37.2, 128x128 8.9 /
+    // 9.5 / 10.7; 2048x2048 262 / 251 / 225 is the one measured mesh of
+    // several rounds where the rule does not pick the fastest.
+    const int64_t slots = 4 * (int64_t)(cus > 0 ? cus : 256);
+    const int64_t t15 = (int64_t)((nx + 15) / 16) * ((ny + 14) / 15);
+    const int64_t t16 = (int64_t)((nx + 15) / 16) * ((ny + 15) / 16);
+    const int64_t t32 = (int64_t)((nx + 31) / 32) * ((ny + 15) / 16);
+    if (t15 <= slots) return MPD_TILE_16X15;
+    if (t32 <= slots / 2) return MPD_TILE_32X16;
+    if (t16 <= slots) return MPD_TILE_16X16;
+    return MPD_TILE_16X15;
+}
+
+void fused32_step(const FusedPlanes32 &in, const FusedPlanes32 &out, const Consts &C, int64_t N, int tile,
                   hipStream_t st)
 {
     const K32 k = make_k32(C);
-    if (C.model == MPD_MODEL_AC_HCZ) {
-        if (tile_rows == 16) fused32_launch<MPD_MODEL_AC_HCZ, 16>(k, in, out, C, N, st);
-        else fused32_launch<MPD_MODEL_AC_HCZ, 15>(k, in, out, C, N, st);
-    } else {
-        if (tile_rows == 16) fused32_launch<MPD_MODEL_AC_P, 16>(k, in, out, C, N, st);
-        else fused32_launch<MPD_MODEL_AC_P, 15>(k, in, out, C, N, st);
-    }
+    if (C.model == MPD_MODEL_AC_HCZ) fused32_launch_shape<MPD_MODEL_AC_HCZ>(k, in, out, C, N, tile, st);
+    else fused32_launch_shape<MPD_MODEL_AC_P>(k, in, out, C, N, tile, st);
 }
 
 }  // namespace mpd

@@ -24,8 +24,8 @@ _SYMBOLS = {
     "mpd_create": ([C.c_void_p, C.c_void_p], C.c_int),
     "mpd_destroy": ([C.c_void_p], C.c_int),
     "mpd_set_path": ([C.c_void_p, C.c_int32], C.c_int),
-    "mpd_set_tile_rows": ([C.c_void_p, C.c_int32], C.c_int),
-    "mpd_get_tile_rows": ([C.c_void_p, C.POINTER(C.c_int32)], C.c_int),
+    "mpd_set_tile": ([C.c_void_p, C.c_int32], C.c_int),
+    "mpd_get_tile": ([C.c_void_p, C.POINTER(C.c_int32)], C.c_int),
     "mpd_upload_state": ([C.c_void_p] + [C.c_void_p] * 5, C.c_int),
     "mpd_download_state": ([C.c_void_p] + [C.c_void_p] * 5, C.c_int),
     "mpd_step": ([C.c_void_p, C.c_int64], C.c_int),
@@ -42,8 +42,8 @@ _SYMBOLS = {
     "mpd_batch_size": ([C.c_void_p, C.c_void_p, C.c_void_p], C.c_int),
     "mpd_batch_step": ([C.c_void_p, C.c_int64], C.c_int),
     "mpd_batch_timing": ([C.c_void_p, C.c_void_p, C.c_void_p], C.c_int),
-    "mpd_batch_set_tile_rows": ([C.c_void_p, C.c_int32], C.c_int),
-    "mpd_batch_tile_rows": ([C.c_void_p, C.POINTER(C.c_int32), C.POINTER(C.c_double), C.POINTER(C.c_double)], C.c_int),
+    "mpd_batch_set_tile": ([C.c_void_p, C.c_int32], C.c_int),
+    "mpd_batch_tile": ([C.c_void_p, C.POINTER(C.c_int32), C.POINTER(C.c_double)], C.c_int),
     "mpd_init_ac_drop": ([C.c_void_p, C.c_double, C.c_double, C.c_double, C.c_int32, C.c_double,
                           C.c_double, C.c_void_p, C.c_void_p, C.c_void_p], C.c_int),
     "mpd_init_ac_interface": ([C.c_void_p, C.c_void_p, C.c_int32, C.c_double, C.c_void_p, C.c_void_p,
@@ -187,13 +187,14 @@ class Solver:
     def set_path(self, path):
         _check(lib().mpd_set_path(self._h, int(path)), "mpd_set_path")
 
-    def set_tile_rows(self, rows):
-        """FP32-bulk Allen-Cahn tile height: 0 chosen per mesh, 15 or 16."""
-        _check(lib().mpd_set_tile_rows(self._h, int(rows)), "mpd_set_tile_rows")
+    def set_tile(self, tile):
+        """FP32-bulk Allen-Cahn tile shape (abi.MPD_TILE_*; MPD_TILE_AUTO chooses per mesh)."""
+        _check(lib().mpd_set_tile(self._h, int(tile)), "mpd_set_tile")
 
-    def tile_rows(self):
+    def tile(self):
+        """The tile shape (abi.MPD_TILE_*) the next step uses."""
         r = C.c_int32()
-        _check(lib().mpd_get_tile_rows(self._h, C.byref(r)), "mpd_get_tile_rows")
+        _check(lib().mpd_get_tile(self._h, C.byref(r)), "mpd_get_tile")
         return r.value
 
     def init_drops(self, drops, sign=-1, p0=1.0, mu0=0.1):
@@ -278,15 +279,15 @@ class Batch:
         _check(lib().mpd_batch_timing(self._b, C.byref(ms), C.byref(n)), "mpd_batch_timing")
         return ms.value, n.value
 
-    def set_tile_rows(self, rows):
-        """0: tuned per membership (default); 15 or 16: forced."""
-        _check(lib().mpd_batch_set_tile_rows(self._b, int(rows)), "mpd_batch_set_tile_rows")
+    def set_tile(self, tile):
+        """abi.MPD_TILE_AUTO: tuned per membership (default); another shape: forced."""
+        _check(lib().mpd_batch_set_tile(self._b, int(tile)), "mpd_batch_set_tile")
 
-    def tile_rows(self):
-        """(rows in use, ms/step measured with 15 rows, with 16 rows) of the last tuning."""
-        r, a, b = C.c_int32(), C.c_double(), C.c_double()
-        _check(lib().mpd_batch_tile_rows(self._b, C.byref(r), C.byref(a), C.byref(b)), "mpd_batch_tile_rows")
-        return r.value, a.value, b.value
+    def tile(self):
+        """(shape in use, [ms/step of 16x15, 16x16, 32x16] from the last tuning, 0 if not measured)."""
+        r, ms = C.c_int32(), (C.c_double * 3)()
+        _check(lib().mpd_batch_tile(self._b, C.byref(r), ms), "mpd_batch_tile")
+        return r.value, list(ms)
 
     def close(self):
         if getattr(self, "_b", None) and self._b.value:

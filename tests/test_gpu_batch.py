@@ -90,33 +90,33 @@ def test_batch_nee_walls_matches_individual_cases():
     b.close()
 
 
-def test_batch_tile_heights_and_autotuner():
-    """Forced 16x15 and 16x16 batches give the same states bit for bit; the
-    autotuner's 16 timed steps are ordinary steps (the tuned batch equals the
-    cases stepped alone) and it keeps the faster height."""
+def test_batch_tile_shapes_and_autotuner():
+    """Forced 16x15, 16x16 and 32x16 batches give the same states bit for bit;
+    the autotuner's 24 tuning steps are ordinary steps (the tuned batch equals
+    the cases stepped alone) and it keeps the fastest shape."""
     specs = cases.ensemble_specs(5, seed=11, sides=(64, 96, 128))
     runs = {}
-    for mode in (15, 16, 0):
+    for mode in (abi.MPD_TILE_16X15, abi.MPD_TILE_16X16, abi.MPD_TILE_32X16, abi.MPD_TILE_AUTO):
         members = [_primed(sp) for sp in specs]
         b = solver.Batch()
         for s in members:
             b.add(s)
-        b.set_tile_rows(mode)
+        b.set_tile(mode)
         b.step(40)
         b.step(3)
-        rows, ms15, ms16 = b.tile_rows()
+        tile, ms = b.tile()
         if mode:
-            assert rows == mode
+            assert tile == mode
         else:
-            assert ms15 > 0 and ms16 > 0 and rows == (16 if ms16 < ms15 else 15), (rows, ms15, ms16)
+            assert min(ms) > 0 and tile == 1 + int(np.argmin(ms)), (tile, ms)
         runs[mode] = [m.download() for m in members]
         b.close()
     single = [_primed(sp) for sp in specs]
     for s in single:
         s.step(43)
     ref = [s.download() for s in single]
-    for mode in (15, 16, 0):
-        for g, r in zip(runs[mode], ref):
+    for mode, got in runs.items():
+        for g, r in zip(got, ref):
             assert np.array_equal(g.f, r.f) and np.array_equal(g.h, r.h) and np.array_equal(g.m, r.m), mode
 
 
@@ -130,10 +130,10 @@ def test_batch_nee_keeps_16x15_unless_forced():
     b = solver.Batch()
     b.add(s)
     b.step(40)
-    assert b.tile_rows() == (15, 0.0, 0.0)
-    b.set_tile_rows(16)
+    assert b.tile() == (abi.MPD_TILE_16X15, [0.0, 0.0, 0.0])
+    b.set_tile(abi.MPD_TILE_32X16)
     b.step(2)
-    assert b.tile_rows()[0] == 16
+    assert b.tile()[0] == abi.MPD_TILE_32X16
     with pytest.raises(solver.MpdError):
-        b.set_tile_rows(17)
+        b.set_tile(4)
     b.close()

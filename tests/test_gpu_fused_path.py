@@ -112,19 +112,19 @@ FP32_WALL_CASES = [
 ]
 
 
-@pytest.mark.parametrize("rows", [15, 16])
+@pytest.mark.parametrize("tile", [1, 2, 3], ids=["16x15", "16x16", "32x16"])
 @pytest.mark.parametrize("model,mesh,bc,steps", FP32_WALL_CASES,
                          ids=["%s_%dx%d_%s" % (c[0], c[1][0], c[1][1], c[2]) for c in FP32_WALL_CASES])
-def test_fp32_bulk_walls_error_vs_fp64_oracle(model, mesh, bc, steps, rows, oracle_mod):
+def test_fp32_bulk_walls_error_vs_fp64_oracle(model, mesh, bc, steps, tile, oracle_mod):
     """FP32-bulk with walls against the FP64 oracle (same bar as periodic), with
-    either tile height: a drop touching neither wall, so the wall faces carry
+    every tile shape: a drop touching neither wall, so the wall faces carry
     the non-equilibrium part of a fluid at rest plus the pressure field."""
     nx, ny = mesh
     cfg, st = make_case(model, nx=nx, ny=ny, walls=True, nee=bc == "nee", sign=-1,
                         radius=0.3 * ny, precision=abi.MPD_FP32_BULK)
     ref = oracle_run(cfg, st, [steps])[steps]
     s = solver.Solver(cfg)
-    s.set_tile_rows(rows)
+    s.set_tile(tile)
     s.upload(st)
     s.step(steps)
     g = s.download()
@@ -133,7 +133,7 @@ def test_fp32_bulk_walls_error_vs_fp64_oracle(model, mesh, bc, steps, rows, orac
     N = cfg.nx * cfg.ny
     for name in ("u", "v"):
         errs[name] = rel_l2(g.m[abi.M[name]], ref.m[abi.M[name]], floor=1e-6 * np.sqrt(N))
-    print("fp32-bulk %s %s walls %d steps, 16x%d tiles, errors: %s" % (model, bc, steps, rows, errs))
+    print("fp32-bulk %s %s walls %d steps, %s tiles, errors: %s" % (model, bc, steps, abi.TILE_NAMES[tile], errs))
     assert errs["rho"] < 1e-5 and errs["phi"] < 1e-5 and errs["p"] < 1e-5, errs
     assert errs["u"] < 1e-3 and errs["v"] < 1e-3, errs
 

@@ -1,10 +1,10 @@
-"""FP32-bulk Allen-Cahn step with 16x15 and 16x16 tiles: the tile height is a
-launch shape only (every cell and face is computed by the same source code
-whichever tile holds it), on meshes that fill neither tile shape, with
+"""FP32-bulk Allen-Cahn step with 16x15, 16x16 and 32x16 tiles: the tile shape
+is a launch shape only (every cell and face is computed by the same source
+code whichever tile holds it), on meshes that fill no tile shape, with
 periodic, BB and NEE y boundaries.  Periodic and BB: the same state bit for
-bit.  NEE: the two kernel instantiations contract the wall-row arithmetic
-into FMAs differently, so the states agree to FP32 rounding (relative L2
-within 1e-6 after 30 steps); each is held to the FP64 oracle in
+bit.  NEE: the kernel instantiations contract the wall-row arithmetic into
+FMAs differently, so the states agree to FP32 rounding (relative L2 within
+1e-6 after 30 steps); each is held to the FP64 oracle in
 test_gpu_fused_path.py."""
 import numpy as np
 import pytest
@@ -24,14 +24,14 @@ CASES = [
 ]
 
 
-def _run(name, kw, rows, steps):
+def _run(name, kw, tile, steps):
     if kw is None:
         cfg, st = cases.rayleigh_taylor(nx=256, ny=1024, precision=abi.MPD_FP32_BULK)
     else:
         cfg, st = make_case(precision=abi.MPD_FP32_BULK, **kw)
     s = solver.Solver(cfg, path=abi.MPD_PATH_FUSED)
-    s.set_tile_rows(rows)
-    assert s.tile_rows() == rows
+    s.set_tile(tile)
+    assert s.tile() == tile
     s.upload(st)
     s.step(steps)
     g = s.download()
@@ -40,9 +40,14 @@ def _run(name, kw, rows, steps):
 
 
 @pytest.mark.parametrize("name,kw", CASES, ids=[c[0] for c in CASES])
-def test_tile_heights_agree_bit_for_bit(name, kw):
+def test_tile_shapes_agree_bit_for_bit(name, kw):
     steps = 30
-    a, b = _run(name, kw, 15, steps), _run(name, kw, 16, steps)
+    a = _run(name, kw, abi.MPD_TILE_16X15, steps)
+    for b in (_run(name, kw, abi.MPD_TILE_16X16, steps), _run(name, kw, abi.MPD_TILE_32X16, steps)):
+        _agree(name, kw, a, b)
+
+
+def _agree(name, kw, a, b):
     nee = kw is not None and kw.get("nee", False)
     for plane in ("h", "f", "m"):
         x, y = getattr(a, plane), getattr(b, plane)
@@ -50,27 +55,27 @@ def test_tile_heights_agree_bit_for_bit(name, kw):
             continue
         if nee:
             d = float(np.sqrt(((x - y) ** 2).sum()) / np.sqrt((y ** 2).sum()))
-            print("%s plane %s: relative L2 %.3e between tile heights" % (name, plane, d))
+            print("%s plane %s: relative L2 %.3e between tile shapes" % (name, plane, d))
             assert d < 1e-6, (name, plane, d)
         elif not np.array_equal(x, y):
             bad = np.argwhere(x != y)
             raise AssertionError("%s: plane %s differs at %d entries (first %s)" % (name, plane, len(bad), bad[0].tolist()))
 
 
-def test_tile_rows_choice_and_validation():
+def test_tile_choice_and_validation():
     cfg, st = cases.rayleigh_taylor(nx=256, ny=1024, precision=abi.MPD_FP32_BULK)
     s = solver.Solver(cfg)
-    # 256x1024: 16x15 tiles make 1,104 workgroups, more than one resident
-    # round on a 256-CU device; 16x16 tiles make exactly 1,024
-    assert s.tile_rows() == 16
+    # 256x1024 on a 256-CU device: 1,104 tiles of 16x15 exceed one round of
+    # resident workgroups (1,024), the 512 tiles of 32x16 fit one (2 per CU)
+    assert s.tile() == abi.MPD_TILE_32X16
     with pytest.raises(solver.MpdError):
-        s.set_tile_rows(14)
-    s.set_tile_rows(15)
-    assert s.tile_rows() == 15
-    s.set_tile_rows(0)
-    assert s.tile_rows() == 16
+        s.set_tile(4)
+    s.set_tile(abi.MPD_TILE_16X16)
+    assert s.tile() == abi.MPD_TILE_16X16
+    s.set_tile(abi.MPD_TILE_AUTO)
+    assert s.tile() == abi.MPD_TILE_32X16
     s.close()
     cfg, st = make_case("ac_hcz", nx=128, ny=128, precision=abi.MPD_FP32_BULK)
     s = solver.Solver(cfg)
-    assert s.tile_rows() == 15   # 72 tiles: one round either way, the smaller tile
+    assert s.tile() == abi.MPD_TILE_16X15   # 72 tiles: one round with any shape, the smallest tile
     s.close()
