@@ -378,7 +378,7 @@ int mpd_set_tile(mpd_handle *h, int32_t tile)
 int mpd_get_tile(mpd_handle *h, int32_t *tile)
 {
     if (!h || !tile) return fail(MPD_ERR_INVALID, "null argument");
-    *tile = h->tile ? h->tile : fused32_tile_auto(h->cfg.nx, h->cfg.ny, h->cus);
+    *tile = h->tile ? h->tile : fused32_tile_auto(h->cfg.nx, h->cfg.ny, h->cus, fused32_state_bytes(h->cfg));
     return 0;
 }
 
@@ -546,7 +546,8 @@ int mpd_step(mpd_handle *h, int64_t nsteps)
             if (r) return fail(MPD_ERR_NOMEM, "fused path allocation failed");
         }
         h->F.pp32.phases = pp32_phases;
-        h->F.tile = h->tile ? h->tile : fused32_tile_auto(h->cfg.nx, h->cfg.ny, h->cus);
+        h->F.tile = h->tile ? h->tile
+                            : fused32_tile_auto(h->cfg.nx, h->cfg.ny, h->cus, fused32_state_bytes(h->cfg));
         int64_t s = 0;
         if (!h->fused_primed && !fused_needs_priming(h->cfg)) {
             // explicit state carried as is (FP32-bulk pseudopotential)

@@ -68,8 +68,11 @@ struct FusedPlanes32 {
 // one FP32-bulk step (fused_fp32.hip) with tiles of shape `tile` (mpd_tile)
 void fused32_step(const FusedPlanes32 &in, const FusedPlanes32 &out, const Consts &C, int64_t N, int tile,
                   hipStream_t st);
-// tile shape (mpd_tile) of the FP32-bulk step for an nx x ny mesh on a GPU with `cus` compute units
-int fused32_tile_auto(int nx, int ny, int cus);
+// tile shape (mpd_tile) of the FP32-bulk step for an nx x ny mesh on a GPU with
+// `cus` compute units, whose two state sets take `state_bytes` of device memory
+int fused32_tile_auto(int nx, int ny, int cus, int64_t state_bytes);
+// device bytes of the two FP32-bulk Allen-Cahn state sets of a configuration
+int64_t fused32_state_bytes(const mpd_config &cfg);
 
 // batched FP32-bulk step over many cases (fused_fp32.hip)
 struct BatchMember {

@@ -1124,26 +1124,29 @@ void fused32_launch_shape(const K32 &k, const FusedPlanes32 &in, const FusedPlan
     else fused32_launch<MODEL, 16, 15>(k, in, out, C, N, st);
 }
 
-int fused32_tile_auto(int nx, int ny, int cus)
+int fused32_tile_auto(int nx, int ny, int cus, int64_t state_bytes)
 {
-    // A launch whose tiles all fit in one round of resident workgroups (4 per
-    // CU of 256 threads, 2 of 512) runs without a partly filled last round.
-    // 32x16 where its tiles fit one round and 16x15 tiles would not; else
-    // 16x16 under the same condition; else 16x15 (the fewest instructions per
-    // cell on meshes of several rounds).  Measured on MI355X (256 CUs), FP32
-    // HCZ with walls, us per step 16x15 / 16x16 / 32x16: 256x1024 19.2 / 18.1
-    // / 17.5, 1024x256 20.5 / 19.0 / 17.2, 512x512 19.5 / 18.2 / 18.0,
-    // 1024x1024 51.9 / 54.2 / 54.7, 768x768 32.8 / 35.5 / 37.2, 128x128 8.9 /
-    // 9.5 / 10.7; 2048x2048 262 / 251 / 225 is the one measured mesh of
-    // several rounds where the rule does not pick the fastest.
+    // One round: a launch whose tiles all fit the resident workgroups (4 per
+    // CU of 256 threads, 2 of 512) has no partly filled last round: 32x16
+    // where its tiles fit one round and 16x15 tiles would not, else 16x16
+    // under the same condition.  Several rounds: 16x15 (the fewest
+    // instructions per cell) while the two state sets fit the 256 MiB
+    // Infinity Cache; beyond it the step streams from HBM and 32x16, with
+    // fewer halo lines re-read per cell, is faster.  Measured on MI355X (256
+    // CUs), FP32 HCZ with walls, us per step 16x15 / 16x16 / 32x16: 256x1024
+    // 19.2 / 18.1 / 17.5, 1024x256 20.5 / 19.0 / 17.2, 512x512 19.5 / 18.2 /
+    // 18.0, 128x128 8.9 / 9.5 / 10.7, 768x768 32.8 / 35.5 / 37.2, 1024x1024
+    // 51.6 / 53.7 / 54.2, 1280x1280 118 / 117 / 104, 2048x1024 137 / 132 /
+    // 121, 2048x2048 264 / 252 / 227, 3072x2048 415 / 410 / 350.
     const int64_t slots = 4 * (int64_t)(cus > 0 ? cus : 256);
     const int64_t t15 = (int64_t)((nx + 15) / 16) * ((ny + 14) / 15);
     const int64_t t16 = (int64_t)((nx + 15) / 16) * ((ny + 15) / 16);
     const int64_t t32 = (int64_t)((nx + 31) / 32) * ((ny + 15) / 16);
+    const int64_t mall = (int64_t)256 << 20;
     if (t15 <= slots) return MPD_TILE_16X15;
     if (t32 <= slots / 2) return MPD_TILE_32X16;
     if (t16 <= slots) return MPD_TILE_16X16;
-    return MPD_TILE_16X15;
+    return state_bytes > mall ? MPD_TILE_32X16 : MPD_TILE_16X15;
 }
 
 void fused32_step(const FusedPlanes32 &in, const FusedPlanes32 &out, const Consts &C, int64_t N, int tile,

@@ -600,6 +600,14 @@ bool fused_supported(const mpd_config &cfg)
 
 bool fused_needs_priming(const mpd_config &cfg) { return cfg.model != MPD_MODEL_PP_D2Q9; }
 
+int64_t fused32_state_bytes(const mpd_config &cfg)
+{
+    // per set: the float planes (fused_alloc) and the FP64 phase field
+    const bool walls = cfg.bc_y != MPD_BC_PERIODIC;
+    const int64_t planes = cfg.model == MPD_MODEL_AC_HCZ ? 22 : (walls ? 20 : 19);
+    return 2 * (int64_t)cfg.nx * cfg.ny * (4 * planes + 8);
+}
+
 int fused_alloc(FusedState &F, const DevState &S, const Consts &C, const mpd_config &cfg)
 {
     fused_free(F);

@@ -11,6 +11,7 @@ import pytest
 
 from mpdugks_amd import abi, cases, solver
 from helpers import make_case
+from mpdugks_amd.config import default_config
 
 pytestmark = pytest.mark.gpu
 
@@ -75,7 +76,11 @@ def test_tile_choice_and_validation():
     s.set_tile(abi.MPD_TILE_AUTO)
     assert s.tile() == abi.MPD_TILE_32X16
     s.close()
-    cfg, st = make_case("ac_hcz", nx=128, ny=128, precision=abi.MPD_FP32_BULK)
-    s = solver.Solver(cfg)
-    assert s.tile() == abi.MPD_TILE_16X15   # 72 tiles: one round with any shape, the smallest tile
-    s.close()
+    for (nx, ny), want in (((128, 128), abi.MPD_TILE_16X15),      # one round with any shape: the smallest
+                           ((1024, 1024), abi.MPD_TILE_16X15),    # several rounds, state within the cache
+                           ((1280, 1280), abi.MPD_TILE_32X16)):   # several rounds, state beyond 256 MiB
+        cfg = default_config(abi.MPD_MODEL_AC_HCZ, nx=nx, ny=ny)
+        cfg.precision = abi.MPD_FP32_BULK
+        s = solver.Solver(cfg)
+        assert s.tile() == want, (nx, ny, s.tile())
+        s.close()
