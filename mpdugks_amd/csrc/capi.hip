@@ -371,6 +371,8 @@ int mpd_set_tile(mpd_handle *h, int32_t tile)
 {
     if (!h) return fail(MPD_ERR_INVALID, "null handle");
     if (tile < MPD_TILE_AUTO || tile > MPD_TILE_32X16) return fail(MPD_ERR_INVALID, "unknown tile shape %d", tile);
+    if (is_pp(h->cfg) && tile == MPD_TILE_16X16)
+        return fail(MPD_ERR_INVALID, "the pseudopotential step has 16x15 and 32x16 tiles");
     h->tile = tile;
     return 0;
 }
@@ -378,7 +380,9 @@ int mpd_set_tile(mpd_handle *h, int32_t tile)
 int mpd_get_tile(mpd_handle *h, int32_t *tile)
 {
     if (!h || !tile) return fail(MPD_ERR_INVALID, "null argument");
-    *tile = h->tile ? h->tile : fused32_tile_auto(h->cfg.nx, h->cfg.ny, h->cus, fused32_state_bytes(h->cfg));
+    if (h->tile) *tile = h->tile;
+    else if (is_pp(h->cfg)) *tile = pp32_tile_auto(h->cfg.nx, h->cfg.ny, h->cus);
+    else *tile = fused32_tile_auto(h->cfg.nx, h->cfg.ny, h->cus, fused32_state_bytes(h->cfg));
     return 0;
 }
 
@@ -548,6 +552,7 @@ int mpd_step(mpd_handle *h, int64_t nsteps)
         h->F.pp32.phases = pp32_phases;
         h->F.tile = h->tile ? h->tile
                             : fused32_tile_auto(h->cfg.nx, h->cfg.ny, h->cus, fused32_state_bytes(h->cfg));
+        h->F.pp32.tile = h->tile;
         int64_t s = 0;
         if (!h->fused_primed && !fused_needs_priming(h->cfg)) {
             // explicit state carried as is (FP32-bulk pseudopotential)

@@ -31,7 +31,11 @@ struct PP32State {
     PP32Planes P;         // the current set, tau, and the scratch of the per-phase kernels
     PP32Set other;        // the second set
     bool phases = false;  // step with one launch per phase instead of the one-launch tile kernel
+    int cus = 256;        // compute units of the device (tile shape choice)
+    int tile = MPD_TILE_AUTO;  // tile shape of the one-launch step: 16x15, 32x16, or chosen per mesh
 };
+// tile shape (MPD_TILE_16X15 or MPD_TILE_32X16) of the FP32-bulk pseudopotential step
+int pp32_tile_auto(int nx, int ny, int cus);
 int pp32_alloc(PP32State &S, void **mem, int64_t *bytes, const Consts &C, int64_t N);
 void pp32_from64(const PP32State &S, const DevState &D, hipStream_t st);
 // force_stale: the stored cell force is the previous step's (every step but the

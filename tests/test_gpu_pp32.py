@@ -13,6 +13,7 @@ import pytest
 
 from mpdugks_amd import abi, solver
 from helpers import make_case, oracle_run
+from mpdugks_amd.config import default_config
 
 pytestmark = pytest.mark.gpu
 
@@ -125,3 +126,28 @@ def test_pp32_switch_paths_mid_run():
     g = s.download()
     s.close()
     assert rel_l2(g.f, r.f) < 1e-6
+
+
+def test_pp32_tile_shapes_agree_bit_for_bit():
+    """The one-launch step with 16x15 and with 32x16 tiles (a launch shape
+    only): the same state bit for bit, on a mesh neither shape fills."""
+    out = []
+    for tile in (abi.MPD_TILE_16X15, abi.MPD_TILE_32X16):
+        cfg = default_config(abi.MPD_MODEL_PP_D2Q9, nx=100, ny=70, pp_tr=0.85)
+        cfg.precision = abi.MPD_FP32_BULK
+        s = solver.Solver(cfg)
+        s.set_tile(tile)
+        assert s.tile() == tile
+        s.init_pp_noise(0.1645, amp=0.01, seed=5)
+        s.step(120)
+        out.append(s.download())
+        s.close()
+    a, b = out
+    assert np.array_equal(a.f, b.f) and np.array_equal(a.m, b.m) and np.array_equal(a.ffx, b.ffx)
+    cfg = default_config(abi.MPD_MODEL_PP_D2Q9, nx=512, ny=512, pp_tr=0.85)
+    cfg.precision = abi.MPD_FP32_BULK
+    s = solver.Solver(cfg)
+    assert s.tile() == abi.MPD_TILE_32X16      # 512 tiles of 32x16: every CU gets two
+    with pytest.raises(solver.MpdError):
+        s.set_tile(abi.MPD_TILE_16X16)
+    s.close()
