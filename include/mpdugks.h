@@ -148,8 +148,10 @@ int mpd_destroy(mpd_handle *h);
 /* Select the kernel path (mpd_path); valid before the first mpd_step. */
 int mpd_set_path(mpd_handle *h, int32_t path);
 
-/* Tile shape of the FP32-bulk step kernels (cells per workgroup): the
- * Allen-Cahn kernel has all three, the pseudopotential kernel 16x15 and 32x16.
+/* Tile shape of the one-launch step kernels (cells per workgroup): the
+ * FP32-bulk Allen-Cahn kernel has all three, the FP32-bulk pseudopotential
+ * kernel 16x15 and 32x16, the FP64 Allen-Cahn kernel 16x15 (other shapes are
+ * refused with MPD_ERR_INVALID).
  * MPD_TILE_AUTO (the default) chooses per mesh; the others force one.  The
  * state does not depend on it (bit for bit with periodic and BB walls, to
  * FP32 rounding with NEE walls).  Takes effect at the next mpd_step;

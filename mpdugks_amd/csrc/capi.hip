@@ -139,6 +139,13 @@ void make_consts(const mpd_config &c, Consts &C)
     C.rtdt = c.rt * c.dt;
     C.wdphi = c.width * (c.phi_l - c.phi_v);
     C.twodx = 2 * c.dx;
+    {
+        int e = 0;
+        // a normal power of two whose reciprocal (and half of it) is normal too
+        C.dx_pow2 = std::frexp(c.dx, &e) == 0.5 && e > -1000 && e < 1000;
+        C.rdx = 1.0 / c.dx;
+        C.rtwodx = 1.0 / C.twodx;
+    }
     C.ah_h = 2.0 * C.taum / (2.0 * C.taum + C.hdt);
     C.bh_h = 1.0 - C.ah_h;
     C.ch_h = C.taum * C.bh_h;
@@ -373,6 +380,8 @@ int mpd_set_tile(mpd_handle *h, int32_t tile)
     if (tile < MPD_TILE_AUTO || tile > MPD_TILE_32X16) return fail(MPD_ERR_INVALID, "unknown tile shape %d", tile);
     if (is_pp(h->cfg) && tile == MPD_TILE_16X16)
         return fail(MPD_ERR_INVALID, "the pseudopotential step has 16x15 and 32x16 tiles");
+    if (!is_pp(h->cfg) && h->cfg.precision == MPD_FP64 && tile != MPD_TILE_AUTO && tile != MPD_TILE_16X15)
+        return fail(MPD_ERR_INVALID, "the FP64 step has 16x15 tiles");
     h->tile = tile;
     return 0;
 }
@@ -382,6 +391,7 @@ int mpd_get_tile(mpd_handle *h, int32_t *tile)
     if (!h || !tile) return fail(MPD_ERR_INVALID, "null argument");
     if (h->tile) *tile = h->tile;
     else if (is_pp(h->cfg)) *tile = pp32_tile_auto(h->cfg.nx, h->cfg.ny, h->cus);
+    else if (h->cfg.precision == MPD_FP64) *tile = MPD_TILE_16X15;
     else *tile = fused32_tile_auto(h->cfg.nx, h->cfg.ny, h->cus, fused32_state_bytes(h->cfg));
     return 0;
 }
@@ -550,9 +560,10 @@ int mpd_step(mpd_handle *h, int64_t nsteps)
             if (r) return fail(MPD_ERR_NOMEM, "fused path allocation failed");
         }
         h->F.pp32.phases = pp32_phases;
-        h->F.tile = h->tile ? h->tile
-                            : fused32_tile_auto(h->cfg.nx, h->cfg.ny, h->cus, fused32_state_bytes(h->cfg));
-        h->F.pp32.tile = h->tile;
+        if (is_pp(h->cfg)) h->F.pp32.tile = h->tile;
+        else if (h->cfg.precision == MPD_FP32_BULK)
+            h->F.tile = h->tile ? h->tile
+                                : fused32_tile_auto(h->cfg.nx, h->cfg.ny, h->cus, fused32_state_bytes(h->cfg));
         int64_t s = 0;
         if (!h->fused_primed && !fused_needs_priming(h->cfg)) {
             // explicit state carried as is (FP32-bulk pseudopotential)

@@ -34,14 +34,19 @@ namespace mpd {
 
 namespace {
 
-constexpr int TX = 16, TY = 15;            // tile
-constexpr int R1X = TX + 2, R1Y = TY + 2;  // one-cell halo (bar-plus, sources)
-constexpr int R2X = TX + 4, R2Y = TY + 4;  // two-cell halo (phi stencils)
-constexpr int R1N = R1X * R1Y, R2N = R2X * R2Y;
-constexpr int NXF = (TX + 1) * TY;         // 255 x-faces
-constexpr int NYF = TX * (TY + 1);         // 256 y-faces
 constexpr int NTHREADS = 256;
-static_assert(NXF <= NTHREADS && NYF <= NTHREADS && TX * TY <= NTHREADS, "tile too large");
+
+template <int TX_, int TY_>
+struct Tile64 {
+    static constexpr int TX = TX_, TY = TY_;
+    static constexpr int R1X = TX + 2, R1Y = TY + 2;  // one-cell halo (bar-plus, sources)
+    static constexpr int R2X = TX + 4, R2Y = TY + 4;  // two-cell halo (phi stencils)
+    static constexpr int R1N = R1X * R1Y, R2N = R2X * R2Y;
+    static constexpr int NXF = (TX + 1) * TY;         // x-faces
+    static constexpr int NYF = TX * (TY + 1);         // y-faces
+    static_assert(TX * TY <= NTHREADS && NXF <= NTHREADS && NYF <= NTHREADS, "one cell and face per thread");
+    static_assert(12 * (NXF + NYF) <= (2 * Q9 + 6) * R1N, "face fluxes must fit over the bar-plus planes");
+};
 
 template <typename T>
 struct Planes {
@@ -112,11 +117,119 @@ __device__ inline void derive_cell(const Consts &C, const T *ht, const T *ft, T 
     m.Fy = s.Fy;
 }
 
-template <typename T, int MODEL, int WALLS>
-__global__ void __launch_bounds__(NTHREADS) k_fused_step(Consts C, Planes<T> in, Planes<T> out, int64_t N, int ntx,
+// x-face t of a tile: face (fi, fj) between R1 cells (fi, fj+1) and
+// (fi+1, fj+1); fluxes of the six populations with e_x != 0
+template <typename T, int MODEL, bool P2, class G>
+__device__ inline void xface64(const Consts &C, const T (*s_bp)[G::R1N], const T (*s_mac)[G::R1N], int t, T *xh,
+                               T *xf)
+{
+    constexpr int TX = G::TX, R1X = G::R1X;
+    constexpr int MX[6] = {1, 2, 4, 5, 6, 8};
+    const T hdt = (T)C.hdt;
+    const int fi = t % (TX + 1), fj = t / (TX + 1);
+    const int ko = (fj + 1) * R1X + fi, kn = ko + 1;
+    T hb[Q9], fb[Q9];
+#pragma unroll
+    for (int q = 0; q < 2 * Q9; ++q) {
+        const T *b = s_bp[q];
+        T v = cd_face_value<T, P2>(C, 0, q % Q9, b[ko], b[kn], b[kn + R1X], b[kn - R1X], b[ko + R1X], b[ko - R1X]);
+        if (q < Q9) hb[q] = v; else fb[q - Q9] = v;
+    }
+    FaceMacro<T> fm;
+    fm.px = (T)0.5 * (s_mac[0][ko] + s_mac[0][kn]);
+    fm.py = (T)0.5 * (s_mac[1][ko] + s_mac[1][kn]);
+    fm.rx = (T)0.5 * (s_mac[2][ko] + s_mac[2][kn]);
+    fm.ry = (T)0.5 * (s_mac[3][ko] + s_mac[3][kn]);
+    fm.Fx = (T)0.5 * (s_mac[4][ko] + s_mac[4][kn]);
+    fm.Fy = (T)0.5 * (s_mac[5][ko] + s_mac[5][kn]);
+    face_macros_ac<T, MODEL>(C, hb, fb, fm);
+    T ah = (T)2.0 * fm.tau / ((T)2.0 * fm.tau + hdt);
+    T bh = (T)1.0 - ah;
+    T ch = fm.tau * bh;
+#pragma unroll
+    for (int z = 0; z < 6; ++z) {
+        const int q = MX[z];
+        T hh, ff;
+        face_bgk<T, MODEL>(C, fm, q, hb[q], fb[q], ah, bh, ch, hh, ff);
+        const T xin = (T)C.dx * (T)C.xi[0][q];
+        xh[z] = hh * xin;
+        xf[z] = ff * xin;
+    }
+}
+
+// y-face t of a tile: face (fi, fj) between R1 cells (fi+1, fj) and
+// (fi+1, fj+1); fluxes of the six populations with e_y != 0, wall faces
+// included
+template <typename T, int MODEL, int WALLS, bool P2, class G>
+__device__ inline void yface64(const Consts &C, const T (*s_bp)[G::R1N], const T (*s_mac)[G::R1N], const T *s_nee,
+                               int t, int j0, int ny, T *yh, T *yf)
+{
+    constexpr int TX = G::TX, R1X = G::R1X;
+    constexpr int MY[6] = {2, 3, 4, 6, 7, 8};
+    const T hdt = (T)C.hdt;
+    const int fi = t % TX, fj = t / TX;
+    const int ko = fj * R1X + fi + 1, kn = ko + R1X;
+    T hb[Q9], fb[Q9];
+#pragma unroll
+    for (int q = 0; q < 2 * Q9; ++q) {
+        const T *b = s_bp[q];
+        T v = cd_face_value<T, P2>(C, 1, q % Q9, b[ko], b[kn], b[kn + 1], b[kn - 1], b[ko + 1], b[ko - 1]);
+        if (q < Q9) hb[q] = v; else fb[q - Q9] = v;
+    }
+    FaceMacro<T> fm;
+    fm.px = (T)0.5 * (s_mac[0][ko] + s_mac[0][kn]);
+    fm.py = (T)0.5 * (s_mac[1][ko] + s_mac[1][kn]);
+    fm.rx = (T)0.5 * (s_mac[2][ko] + s_mac[2][kn]);
+    fm.ry = (T)0.5 * (s_mac[3][ko] + s_mac[3][kn]);
+    fm.Fx = (T)0.5 * (s_mac[4][ko] + s_mac[4][kn]);
+    fm.Fy = (T)0.5 * (s_mac[5][ko] + s_mac[5][kn]);
+    face_macros_ac<T, MODEL>(C, hb, fb, fm);
+    T ah = (T)2.0 * fm.tau / ((T)2.0 * fm.tau + hdt);
+    T bh = (T)1.0 - ah;
+    T ch = fm.tau * bh;
+    // face values of the populations with e_y != 0 (wall faces: the
+    // incoming ones are replaced below by their partners, also e_y != 0)
+    T vh[Q9], vf[Q9];
+#pragma unroll
+    for (int z = 0; z < 6; ++z) {
+        const int q = MY[z];
+        face_bgk<T, MODEL>(C, fm, q, hb[q], fb[q], ah, bh, ch, vh[q], vf[q]);
+    }
+    const int gj = j0 + fj;
+    if (WALLS == 2 && (gj == 0 || gj == ny)) {
+        // Wall_3_NEE: the wall-row cell's values (stage B), then bounce-back
+        const T *v = s_nee + ((gj == 0 ? 0 : 1) * TX + fi) * 2 * Q9;
+#pragma unroll
+        for (int z = 0; z < 6; ++z) { vh[MY[z]] = v[MY[z]]; vf[MY[z]] = v[Q9 + MY[z]]; }
+    }
+    if (WALLS && (gj == 0 || gj == ny)) {
+        // Wall_3_BB (Ark/Cpp/Boundary.cpp:118-139): incoming take the partner's value
+        T h0[Q9], f0[Q9];
+#pragma unroll
+        for (int z = 0; z < 6; ++z) { h0[MY[z]] = vh[MY[z]]; f0[MY[z]] = vf[MY[z]]; }
+#pragma unroll
+        for (int z = 0; z < 6; ++z) {
+            const int q = MY[z];
+            const bool inc = gj == 0 ? (ey9(q) > 0) : (ey9(q) < 0);
+            if (inc) { vh[q] = h0[bb9(q)]; vf[q] = f0[bb9(q)]; }
+        }
+    }
+#pragma unroll
+    for (int z = 0; z < 6; ++z) {
+        const int q = MY[z];
+        const T xin = (T)C.dx * (T)C.xi[1][q];
+        yh[z] = vh[q] * xin;
+        yf[z] = vf[q] * xin;
+    }
+}
+
+template <typename T, int MODEL, int WALLS, bool P2, class G>
+__global__ void __launch_bounds__(NTHREADS, 2) k_fused_step(Consts C, Planes<T> in, Planes<T> out, int64_t N, int ntx,
                                                          int ntiles)
 {
     constexpr bool HCZ = MODEL == MPD_MODEL_AC_HCZ;
+    constexpr int TX = G::TX, TY = G::TY, R1X = G::R1X, R2X = G::R2X, R1N = G::R1N, R2N = G::R2N;
+    constexpr int NXF = G::NXF, NYF = G::NYF;
     // bar-plus (h in 0..8, f in 9..17) and the six face sources px, py, rx,
     // ry, Fx, Fy on the one-cell halo; after the faces, the face fluxes of the
     // six populations with a non-zero normal velocity (x: 12 x NXF, y: 12 x NYF)
@@ -127,7 +240,6 @@ __global__ void __launch_bounds__(NTHREADS) k_fused_step(Consts C, Planes<T> in,
     T(*s_mac)[R1N] = reinterpret_cast<T(*)[R1N]>(s_big + 2 * Q9 * R1N);
     T *s_phi = s_ph, *s_phl = s_ph + R2N;
     T *s_fx = s_big, *s_fy = s_big + 12 * NXF;
-    static_assert(12 * (NXF + NYF) <= (2 * Q9 + 6) * R1N, "face fluxes must fit over the bar-plus planes");
 
     const int tile = tile_of_block(blockIdx.x, ntiles);
     if (tile < 0) return;
@@ -245,99 +357,12 @@ __global__ void __launch_bounds__(NTHREADS) k_fused_step(Consts C, Planes<T> in,
     // with a non-zero normal velocity; the other three carry a zero flux) in
     // registers until every face has read the bar-plus planes, then writes
     // them over those planes.
-    const T hdt = (T)C.hdt;
-    constexpr int MX[6] = {1, 2, 4, 5, 6, 8}, MY[6] = {2, 3, 4, 6, 7, 8};
-    T xh[6], xf[6];
-    // x-faces: thread -> face (fi, fj) between R1 cells (fi, fj+1) and (fi+1, fj+1)
-    if (tid < NXF) {
-        const int fi = tid % (TX + 1), fj = tid / (TX + 1);
-        const int ko = (fj + 1) * R1X + fi, kn = ko + 1;
-        T hb[Q9], fb[Q9];
-#pragma unroll
-        for (int q = 0; q < 2 * Q9; ++q) {
-            const T *b = s_bp[q];
-            T v = cd_face_value<T>(C, 0, q % Q9, b[ko], b[kn], b[kn + R1X], b[kn - R1X], b[ko + R1X], b[ko - R1X]);
-            if (q < Q9) hb[q] = v; else fb[q - Q9] = v;
-        }
-        FaceMacro<T> fm;
-        fm.px = (T)0.5 * (s_mac[0][ko] + s_mac[0][kn]);
-        fm.py = (T)0.5 * (s_mac[1][ko] + s_mac[1][kn]);
-        fm.rx = (T)0.5 * (s_mac[2][ko] + s_mac[2][kn]);
-        fm.ry = (T)0.5 * (s_mac[3][ko] + s_mac[3][kn]);
-        fm.Fx = (T)0.5 * (s_mac[4][ko] + s_mac[4][kn]);
-        fm.Fy = (T)0.5 * (s_mac[5][ko] + s_mac[5][kn]);
-        face_macros_ac<T, MODEL>(C, hb, fb, fm);
-        T ah = (T)2.0 * fm.tau / ((T)2.0 * fm.tau + hdt);
-        T bh = (T)1.0 - ah;
-        T ch = fm.tau * bh;
-#pragma unroll
-        for (int z = 0; z < 6; ++z) {
-            const int q = MX[z];
-            T hh, ff;
-            face_bgk<T, MODEL>(C, fm, q, hb[q], fb[q], ah, bh, ch, hh, ff);
-            const T xin = (T)C.dx * (T)C.xi[0][q];
-            xh[z] = hh * xin;
-            xf[z] = ff * xin;
-        }
-    }
-    // y-faces: thread -> face (fi, fj) between R1 cells (fi+1, fj) and (fi+1, fj+1)
-    T yh[6], yf[6];
-    {
-        const int fi = tid % TX, fj = tid / TX;
-        const int ko = fj * R1X + fi + 1, kn = ko + R1X;
-        T hb[Q9], fb[Q9];
-#pragma unroll
-        for (int q = 0; q < 2 * Q9; ++q) {
-            const T *b = s_bp[q];
-            T v = cd_face_value<T>(C, 1, q % Q9, b[ko], b[kn], b[kn + 1], b[kn - 1], b[ko + 1], b[ko - 1]);
-            if (q < Q9) hb[q] = v; else fb[q - Q9] = v;
-        }
-        FaceMacro<T> fm;
-        fm.px = (T)0.5 * (s_mac[0][ko] + s_mac[0][kn]);
-        fm.py = (T)0.5 * (s_mac[1][ko] + s_mac[1][kn]);
-        fm.rx = (T)0.5 * (s_mac[2][ko] + s_mac[2][kn]);
-        fm.ry = (T)0.5 * (s_mac[3][ko] + s_mac[3][kn]);
-        fm.Fx = (T)0.5 * (s_mac[4][ko] + s_mac[4][kn]);
-        fm.Fy = (T)0.5 * (s_mac[5][ko] + s_mac[5][kn]);
-        face_macros_ac<T, MODEL>(C, hb, fb, fm);
-        T ah = (T)2.0 * fm.tau / ((T)2.0 * fm.tau + hdt);
-        T bh = (T)1.0 - ah;
-        T ch = fm.tau * bh;
-        // face values of the populations with e_y != 0 (wall faces: the
-        // incoming ones are replaced below by their partners, also e_y != 0)
-        T vh[Q9], vf[Q9];
-#pragma unroll
-        for (int z = 0; z < 6; ++z) {
-            const int q = MY[z];
-            face_bgk<T, MODEL>(C, fm, q, hb[q], fb[q], ah, bh, ch, vh[q], vf[q]);
-        }
-        const int gj = j0 + fj;
-        if (WALLS == 2 && (gj == 0 || gj == ny)) {
-            // Wall_3_NEE: the wall-row cell's values (stage B), then bounce-back
-            const T *v = s_nee + ((gj == 0 ? 0 : 1) * TX + fi) * 2 * Q9;
-#pragma unroll
-            for (int z = 0; z < 6; ++z) { vh[MY[z]] = v[MY[z]]; vf[MY[z]] = v[Q9 + MY[z]]; }
-        }
-        if (WALLS && (gj == 0 || gj == ny)) {
-            // Wall_3_BB (Ark/Cpp/Boundary.cpp:118-139): incoming take the partner's value
-            T h0[Q9], f0[Q9];
-#pragma unroll
-            for (int z = 0; z < 6; ++z) { h0[MY[z]] = vh[MY[z]]; f0[MY[z]] = vf[MY[z]]; }
-#pragma unroll
-            for (int z = 0; z < 6; ++z) {
-                const int q = MY[z];
-                const bool inc = gj == 0 ? (ey9(q) > 0) : (ey9(q) < 0);
-                if (inc) { vh[q] = h0[bb9(q)]; vf[q] = f0[bb9(q)]; }
-            }
-        }
-#pragma unroll
-        for (int z = 0; z < 6; ++z) {
-            const int q = MY[z];
-            const T xin = (T)C.dx * (T)C.xi[1][q];
-            yh[z] = vh[q] * xin;
-            yf[z] = vf[q] * xin;
-        }
-    }
+    // keep the two face passes apart: interleaved, they hold both faces'
+    // operands at once (past 256 VGPRs, one wave per SIMD)
+    T xh[6], xf[6], yh[6], yf[6];
+    if (tid < NXF) xface64<T, MODEL, P2, G>(C, s_bp, s_mac, tid, xh, xf);
+    __builtin_amdgcn_sched_barrier(0);
+    if (tid < NYF) yface64<T, MODEL, WALLS, P2, G>(C, s_bp, s_mac, s_nee, tid, j0, ny, yh, yf);
     // own bar-plus before the region is reused for the face fluxes
     const int ci = tid % TX, cj = tid / TX;
     const bool cell = tid < TX * TY;
@@ -355,10 +380,12 @@ __global__ void __launch_bounds__(NTHREADS) k_fused_step(Consts C, Planes<T> in,
             s_fx[(6 + z) * NXF + tid] = xf[z];
         }
     }
+    if (tid < NYF) {
 #pragma unroll
-    for (int z = 0; z < 6; ++z) {
-        s_fy[z * NYF + tid] = yh[z];
-        s_fy[(6 + z) * NYF + tid] = yf[z];
+        for (int z = 0; z < 6; ++z) {
+            s_fy[z * NYF + tid] = yh[z];
+            s_fy[(6 + z) * NYF + tid] = yf[z];
+        }
     }
     __syncthreads();
 
@@ -516,16 +543,25 @@ Planes<T> planes_of(FusedState &F, int which)
     return P;
 }
 
+template <typename T, int MODEL, int WALLS, bool P2>
+void launch_step(FusedState &F, const Consts &C, hipStream_t st)
+{
+    using G = Tile64<16, 15>;
+    Planes<T> in = planes_of<T>(F, F.cur), out = planes_of<T>(F, 1 - F.cur);
+    const int ntx = (C.nx + G::TX - 1) / G::TX, nty = (C.ny + G::TY - 1) / G::TY;
+    const int ntiles = ntx * nty;
+    const int nblocks = ((ntiles + 7) / 8) * 8;
+    hipLaunchKernelGGL((k_fused_step<T, MODEL, WALLS, P2, G>), dim3(nblocks), dim3(NTHREADS), 0, st, C, in, out,
+                       (int64_t)F.ncell, ntx, ntiles);
+    F.cur = 1 - F.cur;
+}
+
 template <typename T, int MODEL, int WALLS>
 void launch_step(FusedState &F, const Consts &C, hipStream_t st)
 {
-    Planes<T> in = planes_of<T>(F, F.cur), out = planes_of<T>(F, 1 - F.cur);
-    const int ntx = (C.nx + TX - 1) / TX, nty = (C.ny + TY - 1) / TY;
-    const int ntiles = ntx * nty;
-    const int nblocks = ((ntiles + 7) / 8) * 8;
-    hipLaunchKernelGGL((k_fused_step<T, MODEL, WALLS>), dim3(nblocks), dim3(NTHREADS), 0, st, C, in, out,
-                       (int64_t)F.ncell, ntx, ntiles);
-    F.cur = 1 - F.cur;
+    // dx a power of two: face gradients by its exact reciprocal
+    if (C.dx_pow2) launch_step<T, MODEL, WALLS, true>(F, C, st);
+    else launch_step<T, MODEL, WALLS, false>(F, C, st);
 }
 
 template <typename T>

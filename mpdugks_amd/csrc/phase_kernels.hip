@@ -116,11 +116,11 @@ __device__ inline double face_value(const DevState &S, const Consts &C, const do
     double bo = cellbp(S, C, bp, gbp, q, oi, oj), bn = cellbp(S, C, bp, gbp, q, ni, nj);
     if (FLUX == MPD_FLUX_CD) {
         if (dir == 0)
-            return cd_face_value<double>(C, 0, q, bo, bn, cellbp(S, C, bp, gbp, q, ni, nj + 1),
+            return cd_face_value<double, false>(C, 0, q, bo, bn, cellbp(S, C, bp, gbp, q, ni, nj + 1),
                                          cellbp(S, C, bp, gbp, q, ni, nj - 1),
                                          cellbp(S, C, bp, gbp, q, oi, oj + 1),
                                          cellbp(S, C, bp, gbp, q, oi, oj - 1));
-        return cd_face_value<double>(C, 1, q, bo, bn, cellbp(S, C, bp, gbp, q, ni + 1, nj),
+        return cd_face_value<double, false>(C, 1, q, bo, bn, cellbp(S, C, bp, gbp, q, ni + 1, nj),
                                      cellbp(S, C, bp, gbp, q, ni - 1, nj),
                                      cellbp(S, C, bp, gbp, q, oi + 1, oj),
                                      cellbp(S, C, bp, gbp, q, oi - 1, oj));

@@ -23,6 +23,7 @@ __host__ __device__ constexpr int bb9(int q) { return q == 0 ? 0 : (q <= 4 ? q +
 struct Consts {
     int32_t nx, ny, model, flux, walls, nyf;
     int32_t nee;      // walls use non-equilibrium extrapolation (Wall_3_NEE) instead of bounce-back
+    int32_t dx_pow2;  // dx is a power of two: rdx, rtwodx are exact reciprocals
     double dx, dt, hdt, rt, lambda0, aphi, bphi, beta, kappa, kp, taum, tau0;
     double phi_l, phi_v, nu_l, nu_v, gx, gy, rho_ref, pp_k, t_cs, pp_tau;
     double dtv;       // dt / volume                    (Cell_2D.cpp:182)
@@ -31,6 +32,7 @@ struct Consts {
     double rtdt;      // RT*dt                          (SourceM.cpp:51)
     double wdphi;     // wI*(PhiL-PhiV)                 (SourceM.cpp:17)
     double twodx;     // 2*dx, CD tangential span       (MeshCartesian.cpp:145,203)
+    double rdx, rtwodx;  // 1/dx, 1/(2*dx)
     double ah_h, bh_h, ch_h, abp_h, bbp_h, cbp_h;      // constant tau_h factors
     double ah_pp, bh_pp, ch_pp;                         // PP face factors
     double xi[2][Q9];
@@ -127,12 +129,15 @@ __device__ inline void cell_barplus(const Consts &C, const CellMacro<T> &m, cons
 // CD face value (CD_Interior_phi_Bh, Ark/Cpp/DUGKS2DSolver.cpp:604-687) on the
 // structured face with owner o (low side) and neighbour n (high side):
 //   normal derivative (n - o)/dx, tangential (0.5*(neigh pair) + 0.5*(owner pair))/(2dx).
-template <typename T>
+template <typename T, bool P2>
 __device__ inline T cd_face_value(const Consts &C, int dir, int q, T bo, T bn, T tn_hi, T tn_lo,
                                   T to_hi, T to_lo)
 {
-    T dn = (bn - bo) / (T)C.dx;
-    T dt = ((T)0.5 * (tn_hi - tn_lo) + (T)0.5 * (to_hi - to_lo)) / (T)C.twodx;
+    // P2 (dx a power of two): x * 2^-k and x / 2^k are the same real number,
+    // so the correctly rounded product equals the quotient bit for bit
+    const T tsum = (T)0.5 * (tn_hi - tn_lo) + (T)0.5 * (to_hi - to_lo);
+    const T dn = P2 ? (bn - bo) * (T)C.rdx : (bn - bo) / (T)C.dx;
+    const T dt = P2 ? tsum * (T)C.rtwodx : tsum / (T)C.twodx;
     const T ex = (T)C.xi[0][q], ey = (T)C.xi[1][q];
     if (dir == 0) return (T)0.5 * (bo + bn) - (T)C.hdt * (dn * ex + dt * ey);
     return (T)0.5 * (bo + bn) - (T)C.hdt * (dt * ex + dn * ey);

@@ -26,6 +26,10 @@ CASES = [
     ("hcz_nee_64x48_droplet", dict(model="ac_hcz", nx=64, ny=48, walls=True, nee=True, sign=-1, radius=21.0)),
     ("acp_nee_70x47", dict(model="ac_p", nx=70, ny=47, walls=True, nee=True, radius=20.0)),
     ("hcz_nee_20x12", dict(model="ac_hcz", nx=20, ny=12, walls=True, nee=True, sign=-1, radius=4.0)),
+    # dx a power of two other than 1 (face gradients by the exact reciprocal)
+    # and one that is not (face gradients by division)
+    ("hcz_walls_48x40_dx0.5", dict(model="ac_hcz", nx=48, ny=40, walls=True, radius=6.0, dx=0.5, dt=0.25)),
+    ("acp_periodic_40x36_dx0.75", dict(model="ac_p", nx=40, ny=36, radius=9.0, dx=0.75, dt=0.375)),
 ]
 
 

@@ -1,11 +1,15 @@
-"""FP32-bulk Allen-Cahn step with 16x15, 16x16 and 32x16 tiles: the tile shape
+"""Tile shapes of the one-launch Allen-Cahn step.
+
+FP32-bulk step with 16x15, 16x16 and 32x16 tiles: the tile shape
 is a launch shape only (every cell and face is computed by the same source
 code whichever tile holds it), on meshes that fill no tile shape, with
 periodic, BB and NEE y boundaries.  Periodic and BB: the same state bit for
 bit.  NEE: the kernel instantiations contract the wall-row arithmetic into
 FMAs differently, so the states agree to FP32 rounding (relative L2 within
 1e-6 after 30 steps); each is held to the FP64 oracle in
-test_gpu_fused_path.py."""
+test_gpu_fused_path.py.
+
+FP64 step: 16x15 tiles only; other shapes are refused."""
 import numpy as np
 import pytest
 
@@ -84,3 +88,17 @@ def test_tile_choice_and_validation():
         s = solver.Solver(cfg)
         assert s.tile() == want, (nx, ny, s.tile())
         s.close()
+
+
+def test_fp64_tile_validation():
+    cfg, st = cases.rayleigh_taylor(nx=256, ny=1024, precision=abi.MPD_FP64)
+    s = solver.Solver(cfg, path=abi.MPD_PATH_FUSED)
+    assert s.tile() == abi.MPD_TILE_16X15
+    for t in (abi.MPD_TILE_16X16, abi.MPD_TILE_32X16):
+        with pytest.raises(solver.MpdError):
+            s.set_tile(t)
+    s.set_tile(abi.MPD_TILE_16X15)
+    assert s.tile() == abi.MPD_TILE_16X15
+    s.set_tile(abi.MPD_TILE_AUTO)
+    assert s.tile() == abi.MPD_TILE_16X15
+    s.close()
