@@ -139,13 +139,8 @@ void make_consts(const mpd_config &c, Consts &C)
     C.rtdt = c.rt * c.dt;
     C.wdphi = c.width * (c.phi_l - c.phi_v);
     C.twodx = 2 * c.dx;
-    {
-        int e = 0;
-        // a normal power of two whose reciprocal (and half of it) is normal too
-        C.dx_pow2 = std::frexp(c.dx, &e) == 0.5 && e > -1000 && e < 1000;
-        C.rdx = 1.0 / c.dx;
-        C.rtwodx = 1.0 / C.twodx;
-    }
+    C.rdx = 1.0 / c.dx;
+    C.rtwodx = 1.0 / C.twodx;
     C.ah_h = 2.0 * C.taum / (2.0 * C.taum + C.hdt);
     C.bh_h = 1.0 - C.ah_h;
     C.ch_h = C.taum * C.bh_h;
@@ -163,6 +158,11 @@ void make_consts(const mpd_config &c, Consts &C)
         C.xi[1][q] = maspan * ey9(q);
         C.w[q] = w9[q];
     }
+    // unit lattice: dx a normal power of two whose reciprocal (and half of it)
+    // is normal too, and the lattice velocities exactly 0 and +-1
+    int e = 0;
+    C.unit = std::frexp(c.dx, &e) == 0.5 && e > -1000 && e < 1000;
+    for (int q = 0; q < Q9; ++q) C.unit = C.unit && C.xi[0][q] == ex9(q) && C.xi[1][q] == ey9(q);
 }
 
 // ---------------- reductions ----------------

@@ -101,13 +101,13 @@ __device__ inline void lag_sources(const Consts &C, T phi, const T *c, const T *
 
 // All reference macros of one cell from the derived state.  pc/pd: phi
 // stencil (E,N,W,S / NE,NW,SW,SE), lc/ld: phi_lag stencil (HCZ).
-template <typename T, int MODEL>
+template <typename T, int MODEL, bool UNIT = false>
 __device__ inline void derive_cell(const Consts &C, const T *ht, const T *ft, T phi_lag, const T *lc, const T *ld,
                                    T ppu, T ppv, const T *pc, const T *pd, CellMacro<T> &m, AcSource<T> &s, T &mu)
 {
     T Fl_x = 0, Fl_y = 0, rl_x = 0, rl_y = 0;
     if (MODEL == MPD_MODEL_AC_HCZ) lag_sources<T>(C, phi_lag, lc, ld, Fl_x, Fl_y, rl_x, rl_y);
-    cell_macrovar_ac<T, MODEL>(C, ht, ft, Fl_x, Fl_y, rl_x, rl_y, m.phi, m.rho, m.U, m.V, m.p, mu, m.tau);
+    cell_macrovar_ac<T, MODEL, UNIT>(C, ht, ft, Fl_x, Fl_y, rl_x, rl_y, m.phi, m.rho, m.U, m.V, m.p, mu, m.tau);
     ac_source<T>(C, m.phi, m.rho, m.U, m.V, ppu, ppv, pc, pd, s);
     m.px = s.sx;
     m.py = s.sy;
@@ -119,7 +119,7 @@ __device__ inline void derive_cell(const Consts &C, const T *ht, const T *ft, T 
 
 // x-face t of a tile: face (fi, fj) between R1 cells (fi, fj+1) and
 // (fi+1, fj+1); fluxes of the six populations with e_x != 0
-template <typename T, int MODEL, bool P2, class G>
+template <typename T, int MODEL, bool UNIT, class G>
 __device__ inline void xface64(const Consts &C, const T (*s_bp)[G::R1N], const T (*s_mac)[G::R1N], int t, T *xh,
                                T *xf)
 {
@@ -132,7 +132,7 @@ __device__ inline void xface64(const Consts &C, const T (*s_bp)[G::R1N], const T
 #pragma unroll
     for (int q = 0; q < 2 * Q9; ++q) {
         const T *b = s_bp[q];
-        T v = cd_face_value<T, P2>(C, 0, q % Q9, b[ko], b[kn], b[kn + R1X], b[kn - R1X], b[ko + R1X], b[ko - R1X]);
+        T v = cd_face_value<T, UNIT>(C, 0, q % Q9, b[ko], b[kn], b[kn + R1X], b[kn - R1X], b[ko + R1X], b[ko - R1X]);
         if (q < Q9) hb[q] = v; else fb[q - Q9] = v;
     }
     FaceMacro<T> fm;
@@ -142,7 +142,7 @@ __device__ inline void xface64(const Consts &C, const T (*s_bp)[G::R1N], const T
     fm.ry = (T)0.5 * (s_mac[3][ko] + s_mac[3][kn]);
     fm.Fx = (T)0.5 * (s_mac[4][ko] + s_mac[4][kn]);
     fm.Fy = (T)0.5 * (s_mac[5][ko] + s_mac[5][kn]);
-    face_macros_ac<T, MODEL>(C, hb, fb, fm);
+    face_macros_ac<T, MODEL, UNIT>(C, hb, fb, fm);
     T ah = (T)2.0 * fm.tau / ((T)2.0 * fm.tau + hdt);
     T bh = (T)1.0 - ah;
     T ch = fm.tau * bh;
@@ -150,8 +150,8 @@ __device__ inline void xface64(const Consts &C, const T (*s_bp)[G::R1N], const T
     for (int z = 0; z < 6; ++z) {
         const int q = MX[z];
         T hh, ff;
-        face_bgk<T, MODEL>(C, fm, q, hb[q], fb[q], ah, bh, ch, hh, ff);
-        const T xin = (T)C.dx * (T)C.xi[0][q];
+        face_bgk<T, MODEL, UNIT>(C, fm, q, hb[q], fb[q], ah, bh, ch, hh, ff);
+        const T xin = (T)C.dx * xi_of<UNIT, T>(C, 0, q);
         xh[z] = hh * xin;
         xf[z] = ff * xin;
     }
@@ -160,7 +160,7 @@ __device__ inline void xface64(const Consts &C, const T (*s_bp)[G::R1N], const T
 // y-face t of a tile: face (fi, fj) between R1 cells (fi+1, fj) and
 // (fi+1, fj+1); fluxes of the six populations with e_y != 0, wall faces
 // included
-template <typename T, int MODEL, int WALLS, bool P2, class G>
+template <typename T, int MODEL, int WALLS, bool UNIT, class G>
 __device__ inline void yface64(const Consts &C, const T (*s_bp)[G::R1N], const T (*s_mac)[G::R1N], const T *s_nee,
                                int t, int j0, int ny, T *yh, T *yf)
 {
@@ -173,7 +173,7 @@ __device__ inline void yface64(const Consts &C, const T (*s_bp)[G::R1N], const T
 #pragma unroll
     for (int q = 0; q < 2 * Q9; ++q) {
         const T *b = s_bp[q];
-        T v = cd_face_value<T, P2>(C, 1, q % Q9, b[ko], b[kn], b[kn + 1], b[kn - 1], b[ko + 1], b[ko - 1]);
+        T v = cd_face_value<T, UNIT>(C, 1, q % Q9, b[ko], b[kn], b[kn + 1], b[kn - 1], b[ko + 1], b[ko - 1]);
         if (q < Q9) hb[q] = v; else fb[q - Q9] = v;
     }
     FaceMacro<T> fm;
@@ -183,7 +183,7 @@ __device__ inline void yface64(const Consts &C, const T (*s_bp)[G::R1N], const T
     fm.ry = (T)0.5 * (s_mac[3][ko] + s_mac[3][kn]);
     fm.Fx = (T)0.5 * (s_mac[4][ko] + s_mac[4][kn]);
     fm.Fy = (T)0.5 * (s_mac[5][ko] + s_mac[5][kn]);
-    face_macros_ac<T, MODEL>(C, hb, fb, fm);
+    face_macros_ac<T, MODEL, UNIT>(C, hb, fb, fm);
     T ah = (T)2.0 * fm.tau / ((T)2.0 * fm.tau + hdt);
     T bh = (T)1.0 - ah;
     T ch = fm.tau * bh;
@@ -193,7 +193,7 @@ __device__ inline void yface64(const Consts &C, const T (*s_bp)[G::R1N], const T
 #pragma unroll
     for (int z = 0; z < 6; ++z) {
         const int q = MY[z];
-        face_bgk<T, MODEL>(C, fm, q, hb[q], fb[q], ah, bh, ch, vh[q], vf[q]);
+        face_bgk<T, MODEL, UNIT>(C, fm, q, hb[q], fb[q], ah, bh, ch, vh[q], vf[q]);
     }
     const int gj = j0 + fj;
     if (WALLS == 2 && (gj == 0 || gj == ny)) {
@@ -217,13 +217,13 @@ __device__ inline void yface64(const Consts &C, const T (*s_bp)[G::R1N], const T
 #pragma unroll
     for (int z = 0; z < 6; ++z) {
         const int q = MY[z];
-        const T xin = (T)C.dx * (T)C.xi[1][q];
+        const T xin = (T)C.dx * xi_of<UNIT, T>(C, 1, q);
         yh[z] = vh[q] * xin;
         yf[z] = vf[q] * xin;
     }
 }
 
-template <typename T, int MODEL, int WALLS, bool P2, class G>
+template <typename T, int MODEL, int WALLS, bool UNIT, class G>
 __global__ void __launch_bounds__(NTHREADS, 2) k_fused_step(Consts C, Planes<T> in, Planes<T> out, int64_t N, int ntx,
                                                          int ntiles)
 {
@@ -302,9 +302,9 @@ __global__ void __launch_bounds__(NTHREADS, 2) k_fused_step(Consts C, Planes<T> 
         CellMacro<T> m;
         AcSource<T> s;
         T mu;
-        derive_cell<T, MODEL>(C, ht, ft, pl, lc, ld, ppu, ppv, pc, pd, m, s, mu);
+        derive_cell<T, MODEL, UNIT>(C, ht, ft, pl, lc, ld, ppu, ppv, pc, pd, m, s, mu);
         T hb[Q9], fb[Q9];
-        cell_barplus<T, MODEL>(C, m, ht, ft, hb, fb);
+        cell_barplus<T, MODEL, UNIT>(C, m, ht, ft, hb, fb);
 #pragma unroll
         for (int q = 0; q < Q9; ++q) {
             s_bp[q][k] = hb[q];
@@ -318,7 +318,7 @@ __global__ void __launch_bounds__(NTHREADS, 2) k_fused_step(Consts C, Planes<T> 
             // expressions: same macros, same tilde populations)
             T *v = s_nee + ((gj == 0 ? 0 : 1) * TX + (rx - 1)) * 2 * Q9;
             const T xc = ((T)gi + (T)0.5) * (T)C.dx;
-            nee_face<T, MODEL>(C, m, ht, ft, xc, (T)(gj == 0 ? 0 : ny) * (T)C.dx, xc,
+            nee_face<T, MODEL, UNIT>(C, m, ht, ft, xc, (T)(gj == 0 ? 0 : ny) * (T)C.dx, xc,
                                ((T)gj + (T)0.5) * (T)C.dx, v, v + Q9);
         }
         if (interior && (HCZ || WALLS)) out.phl[n] = m.phi;
@@ -360,9 +360,9 @@ __global__ void __launch_bounds__(NTHREADS, 2) k_fused_step(Consts C, Planes<T> 
     // keep the two face passes apart: interleaved, they hold both faces'
     // operands at once (past 256 VGPRs, one wave per SIMD)
     T xh[6], xf[6], yh[6], yf[6];
-    if (tid < NXF) xface64<T, MODEL, P2, G>(C, s_bp, s_mac, tid, xh, xf);
+    if (tid < NXF) xface64<T, MODEL, UNIT, G>(C, s_bp, s_mac, tid, xh, xf);
     __builtin_amdgcn_sched_barrier(0);
-    if (tid < NYF) yface64<T, MODEL, WALLS, P2, G>(C, s_bp, s_mac, s_nee, tid, j0, ny, yh, yf);
+    if (tid < NYF) yface64<T, MODEL, WALLS, UNIT, G>(C, s_bp, s_mac, s_nee, tid, j0, ny, yh, yf);
     // own bar-plus before the region is reused for the face fluxes
     const int ci = tid % TX, cj = tid / TX;
     const bool cell = tid < TX * TY;
@@ -543,7 +543,7 @@ Planes<T> planes_of(FusedState &F, int which)
     return P;
 }
 
-template <typename T, int MODEL, int WALLS, bool P2>
+template <typename T, int MODEL, int WALLS, bool UNIT>
 void launch_step(FusedState &F, const Consts &C, hipStream_t st)
 {
     using G = Tile64<16, 15>;
@@ -551,7 +551,7 @@ void launch_step(FusedState &F, const Consts &C, hipStream_t st)
     const int ntx = (C.nx + G::TX - 1) / G::TX, nty = (C.ny + G::TY - 1) / G::TY;
     const int ntiles = ntx * nty;
     const int nblocks = ((ntiles + 7) / 8) * 8;
-    hipLaunchKernelGGL((k_fused_step<T, MODEL, WALLS, P2, G>), dim3(nblocks), dim3(NTHREADS), 0, st, C, in, out,
+    hipLaunchKernelGGL((k_fused_step<T, MODEL, WALLS, UNIT, G>), dim3(nblocks), dim3(NTHREADS), 0, st, C, in, out,
                        (int64_t)F.ncell, ntx, ntiles);
     F.cur = 1 - F.cur;
 }
@@ -559,8 +559,9 @@ void launch_step(FusedState &F, const Consts &C, hipStream_t st)
 template <typename T, int MODEL, int WALLS>
 void launch_step(FusedState &F, const Consts &C, hipStream_t st)
 {
-    // dx a power of two: face gradients by its exact reciprocal
-    if (C.dx_pow2) launch_step<T, MODEL, WALLS, true>(F, C, st);
+    // MaSpan 1 and dx a power of two: constant lattice velocities and face
+    // gradients by the exact reciprocal of dx (the same results bit for bit)
+    if (C.unit) launch_step<T, MODEL, WALLS, true>(F, C, st);
     else launch_step<T, MODEL, WALLS, false>(F, C, st);
 }
 

@@ -23,7 +23,7 @@ __host__ __device__ constexpr int bb9(int q) { return q == 0 ? 0 : (q <= 4 ? q +
 struct Consts {
     int32_t nx, ny, model, flux, walls, nyf;
     int32_t nee;      // walls use non-equilibrium extrapolation (Wall_3_NEE) instead of bounce-back
-    int32_t dx_pow2;  // dx is a power of two: rdx, rtwodx are exact reciprocals
+    int32_t unit;     // dx a power of two and xi exactly the integer directions (MaSpan = 1)
     double dx, dt, hdt, rt, lambda0, aphi, bphi, beta, kappa, kp, taum, tau0;
     double phi_l, phi_v, nu_l, nu_v, gx, gy, rho_ref, pp_k, t_cs, pp_tau;
     double dtv;       // dt / volume                    (Cell_2D.cpp:182)
@@ -32,7 +32,7 @@ struct Consts {
     double rtdt;      // RT*dt                          (SourceM.cpp:51)
     double wdphi;     // wI*(PhiL-PhiV)                 (SourceM.cpp:17)
     double twodx;     // 2*dx, CD tangential span       (MeshCartesian.cpp:145,203)
-    double rdx, rtwodx;  // 1/dx, 1/(2*dx)
+    double rdx, rtwodx;  // 1/dx, 1/(2*dx): exact reciprocals when unit
     double ah_h, bh_h, ch_h, abp_h, bbp_h, cbp_h;      // constant tau_h factors
     double ah_pp, bh_pp, ch_pp;                         // PP face factors
     double xi[2][Q9];
@@ -61,14 +61,24 @@ struct CellMacro {
     T phi, rho, U, V, p, tau, px, py, rx, ry, Fx, Fy;
 };
 
+// Lattice velocity component d of population q.  UNIT (Consts::unit: MaSpan
+// exactly 1) gives the integer direction as a constant: the multiplications by
+// +-1 then fold away, which is exact, and those by 0 stay (IEEE does not let
+// the compiler drop them), so every result is the same bit for bit.
+template <bool UNIT, typename T>
+__device__ inline T xi_of(const Consts &C, int d, int q)
+{
+    return UNIT ? (T)(d == 0 ? ex9(q) : ey9(q)) : (T)C.xi[d][q];
+}
+
 // Equilibria and sources of population q of one cell for the Allen-Cahn
 // models (the expressions of cell_barplus below).
-template <typename T, int MODEL>
+template <typename T, int MODEL, bool UNIT = false>
 __device__ inline void cell_eq_so(const Consts &C, const CellMacro<T> &m, int q, T &heq, T &feq, T &hso, T &fso)
 {
     const T rt = (T)C.rt;
     const T uu = m.U * m.U + m.V * m.V;
-    const T ex = (T)C.xi[0][q], ey = (T)C.xi[1][q], w = (T)C.w[q];
+    const T ex = xi_of<UNIT, T>(C, 0, q), ey = xi_of<UNIT, T>(C, 1, q), w = (T)C.w[q];
     T u1 = (ex * m.U + ey * m.V) / rt;
     T G = u1 + (T)0.5 * u1 * u1 - uu * (T)C.lambda0;
     heq = w * ((T)1.0 + u1) * m.phi;
@@ -88,7 +98,7 @@ __device__ inline void cell_eq_so(const Consts &C, const CellMacro<T> &m, int q,
 // (HCZ Ark/Lib/D2Q9HCZ.cpp:54-92,244-257; AC-p Ark/Lib/D2Q9AC.cpp:69-105,331-344;
 //  PP Ark/Lib/D2Q9PP.cpp:51-62,137-145; Ark/Cpp/DUGKS2DSolver.cpp:379-401;
 //  factors Ark/Cpp/Mesh/Cell_2D.cpp:120-125).
-template <typename T, int MODEL>
+template <typename T, int MODEL, bool UNIT = false>
 __device__ inline void cell_barplus(const Consts &C, const CellMacro<T> &m, const T *ht, const T *ft,
                                     T *hbp, T *fbp)
 {
@@ -99,7 +109,7 @@ __device__ inline void cell_barplus(const Consts &C, const CellMacro<T> &m, cons
     T uu = m.U * m.U + m.V * m.V;
 #pragma unroll
     for (int q = 0; q < Q9; ++q) {
-        const T ex = (T)C.xi[0][q], ey = (T)C.xi[1][q], w = (T)C.w[q];
+        const T ex = xi_of<UNIT, T>(C, 0, q), ey = xi_of<UNIT, T>(C, 1, q), w = (T)C.w[q];
         T feq, fso;
         T u1 = (ex * m.U + ey * m.V) / rt;
         if (MODEL == MPD_MODEL_PP_D2Q9) {
@@ -129,16 +139,16 @@ __device__ inline void cell_barplus(const Consts &C, const CellMacro<T> &m, cons
 // CD face value (CD_Interior_phi_Bh, Ark/Cpp/DUGKS2DSolver.cpp:604-687) on the
 // structured face with owner o (low side) and neighbour n (high side):
 //   normal derivative (n - o)/dx, tangential (0.5*(neigh pair) + 0.5*(owner pair))/(2dx).
-template <typename T, bool P2>
+template <typename T, bool UNIT = false>
 __device__ inline T cd_face_value(const Consts &C, int dir, int q, T bo, T bn, T tn_hi, T tn_lo,
                                   T to_hi, T to_lo)
 {
-    // P2 (dx a power of two): x * 2^-k and x / 2^k are the same real number,
-    // so the correctly rounded product equals the quotient bit for bit
+    // UNIT (dx a power of two): x * 2^-k and x / 2^k are the same real
+    // number, so the correctly rounded product equals the quotient bit for bit
     const T tsum = (T)0.5 * (tn_hi - tn_lo) + (T)0.5 * (to_hi - to_lo);
-    const T dn = P2 ? (bn - bo) * (T)C.rdx : (bn - bo) / (T)C.dx;
-    const T dt = P2 ? tsum * (T)C.rtwodx : tsum / (T)C.twodx;
-    const T ex = (T)C.xi[0][q], ey = (T)C.xi[1][q];
+    const T dn = UNIT ? (bn - bo) * (T)C.rdx : (bn - bo) / (T)C.dx;
+    const T dt = UNIT ? tsum * (T)C.rtwodx : tsum / (T)C.twodx;
+    const T ex = xi_of<UNIT, T>(C, 0, q), ey = xi_of<UNIT, T>(C, 1, q);
     if (dir == 0) return (T)0.5 * (bo + bn) - (T)C.hdt * (dn * ex + dt * ey);
     return (T)0.5 * (bo + bn) - (T)C.hdt * (dt * ex + dn * ey);
 }
@@ -151,7 +161,7 @@ struct FaceMacro {
 // Update_MacroVar_h for the Allen-Cahn models (HCZ Ark/Lib/D2Q9HCZ.cpp:183-242,
 // AC-p Ark/Lib/D2Q9AC.cpp:232-329); the six averaged sources come in already
 // averaged.  Moment sums in sequential order (Ark/Lib/GaussHermite.h:1-14).
-template <typename T, int MODEL>
+template <typename T, int MODEL, bool UNIT = false>
 __device__ inline void face_macros_ac(const Consts &C, const T *hb, const T *fb, FaceMacro<T> &fm)
 {
     const T rt = (T)C.rt, hdt = (T)C.hdt;
@@ -167,9 +177,9 @@ __device__ inline void face_macros_ac(const Consts &C, const T *hb, const T *fb,
     if (MODEL == MPD_MODEL_AC_HCZ) {
         T sx = 0, sy = 0;
 #pragma unroll
-        for (int q = 0; q < Q9; ++q) sx += fb[q] * (T)C.xi[0][q];
+        for (int q = 0; q < Q9; ++q) sx += fb[q] * xi_of<UNIT, T>(C, 0, q);
 #pragma unroll
-        for (int q = 0; q < Q9; ++q) sy += fb[q] * (T)C.xi[1][q];
+        for (int q = 0; q < Q9; ++q) sy += fb[q] * xi_of<UNIT, T>(C, 1, q);
         fm.U = sx / (fm.rho * rt);
         fm.V = sy / (fm.rho * rt);
         fm.U += (T)0.5 * hdt * fm.Fx / (fm.rho * rt);
@@ -192,12 +202,12 @@ __device__ inline void face_macros_ac(const Consts &C, const T *hb, const T *fb,
 // Update_phi_Eqh + Update_DVDF_Source_h + Update_phi_h for population q
 // (HCZ Ark/Lib/D2Q9HCZ.cpp:93-131,258-271; AC-p Ark/Lib/D2Q9AC.cpp:106-144,345-358;
 //  PP Ark/Lib/D2Q9PP.cpp:63-74,146-154; Ark/Cpp/DUGKS2DSolver.cpp:402-423).
-template <typename T, int MODEL>
+template <typename T, int MODEL, bool UNIT = false>
 __device__ inline void face_bgk(const Consts &C, const FaceMacro<T> &fm, int q, T hb, T fb, T ah, T bh,
                                 T ch, T &hout, T &fout)
 {
     const T rt = (T)C.rt;
-    const T ex = (T)C.xi[0][q], ey = (T)C.xi[1][q], w = (T)C.w[q];
+    const T ex = xi_of<UNIT, T>(C, 0, q), ey = xi_of<UNIT, T>(C, 1, q), w = (T)C.w[q];
     T uu = fm.U * fm.U + fm.V * fm.V;
     T u1 = (ex * fm.U + ey * fm.V) / rt;
     T feq, fso;
@@ -233,7 +243,7 @@ __device__ inline void face_bgk(const Consts &C, const FaceMacro<T> &fm, int q, 
 // = 0 (the reference never sets the face velocity otherwise) -- and the
 // cell's non-equilibrium part 2 tau/(2 tau + dt) (x~ - x_eq + dt/2 x_so) is
 // added.  Bounce-back follows, in the caller.
-template <typename T, int MODEL>
+template <typename T, int MODEL, bool UNIT = false>
 __device__ inline void nee_face(const Consts &C, const CellMacro<T> &cm, const T *ht, const T *ft, T xf, T yf,
                                 T xc, T yc, T *hh, T *ff)
 {
@@ -244,8 +254,8 @@ __device__ inline void nee_face(const Consts &C, const CellMacro<T> &cm, const T
     const T fneq = (T)2.0 * cm.tau / ((T)2.0 * cm.tau + (T)C.dt);
     for (int q = 0; q < Q9; ++q) {
         T heq, feq, hso, fso;
-        cell_eq_so<T, MODEL>(C, cm, q, heq, feq, hso, fso);
-        const T ex = (T)C.xi[0][q], ey = (T)C.xi[1][q], w = (T)C.w[q];
+        cell_eq_so<T, MODEL, UNIT>(C, cm, q, heq, feq, hso, fso);
+        const T ex = xi_of<UNIT, T>(C, 0, q), ey = xi_of<UNIT, T>(C, 1, q), w = (T)C.w[q];
         const T u1 = (ex * Uf + ey * Vf) / (T)C.rt;
         const T G = u1 + (T)0.5 * u1 * u1 - uu * (T)C.lambda0;
         const T heqf = w * phf * ((T)1.0 + u1);
@@ -259,7 +269,7 @@ __device__ inline void nee_face(const Consts &C, const CellMacro<T> &cm, const T
 // Update_MacroVar for the Allen-Cahn models (HCZ Ark/Lib/D2Q9HCZ.cpp:133-182,
 // AC-p Ark/Lib/D2Q9AC.cpp:146-231).  Fx/Fy/rx/ry are the sources of the
 // previous MacroSource.  Writes phi, rho, U, V, p, mu, tau.
-template <typename T, int MODEL>
+template <typename T, int MODEL, bool UNIT = false>
 __device__ inline void cell_macrovar_ac(const Consts &C, const T *ht, const T *ft, T Fx, T Fy, T rx, T ry,
                                         T &phi, T &rho, T &U, T &V, T &p, T &mu, T &tau)
 {
@@ -275,9 +285,9 @@ __device__ inline void cell_macrovar_ac(const Consts &C, const T *ht, const T *f
     if (MODEL == MPD_MODEL_AC_HCZ) {
         T sx = 0, sy = 0;
 #pragma unroll
-        for (int q = 0; q < Q9; ++q) sx += ft[q] * (T)C.xi[0][q];
+        for (int q = 0; q < Q9; ++q) sx += ft[q] * xi_of<UNIT, T>(C, 0, q);
 #pragma unroll
-        for (int q = 0; q < Q9; ++q) sy += ft[q] * (T)C.xi[1][q];
+        for (int q = 0; q < Q9; ++q) sy += ft[q] * xi_of<UNIT, T>(C, 1, q);
         U = sx / (rho * rt);
         V = sy / (rho * rt);
         U += hdt * Fx / (rho * rt);
