@@ -35,7 +35,7 @@ __device__ inline double cellbp(const DevState &S, const Consts &C, const double
 }
 
 // phase 1: equilibria, sources and bar-plus of every cell
-template <int MODEL>
+template <int MODEL, bool UNIT>
 __global__ void __launch_bounds__(256) k_phase1(DevState S, Consts C)
 {
     size_t n = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -55,7 +55,7 @@ __global__ void __launch_bounds__(256) k_phase1(DevState S, Consts C)
         ft[q] = S.f[q * N + n];
         ht[q] = MODEL == MPD_MODEL_PP_D2Q9 ? 0.0 : S.h[q * N + n];
     }
-    cell_barplus<double, MODEL>(C, cm, ht, ft, hb, fb);
+    cell_barplus<double, MODEL, UNIT>(C, cm, ht, ft, hb, fb);
 #pragma unroll
     for (int q = 0; q < Q9; ++q) {
         S.fbp[q * N + n] = fb[q];
@@ -108,7 +108,7 @@ __global__ void __launch_bounds__(256) k_grad(DevState S, Consts C, const double
 }
 
 // f-bar at a face (CD: Ark/Cpp/DUGKS2DSolver.cpp:604-687; UW: :560-603,688-706)
-template <int FLUX>
+template <int FLUX, bool UNIT>
 __device__ inline double face_value(const DevState &S, const Consts &C, const double *bp, const double *gbp,
                                     const double *bpx, const double *bpy, int dir, int q, int oi, int oj,
                                     int ni, int nj, double xf, double yf)
@@ -116,16 +116,16 @@ __device__ inline double face_value(const DevState &S, const Consts &C, const do
     double bo = cellbp(S, C, bp, gbp, q, oi, oj), bn = cellbp(S, C, bp, gbp, q, ni, nj);
     if (FLUX == MPD_FLUX_CD) {
         if (dir == 0)
-            return cd_face_value<double, false>(C, 0, q, bo, bn, cellbp(S, C, bp, gbp, q, ni, nj + 1),
+            return cd_face_value<double, UNIT>(C, 0, q, bo, bn, cellbp(S, C, bp, gbp, q, ni, nj + 1),
                                          cellbp(S, C, bp, gbp, q, ni, nj - 1),
                                          cellbp(S, C, bp, gbp, q, oi, oj + 1),
                                          cellbp(S, C, bp, gbp, q, oi, oj - 1));
-        return cd_face_value<double, false>(C, 1, q, bo, bn, cellbp(S, C, bp, gbp, q, ni + 1, nj),
+        return cd_face_value<double, UNIT>(C, 1, q, bo, bn, cellbp(S, C, bp, gbp, q, ni + 1, nj),
                                      cellbp(S, C, bp, gbp, q, ni - 1, nj),
                                      cellbp(S, C, bp, gbp, q, oi + 1, oj),
                                      cellbp(S, C, bp, gbp, q, oi - 1, oj));
     }
-    double ex = C.xi[0][q], ey = C.xi[1][q];
+    double ex = xi_of<UNIT, double>(C, 0, q), ey = xi_of<UNIT, double>(C, 1, q);
     double xin = dir == 0 ? ex : ey;
     double xo = (oi + 0.5) * C.dx, yo = (oj + 0.5) * C.dx, xn = (ni + 0.5) * C.dx, yn = (nj + 0.5) * C.dx;
     size_t ko = (size_t)q * S.N + (size_t)wrapi(oj, C.ny) * C.nx + wrapi(oi, C.nx);
@@ -146,7 +146,7 @@ __device__ inline void face_geom(const Consts &C, int dir, int i, int j, int &oi
 
 // phase 4 + 6 for the Allen-Cahn models: Flux_2D on every face, Wall_3_BB on
 // wall faces, times xi.n dS (Ark/Cpp/DUGKS2DSolver.cpp:219-273)
-template <int MODEL, int FLUX>
+template <int MODEL, int FLUX, bool UNIT>
 __global__ void __launch_bounds__(256) k_faces_ac(DevState S, Consts C)
 {
     size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -161,8 +161,8 @@ __global__ void __launch_bounds__(256) k_faces_ac(DevState S, Consts C)
     double hb[Q9], fb[Q9];
 #pragma unroll
     for (int q = 0; q < Q9; ++q) {
-        hb[q] = face_value<FLUX>(S, C, S.hbp, S.ghbp, S.hbpx, S.hbpy, dir, q, oi, oj, ni, nj, xf, yf);
-        fb[q] = face_value<FLUX>(S, C, S.fbp, S.gfbp, S.fbpx, S.fbpy, dir, q, oi, oj, ni, nj, xf, yf);
+        hb[q] = face_value<FLUX, UNIT>(S, C, S.hbp, S.ghbp, S.hbpx, S.hbpy, dir, q, oi, oj, ni, nj, xf, yf);
+        fb[q] = face_value<FLUX, UNIT>(S, C, S.fbp, S.gfbp, S.fbpx, S.fbpy, dir, q, oi, oj, ni, nj, xf, yf);
     }
     FaceMacro<double> fm;
     fm.px = 0.5 * (cellm(S, C, MPD_M_PHIX, oi, oj) + cellm(S, C, MPD_M_PHIX, ni, nj));
@@ -171,13 +171,13 @@ __global__ void __launch_bounds__(256) k_faces_ac(DevState S, Consts C)
     fm.ry = 0.5 * (cellm(S, C, MPD_M_RHOY, oi, oj) + cellm(S, C, MPD_M_RHOY, ni, nj));
     fm.Fx = 0.5 * (cellm(S, C, MPD_M_FX, oi, oj) + cellm(S, C, MPD_M_FX, ni, nj));
     fm.Fy = 0.5 * (cellm(S, C, MPD_M_FY, oi, oj) + cellm(S, C, MPD_M_FY, ni, nj));
-    face_macros_ac<double, MODEL>(C, hb, fb, fm);
+    face_macros_ac<double, MODEL, UNIT>(C, hb, fb, fm);
     double ah = 2.0 * fm.tau / (2.0 * fm.tau + C.hdt);
     double bh = 1.0 - ah;
     double ch = fm.tau * bh;
     double hh[Q9], ff[Q9];
 #pragma unroll
-    for (int q = 0; q < Q9; ++q) face_bgk<double, MODEL>(C, fm, q, hb[q], fb[q], ah, bh, ch, hh[q], ff[q]);
+    for (int q = 0; q < Q9; ++q) face_bgk<double, MODEL, UNIT>(C, fm, q, hb[q], fb[q], ah, bh, ch, hh[q], ff[q]);
     if (C.walls && dir == 1 && (j == 0 || j == C.ny) && C.nee) {
         // Wall_3_NEE (Ark/Cpp/Boundary.cpp:140-184) from the fluid cell's
         // macros and tilde populations; bounce-back follows
@@ -194,7 +194,7 @@ __global__ void __launch_bounds__(256) k_faces_ac(DevState S, Consts C)
         double ht[Q9], ft[Q9];
 #pragma unroll
         for (int q = 0; q < Q9; ++q) { ht[q] = S.h[q * N + n]; ft[q] = S.f[q * N + n]; }
-        nee_face<double, MODEL>(C, cm, ht, ft, xf, yf, (i + 0.5) * C.dx, (jc + 0.5) * C.dx, hh, ff);
+        nee_face<double, MODEL, UNIT>(C, cm, ht, ft, xf, yf, (i + 0.5) * C.dx, (jc + 0.5) * C.dx, hh, ff);
     }
     if (C.walls && dir == 1 && (j == 0 || j == C.ny)) {
         // Wall_3_BB (Ark/Cpp/Boundary.cpp:118-139)
@@ -220,7 +220,7 @@ __global__ void __launch_bounds__(256) k_faces_ac(DevState S, Consts C)
 // Pseudopotential faces, pass 1: face macros and Psi of every face
 // (Update_MacroVar_h, Ark/Lib/D2Q9PP.cpp:110-136; Update_PseudoPsi(face),
 // Ark/Cpp/PseudoP.cpp:23-30).  All Psi are written before any face force reads them.
-template <int FLUX>
+template <int FLUX, bool UNIT>
 __global__ void __launch_bounds__(256) k_pp_face_psi(DevState S, Consts C)
 {
     size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -234,13 +234,13 @@ __global__ void __launch_bounds__(256) k_pp_face_psi(DevState S, Consts C)
     double s = 0, sx = 0, sy = 0, fb[Q9];
 #pragma unroll
     for (int q = 0; q < Q9; ++q)
-        fb[q] = face_value<FLUX>(S, C, S.fbp, S.gfbp, S.fbpx, S.fbpy, dir, q, oi, oj, ni, nj, xf, yf);
+        fb[q] = face_value<FLUX, UNIT>(S, C, S.fbp, S.gfbp, S.fbpx, S.fbpy, dir, q, oi, oj, ni, nj, xf, yf);
 #pragma unroll
     for (int q = 0; q < Q9; ++q) s += fb[q];
 #pragma unroll
-    for (int q = 0; q < Q9; ++q) sx += fb[q] * C.xi[0][q];
+    for (int q = 0; q < Q9; ++q) sx += fb[q] * xi_of<UNIT, double>(C, 0, q);
 #pragma unroll
-    for (int q = 0; q < Q9; ++q) sy += fb[q] * C.xi[1][q];
+    for (int q = 0; q < Q9; ++q) sy += fb[q] * xi_of<UNIT, double>(C, 1, q);
     double rho = s;
     double U = sx / rho, V = sy / rho;
     U += 0.5 * C.hdt * S.ffx[t] / rho;
@@ -258,7 +258,7 @@ __global__ void __launch_bounds__(256) k_pp_face_psi(DevState S, Consts C)
 // Pseudopotential faces, pass 2: face force from the face-lattice gradient of
 // Psi (update_MQ_x/y(Face_2D*), Ark/Lib/GradSchemeBasic.h:181-208;
 // Update_PseudoForce(face), Ark/Cpp/PseudoP.cpp:39-46), then the BGK half step.
-template <int FLUX>
+template <int FLUX, bool UNIT>
 __global__ void __launch_bounds__(256) k_pp_face_flux(DevState S, Consts C)
 {
     size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -290,9 +290,9 @@ __global__ void __launch_bounds__(256) k_pp_face_flux(DevState S, Consts C)
     double *ffl = S.ffl + (dir ? (size_t)Q9 * S.N : 0);
 #pragma unroll
     for (int q = 0; q < Q9; ++q) {
-        double fb = face_value<FLUX>(S, C, S.fbp, S.gfbp, S.fbpx, S.fbpy, dir, q, oi, oj, ni, nj, xf, yf);
+        double fb = face_value<FLUX, UNIT>(S, C, S.fbp, S.gfbp, S.fbpx, S.fbpy, dir, q, oi, oj, ni, nj, xf, yf);
         double hh, ff;
-        face_bgk<double, MPD_MODEL_PP_D2Q9>(C, fm, q, 0.0, fb, C.ah_pp, C.bh_pp, C.ch_pp, hh, ff);
+        face_bgk<double, MPD_MODEL_PP_D2Q9, UNIT>(C, fm, q, 0.0, fb, C.ah_pp, C.bh_pp, C.ch_pp, hh, ff);
         double xin = C.dx * (dir == 0 ? C.xi[0][q] : C.xi[1][q]);
         ffl[(size_t)q * S.N + fidx] = ff * xin;
     }
@@ -300,7 +300,7 @@ __global__ void __launch_bounds__(256) k_pp_face_flux(DevState S, Consts C)
 
 // phase 8 + 9: Update_phi_T (Ark/Cpp/DUGKS2DSolver.cpp:748-792, faces E,N,W,S)
 // and Update_MacroVar of the cell.
-template <int MODEL>
+template <int MODEL, bool UNIT>
 __global__ void __launch_bounds__(256) k_update(DevState S, Consts C)
 {
     size_t n = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -341,9 +341,9 @@ __global__ void __launch_bounds__(256) k_update(DevState S, Consts C)
 #pragma unroll
         for (int q = 0; q < Q9; ++q) s += ft[q];
 #pragma unroll
-        for (int q = 0; q < Q9; ++q) sx += ft[q] * C.xi[0][q];
+        for (int q = 0; q < Q9; ++q) sx += ft[q] * xi_of<UNIT, double>(C, 0, q);
 #pragma unroll
-        for (int q = 0; q < Q9; ++q) sy += ft[q] * C.xi[1][q];
+        for (int q = 0; q < Q9; ++q) sy += ft[q] * xi_of<UNIT, double>(C, 1, q);
         double rho = s, U = sx / rho, V = sy / rho;
         U += C.hdt * m[MPD_M_FX * N + n] / rho;
         V += C.hdt * m[MPD_M_FY * N + n] / rho;
@@ -358,7 +358,7 @@ __global__ void __launch_bounds__(256) k_update(DevState S, Consts C)
         return;
     }
     double phi, rho, U, V, p, mu, tau;
-    cell_macrovar_ac<double, MODEL>(C, ht, ft, m[MPD_M_FX * N + n], m[MPD_M_FY * N + n],
+    cell_macrovar_ac<double, MODEL, UNIT>(C, ht, ft, m[MPD_M_FX * N + n], m[MPD_M_FY * N + n],
                                     m[MPD_M_RHOX * N + n], m[MPD_M_RHOY * N + n], phi, rho, U, V, p, mu, tau);
     m[MPD_M_PHI * N + n] = phi;
     m[MPD_M_RHO * N + n] = rho;
@@ -599,10 +599,10 @@ __global__ void __launch_bounds__(256) k_v16_update(DevState S, Consts C)
 
 static inline unsigned nblk(size_t n) { return (unsigned)((n + 255) / 256); }
 
-template <int MODEL, int FLUX>
+template <int MODEL, int FLUX, bool UNIT>
 static void phase_step_t(const DevState &S, const Consts &C, hipStream_t st)
 {
-    hipLaunchKernelGGL((k_phase1<MODEL>), dim3(nblk(S.N)), dim3(256), 0, st, S, C);
+    hipLaunchKernelGGL((k_phase1<MODEL, UNIT>), dim3(nblk(S.N)), dim3(256), 0, st, S, C);
     if (C.walls) hipLaunchKernelGGL(k_ghosts, dim3(nblk(2 * C.nx)), dim3(256), 0, st, S, C);
     if (FLUX == MPD_FLUX_UW) {
         if (MODEL != MPD_MODEL_PP_D2Q9)
@@ -610,13 +610,13 @@ static void phase_step_t(const DevState &S, const Consts &C, hipStream_t st)
         hipLaunchKernelGGL(k_grad, dim3(nblk(S.N)), dim3(256), 0, st, S, C, (const double *)S.fbp, S.fbpx, S.fbpy);
     }
     if (MODEL == MPD_MODEL_PP_D2Q9) {
-        hipLaunchKernelGGL((k_pp_face_psi<FLUX>), dim3(nblk(2 * S.N)), dim3(256), 0, st, S, C);
-        hipLaunchKernelGGL((k_pp_face_flux<FLUX>), dim3(nblk(2 * S.N)), dim3(256), 0, st, S, C);
+        hipLaunchKernelGGL((k_pp_face_psi<FLUX, UNIT>), dim3(nblk(2 * S.N)), dim3(256), 0, st, S, C);
+        hipLaunchKernelGGL((k_pp_face_flux<FLUX, UNIT>), dim3(nblk(2 * S.N)), dim3(256), 0, st, S, C);
     } else {
         size_t nf = S.N + (size_t)C.nyf * C.nx;
-        hipLaunchKernelGGL((k_faces_ac<MODEL, FLUX>), dim3(nblk(nf)), dim3(256), 0, st, S, C);
+        hipLaunchKernelGGL((k_faces_ac<MODEL, FLUX, UNIT>), dim3(nblk(nf)), dim3(256), 0, st, S, C);
     }
-    hipLaunchKernelGGL((k_update<MODEL>), dim3(nblk(S.N)), dim3(256), 0, st, S, C);
+    hipLaunchKernelGGL((k_update<MODEL, UNIT>), dim3(nblk(S.N)), dim3(256), 0, st, S, C);
     if (MODEL == MPD_MODEL_PP_D2Q9) hipLaunchKernelGGL(k_ppforce, dim3(nblk(S.N)), dim3(256), 0, st, S, C);
     else hipLaunchKernelGGL(k_source, dim3(nblk(S.N)), dim3(256), 0, st, S, C);
 }
@@ -630,21 +630,30 @@ static void phase_step_v16(const DevState &S, const Consts &C, hipStream_t st)
     hipLaunchKernelGGL(k_ppforce, dim3(nblk(S.N)), dim3(256), 0, st, S, C);
 }
 
+// UNIT (Consts::unit): the lattice velocities as constants and 1/dx exact;
+// the same results bit for bit as the generic instantiation
+template <bool UNIT>
+static void phase_step_m(const DevState &S, const Consts &C, hipStream_t st)
+{
+    switch (C.model * 2 + C.flux) {
+    case MPD_MODEL_AC_P * 2 + MPD_FLUX_CD: phase_step_t<MPD_MODEL_AC_P, MPD_FLUX_CD, UNIT>(S, C, st); break;
+    case MPD_MODEL_AC_P * 2 + MPD_FLUX_UW: phase_step_t<MPD_MODEL_AC_P, MPD_FLUX_UW, UNIT>(S, C, st); break;
+    case MPD_MODEL_AC_HCZ * 2 + MPD_FLUX_CD: phase_step_t<MPD_MODEL_AC_HCZ, MPD_FLUX_CD, UNIT>(S, C, st); break;
+    case MPD_MODEL_AC_HCZ * 2 + MPD_FLUX_UW: phase_step_t<MPD_MODEL_AC_HCZ, MPD_FLUX_UW, UNIT>(S, C, st); break;
+    case MPD_MODEL_PP_D2Q9 * 2 + MPD_FLUX_CD: phase_step_t<MPD_MODEL_PP_D2Q9, MPD_FLUX_CD, UNIT>(S, C, st); break;
+    case MPD_MODEL_PP_D2Q9 * 2 + MPD_FLUX_UW: phase_step_t<MPD_MODEL_PP_D2Q9, MPD_FLUX_UW, UNIT>(S, C, st); break;
+    default: break;
+    }
+}
+
 void phase_step(const DevState &S, const Consts &C, hipStream_t st)
 {
     if (C.model == MPD_MODEL_PP_D2V16) {
         phase_step_v16(S, C, st);
         return;
     }
-    switch (C.model * 2 + C.flux) {
-    case MPD_MODEL_AC_P * 2 + MPD_FLUX_CD: phase_step_t<MPD_MODEL_AC_P, MPD_FLUX_CD>(S, C, st); break;
-    case MPD_MODEL_AC_P * 2 + MPD_FLUX_UW: phase_step_t<MPD_MODEL_AC_P, MPD_FLUX_UW>(S, C, st); break;
-    case MPD_MODEL_AC_HCZ * 2 + MPD_FLUX_CD: phase_step_t<MPD_MODEL_AC_HCZ, MPD_FLUX_CD>(S, C, st); break;
-    case MPD_MODEL_AC_HCZ * 2 + MPD_FLUX_UW: phase_step_t<MPD_MODEL_AC_HCZ, MPD_FLUX_UW>(S, C, st); break;
-    case MPD_MODEL_PP_D2Q9 * 2 + MPD_FLUX_CD: phase_step_t<MPD_MODEL_PP_D2Q9, MPD_FLUX_CD>(S, C, st); break;
-    case MPD_MODEL_PP_D2Q9 * 2 + MPD_FLUX_UW: phase_step_t<MPD_MODEL_PP_D2Q9, MPD_FLUX_UW>(S, C, st); break;
-    default: break;
-    }
+    if (C.unit) phase_step_m<true>(S, C, st);
+    else phase_step_m<false>(S, C, st);
 }
 
 }  // namespace mpd
