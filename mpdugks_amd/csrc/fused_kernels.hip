@@ -223,6 +223,9 @@ __device__ inline void yface64(const Consts &C, const T (*s_bp)[G::R1N], const T
     }
 }
 
+// Two workgroups per CU (the LDS allows two): the generic (non-UNIT) AC-p
+// instantiations would take a few registers over 256 and fall to one; with the
+// bound they spill a few dwords instead.
 template <typename T, int MODEL, int WALLS, bool UNIT, class G>
 __global__ void __launch_bounds__(NTHREADS, 2) k_fused_step(Consts C, Planes<T> in, Planes<T> out, int64_t N, int ntx,
                                                          int ntiles)
