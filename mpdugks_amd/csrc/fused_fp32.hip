@@ -779,10 +779,15 @@ __device__ __forceinline__ void tile32(const K32 &k, int nx, int ny, const Fused
     // ---- stage B ----
     float base[17];
     cell_bp<MODEL, WALLS, G>(k, out, nx, ny, rx1, ry1, x1, s_big, s_phi, s_phl, base, inside, s_nee);
+    // 16x16 and 32x16: the waves with a second pass (ring cells here, faces in
+    // stage C) are the critical path of their stage; issue them first
+    constexpr bool PRIO = G::EXF > 0 || G::EYF > 0;
     if (tid < NR2) {
+        if (PRIO) __builtin_amdgcn_s_setprio(2);
         if (!PRELOAD_RING) cell_load<MODEL, WALLS>(in, nx, ny, small, i0, j0, rx2, ry2, x2);
         float dummy[17];
         cell_bp<MODEL, WALLS, G>(k, out, nx, ny, rx2, ry2, x2, s_big, s_phi, s_phl, dummy, false, s_nee);
+        if (PRIO) __builtin_amdgcn_s_setprio(0);
     }
     __syncthreads();
 
@@ -814,6 +819,7 @@ __device__ __forceinline__ void tile32(const K32 &k, int nx, int ny, const Fused
     constexpr int EXF = G::EXF, EYF = G::EYF, NE = G::NE, XE0 = G::XE0, YE0 = G::YE0;
     __shared__ float2 s_ext[EXF > 0 || EYF > 0 ? 2 * NFL * NE : 1];
     int et = -1;
+    if (PRIO && tid >= XE0) __builtin_amdgcn_s_setprio(2);  // the last two waves
     if (EXF > 0 && tid >= XE0 && tid < XE0 + EXF) {
         et = tid - XE0;
         float h[6], f[6], m = 0.f;
@@ -836,6 +842,7 @@ __device__ __forceinline__ void tile32(const K32 &k, int nx, int ny, const Fused
     // interleave the two (past 128 VGPRs, below 4 waves per SIMD); keep them apart
     if (NXF >= NT) __builtin_amdgcn_sched_barrier(0);
     if (tid < NYF) yface<MODEL, WALLS, G>(k, s_big, s_nee, tid, j0, ny, yh, yf, ym);
+    if (PRIO) __builtin_amdgcn_s_setprio(0);
     __syncthreads();
     float2 *s_fx = s_big, *s_fy = s_big + NFL * FS;
     if (tid < NXF) {
