@@ -30,6 +30,8 @@ CASES = [
     # and one that is not (face gradients by division)
     ("hcz_walls_48x40_dx0.5", dict(model="ac_hcz", nx=48, ny=40, walls=True, radius=6.0, dx=0.5, dt=0.25)),
     ("acp_periodic_40x36_dx0.75", dict(model="ac_p", nx=40, ny=36, radius=9.0, dx=0.75, dt=0.375)),
+    ("acp_nee_48x40_dx0.5", dict(model="ac_p", nx=48, ny=40, walls=True, nee=True, radius=6.0, dx=0.5, dt=0.25)),
+    ("hcz_periodic_64x48_dx2", dict(model="ac_hcz", nx=64, ny=48, radius=30.0, dx=2.0, dt=1.0)),
 ]
 
 
