@@ -233,7 +233,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--nx", type=int, default=256)
     ap.add_argument("--ny", type=int, default=1024)
-    ap.add_argument("--only", choices=["fp32_fused", "fp64_fused", "fp64_plain", "pp"], default=None,
+    ap.add_argument("--only", choices=["fp32_fused", "fp64_fused", "fp64_plain", "pp", "ensemble"], default=None,
                     help="run one variant only (for profiling); prints its numbers, not the headline line")
     ap.add_argument("--ensemble-cases", type=int, default=256)
     ap.add_argument("--ensemble-steps", type=int, default=400)
@@ -259,6 +259,9 @@ def main():
         copy_calibration(args.copy_mb)
     if args.only == "pp":
         print(json.dumps(pp_bench(steps=args.pp_steps)))
+        return
+    if args.only == "ensemble":
+        print(json.dumps(ensemble_bench(args.ensemble_cases, args.ensemble_steps)))
         return
     results = {}
     variants = (("fp32_fused", abi.MPD_FP32_BULK, abi.MPD_PATH_FUSED),
