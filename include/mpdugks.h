@@ -148,6 +148,27 @@ int mpd_destroy(mpd_handle *h);
 /* Select the kernel path (mpd_path); valid before the first mpd_step. */
 int mpd_set_path(mpd_handle *h, int32_t path);
 
+/* Visiting order of the pseudopotential face force.  The reference computes
+ * each face's Psi and reads its neighbour faces' Psi inside one OpenMP loop
+ * (Ark/Cpp/DUGKS2DSolver.cpp:219-230, 733-746; Ark/Lib/GradSchemeBasic.h:181-208),
+ * so its result depends on the visiting order; with one thread a neighbour
+ * visited earlier contributes its new Psi and one visited later the Psi of the
+ * previous step.  MPD_FACE_ORDER_JACOBI (default; every path): all face Psi
+ * first, then all forces.  MPD_FACE_ORDER_REFERENCE (FP64 pseudopotential D2Q9,
+ * CD, periodic; phase path): the reference's one-thread order for a mesh
+ * whose faces are numbered as mpdugks_amd.mesh.write_cas writes them
+ * (interior x-faces by rows, interior y-faces by rows, then the periodic zones
+ * left, right, bottom, top) -- bit for bit the reference's OMP_NUM_THREADS=1 run. */
+typedef enum mpd_face_order { MPD_FACE_ORDER_JACOBI = 0, MPD_FACE_ORDER_REFERENCE = 1 } mpd_face_order;
+int mpd_set_face_order(mpd_handle *h, int32_t order);
+
+/* Face Psi of the last step (x-faces [ncell] then y-faces [ncell], layout of
+ * the face forces), the state the reference order reads for faces not yet
+ * visited (MacroQuantity::Psi of each Face_2D).  mpd_upload_state sets it to
+ * 0, as the reference's initialisers leave it.  FP64 pseudopotential D2Q9. */
+int mpd_upload_face_psi(mpd_handle *h, const double *psi);
+int mpd_download_face_psi(mpd_handle *h, double *psi);
+
 /* Tile shape of the one-launch step kernels (cells per workgroup): the
  * FP32-bulk Allen-Cahn kernel has all three, the FP32-bulk pseudopotential
  * kernel 16x15 and 32x16, the FP64 Allen-Cahn kernel 16x15 (other shapes are
@@ -180,7 +201,12 @@ int mpd_download_state(mpd_handle *h, double *tilde_h, double *tilde_f, double *
                        double *face_fx, double *face_fy);
 
 /* Advance nsteps time steps (the `while` body, DUGKS2DSolver.cpp:164-328).
- * Asynchronous on the handle's stream. */
+ * Queues the step kernels on the handle's stream and returns without waiting
+ * (except with per-launch timing on, and every 256 calls when nothing else
+ * synchronises, to read back the timing events); mpd_synchronize,
+ * mpd_download_state and mpd_reductions wait for them.  The first call after
+ * an upload on an FP64 or Allen-Cahn fused handle also allocates the phase
+ * path's scratch. */
 int mpd_step(mpd_handle *h, int64_t nsteps);
 
 /* Global sums (SumRho / Residual, DUGKS2DSolver.cpp:793-879).  Synchronises. */
@@ -204,11 +230,14 @@ int mpd_kernel_timing(mpd_handle *h, double *avg_ms, int64_t *launches);
 
 /* ---- batched ensembles ----
  * A batch advances many FP32-bulk cases (Allen-Cahn, CD; same model and y
- * boundary -- periodic, BB or NEE -- any sizes and constants) with one kernel launch per step: every
- * workgroup takes one 16x15 tile of one case from a work list, and the step
- * pair is captured once in a HIP graph and replayed.  Members keep their own
- * state; mpd_download_state / mpd_reductions on a member synchronise the batch.
- * No counterpart in the reference, which runs one case per process. */
+ * boundary -- periodic, BB or NEE -- any sizes and constants) with one kernel
+ * launch per step: every workgroup takes one tile (16x15, 16x16 or 32x16
+ * cells, mpd_batch_set_tile) of one case from a work list, and the step pair
+ * is captured once in a HIP graph and replayed.  Members keep their own state;
+ * mpd_download_state / mpd_reductions on a member synchronise the batch; a
+ * member cannot be stepped, uploaded to, re-initialised or switched to another
+ * path on its own (MPD_ERR_STATE).  No counterpart in the reference, which runs
+ * one case per process. */
 typedef struct mpd_batch mpd_batch;
 int mpd_batch_create(mpd_batch **out);
 int mpd_batch_destroy(mpd_batch *b);

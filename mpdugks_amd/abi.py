@@ -49,6 +49,7 @@ def has_h(model):
     return model in (MPD_MODEL_AC_P, MPD_MODEL_AC_HCZ)
 
 MPD_PATH_AUTO, MPD_PATH_PHASES, MPD_PATH_FUSED = 0, 1, 2
+MPD_FACE_ORDER_JACOBI, MPD_FACE_ORDER_REFERENCE = 0, 1
 MPD_TILE_AUTO, MPD_TILE_16X15, MPD_TILE_16X16, MPD_TILE_32X16 = 0, 1, 2, 3
 TILE_NAMES = {MPD_TILE_16X15: "16x15", MPD_TILE_16X16: "16x16", MPD_TILE_32X16: "32x16"}
 

@@ -7,6 +7,7 @@
 #include <cstdio>
 #include <cstring>
 #include <string>
+#include <utility>
 #include <vector>
 
 #include "../../include/mpdugks.h"
@@ -63,6 +64,10 @@ struct mpd_handle {
     int64_t launches = 0;
     bool per_launch = false;
     std::vector<hipEvent_t> evs;  // per-launch timing pairs
+    // mpd_step brackets its launches with an event pair and returns; the pairs
+    // are read (and recycled) when the caller synchronises or asks for timing
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> pending;
+    std::vector<hipEvent_t> ev_free;
     double *d_part = nullptr;
     int red_blocks = 0;
     mpd_batch *batch = nullptr;  // set while the handle is a member of a batch
@@ -255,6 +260,15 @@ int check_cfg(const mpd_config *c)
         return fail(MPD_ERR_INVALID, "no FP32-bulk kernel for this model / flux / boundary (FP32-bulk: Allen-Cahn "
                                      "with CD faces and periodic or bounce-back y; pseudopotential D2Q9 with CD "
                                      "faces, periodic)");
+    if (c->precision == MPD_FP32_BULK && !is_pp(*c)) {
+        // the FP32 step kernel addresses each state set through one 32-bit buffer
+        // resource (all float planes) and the FP64 phase field through another
+        const int64_t n = (int64_t)c->nx * c->ny, lim = (int64_t)1 << 32;
+        if (4 * n * fused_nplanes(*c) >= lim || 8 * n >= lim)
+            return fail(MPD_ERR_INVALID, "mesh %dx%d too large for the FP32-bulk Allen-Cahn kernel (%d float "
+                                         "planes of 4 B per cell must span < 4 GiB: at most %lld cells)",
+                        c->nx, c->ny, fused_nplanes(*c), (long long)((lim - 1) / (4 * fused_nplanes(*c))));
+    }
     if (!(c->dx > 0) || !(c->dt > 0) || !(c->rt > 0)) return fail(MPD_ERR_INVALID, "dx, dt, rt must be > 0");
     if (c->phi_l == c->phi_v || !(c->width > 0)) return fail(MPD_ERR_INVALID, "phi_l == phi_v or width <= 0");
     return 0;
@@ -330,7 +344,8 @@ int mpd_create(const mpd_config *cfg, mpd_handle **out)
     bool pp = is_pp(*cfg), ac = !pp, uw = cfg->flux == MPD_FLUX_UW;
     if ((r = dalloc(h, &S.h, ac ? QN : 0)) || (r = dalloc(h, &S.f, QN)) ||
         (r = dalloc(h, &S.m, (size_t)MPD_M_COUNT * N)) || (r = dalloc(h, &S.ffx, pp ? 2 * N : 0)) ||
-        (r = dalloc(h, &S.ffy, pp ? 2 * N : 0)) || (r = dalloc(h, &S.rho1k, N))) {
+        (r = dalloc(h, &S.ffy, pp ? 2 * N : 0)) || (r = dalloc(h, &S.fpsi_old, pp ? 2 * N : 0)) ||
+        (r = dalloc(h, &S.rho1k, N))) {
         mpd_destroy(h);
         return r;
     }
@@ -354,6 +369,11 @@ int mpd_destroy(mpd_handle *h)
     if (h->stream) (void)hipStreamSynchronize(h->stream);
     for (void *p : h->allocs) (void)hipFree(p);
     for (auto e : h->evs) (void)hipEventDestroy(e);
+    for (auto &pr : h->pending) {
+        (void)hipEventDestroy(pr.first);
+        (void)hipEventDestroy(pr.second);
+    }
+    for (auto e : h->ev_free) (void)hipEventDestroy(e);
     fused_free(h->F);
     if (h->ev0) (void)hipEventDestroy(h->ev0);
     if (h->ev1) (void)hipEventDestroy(h->ev1);
@@ -365,12 +385,50 @@ int mpd_destroy(mpd_handle *h)
 int mpd_set_path(mpd_handle *h, int32_t path)
 {
     if (!h) return fail(MPD_ERR_INVALID, "null handle");
+    if (h->batch) return fail(MPD_ERR_STATE, "handle is a member of a batch");
     if (path < MPD_PATH_AUTO || path > MPD_PATH_FUSED) return fail(MPD_ERR_INVALID, "unknown path %d", path);
     if (path == MPD_PATH_FUSED && !fused_supported(h->cfg))
         return fail(MPD_ERR_INVALID, "no fused kernel for this configuration");
     if (h->cfg.precision == MPD_FP32_BULK && path == MPD_PATH_PHASES && !is_pp(h->cfg))
         return fail(MPD_ERR_INVALID, "the phase path is FP64 only (and FP32 for the pseudopotential model)");
     h->path = path;
+    return 0;
+}
+
+int mpd_set_face_order(mpd_handle *h, int32_t order)
+{
+    if (!h) return fail(MPD_ERR_INVALID, "null handle");
+    if (h->batch) return fail(MPD_ERR_STATE, "handle is a member of a batch");
+    if (order != MPD_FACE_ORDER_JACOBI && order != MPD_FACE_ORDER_REFERENCE)
+        return fail(MPD_ERR_INVALID, "unknown face order %d", order);
+    if (order == MPD_FACE_ORDER_REFERENCE &&
+        !(h->cfg.model == MPD_MODEL_PP_D2Q9 && h->cfg.precision == MPD_FP64 && h->cfg.flux == MPD_FLUX_CD &&
+          h->cfg.bc_y == MPD_BC_PERIODIC))
+        return fail(MPD_ERR_INVALID, "the reference face order is a mode of the FP64 pseudopotential D2Q9 step "
+                                     "(CD faces, periodic mesh)");
+    h->C.face_ref = order == MPD_FACE_ORDER_REFERENCE;
+    return 0;
+}
+
+int mpd_upload_face_psi(mpd_handle *h, const double *psi)
+{
+    if (!h || !psi) return fail(MPD_ERR_INVALID, "null argument");
+    if (h->cfg.model != MPD_MODEL_PP_D2Q9 || h->cfg.precision != MPD_FP64)
+        return fail(MPD_ERR_INVALID, "face Psi is state of the FP64 pseudopotential D2Q9 step");
+    HIPCHK(hipMemcpyAsync(h->S.fpsi_old, psi, 2 * (size_t)h->S.N * sizeof(double), hipMemcpyHostToDevice,
+                          h->stream));
+    HIPCHK(hipStreamSynchronize(h->stream));
+    return 0;
+}
+
+int mpd_download_face_psi(mpd_handle *h, double *psi)
+{
+    if (!h || !psi) return fail(MPD_ERR_INVALID, "null argument");
+    if (h->cfg.model != MPD_MODEL_PP_D2Q9 || h->cfg.precision != MPD_FP64)
+        return fail(MPD_ERR_INVALID, "face Psi is state of the FP64 pseudopotential D2Q9 step");
+    HIPCHK(hipMemcpyAsync(psi, h->S.fpsi_old, 2 * (size_t)h->S.N * sizeof(double), hipMemcpyDeviceToHost,
+                          h->stream));
+    HIPCHK(hipStreamSynchronize(h->stream));
     return 0;
 }
 
@@ -416,10 +474,38 @@ static int alloc_phase_scratch(mpd_handle *h)
     return 0;
 }
 
+static int take_event(mpd_handle *h, hipEvent_t *e)
+{
+    if (!h->ev_free.empty()) {
+        *e = h->ev_free.back();
+        h->ev_free.pop_back();
+        return 0;
+    }
+    HIPCHK(hipEventCreate(e));
+    return 0;
+}
+
+// Add the elapsed time of every pending event pair to the timing counters
+// (waits for the last pair; the stream is usually idle by then).
+static int resolve_timing(mpd_handle *h)
+{
+    for (auto &pr : h->pending) {
+        HIPCHK(hipEventSynchronize(pr.second));
+        float ms = 0.f;
+        HIPCHK(hipEventElapsedTime(&ms, pr.first, pr.second));
+        h->acc_ms += ms;
+        h->ev_free.push_back(pr.first);
+        h->ev_free.push_back(pr.second);
+    }
+    h->pending.clear();
+    return 0;
+}
+
 int mpd_upload_state(mpd_handle *h, const double *tilde_h, const double *tilde_f, const double *macro,
                      const double *face_fx, const double *face_fy)
 {
     if (!h) return fail(MPD_ERR_INVALID, "null handle");
+    if (h->batch) return fail(MPD_ERR_STATE, "handle is a member of a batch: remove it first");
     if (!tilde_f || !macro) return fail(MPD_ERR_INVALID, "tilde_f and macro are required");
     bool ac = !is_pp(h->cfg);
     if (ac && !tilde_h) return fail(MPD_ERR_INVALID, "tilde_h is required for the Allen-Cahn models");
@@ -433,6 +519,9 @@ int mpd_upload_state(mpd_handle *h, const double *tilde_h, const double *tilde_f
         else HIPCHK(hipMemsetAsync(h->S.ffx, 0, 2 * N * sizeof(double), h->stream));
         if (face_fy) HIPCHK(hipMemcpyAsync(h->S.ffy, face_fy, 2 * N * sizeof(double), hipMemcpyHostToDevice, h->stream));
         else HIPCHK(hipMemsetAsync(h->S.ffy, 0, 2 * N * sizeof(double), h->stream));
+        // face Psi before the first step: 0, as the reference's initialisers leave it
+        // (Ark/Cpp/Preprocess.cpp:337-401 never set it); mpd_upload_face_psi sets it
+        HIPCHK(hipMemsetAsync(h->S.fpsi_old, 0, 2 * N * sizeof(double), h->stream));
     }
     // Rho_1k starts at 0 in the reference (Ark/Header/Macroscopic.h:44)
     HIPCHK(hipMemsetAsync(h->S.rho1k, 0, N * sizeof(double), h->stream));
@@ -550,10 +639,19 @@ int mpd_step(mpd_handle *h, int64_t nsteps)
         h->prime_stage = 0;
     }
     h->effective_path = path;
-    HIPCHK(hipEventRecord(h->ev0, h->stream));
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (!h->per_launch) {
+        if (int r = take_event(h, &e0)) return r;
+        if (int r = take_event(h, &e1)) return r;
+        HIPCHK(hipEventRecord(e0, h->stream));
+    }
     if (path == MPD_PATH_PHASES) {
         if (int r = alloc_phase_scratch(h)) return r;
-        for (int64_t s = 0; s < nsteps; ++s) phase_step(h->S, h->C, h->stream);
+        for (int64_t s = 0; s < nsteps; ++s) {
+            phase_step(h->S, h->C, h->stream);
+            // this step's face Psi becomes the previous step's for the next one
+            if (is_pp(h->cfg)) std::swap(h->S.fpsi, h->S.fpsi_old);
+        }
     } else {
         if (!h->F.ready) {
             int r = fused_alloc(h->F, h->S, h->C, h->cfg);
@@ -615,11 +713,13 @@ int mpd_step(mpd_handle *h, int64_t nsteps)
         for (; s < nsteps; ++s) fused_step(h->F, h->S, h->C, h->cfg, 0, h->stream);
     }
     HIPCHK(hipGetLastError());
-    HIPCHK(hipEventRecord(h->ev1, h->stream));
-    HIPCHK(hipEventSynchronize(h->ev1));
-    float ms = 0.f;
-    HIPCHK(hipEventElapsedTime(&ms, h->ev0, h->ev1));
-    h->acc_ms += ms;
+    if (e1) {
+        HIPCHK(hipEventRecord(e1, h->stream));
+        h->pending.emplace_back(e0, e1);
+        // bound the pool: a caller that never synchronises waits here now and then
+        if (h->pending.size() >= 256)
+            if (int r = resolve_timing(h)) return r;
+    }
     h->launches += nsteps;
     return 0;
 }
@@ -627,6 +727,7 @@ int mpd_step(mpd_handle *h, int64_t nsteps)
 int mpd_kernel_timing(mpd_handle *h, double *avg_ms, int64_t *launches)
 {
     if (!h) return fail(MPD_ERR_INVALID, "null handle");
+    if (int r = resolve_timing(h)) return r;
     if (avg_ms) *avg_ms = h->launches ? h->acc_ms / (double)h->launches : 0.0;
     if (launches) *launches = h->launches;
     h->acc_ms = 0.0;
@@ -637,6 +738,7 @@ int mpd_kernel_timing(mpd_handle *h, double *avg_ms, int64_t *launches)
 int mpd_set_kernel_timing(mpd_handle *h, int enable)
 {
     if (!h) return fail(MPD_ERR_INVALID, "null handle");
+    if (int r = resolve_timing(h)) return r;
     h->per_launch = enable != 0;
     h->acc_ms = 0.0;
     h->launches = 0;
@@ -647,7 +749,7 @@ int mpd_synchronize(mpd_handle *h)
 {
     if (!h) return fail(MPD_ERR_INVALID, "null handle");
     HIPCHK(hipStreamSynchronize(h->stream));
-    return 0;
+    return resolve_timing(h);
 }
 
 int mpd_device_bytes(const mpd_handle *h, int64_t *bytes)

@@ -18,6 +18,7 @@ struct DevState {
     double *hfl, *ffl;            // face fluxes: x-faces [Q][N] then y-faces [Q][nyf*nx]
     double *gm, *ghbp, *gfbp;     // wall ghost rows
     double *fpsi, *fscr;          // PP face Psi [2N], face scratch [4][2N]
+    double *fpsi_old;             // PP face Psi of the previous step [2N] (0 before the first step)
     double *rho1k;                // density of the last residual evaluation
 };
 

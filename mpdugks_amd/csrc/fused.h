@@ -77,6 +77,8 @@ void fused32_step(const FusedPlanes32 &in, const FusedPlanes32 &out, const Const
 int fused32_tile_auto(int nx, int ny, int cus, int64_t state_bytes);
 // device bytes of the two FP32-bulk Allen-Cahn state sets of a configuration
 int64_t fused32_state_bytes(const mpd_config &cfg);
+// planes of one Allen-Cahn state set (fused_alloc)
+int fused_nplanes(const mpd_config &cfg);
 
 // batched FP32-bulk step over many cases (fused_fp32.hip)
 struct BatchMember {

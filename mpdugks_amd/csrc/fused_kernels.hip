@@ -640,12 +640,16 @@ bool fused_supported(const mpd_config &cfg)
 
 bool fused_needs_priming(const mpd_config &cfg) { return cfg.model != MPD_MODEL_PP_D2Q9; }
 
+int fused_nplanes(const mpd_config &cfg)
+{
+    // h~(9) f~(9) phi, + phi_lag (HCZ, or walls for the ghost rows), + ppu ppv (HCZ)
+    return cfg.model == MPD_MODEL_AC_HCZ ? 22 : (cfg.bc_y != MPD_BC_PERIODIC ? 20 : 19);
+}
+
 int64_t fused32_state_bytes(const mpd_config &cfg)
 {
     // per set: the float planes (fused_alloc) and the FP64 phase field
-    const bool walls = cfg.bc_y != MPD_BC_PERIODIC;
-    const int64_t planes = cfg.model == MPD_MODEL_AC_HCZ ? 22 : (walls ? 20 : 19);
-    return 2 * (int64_t)cfg.nx * cfg.ny * (4 * planes + 8);
+    return 2 * (int64_t)cfg.nx * cfg.ny * (4 * (int64_t)fused_nplanes(cfg) + 8);
 }
 
 int fused_alloc(FusedState &F, const DevState &S, const Consts &C, const mpd_config &cfg)
@@ -663,8 +667,7 @@ int fused_alloc(FusedState &F, const DevState &S, const Consts &C, const mpd_con
         F.ready = true;
         return 0;
     }
-    // h~(9) f~(9) phi, + phi_lag (HCZ, or walls for the ghost rows), + ppu ppv (HCZ)
-    F.nplanes = cfg.model == MPD_MODEL_AC_HCZ ? 22 : (C.walls ? 20 : 19);
+    F.nplanes = fused_nplanes(cfg);
     // FP32 keeps f as a deviation from the rest equilibrium at p_ref = 1 (p0 of
     // the reference, Ark/Constant/ZeroReference.h:86)
     F.p_ref = F.fp32 ? 1.0 : 0.0;

@@ -255,6 +255,41 @@ __global__ void __launch_bounds__(256) k_pp_face_psi(DevState S, Consts C)
     S.fscr[3 * 2 * (size_t)S.N + t] = scg;
 }
 
+// Face objects of the reference's one-thread face order (Consts::face_ref).
+// The reference stores each periodic face twice (zones 12 and 8) and visits
+// its faces in FaceArray order: interior x-faces row by row, interior y-faces
+// row by row, then the left, right, bottom and top periodic zones
+// (Ark/Cpp/DUGKS2DSolver.cpp:219-230, for meshes written by
+// mpdugks_amd.mesh.write_cas).  An x-face object is (i, j), i in [0, nx]: i = 0
+// and i = nx are the two copies of the periodic face; y-faces likewise in j.
+// Both copies see the same new/old pattern of neighbours, so their forces stay
+// equal and the copy at i = 0 (j = 0) stands for the pair.
+__device__ inline int64_t face_rank(const Consts &C, int dir, int i, int j)
+{
+    const int64_t nx = C.nx, ny = C.ny, NX = ny * (nx - 1), NI = NX + (ny - 1) * nx;
+    if (dir == 0) return i == 0 ? NI + j : (i == nx ? NI + ny + j : (int64_t)j * (nx - 1) + (i - 1));
+    return j == 0 ? NI + 2 * ny + i : (j == ny ? NI + 2 * ny + nx + i : NX + (int64_t)(j - 1) * nx + i);
+}
+
+// Psi of the stencil neighbour at (di, dj) as the object (dir, i, j) of rank r0
+// reads it: faceFaces (Ark/Cpp/Mesh/MeshCartesian.cpp:286-368) reach the copy
+// on the referencing face's side of a periodic line; a neighbour visited
+// earlier gives its new Psi, one visited later the Psi of the previous step.
+__device__ inline double psi_seen(const DevState &S, const Consts &C, int dir, int i, int j, int di, int dj,
+                                  int64_t r0)
+{
+    int a = i + di, b = j + dj;
+    if (dir == 0) {
+        a = a < 0 ? a + C.nx : (a > C.nx ? a - C.nx : a);
+        b = wrapi(b, C.ny);
+    } else {
+        a = wrapi(a, C.nx);
+        b = b < 0 ? b + C.ny : (b > C.ny ? b - C.ny : b);
+    }
+    const size_t fi = (size_t)dir * S.N + (size_t)(b == C.ny ? 0 : b) * C.nx + (a == C.nx ? 0 : a);
+    return face_rank(C, dir, a, b) < r0 ? S.fpsi[fi] : S.fpsi_old[fi];
+}
+
 // Pseudopotential faces, pass 2: face force from the face-lattice gradient of
 // Psi (update_MQ_x/y(Face_2D*), Ark/Lib/GradSchemeBasic.h:181-208;
 // Update_PseudoForce(face), Ark/Cpp/PseudoP.cpp:39-46), then the BGK half step.
@@ -266,10 +301,23 @@ __global__ void __launch_bounds__(256) k_pp_face_flux(DevState S, Consts C)
     int dir = t < (size_t)S.N ? 0 : 1;
     size_t fidx = dir == 0 ? t : t - S.N;
     int i = (int)(fidx % C.nx), j = (int)(fidx / C.nx);
-    const double *ps = S.fpsi + (size_t)dir * S.N;
-    auto P = [&](int a, int b) { return ps[(size_t)wrapi(b, C.ny) * C.nx + wrapi(a, C.nx)]; };
-    double c[4] = {P(i + 1, j), P(i, j + 1), P(i - 1, j), P(i, j - 1)};
-    double d[4] = {P(i + 1, j + 1), P(i - 1, j + 1), P(i - 1, j - 1), P(i + 1, j - 1)};
+    double c[4], d[4];
+    if (C.face_ref) {
+        const int64_t r0 = face_rank(C, dir, i, j);
+        c[0] = psi_seen(S, C, dir, i, j, 1, 0, r0);
+        c[1] = psi_seen(S, C, dir, i, j, 0, 1, r0);
+        c[2] = psi_seen(S, C, dir, i, j, -1, 0, r0);
+        c[3] = psi_seen(S, C, dir, i, j, 0, -1, r0);
+        d[0] = psi_seen(S, C, dir, i, j, 1, 1, r0);
+        d[1] = psi_seen(S, C, dir, i, j, -1, 1, r0);
+        d[2] = psi_seen(S, C, dir, i, j, -1, -1, r0);
+        d[3] = psi_seen(S, C, dir, i, j, 1, -1, r0);
+    } else {
+        const double *ps = S.fpsi + (size_t)dir * S.N;
+        auto P = [&](int a, int b) { return ps[(size_t)wrapi(b, C.ny) * C.nx + wrapi(a, C.nx)]; };
+        c[0] = P(i + 1, j); c[1] = P(i, j + 1); c[2] = P(i - 1, j); c[3] = P(i, j - 1);
+        d[0] = P(i + 1, j + 1); d[1] = P(i - 1, j + 1); d[2] = P(i - 1, j - 1); d[3] = P(i + 1, j - 1);
+    }
     double gx, gy;
     grad9(C, c, d, gx, gy);
     const size_t F2 = 2 * (size_t)S.N;

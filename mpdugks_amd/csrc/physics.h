@@ -24,6 +24,7 @@ struct Consts {
     int32_t nx, ny, model, flux, walls, nyf;
     int32_t nee;      // walls use non-equilibrium extrapolation (Wall_3_NEE) instead of bounce-back
     int32_t unit;     // dx a power of two and xi exactly the integer directions (MaSpan = 1)
+    int32_t face_ref; // pseudopotential faces in the reference's one-thread order (mpd_set_face_order)
     double dx, dt, hdt, rt, lambda0, aphi, bphi, beta, kappa, kp, taum, tau0;
     double phi_l, phi_v, nu_l, nu_v, gx, gy, rho_ref, pp_k, t_cs, pp_tau;
     double dtv;       // dt / volume                    (Cell_2D.cpp:182)

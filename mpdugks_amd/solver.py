@@ -25,6 +25,9 @@ _SYMBOLS = {
     "mpd_destroy": ([C.c_void_p], C.c_int),
     "mpd_set_path": ([C.c_void_p, C.c_int32], C.c_int),
     "mpd_set_tile": ([C.c_void_p, C.c_int32], C.c_int),
+    "mpd_set_face_order": ([C.c_void_p, C.c_int32], C.c_int),
+    "mpd_upload_face_psi": ([C.c_void_p, C.c_void_p], C.c_int),
+    "mpd_download_face_psi": ([C.c_void_p, C.c_void_p], C.c_int),
     "mpd_get_tile": ([C.c_void_p, C.POINTER(C.c_int32)], C.c_int),
     "mpd_upload_state": ([C.c_void_p] + [C.c_void_p] * 5, C.c_int),
     "mpd_download_state": ([C.c_void_p] + [C.c_void_p] * 5, C.c_int),
@@ -186,6 +189,20 @@ class Solver:
 
     def set_path(self, path):
         _check(lib().mpd_set_path(self._h, int(path)), "mpd_set_path")
+
+    def set_face_order(self, order):
+        """Pseudopotential face force order (abi.MPD_FACE_ORDER_*): Jacobi, or the
+        reference's one-thread order (FP64 D2Q9, CD, periodic)."""
+        _check(lib().mpd_set_face_order(self._h, int(order)), "mpd_set_face_order")
+
+    def upload_face_psi(self, psi):
+        p = _f64(psi)
+        _check(lib().mpd_upload_face_psi(self._h, _p(p)), "mpd_upload_face_psi")
+
+    def download_face_psi(self):
+        p = np.zeros(2 * self.cfg.nx * self.cfg.ny)
+        _check(lib().mpd_download_face_psi(self._h, _p(p)), "mpd_download_face_psi")
+        return p
 
     def set_tile(self, tile):
         """FP32-bulk Allen-Cahn tile shape (abi.MPD_TILE_*; MPD_TILE_AUTO chooses per mesh)."""

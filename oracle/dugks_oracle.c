@@ -26,11 +26,16 @@
  * the same bits as the single structured face here because every operation
  * that differs is an exact sign flip (see face_recon()).
  *
- * One deliberate difference: the pseudopotential face force.  The reference
- * writes each face's Psi and reads its neighbour faces' Psi inside the same
- * OpenMP loop (Ark/Cpp/DUGKS2DSolver.cpp:219-230, 733-746), so its result
- * depends on the face visiting order.  Here all face Psi are computed first
- * and the face forces second (Jacobi order), the same order the GPU uses.
+ * The pseudopotential face force has two modes.  The reference writes each
+ * face's Psi and reads its neighbour faces' Psi inside the same OpenMP loop
+ * (Ark/Cpp/DUGKS2DSolver.cpp:219-230, 733-746), so its result depends on the
+ * face visiting order.  The default (Jacobi) mode computes all face Psi first
+ * and the face forces second, the order the GPU uses.  The reference-order
+ * mode (ora_set_face_order) reproduces one thread's visit instead: a stencil
+ * neighbour visited earlier contributes its new Psi, one visited later the Psi
+ * it kept from the previous step; the two stored copies of each periodic face
+ * keep their own forces and fluxes (see pp_forces_ref()).  Both modes share
+ * every other function.
  */
 #include "dugks_oracle.h"
 
@@ -66,6 +71,13 @@ struct ora {
     double *gm;               /* wall ghosts [MPD_M_COUNT][2][nx] */
     double *ghbp, *gfbp;      /* wall ghost bar-plus [Q][2][nx] */
     int nyf;                  /* rows of y-faces: ny (periodic) or ny+1 (walls) */
+    /* pseudopotential, reference face order (ora_set_face_order) */
+    int ref_order;
+    double *fpsi_old;         /* face Psi kept from the previous step [2N] (0 at start:
+                                 the initialisers never set it, Ark/Cpp/Preprocess.cpp:337-401) */
+    double *cbfx, *cbfy;      /* force of the second copy (zone 8) of each periodic face:
+                                 x-faces i = 0 by row [ny], then y-faces j = 0 by column [nx] */
+    double *cbfl;             /* flux of those copies [Q][ny + nx] */
 };
 
 /* ---------------- small helpers ---------------- */
@@ -297,6 +309,10 @@ ora *ora_create(const mpd_config *cfg)
     o->gm = calloc((size_t)MPD_M_COUNT * 2 * o->nx, sizeof(double));
     o->ghbp = calloc((size_t)Q9 * 2 * o->nx, sizeof(double));
     o->gfbp = calloc((size_t)Q9 * 2 * o->nx, sizeof(double));
+    o->fpsi_old = calloc(2 * N, sizeof(double));
+    o->cbfx = calloc((size_t)o->nx + o->ny, sizeof(double));
+    o->cbfy = calloc((size_t)o->nx + o->ny, sizeof(double));
+    o->cbfl = calloc((size_t)o->Q * (o->nx + o->ny), sizeof(double));
     return o;
 }
 
@@ -304,7 +320,8 @@ void ora_destroy(ora *o)
 {
     if (!o) return;
     double *bufs[] = {o->h, o->f, o->m, o->ffx, o->ffy, o->fpsi, o->hbp, o->fbp, o->hbpx,
-                      o->hbpy, o->fbpx, o->fbpy, o->hfl, o->ffl, o->gm, o->ghbp, o->gfbp};
+                      o->hbpy, o->fbpx, o->fbpy, o->hfl, o->ffl, o->gm, o->ghbp, o->gfbp,
+                      o->fpsi_old, o->cbfx, o->cbfy, o->cbfl};
     for (size_t k = 0; k < sizeof(bufs) / sizeof(bufs[0]); ++k) free(bufs[k]);
     free(o);
 }
@@ -318,6 +335,39 @@ int ora_set_state(ora *o, const double *h, const double *f, const double *m,
     if (m) memcpy(o->m, m, (size_t)MPD_M_COUNT * N * sizeof(double));
     if (face_fx) memcpy(o->ffx, face_fx, 2 * N * sizeof(double));
     if (face_fy) memcpy(o->ffy, face_fy, 2 * N * sizeof(double));
+    /* both stored copies of a periodic face start from the uploaded force */
+    for (int j = 0; j < o->ny; ++j) {
+        o->cbfx[j] = o->ffx[(size_t)j * o->nx];
+        o->cbfy[j] = o->ffy[(size_t)j * o->nx];
+    }
+    for (int i = 0; i < o->nx; ++i) {
+        o->cbfx[o->ny + i] = o->ffx[N + i];
+        o->cbfy[o->ny + i] = o->ffy[N + i];
+    }
+    return 0;
+}
+
+int ora_set_face_order(ora *o, int ref_order)
+{
+    if (ref_order && !(o->c.model == MPD_MODEL_PP_D2Q9 && o->c.flux == MPD_FLUX_CD && !o->walls))
+        return -1;
+    o->ref_order = ref_order != 0;
+    return 0;
+}
+
+int ora_set_face_extra(ora *o, const double *psi_old, const double *copy_fx, const double *copy_fy)
+{
+    if (psi_old) memcpy(o->fpsi_old, psi_old, 2 * (size_t)o->N * sizeof(double));
+    if (copy_fx) memcpy(o->cbfx, copy_fx, ((size_t)o->nx + o->ny) * sizeof(double));
+    if (copy_fy) memcpy(o->cbfy, copy_fy, ((size_t)o->nx + o->ny) * sizeof(double));
+    return 0;
+}
+
+int ora_get_face_extra(const ora *o, double *psi, double *copy_fx, double *copy_fy)
+{
+    if (psi) memcpy(psi, o->fpsi_old, 2 * (size_t)o->N * sizeof(double));
+    if (copy_fx) memcpy(copy_fx, o->cbfx, ((size_t)o->nx + o->ny) * sizeof(double));
+    if (copy_fy) memcpy(copy_fy, o->cbfy, ((size_t)o->nx + o->ny) * sizeof(double));
     return 0;
 }
 
@@ -599,6 +649,110 @@ static inline double face_psi(const ora *o, int dir, int i, int j)
     return o->fpsi[(size_t)dir * o->N + (size_t)wrap(j, o->ny) * o->nx + wrap(i, o->nx)];
 }
 
+/* ---- pseudopotential faces in the reference's one-thread order ----
+ *
+ * Face objects.  An x-face object is (i, j) with i in [0, nx], j in [0, ny):
+ * i = 0 and i = nx are the two stored copies of the periodic face between
+ * columns nx-1 and 0 (zone 12, owner (0,j); zone 8, owner (nx-1,j)); y-face
+ * objects likewise with j in [0, ny].
+ *
+ * Rank: the position of the object in the visit of one thread, i.e. its index
+ * in FaceArray (the interior loop runs over InteriorFaceA, then the periodic
+ * loop over PeriodicFaceA, Ark/Cpp/DUGKS2DSolver.cpp:219-230, both in FaceArray
+ * order).  FaceArray follows the face sections of the mesh file; for the mesh
+ * written by mpdugks_amd.mesh.write_cas that is: interior x-faces row by row,
+ * interior y-faces row by row, then the left (12), right (8), bottom (12) and
+ * top (8) periodic zones. */
+static long face_rank(const ora *o, int dir, int i, int j)
+{
+    long nx = o->nx, ny = o->ny, NX = ny * (nx - 1), NI = NX + (ny - 1) * nx;
+    if (dir == 0) {
+        if (i == 0) return NI + j;
+        if (i == nx) return NI + ny + j;
+        return (long)j * (nx - 1) + (i - 1);
+    }
+    if (j == 0) return NI + 2 * ny + i;
+    if (j == ny) return NI + 2 * ny + nx + i;
+    return NX + (long)(j - 1) * nx + i;
+}
+
+/* The object a face stencil reaches from object (dir, *i, *j) at offset
+ * (di, dj).  faceFaces (Ark/Cpp/Mesh/MeshCartesian.cpp:296-368) take a periodic
+ * face through the cells' own Face_C, or through DiagonalFacesCheck (:286-294)
+ * for the diagonal entries; either way the copy on the referencing face's side
+ * of the domain, the geometrically nearer one: one step across the periodic
+ * line from i = 0 lands on i = nx - 1, from i = nx on 1. */
+static void face_obj_step(const ora *o, int dir, int *i, int *j, int di, int dj)
+{
+    int nx = o->nx, ny = o->ny, a = *i + di, b = *j + dj;
+    if (dir == 0) {
+        if (a < 0) a += nx;
+        else if (a > nx) a -= nx;
+        b = wrap(b, ny);
+    } else {
+        a = wrap(a, nx);
+        if (b < 0) b += ny;
+        else if (b > ny) b -= ny;
+    }
+    *i = a;
+    *j = b;
+}
+
+/* Psi of the neighbour at (di, dj) as object `self` (rank r0) reads it: the new
+ * value if one thread has already visited it this step, else the value it kept
+ * from the previous step (msqh->Psi, Ark/Lib/GradSchemeBasic.h:181-208). */
+static double psi_seen(const ora *o, int dir, int i, int j, int di, int dj, long r0)
+{
+    face_obj_step(o, dir, &i, &j, di, dj);
+    size_t fi = (size_t)dir * o->N + (size_t)wrap(j, o->ny) * o->nx + wrap(i, o->nx);
+    return face_rank(o, dir, i, j) < r0 ? o->fpsi[fi] : o->fpsi_old[fi];
+}
+
+/* Update_PseudoForce(Face_2D&) (Ark/Cpp/PseudoP.cpp:39-46) of object (dir, i, j)
+ * with the face-lattice gradient update_MQ_x/y(Face_2D*) (GradSchemeBasic.h:181-208). */
+static void pp_force_ref(const ora *o, int dir, int i, int j, double scg, double psi, double *fx, double *fy)
+{
+    long r0 = face_rank(o, dir, i, j);
+    double d2x = 2.0 * o->c.dx;
+    double F1 = psi_seen(o, dir, i, j, 1, 0, r0), F5 = psi_seen(o, dir, i, j, -1, 0, r0);
+    double F3 = psi_seen(o, dir, i, j, 0, 1, r0), F7 = psi_seen(o, dir, i, j, 0, -1, r0);
+    double F2 = psi_seen(o, dir, i, j, 1, 1, r0), F4 = psi_seen(o, dir, i, j, -1, 1, r0);
+    double F6 = psi_seen(o, dir, i, j, -1, -1, r0), F8 = psi_seen(o, dir, i, j, 1, -1, r0);
+    double gxp = (4 * (F1 - F5) + 1 * (F2 - F4 + F8 - F6)) / (d2x * 6);
+    double gyp = (4 * (F3 - F7) + 1 * (F2 - F8 + F4 - F6)) / (d2x * 6);
+    gxp *= -scg * psi;
+    gyp *= -scg * psi;
+    *fx = gxp;
+    *fy = gyp;
+}
+
+/* Forces of every face object in reference order.  fms holds the macros of each
+ * structured face with the force of its first copy, fmb those of the second
+ * copies of the periodic faces (index as cbfx); only U, V differ between the
+ * copies (they carry their own previous force, D2Q9PP.cpp:119-122), rho, scG
+ * and Psi are the same bits. */
+static void pp_forces_ref(ora *o, face_macro *fms, face_macro *fmb)
+{
+    int nx = o->nx, ny = o->ny;
+    size_t N = o->N;
+    for (int dir = 0; dir < 2; ++dir)
+        for (int j = 0; j < ny; ++j)
+            for (int i = 0; i < nx; ++i) {
+                size_t fi = (size_t)dir * N + (size_t)j * nx + i;
+                face_macro *fm = &fms[fi];
+                pp_force_ref(o, dir, i, j, fm->scg, fm->psi, &fm->Fx, &fm->Fy);
+                o->ffx[fi] = fm->Fx;
+                o->ffy[fi] = fm->Fy;
+                if ((dir == 0 && i == 0) || (dir == 1 && j == 0)) {
+                    size_t b = dir == 0 ? (size_t)j : (size_t)ny + i;
+                    face_macro *mb = &fmb[b];
+                    pp_force_ref(o, dir, dir == 0 ? nx : i, dir == 0 ? j : ny, mb->scg, mb->psi, &mb->Fx, &mb->Fy);
+                    o->cbfx[b] = mb->Fx;
+                    o->cbfy[b] = mb->Fy;
+                }
+            }
+}
+
 /* Equilibria and sources of population q of cell n (the AC part of
  * phase1_cell, same expressions). */
 static void cell_eq_so(const ora *o, size_t n, int q, double *heq, double *feq, double *hso, double *fso)
@@ -665,8 +819,10 @@ static void phase4_faces(ora *o)
     int nx = o->nx, ny = o->ny, N = o->N;
     int pp = o->pp, Q = o->Q;
     double hb[QMAX], fb[QMAX];
-    face_macro *fms = NULL;
+    face_macro *fms = NULL, *fmb = NULL;
+    int ref = pp && o->ref_order;
     if (pp) fms = (face_macro *)malloc(sizeof(face_macro) * 2 * (size_t)N);
+    if (ref) fmb = (face_macro *)malloc(sizeof(face_macro) * ((size_t)nx + ny));
     /* PP, first pass: face macros and Psi of every face (Jacobi order). */
     if (pp) {
         for (int dir = 0; dir < 2; ++dir)
@@ -680,7 +836,14 @@ static void phase4_faces(ora *o)
                     face_macro *fm = &fms[fi];
                     face_macros(o, &F, hb, fb, o->ffx[fi], o->ffy[fi], fm);
                     o->fpsi[fi] = fm->psi;
+                    if (ref && ((dir == 0 && i == 0) || (dir == 1 && j == 0))) {
+                        size_t b = dir == 0 ? (size_t)j : (size_t)ny + i;
+                        face_macros(o, &F, hb, fb, o->cbfx[b], o->cbfy[b], &fmb[b]);
+                    }
                 }
+    }
+    if (ref) pp_forces_ref(o, fms, fmb);
+    else if (pp) {
         /* second pass: face force from the face-lattice gradient of Psi,
          * update_MQ_x/y(Face_2D*) (Ark/Lib/GradSchemeBasic.h:181-208) and
          * Update_PseudoForce(face) (Ark/Cpp/PseudoP.cpp:39-46) */
@@ -754,9 +917,21 @@ static void phase4_faces(ora *o)
                     hfl[k] = hh[q] * xin;
                     ffl[k] = ff[q] * xin;
                 }
+                /* reference order: the second copy of a periodic face has its own
+                 * macros and force, so its own flux (taken by the cell that owns it) */
+                if (ref && ((dir == 0 && i == 0) || (dir == 1 && j == 0))) {
+                    size_t b = dir == 0 ? (size_t)j : (size_t)ny + i;
+                    for (int q = 0; q < Q; ++q) {
+                        face_bgk(o, &fmb[b], q, 0, fb[q], ah, bh, ch, &hh[q], &ff[q]);
+                        double xin = o->c.dx * (dir == 0 ? o->xi[0][q] : o->xi[1][q]);
+                        o->cbfl[(size_t)q * (nx + ny) + b] = ff[q] * xin;
+                    }
+                }
             }
     }
+    if (pp) memcpy(o->fpsi_old, o->fpsi, 2 * (size_t)N * sizeof(double));
     free(fms);
+    free(fmb);
 }
 
 /* ---------------- phase 8: cell update ---------------- */
@@ -790,9 +965,14 @@ static void phase8_update(ora *o)
                     s += 1 * hy[kS];
                     o->h[k] = aTP * o->hbp[k] - bTP * o->h[k] + o->dtv * s;
                 }
+                /* reference order: the last column / row owns the second copy of
+                 * its periodic east / north face (Face_C, MeshCartesian.cpp:28-60) */
+                double fE = fx[kE], fN = fy[kN];
+                if (o->ref_order && i == nx - 1) fE = o->cbfl[(size_t)q * (nx + ny) + j];
+                if (o->ref_order && j == ny - 1) fN = o->cbfl[(size_t)q * (nx + ny) + ny + i];
                 double s = 0.0;
-                s += -1 * fx[kE];
-                s += -1 * fy[kN];
+                s += -1 * fE;
+                s += -1 * fN;
                 s += 1 * fx[kW];
                 s += 1 * fy[kS];
                 o->f[k] = aTP * o->fbp[k] - bTP * o->f[k] + o->dtv * s;

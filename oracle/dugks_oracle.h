@@ -16,6 +16,13 @@ int ora_get_state(const ora *o, double *h, double *f, double *m, double *face_fx
 int ora_step(ora *o, int nsteps);
 double ora_wall_flux_residual(const ora *o);
 int ora_cell_phases(ora *o);
+/* pseudopotential D2Q9 (periodic, CD): 1 = visit faces in the reference's
+ * one-thread order, 0 = Jacobi order (default) */
+int ora_set_face_order(ora *o, int ref_order);
+/* face Psi of the last step [2N]; forces of the second copies of the periodic
+ * faces [nx + ny] (x-faces by row, then y-faces by column) */
+int ora_set_face_extra(ora *o, const double *psi_old, const double *copy_fx, const double *copy_fy);
+int ora_get_face_extra(const ora *o, double *psi, double *copy_fx, double *copy_fy);
 void ora_d2v16_tables(double *a, double *xi_x, double *xi_y, double *e);
 void ora_d2v16_feq(double rho, double U, double V, double rt, double *feq);
 #ifdef __cplusplus

@@ -41,6 +41,11 @@ def lib():
         _lib.ora_d2v16_tables.restype = None
         _lib.ora_d2v16_feq.argtypes = [C.c_double] * 4 + [C.c_void_p]
         _lib.ora_d2v16_feq.restype = None
+        _lib.ora_set_face_order.argtypes = [C.c_void_p, C.c_int]
+        _lib.ora_set_face_order.restype = C.c_int
+        for fn in (_lib.ora_set_face_extra, _lib.ora_get_face_extra):
+            fn.argtypes = [C.c_void_p] * 4
+            fn.restype = C.c_int
     return _lib
 
 
@@ -84,6 +89,25 @@ class Oracle:
     def cell_phases(self):
         """Phases 9-11 (macros and sources) on the current populations."""
         lib().ora_cell_phases(self._h)
+
+    def set_face_order(self, reference):
+        """Pseudopotential D2Q9: True = visit faces in the reference's
+        one-thread order (Ark/Cpp/DUGKS2DSolver.cpp:219-230), False = Jacobi."""
+        if lib().ora_set_face_order(self._h, 1 if reference else 0) != 0:
+            raise ValueError("reference face order: pseudopotential D2Q9, CD, periodic only")
+
+    def set_face_extra(self, psi_old=None, copy_fx=None, copy_fy=None):
+        arrs = [None if a is None else np.ascontiguousarray(a, dtype=np.float64)
+                for a in (psi_old, copy_fx, copy_fy)]
+        lib().ora_set_face_extra(self._h, *[_ptr(a) for a in arrs])
+
+    def get_face_extra(self):
+        """(face Psi of the last step [2N], second-copy forces x [nx+ny], y [nx+ny])."""
+        psi = np.zeros(2 * self.N)
+        n = self.cfg.nx + self.cfg.ny
+        cx = np.zeros(n); cy = np.zeros(n)
+        lib().ora_get_face_extra(self._h, _ptr(psi), _ptr(cx), _ptr(cy))
+        return psi, cx, cy
 
     def wall_flux_residual(self):
         return lib().ora_wall_flux_residual(self._h)

@@ -141,11 +141,46 @@ void dump_cells(int k)
         }
     std::fprintf(stderr, "[ref_harness] dumped step %d\n", k);
 }
+
+// Face topology (MPD_REF_TOPO=1): topo.bin = int32 Faces, InteriorFaceNum,
+// PeriodicFaceNum, WallFaceNum, then per face object in FaceArray order 9 int32
+// faceFaces indices (-1 = none), the shadowF index, bc_type, then 6 float64
+// (xf, yf, owner xc, owner yc, neigh xc, neigh yc); then the FaceArray index of
+// every InteriorFaceA, PeriodicFaceA and WallFaceA entry, i.e. the order in which
+// one thread visits the faces (Ark/Cpp/DUGKS2DSolver.cpp:219-250).
+void dump_topology()
+{
+    char path[4096];
+    std::snprintf(path, sizeof(path), "%s/topo.bin", g_dir.c_str());
+    FILE *fp = std::fopen(path, "wb");
+    if (!fp) { std::perror(path); std::exit(3); }
+    auto index = [](Face_2D const *f) { return f ? static_cast<int>(f - FaceArray) : -1; };
+    int hdr[4] = {Faces, InteriorFaceNum, PeriodicFaceNum, WallFaceNum};
+    std::fwrite(hdr, sizeof(int), 4, fp);
+    for (int n = 0; n < Faces; ++n) {
+        Face_2D const &f = FaceArray[n];
+        int iv[11];
+        for (int k = 0; k < 9; ++k) iv[k] = index(f.faceFaces[k]);
+        iv[9] = index(f.shadowF);
+        iv[10] = f.bc_type;
+        double dv[6] = {f.xf, f.yf, f.owner ? f.owner->xc : 0.0, f.owner ? f.owner->yc : 0.0,
+                        f.neigh ? f.neigh->xc : 0.0, f.neigh ? f.neigh->yc : 0.0};
+        std::fwrite(iv, sizeof(int), 11, fp);
+        std::fwrite(dv, sizeof(double), 6, fp);
+    }
+    for (int n = 0; n < InteriorFaceNum; ++n) { int v = index(InteriorFaceA[n]); std::fwrite(&v, sizeof(int), 1, fp); }
+    for (int n = 0; n < PeriodicFaceNum; ++n) { int v = index(PeriodicFaceA[n]); std::fwrite(&v, sizeof(int), 1, fp); }
+    for (int n = 0; n < WallFaceNum; ++n) { int v = index(WallFaceA[n]); std::fwrite(&v, sizeof(int), 1, fp); }
+    std::fclose(fp);
+}
 }  // namespace
 
 extern "C" void __wrap__Z16Output_FlowfieldRKdi(double const &t, int step)
 {
     parse_env();
+    if (!g_started)
+        if (const char *s = std::getenv("MPD_REF_TOPO"))
+            if (s[0] == '1') dump_topology();
     g_started = true;
     if (const char *s = std::getenv("MPD_REF_FLOWFIELD"))
         if (s[0] == '1') __real__Z16Output_FlowfieldRKdi(t, step);

@@ -32,6 +32,9 @@ CONFIGS = {
     "acp_uw": ("D2Q9AC", "-include " + os.path.join(CFGDIR, "uw", "flags.h"), False, ""),
     "pp": ("D2Q9PP", "-include " + os.path.join(CFGDIR, "pp", "flags.h"), False,
            os.path.join(CFGDIR, "pp")),
+    # SCMP_Drop shifted by (60, 70) cells: the drop crosses both periodic lines
+    "pp_shift": ("D2Q9PP", "-include " + os.path.join(CFGDIR, "pp", "flags.h"), False,
+                 os.path.join(CFGDIR, "pp_shift")),
 }
 
 CELL_SCALARS = ["Phi", "Rho", "U", "V", "p", "Phi_x", "Phi_y", "Rho_x", "Rho_y",

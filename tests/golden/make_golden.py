@@ -36,8 +36,12 @@ CONFIGS = {"hcz": ("ac_hcz", STEPS), "acp": ("ac_p", STEPS), "hcz_wall": ("ac_hc
            # UW face scheme: bit-exact at step 1; afterwards the reference's two copies
            # of each periodic face differ in the last bit (see tests/test_oracle_ref_live.py)
            "hcz_uw": ("ac_hcz", [0, 1]), "acp_uw": ("ac_p", [0, 1]),
-           # pseudopotential: the initial state only (its face force is order dependent)
-           "pp": ("pp_d2q9", [0])}
+           # pseudopotential: the face force depends on the face visiting order; the
+           # reference runs with one thread, which the oracle's reference-order mode follows
+           "pp": ("pp_d2q9", STEPS),
+           # the same drop shifted cyclically by (60, 70) cells: it crosses both periodic lines
+           "pp_shift": ("pp_d2q9", STEPS)}
+SHIFT = {"pp_shift": [60, 70]}  # (x, y) cells, oracle/refcfg/pp_shift/ZeroCondition.h
 
 
 def digest(a):
@@ -63,8 +67,10 @@ def main(names=None):
                "bc": "nee" if cfg.endswith("_nee") else ("bb" if cfg.endswith("_wall") else "periodic"),
                "nx": 128, "ny": 128,
                "source": "reference Ark, unmodified sources, OMP_NUM_THREADS=1, %s"
-                         % ("SCMP_Drop" if cfg == "pp" else "AC_Drop"),
+                         % ("SCMP_Drop" if cfg.startswith("pp") else "AC_Drop"),
                "steps": {}}
+        if cfg in SHIFT:
+            rec["shift"] = SHIFT[cfg]
         for k in steps:
             d = load_cells(os.path.join(out, "cells_step%d.bin" % k))
             r = state_digests(d)

@@ -77,3 +77,34 @@ def test_fp32_bulk_without_a_kernel_is_refused_at_create():
         h = C.c_void_p()
         assert L.mpd_create(C.byref(cfg), C.byref(h)) == -1, (model, over)  # MPD_ERR_INVALID
         assert b"FP32-bulk" in L.mpd_last_error()
+
+
+def test_header_compiles_as_c99_and_links(tmp_path):
+    """include/mpdugks.h as a C99 header: a C translation unit compiled with
+    -std=c99 -pedantic -Werror, linked against libmpdugks_hip.so, runs the
+    host-only entry points through it."""
+    import subprocess
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    src = os.path.join(root, "tests", "c", "abi_smoke.c")
+    libdir = os.path.dirname(solver.LIB_PATH)
+    exe = str(tmp_path / "abi_smoke")
+    subprocess.run(["gcc", "-std=c99", "-pedantic", "-Wall", "-Wextra", "-Werror", "-I" + os.path.join(root, "include"),
+                    src, "-o", exe, "-L" + libdir, "-lmpdugks_hip", "-Wl,-rpath," + libdir], check=True)
+    out = subprocess.run([exe], check=True, capture_output=True, text=True).stdout
+    mass = float(out.split()[-1])
+    cfg = default_config(abi.MPD_MODEL_AC_HCZ, nx=32, ny=24)
+    st = solver.ac_drop(cfg, cx=16.0, cy=12.0, radius=8.0, sign=-1)
+    assert abs(mass - st.m[abi.M["phi"]].sum()) < 1e-6
+
+
+def test_fp32_mesh_beyond_buffer_range_refused():
+    """FP32-bulk Allen-Cahn handles address a state set through 32-bit buffer
+    resources: a mesh whose float planes would span 4 GiB or more is refused
+    with MPD_ERR_INVALID by mpd_create, before any device call."""
+    for model, side in ((abi.MPD_MODEL_AC_HCZ, 7000), (abi.MPD_MODEL_AC_P, 8000)):
+        cfg = default_config(model, nx=side, ny=side)
+        cfg.precision = abi.MPD_FP32_BULK
+        h = C.c_void_p()
+        rc = solver.lib().mpd_create(C.byref(cfg), C.byref(h))
+        assert rc == -1 and not h.value
+        assert b"too large for the FP32-bulk" in solver.lib().mpd_last_error()

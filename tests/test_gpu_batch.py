@@ -49,8 +49,15 @@ def test_member_cannot_step_alone():
     b.add(s)
     with pytest.raises(solver.MpdError):
         s.step(1)
+    # nor be uploaded to or switched to another path while the batch owns it
+    st = s.download()
+    with pytest.raises(solver.MpdError, match="MPD_ERR_STATE"):
+        s.upload(st)
+    with pytest.raises(solver.MpdError, match="MPD_ERR_STATE"):
+        s.set_path(abi.MPD_PATH_PHASES)
     b.remove(s)
     s.step(1)
+    s.upload(st)
     b.close()
 
 

@@ -81,3 +81,72 @@ def test_nonfinite_state_is_reported():
     r = s.reductions()
     assert r["status"] == -3 and r["nonfinite"] >= 1
     s.close()
+
+
+@pytest.mark.parametrize("name", ["pp", "pp_shift"])
+def test_pp_reference_face_order_bit_exact_vs_reference(name):
+    """The FP64 pseudopotential step in the reference's one-thread face order
+    reproduces the compiled reference's own state digests (tests/golden,
+    OMP_NUM_THREADS=1) after 1, 2, 10 and 100 steps, bit for bit."""
+    from test_oracle_golden import check_state, load, pp_case
+    g = load(name)
+    cfg, st = pp_case(g)
+    s = solver.Solver(cfg, path=abi.MPD_PATH_PHASES)
+    s.set_face_order(abi.MPD_FACE_ORDER_REFERENCE)
+    s.upload(st)
+    done = 0
+    for k in sorted(int(x) for x in g["steps"] if int(x) > 0):
+        s.step(k - done)
+        done = k
+        bad = check_state(s.download(), g["steps"][str(k)])
+        assert bad == [], "%s step %d: planes %s" % (name, k, bad)
+    s.close()
+
+
+def test_pp_reference_face_order_vs_oracle_nonsquare(oracle_mod):
+    """Reference face order on a 48x40 mesh with the drop across a corner: the
+    face ranks depend on nx and ny separately; GPU state, face forces and face
+    Psi equal the oracle's reference-order mode bit for bit."""
+    cfg, st = make_case("pp_d2q9", nx=48, ny=40, radius=12.0)
+    sx, sy = 20, 17
+    roll = lambda a: np.roll(a.reshape(a.shape[0], 40, 48), (sy, sx), axis=(1, 2)).reshape(a.shape)
+    st = solver.State(cfg, None, roll(st.f), roll(st.m), st.ffx, st.ffy)
+    o = oracle_mod.Oracle(cfg)
+    o.set_state(np.zeros_like(st.f), st.f, st.m, st.ffx, st.ffy)
+    o.set_face_order(True)
+    s = solver.Solver(cfg, path=abi.MPD_PATH_PHASES)
+    s.set_face_order(abi.MPD_FACE_ORDER_REFERENCE)
+    s.upload(st)
+    done = 0
+    for k in (1, 3, 20):
+        o.step(k - done)
+        s.step(k - done)
+        done = k
+        h, f, m, ffx, ffy = o.get_state()
+        _cmp(s.download(), solver.State(cfg, None, f, m, ffx, ffy), "48x40 step %d" % k)
+        assert np.array_equal(s.download_face_psi(), o.get_face_extra()[0])
+    s.close()
+
+
+def test_pp_face_psi_resume_matches_continuous_run():
+    """Stopping after 5 steps, downloading the state and the face Psi and
+    resuming in a new handle gives the continuous 10-step run bit for bit."""
+    cfg, st = make_case("pp_d2q9", nx=64, ny=64)
+    a = solver.Solver(cfg, path=abi.MPD_PATH_PHASES)
+    a.set_face_order(abi.MPD_FACE_ORDER_REFERENCE)
+    a.upload(st)
+    a.step(10)
+    ref = a.download()
+    b = solver.Solver(cfg, path=abi.MPD_PATH_PHASES)
+    b.set_face_order(abi.MPD_FACE_ORDER_REFERENCE)
+    b.upload(st)
+    b.step(5)
+    mid, psi = b.download(), b.download_face_psi()
+    c = solver.Solver(cfg, path=abi.MPD_PATH_PHASES)
+    c.set_face_order(abi.MPD_FACE_ORDER_REFERENCE)
+    c.upload(mid)
+    c.upload_face_psi(psi)
+    c.step(5)
+    _cmp(c.download(), ref, "resumed")
+    for x in (a, b, c):
+        x.close()
