@@ -302,6 +302,15 @@ int mpd_device_init_drops(mpd_handle *h, int32_t ndrops, const double *cx, const
 /* mpd_init_ac_interface on the device; y0[nx] is a host array. */
 int mpd_device_init_interface(mpd_handle *h, const double *y0, int32_t sign, double p0);
 
+/* mpd_device_init_interface without any device allocation: the y0 staging
+ * buffer is a fixed device array of the library (16384 columns; larger nx is
+ * refused with MPD_ERR_INVALID) instead of a per-call hipMalloc/hipFree.  Same
+ * state bit for bit.  Calls that overlap on one GPU share that array and race.
+ * No counterpart in the reference: for callers that cannot allocate device
+ * memory (serving loops that must not allocate per request, code running
+ * while allocations are forbidden). */
+int mpd_device_init_interface_noalloc(mpd_handle *h, const double *y0, int32_t sign, double p0);
+
 /* Pseudopotential noisy start: rho = rho0 (1 + U(-amp, amp)) per cell with a
  * counter-based splitmix64 stream of (seed, cell) (not the host's mt19937_64
  * sequence), u = 0, tilde = equilibrium, face forces 0. */

@@ -6,6 +6,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <utility>
 #include <vector>
@@ -585,6 +586,63 @@ int mpd_device_init_interface(mpd_handle *h, const double *y0, int32_t sign, dou
     (void)hipFree(d_y0);
     if (e != hipSuccess) return fail(MPD_ERR_DEVICE, "interface start: %s", hipGetErrorString(e));
     return device_state_ready(h);
+}
+
+}  // extern "C"
+
+// ---- the allocation-free variant of mpd_device_init_interface ----
+// The per-column heights are staged in this file-scope device array instead of
+// a hipMalloc'ed buffer.  Each GPU holds its own copy at its own address, which
+// host code takes from hipGetSymbolAddress (the array's name in host code is a
+// host-side stand-in, not a device address).  Every call on one GPU shares that
+// GPU's copy: calls that overlap on one GPU race.  Each call overwrites the nx
+// values it uses, so nothing needs clearing.
+namespace {
+constexpr int NOALLOC_Y0_MAX = 16384;  // columns (128 KiB)
+__device__ __align__(256) double scratch[NOALLOC_Y0_MAX];
+constexpr int MAX_DEVICES = 64;
+std::mutex g_scratch_mu;
+double *g_scratch_ptr[MAX_DEVICES] = {};
+
+// address of `scratch` on the current device `dev` (looked up once per device)
+int scratch_on(int dev, double **out)
+{
+    if (dev < 0 || dev >= MAX_DEVICES) return fail(MPD_ERR_DEVICE, "device %d out of range", dev);
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void *p = nullptr;
+        hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            return fail(MPD_ERR_DEVICE, "hipGetSymbolAddress(scratch): %s", hipGetErrorString(e));
+        g_scratch_ptr[dev] = (double *)p;
+    }
+    *out = g_scratch_ptr[dev];
+    return 0;
+}
+}  // namespace
+
+extern "C" {
+
+int mpd_device_init_interface_noalloc(mpd_handle *h, const double *y0, int32_t sign, double p0)
+{
+    if (!h) return fail(MPD_ERR_INVALID, "null handle");
+    if (h->batch) return fail(MPD_ERR_STATE, "handle is a member of a batch");
+    if (is_pp(h->cfg) || !y0) return fail(MPD_ERR_INVALID, "interface needs an Allen-Cahn model and y0[nx]");
+    if (h->cfg.nx > NOALLOC_Y0_MAX)
+        return fail(MPD_ERR_INVALID, "mpd_device_init_interface_noalloc takes at most %d columns (nx = %d); "
+                                     "use mpd_device_init_interface", NOALLOC_Y0_MAX, h->cfg.nx);
+    int cur = 0;
+    HIPCHK(hipGetDevice(&cur));
+    if (cur != h->device) HIPCHK(hipSetDevice(h->device));
+    double *d_y0 = nullptr;
+    int r = scratch_on(h->device, &d_y0);
+    hipError_t e = hipSuccess;
+    if (!r) e = hipMemcpyAsync(d_y0, y0, sizeof(double) * (size_t)h->cfg.nx, hipMemcpyHostToDevice, h->stream);
+    if (!r && e == hipSuccess) devinit_interface(h->S, h->C, d_y0, sign, p0, h->cfg.width, h->stream);
+    if (!r && e == hipSuccess) r = device_state_ready(h);  // waits: the scratch is free again on return
+    else if (!r) r = fail(MPD_ERR_DEVICE, "interface start: %s", hipGetErrorString(e));
+    if (cur != h->device) (void)hipSetDevice(cur);
+    return r;
 }
 
 int mpd_device_init_pp_noise(mpd_handle *h, double rho0, double amp, uint64_t seed, double mu0)

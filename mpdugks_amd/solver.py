@@ -59,6 +59,7 @@ _SYMBOLS = {
     "mpd_device_init_drops": ([C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int32, C.c_double,
                                C.c_double], C.c_int),
     "mpd_device_init_interface": ([C.c_void_p, C.c_void_p, C.c_int32, C.c_double], C.c_int),
+    "mpd_device_init_interface_noalloc": ([C.c_void_p, C.c_void_p, C.c_int32, C.c_double], C.c_int),
     "mpd_device_init_pp_noise": ([C.c_void_p, C.c_double, C.c_double, C.c_uint64, C.c_double], C.c_int),
     "mpd_default_output_info": ([C.c_void_p, C.c_void_p], C.c_int),
     "mpd_output_name": ([C.c_int32, C.c_void_p, C.c_int64, C.c_double, C.c_char_p, C.c_int32], C.c_int),
@@ -224,6 +225,12 @@ class Solver:
     def init_interface(self, y0, sign=+1, p0=1.0):
         y = np.ascontiguousarray(y0, dtype=np.float64)
         _check(lib().mpd_device_init_interface(self._h, _p(y), int(sign), p0), "mpd_device_init_interface")
+
+    def init_interface_noalloc(self, y0, sign=+1, p0=1.0):
+        """init_interface through the entry point that allocates no device memory."""
+        y = np.ascontiguousarray(y0, dtype=np.float64)
+        _check(lib().mpd_device_init_interface_noalloc(self._h, _p(y), int(sign), p0),
+               "mpd_device_init_interface_noalloc")
 
     def init_pp_noise(self, rho0, amp=0.01, seed=0, mu0=0.1):
         _check(lib().mpd_device_init_pp_noise(self._h, rho0, amp, int(seed), mu0), "mpd_device_init_pp_noise")
