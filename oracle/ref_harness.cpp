@@ -12,7 +12,8 @@
 //   loop, `k` complete steps have been taken, and the whole cell state is
 //   written out before the step proceeds.
 //
-// Environment: MPD_REF_DUMP_STEPS="0,1,2,10" (steps to dump), MPD_REF_MAX_STEP
+// Environment: MPD_REF_DUMP_STEPS="0,1,2,10" (steps to dump), MPD_REF_TIME_STEPS
+// (steps at whose start a steady-clock timestamp goes to stderr), MPD_REF_MAX_STEP
 // (exit(0) when reached), MPD_REF_DUMP_DIR (output directory),
 // MPD_REF_WRITERS=1 (at each dumped step also call the reference's own
 // Output_Flowfield, Output_Diag, Output_MidX and Output_MidY, Ark/Cpp/Output.cpp,
@@ -21,6 +22,7 @@
 // Dump layout (little-endian): int32 nx, ny, nfields, then nfields planes of
 // nx*ny float64 in structured order idx = j*nx + i, with (i,j) taken from the
 // reference's own CarCellArray[i+1][j+1] (Ark/Cpp/Mesh/MeshCartesian.cpp:369-408).
+#include <chrono>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -41,6 +43,7 @@ namespace {
 bool g_started = false;
 int g_steps_seen = -1;
 std::set<int> g_dump;
+std::set<int> g_time;  // steps at whose start a timestamp is printed
 int g_max_step = -1;
 std::string g_dir = ".";
 
@@ -56,6 +59,16 @@ void parse_env()
             size_t q = v.find(',', p);
             if (q == std::string::npos) q = v.size();
             if (q > p) g_dump.insert(std::atoi(v.substr(p, q - p).c_str()));
+            p = q + 1;
+        }
+    }
+    if (const char *s = std::getenv("MPD_REF_TIME_STEPS")) {
+        std::string v(s);
+        size_t p = 0;
+        while (p < v.size()) {
+            size_t q = v.find(',', p);
+            if (q == std::string::npos) q = v.size();
+            if (q > p) g_time.insert(std::atoi(v.substr(p, q - p).c_str()));
             p = q + 1;
         }
     }
@@ -190,6 +203,10 @@ extern "C" void __wrap__Z13Update_phi_EqR7Cell_2D(Cell_2D &cell)
 {
     if (g_started && &cell == &CellArray[0]) {
         ++g_steps_seen;
+        if (g_time.count(g_steps_seen)) {
+            const double t = std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+            std::fprintf(stderr, "[ref_harness] step %d t %.6f\n", g_steps_seen, t);
+        }
         if (g_dump.count(g_steps_seen)) dump_cells(g_steps_seen);
         if (g_max_step >= 0 && g_steps_seen >= g_max_step) std::exit(0);
     }

@@ -42,12 +42,51 @@ CELL_SCALARS = ["Phi", "Rho", "U", "V", "p", "Phi_x", "Phi_y", "Rho_x", "Rho_y",
                 "Mu", "ftau", "faBP", "fbBP", "fcBP", "haBP", "hbBP", "hcBP", "xc", "yc"]
 
 
+# Configurations on another mesh: the reference's mesh size is compile-time
+# (const Nx, Ny, NL, ChLength, X_End, Y_End in Ark/Constant/ZeroReference.h).
+# build() writes a copy of that header with these constants replaced into
+# oracle/_ref/<cfg>/inc/ and force-includes it; its include guard then keeps
+# the original out.  Rayleigh-Taylor size, bounce-back walls in y, no gravity
+# (the reference has none, Ark/Cpp/SourceM.cpp:32-33): the CPU baseline of the
+# headline workload.
+MESH = {"hcz_rt": (256, 1024)}
+CONFIGS["hcz_rt"] = ("D2Q9HCZ", "-D_Wall_3_BCs_FLIP=3 -D_Wall_3_BCs_BB", True, "")
+
+
+def mesh_of(cfg):
+    return MESH.get(cfg, (128, 128))
+
+
+def _mesh_header(cfg):
+    """The reference's ZeroReference.h with the mesh constants of `cfg`
+    (unit cells: ChLength = NL = nx, so MinL = 1), written under oracle/_ref."""
+    import re
+    nx, ny = MESH[cfg]
+    with open(os.path.join(REF_ROOT, "Constant", "ZeroReference.h")) as fh:
+        text = fh.read()
+    subs = [(r"\bNx = \d+", "Nx = %d" % nx), (r"\bNy = \d+", "Ny = %d" % ny), (r"\bNL = \d+", "NL = %d" % nx),
+            (r"\bChLength = [0-9.]+", "ChLength = %d.0" % nx), (r"\bX_End = [0-9.]+", "X_End = %d" % nx),
+            (r"\bY_End = [0-9.]+", "Y_End = %d.0" % ny)]
+    for pat, rep in subs:
+        text, n = re.subn(pat, rep, text, count=1)
+        assert n == 1, pat
+    d = os.path.join(HERE, "_ref", cfg, "inc")
+    os.makedirs(d, exist_ok=True)
+    path = os.path.join(d, "ZeroReference.h")
+    if not os.path.exists(path) or open(path).read() != text:
+        with open(path, "w") as fh:
+            fh.write(text)
+    return path
+
+
 def available():
     return os.path.isdir(REF_ROOT)
 
 
 def build(cfg):
     model, defs, _, idir = CONFIGS[cfg]
+    if cfg in MESH:
+        defs = defs + " -include " + _mesh_header(cfg)
     cmd = ["make", "-s", "-f", os.path.join(HERE, "Makefile.ref"), "-j8",
            "CFG=" + cfg, "MODEL=" + model, "DEFS=" + defs, "IQUOTE=" + ("-I" + idir if idir else "")]
     subprocess.run(cmd, check=True, cwd=os.path.dirname(HERE))
@@ -65,7 +104,7 @@ def run(cfg, steps, max_step, outdir=None, flowfield=False, timeout=3600, writer
         os.makedirs(d, exist_ok=True)
     sys.path.insert(0, HERE)
     from casmesh import write_cas
-    write_cas(os.path.join(home, "Mesh", MESH_NAME + ".cas"), 128, 128,
+    write_cas(os.path.join(home, "Mesh", MESH_NAME + ".cas"), *mesh_of(cfg),
               walls_y=CONFIGS[cfg][2], cell_order=cell_order)
     outdir = outdir or os.path.join(base, "dump")
     if writers:  # fresh output tree: the convergence files are appended to
@@ -129,7 +168,7 @@ def time_prebuilt(cfg="hcz", steps=2000, threads=1, timeout=120):
         os.makedirs(d, exist_ok=True)
     sys.path.insert(0, HERE)
     from casmesh import write_cas
-    write_cas(os.path.join(home, "Mesh", MESH_NAME + ".cas"), 128, 128, walls_y=CONFIGS[cfg][2])
+    write_cas(os.path.join(home, "Mesh", MESH_NAME + ".cas"), *mesh_of(cfg), walls_y=CONFIGS[cfg][2])
     env = dict(os.environ)
     env.update({"HOME": home, "OMP_NUM_THREADS": str(threads), "MPD_REF_DUMP_STEPS": "",
                 "MPD_REF_MAX_STEP": str(steps + 1), "MPD_REF_DUMP_DIR": base, "MPD_REF_FLOWFIELD": "0",
@@ -145,3 +184,39 @@ def time_prebuilt(cfg="hcz", steps=2000, threads=1, timeout=120):
         if m:
             out.append((int(m.group(1)), int(m.group(2))))
     return out
+
+
+def time_window(cfg="hcz_rt", first=10, last=110, threads=1, timeout=300):
+    """Time steps first..last-1 of the already-built reference binary of `cfg`
+    (no rebuild) from the harness's step-start timestamps, excluding mesh
+    reading and set-up: returns (steps, seconds) or None."""
+    import re
+    import shutil
+    import tempfile
+    exe = os.path.join(HERE, "_ref", cfg, "ark")
+    if not os.path.exists(exe):
+        return None
+    base = tempfile.mkdtemp(prefix="mpd_refrun_")
+    try:
+        home = os.path.join(base, "home")
+        work = os.path.join(base, "work")
+        for d in (os.path.join(home, "Mesh"), work, os.path.join(base, "FlowField", "global"),
+                  os.path.join(base, "FlowField", "Convergence")):
+            os.makedirs(d, exist_ok=True)
+        sys.path.insert(0, HERE)
+        from casmesh import write_cas
+        write_cas(os.path.join(home, "Mesh", MESH_NAME + ".cas"), *mesh_of(cfg), walls_y=CONFIGS[cfg][2])
+        env = dict(os.environ)
+        env.update({"HOME": home, "OMP_NUM_THREADS": str(threads), "MPD_REF_DUMP_STEPS": "",
+                    "MPD_REF_TIME_STEPS": "%d,%d" % (first, last), "MPD_REF_MAX_STEP": str(last),
+                    "MPD_REF_DUMP_DIR": base, "MPD_REF_FLOWFIELD": "0", "MPD_REF_WRITERS": "0"})
+        with open(os.devnull) as dn:
+            r = subprocess.run([exe], cwd=work, env=env, stdin=dn, stdout=subprocess.DEVNULL,
+                               stderr=subprocess.PIPE, timeout=timeout)
+        t = {int(m.group(1)): float(m.group(2))
+             for m in re.finditer(r"\[ref_harness\] step (\d+) t ([0-9.]+)", r.stderr.decode(errors="replace"))}
+    finally:
+        shutil.rmtree(base, ignore_errors=True)
+    if first not in t or last not in t:
+        return None
+    return last - first, t[last] - t[first]

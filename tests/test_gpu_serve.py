@@ -1,0 +1,36 @@
+"""The serving loop (mpdugks_amd.serve) with the real GPU engine: Poisson
+two-droplet coalescence requests, admitted by an HBM budget small enough to
+make requests queue, stepped together in one batch, each retired at a kinetic-
+energy residual of 1e-5 (reachable in FP32-bulk, whose residual floor is
+1e-6..3e-5) or at the step cap.  A retired request's state equals the same case
+stepped alone on the FP32 fused path for the same number of steps, bit for bit."""
+import numpy as np
+import pytest
+
+from mpdugks_amd import abi, cases, serve, solver
+
+pytestmark = pytest.mark.gpu
+
+
+def test_server_with_gpu_engine():
+    reqs = serve.coalescence_requests(4, rate=40.0, seed=5, radii=(8.0, 10.0))
+    assert len(reqs) == 4 and all(r.params["side"] == 64 for r in reqs)
+    budget = int(2.5 * reqs[0].bytes_estimate)        # two resident at most: the others queue
+    eng = serve.GpuEngine(keep_final=True)
+    srv = serve.Server(eng, hbm_budget=budget, check_every=1000, ke_tol=1e-5, max_steps=60000)
+    stats = srv.run(reqs)
+    eng.close()
+    assert stats["completed"] == 4 and stats["rejected"] == 0
+    assert stats["max_active"] <= 2 and stats["queued_for_budget"] >= 1
+    assert stats["converged"] >= 1
+    assert stats["p99_latency_s"] >= stats["p50_latency_s"] > 0
+    # one retired request against the same case stepped alone
+    r = next(q for q in reqs if q.converged)
+    cfg, drops = r.make()
+    s = solver.Solver(cfg, path=abi.MPD_PATH_FUSED)
+    s.init_drops(drops, sign=-1)
+    s.step(r.steps)
+    alone = s.download()
+    s.close()
+    assert np.array_equal(alone.f, r.final.f) and np.array_equal(alone.h, r.final.h)
+    assert np.array_equal(alone.m, r.final.m)
