@@ -336,7 +336,8 @@ typedef struct mpd_output_info {
     double m_phi;             /* PhaseFieldAC::M_Phi, printed in profile zone titles */
     double rho_l, rho_v;      /* density ratio in profile file names */
     double temperature;       /* the T zone of the flow field (MacroQuantity::T = T0) */
-    int32_t nl;               /* NL, the mesh-size tag of file names */
+    int32_t mesh_nl;          /* NL, the mesh-size tag of file names (not `nl`: the reference
+                                 defines `nl` as a macro, Ark/Header/ZeroMacros.h:19) */
     char mesh_type[32];       /* _MESHTYPE_ARK */
     char mesh_file[128];      /* _MESHFILE_NAME_ARK */
 } mpd_output_info;
@@ -369,7 +370,7 @@ typedef struct mpd_case_info {
     char velocity_model[16], force_model[32], eos_model[32];
     int32_t zone_left, zone_right, zone_top, zone_bottom;
     double omega0, prandtl, nk, cv, gamma;
-    int32_t qu, qv, nl;
+    int32_t qu, qv, mesh_nl;  /* Qu, Qv, NL */
     double ch_length, min_l, kn, ma, re, u0, r0, t0, mu0, nu0, tau0, lambda0, cfl, dt, residual;
     int32_t end_step;
     double maspan, eta;

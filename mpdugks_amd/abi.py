@@ -59,7 +59,7 @@ MPD_OUT_FLOWFIELD, MPD_OUT_SUMRHO, MPD_OUT_RESIDUAL, MPD_OUT_MIDX, MPD_OUT_MIDY,
 class MpdOutputInfo(C.Structure):
     _fields_ = [("ma", C.c_double), ("cfl", C.c_double), ("mu0", C.c_double), ("re", C.c_double),
                 ("m_phi", C.c_double), ("rho_l", C.c_double), ("rho_v", C.c_double),
-                ("temperature", C.c_double), ("nl", C.c_int32), ("mesh_type", C.c_char * 32),
+                ("temperature", C.c_double), ("mesh_nl", C.c_int32), ("mesh_type", C.c_char * 32),
                 ("mesh_file", C.c_char * 128)]
 
 
@@ -69,7 +69,7 @@ class MpdCaseInfo(C.Structure):
                                                 ("force_model", 32), ("eos_model", 32))]
                 + [(k, C.c_int32) for k in ("zone_left", "zone_right", "zone_top", "zone_bottom")]
                 + [(k, C.c_double) for k in ("omega0", "prandtl", "nk", "cv", "gamma")]
-                + [(k, C.c_int32) for k in ("qu", "qv", "nl")]
+                + [(k, C.c_int32) for k in ("qu", "qv", "mesh_nl")]
                 + [(k, C.c_double) for k in ("ch_length", "min_l", "kn", "ma", "re", "u0", "r0", "t0", "mu0",
                                              "nu0", "tau0", "lambda0", "cfl", "dt", "residual")]
                 + [("end_step", C.c_int32)]

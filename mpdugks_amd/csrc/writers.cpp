@@ -30,7 +30,7 @@ std::string tag(const mpd_output_info &o)
 {
     // "Ma<Ma>_<mesh type><NL>_CFL<CFL>" (Output.cpp:319-320,487-488)
     std::ostringstream s;
-    s << "Ma" << o.ma << "_" << o.mesh_type << o.nl << "_CFL" << o.cfl;
+    s << "Ma" << o.ma << "_" << o.mesh_type << o.mesh_nl << "_CFL" << o.cfl;
     return s.str();
 }
 
@@ -76,7 +76,7 @@ int mpd_default_output_info(const mpd_config *cfg, mpd_output_info *o)
     o->rho_l = c.rho_l;
     o->rho_v = c.rho_v;
     o->temperature = c.rt;                   // T0 with R0 = 1
-    o->nl = c.nx;
+    o->mesh_nl = c.nx;
     copy_str(o->mesh_type, sizeof(o->mesh_type), "Car");
     copy_str(o->mesh_file, sizeof(o->mesh_file),
              std::to_string(c.nx) + "_" + std::to_string(c.ny) + "_Car_Periodic_Square_LBM");
@@ -114,7 +114,7 @@ int mpd_default_case_info(const mpd_config *cfg, mpd_case_info *ci)
     ci->gamma = (nk + 5.0) / (nk + 3.0);
     ci->qu = 1;
     ci->qv = v16 ? 16 : 9;
-    ci->nl = c.nx;
+    ci->mesh_nl = c.nx;
     ci->ch_length = c.nx * c.dx;
     ci->min_l = c.dx;
     const double r0 = 1.0, t0 = c.rt / r0, rho0 = 1.0, lambda0 = 1 / (2.0 * r0 * t0);
@@ -299,7 +299,7 @@ int mpd_write_case(const char *path, const mpd_case_info *c)
       << "#----------------Reference----------------" << std::endl
       << c->qu << e << "Qu" << std::endl
       << c->qv << e << "Qv" << std::endl
-      << c->nl << e << "NL" << std::endl
+      << c->mesh_nl << e << "NL" << std::endl
       << c->ch_length << e << "Characteristic Length" << std::endl
       << c->min_l << e << "MinL" << std::endl
       << c->kn << e << "Knudsen" << std::endl

@@ -166,3 +166,25 @@ def test_face_numbering_matches_reference_topology(oracle_mod):
         assert rank(d, i, j) == k
         for s, (di, dj) in offs.items():
             assert rank(d, *step(d, i, j, di, dj)) == faces[k]["ff"][s], (k, s)
+
+
+def test_integration_snippet_compiles_against_reference_headers(tmp_path):
+    """The driver-side replacement of DUGKS2DSolver() shown in INTEGRATION.md
+    compiles (g++ -fsyntax-only) against the reference's own headers and
+    include/mpdugks.h.  The prelude declares what the reference's
+    DUGKS2DSolver.cpp itself declares at file scope."""
+    import re
+    import subprocess
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    text = open(os.path.join(root, "INTEGRATION.md")).read()
+    block = next(b for b in re.findall(r"```cpp\n(.*?)```", text, re.S) if "void DUGKS2DSolver()" in b)
+    prelude = ("#include \"DUGKSDeclaration.h\"\n"
+               "static double ResidualPer1k = 1.0;\n"
+               "void Output_Flowfield(double const &t, int step);\n"
+               "void Output_Diag(int step);\nvoid Output_MidX(int step);\n")
+    src = tmp_path / "driver.cpp"
+    src.write_text(prelude + block)
+    ark = ref_run.REF_ROOT
+    subprocess.run(["g++", "-std=c++11", "-fsyntax-only", "-w", "-I" + os.path.join(ark, "Header"),
+                    "-I" + os.path.join(ark, "Lib"), "-I" + os.path.join(ark, "Constant"),
+                    "-I" + os.path.join(root, "include"), str(src)], check=True)
