@@ -32,6 +32,10 @@ CONFIGS = {
     "acp_uw": ("D2Q9AC", "-include " + os.path.join(CFGDIR, "uw", "flags.h"), False, ""),
     "pp": ("D2Q9PP", "-include " + os.path.join(CFGDIR, "pp", "flags.h"), False,
            os.path.join(CFGDIR, "pp")),
+    # pseudopotential with bounce-back walls in y (the reference cannot run it: with walls
+    # on, CreatMesh.h:11-13 skips CarfaceFacesConstruct, so the face force stencil is null)
+    "pp_wall": ("D2Q9PP", "-include " + os.path.join(CFGDIR, "pp", "flags.h") +
+                " -D_Wall_3_BCs_FLIP=3 -D_Wall_3_BCs_BB", True, os.path.join(CFGDIR, "pp")),
     # SCMP_Drop shifted by (60, 70) cells: the drop crosses both periodic lines
     "pp_shift": ("D2Q9PP", "-include " + os.path.join(CFGDIR, "pp", "flags.h"), False,
                  os.path.join(CFGDIR, "pp_shift")),

@@ -188,3 +188,21 @@ def test_integration_snippet_compiles_against_reference_headers(tmp_path):
     subprocess.run(["g++", "-std=c++11", "-fsyntax-only", "-w", "-I" + os.path.join(ark, "Header"),
                     "-I" + os.path.join(ark, "Lib"), "-I" + os.path.join(ark, "Constant"),
                     "-I" + os.path.join(root, "include"), str(src)], check=True)
+
+
+def test_reference_pseudopotential_with_walls_cannot_run():
+    """The library refuses pseudopotential handles with walls (mpd_create:
+    MPD_ERR_INVALID), as the reference cannot run them: with walls on,
+    CreatMesh.h:11-13 skips CarfaceFacesConstruct, so every face's faceFaces
+    stays null and the face force (GradSchemeBasic.h:181-208) dereferences it.
+    The reference built with the pseudopotential flags and bounce-back walls
+    dies with SIGSEGV in its first step."""
+    import subprocess
+    with pytest.raises(subprocess.CalledProcessError) as e:
+        ref_run.run("pp_wall", [1], 1, timeout=120)
+    assert e.value.returncode == -11
+    cfg = default_config(abi.MPD_MODEL_PP_D2Q9, nx=32, ny=32)
+    cfg.bc_y = abi.MPD_BC_WALL_BB
+    import ctypes as C
+    from mpdugks_amd import solver
+    assert solver.lib().mpd_create(C.byref(cfg), C.byref(C.c_void_p())) == -1
