@@ -103,16 +103,43 @@ def test_fp32_bulk_laplace_and_spurious_currents(runs, k):
     assert abs(o["sum_phi"] / g["sum_phi"] - 1) < 1e-9
 
 
+def _perturbed_fp64(seed, steps):
+    """The FP64 fused run (bit-exact with the oracle) from the start state with
+    every population multiplied by 1 + 6e-8 U(-1, 1): the problem's own
+    sensitivity to a round-off-sized difference."""
+    cfg = default_config(abi.MPD_MODEL_AC_HCZ, nx=128, ny=128, sigma=1e-3, width=4.0)
+    st = solver.ac_drop(cfg, cx=64.0, cy=64.0, radius=32.0, sign=-1)
+    rng = np.random.default_rng(seed)
+    st.h = st.h * (1 + 6e-8 * rng.uniform(-1, 1, st.h.shape))
+    st.f = st.f * (1 + 6e-8 * rng.uniform(-1, 1, st.f.shape))
+    s = solver.Solver(cfg, path=abi.MPD_PATH_FUSED)
+    s.upload(st)
+    out, done = {}, 0
+    for k in steps:
+        s.step(k - done)
+        done = k
+        out[k] = s.download().m
+    s.close()
+    return out
+
+
 def test_fp32_bulk_fields_vs_fp64(runs):
+    """rho, phi, p of FP32-bulk against FP64: 1e-5 relative L2 after 10^3 steps;
+    after 10^4, twice the largest distance of three perturbed FP64 runs (seeds
+    1, 2, 3; the golden record holds seed 1's, computed by the oracle), or 1e-5."""
     def rel(a, b):
         return float(np.sqrt(((a - b) ** 2).sum()) / np.sqrt((b ** 2).sum()))
+    pert = [_perturbed_fp64(seed, (1000, 10000)) for seed in (1, 2, 3)]
     for k in (1000, 10000):
         a, b = runs[(abi.MPD_FP32_BULK, k)], runs[(abi.MPD_FP64, k)]
-        floor = gold()[str(k)]["perturbed_rel_l2"]
         for name in ("rho", "phi", "p"):
             e = rel(a[abi.M[name]], b[abi.M[name]])
-            bound = 1e-5 if k == 1000 else max(1e-5, 2 * floor[name])
-            print("fp32 droplet step %d %s: rel L2 %.3e (bound %.3e, perturbed FP64 %.3e)" % (k, name, e, bound, floor[name]))
+            floors = [rel(q[k][abi.M[name]], b[abi.M[name]]) for q in pert]
+            if k == 10000:   # seed 1 on the GPU equals the oracle's golden value
+                assert abs(floors[0] / gold()[str(k)]["perturbed_rel_l2"][name] - 1) < 1e-6
+            bound = 1e-5 if k == 1000 else max(1e-5, 2 * max(floors))
+            print("fp32 droplet step %d %s: rel L2 %.3e (bound %.3e, perturbed FP64 %s)"
+                  % (k, name, e, bound, " ".join("%.3e" % f for f in floors)))
             assert e < bound, (k, name, e, bound)
 
 

@@ -10,7 +10,8 @@ a start perturbed at FP32 round-off size) and checks what the flow does:
 * kinetic energy within 1e-3 relative;
 * sum(phi) within 1e-10 relative (conserved by the scheme; the FP64 path's own
   rounding moves it by ~1e-12 over the run, FP32-bulk carries phi in FP64);
-* relative L2 of phi within 3x the FP64-perturbed distance, or 1e-5.
+* relative L2 of phi within 2x the FP64-perturbed distance (the largest over
+  three perturbation seeds), or 1e-5.
 """
 import numpy as np
 import pytest
@@ -22,10 +23,10 @@ pytestmark = pytest.mark.gpu
 STEPS = (5000, 10000, 20000)
 
 
-def _run(precision, perturb=False):
+def _run(precision, perturb=None):
     cfg, st = cases.rayleigh_taylor(nx=256, ny=1024, precision=precision)
-    if perturb:
-        rng = np.random.default_rng(1)
+    if perturb is not None:
+        rng = np.random.default_rng(perturb)
         st.h = st.h * (1 + 6e-8 * rng.uniform(-1, 1, st.h.shape))
         st.f = st.f * (1 + 6e-8 * rng.uniform(-1, 1, st.f.shape))
     s = solver.Solver(cfg, path=abi.MPD_PATH_FUSED)
@@ -51,14 +52,17 @@ def _rel(a, b):
 
 @pytest.fixture(scope="module")
 def runs():
-    return {"fp64": _run(abi.MPD_FP64), "fp32": _run(abi.MPD_FP32_BULK),
-            "fp64_perturbed": _run(abi.MPD_FP64, perturb=True)}
+    out = {"fp64": _run(abi.MPD_FP64), "fp32": _run(abi.MPD_FP32_BULK)}
+    for seed in (1, 2, 3):
+        out["fp64_perturbed_%d" % seed] = _run(abi.MPD_FP64, perturb=seed)
+    return out
 
 
 @pytest.mark.parametrize("k", STEPS)
 def test_rt_observables_fp32_vs_fp64(runs, k):
-    a, b, c = runs["fp32"][k], runs["fp64"][k], runs["fp64_perturbed"][k]
-    e, floor = _rel(a["phi"], b["phi"]), _rel(c["phi"], b["phi"])
+    a, b = runs["fp32"][k], runs["fp64"][k]
+    e = _rel(a["phi"], b["phi"])
+    floor = max(_rel(runs["fp64_perturbed_%d" % seed][k]["phi"], b["phi"]) for seed in (1, 2, 3))
     print("rt step %d: spike %d/%d bubble %d/%d KE rel %.2e sum_phi rel %.2e phi L2 %.2e (perturbed FP64 %.2e)"
           % (k, a["spike"], b["spike"], a["bubble"], b["bubble"], abs(a["ke"] / b["ke"] - 1),
              abs(a["sum_phi"] / b["sum_phi"] - 1), e, floor))
@@ -66,7 +70,7 @@ def test_rt_observables_fp32_vs_fp64(runs, k):
     assert abs(a["spike"] - b["spike"]) <= 1 and abs(a["bubble"] - b["bubble"]) <= 1
     assert abs(a["ke"] / b["ke"] - 1) < 1e-3
     assert abs(a["sum_phi"] / b["sum_phi"] - 1) < 1e-10
-    assert e < max(3 * floor, 1e-5), (e, floor)
+    assert e < max(2 * floor, 1e-5), (e, floor)
 
 
 def test_rt_mixing_layer_grows(runs):
