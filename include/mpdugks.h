@@ -47,7 +47,9 @@ typedef enum mpd_model {
     MPD_MODEL_PP_D2V16 = 3   /* Ark/Lib/D2V16.cpp   : 16-velocity set with pseudopotential */
 } mpd_model;
 
-/* Face reconstruction: Ark/Constant/ZeroFlip.h:27-35. */
+/* Face reconstruction: Ark/Constant/ZeroFlip.h:27-35.  CD with every model;
+ * UW with the Allen-Cahn models (periodic or walled y; FP32-bulk: periodic y)
+ * and the pseudopotential D2Q9 model, not with D2V16. */
 typedef enum mpd_flux { MPD_FLUX_CD = 0, MPD_FLUX_UW = 1 } mpd_flux;
 
 /* Boundary in y (x is always periodic): Ark/Constant/ZeroFlip.h:59-77. */
@@ -170,7 +172,8 @@ int mpd_upload_face_psi(mpd_handle *h, const double *psi);
 int mpd_download_face_psi(mpd_handle *h, double *psi);
 
 /* Tile shape of the one-launch step kernels (cells per workgroup): the
- * FP32-bulk Allen-Cahn kernel has all three, the FP32-bulk pseudopotential
+ * FP32-bulk Allen-Cahn kernel has all three with CD faces and 16x15 with UW
+ * faces (a 3-cell halo), the FP32-bulk pseudopotential
  * kernel 16x15 and 32x16, the FP64 Allen-Cahn kernel 16x15 (other shapes are
  * refused with MPD_ERR_INVALID).
  * MPD_TILE_AUTO (the default) chooses per mesh; the others force one.  The

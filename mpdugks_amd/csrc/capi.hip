@@ -253,14 +253,14 @@ int check_cfg(const mpd_config *c)
     if (c->flux != MPD_FLUX_CD && c->flux != MPD_FLUX_UW) return fail(MPD_ERR_INVALID, "unknown flux %d", c->flux);
     if (c->bc_y != MPD_BC_PERIODIC && c->bc_y != MPD_BC_WALL_BB && c->bc_y != MPD_BC_WALL_NEE)
         return fail(MPD_ERR_INVALID, "unknown bc %d", c->bc_y);
-    if (c->bc_y != MPD_BC_PERIODIC && (c->flux == MPD_FLUX_UW || c->model == MPD_MODEL_PP_D2Q9))
-        return fail(MPD_ERR_INVALID, "walls are supported with CD and the Allen-Cahn models only");
+    if (c->bc_y != MPD_BC_PERIODIC && c->model == MPD_MODEL_PP_D2Q9)
+        return fail(MPD_ERR_INVALID, "walls are supported with the Allen-Cahn models only");
     if (c->precision != MPD_FP64 && c->precision != MPD_FP32_BULK)
         return fail(MPD_ERR_INVALID, "unknown precision %d", c->precision);
     if (c->precision == MPD_FP32_BULK && !fused_supported(*c))
         return fail(MPD_ERR_INVALID, "no FP32-bulk kernel for this model / flux / boundary (FP32-bulk: Allen-Cahn "
-                                     "with CD faces and periodic or bounce-back y; pseudopotential D2Q9 with CD "
-                                     "faces, periodic)");
+                                     "with CD faces and periodic, bounce-back or NEE y, or UW faces on a periodic "
+                                     "mesh; pseudopotential D2Q9 with CD faces, periodic)");
     if (c->precision == MPD_FP32_BULK && !is_pp(*c)) {
         // the FP32 step kernel addresses each state set through one 32-bit buffer
         // resource (all float planes) and the FP64 phase field through another
@@ -441,6 +441,8 @@ int mpd_set_tile(mpd_handle *h, int32_t tile)
         return fail(MPD_ERR_INVALID, "the pseudopotential step has 16x15 and 32x16 tiles");
     if (!is_pp(h->cfg) && h->cfg.precision == MPD_FP64 && tile != MPD_TILE_AUTO && tile != MPD_TILE_16X15)
         return fail(MPD_ERR_INVALID, "the FP64 step has 16x15 tiles");
+    if (h->cfg.flux == MPD_FLUX_UW && tile != MPD_TILE_AUTO && tile != MPD_TILE_16X15)
+        return fail(MPD_ERR_INVALID, "the UW step has 16x15 tiles");
     h->tile = tile;
     return 0;
 }
@@ -450,7 +452,7 @@ int mpd_get_tile(mpd_handle *h, int32_t *tile)
     if (!h || !tile) return fail(MPD_ERR_INVALID, "null argument");
     if (h->tile) *tile = h->tile;
     else if (is_pp(h->cfg)) *tile = pp32_tile_auto(h->cfg.nx, h->cfg.ny, h->cus);
-    else if (h->cfg.precision == MPD_FP64) *tile = MPD_TILE_16X15;
+    else if (h->cfg.precision == MPD_FP64 || h->cfg.flux == MPD_FLUX_UW) *tile = MPD_TILE_16X15;
     else *tile = fused32_tile_auto(h->cfg.nx, h->cfg.ny, h->cus, fused32_state_bytes(h->cfg));
     return 0;
 }
@@ -470,6 +472,8 @@ static int alloc_phase_scratch(mpd_handle *h)
         (r = dalloc(h, &S.gm, h->C.walls ? (size_t)MPD_M_COUNT * 2 * h->cfg.nx : 0)) ||
         (r = dalloc(h, &S.ghbp, h->C.walls ? (size_t)Q9 * 2 * h->cfg.nx : 0)) ||
         (r = dalloc(h, &S.gfbp, h->C.walls ? (size_t)Q9 * 2 * h->cfg.nx : 0)) ||
+        (r = dalloc(h, &S.cbhl, ac && uw ? (size_t)Q9 * (h->cfg.nx + h->cfg.ny) : 0)) ||
+        (r = dalloc(h, &S.cbfl, ac && uw ? (size_t)Q9 * (h->cfg.nx + h->cfg.ny) : 0)) ||
         (r = dalloc(h, &S.fpsi, pp ? 2 * N : 0)) || (r = dalloc(h, &S.fscr, pp ? 8 * N : 0)))
         return r;
     return 0;
@@ -898,7 +902,8 @@ int mpd_batch_add(mpd_batch *b, mpd_handle *h)
 {
     if (!b || !h) return fail(MPD_ERR_INVALID, "null argument");
     if (h->batch) return fail(MPD_ERR_STATE, "handle already belongs to a batch");
-    if (h->cfg.precision != MPD_FP32_BULK || !fused_supported(h->cfg) || is_pp(h->cfg))
+    if (h->cfg.precision != MPD_FP32_BULK || !fused_supported(h->cfg) || is_pp(h->cfg) ||
+        h->cfg.flux != MPD_FLUX_CD)
         return fail(MPD_ERR_INVALID, "batches take FP32-bulk cases of the fused path (Allen-Cahn, CD)");
     if (!h->fused_primed || h->effective_path != MPD_PATH_FUSED)
         return fail(MPD_ERR_STATE, "step the case at least twice on the fused path before adding it");

@@ -17,6 +17,8 @@ struct DevState {
     double *hbpx, *hbpy, *fbpx, *fbpy; // [Q][N] UW gradients
     double *hfl, *ffl;            // face fluxes: x-faces [Q][N] then y-faces [Q][nyf*nx]
     double *gm, *ghbp, *gfbp;     // wall ghost rows
+    double *cbhl, *cbfl;          // Allen-Cahn UW: fluxes of the second copies of the periodic
+                                  // faces [Q][nx + ny] (x-faces by row, then y-faces by column)
     double *fpsi, *fscr;          // PP face Psi [2N], face scratch [4][2N]
     double *fpsi_old;             // PP face Psi of the previous step [2N] (0 before the first step)
     double *rho1k;                // density of the last residual evaluation

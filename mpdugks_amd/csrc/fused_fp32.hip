@@ -44,24 +44,28 @@ constexpr bool PRELOAD_RING = false;       // also hoist the second-pass loads a
 // not multiples of 32 float2 (64 dwords), or the compiler would pair reads of
 // two planes at one site into one two-address instruction whose two
 // addresses share a bank.
-template <int TX_, int TY_>
+// H is the halo of the bar-plus region: 1 for CD faces; 2 for UW faces, whose
+// upwind cell can be a halo cell and needs the 9-point gradient there.  The phi
+// region has one cell more (the sources of every bar-plus cell).
+template <int TX_, int TY_, int H_ = 1>
 struct TileK {
-    static constexpr int TX = TX_, TY = TY_;
+    static constexpr int TX = TX_, TY = TY_, H = H_;
     static constexpr int NT = TX * TY > 256 ? 512 : 256;   // threads
-    static constexpr int R1X = TX + 2, R1Y = TY + 2;       // bar-plus region (one-cell halo)
-    static constexpr int R2X = TX + 4, R2Y = TY + 4;       // phi region (two-cell halo)
+    static constexpr int R1X = TX + 2 * H, R1Y = TY + 2 * H;  // bar-plus region
+    static constexpr int R2X = R1X + 2, R2Y = R1Y + 2;        // phi region
     static constexpr int R1N = R1X * R1Y, R2N = R2X * R2Y;
     static constexpr int NIN = TX * TY;                    // interior cells
     static constexpr int NRING = R1N - NIN;                // halo ring of the bar-plus region
     static constexpr int NXF = (TX + 1) * TY;              // x-faces
     static constexpr int NYF = TX * (TY + 1);              // y-faces
-    static constexpr int PS = TX == 16 ? 324 : 612;
+    static constexpr int PS = H == 2 ? 388 : (TX == 16 ? 324 : 612);
     static constexpr int FS = TX == 32 ? 548 : (TY == 15 ? 260 : 276);
     static constexpr int EXF = NXF - NT, EYF = NYF - NT;   // second-pass faces
     static constexpr int NE = 32;                          // second-pass slots per direction
     static constexpr int XE0 = NT - 128, YE0 = NT - 64;    // first thread of the x / y second pass
     static constexpr int NBIG = NBP * PS > 2 * NFL * FS ? NBP * PS : 2 * NFL * FS;  // s_big, float2
     static_assert((TX == 16 && (TY == 15 || TY == 16)) || (TX == 32 && TY == 16), "tile shapes");
+    static_assert(H == 1 || (H == 2 && TX == 16 && TY == 15), "UW faces: 16x15 tiles");
     static_assert(R1N <= PS && NXF <= FS && NYF <= FS, "plane strides");
     static_assert(PS % 32 != 0 && FS % 32 != 0, "plane strides");
     static_assert(NIN <= NT && NRING - (NT - NIN) <= NT, "stage-B mapping");
@@ -82,6 +86,7 @@ struct K32 {
     float cbpw0[3], chw0[3];
     float w[3];                       // weights rest / axis / diagonal, w0 = 1 - 4 w1 - 4 w2
     float hneq, fneq0, hdx;           // NEE walls: 2 tau/(2 tau + dt) of h (and of f, AC-p), dx/2
+    float uw_n, uw_t, uw_0;           // UW faces: (1/2 - A)/12, A/12, 1/24 (A = hdt MaSpan / dx)
 };
 
 // one case of a batched launch (device table entry)
@@ -191,16 +196,16 @@ struct Col {
 template <class G>
 __device__ __forceinline__ Col col1(int c)
 {
-    constexpr int TX = G::TX;
-    const bool in = c >= 1 && c <= TX;
-    return Col{in ? TX : 2, in ? c - 1 : TX * G::R1Y + (c > TX ? 1 : 0)};
+    constexpr int TX = G::TX, H = G::H;
+    const bool in = c >= H && c < H + TX;
+    return Col{in ? TX : 2 * H, in ? c - H : TX * G::R1Y + (c < H ? c : c - TX)};
 }
 template <class G>
 __device__ __forceinline__ Col col2(int c)
 {
-    constexpr int TX = G::TX;
-    const bool in = c >= 2 && c <= TX + 1;
-    return Col{in ? TX : 4, in ? c - 2 : TX * G::R2Y + (c < 2 ? c : c - TX)};
+    constexpr int TX = G::TX, H2 = G::H + 1;
+    const bool in = c >= H2 && c < H2 + TX;
+    return Col{in ? TX : 2 * H2, in ? c - H2 : TX * G::R2Y + (c < H2 ? c : c - TX)};
 }
 __device__ __forceinline__ int at(Col c, int row) { return row * c.pitch + c.off; }
 template <class G>
@@ -301,12 +306,12 @@ struct CellIn {
     int gi, gj;
     bool valid;
 };
-template <int MODEL, int WALLS>
+template <int MODEL, int WALLS, int H>
 __device__ __forceinline__ void cell_load(const Rs &in, int nx, int ny, bool small, int i0, int j0, int rx, int ry,
                                           CellIn &x)
 {
-    x.gi = wrap(i0 - 1 + rx, nx, small);
-    int gj = j0 - 1 + ry;
+    x.gi = wrap(i0 - H + rx, nx, small);
+    int gj = j0 - H + ry;
     x.valid = !(WALLS && (gj < 0 || gj >= ny));  // wall ghost rows: extrapolated afterwards
     x.gj = gj = WALLS ? gj : wrap(gj, ny, small);
     x.n = (uint32_t)(gj * nx + x.gi);
@@ -525,6 +530,91 @@ __device__ __forceinline__ float2 recon(const K32 &k, const float2 *b, const Fac
     return v;
 }
 
+// Upwind faces (UW_Interior_phi_Bh + Update_phi_fBh, DUGKS2DSolver.cpp:560-603,
+// 688-706; gradients Grad_VS_6points, GradSchemeBasic.h:95-126):
+// f(x_f) = b_c + (x_f - h xi - x_c) . grad b_c with c the upwind cell (owner for
+// e_n > 0, neighbour for e_n < 0, the mean of both for e_n = 0), and
+// grad b = (4 (E - W) + (NE - NW + SE - SW), 4 (N - S) + (NE - SE + NW - SW)) / (12 dx).
+// The two cells' 3x3 neighbourhoods: x-face 4 columns x 3 rows (a[c * 3 + r],
+// owner centre (1, 1), neighbour (2, 1)); y-face 3 columns x 4 rows
+// (a[c * 4 + r], owner (1, 1), neighbour (1, 2)).
+struct FaceIdxUW {
+    int o, n;
+    int a[12];
+};
+template <int DIR, class G>
+__device__ __forceinline__ FaceIdxUW face_idx_uw(int oc, int orow)
+{
+    FaceIdxUW x;
+    constexpr int NC = DIR == 0 ? 4 : 3, NR = DIR == 0 ? 3 : 4;
+#pragma unroll
+    for (int c = 0; c < NC; ++c) {
+        const Col cc = col1<G>(oc - 1 + c);
+#pragma unroll
+        for (int r = 0; r < NR; ++r) x.a[c * NR + r] = opaque(at(cc, orow - 1 + r));
+    }
+    x.o = x.a[1 * NR + 1];
+    x.n = DIR == 0 ? x.a[2 * NR + 1] : x.a[1 * NR + 2];
+    return x;
+}
+// normal and tangential gradient sums (x 12 dx) of one plane around the
+// centre (CC, CR) of a face neighbourhood
+template <int DIR, int CC, int CR>
+__device__ __forceinline__ void grad_uw(const float2 *b, const FaceIdxUW &x, float2 &gn, float2 &gt)
+{
+    constexpr int NR = DIR == 0 ? 3 : 4;
+    auto v = [&](int dc, int dr) { return b[x.a[(CC + dc) * NR + (CR + dr)]]; };
+    const float2 e = v(1, 0), w = v(-1, 0), n = v(0, 1), s = v(0, -1);
+    const float2 ne = v(1, 1), nw = v(-1, 1), sw = v(-1, -1), se = v(1, -1);
+    float2 gx, gy;
+    gx.x = 4.f * (e.x - w.x) + ((ne.x - nw.x) + (se.x - sw.x));
+    gx.y = 4.f * (e.y - w.y) + ((ne.y - nw.y) + (se.y - sw.y));
+    gy.x = 4.f * (n.x - s.x) + ((ne.x - se.x) + (nw.x - sw.x));
+    gy.y = 4.f * (n.y - s.y) + ((ne.y - se.y) + (nw.y - sw.y));
+    gn = DIR == 0 ? gx : gy;
+    gt = DIR == 0 ? gy : gx;
+}
+template <int DIR, int Q>
+__device__ __forceinline__ float2 recon_uw(const K32 &k, const float2 *b, const FaceIdxUW &x)
+{
+    constexpr int EN = DIR == 0 ? ex9(Q) : ey9(Q), ET = DIR == 0 ? ey9(Q) : ex9(Q);
+    constexpr int OC = 1, OR = 1, NC = DIR == 0 ? 2 : 1, NR = DIR == 0 ? 1 : 2;
+    const float ct = ET == 0 ? 0.f : (ET > 0 ? -k.uw_t : k.uw_t);   // -h xi_t / (12 dx)
+    float2 v;
+    if (EN != 0) {
+        float2 gn, gt;
+        if (EN > 0) grad_uw<DIR, OC, OR>(b, x, gn, gt);
+        else grad_uw<DIR, NC, NR>(b, x, gn, gt);
+        const float2 c = b[EN > 0 ? x.o : x.n];
+        const float cn = EN > 0 ? k.uw_n : -k.uw_n;                  // (x_f - h xi_n - x_c) / (12 dx)
+        v.x = fmaf(cn, gn.x, c.x);
+        v.y = fmaf(cn, gn.y, c.y);
+        if (ET != 0) {
+            v.x = fmaf(ct, gt.x, v.x);
+            v.y = fmaf(ct, gt.y, v.y);
+        }
+    } else {
+        float2 gno, gto, gnn, gtn;
+        grad_uw<DIR, OC, OR>(b, x, gno, gto);
+        grad_uw<DIR, NC, NR>(b, x, gnn, gtn);
+        const float2 bo = b[x.o], bn = b[x.n];
+        float2 vo, vn;
+        vo.x = fmaf(k.uw_0, gno.x, bo.x);
+        vo.y = fmaf(k.uw_0, gno.y, bo.y);
+        vn.x = fmaf(-k.uw_0, gnn.x, bn.x);
+        vn.y = fmaf(-k.uw_0, gnn.y, bn.y);
+        if (ET != 0) {
+            vo.x = fmaf(ct, gto.x, vo.x);
+            vo.y = fmaf(ct, gto.y, vo.y);
+            vn.x = fmaf(ct, gtn.x, vn.x);
+            vn.y = fmaf(ct, gtn.y, vn.y);
+        }
+        v.x = 0.5f * (vo.x + vn.x);
+        v.y = 0.5f * (vo.y + vn.y);
+    }
+    return v;
+}
+
 // Indices of the face between R1 cells (oc, orow) and its neighbour in
 // direction DIR, with the tangential neighbours of both (LDS layout above).
 template <int DIR, class G>
@@ -555,19 +645,21 @@ __device__ __forceinline__ FaceIdx face_idx(int oc, int orow)
 // D2Q9AC.cpp:232-329), BGK half step (Update_phi_h, DUGKS2DSolver.cpp:402-423)
 // and micro-flux (Update_phiFlux_h, :424-441) of the six populations with a
 // non-zero normal velocity, in pair order (first member e_n = +1).
-template <int DIR, int MODEL, int WALLS, class G>
-__device__ __forceinline__ void face(const K32 &k, const float2 *S, const FaceIdx &x, int wall, float *hfl,
+template <int DIR, int MODEL, int WALLS, int FLUX, class G, class X>
+__device__ __forceinline__ void face(const K32 &k, const float2 *S, const X &x, int wall, float *hfl,
                                      float *ffl, float &mass)
 {
     constexpr bool HCZ = MODEL == MPD_MODEL_AC_HCZ;
     constexpr int PS = G::PS;
     const int ko = x.o;
     float hv[9], fv[9];
-#define MPD_RECON(Q)                                      \
-    {                                                     \
-        const float2 v = recon<DIR, Q>(k, S + Q * PS, x); \
-        hv[Q] = v.x;                                      \
-        fv[Q] = v.y;                                      \
+#define MPD_RECON(Q)                                                                             \
+    {                                                                                            \
+        float2 v;                                                                                \
+        if constexpr (FLUX == MPD_FLUX_UW) v = recon_uw<DIR, Q>(k, S + Q * PS, x);               \
+        else v = recon<DIR, Q>(k, S + Q * PS, x);                                                \
+        hv[Q] = v.x;                                                                             \
+        fv[Q] = v.y;                                                                             \
     }
     MPD_RECON(0) MPD_RECON(1) MPD_RECON(2) MPD_RECON(3) MPD_RECON(4)
     MPD_RECON(5) MPD_RECON(6) MPD_RECON(7) MPD_RECON(8)
@@ -669,24 +761,32 @@ __device__ __forceinline__ void face(const K32 &k, const float2 *S, const FaceId
 
 // x-face t of a tile: column t % TX, row t / TX for t < TX*TY, then the last
 // column (face index TX*TY + row)
-template <int MODEL, int WALLS, class G>
+template <int MODEL, int WALLS, int FLUX, class G>
 __device__ __forceinline__ void xface(const K32 &k, const float2 *S, int t, float *h, float *f, float &m)
 {
-    constexpr int NIN = G::NIN, TX = G::TX;
+    constexpr int NIN = G::NIN, TX = G::TX, H = G::H;
     const int fi = t < NIN ? t % TX : TX, fj = t < NIN ? t / TX : t - NIN;
-    face<0, MODEL, WALLS, G>(k, S, face_idx<0, G>(fi, fj + 1), 0, h, f, m);
+    // owner: bar-plus region column fi + H - 1, row fj + H
+    if constexpr (FLUX == MPD_FLUX_UW)
+        face<0, MODEL, WALLS, FLUX, G>(k, S, face_idx_uw<0, G>(fi + H - 1, fj + H), 0, h, f, m);
+    else
+        face<0, MODEL, WALLS, FLUX, G>(k, S, face_idx<0, G>(fi + H - 1, fj + H), 0, h, f, m);
 }
 
 // y-face t of a tile: column t % TX, row t / TX
-template <int MODEL, int WALLS, class G>
+template <int MODEL, int WALLS, int FLUX, class G>
 __device__ __forceinline__ void yface(const K32 &k, const float2 *S, const float2 *s_nee, int t, int j0, int ny,
                                       float *h, float *f, float &m)
 {
-    constexpr int TX = G::TX;
+    constexpr int TX = G::TX, H = G::H;
     const int fi = t % TX, fj = t / TX;
     const int gj = j0 + fj;
     const int wall = WALLS ? (gj == 0 ? -1 : (gj == ny ? 1 : 0)) : 0;
-    face<1, MODEL, WALLS, G>(k, S, face_idx<1, G>(fi + 1, fj), wall, h, f, m);
+    // owner: bar-plus region column fi + H, row fj + H - 1
+    if constexpr (FLUX == MPD_FLUX_UW)
+        face<1, MODEL, WALLS, FLUX, G>(k, S, face_idx_uw<1, G>(fi + H, fj + H - 1), wall, h, f, m);
+    else
+        face<1, MODEL, WALLS, FLUX, G>(k, S, face_idx<1, G>(fi + H, fj + H - 1), wall, h, f, m);
     if (WALLS == 2 && wall != 0) {
         // NEE wall face: the values of the wall-row cell (stage B), both
         // members of each pair alike, so no phase-field mass crosses
@@ -707,11 +807,38 @@ __device__ __forceinline__ void yface(const K32 &k, const float2 *S, const float
 //  C: every x- and y-face of the tile (one of each per thread, then for TY = 16
 //     the last 16 of each direction on waves 2 and 3);
 //  D: Update_phi_T, phi moved by the antisymmetric face mass fluxes.
-template <int MODEL, int WALLS, class G>
+// position (rx, ry) of cell r of the halo ring of the bar-plus region
+template <class G>
+__device__ __forceinline__ void ring_pos(int r, int &rx, int &ry)
+{
+    constexpr int R1X = G::R1X, R1Y = G::R1Y, TY = G::TY, H = G::H;
+    if constexpr (H == 1) {
+        // bottom row, top row, left column, right column
+        if (r < R1X) { rx = r; ry = 0; }
+        else if (r < 2 * R1X) { rx = r - R1X; ry = R1Y - 1; }
+        else if (r < 2 * R1X + TY) { rx = 0; ry = r - 2 * R1X + 1; }
+        else { rx = R1X - 1; ry = r - 2 * R1X - TY + 1; }
+    } else {
+        // the H bottom rows, the H top rows, then the 2H halo columns of each other row
+        if (r < H * R1X) { ry = r / R1X; rx = r % R1X; }
+        else if (r < 2 * H * R1X) { r -= H * R1X; ry = R1Y - H + r / R1X; rx = r % R1X; }
+        else {
+            r -= 2 * H * R1X;
+            const int c = r % (2 * H);
+            ry = H + r / (2 * H);
+            rx = c < H ? c : R1X - 2 * H + c;
+        }
+    }
+}
+
+template <int MODEL, int WALLS, int FLUX, class G>
 __device__ __forceinline__ void tile32(const K32 &k, int nx, int ny, const FusedPlanes32 &pin,
                                        const FusedPlanes32 &pout, int64_t N, int i0, int j0)
 {
     constexpr bool HCZ = MODEL == MPD_MODEL_AC_HCZ;
+    constexpr int H = G::H, H2 = G::H + 1;
+    static_assert(H == 1 || WALLS == 0, "UW faces: periodic meshes");
+    static_assert((FLUX == MPD_FLUX_UW) == (H == 2), "halo of the face scheme");
     constexpr int TX = G::TX, TY = G::TY, NT = G::NT, R1X = G::R1X, R1Y = G::R1Y, R2Y = G::R2Y, R2N = G::R2N;
     constexpr int NIN = G::NIN, NRING = G::NRING, NXF = G::NXF, NYF = G::NYF, PS = G::PS, FS = G::FS;
     __shared__ float2 s_big[G::NBIG];
@@ -719,7 +846,7 @@ __device__ __forceinline__ void tile32(const K32 &k, int nx, int ny, const Fused
     __shared__ float s_phl[HCZ ? R2N : 1];
     __shared__ float2 s_nee[WALLS == 2 ? 6 * TX : 1];  // NEE wall values: [bottom, top][column][pair]
     const int tid = threadIdx.x;
-    const bool small = nx < TX + 4 || ny < TY + 4;
+    const bool small = nx < TX + 2 * H2 || ny < TY + 2 * H2;
     constexpr int NPL = HCZ ? 22 : (WALLS ? 20 : 19);
     const Rs in = make_rs(pin, N, nx, NPL), out = make_rs(pout, N, nx, NPL);
 
@@ -731,16 +858,13 @@ __device__ __forceinline__ void tile32(const K32 &k, int nx, int ny, const Fused
     constexpr int NR2 = NRING - (NT - NIN);   // ring cells of the second pass
     CellIn x1, x2;
     int rx1, ry1, rx2 = 0, ry2 = 0;
-    if (cell) { rx1 = ci + 1; ry1 = cj + 1; }
-    else { rx1 = tid - NIN; ry1 = 0; }
-    cell_load<MODEL, WALLS>(in, nx, ny, small, i0, j0, rx1, ry1, x1);
+    if (cell) { rx1 = ci + H; ry1 = cj + H; }
+    else if constexpr (H == 1) { rx1 = tid - NIN; ry1 = 0; }   // the first NT - NIN < R1X ring cells: bottom row
+    else ring_pos<G>(tid - NIN, rx1, ry1);
+    cell_load<MODEL, WALLS, G::H>(in, nx, ny, small, i0, j0, rx1, ry1, x1);
     if (tid < NR2) {
-        const int r = tid + (NT - NIN);
-        if (r < R1X) { rx2 = r; ry2 = 0; }
-        else if (r < 2 * R1X) { rx2 = r - R1X; ry2 = R1Y - 1; }
-        else if (r < 2 * R1X + TY) { rx2 = 0; ry2 = r - 2 * R1X + 1; }
-        else { rx2 = R1X - 1; ry2 = r - 2 * R1X - TY + 1; }
-        if (PRELOAD_RING) cell_load<MODEL, WALLS>(in, nx, ny, small, i0, j0, rx2, ry2, x2);
+        ring_pos<G>(tid + (NT - NIN), rx2, ry2);
+        if (PRELOAD_RING) cell_load<MODEL, WALLS, G::H>(in, nx, ny, small, i0, j0, rx2, ry2, x2);
     }
 
     // ---- stage A ----
@@ -749,14 +873,14 @@ __device__ __forceinline__ void tile32(const K32 &k, int nx, int ny, const Fused
         int rx, ry;
         if (q < TX * R2Y) {
             ry = q / TX;
-            rx = q % TX + 2;
+            rx = q % TX + H2;
         } else {
-            const int t = q - TX * R2Y, hc = t % 4;
-            ry = t / 4;
-            rx = hc < 2 ? hc : hc + TX;
+            const int t = q - TX * R2Y, hc = t % (2 * H2);
+            ry = t / (2 * H2);
+            rx = hc < H2 ? hc : hc + TX;
         }
-        const int gi = wrap(i0 - 2 + rx, nx, small);
-        int gj = j0 - 2 + ry;
+        const int gi = wrap(i0 - H2 + rx, nx, small);
+        int gj = j0 - H2 + ry;
         float p = 0.f, pl = 0.f;
         if (WALLS && (gj < 0 || gj >= ny)) {
             if (gj == -1 || gj == ny) {
@@ -784,7 +908,7 @@ __device__ __forceinline__ void tile32(const K32 &k, int nx, int ny, const Fused
     constexpr bool PRIO = G::EXF > 0 || G::EYF > 0;
     if (tid < NR2) {
         if (PRIO) __builtin_amdgcn_s_setprio(2);
-        if (!PRELOAD_RING) cell_load<MODEL, WALLS>(in, nx, ny, small, i0, j0, rx2, ry2, x2);
+        if (!PRELOAD_RING) cell_load<MODEL, WALLS, G::H>(in, nx, ny, small, i0, j0, rx2, ry2, x2);
         float dummy[17];
         cell_bp<MODEL, WALLS, G>(k, out, nx, ny, rx2, ry2, x2, s_big, s_phi, s_phl, dummy, false, s_nee);
         if (PRIO) __builtin_amdgcn_s_setprio(0);
@@ -823,7 +947,7 @@ __device__ __forceinline__ void tile32(const K32 &k, int nx, int ny, const Fused
     if (EXF > 0 && tid >= XE0 && tid < XE0 + EXF) {
         et = tid - XE0;
         float h[6], f[6], m = 0.f;
-        xface<MODEL, WALLS, G>(k, s_big, NT + et, h, f, m);
+        xface<MODEL, WALLS, FLUX, G>(k, s_big, NT + et, h, f, m);
 #pragma unroll
         for (int z = 0; z < 6; ++z) s_ext[z * NE + et] = make_float2(h[z], f[z]);
         s_ext[6 * NE + et] = make_float2(m, 0.f);
@@ -831,17 +955,17 @@ __device__ __forceinline__ void tile32(const K32 &k, int nx, int ny, const Fused
     if (EYF > 0 && tid >= YE0 && tid < YE0 + EYF) {
         et = tid - YE0;
         float h[6], f[6], m = 0.f;
-        yface<MODEL, WALLS, G>(k, s_big, s_nee, NT + et, j0, ny, h, f, m);
+        yface<MODEL, WALLS, FLUX, G>(k, s_big, s_nee, NT + et, j0, ny, h, f, m);
 #pragma unroll
         for (int z = 0; z < 6; ++z) s_ext[(NFL + z) * NE + et] = make_float2(h[z], f[z]);
         s_ext[(NFL + 6) * NE + et] = make_float2(m, 0.f);
     }
     float xh[6], xf[6], xm = 0.f, yh[6], yf[6], ym = 0.f;
-    if (tid < NXF) xface<MODEL, WALLS, G>(k, s_big, tid, xh, xf, xm);
+    if (tid < NXF) xface<MODEL, WALLS, FLUX, G>(k, s_big, tid, xh, xf, xm);
     // NXF >= NT: every thread has both faces, and the scheduler would
     // interleave the two (past 128 VGPRs, below 4 waves per SIMD); keep them apart
     if (NXF >= NT) __builtin_amdgcn_sched_barrier(0);
-    if (tid < NYF) yface<MODEL, WALLS, G>(k, s_big, s_nee, tid, j0, ny, yh, yf, ym);
+    if (tid < NYF) yface<MODEL, WALLS, FLUX, G>(k, s_big, s_nee, tid, j0, ny, yh, yf, ym);
     if (PRIO) __builtin_amdgcn_s_setprio(0);
     __syncthreads();
     float2 *s_fx = s_big, *s_fy = s_big + NFL * FS;
@@ -915,15 +1039,16 @@ __device__ __forceinline__ void tile32(const K32 &k, int nx, int ny, const Fused
     st(out.a, n, phi, P_PHI * out.pb);
 }
 
-template <int MODEL, int WALLS, int TX, int TY>
+template <int MODEL, int WALLS, int TX, int TY, int FLUX = MPD_FLUX_CD>
 __global__ void __launch_bounds__(TX * TY > 256 ? 512 : 256) k_fused32(K32 k, int nx, int ny, FusedPlanes32 in,
                                                                         FusedPlanes32 out, int64_t N, int ntx,
                                                                         int ntiles)
 {
-    static_assert((TX * TY > 256 ? 512 : 256) == TileK<TX, TY>::NT, "launch bounds");
+    using G = TileK<TX, TY, FLUX == MPD_FLUX_UW ? 2 : 1>;
+    static_assert((TX * TY > 256 ? 512 : 256) == G::NT, "launch bounds");
     const int tile = tile_of_block(blockIdx.x, ntiles);
     if (tile < 0) return;
-    tile32<MODEL, WALLS, TileK<TX, TY>>(k, nx, ny, in, out, N, (tile % ntx) * TX, (tile / ntx) * TY);
+    tile32<MODEL, WALLS, FLUX, G>(k, nx, ny, in, out, N, (tile % ntx) * TX, (tile / ntx) * TY);
 }
 
 // Batched step: every workgroup takes one (case, tile) entry of a work list
@@ -939,8 +1064,8 @@ __global__ void __launch_bounds__(TX * TY > 256 ? 512 : 256) k_fused32_batch(con
     const BatchCase &c = cases[w.x];
     const int sel = c.cur ^ parity;
     const int ntx = (c.nx + TX - 1) / TX;
-    tile32<MODEL, WALLS, TileK<TX, TY>>(c.k, c.nx, c.ny, c.pl[sel], c.pl[sel ^ 1], c.N, (w.y % ntx) * TX,
-                                        (w.y / ntx) * TY);
+    tile32<MODEL, WALLS, MPD_FLUX_CD, TileK<TX, TY>>(c.k, c.nx, c.ny, c.pl[sel], c.pl[sel ^ 1], c.N,
+                                                     (w.y % ntx) * TX, (w.y / ntx) * TY);
 }
 
 K32 make_k32(const Consts &C)
@@ -1007,6 +1132,9 @@ K32 make_k32(const Consts &C)
     k.hneq = (float)(2.0 * C.taum / (2.0 * C.taum + C.dt));
     k.fneq0 = (float)(2.0 * C.tau0 / (2.0 * C.tau0 + C.dt));
     k.hdx = (float)(0.5 * C.dx);
+    k.uw_n = (float)((0.5 - A) / 12.0);
+    k.uw_t = (float)(A / 12.0);
+    k.uw_0 = (float)(1.0 / 24.0);
     return k;
 }
 
@@ -1156,10 +1284,27 @@ int fused32_tile_auto(int nx, int ny, int cus, int64_t state_bytes)
     return state_bytes > mall ? MPD_TILE_32X16 : MPD_TILE_16X15;
 }
 
+template <int MODEL>
+void fused32_launch_uw(const K32 &k, const FusedPlanes32 &in, const FusedPlanes32 &out, const Consts &C, int64_t N,
+                       hipStream_t st)
+{
+    const int ntx = (C.nx + 15) / 16, nty = (C.ny + 14) / 15;
+    const int ntiles = ntx * nty;
+    dim3 g((unsigned)(((ntiles + 7) / 8) * 8)), b(256);
+    hipLaunchKernelGGL((k_fused32<MODEL, 0, 16, 15, MPD_FLUX_UW>), g, b, 0, st, k, C.nx, C.ny, in, out, N, ntx,
+                       ntiles);
+}
+
 void fused32_step(const FusedPlanes32 &in, const FusedPlanes32 &out, const Consts &C, int64_t N, int tile,
                   hipStream_t st)
 {
     const K32 k = make_k32(C);
+    if (C.flux == MPD_FLUX_UW) {
+        // UW faces: 16x15 tiles, periodic meshes
+        if (C.model == MPD_MODEL_AC_HCZ) fused32_launch_uw<MPD_MODEL_AC_HCZ>(k, in, out, C, N, st);
+        else fused32_launch_uw<MPD_MODEL_AC_P>(k, in, out, C, N, st);
+        return;
+    }
     if (C.model == MPD_MODEL_AC_HCZ) fused32_launch_shape<MPD_MODEL_AC_HCZ>(k, in, out, C, N, tile, st);
     else fused32_launch_shape<MPD_MODEL_AC_P>(k, in, out, C, N, tile, st);
 }

@@ -634,8 +634,10 @@ bool fused_supported(const mpd_config &cfg)
     // FP32-bulk pseudopotential D2Q9: the FP32 phase kernels of pp32.hip
     if (cfg.model == MPD_MODEL_PP_D2Q9)
         return cfg.precision == MPD_FP32_BULK && cfg.flux == MPD_FLUX_CD && cfg.bc_y == MPD_BC_PERIODIC;
-    return (cfg.model == MPD_MODEL_AC_HCZ || cfg.model == MPD_MODEL_AC_P) && cfg.flux == MPD_FLUX_CD &&
-           cfg.nx >= 4 && cfg.ny >= 4;
+    const bool ac = (cfg.model == MPD_MODEL_AC_HCZ || cfg.model == MPD_MODEL_AC_P) && cfg.nx >= 4 && cfg.ny >= 4;
+    // UW faces: the FP32-bulk tile kernel on periodic meshes (the FP64 fused kernel is CD only)
+    if (cfg.flux == MPD_FLUX_UW) return ac && cfg.precision == MPD_FP32_BULK && cfg.bc_y == MPD_BC_PERIODIC;
+    return ac && cfg.flux == MPD_FLUX_CD;
 }
 
 bool fused_needs_priming(const mpd_config &cfg) { return cfg.model != MPD_MODEL_PP_D2Q9; }

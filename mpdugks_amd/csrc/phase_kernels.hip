@@ -86,17 +86,18 @@ __global__ void __launch_bounds__(256) k_ghosts(DevState S, Consts C)
     }
 }
 
-// phase 3 (UW only): 6-point gradients of bar-plus (Ark/Lib/GradSchemeBasic.h:95-126)
-// (periodic in both directions: the UW scheme has no wall path)
-__global__ void __launch_bounds__(256) k_grad(DevState S, Consts C, const double *bp, double *gx,
-                                              double *gy)
+// phase 3 (UW only): 6-point gradients of bar-plus (Ark/Lib/GradSchemeBasic.h:95-126).
+// With walls a wall-row cell's stencil takes the ghost rows, wrapped in x (its
+// diagonal across the periodic line is the corner cell PeriodicShadowC_*, an alias
+// of the wall shadow cell at the row's other end: Ark/Cpp/Mesh/MeshCartesian.cpp:61-118,
+// Ark/Cpp/Mesh/GhostCells.cpp:20-64).
+__global__ void __launch_bounds__(256) k_grad(DevState S, Consts C, const double *bp, const double *gbp,
+                                              double *gx, double *gy)
 {
     size_t n = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (n >= (size_t)S.N) return;
     int i = (int)(n % C.nx), j = (int)(n / C.nx);
-    auto B = [&](int q, int a, int b) {
-        return bp[(size_t)q * S.N + (size_t)wrapi(b, C.ny) * C.nx + wrapi(a, C.nx)];
-    };
+    auto B = [&](int q, int a, int b) { return cellbp(S, C, bp, gbp, q, a, b); };
     for (int q = 0; q < Q9; ++q) {
         double c[4] = {B(q, i + 1, j), B(q, i, j + 1), B(q, i - 1, j), B(q, i, j - 1)};
         double d[4] = {B(q, i + 1, j + 1), B(q, i - 1, j + 1), B(q, i - 1, j - 1), B(q, i + 1, j - 1)};
@@ -132,9 +133,19 @@ __device__ inline double face_value(const DevState &S, const Consts &C, const do
     size_t kn = (size_t)q * S.N + (size_t)wrapi(nj, C.ny) * C.nx + wrapi(ni, C.nx);
     double dxo = xf - C.hdt * ex - xo, dyo = yf - C.hdt * ey - yo;
     double dxn = xf - C.hdt * ex - xn, dyn = yf - C.hdt * ey - yn;
-    if (xin > 0) return bo + (dxo * bpx[ko] + dyo * bpy[ko]);
-    if (xin < 0) return bn + (dxn * bpx[kn] + dyn * bpy[kn]);
-    return (bo + (dxo * bpx[ko] + dyo * bpy[ko]) + bn + (dxn * bpx[kn] + dyn * bpy[kn])) / 2;
+    // a wall face's other side is the wall's shadow cell, whose gradient the
+    // reference never computes (its arrays from `new double[]`,
+    // Ark/Cpp/Mesh/Mesh_2D.cpp:8-13, hold 0 in its runs): 0
+    const bool go = C.walls && oj < 0, gn = C.walls && nj >= C.ny;
+    const double gxo = go ? 0.0 : bpx[ko], gyo = go ? 0.0 : bpy[ko];
+    const double gxn = gn ? 0.0 : bpx[kn], gyn = gn ? 0.0 : bpy[kn];
+    if (xin > 0) return bo + (dxo * gxo + dyo * gyo);
+    if (xin < 0) return bn + (dxn * gxn + dyn * gyn);
+    // xi.n = 0: the average adds the reference face's owner first, its mesh
+    // cell for a face at index 0 of its normal (the zone-12 copy of a periodic
+    // face, the bottom wall; Ark/Lib/MeshConstructFunc.h:216-233)
+    if (oi < 0 || oj < 0) return (bn + (dxn * gxn + dyn * gyn) + bo + (dxo * gxo + dyo * gyo)) / 2;
+    return (bo + (dxo * gxo + dyo * gyo) + bn + (dxn * gxn + dyn * gyn)) / 2;
 }
 
 __device__ inline void face_geom(const Consts &C, int dir, int i, int j, int &oi, int &oj, int &ni, int &nj,
@@ -144,17 +155,37 @@ __device__ inline void face_geom(const Consts &C, int dir, int i, int j, int &oi
     else { oi = i; oj = j - 1; ni = i; nj = j; xf = (i + 0.5) * C.dx; yf = j * C.dx; }
 }
 
+// Faces of the second copies: x-faces of column nx (one per row), then, with
+// periodic y, y-faces of row ny (one per column).
+__host__ __device__ inline int uw_copies(const Consts &C) { return C.ny + (C.walls ? 0 : C.nx); }
+
 // phase 4 + 6 for the Allen-Cahn models: Flux_2D on every face, Wall_3_BB on
-// wall faces, times xi.n dS (Ark/Cpp/DUGKS2DSolver.cpp:219-273)
+// wall faces, times xi.n dS (Ark/Cpp/DUGKS2DSolver.cpp:219-273).  UW: the
+// reference stores each periodic face twice (zone 12 owned by the first
+// column / row, zone 8 by the last) and the two round apart in the xi.n = 0
+// average, so threads past the structured faces evaluate the zone-8 copies
+// into S.cbhl / S.cbfl [Q][nx + ny], which the last column / row take.
 template <int MODEL, int FLUX, bool UNIT>
 __global__ void __launch_bounds__(256) k_faces_ac(DevState S, Consts C)
 {
     size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     size_t nxf = S.N, nyf = (size_t)C.nyf * C.nx;
-    if (t >= nxf + nyf) return;
-    int dir = t < nxf ? 0 : 1;
-    size_t fidx = dir == 0 ? t : t - nxf;
-    int i = (int)(fidx % C.nx), j = (int)(fidx / C.nx);
+    const size_t ncopy = FLUX == MPD_FLUX_UW ? (size_t)uw_copies(C) : 0;
+    if (t >= nxf + nyf + ncopy) return;
+    int dir, i, j;
+    size_t fidx;
+    const bool copy = t >= nxf + nyf;
+    if (copy) {
+        fidx = t - nxf - nyf;
+        dir = fidx < (size_t)C.ny ? 0 : 1;
+        i = dir == 0 ? C.nx : (int)(fidx - C.ny);
+        j = dir == 0 ? (int)fidx : C.ny;
+    } else {
+        dir = t < nxf ? 0 : 1;
+        fidx = dir == 0 ? t : t - nxf;
+        i = (int)(fidx % C.nx);
+        j = (int)(fidx / C.nx);
+    }
     int oi, oj, ni, nj;
     double xf, yf;
     face_geom(C, dir, i, j, oi, oj, ni, nj, xf, yf);
@@ -209,6 +240,11 @@ __global__ void __launch_bounds__(256) k_faces_ac(DevState S, Consts C)
     }
     size_t plane = dir == 0 ? nxf : nyf;
     double *hfl = S.hfl + (dir ? (size_t)Q9 * nxf : 0), *ffl = S.ffl + (dir ? (size_t)Q9 * nxf : 0);
+    if (copy) {
+        plane = (size_t)C.nx + C.ny;
+        hfl = S.cbhl;
+        ffl = S.cbfl;
+    }
 #pragma unroll
     for (int q = 0; q < Q9; ++q) {
         double xin = C.dx * (dir == 0 ? C.xi[0][q] : C.xi[1][q]);
@@ -362,21 +398,26 @@ __global__ void __launch_bounds__(256) k_update(DevState S, Consts C)
     const double aTP = 4.0 / 3.0, bTP = 1.0 / 3.0;
     const double *hx = S.hfl, *hy = S.hfl + (size_t)Q9 * N;
     const double *fx = S.ffl, *fy = S.ffl + (size_t)Q9 * N;
+    // Allen-Cahn UW: the last column / row owns the second copy of its periodic
+    // east / north face (Face_C, Ark/Cpp/Mesh/MeshCartesian.cpp:28-60)
+    const bool copies = MODEL != MPD_MODEL_PP_D2Q9 && C.flux == MPD_FLUX_UW;
+    const bool cE = copies && i == C.nx - 1, cN = copies && !C.walls && j == C.ny - 1;
+    const size_t nb = (size_t)C.nx + C.ny;
     double ht[Q9], ft[Q9];
 #pragma unroll
     for (int q = 0; q < Q9; ++q) {
         if (MODEL != MPD_MODEL_PP_D2Q9) {
             double s = 0.0;
-            s += -1 * hx[q * N + kE];
-            s += -1 * hy[q * yplane + kN];
+            s += -1 * (cE ? S.cbhl[q * nb + j] : hx[q * N + kE]);
+            s += -1 * (cN ? S.cbhl[q * nb + C.ny + i] : hy[q * yplane + kN]);
             s += 1 * hx[q * N + kW];
             s += 1 * hy[q * yplane + kS];
             ht[q] = aTP * S.hbp[q * N + n] - bTP * S.h[q * N + n] + C.dtv * s;
             S.h[q * N + n] = ht[q];
         }
         double s = 0.0;
-        s += -1 * fx[q * N + kE];
-        s += -1 * fy[q * yplane + kN];
+        s += -1 * (cE ? S.cbfl[q * nb + j] : fx[q * N + kE]);
+        s += -1 * (cN ? S.cbfl[q * nb + C.ny + i] : fy[q * yplane + kN]);
         s += 1 * fx[q * N + kW];
         s += 1 * fy[q * yplane + kS];
         ft[q] = aTP * S.fbp[q * N + n] - bTP * S.f[q * N + n] + C.dtv * s;
@@ -654,14 +695,16 @@ static void phase_step_t(const DevState &S, const Consts &C, hipStream_t st)
     if (C.walls) hipLaunchKernelGGL(k_ghosts, dim3(nblk(2 * C.nx)), dim3(256), 0, st, S, C);
     if (FLUX == MPD_FLUX_UW) {
         if (MODEL != MPD_MODEL_PP_D2Q9)
-            hipLaunchKernelGGL(k_grad, dim3(nblk(S.N)), dim3(256), 0, st, S, C, (const double *)S.hbp, S.hbpx, S.hbpy);
-        hipLaunchKernelGGL(k_grad, dim3(nblk(S.N)), dim3(256), 0, st, S, C, (const double *)S.fbp, S.fbpx, S.fbpy);
+            hipLaunchKernelGGL(k_grad, dim3(nblk(S.N)), dim3(256), 0, st, S, C, (const double *)S.hbp,
+                               (const double *)S.ghbp, S.hbpx, S.hbpy);
+        hipLaunchKernelGGL(k_grad, dim3(nblk(S.N)), dim3(256), 0, st, S, C, (const double *)S.fbp,
+                           (const double *)S.gfbp, S.fbpx, S.fbpy);
     }
     if (MODEL == MPD_MODEL_PP_D2Q9) {
         hipLaunchKernelGGL((k_pp_face_psi<FLUX, UNIT>), dim3(nblk(2 * S.N)), dim3(256), 0, st, S, C);
         hipLaunchKernelGGL((k_pp_face_flux<FLUX, UNIT>), dim3(nblk(2 * S.N)), dim3(256), 0, st, S, C);
     } else {
-        size_t nf = S.N + (size_t)C.nyf * C.nx;
+        size_t nf = S.N + (size_t)C.nyf * C.nx + (FLUX == MPD_FLUX_UW ? uw_copies(C) : 0);
         hipLaunchKernelGGL((k_faces_ac<MODEL, FLUX, UNIT>), dim3(nblk(nf)), dim3(256), 0, st, S, C);
     }
     hipLaunchKernelGGL((k_update<MODEL, UNIT>), dim3(nblk(S.N)), dim3(256), 0, st, S, C);

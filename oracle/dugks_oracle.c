@@ -78,6 +78,8 @@ struct ora {
     double *cbfx, *cbfy;      /* force of the second copy (zone 8) of each periodic face:
                                  x-faces i = 0 by row [ny], then y-faces j = 0 by column [nx] */
     double *cbfl;             /* flux of those copies [Q][ny + nx] */
+    double *cbhl;             /* h flux of those copies (Allen-Cahn UW) [Q][ny + nx] */
+    int one_copy;             /* test knob (ora_set_one_copy): UW without the second copies */
 };
 
 /* ---------------- small helpers ---------------- */
@@ -249,8 +251,8 @@ ora *ora_create(const mpd_config *cfg)
         return NULL;
     if (cfg->model == MPD_MODEL_PP_D2V16 && (cfg->flux != MPD_FLUX_CD || cfg->bc_y != MPD_BC_PERIODIC))
         return NULL; /* D2V16: CD faces on a periodic mesh */
-    if (cfg->bc_y != MPD_BC_PERIODIC && (cfg->flux == MPD_FLUX_UW || cfg->model == MPD_MODEL_PP_D2Q9))
-        return NULL; /* the reference has no working wall path for these */
+    if (cfg->bc_y != MPD_BC_PERIODIC && cfg->model == MPD_MODEL_PP_D2Q9)
+        return NULL; /* the reference has no working wall path for it */
     ora *o = (ora *)calloc(1, sizeof(ora));
     o->c = *cfg;
     o->nx = cfg->nx;
@@ -313,6 +315,7 @@ ora *ora_create(const mpd_config *cfg)
     o->cbfx = calloc((size_t)o->nx + o->ny, sizeof(double));
     o->cbfy = calloc((size_t)o->nx + o->ny, sizeof(double));
     o->cbfl = calloc((size_t)o->Q * (o->nx + o->ny), sizeof(double));
+    o->cbhl = calloc((size_t)o->Q * (o->nx + o->ny), sizeof(double));
     return o;
 }
 
@@ -321,7 +324,7 @@ void ora_destroy(ora *o)
     if (!o) return;
     double *bufs[] = {o->h, o->f, o->m, o->ffx, o->ffy, o->fpsi, o->hbp, o->fbp, o->hbpx,
                       o->hbpy, o->fbpx, o->fbpy, o->hfl, o->ffl, o->gm, o->ghbp, o->gfbp,
-                      o->fpsi_old, o->cbfx, o->cbfy, o->cbfl};
+                      o->fpsi_old, o->cbfx, o->cbfy, o->cbfl, o->cbhl};
     for (size_t k = 0; k < sizeof(bufs) / sizeof(bufs[0]); ++k) free(bufs[k]);
     free(o);
 }
@@ -352,6 +355,14 @@ int ora_set_face_order(ora *o, int ref_order)
     if (ref_order && !(o->c.model == MPD_MODEL_PP_D2Q9 && o->c.flux == MPD_FLUX_CD && !o->walls))
         return -1;
     o->ref_order = ref_order != 0;
+    return 0;
+}
+
+/* Test knob: Allen-Cahn UW with one face per location (the last column and
+ * row take the first copy's flux), i.e. without the reference's second copies. */
+int ora_set_one_copy(ora *o, int one)
+{
+    o->one_copy = one != 0;
     return 0;
 }
 
@@ -465,18 +476,24 @@ static void phase2_walls(ora *o)
 
 /* ---------------- phase 3: UW gradients ---------------- */
 
-/* update_DVDF_Grad6points, Ark/Lib/GradSchemeBasic.h:95-126 (apsi = 4, bpsi = 1) */
-static void phase3_grad(ora *o, const double *bp, double *gx, double *gy)
+/* update_DVDF_Grad6points, Ark/Lib/GradSchemeBasic.h:95-126 (apsi = 4, bpsi = 1).
+ * With walls, a wall-row cell's stencil reaches the wall's shadow cells, which
+ * hold the extrapolated bar-plus of phase 2; its diagonal neighbour across the
+ * periodic x line is PeriodicShadowC_SE/SW/NE/NW (DiagonalCellConstruct,
+ * Ark/Cpp/Mesh/MeshCartesian.cpp:61-118), an alias of the shadow cell of the
+ * wall row's other end (ShadowCellAssign shares the population arrays,
+ * Ark/Cpp/Mesh/GhostCells.cpp:20-28,38-64), i.e. the ghost row wrapped in x. */
+static void phase3_grad(ora *o, const double *bp, const double *gbp, double *gx, double *gy)
 {
     int nx = o->nx, ny = o->ny;
     double d2x = 2.0 * o->c.dx;
     for (int q = 0; q < Q9; ++q)
         for (int j = 0; j < ny; ++j)
             for (int i = 0; i < nx; ++i) {
-                double C0 = BP(o, bp, NULL, q, i + 1, j), C1 = BP(o, bp, NULL, q, i, j + 1);
-                double C2 = BP(o, bp, NULL, q, i - 1, j), C3 = BP(o, bp, NULL, q, i, j - 1);
-                double D0 = BP(o, bp, NULL, q, i + 1, j + 1), D1 = BP(o, bp, NULL, q, i - 1, j + 1);
-                double D2 = BP(o, bp, NULL, q, i - 1, j - 1), D3 = BP(o, bp, NULL, q, i + 1, j - 1);
+                double C0 = BP(o, bp, gbp, q, i + 1, j), C1 = BP(o, bp, gbp, q, i, j + 1);
+                double C2 = BP(o, bp, gbp, q, i - 1, j), C3 = BP(o, bp, gbp, q, i, j - 1);
+                double D0 = BP(o, bp, gbp, q, i + 1, j + 1), D1 = BP(o, bp, gbp, q, i - 1, j + 1);
+                double D2 = BP(o, bp, gbp, q, i - 1, j - 1), D3 = BP(o, bp, gbp, q, i + 1, j - 1);
                 size_t k = (size_t)q * o->N + (size_t)j * nx + i;
                 gx[k] = (4 * (C0 - C2) + 1 * (D0 - D1 + D3 - D2)) / (d2x * 6);
                 gy[k] = (4 * (C1 - C3) + 1 * (D0 - D3 + D1 - D2)) / (d2x * 6);
@@ -527,9 +544,25 @@ static double face_recon(const ora *o, const double *bp, const double *gbp, cons
     size_t kn = (size_t)q * o->N + (size_t)wrap(F->nj, o->ny) * o->nx + wrap(F->ni, o->nx);
     double dxo = F->xf - o->hdt * ex - xo, dyo = F->yf - o->hdt * ey - yo;
     double dxn = F->xf - o->hdt * ex - xn, dyn = F->yf - o->hdt * ey - yn;
-    if (xin > 0) return bo + (dxo * gx[ko] + dyo * gy[ko]);
-    if (xin < 0) return bn + (dxn * gx[kn] + dyn * gy[kn]);
-    return (bo + (dxo * gx[ko] + dyo * gy[ko]) + bn + (dxn * gx[kn] + dyn * gy[kn])) / 2;
+    /* A wall face's other side is the wall's shadow cell.  Its gradient is
+     * never computed (Grad_VS_6points runs over the mesh cells, :197-206) and
+     * its arrays come from `new double[]` (AllocateARK, Ark/Cpp/Mesh/Mesh_2D.cpp:8-13),
+     * so the reference reads whatever that memory holds: 0 in the compiled
+     * reference's runs, and 0 here.  It enters the face macros through the
+     * populations that Wall_3_BB then replaces and through the xi.n = 0 average. */
+    double gxo = 0.0, gyo = 0.0, gxn = 0.0, gyn = 0.0;
+    if (!(o->walls && F->oj < 0)) { gxo = gx[ko]; gyo = gy[ko]; }
+    if (!(o->walls && F->nj >= o->ny)) { gxn = gx[kn]; gyn = gy[kn]; }
+    double ro = bo + (dxo * gxo + dyo * gyo), rn = bn + (dxn * gxn + dyn * gyn);
+    if (xin > 0) return ro;
+    if (xin < 0) return rn;
+    /* xi.n = 0: the average adds the reference face's owner first.  A boundary
+     * face's owner is its mesh cell (ConstructFacesCells,
+     * Ark/Lib/MeshConstructFunc.h:216-233): at index 0 of its normal (the
+     * zone-12 copy of a periodic face, the bottom wall) that is this face's
+     * neighbour; the zone-8 copy (make_face at i = nx / j = ny) has it as owner. */
+    if (F->oi < 0 || F->oj < 0) return (bn + (dxn * gxn + dyn * gyn) + bo + (dxo * gxo + dyo * gyo)) / 2;
+    return (bo + (dxo * gxo + dyo * gyo) + bn + (dxn * gxn + dyn * gyn)) / 2;
 }
 
 typedef struct {
@@ -929,6 +962,30 @@ static void phase4_faces(ora *o)
                 }
             }
     }
+    /* Allen-Cahn UW: the second stored copy (zone 8, owned by the last column /
+     * row) of each periodic face, whose xi.n = 0 average adds the other cell
+     * first, so its bits can differ from the first copy's. */
+    if (!pp && o->c.flux == MPD_FLUX_UW && !o->one_copy) {
+        int ncopy = o->walls ? ny : ny + nx;
+        for (int b = 0; b < ncopy; ++b) {
+            face_t F;
+            if (b < ny) make_face(o, 0, nx, b, &F);
+            else make_face(o, 1, b - ny, ny, &F);
+            for (int q = 0; q < Q; ++q) {
+                hb[q] = face_recon(o, o->hbp, o->ghbp, o->hbpx, o->hbpy, &F, q);
+                fb[q] = face_recon(o, o->fbp, o->gfbp, o->fbpx, o->fbpy, &F, q);
+            }
+            face_macro fm;
+            face_macros(o, &F, hb, fb, 0, 0, &fm);
+            double ah = 2.0 * fm.tau / (2.0 * fm.tau + o->hdt), bh = 1.0 - ah, ch = fm.tau * bh;
+            for (int q = 0; q < Q; ++q) {
+                double hh, ff, xin = o->c.dx * (F.dir == 0 ? o->xi[0][q] : o->xi[1][q]);
+                face_bgk(o, &fm, q, hb[q], fb[q], ah, bh, ch, &hh, &ff);
+                o->cbhl[(size_t)q * (nx + ny) + b] = hh * xin;
+                o->cbfl[(size_t)q * (nx + ny) + b] = ff * xin;
+            }
+        }
+    }
     if (pp) memcpy(o->fpsi_old, o->fpsi, 2 * (size_t)N * sizeof(double));
     free(fms);
     free(fmb);
@@ -946,9 +1003,15 @@ static void phase8_update(ora *o)
     const double aTP = 4.0 / 3.0, bTP = 1.0 / 3.0;
     size_t yplane = (size_t)o->nyf * nx;
     int pp = o->pp, Q = o->Q;
+    /* the last column / row owns the second copy of its periodic east / north
+     * face (Face_C, MeshCartesian.cpp:28-60): its own flux in the pseudopotential
+     * reference order and with Allen-Cahn UW faces */
+    int copies = o->ref_order || (!pp && o->c.flux == MPD_FLUX_UW && !o->one_copy);
+    size_t nb = (size_t)nx + ny;
     for (int q = 0; q < Q; ++q)
         for (int j = 0; j < ny; ++j)
             for (int i = 0; i < nx; ++i) {
+                int cE = copies && i == nx - 1, cN = copies && !o->walls && j == ny - 1;
                 size_t k = (size_t)q * N + (size_t)j * nx + i;
                 size_t kE = (size_t)q * N + (size_t)j * nx + wrap(i + 1, nx);
                 size_t kW = k;
@@ -959,17 +1022,14 @@ static void phase8_update(ora *o)
                 const double *fx = o->ffl, *fy = o->ffl + (size_t)Q * N;
                 if (!pp) {
                     double s = 0.0;
-                    s += -1 * hx[kE];
-                    s += -1 * hy[kN];
+                    s += -1 * (cE ? o->cbhl[q * nb + j] : hx[kE]);
+                    s += -1 * (cN ? o->cbhl[q * nb + ny + i] : hy[kN]);
                     s += 1 * hx[kW];
                     s += 1 * hy[kS];
                     o->h[k] = aTP * o->hbp[k] - bTP * o->h[k] + o->dtv * s;
                 }
-                /* reference order: the last column / row owns the second copy of
-                 * its periodic east / north face (Face_C, MeshCartesian.cpp:28-60) */
-                double fE = fx[kE], fN = fy[kN];
-                if (o->ref_order && i == nx - 1) fE = o->cbfl[(size_t)q * (nx + ny) + j];
-                if (o->ref_order && j == ny - 1) fN = o->cbfl[(size_t)q * (nx + ny) + ny + i];
+                double fE = cE ? o->cbfl[q * nb + j] : fx[kE];
+                double fN = cN ? o->cbfl[q * nb + ny + i] : fy[kN];
                 double s = 0.0;
                 s += -1 * fE;
                 s += -1 * fN;
@@ -1141,8 +1201,8 @@ int ora_step(ora *o, int nsteps)
         for (size_t n = 0; n < (size_t)o->N; ++n) phase1_cell(o, n);
         if (o->walls) phase2_walls(o);
         if (o->c.flux == MPD_FLUX_UW) {
-            if (!o->pp) phase3_grad(o, o->hbp, o->hbpx, o->hbpy);
-            phase3_grad(o, o->fbp, o->fbpx, o->fbpy);
+            if (!o->pp) phase3_grad(o, o->hbp, o->ghbp, o->hbpx, o->hbpy);
+            phase3_grad(o, o->fbp, o->gfbp, o->fbpx, o->fbpy);
         }
         phase4_faces(o);
         phase8_update(o);

@@ -19,6 +19,7 @@ int ora_cell_phases(ora *o);
 /* pseudopotential D2Q9 (periodic, CD): 1 = visit faces in the reference's
  * one-thread order, 0 = Jacobi order (default) */
 int ora_set_face_order(ora *o, int ref_order);
+int ora_set_one_copy(ora *o, int one);  /* test knob: UW without the second face copies */
 /* face Psi of the last step [2N]; forces of the second copies of the periodic
  * faces [nx + ny] (x-faces by row, then y-faces by column) */
 int ora_set_face_extra(ora *o, const double *psi_old, const double *copy_fx, const double *copy_fy);

@@ -43,6 +43,8 @@ def lib():
         _lib.ora_d2v16_feq.restype = None
         _lib.ora_set_face_order.argtypes = [C.c_void_p, C.c_int]
         _lib.ora_set_face_order.restype = C.c_int
+        _lib.ora_set_one_copy.argtypes = [C.c_void_p, C.c_int]
+        _lib.ora_set_one_copy.restype = C.c_int
         for fn in (_lib.ora_set_face_extra, _lib.ora_get_face_extra):
             fn.argtypes = [C.c_void_p] * 4
             fn.restype = C.c_int
@@ -95,6 +97,11 @@ class Oracle:
         one-thread order (Ark/Cpp/DUGKS2DSolver.cpp:219-230), False = Jacobi."""
         if lib().ora_set_face_order(self._h, 1 if reference else 0) != 0:
             raise ValueError("reference face order: pseudopotential D2Q9, CD, periodic only")
+
+    def set_one_copy(self, one):
+        """Test knob: Allen-Cahn UW with one face per location (without the
+        second stored copy of each periodic face the reference keeps)."""
+        lib().ora_set_one_copy(self._h, 1 if one else 0)
 
     def set_face_extra(self, psi_old=None, copy_fx=None, copy_fy=None):
         arrs = [None if a is None else np.ascontiguousarray(a, dtype=np.float64)

@@ -33,9 +33,10 @@ STEPS = [0, 1, 2, 10, 100]
 CONFIGS = {"hcz": ("ac_hcz", STEPS), "acp": ("ac_p", STEPS), "hcz_wall": ("ac_hcz", STEPS),
            # walls with non-equilibrium extrapolation (Wall_3_NEE)
            "hcz_nee": ("ac_hcz", STEPS), "acp_nee": ("ac_p", STEPS),
-           # UW face scheme: bit-exact at step 1; afterwards the reference's two copies
-           # of each periodic face differ in the last bit (see tests/test_oracle_ref_live.py)
-           "hcz_uw": ("ac_hcz", [0, 1]), "acp_uw": ("ac_p", [0, 1]),
+           # UW face scheme, periodic and with walls (the reference keeps two copies of
+           # each periodic face; the oracle follows both, tests/test_oracle_ref_live.py)
+           "hcz_uw": ("ac_hcz", STEPS), "acp_uw": ("ac_p", STEPS), "hcz_uw_wall": ("ac_hcz", STEPS),
+           "hcz_uw_nee": ("ac_hcz", STEPS), "acp_uw_nee": ("ac_p", STEPS),
            # pseudopotential: the face force depends on the face visiting order; the
            # reference runs with one thread, which the oracle's reference-order mode follows
            "pp": ("pp_d2q9", STEPS),
@@ -63,7 +64,7 @@ def main(names=None):
         if names and cfg not in names:
             continue
         out = run(cfg, steps, max(steps))
-        rec = {"model": model, "walls_y": cfg.endswith(("_wall", "_nee")), "uw": cfg.endswith("_uw"),
+        rec = {"model": model, "walls_y": cfg.endswith(("_wall", "_nee")), "uw": "_uw" in cfg,
                "bc": "nee" if cfg.endswith("_nee") else ("bb" if cfg.endswith("_wall") else "periodic"),
                "nx": 128, "ny": 128,
                "source": "reference Ark, unmodified sources, OMP_NUM_THREADS=1, %s"

@@ -24,6 +24,13 @@ CASES = [
     ("acp_cd_nee_64x48", dict(model="ac_p", nx=64, ny=48, walls=True, nee=True, radius=21.0)),
     ("hcz_uw_periodic_96x64", dict(model="ac_hcz", nx=96, ny=64, flux=abi.MPD_FLUX_UW, radius=20.0)),
     ("acp_uw_periodic_64", dict(model="ac_p", nx=64, ny=64, flux=abi.MPD_FLUX_UW, radius=20.0)),
+    # UW with walls: ghost rows in the 6-point gradients, the wall face's shadow side
+    ("hcz_uw_walls_64x48_droplet", dict(model="ac_hcz", nx=64, ny=48, flux=abi.MPD_FLUX_UW, walls=True, sign=-1,
+                                        radius=21.0)),
+    ("hcz_uw_nee_70x47", dict(model="ac_hcz", nx=70, ny=47, flux=abi.MPD_FLUX_UW, walls=True, nee=True,
+                              radius=20.0)),
+    ("acp_uw_nee_64x48", dict(model="ac_p", nx=64, ny=48, flux=abi.MPD_FLUX_UW, walls=True, nee=True,
+                              radius=21.0)),
     ("pp_cd_periodic_64", dict(model="pp_d2q9", nx=64, ny=64)),
     ("pp_uw_periodic_80x48", dict(model="pp_d2q9", nx=80, ny=48, flux=abi.MPD_FLUX_UW, radius=15.0)),
 ]
@@ -150,3 +157,22 @@ def test_pp_face_psi_resume_matches_continuous_run():
     _cmp(c.download(), ref, "resumed")
     for x in (a, b, c):
         x.close()
+
+
+@pytest.mark.parametrize("name", ["hcz_uw", "acp_uw", "hcz_uw_wall", "hcz_uw_nee", "acp_uw_nee"])
+def test_uw_bit_exact_vs_reference(name):
+    """The FP64 UW step (periodic, bounce-back and NEE walls) reproduces the
+    compiled reference's own state digests after 1, 2, 10 and 100 steps, bit
+    for bit: the second stored copy of each periodic face included."""
+    from test_oracle_golden import check_state, golden_case, load
+    g = load(name)
+    cfg, st = golden_case(g)
+    s = solver.Solver(cfg, path=abi.MPD_PATH_PHASES)
+    s.upload(st)
+    done = 0
+    for k in sorted(int(x) for x in g["steps"] if int(x) > 0):
+        s.step(k - done)
+        done = k
+        bad = check_state(s.download(), g["steps"][str(k)])
+        assert bad == [], "%s step %d: planes %s" % (name, k, bad)
+    s.close()

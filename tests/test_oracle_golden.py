@@ -3,11 +3,12 @@
 tests/golden/ref_*.json hold SHA-256 digests of the full cell state of the
 compiled reference (Ark, unmodified sources) after 0, 1, 2, 10 and 100 steps of
 its AC_Drop case on its 128x128 mesh (D2Q9HCZ, D2Q9AC, and both with walls in
-y: bounce-back for HCZ, non-equilibrium extrapolation for HCZ and AC-p), and of
-its SCMP_Drop case (D2Q9PP, Carnahan-Starling), centred and shifted across both
-periodic lines.  The host initialiser and the oracle must reproduce every
-digest bit for bit; for the pseudopotential model that is the oracle's
-reference face order (one thread's visit, Ark/Cpp/DUGKS2DSolver.cpp:219-230).
+y: bounce-back for HCZ, non-equilibrium extrapolation for HCZ and AC-p; each
+with CD faces and with UW faces), and of its SCMP_Drop case (D2Q9PP,
+Carnahan-Starling), centred and shifted across both periodic lines.  The host
+initialiser and the oracle must reproduce every digest bit for bit; for the
+pseudopotential model that is the oracle's reference face order (one thread's
+visit, Ark/Cpp/DUGKS2DSolver.cpp:219-230).
 """
 import json
 import os
@@ -21,7 +22,7 @@ from helpers import digest, make_case, oracle_run
 GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
 CASES = ["hcz", "acp", "hcz_wall", "hcz_nee", "acp_nee"]
 PP_CASES = ["pp", "pp_shift"]
-UW_CASES = ["hcz_uw", "acp_uw"]
+UW_CASES = ["hcz_uw", "acp_uw", "hcz_uw_wall", "hcz_uw_nee", "acp_uw_nee"]
 
 
 def load(name):
@@ -55,6 +56,11 @@ def shifted(st, shift):
     return solver.State(st.cfg, roll(st.h), roll(st.f), roll(st.m), st.ffx, st.ffy)
 
 
+def golden_case(g):
+    return make_case(g["model"], walls=g["walls_y"], nee=g.get("bc") == "nee",
+                     flux=abi.MPD_FLUX_UW if g.get("uw") else abi.MPD_FLUX_CD)
+
+
 def pp_case(g):
     cfg, st = make_case(g["model"])
     return cfg, shifted(st, g.get("shift"))
@@ -66,7 +72,7 @@ def test_initial_state_matches_reference(name):
     if name in PP_CASES:
         cfg, st = pp_case(g)
     else:
-        cfg, st = make_case(g["model"], walls=g["walls_y"], nee=g.get("bc") == "nee")
+        cfg, st = golden_case(g)
     assert check_state(st, g["steps"]["0"]) == []
 
 
@@ -136,18 +142,14 @@ def test_oracle_pp_jacobi_order_gap(oracle_mod):
     assert abs(rho_ref.sum() - rho_jac.sum()) < 1e-12 * rho_ref.sum()
 
 
-@pytest.mark.parametrize("name", UW_CASES)
-def test_oracle_uw_first_step_bit_exact(name, oracle_mod):
-    g = load(name)
-    cfg, st = make_case(g["model"], flux=abi.MPD_FLUX_UW)
-    out = oracle_run(cfg, st, [1])
-    assert check_state(out[1], g["steps"]["1"]) == []
-
-
-@pytest.mark.parametrize("name", CASES)
+@pytest.mark.parametrize("name", CASES + UW_CASES)
 def test_oracle_bit_exact_vs_reference(name, oracle_mod):
+    """UW cases: the reference stores each periodic face twice (zones 12 and 8,
+    each owned by its mesh cell) and its xi.n = 0 average adds the owner first,
+    so the two copies round apart; the oracle carries the second copy's flux for
+    the last column (and row, periodic y), and every digest holds to 100 steps."""
     g = load(name)
-    cfg, st = make_case(g["model"], walls=g["walls_y"], nee=g.get("bc") == "nee")
+    cfg, st = golden_case(g)
     steps = sorted(int(k) for k in g["steps"] if int(k) > 0)
     out = oracle_run(cfg, st, steps)
     for k in steps:

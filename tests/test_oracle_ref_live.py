@@ -2,12 +2,13 @@
 that need full reference fields as input.  Skipped where the reference sources
 (and so the compiled reference) are absent.
 
-* UW face scheme (reference built with the UW flag set): bit-exact at step 1
-  (also pinned by tests/golden/ref_*_uw.json); later the reference's two stored
-  copies of each periodic boundary face (zones 12 and 8, owner swapped) can
-  differ in the last bit, because for xi.n = 0 its UW average adds owner and
-  neighbour in owner-first order (Ark/Cpp/DUGKS2DSolver.cpp:580-603); the
-  oracle keeps one face, so the states agree to a few ulp.
+* UW face scheme (reference built with the UW flag set), periodic and with
+  bounce-back or NEE walls: bit-exact from the reference's own state over 10
+  steps (and from the initialiser over 100, tests/golden/ref_*_uw*.json).  The
+  reference's two stored copies of each periodic face (zones 12 and 8, each
+  owned by its mesh cell) round apart, because for xi.n = 0 its UW average adds
+  the owner first (Ark/Cpp/DUGKS2DSolver.cpp:580-603); the oracle carries the
+  second copy's flux for the last column and row.
 * Pseudopotential: the reference's face force reads neighbour-face Psi inside
   the loop that writes them (Ark/Cpp/DUGKS2DSolver.cpp:219-230, 733-746), so a
   whole step is order dependent.  The cell phases (Update_MacroVar with the
@@ -55,22 +56,49 @@ def face_forces(path):
     return fx, fy
 
 
-@pytest.mark.parametrize("name,model", [("hcz_uw", abi.MPD_MODEL_AC_HCZ), ("acp_uw", abi.MPD_MODEL_AC_P)])
-def test_uw_against_reference(name, model, oracle_mod):
+UW_LIVE = [("hcz_uw", abi.MPD_MODEL_AC_HCZ, abi.MPD_BC_PERIODIC),
+           ("acp_uw", abi.MPD_MODEL_AC_P, abi.MPD_BC_PERIODIC),
+           ("hcz_uw_wall", abi.MPD_MODEL_AC_HCZ, abi.MPD_BC_WALL_BB),
+           ("acp_uw_nee", abi.MPD_MODEL_AC_P, abi.MPD_BC_WALL_NEE)]
+
+
+@pytest.mark.parametrize("name,model,bc", UW_LIVE, ids=[c[0] for c in UW_LIVE])
+def test_uw_against_reference(name, model, bc, oracle_mod):
     out = ref_run.run(name, [0, 1, 10], 10)
     cfg = default_config(model)
-    cfg.flux = abi.MPD_FLUX_UW
+    cfg.flux, cfg.bc_y = abi.MPD_FLUX_UW, bc
     o = oracle_mod.Oracle(cfg)
     o.set_state(*to_state(ref_run.load_cells(os.path.join(out, "cells_step0.bin"))))
-    o.step(1)
-    h, f, m, _, _ = o.get_state()
-    rh, rf, rm = to_state(ref_run.load_cells(os.path.join(out, "cells_step1.bin")))
-    assert np.array_equal(h, rh) and np.array_equal(f, rf) and np.array_equal(m, rm)
-    o.step(9)
-    h, f, m, _, _ = o.get_state()
-    rh, rf, rm = to_state(ref_run.load_cells(os.path.join(out, "cells_step10.bin")))
-    assert np.abs(h - rh).max() <= 1e-14 * np.abs(rh).max()
-    assert np.abs(f - rf).max() <= 1e-14 * np.abs(rf).max()
+    done = 0
+    for k in (1, 10):
+        o.step(k - done)
+        done = k
+        h, f, m, _, _ = o.get_state()
+        rh, rf, rm = to_state(ref_run.load_cells(os.path.join(out, "cells_step%d.bin" % k)))
+        assert np.array_equal(h, rh) and np.array_equal(f, rf) and np.array_equal(m, rm), k
+
+
+def test_uw_second_copy_matters(oracle_mod):
+    """The second copy is not redundant: from the reference's step-1 state, one
+    periodic UW step with one face per location (the oracle's test knob; the
+    last column and row then take the first copy's flux) misses the
+    reference's step in the last bits, and the default oracle hits it."""
+    out = ref_run.run("hcz_uw", [1, 2], 2)
+    cfg = default_config(abi.MPD_MODEL_AC_HCZ)
+    cfg.flux = abi.MPD_FLUX_UW
+    s1 = to_state(ref_run.load_cells(os.path.join(out, "cells_step1.bin")))
+    r2 = to_state(ref_run.load_cells(os.path.join(out, "cells_step2.bin")))
+    got = {}
+    for one in (False, True):
+        o = oracle_mod.Oracle(cfg)
+        o.set_state(*s1)
+        o.set_one_copy(one)
+        o.step(1)
+        got[one] = o.get_state()[:3]
+    assert all(np.array_equal(a, b) for a, b in zip(got[False], r2))
+    assert not all(np.array_equal(a, b) for a, b in zip(got[True], r2))
+    dev = max(float(np.abs(a - b).max() / np.abs(b).max()) for a, b in zip(got[True], r2))
+    assert dev < 1e-14, dev
 
 
 def test_pseudopotential_cell_phases_bit_exact(oracle_mod):
