@@ -284,6 +284,14 @@ int mpd_init_pp_drop(const mpd_config *cfg, double cx, double cy, double radius,
                      double rho_v, double width, double mu0, double *tilde_f, double *macro,
                      double *face_fx, double *face_fy);
 
+/* SCMP_FlatInterface (Ark/Cpp/Preprocess.cpp:291-336): a liquid band,
+ * rho = rho_v + (rho_l - rho_v)/2 (tanh(slope (y - y_lo)) - tanh(slope (y - y_hi))),
+ * u = 0, tilde = equilibrium; the reference's case is slope = 0.4,
+ * y_lo = ChLength/4, y_hi = 3 ChLength/4. */
+int mpd_init_pp_flat(const mpd_config *cfg, double y_lo, double y_hi, double slope, double rho_l,
+                     double rho_v, double mu0, double *tilde_f, double *macro, double *face_fx,
+                     double *face_fy);
+
 /* Pseudopotential liquid-vapour start: rho = rho0*(1 + U(-amp, amp)) per cell,
  * std::mt19937_64(seed), u = 0, tilde = equilibrium. */
 int mpd_init_pp_noise(const mpd_config *cfg, double rho0, double amp, uint64_t seed, double mu0,
@@ -316,8 +324,15 @@ int mpd_device_init_interface_noalloc(mpd_handle *h, const double *y0, int32_t s
 
 /* Pseudopotential noisy start: rho = rho0 (1 + U(-amp, amp)) per cell with a
  * counter-based splitmix64 stream of (seed, cell) (not the host's mt19937_64
- * sequence), u = 0, tilde = equilibrium, face forces 0. */
+ * sequence), u = 0, tilde = equilibrium, face forces and face Psi 0. */
 int mpd_device_init_pp_noise(mpd_handle *h, double rho0, double amp, uint64_t seed, double mu0);
+
+/* mpd_init_pp_drop (SCMP_Drop) and mpd_init_pp_flat (SCMP_FlatInterface) on the
+ * device; face forces and the previous step's face Psi start at 0. */
+int mpd_device_init_pp_drop(mpd_handle *h, double cx, double cy, double radius, double rho_l, double rho_v,
+                            double width, double mu0);
+int mpd_device_init_pp_flat(mpd_handle *h, double y_lo, double y_hi, double slope, double rho_l, double rho_v,
+                            double mu0);
 
 /* ---------------- output files of the reference driver ----------------
  * Byte-identical writers of the files the reference's driver writes

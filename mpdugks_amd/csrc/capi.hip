@@ -658,6 +658,27 @@ int mpd_device_init_pp_noise(mpd_handle *h, double rho0, double amp, uint64_t se
     return device_state_ready(h);
 }
 
+int mpd_device_init_pp_drop(mpd_handle *h, double cx, double cy, double radius, double rho_l, double rho_v,
+                            double width, double mu0)
+{
+    if (!h) return fail(MPD_ERR_INVALID, "null handle");
+    if (h->batch) return fail(MPD_ERR_STATE, "handle is a member of a batch");
+    if (!is_pp(h->cfg)) return fail(MPD_ERR_INVALID, "SCMP_Drop starts the pseudopotential models");
+    if (!(width > 0.0)) return fail(MPD_ERR_INVALID, "width must be positive");
+    devinit_pp_drop(h->S, h->C, cx, cy, radius, rho_l, rho_v, width, mu0, h->stream);
+    return device_state_ready(h);
+}
+
+int mpd_device_init_pp_flat(mpd_handle *h, double y_lo, double y_hi, double slope, double rho_l, double rho_v,
+                            double mu0)
+{
+    if (!h) return fail(MPD_ERR_INVALID, "null handle");
+    if (h->batch) return fail(MPD_ERR_STATE, "handle is a member of a batch");
+    if (!is_pp(h->cfg)) return fail(MPD_ERR_INVALID, "SCMP_FlatInterface starts the pseudopotential models");
+    devinit_pp_flat(h->S, h->C, y_lo, y_hi, slope, rho_l, rho_v, mu0, h->stream);
+    return device_state_ready(h);
+}
+
 int mpd_download_state(mpd_handle *h, double *tilde_h, double *tilde_f, double *macro, double *face_fx,
                        double *face_fy)
 {

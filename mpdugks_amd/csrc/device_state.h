@@ -40,5 +40,9 @@ void devinit_interface(const DevState &S, const Consts &C, const double *d_y0, i
                        hipStream_t st);
 void devinit_noise(const DevState &S, const Consts &C, double rho0, double amp, uint64_t seed, double mu0,
                    hipStream_t st);
+void devinit_pp_drop(const DevState &S, const Consts &C, double cx, double cy, double radius, double rho_l,
+                     double rho_v, double width, double mu0, hipStream_t st);
+void devinit_pp_flat(const DevState &S, const Consts &C, double y_lo, double y_hi, double slope, double rho_l,
+                     double rho_v, double mu0, hipStream_t st);
 
 }  // namespace mpd

@@ -2,7 +2,8 @@
 // no host arrays and no upload (a 1024x1024 FP64 state is ~300 MB over PCIe).
 // They follow the host initialisers of hostinit.cpp (AC_Drop,
 // Ark/Cpp/Preprocess.cpp:705-760 with AnalyticalPhiAC_Drop :402-419; the
-// interface layer; the noisy pseudopotential start), with the device's tanh
+// interface layer; SCMP_Drop and SCMP_FlatInterface, :291-401; the noisy
+// pseudopotential start), with the device's tanh
 // and sqrt: the states agree with the host ones to a few units in the last
 // place, not bit for bit.  Compiled without contraction.
 #include "d2v16.h"
@@ -105,6 +106,30 @@ __global__ void __launch_bounds__(256) k_init_noise(DevState S, Consts C, double
     write_rest_state(S, C, n, 0.0, rho, rho * C.rt, mu0, C.pp_tau);
 }
 
+// SCMP_Drop (Ark/Cpp/Preprocess.cpp:337-401), the expression of mpd_init_pp_drop
+__global__ void __launch_bounds__(256) k_init_pp_drop(DevState S, Consts C, double cx, double cy, double radius,
+                                                      double rho_l, double rho_v, double width, double mu0)
+{
+    const size_t n = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= (size_t)S.N) return;
+    const int i = (int)(n % C.nx), j = (int)(n / C.nx);
+    const double x = (i + 0.5) * C.dx, y = (j + 0.5) * C.dx;
+    const double rho = 0.5 * (rho_l + rho_v) -
+                       0.5 * (-rho_l + rho_v) * tanh(2 * (sqrt((x - cx) * (x - cx) + (y - cy) * (y - cy)) - radius) / width);
+    write_rest_state(S, C, n, 0.0, rho, rho * C.rt, mu0, C.pp_tau);
+}
+
+// SCMP_FlatInterface (Ark/Cpp/Preprocess.cpp:291-336), the expression of mpd_init_pp_flat
+__global__ void __launch_bounds__(256) k_init_pp_flat(DevState S, Consts C, double y_lo, double y_hi, double slope,
+                                                      double rho_l, double rho_v, double mu0)
+{
+    const size_t n = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= (size_t)S.N) return;
+    const double y = ((int)(n / C.nx) + 0.5) * C.dx;
+    const double rho = rho_v + 0.5 * (rho_l - rho_v) * (tanh(slope * (y - y_lo)) - tanh(slope * (y - y_hi)));
+    write_rest_state(S, C, n, 0.0, rho, rho * C.rt, mu0, C.pp_tau);
+}
+
 __global__ void k_zero(double *p, size_t n)
 {
     const size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -127,13 +152,36 @@ void devinit_interface(const DevState &S, const Consts &C, const double *d_y0, i
     hipLaunchKernelGGL(k_init_interface, dim3(blocks(S.N)), dim3(256), 0, st, S, C, d_y0, sign, p0, width);
 }
 
+// pseudopotential face forces, and the face Psi the reference face order keeps
+// from the previous step, start at zero (as mpd_upload_state leaves them)
+static void zero_pp_faces(const DevState &S, hipStream_t st)
+{
+    double *planes[] = {S.ffx, S.ffy, S.fpsi_old};
+    for (double *p : planes)
+        if (p) hipLaunchKernelGGL(k_zero, dim3(blocks(2 * S.N)), dim3(256), 0, st, p, 2 * (size_t)S.N);
+}
+
 void devinit_noise(const DevState &S, const Consts &C, double rho0, double amp, uint64_t seed, double mu0,
                    hipStream_t st)
 {
     hipLaunchKernelGGL(k_init_noise, dim3(blocks(S.N)), dim3(256), 0, st, S, C, rho0, amp, seed, mu0);
-    // pseudopotential face forces start at zero
-    if (S.ffx) hipLaunchKernelGGL(k_zero, dim3(blocks(2 * S.N)), dim3(256), 0, st, S.ffx, 2 * (size_t)S.N);
-    if (S.ffy) hipLaunchKernelGGL(k_zero, dim3(blocks(2 * S.N)), dim3(256), 0, st, S.ffy, 2 * (size_t)S.N);
+    zero_pp_faces(S, st);
+}
+
+void devinit_pp_drop(const DevState &S, const Consts &C, double cx, double cy, double radius, double rho_l,
+                     double rho_v, double width, double mu0, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_init_pp_drop, dim3(blocks(S.N)), dim3(256), 0, st, S, C, cx, cy, radius, rho_l, rho_v,
+                       width, mu0);
+    zero_pp_faces(S, st);
+}
+
+void devinit_pp_flat(const DevState &S, const Consts &C, double y_lo, double y_hi, double slope, double rho_l,
+                     double rho_v, double mu0, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_init_pp_flat, dim3(blocks(S.N)), dim3(256), 0, st, S, C, y_lo, y_hi, slope, rho_l, rho_v,
+                       mu0);
+    zero_pp_faces(S, st);
 }
 
 }  // namespace mpd

@@ -166,6 +166,37 @@ int mpd_init_pp_drop(const mpd_config *cfg, double cx, double cy, double radius,
     return 0;
 }
 
+// SCMP_FlatInterface (Ark/Cpp/Preprocess.cpp:291-336): a liquid band between
+// y_lo and y_hi, rho = rho_v + (rho_l - rho_v)/2 (tanh(slope (y - y_lo)) -
+// tanh(slope (y - y_hi))), p = rho RT, u = 0, tilde = equilibrium.  The
+// reference's case is slope 0.4, y_lo = ChLength/4, y_hi = 3 ChLength/4.
+int mpd_init_pp_flat(const mpd_config *cfg, double y_lo, double y_hi, double slope, double rho_l, double rho_v,
+                     double mu0, double *tilde_f, double *macro, double *face_fx, double *face_fy)
+{
+    if (!valid(cfg) || cfg->model < MPD_MODEL_PP_D2Q9 || !tilde_f || !macro) return MPD_ERR_INVALID;
+    const mpd_config &c = *cfg;
+    const size_t N = (size_t)c.nx * c.ny;
+    std::memset(macro, 0, sizeof(double) * MPD_M_COUNT * N);
+    double feq[16], heq[16];
+    const int nq = num_q(c);
+    for (int j = 0; j < c.ny; ++j) {
+        const double y = (j + 0.5) * c.dx;
+        const double rho = rho_v + 0.5 * (rho_l - rho_v) * (std::tanh(slope * (y - y_lo)) - std::tanh(slope * (y - y_hi)));
+        cell_eq(c, 0.0, rho, 0.0, 0.0, rho * c.rt, heq, feq);
+        for (int i = 0; i < c.nx; ++i) {
+            const size_t n = (size_t)j * c.nx + i;
+            macro[MPD_M_RHO * N + n] = rho;
+            macro[MPD_M_P * N + n] = rho * c.rt;
+            macro[MPD_M_MU * N + n] = mu0;
+            macro[MPD_M_TAU * N + n] = c.pp_tau;
+            for (int q = 0; q < nq; ++q) tilde_f[q * N + n] = feq[q];
+        }
+    }
+    if (face_fx) std::memset(face_fx, 0, sizeof(double) * 2 * N);
+    if (face_fy) std::memset(face_fy, 0, sizeof(double) * 2 * N);
+    return 0;
+}
+
 // Liquid-vapour separation start (pseudopotential): rho = rho0*(1 + U(-amp, amp))
 // per cell from a seeded std::mt19937_64, u = 0, tilde = equilibrium.
 int mpd_init_pp_noise(const mpd_config *cfg, double rho0, double amp, uint64_t seed, double mu0,

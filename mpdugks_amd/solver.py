@@ -54,6 +54,8 @@ _SYMBOLS = {
     "mpd_init_pp_drop": ([C.c_void_p, C.c_double, C.c_double, C.c_double, C.c_double, C.c_double,
                           C.c_double, C.c_double, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p],
                          C.c_int),
+    "mpd_init_pp_flat": ([C.c_void_p, C.c_double, C.c_double, C.c_double, C.c_double, C.c_double,
+                          C.c_double, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p], C.c_int),
     "mpd_init_pp_noise": ([C.c_void_p, C.c_double, C.c_double, C.c_uint64, C.c_double, C.c_void_p,
                            C.c_void_p, C.c_void_p, C.c_void_p], C.c_int),
     "mpd_device_init_drops": ([C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int32, C.c_double,
@@ -61,6 +63,8 @@ _SYMBOLS = {
     "mpd_device_init_interface": ([C.c_void_p, C.c_void_p, C.c_int32, C.c_double], C.c_int),
     "mpd_device_init_interface_noalloc": ([C.c_void_p, C.c_void_p, C.c_int32, C.c_double], C.c_int),
     "mpd_device_init_pp_noise": ([C.c_void_p, C.c_double, C.c_double, C.c_uint64, C.c_double], C.c_int),
+    "mpd_device_init_pp_drop": ([C.c_void_p] + [C.c_double] * 7, C.c_int),
+    "mpd_device_init_pp_flat": ([C.c_void_p] + [C.c_double] * 6, C.c_int),
     "mpd_default_output_info": ([C.c_void_p, C.c_void_p], C.c_int),
     "mpd_output_name": ([C.c_int32, C.c_void_p, C.c_int64, C.c_double, C.c_char_p, C.c_int32], C.c_int),
     "mpd_write_flowfield": ([C.c_char_p, C.c_int32, C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p], C.c_int),
@@ -160,6 +164,19 @@ def scmp_drop(cfg, cx=None, cy=None, radius=None, rho_l=2.4806E-1, rho_v=4.5435E
     return s
 
 
+def scmp_flat(cfg, y_lo=None, y_hi=None, slope=0.4, rho_l=2.4806E-1, rho_v=4.5435E-2, mu0=0.1):
+    """Initial state of SCMP_FlatInterface (Ark/Cpp/Preprocess.cpp:291-336): a
+    liquid band between y_lo and y_hi (default a quarter and three quarters of
+    the mesh width, the reference's ChLength)."""
+    L = cfg.nx * cfg.dx
+    y_lo = 0.25 * L if y_lo is None else y_lo
+    y_hi = 0.75 * L if y_hi is None else y_hi
+    s = State(cfg)
+    _check(lib().mpd_init_pp_flat(C.byref(cfg), y_lo, y_hi, slope, rho_l, rho_v, mu0, _p(s.f), _p(s.m),
+                                  _p(s.ffx), _p(s.ffy)), "mpd_init_pp_flat")
+    return s
+
+
 def pp_noise(cfg, rho0, amp=0.01, seed=0, mu0=0.1):
     s = State(cfg)
     _check(lib().mpd_init_pp_noise(C.byref(cfg), rho0, amp, int(seed), mu0, _p(s.f), _p(s.m),
@@ -234,6 +251,23 @@ class Solver:
 
     def init_pp_noise(self, rho0, amp=0.01, seed=0, mu0=0.1):
         _check(lib().mpd_device_init_pp_noise(self._h, rho0, amp, int(seed), mu0), "mpd_device_init_pp_noise")
+
+    def init_pp_drop(self, cx=None, cy=None, radius=None, rho_l=2.4806E-1, rho_v=4.5435E-2, width=5.0, mu0=0.1):
+        """SCMP_Drop built on the device (defaults as scmp_drop)."""
+        L = self.cfg.nx * self.cfg.dx
+        cx = 0.5 * L if cx is None else cx
+        cy = 0.5 * self.cfg.ny * self.cfg.dx if cy is None else cy
+        radius = 0.25 * L if radius is None else radius
+        _check(lib().mpd_device_init_pp_drop(self._h, cx, cy, radius, rho_l, rho_v, width, mu0),
+               "mpd_device_init_pp_drop")
+
+    def init_pp_flat(self, y_lo=None, y_hi=None, slope=0.4, rho_l=2.4806E-1, rho_v=4.5435E-2, mu0=0.1):
+        """SCMP_FlatInterface built on the device (defaults as scmp_flat)."""
+        L = self.cfg.nx * self.cfg.dx
+        y_lo = 0.25 * L if y_lo is None else y_lo
+        y_hi = 0.75 * L if y_hi is None else y_hi
+        _check(lib().mpd_device_init_pp_flat(self._h, y_lo, y_hi, slope, rho_l, rho_v, mu0),
+               "mpd_device_init_pp_flat")
 
     def upload(self, s):
         arrs = [_f64(a) for a in (s.h, s.f, s.m, s.ffx, s.ffy)]

@@ -45,6 +45,9 @@ CONFIGS = {
     # SCMP_Drop shifted by (60, 70) cells: the drop crosses both periodic lines
     "pp_shift": ("D2Q9PP", "-include " + os.path.join(CFGDIR, "pp", "flags.h"), False,
                  os.path.join(CFGDIR, "pp_shift")),
+    # SCMP_FlatInterface: a liquid band across the periodic x line
+    "pp_flat": ("D2Q9PP", "-include " + os.path.join(CFGDIR, "pp", "flags.h"), False,
+                os.path.join(CFGDIR, "pp_flat")),
 }
 
 CELL_SCALARS = ["Phi", "Rho", "U", "V", "p", "Phi_x", "Phi_y", "Rho_x", "Rho_y",

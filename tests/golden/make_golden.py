@@ -41,7 +41,9 @@ CONFIGS = {"hcz": ("ac_hcz", STEPS), "acp": ("ac_p", STEPS), "hcz_wall": ("ac_hc
            # reference runs with one thread, which the oracle's reference-order mode follows
            "pp": ("pp_d2q9", STEPS),
            # the same drop shifted cyclically by (60, 70) cells: it crosses both periodic lines
-           "pp_shift": ("pp_d2q9", STEPS)}
+           "pp_shift": ("pp_d2q9", STEPS),
+           # SCMP_FlatInterface: a liquid band across the periodic x line
+           "pp_flat": ("pp_d2q9", STEPS)}
 SHIFT = {"pp_shift": [60, 70]}  # (x, y) cells, oracle/refcfg/pp_shift/ZeroCondition.h
 
 
@@ -68,10 +70,13 @@ def main(names=None):
                "bc": "nee" if cfg.endswith("_nee") else ("bb" if cfg.endswith("_wall") else "periodic"),
                "nx": 128, "ny": 128,
                "source": "reference Ark, unmodified sources, OMP_NUM_THREADS=1, %s"
-                         % ("SCMP_Drop" if cfg.startswith("pp") else "AC_Drop"),
+                         % ("SCMP_FlatInterface" if cfg == "pp_flat" else
+                            "SCMP_Drop" if cfg.startswith("pp") else "AC_Drop"),
                "steps": {}}
         if cfg in SHIFT:
             rec["shift"] = SHIFT[cfg]
+        if cfg == "pp_flat":
+            rec["start"] = "flat"
         for k in steps:
             d = load_cells(os.path.join(out, "cells_step%d.bin" % k))
             r = state_digests(d)

@@ -89,3 +89,45 @@ def test_device_start_then_fused_fp32():
     s.step(50)
     r = s.reductions()
     assert r["status"] == 0 and r["nonfinite"] == 0
+
+
+@pytest.mark.parametrize("model,start", [("pp_d2q9", "drop"), ("pp_d2q9", "flat"), ("pp_d2v16", "drop")])
+def test_pp_starts_match_host(model, start):
+    """SCMP_Drop and SCMP_FlatInterface built on the device against the host
+    initialisers (bit-exact with the reference at step 0), on a handle that
+    has stepped before: the face forces and the previous step's face Psi (read
+    by the reference face order) are reset as by an upload, so a step from the
+    device start equals a step from the uploaded host start to round-off."""
+    m = abi.MODEL_NAMES[model]
+    cfg = default_config(m, nx=96, ny=80)
+    if start == "drop":
+        host = solver.scmp_drop(cfg, cx=40.0, cy=37.5, radius=21.0)
+    else:
+        host = solver.scmp_flat(cfg, y_lo=20.0, y_hi=60.0)
+    ref_order = m == abi.MPD_MODEL_PP_D2Q9
+    runs = []
+    for device in (True, False):
+        s = solver.Solver(cfg, path=abi.MPD_PATH_PHASES)
+        if ref_order:
+            s.set_face_order(abi.MPD_FACE_ORDER_REFERENCE)
+        s.init_pp_noise(0.16, amp=0.01, seed=1)
+        s.step(3)  # leaves non-zero face forces and face Psi behind
+        if not device:
+            s.upload(host)
+        elif start == "drop":
+            s.init_pp_drop(cx=40.0, cy=37.5, radius=21.0)
+        else:
+            s.init_pp_flat(y_lo=20.0, y_hi=60.0)
+        st = s.download()
+        if device:
+            assert _rel(st.f, host.f) < 1e-14 and _rel(st.m, host.m) < 1e-14
+            assert not st.ffx.any() and not st.ffy.any()
+            if ref_order:
+                assert not s.download_face_psi().any()
+        s.step(5)
+        runs.append(s.download())
+        s.close()
+    a, b = runs
+    assert _rel(a.f, b.f) < 1e-12 and _rel(a.m[abi.M["rho"]], b.m[abi.M["rho"]]) < 1e-12
+    if ref_order:
+        assert _rel(a.ffx, b.ffx) < 1e-9

@@ -90,7 +90,7 @@ def test_nonfinite_state_is_reported():
     s.close()
 
 
-@pytest.mark.parametrize("name", ["pp", "pp_shift"])
+@pytest.mark.parametrize("name", ["pp", "pp_shift", "pp_flat"])
 def test_pp_reference_face_order_bit_exact_vs_reference(name):
     """The FP64 pseudopotential step in the reference's one-thread face order
     reproduces the compiled reference's own state digests (tests/golden,

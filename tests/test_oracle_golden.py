@@ -5,7 +5,8 @@ compiled reference (Ark, unmodified sources) after 0, 1, 2, 10 and 100 steps of
 its AC_Drop case on its 128x128 mesh (D2Q9HCZ, D2Q9AC, and both with walls in
 y: bounce-back for HCZ, non-equilibrium extrapolation for HCZ and AC-p; each
 with CD faces and with UW faces), and of its SCMP_Drop case (D2Q9PP,
-Carnahan-Starling), centred and shifted across both periodic lines.  The host
+Carnahan-Starling), centred and shifted across both periodic lines, and of its
+SCMP_FlatInterface case (a liquid band).  The host
 initialiser and the oracle must reproduce every digest bit for bit; for the
 pseudopotential model that is the oracle's reference face order (one thread's
 visit, Ark/Cpp/DUGKS2DSolver.cpp:219-230).
@@ -21,7 +22,7 @@ from helpers import digest, make_case, oracle_run
 
 GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
 CASES = ["hcz", "acp", "hcz_wall", "hcz_nee", "acp_nee"]
-PP_CASES = ["pp", "pp_shift"]
+PP_CASES = ["pp", "pp_shift", "pp_flat"]
 UW_CASES = ["hcz_uw", "acp_uw", "hcz_uw_wall", "hcz_uw_nee", "acp_uw_nee"]
 
 
@@ -63,6 +64,8 @@ def golden_case(g):
 
 def pp_case(g):
     cfg, st = make_case(g["model"])
+    if g.get("start") == "flat":
+        st = solver.scmp_flat(cfg)
     return cfg, shifted(st, g.get("shift"))
 
 
