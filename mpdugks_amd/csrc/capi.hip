@@ -259,8 +259,8 @@ int check_cfg(const mpd_config *c)
         return fail(MPD_ERR_INVALID, "unknown precision %d", c->precision);
     if (c->precision == MPD_FP32_BULK && !fused_supported(*c))
         return fail(MPD_ERR_INVALID, "no FP32-bulk kernel for this model / flux / boundary (FP32-bulk: Allen-Cahn "
-                                     "with CD faces and periodic, bounce-back or NEE y, or UW faces on a periodic "
-                                     "mesh; pseudopotential D2Q9 with CD faces, periodic)");
+                                     "with CD or UW faces and periodic, bounce-back or NEE y; pseudopotential D2Q9 "
+                                     "with CD faces, periodic)");
     if (c->precision == MPD_FP32_BULK && !is_pp(*c)) {
         // the FP32 step kernel addresses each state set through one 32-bit buffer
         // resource (all float planes) and the FP64 phase field through another

@@ -484,7 +484,7 @@ __device__ __forceinline__ void cell_bp(const K32 &k, const Rs &out, int nx, int
     S[11 * PS + c] = make_float2(0.5f * Fx, 0.5f * Fy);
     if (WALLS == 2 && interior && (gj == 0 || top)) {
 #pragma unroll
-        for (int z = 0; z < 3; ++z) s_nee[(top ? 3 * TX : 0) + 3 * (rx - 1) + z] = make_float2(nh[z], nf[z]);
+        for (int z = 0; z < 3; ++z) s_nee[(top ? 3 * TX : 0) + 3 * (rx - G::H) + z] = make_float2(nh[z], nf[z]);
     }
     if (interior) {
         if (HCZ || WALLS) st(out.a, n, phi, P_PHL * out.pb);
@@ -574,17 +574,24 @@ __device__ __forceinline__ void grad_uw(const float2 *b, const FaceIdxUW &x, flo
     gn = DIR == 0 ? gx : gy;
     gt = DIR == 0 ? gy : gx;
 }
-template <int DIR, int Q>
-__device__ __forceinline__ float2 recon_uw(const K32 &k, const float2 *b, const FaceIdxUW &x)
+// wall < 0 (> 0): a y-face on the bottom (top) wall, whose owner (neighbour) is
+// the wall's shadow cell; its gradient is 0, as in the reference, which never
+// computes the gradients of shadow cells (Grad_VS_6points runs over the mesh
+// cells, DUGKS2DSolver.cpp:197-206)
+template <int DIR, int Q, int WALLS>
+__device__ __forceinline__ float2 recon_uw(const K32 &k, const float2 *b, const FaceIdxUW &x, int wall)
 {
     constexpr int EN = DIR == 0 ? ex9(Q) : ey9(Q), ET = DIR == 0 ? ey9(Q) : ex9(Q);
     constexpr int OC = 1, OR = 1, NC = DIR == 0 ? 2 : 1, NR = DIR == 0 ? 1 : 2;
     const float ct = ET == 0 ? 0.f : (ET > 0 ? -k.uw_t : k.uw_t);   // -h xi_t / (12 dx)
+    const bool zo = WALLS && DIR == 1 && wall < 0, zn = WALLS && DIR == 1 && wall > 0;
+    const float2 zero = make_float2(0.f, 0.f);
     float2 v;
     if (EN != 0) {
         float2 gn, gt;
         if (EN > 0) grad_uw<DIR, OC, OR>(b, x, gn, gt);
         else grad_uw<DIR, NC, NR>(b, x, gn, gt);
+        if (EN > 0 ? zo : zn) gn = gt = zero;
         const float2 c = b[EN > 0 ? x.o : x.n];
         const float cn = EN > 0 ? k.uw_n : -k.uw_n;                  // (x_f - h xi_n - x_c) / (12 dx)
         v.x = fmaf(cn, gn.x, c.x);
@@ -597,6 +604,8 @@ __device__ __forceinline__ float2 recon_uw(const K32 &k, const float2 *b, const 
         float2 gno, gto, gnn, gtn;
         grad_uw<DIR, OC, OR>(b, x, gno, gto);
         grad_uw<DIR, NC, NR>(b, x, gnn, gtn);
+        if (zo) gno = gto = zero;
+        if (zn) gnn = gtn = zero;
         const float2 bo = b[x.o], bn = b[x.n];
         float2 vo, vn;
         vo.x = fmaf(k.uw_0, gno.x, bo.x);
@@ -656,7 +665,7 @@ __device__ __forceinline__ void face(const K32 &k, const float2 *S, const X &x, 
 #define MPD_RECON(Q)                                                                             \
     {                                                                                            \
         float2 v;                                                                                \
-        if constexpr (FLUX == MPD_FLUX_UW) v = recon_uw<DIR, Q>(k, S + Q * PS, x);               \
+        if constexpr (FLUX == MPD_FLUX_UW) v = recon_uw<DIR, Q, WALLS>(k, S + Q * PS, x, wall);  \
         else v = recon<DIR, Q>(k, S + Q * PS, x);                                                \
         hv[Q] = v.x;                                                                             \
         fv[Q] = v.y;                                                                             \
@@ -837,7 +846,6 @@ __device__ __forceinline__ void tile32(const K32 &k, int nx, int ny, const Fused
 {
     constexpr bool HCZ = MODEL == MPD_MODEL_AC_HCZ;
     constexpr int H = G::H, H2 = G::H + 1;
-    static_assert(H == 1 || WALLS == 0, "UW faces: periodic meshes");
     static_assert((FLUX == MPD_FLUX_UW) == (H == 2), "halo of the face scheme");
     constexpr int TX = G::TX, TY = G::TY, NT = G::NT, R1X = G::R1X, R1Y = G::R1Y, R2Y = G::R2Y, R2N = G::R2N;
     constexpr int NIN = G::NIN, NRING = G::NRING, NXF = G::NXF, NYF = G::NYF, PS = G::PS, FS = G::FS;
@@ -918,8 +926,8 @@ __device__ __forceinline__ void tile32(const K32 &k, int nx, int ny, const Fused
     if (WALLS) {
         // ghost rows (j = -1 below the bottom wall, j = ny above the top wall):
         // WallShadowC_fBP / ExtrapolationfBP (Boundary.cpp:38-117), 2 near - next
-        const int gy0 = j0 == 0 ? 0 : -1;                       // R1 row of j = -1
-        const int gy1 = ny - j0 + 1 < R1Y ? ny - j0 + 1 : -1;   // R1 row of j = ny
+        const int gy0 = j0 == 0 ? H - 1 : -1;                   // R1 row of j = -1
+        const int gy1 = ny - j0 + H < R1Y ? ny - j0 + H : -1;   // R1 row of j = ny
         const int row = tid < R1X ? gy0 : (tid < 2 * R1X ? gy1 : -1);
         if (row >= 0) {
             const int rx = tid < R1X ? tid : tid - R1X;
@@ -1291,8 +1299,16 @@ void fused32_launch_uw(const K32 &k, const FusedPlanes32 &in, const FusedPlanes3
     const int ntx = (C.nx + 15) / 16, nty = (C.ny + 14) / 15;
     const int ntiles = ntx * nty;
     dim3 g((unsigned)(((ntiles + 7) / 8) * 8)), b(256);
-    hipLaunchKernelGGL((k_fused32<MODEL, 0, 16, 15, MPD_FLUX_UW>), g, b, 0, st, k, C.nx, C.ny, in, out, N, ntx,
-                       ntiles);
+    const int w = walls_code(C);
+    if (w == 2)
+        hipLaunchKernelGGL((k_fused32<MODEL, 2, 16, 15, MPD_FLUX_UW>), g, b, 0, st, k, C.nx, C.ny, in, out, N, ntx,
+                           ntiles);
+    else if (w)
+        hipLaunchKernelGGL((k_fused32<MODEL, 1, 16, 15, MPD_FLUX_UW>), g, b, 0, st, k, C.nx, C.ny, in, out, N, ntx,
+                           ntiles);
+    else
+        hipLaunchKernelGGL((k_fused32<MODEL, 0, 16, 15, MPD_FLUX_UW>), g, b, 0, st, k, C.nx, C.ny, in, out, N, ntx,
+                           ntiles);
 }
 
 void fused32_step(const FusedPlanes32 &in, const FusedPlanes32 &out, const Consts &C, int64_t N, int tile,
@@ -1300,7 +1316,7 @@ void fused32_step(const FusedPlanes32 &in, const FusedPlanes32 &out, const Const
 {
     const K32 k = make_k32(C);
     if (C.flux == MPD_FLUX_UW) {
-        // UW faces: 16x15 tiles, periodic meshes
+        // UW faces: 16x15 tiles (the 3-cell halo)
         if (C.model == MPD_MODEL_AC_HCZ) fused32_launch_uw<MPD_MODEL_AC_HCZ>(k, in, out, C, N, st);
         else fused32_launch_uw<MPD_MODEL_AC_P>(k, in, out, C, N, st);
         return;

@@ -636,7 +636,7 @@ bool fused_supported(const mpd_config &cfg)
         return cfg.precision == MPD_FP32_BULK && cfg.flux == MPD_FLUX_CD && cfg.bc_y == MPD_BC_PERIODIC;
     const bool ac = (cfg.model == MPD_MODEL_AC_HCZ || cfg.model == MPD_MODEL_AC_P) && cfg.nx >= 4 && cfg.ny >= 4;
     // UW faces: the FP32-bulk tile kernel on periodic meshes (the FP64 fused kernel is CD only)
-    if (cfg.flux == MPD_FLUX_UW) return ac && cfg.precision == MPD_FP32_BULK && cfg.bc_y == MPD_BC_PERIODIC;
+    if (cfg.flux == MPD_FLUX_UW) return ac && cfg.precision == MPD_FP32_BULK;
     return ac && cfg.flux == MPD_FLUX_CD;
 }
 

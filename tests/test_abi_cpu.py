@@ -61,13 +61,11 @@ def test_interface_initialiser_profile():
 
 
 def test_fp32_bulk_without_a_kernel_is_refused_at_create():
-    """FP32-bulk exists for the Allen-Cahn models (CD: periodic, bounce-back or
-    NEE y; UW: periodic) and the pseudopotential D2Q9 model (CD, periodic);
+    """FP32-bulk exists for the Allen-Cahn models (CD and UW faces; periodic,
+    bounce-back or NEE y) and the pseudopotential D2Q9 model (CD, periodic);
     other combinations are refused by mpd_create before any device call."""
     L = solver.lib()
     bad = [
-        (abi.MPD_MODEL_AC_HCZ, dict(flux=abi.MPD_FLUX_UW), abi.MPD_BC_WALL_BB),
-        (abi.MPD_MODEL_AC_P, dict(flux=abi.MPD_FLUX_UW), abi.MPD_BC_WALL_NEE),
         (abi.MPD_MODEL_PP_D2Q9, dict(flux=abi.MPD_FLUX_UW), abi.MPD_BC_PERIODIC),
         (abi.MPD_MODEL_PP_D2V16, dict(), abi.MPD_BC_PERIODIC),
     ]
@@ -113,16 +111,19 @@ def test_fp32_mesh_beyond_buffer_range_refused():
 
 
 def test_fp32_uw_configurations():
-    """FP32-bulk UW faces: Allen-Cahn on periodic meshes (past the checks: on a
-    machine without a GPU mpd_create then fails with MPD_ERR_DEVICE); with walls
-    they are refused with MPD_ERR_INVALID before any device call."""
+    """FP32-bulk UW faces: the Allen-Cahn models with periodic, bounce-back and
+    NEE y pass the checks (on a machine without a GPU mpd_create then fails
+    with MPD_ERR_DEVICE); the pseudopotential model is refused with
+    MPD_ERR_INVALID before any device call."""
     for model in (abi.MPD_MODEL_AC_HCZ, abi.MPD_MODEL_AC_P):
-        cfg = default_config(model, nx=48, ny=32)
-        cfg.precision, cfg.flux = abi.MPD_FP32_BULK, abi.MPD_FLUX_UW
-        h = C.c_void_p()
-        rc = solver.lib().mpd_create(C.byref(cfg), C.byref(h))
-        if rc == 0:
-            solver.lib().mpd_destroy(h)
-        assert rc in (0, -2), rc
-        cfg.bc_y = abi.MPD_BC_WALL_BB
-        assert solver.lib().mpd_create(C.byref(cfg), C.byref(C.c_void_p())) == -1
+        for bc in (abi.MPD_BC_PERIODIC, abi.MPD_BC_WALL_BB, abi.MPD_BC_WALL_NEE):
+            cfg = default_config(model, nx=48, ny=32)
+            cfg.precision, cfg.flux, cfg.bc_y = abi.MPD_FP32_BULK, abi.MPD_FLUX_UW, bc
+            h = C.c_void_p()
+            rc = solver.lib().mpd_create(C.byref(cfg), C.byref(h))
+            if rc == 0:
+                solver.lib().mpd_destroy(h)
+            assert rc in (0, -2), (model, bc, rc)
+    cfg = default_config(abi.MPD_MODEL_PP_D2Q9, nx=48, ny=32)
+    cfg.precision, cfg.flux = abi.MPD_FP32_BULK, abi.MPD_FLUX_UW
+    assert solver.lib().mpd_create(C.byref(cfg), C.byref(C.c_void_p())) == -1
