@@ -174,8 +174,8 @@ int mpd_download_face_psi(mpd_handle *h, double *psi);
 /* Tile shape of the one-launch step kernels (cells per workgroup): the
  * FP32-bulk Allen-Cahn kernel has all three with CD faces and 16x15 with UW
  * faces (a 3-cell halo), the FP32-bulk pseudopotential
- * kernel 16x15 and 32x16, the FP64 Allen-Cahn kernel 16x15 (other shapes are
- * refused with MPD_ERR_INVALID).
+ * kernel 16x15 and 32x16, the FP64 Allen-Cahn kernel 16x15 and 16x16 (other
+ * shapes are refused with MPD_ERR_INVALID).
  * MPD_TILE_AUTO (the default) chooses per mesh; the others force one.  The
  * state does not depend on it (bit for bit with periodic and BB walls, to
  * FP32 rounding with NEE walls).  Takes effect at the next mpd_step;

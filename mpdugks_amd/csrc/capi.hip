@@ -439,8 +439,8 @@ int mpd_set_tile(mpd_handle *h, int32_t tile)
     if (tile < MPD_TILE_AUTO || tile > MPD_TILE_32X16) return fail(MPD_ERR_INVALID, "unknown tile shape %d", tile);
     if (is_pp(h->cfg) && tile == MPD_TILE_16X16)
         return fail(MPD_ERR_INVALID, "the pseudopotential step has 16x15 and 32x16 tiles");
-    if (!is_pp(h->cfg) && h->cfg.precision == MPD_FP64 && tile != MPD_TILE_AUTO && tile != MPD_TILE_16X15)
-        return fail(MPD_ERR_INVALID, "the FP64 step has 16x15 tiles");
+    if (!is_pp(h->cfg) && h->cfg.precision == MPD_FP64 && tile == MPD_TILE_32X16)
+        return fail(MPD_ERR_INVALID, "the FP64 step has 16x15 and 16x16 tiles");
     if (h->cfg.flux == MPD_FLUX_UW && tile != MPD_TILE_AUTO && tile != MPD_TILE_16X15)
         return fail(MPD_ERR_INVALID, "the UW step has 16x15 tiles");
     h->tile = tile;
@@ -452,7 +452,8 @@ int mpd_get_tile(mpd_handle *h, int32_t *tile)
     if (!h || !tile) return fail(MPD_ERR_INVALID, "null argument");
     if (h->tile) *tile = h->tile;
     else if (is_pp(h->cfg)) *tile = pp32_tile_auto(h->cfg.nx, h->cfg.ny, h->cus);
-    else if (h->cfg.precision == MPD_FP64 || h->cfg.flux == MPD_FLUX_UW) *tile = MPD_TILE_16X15;
+    else if (h->cfg.flux == MPD_FLUX_UW) *tile = MPD_TILE_16X15;
+    else if (h->cfg.precision == MPD_FP64) *tile = fused64_tile_auto(h->cfg.nx, h->cfg.ny, h->cus);
     else *tile = fused32_tile_auto(h->cfg.nx, h->cfg.ny, h->cus, fused32_state_bytes(h->cfg));
     return 0;
 }
@@ -745,6 +746,8 @@ int mpd_step(mpd_handle *h, int64_t nsteps)
         else if (h->cfg.precision == MPD_FP32_BULK)
             h->F.tile = h->tile ? h->tile
                                 : fused32_tile_auto(h->cfg.nx, h->cfg.ny, h->cus, fused32_state_bytes(h->cfg));
+        else
+            h->F.tile = h->tile ? h->tile : fused64_tile_auto(h->cfg.nx, h->cfg.ny, h->cus);
         int64_t s = 0;
         if (!h->fused_primed && !fused_needs_priming(h->cfg)) {
             // explicit state carried as is (FP32-bulk pseudopotential)

@@ -60,7 +60,7 @@ struct FusedState {
     bool pp = false;     // FP32-bulk pseudopotential: explicit FP32 planes, no derived state
     PP32State pp32;
     bool pp_force_stale = false;  // the stored cell force predates the last update
-    int tile = MPD_TILE_16X15;   // FP32-bulk Allen-Cahn: tile shape of the step kernel (mpd_tile)
+    int tile = MPD_TILE_16X15;   // Allen-Cahn: tile shape of the step kernel (mpd_tile)
 };
 
 // FP32 planes of the derived state (see fused_kernels.hip); phd is the
@@ -75,6 +75,8 @@ void fused32_step(const FusedPlanes32 &in, const FusedPlanes32 &out, const Const
 // tile shape (mpd_tile) of the FP32-bulk step for an nx x ny mesh on a GPU with
 // `cus` compute units, whose two state sets take `state_bytes` of device memory
 int fused32_tile_auto(int nx, int ny, int cus, int64_t state_bytes);
+// tile shape (MPD_TILE_16X15 or MPD_TILE_16X16) of the FP64 Allen-Cahn step
+int fused64_tile_auto(int nx, int ny, int cus);
 // device bytes of the two FP32-bulk Allen-Cahn state sets of a configuration
 int64_t fused32_state_bytes(const mpd_config &cfg);
 // planes of one Allen-Cahn state set (fused_alloc)

@@ -21,10 +21,12 @@
 // (w_q - delta_q0) p_ref/RT for AC-p).  Every expression that touches p is
 // linear in p, so the same code runs with p read as p - p_ref.
 //
-// Tile: 16 x 15 cells per 256-thread workgroup.  LDS holds bar-plus (18
-// planes) and the six face-averaged sources on the tile plus a one-cell halo,
-// phi/phi_lag on a two-cell halo, and the face fluxes: x-faces (17x15 = 255)
-// and y-faces (16x16 = 256) map one face per thread.
+// Tile: 16 x 15 or 16 x 16 cells per 256-thread workgroup.  LDS holds
+// bar-plus (18 planes) and the six face-averaged sources on the tile plus a
+// one-cell halo, phi/phi_lag on a two-cell halo, and the face fluxes.  16 x 15:
+// x-faces (17x15 = 255) and y-faces (16x16 = 256) map one face per thread;
+// 16 x 16: 272 of each, the last 16 of each direction on one wave each in a
+// second pass.
 #include <hip/hip_runtime.h>
 
 #include "fused.h"
@@ -44,7 +46,10 @@ struct Tile64 {
     static constexpr int R1N = R1X * R1Y, R2N = R2X * R2Y;
     static constexpr int NXF = (TX + 1) * TY;         // x-faces
     static constexpr int NYF = TX * (TY + 1);         // y-faces
-    static_assert(TX * TY <= NTHREADS && NXF <= NTHREADS && NYF <= NTHREADS, "one cell and face per thread");
+    // faces beyond one per thread: a second pass on wave 2 (x) and wave 3 (y)
+    static constexpr int EXF = NXF > NTHREADS ? NXF - NTHREADS : 0;
+    static constexpr int EYF = NYF > NTHREADS ? NYF - NTHREADS : 0;
+    static_assert(TX * TY <= NTHREADS && EXF <= 64 && EYF <= 64, "one cell and at most two face passes per thread");
     static_assert(12 * (NXF + NYF) <= (2 * Q9 + 6) * R1N, "face fluxes must fit over the bar-plus planes");
 };
 
@@ -360,12 +365,36 @@ __global__ void __launch_bounds__(NTHREADS, 2) k_fused_step(Consts C, Planes<T> 
     // with a non-zero normal velocity; the other three carry a zero flux) in
     // registers until every face has read the bar-plus planes, then writes
     // them over those planes.
+    // 16 x 16: the faces past one per thread first, on waves 2 (x) and 3 (y),
+    // straight into LDS of their own (the face planes overlay the bar-plus
+    // planes, which the other faces still read)
+    constexpr int EXF = G::EXF, EYF = G::EYF;
+    constexpr int MXF = NXF < NTHREADS ? NXF : NTHREADS, MYF = NYF < NTHREADS ? NYF : NTHREADS;
+    __shared__ T s_ext[EXF + EYF > 0 ? 12 * (EXF + EYF) : 1];
+    if (EXF > 0 && tid >= 128 && tid < 128 + EXF) {
+        const int e = tid - 128;
+        T h[6], f[6];
+        xface64<T, MODEL, UNIT, G>(C, s_bp, s_mac, NTHREADS + e, h, f);
+#pragma unroll
+        for (int z = 0; z < 6; ++z) { s_ext[z * EXF + e] = h[z]; s_ext[(6 + z) * EXF + e] = f[z]; }
+    }
+    if (EYF > 0 && tid >= 192 && tid < 192 + EYF) {
+        const int e = tid - 192;
+        T h[6], f[6];
+        yface64<T, MODEL, WALLS, UNIT, G>(C, s_bp, s_mac, s_nee, NTHREADS + e, j0, ny, h, f);
+#pragma unroll
+        for (int z = 0; z < 6; ++z) {
+            s_ext[12 * EXF + z * EYF + e] = h[z];
+            s_ext[12 * EXF + (6 + z) * EYF + e] = f[z];
+        }
+    }
+    if (EXF + EYF > 0) __builtin_amdgcn_sched_barrier(0);
     // keep the two face passes apart: interleaved, they hold both faces'
     // operands at once (past 256 VGPRs, one wave per SIMD)
     T xh[6], xf[6], yh[6], yf[6];
-    if (tid < NXF) xface64<T, MODEL, UNIT, G>(C, s_bp, s_mac, tid, xh, xf);
+    if (tid < MXF) xface64<T, MODEL, UNIT, G>(C, s_bp, s_mac, tid, xh, xf);
     __builtin_amdgcn_sched_barrier(0);
-    if (tid < NYF) yface64<T, MODEL, WALLS, UNIT, G>(C, s_bp, s_mac, s_nee, tid, j0, ny, yh, yf);
+    if (tid < MYF) yface64<T, MODEL, WALLS, UNIT, G>(C, s_bp, s_mac, s_nee, tid, j0, ny, yh, yf);
     // own bar-plus before the region is reused for the face fluxes
     const int ci = tid % TX, cj = tid / TX;
     const bool cell = tid < TX * TY;
@@ -376,19 +405,28 @@ __global__ void __launch_bounds__(NTHREADS, 2) k_fused_step(Consts C, Planes<T> 
         for (int q = 0; q < Q9; ++q) { hbp[q] = s_bp[q][kc]; fbp[q] = s_bp[Q9 + q][kc]; }
     }
     __syncthreads();
-    if (tid < NXF) {
+    if (tid < MXF) {
 #pragma unroll
         for (int z = 0; z < 6; ++z) {
             s_fx[z * NXF + tid] = xh[z];
             s_fx[(6 + z) * NXF + tid] = xf[z];
         }
     }
-    if (tid < NYF) {
+    if (tid < MYF) {
 #pragma unroll
         for (int z = 0; z < 6; ++z) {
             s_fy[z * NYF + tid] = yh[z];
             s_fy[(6 + z) * NYF + tid] = yf[z];
         }
+    }
+    if (EXF > 0 && tid < EXF) {
+#pragma unroll
+        for (int z = 0; z < 12; ++z) s_fx[z * NXF + NTHREADS + tid] = s_ext[z * EXF + tid];
+    }
+    if (EYF > 0 && tid >= 64 && tid < 64 + EYF) {
+        const int e = tid - 64;
+#pragma unroll
+        for (int z = 0; z < 12; ++z) s_fy[z * NYF + NTHREADS + e] = s_ext[12 * EXF + z * EYF + e];
     }
     __syncthreads();
 
@@ -546,10 +584,9 @@ Planes<T> planes_of(FusedState &F, int which)
     return P;
 }
 
-template <typename T, int MODEL, int WALLS, bool UNIT>
+template <typename T, int MODEL, int WALLS, bool UNIT, class G>
 void launch_step(FusedState &F, const Consts &C, hipStream_t st)
 {
-    using G = Tile64<16, 15>;
     Planes<T> in = planes_of<T>(F, F.cur), out = planes_of<T>(F, 1 - F.cur);
     const int ntx = (C.nx + G::TX - 1) / G::TX, nty = (C.ny + G::TY - 1) / G::TY;
     const int ntiles = ntx * nty;
@@ -564,8 +601,13 @@ void launch_step(FusedState &F, const Consts &C, hipStream_t st)
 {
     // MaSpan 1 and dx a power of two: constant lattice velocities and face
     // gradients by the exact reciprocal of dx (the same results bit for bit)
-    if (C.unit) launch_step<T, MODEL, WALLS, true>(F, C, st);
-    else launch_step<T, MODEL, WALLS, false>(F, C, st);
+    if (F.tile == MPD_TILE_16X16) {
+        if (C.unit) launch_step<T, MODEL, WALLS, true, Tile64<16, 16>>(F, C, st);
+        else launch_step<T, MODEL, WALLS, false, Tile64<16, 16>>(F, C, st);
+    } else {
+        if (C.unit) launch_step<T, MODEL, WALLS, true, Tile64<16, 15>>(F, C, st);
+        else launch_step<T, MODEL, WALLS, false, Tile64<16, 15>>(F, C, st);
+    }
 }
 
 template <typename T>
@@ -629,13 +671,24 @@ FusedPlanes32 fused32_planes(FusedState &F, int which)
     return FusedPlanes32{a.h, a.f, a.phi, a.phl, a.ppu, a.ppv, a.gl, a.phd};
 }
 
+int fused64_tile_auto(int nx, int ny, int cus)
+{
+    // two workgroups per CU (LDS); 16 x 16 where its tiles take fewer rounds of
+    // those slots than 16 x 15 tiles (e.g. 256 x 1024: 1,024 tiles, two full
+    // rounds, against 1,104), else 16 x 15 (fewer instructions per cell)
+    const int64_t slots = 2 * (int64_t)(cus > 0 ? cus : 256);
+    const int64_t t15 = (int64_t)((nx + 15) / 16) * ((ny + 14) / 15);
+    const int64_t t16 = (int64_t)((nx + 15) / 16) * ((ny + 15) / 16);
+    return (t16 + slots - 1) / slots < (t15 + slots - 1) / slots ? MPD_TILE_16X16 : MPD_TILE_16X15;
+}
+
 bool fused_supported(const mpd_config &cfg)
 {
     // FP32-bulk pseudopotential D2Q9: the FP32 phase kernels of pp32.hip
     if (cfg.model == MPD_MODEL_PP_D2Q9)
         return cfg.precision == MPD_FP32_BULK && cfg.flux == MPD_FLUX_CD && cfg.bc_y == MPD_BC_PERIODIC;
     const bool ac = (cfg.model == MPD_MODEL_AC_HCZ || cfg.model == MPD_MODEL_AC_P) && cfg.nx >= 4 && cfg.ny >= 4;
-    // UW faces: the FP32-bulk tile kernel on periodic meshes (the FP64 fused kernel is CD only)
+    // UW faces: the FP32-bulk tile kernel (the FP64 fused kernel is CD only)
     if (cfg.flux == MPD_FLUX_UW) return ac && cfg.precision == MPD_FP32_BULK;
     return ac && cfg.flux == MPD_FLUX_CD;
 }

@@ -61,6 +61,34 @@ def test_fused_fp64_bit_exact_vs_oracle(name, kw, oracle_mod):
     s.close()
 
 
+FP64_16X16 = [
+    ("hcz_periodic_96x64", dict(model="ac_hcz", nx=96, ny=64, radius=20.0)),
+    ("acp_walls_70x47", dict(model="ac_p", nx=70, ny=47, walls=True, radius=15.0)),
+    ("hcz_nee_64x48_droplet", dict(model="ac_hcz", nx=64, ny=48, walls=True, nee=True, sign=-1, radius=21.0)),
+    ("hcz_walls_48x40_dx0.5", dict(model="ac_hcz", nx=48, ny=40, walls=True, radius=6.0, dx=0.5, dt=0.25)),
+    ("hcz_periodic_8x6", dict(model="ac_hcz", nx=8, ny=6, radius=2.5)),
+]
+
+
+@pytest.mark.parametrize("name,kw", FP64_16X16, ids=[c[0] for c in FP64_16X16])
+def test_fused_fp64_16x16_bit_exact_vs_oracle(name, kw, oracle_mod):
+    """16x16 tiles (the faces past one per thread in a second pass) give the
+    oracle's state bit for bit, as 16x15 tiles do."""
+    cfg, st = make_case(**kw)
+    steps = [1, 3, 25]
+    ref = oracle_run(cfg, st, steps)
+    s = solver.Solver(cfg, path=abi.MPD_PATH_FUSED)
+    s.set_tile(abi.MPD_TILE_16X16)
+    s.upload(st)
+    done = 0
+    for k in steps:
+        s.step(k - done)
+        done = k
+        _assert_equal(s.download(), ref[k], "%s 16x16 step %d" % (name, k))
+    assert s.tile() == abi.MPD_TILE_16X16
+    s.close()
+
+
 def test_fused_then_phases_continues_exactly(oracle_mod):
     cfg, st = make_case("ac_hcz", nx=64, ny=64, radius=20.0)
     ref = oracle_run(cfg, st, [12, 20])

@@ -91,14 +91,24 @@ def test_tile_choice_and_validation():
 
 
 def test_fp64_tile_validation():
+    """The FP64 step has 16x15 and 16x16 tiles (two workgroups per CU by LDS).
+    256x1024 on a 256-CU device: 1,104 tiles of 16x15 take three rounds of the
+    512 slots, 1,024 tiles of 16x16 two, so the host picks 16x16; where both
+    take the same rounds it keeps 16x15."""
     cfg, st = cases.rayleigh_taylor(nx=256, ny=1024, precision=abi.MPD_FP64)
     s = solver.Solver(cfg, path=abi.MPD_PATH_FUSED)
-    assert s.tile() == abi.MPD_TILE_16X15
-    for t in (abi.MPD_TILE_16X16, abi.MPD_TILE_32X16):
-        with pytest.raises(solver.MpdError):
-            s.set_tile(t)
+    assert s.tile() == abi.MPD_TILE_16X16
+    with pytest.raises(solver.MpdError):
+        s.set_tile(abi.MPD_TILE_32X16)
     s.set_tile(abi.MPD_TILE_16X15)
     assert s.tile() == abi.MPD_TILE_16X15
     s.set_tile(abi.MPD_TILE_AUTO)
-    assert s.tile() == abi.MPD_TILE_16X15
+    assert s.tile() == abi.MPD_TILE_16X16
     s.close()
+    for (nx, ny), want in (((128, 128), abi.MPD_TILE_16X15),     # one round either way
+                           ((768, 768), abi.MPD_TILE_16X15),     # 2,496 / 2,304 tiles: five rounds either way
+                           ((1024, 1024), abi.MPD_TILE_16X16)):  # 4,416 / 4,096 tiles: nine / eight rounds
+        cfg = default_config(abi.MPD_MODEL_AC_HCZ, nx=nx, ny=ny)
+        s = solver.Solver(cfg, path=abi.MPD_PATH_FUSED)
+        assert s.tile() == want, (nx, ny, s.tile())
+        s.close()
