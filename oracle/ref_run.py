@@ -28,6 +28,7 @@ CONFIGS = {
     "hcz_wall": ("D2Q9HCZ", "-D_Wall_3_BCs_FLIP=3 -D_Wall_3_BCs_BB", True, ""),
     "hcz_nee": ("D2Q9HCZ", "-D_Wall_3_BCs_FLIP=3 -D_Wall_3_BCs_NEE", True, ""),
     "acp_nee": ("D2Q9AC", "-D_Wall_3_BCs_FLIP=3 -D_Wall_3_BCs_NEE", True, ""),
+    "acp_wall": ("D2Q9AC", "-D_Wall_3_BCs_FLIP=3 -D_Wall_3_BCs_BB", True, ""),
     "hcz_uw": ("D2Q9HCZ", "-include " + os.path.join(CFGDIR, "uw", "flags.h"), False, ""),
     "acp_uw": ("D2Q9AC", "-include " + os.path.join(CFGDIR, "uw", "flags.h"), False, ""),
     "hcz_uw_wall": ("D2Q9HCZ", "-include " + os.path.join(CFGDIR, "uw", "flags.h") +
@@ -36,6 +37,8 @@ CONFIGS = {
                    " -D_Wall_3_BCs_FLIP=3 -D_Wall_3_BCs_NEE", True, ""),
     "acp_uw_nee": ("D2Q9AC", "-include " + os.path.join(CFGDIR, "uw", "flags.h") +
                    " -D_Wall_3_BCs_FLIP=3 -D_Wall_3_BCs_NEE", True, ""),
+    "acp_uw_wall": ("D2Q9AC", "-include " + os.path.join(CFGDIR, "uw", "flags.h") +
+                    " -D_Wall_3_BCs_FLIP=3 -D_Wall_3_BCs_BB", True, ""),
     "pp": ("D2Q9PP", "-include " + os.path.join(CFGDIR, "pp", "flags.h"), False,
            os.path.join(CFGDIR, "pp")),
     # pseudopotential with bounce-back walls in y (the reference cannot run it: with walls

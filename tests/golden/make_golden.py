@@ -31,12 +31,13 @@ REF_KEYS = {"phi": "Phi", "rho": "Rho", "u": "U", "v": "V", "p": "p", "phix": "P
 STEPS = [0, 1, 2, 10, 100]
 # name -> (model, steps recorded)
 CONFIGS = {"hcz": ("ac_hcz", STEPS), "acp": ("ac_p", STEPS), "hcz_wall": ("ac_hcz", STEPS),
+           "acp_wall": ("ac_p", STEPS),
            # walls with non-equilibrium extrapolation (Wall_3_NEE)
            "hcz_nee": ("ac_hcz", STEPS), "acp_nee": ("ac_p", STEPS),
            # UW face scheme, periodic and with walls (the reference keeps two copies of
            # each periodic face; the oracle follows both, tests/test_oracle_ref_live.py)
            "hcz_uw": ("ac_hcz", STEPS), "acp_uw": ("ac_p", STEPS), "hcz_uw_wall": ("ac_hcz", STEPS),
-           "hcz_uw_nee": ("ac_hcz", STEPS), "acp_uw_nee": ("ac_p", STEPS),
+           "hcz_uw_nee": ("ac_hcz", STEPS), "acp_uw_nee": ("ac_p", STEPS), "acp_uw_wall": ("ac_p", STEPS),
            # pseudopotential: the face force depends on the face visiting order; the
            # reference runs with one thread, which the oracle's reference-order mode follows
            "pp": ("pp_d2q9", STEPS),

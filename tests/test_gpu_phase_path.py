@@ -159,7 +159,7 @@ def test_pp_face_psi_resume_matches_continuous_run():
         x.close()
 
 
-@pytest.mark.parametrize("name", ["hcz_uw", "acp_uw", "hcz_uw_wall", "hcz_uw_nee", "acp_uw_nee"])
+@pytest.mark.parametrize("name", ["hcz_uw", "acp_uw", "hcz_uw_wall", "acp_uw_wall", "hcz_uw_nee", "acp_uw_nee"])
 def test_uw_bit_exact_vs_reference(name):
     """The FP64 UW step (periodic, bounce-back and NEE walls) reproduces the
     compiled reference's own state digests after 1, 2, 10 and 100 steps, bit

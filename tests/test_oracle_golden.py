@@ -3,7 +3,7 @@
 tests/golden/ref_*.json hold SHA-256 digests of the full cell state of the
 compiled reference (Ark, unmodified sources) after 0, 1, 2, 10 and 100 steps of
 its AC_Drop case on its 128x128 mesh (D2Q9HCZ, D2Q9AC, and both with walls in
-y: bounce-back for HCZ, non-equilibrium extrapolation for HCZ and AC-p; each
+y: bounce-back and non-equilibrium extrapolation for both; each
 with CD faces and with UW faces), and of its SCMP_Drop case (D2Q9PP,
 Carnahan-Starling), centred and shifted across both periodic lines, and of its
 SCMP_FlatInterface case (a liquid band).  The host
@@ -21,9 +21,9 @@ from mpdugks_amd import abi, solver
 from helpers import digest, make_case, oracle_run
 
 GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
-CASES = ["hcz", "acp", "hcz_wall", "hcz_nee", "acp_nee"]
+CASES = ["hcz", "acp", "hcz_wall", "acp_wall", "hcz_nee", "acp_nee"]
 PP_CASES = ["pp", "pp_shift", "pp_flat"]
-UW_CASES = ["hcz_uw", "acp_uw", "hcz_uw_wall", "hcz_uw_nee", "acp_uw_nee"]
+UW_CASES = ["hcz_uw", "acp_uw", "hcz_uw_wall", "acp_uw_wall", "hcz_uw_nee", "acp_uw_nee"]
 
 
 def load(name):
