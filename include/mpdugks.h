@@ -157,10 +157,13 @@ int mpd_set_path(mpd_handle *h, int32_t path);
  * visited earlier contributes its new Psi and one visited later the Psi of the
  * previous step.  MPD_FACE_ORDER_JACOBI (default; every path): all face Psi
  * first, then all forces.  MPD_FACE_ORDER_REFERENCE (FP64 pseudopotential D2Q9,
- * CD, periodic; phase path): the reference's one-thread order for a mesh
- * whose faces are numbered as mpdugks_amd.mesh.write_cas writes them
+ * CD or UW faces, periodic; phase path): the reference's one-thread order for a
+ * mesh whose faces are numbered as mpdugks_amd.mesh.write_cas writes them
  * (interior x-faces by rows, interior y-faces by rows, then the periodic zones
- * left, right, bottom, top) -- bit for bit the reference's OMP_NUM_THREADS=1 run. */
+ * left, right, bottom, top) -- bit for bit the reference's OMP_NUM_THREADS=1 run.
+ * With UW faces the two stored copies of each periodic face carry their own Psi
+ * and force, which the downloadable state does not hold: after an upload both
+ * copies start from the first copy's (exact with CD faces, where they agree). */
 typedef enum mpd_face_order { MPD_FACE_ORDER_JACOBI = 0, MPD_FACE_ORDER_REFERENCE = 1 } mpd_face_order;
 int mpd_set_face_order(mpd_handle *h, int32_t order);
 

@@ -350,6 +350,15 @@ int mpd_create(const mpd_config *cfg, mpd_handle **out)
         mpd_destroy(h);
         return r;
     }
+    // FP64 pseudopotential D2Q9 with UW faces: the second copies of the periodic
+    // faces, carried for the reference face order (their previous Psi and force)
+    const bool pp_uw = cfg->model == MPD_MODEL_PP_D2Q9 && uw && cfg->precision == MPD_FP64;
+    const size_t nb = (size_t)cfg->nx + cfg->ny;
+    if ((r = dalloc(h, &S.cbpsi_old, pp_uw ? nb : 0)) || (r = dalloc(h, &S.cbffx, pp_uw ? nb : 0)) ||
+        (r = dalloc(h, &S.cbffy, pp_uw ? nb : 0))) {
+        mpd_destroy(h);
+        return r;
+    }
     // phase-path scratch is allocated lazily in mpd_step (fused path does not need it)
     (void)QN; (void)yfn; (void)uw;
     h->red_blocks = 512;
@@ -403,10 +412,9 @@ int mpd_set_face_order(mpd_handle *h, int32_t order)
     if (order != MPD_FACE_ORDER_JACOBI && order != MPD_FACE_ORDER_REFERENCE)
         return fail(MPD_ERR_INVALID, "unknown face order %d", order);
     if (order == MPD_FACE_ORDER_REFERENCE &&
-        !(h->cfg.model == MPD_MODEL_PP_D2Q9 && h->cfg.precision == MPD_FP64 && h->cfg.flux == MPD_FLUX_CD &&
-          h->cfg.bc_y == MPD_BC_PERIODIC))
+        !(h->cfg.model == MPD_MODEL_PP_D2Q9 && h->cfg.precision == MPD_FP64 && h->cfg.bc_y == MPD_BC_PERIODIC))
         return fail(MPD_ERR_INVALID, "the reference face order is a mode of the FP64 pseudopotential D2Q9 step "
-                                     "(CD faces, periodic mesh)");
+                                     "(periodic mesh)");
     h->C.face_ref = order == MPD_FACE_ORDER_REFERENCE;
     return 0;
 }
@@ -418,6 +426,9 @@ int mpd_upload_face_psi(mpd_handle *h, const double *psi)
         return fail(MPD_ERR_INVALID, "face Psi is state of the FP64 pseudopotential D2Q9 step");
     HIPCHK(hipMemcpyAsync(h->S.fpsi_old, psi, 2 * (size_t)h->S.N * sizeof(double), hipMemcpyHostToDevice,
                           h->stream));
+    // the second copies of the periodic faces take the first copies' Psi
+    pp_copies_from_first(h->S, h->C, false, true, h->stream);
+    HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(h->stream));
     return 0;
 }
@@ -474,7 +485,9 @@ static int alloc_phase_scratch(mpd_handle *h)
         (r = dalloc(h, &S.ghbp, h->C.walls ? (size_t)Q9 * 2 * h->cfg.nx : 0)) ||
         (r = dalloc(h, &S.gfbp, h->C.walls ? (size_t)Q9 * 2 * h->cfg.nx : 0)) ||
         (r = dalloc(h, &S.cbhl, ac && uw ? (size_t)Q9 * (h->cfg.nx + h->cfg.ny) : 0)) ||
-        (r = dalloc(h, &S.cbfl, ac && uw ? (size_t)Q9 * (h->cfg.nx + h->cfg.ny) : 0)) ||
+        (r = dalloc(h, &S.cbfl, (ac && uw) || S.cbpsi_old ? (size_t)Q9 * (h->cfg.nx + h->cfg.ny) : 0)) ||
+        (r = dalloc(h, &S.cbpsi, S.cbpsi_old ? (size_t)(h->cfg.nx + h->cfg.ny) : 0)) ||
+        (r = dalloc(h, &S.cbscr, S.cbpsi_old ? (size_t)4 * (h->cfg.nx + h->cfg.ny) : 0)) ||
         (r = dalloc(h, &S.fpsi, pp ? 2 * N : 0)) || (r = dalloc(h, &S.fscr, pp ? 8 * N : 0)))
         return r;
     return 0;
@@ -528,6 +541,9 @@ int mpd_upload_state(mpd_handle *h, const double *tilde_h, const double *tilde_f
         // face Psi before the first step: 0, as the reference's initialisers leave it
         // (Ark/Cpp/Preprocess.cpp:337-401 never set it); mpd_upload_face_psi sets it
         HIPCHK(hipMemsetAsync(h->S.fpsi_old, 0, 2 * N * sizeof(double), h->stream));
+        // both stored copies of a periodic face start from the uploaded force
+        pp_copies_from_first(h->S, h->C, true, true, h->stream);
+        HIPCHK(hipGetLastError());
     }
     // Rho_1k starts at 0 in the reference (Ark/Header/Macroscopic.h:44)
     HIPCHK(hipMemsetAsync(h->S.rho1k, 0, N * sizeof(double), h->stream));
@@ -735,6 +751,7 @@ int mpd_step(mpd_handle *h, int64_t nsteps)
             phase_step(h->S, h->C, h->stream);
             // this step's face Psi becomes the previous step's for the next one
             if (is_pp(h->cfg)) std::swap(h->S.fpsi, h->S.fpsi_old);
+            if (h->S.cbpsi) std::swap(h->S.cbpsi, h->S.cbpsi_old);
         }
     } else {
         if (!h->F.ready) {

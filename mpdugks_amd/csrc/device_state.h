@@ -17,8 +17,11 @@ struct DevState {
     double *hbpx, *hbpy, *fbpx, *fbpy; // [Q][N] UW gradients
     double *hfl, *ffl;            // face fluxes: x-faces [Q][N] then y-faces [Q][nyf*nx]
     double *gm, *ghbp, *gfbp;     // wall ghost rows
-    double *cbhl, *cbfl;          // Allen-Cahn UW: fluxes of the second copies of the periodic
-                                  // faces [Q][nx + ny] (x-faces by row, then y-faces by column)
+    double *cbhl, *cbfl;          // UW: fluxes of the second copies of the periodic faces
+                                  // [Q][nx + ny] (x-faces by row, then y-faces by column)
+    // pseudopotential D2Q9 with UW faces in the reference face order: the second
+    // copies' Psi (this step, previous step), previous force and face macros [nx + ny]
+    double *cbpsi, *cbpsi_old, *cbffx, *cbffy, *cbscr;
     double *fpsi, *fscr;          // PP face Psi [2N], face scratch [4][2N]
     double *fpsi_old;             // PP face Psi of the previous step [2N] (0 before the first step)
     double *rho1k;                // density of the last residual evaluation
@@ -26,6 +29,9 @@ struct DevState {
 
 struct Consts;
 void phase_step(const DevState &S, const Consts &C, hipStream_t st);
+// second copies of the periodic faces from the first copies (the face at i = 0
+// of each row, j = 0 of each column): forces (ffx, ffy) and/or previous Psi
+void pp_copies_from_first(const DevState &S, const Consts &C, bool forces, bool psi, hipStream_t st);
 
 // start states built on the device (devinit.hip)
 constexpr int MAX_DROPS = 8;

@@ -154,18 +154,23 @@ void devinit_interface(const DevState &S, const Consts &C, const double *d_y0, i
 
 // pseudopotential face forces, and the face Psi the reference face order keeps
 // from the previous step, start at zero (as mpd_upload_state leaves them)
-static void zero_pp_faces(const DevState &S, hipStream_t st)
+static void zero_pp_faces(const DevState &S, const Consts &C, hipStream_t st)
 {
     double *planes[] = {S.ffx, S.ffy, S.fpsi_old};
     for (double *p : planes)
         if (p) hipLaunchKernelGGL(k_zero, dim3(blocks(2 * S.N)), dim3(256), 0, st, p, 2 * (size_t)S.N);
+    // the second copies of the periodic faces (UW, reference face order)
+    const size_t nb = (size_t)C.nx + C.ny;
+    double *copies[] = {S.cbffx, S.cbffy, S.cbpsi_old};
+    for (double *p : copies)
+        if (p) hipLaunchKernelGGL(k_zero, dim3(blocks(nb)), dim3(256), 0, st, p, nb);
 }
 
 void devinit_noise(const DevState &S, const Consts &C, double rho0, double amp, uint64_t seed, double mu0,
                    hipStream_t st)
 {
     hipLaunchKernelGGL(k_init_noise, dim3(blocks(S.N)), dim3(256), 0, st, S, C, rho0, amp, seed, mu0);
-    zero_pp_faces(S, st);
+    zero_pp_faces(S, C, st);
 }
 
 void devinit_pp_drop(const DevState &S, const Consts &C, double cx, double cy, double radius, double rho_l,
@@ -173,7 +178,7 @@ void devinit_pp_drop(const DevState &S, const Consts &C, double cx, double cy, d
 {
     hipLaunchKernelGGL(k_init_pp_drop, dim3(blocks(S.N)), dim3(256), 0, st, S, C, cx, cy, radius, rho_l, rho_v,
                        width, mu0);
-    zero_pp_faces(S, st);
+    zero_pp_faces(S, C, st);
 }
 
 void devinit_pp_flat(const DevState &S, const Consts &C, double y_lo, double y_hi, double slope, double rho_l,
@@ -181,7 +186,7 @@ void devinit_pp_flat(const DevState &S, const Consts &C, double y_lo, double y_h
 {
     hipLaunchKernelGGL(k_init_pp_flat, dim3(blocks(S.N)), dim3(256), 0, st, S, C, y_lo, y_hi, slope, rho_l, rho_v,
                        mu0);
-    zero_pp_faces(S, st);
+    zero_pp_faces(S, C, st);
 }
 
 }  // namespace mpd

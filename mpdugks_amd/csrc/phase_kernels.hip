@@ -253,6 +253,16 @@ __global__ void __launch_bounds__(256) k_faces_ac(DevState S, Consts C)
     }
 }
 
+// With UW faces in the reference face order the second stored copy of each
+// periodic face (zone 8, owned by the last column / row) has its own
+// reconstruction (the xi.n = 0 average adds its owner first), so its own macros,
+// Psi, force and flux: the face passes evaluate those nx + ny objects too.
+template <int FLUX>
+__device__ inline bool pp_copies(const DevState &S, const Consts &C)
+{
+    return FLUX == MPD_FLUX_UW && C.face_ref && S.cbpsi != nullptr;
+}
+
 // Pseudopotential faces, pass 1: face macros and Psi of every face
 // (Update_MacroVar_h, Ark/Lib/D2Q9PP.cpp:110-136; Update_PseudoPsi(face),
 // Ark/Cpp/PseudoP.cpp:23-30).  All Psi are written before any face force reads them.
@@ -260,10 +270,22 @@ template <int FLUX, bool UNIT>
 __global__ void __launch_bounds__(256) k_pp_face_psi(DevState S, Consts C)
 {
     size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= 2 * (size_t)S.N) return;
-    int dir = t < (size_t)S.N ? 0 : 1;
-    size_t fidx = dir == 0 ? t : t - S.N;
-    int i = (int)(fidx % C.nx), j = (int)(fidx / C.nx);
+    const size_t F2 = 2 * (size_t)S.N, nb = (size_t)C.nx + C.ny;
+    // threads past the structured faces: the second copies (pp_copies)
+    const bool copy = t >= F2;
+    if (t >= F2 + (pp_copies<FLUX>(S, C) ? nb : 0)) return;
+    int dir, i, j;
+    if (copy) {
+        const int b = (int)(t - F2);
+        dir = b < C.ny ? 0 : 1;
+        i = dir == 0 ? C.nx : b - C.ny;
+        j = dir == 0 ? b : C.ny;
+    } else {
+        dir = t < (size_t)S.N ? 0 : 1;
+        const size_t fidx = dir == 0 ? t : t - S.N;
+        i = (int)(fidx % C.nx);
+        j = (int)(fidx / C.nx);
+    }
     int oi, oj, ni, nj;
     double xf, yf;
     face_geom(C, dir, i, j, oi, oj, ni, nj, xf, yf);
@@ -279,16 +301,25 @@ __global__ void __launch_bounds__(256) k_pp_face_psi(DevState S, Consts C)
     for (int q = 0; q < Q9; ++q) sy += fb[q] * xi_of<UNIT, double>(C, 1, q);
     double rho = s;
     double U = sx / rho, V = sy / rho;
-    U += 0.5 * C.hdt * S.ffx[t] / rho;
-    V += 0.5 * C.hdt * S.ffy[t] / rho;
+    const size_t b = t - F2;
+    U += 0.5 * C.hdt * (copy ? S.cbffx[b] : S.ffx[t]) / rho;
+    V += 0.5 * C.hdt * (copy ? S.cbffy[b] : S.ffy[t]) / rho;
     double p = C.pp_k * carnahan_starling(rho, C.t_cs);
     double scg, psi;
     calc_psi(p, rho, C.rt, scg, psi);
+    if (copy) {
+        S.cbpsi[b] = psi;
+        S.cbscr[0 * nb + b] = rho;
+        S.cbscr[1 * nb + b] = U;
+        S.cbscr[2 * nb + b] = V;
+        S.cbscr[3 * nb + b] = scg;
+        return;
+    }
     S.fpsi[t] = psi;
-    S.fscr[0 * 2 * (size_t)S.N + t] = rho;
-    S.fscr[1 * 2 * (size_t)S.N + t] = U;
-    S.fscr[2 * 2 * (size_t)S.N + t] = V;
-    S.fscr[3 * 2 * (size_t)S.N + t] = scg;
+    S.fscr[0 * F2 + t] = rho;
+    S.fscr[1 * F2 + t] = U;
+    S.fscr[2 * F2 + t] = V;
+    S.fscr[3 * F2 + t] = scg;
 }
 
 // Face objects of the reference's one-thread face order (Consts::face_ref).
@@ -312,7 +343,7 @@ __device__ inline int64_t face_rank(const Consts &C, int dir, int i, int j)
 // on the referencing face's side of a periodic line; a neighbour visited
 // earlier gives its new Psi, one visited later the Psi of the previous step.
 __device__ inline double psi_seen(const DevState &S, const Consts &C, int dir, int i, int j, int di, int dj,
-                                  int64_t r0)
+                                  int64_t r0, bool copies)
 {
     int a = i + di, b = j + dj;
     if (dir == 0) {
@@ -321,6 +352,10 @@ __device__ inline double psi_seen(const DevState &S, const Consts &C, int dir, i
     } else {
         a = wrapi(a, C.nx);
         b = b < 0 ? b + C.ny : (b > C.ny ? b - C.ny : b);
+    }
+    if (copies && ((dir == 0 && a == C.nx) || (dir == 1 && b == C.ny))) {  // a second copy
+        const size_t k = dir == 0 ? (size_t)b : (size_t)C.ny + a;
+        return face_rank(C, dir, a, b) < r0 ? S.cbpsi[k] : S.cbpsi_old[k];
     }
     const size_t fi = (size_t)dir * S.N + (size_t)(b == C.ny ? 0 : b) * C.nx + (a == C.nx ? 0 : a);
     return face_rank(C, dir, a, b) < r0 ? S.fpsi[fi] : S.fpsi_old[fi];
@@ -333,21 +368,33 @@ template <int FLUX, bool UNIT>
 __global__ void __launch_bounds__(256) k_pp_face_flux(DevState S, Consts C)
 {
     size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= 2 * (size_t)S.N) return;
-    int dir = t < (size_t)S.N ? 0 : 1;
-    size_t fidx = dir == 0 ? t : t - S.N;
-    int i = (int)(fidx % C.nx), j = (int)(fidx / C.nx);
+    const size_t F2 = 2 * (size_t)S.N, nb = (size_t)C.nx + C.ny;
+    const bool copies = pp_copies<FLUX>(S, C), copy = t >= F2;
+    if (t >= F2 + (copies ? nb : 0)) return;
+    int dir, i, j;
+    size_t fidx = 0;
+    if (copy) {
+        const int b = (int)(t - F2);
+        dir = b < C.ny ? 0 : 1;
+        i = dir == 0 ? C.nx : b - C.ny;
+        j = dir == 0 ? b : C.ny;
+    } else {
+        dir = t < (size_t)S.N ? 0 : 1;
+        fidx = dir == 0 ? t : t - S.N;
+        i = (int)(fidx % C.nx);
+        j = (int)(fidx / C.nx);
+    }
     double c[4], d[4];
     if (C.face_ref) {
         const int64_t r0 = face_rank(C, dir, i, j);
-        c[0] = psi_seen(S, C, dir, i, j, 1, 0, r0);
-        c[1] = psi_seen(S, C, dir, i, j, 0, 1, r0);
-        c[2] = psi_seen(S, C, dir, i, j, -1, 0, r0);
-        c[3] = psi_seen(S, C, dir, i, j, 0, -1, r0);
-        d[0] = psi_seen(S, C, dir, i, j, 1, 1, r0);
-        d[1] = psi_seen(S, C, dir, i, j, -1, 1, r0);
-        d[2] = psi_seen(S, C, dir, i, j, -1, -1, r0);
-        d[3] = psi_seen(S, C, dir, i, j, 1, -1, r0);
+        c[0] = psi_seen(S, C, dir, i, j, 1, 0, r0, copies);
+        c[1] = psi_seen(S, C, dir, i, j, 0, 1, r0, copies);
+        c[2] = psi_seen(S, C, dir, i, j, -1, 0, r0, copies);
+        c[3] = psi_seen(S, C, dir, i, j, 0, -1, r0, copies);
+        d[0] = psi_seen(S, C, dir, i, j, 1, 1, r0, copies);
+        d[1] = psi_seen(S, C, dir, i, j, -1, 1, r0, copies);
+        d[2] = psi_seen(S, C, dir, i, j, -1, -1, r0, copies);
+        d[3] = psi_seen(S, C, dir, i, j, 1, -1, r0, copies);
     } else {
         const double *ps = S.fpsi + (size_t)dir * S.N;
         auto P = [&](int a, int b) { return ps[(size_t)wrapi(b, C.ny) * C.nx + wrapi(a, C.nx)]; };
@@ -356,29 +403,37 @@ __global__ void __launch_bounds__(256) k_pp_face_flux(DevState S, Consts C)
     }
     double gx, gy;
     grad9(C, c, d, gx, gy);
-    const size_t F2 = 2 * (size_t)S.N;
+    const size_t b = t - F2;
+    const double *sc = copy ? S.cbscr + b : S.fscr + t;
+    const size_t ss = copy ? nb : F2;
     FaceMacro<double> fm;
-    fm.rho = S.fscr[0 * F2 + t];
-    fm.U = S.fscr[1 * F2 + t];
-    fm.V = S.fscr[2 * F2 + t];
-    double scg = S.fscr[3 * F2 + t], psi = S.fpsi[t];
+    fm.rho = sc[0 * ss];
+    fm.U = sc[1 * ss];
+    fm.V = sc[2 * ss];
+    double scg = sc[3 * ss], psi = copy ? S.cbpsi[b] : S.fpsi[t];
     gx *= -scg * psi;
     gy *= -scg * psi;
     fm.Fx = gx;
     fm.Fy = gy;
-    S.ffx[t] = gx;
-    S.ffy[t] = gy;
+    if (copy) {
+        S.cbffx[b] = gx;
+        S.cbffy[b] = gy;
+    } else {
+        S.ffx[t] = gx;
+        S.ffy[t] = gy;
+    }
     int oi, oj, ni, nj;
     double xf, yf;
     face_geom(C, dir, i, j, oi, oj, ni, nj, xf, yf);
-    double *ffl = S.ffl + (dir ? (size_t)Q9 * S.N : 0);
+    double *ffl = copy ? S.cbfl + b : S.ffl + (dir ? (size_t)Q9 * S.N : 0) + fidx;
+    const size_t fs = copy ? nb : (size_t)S.N;
 #pragma unroll
     for (int q = 0; q < Q9; ++q) {
         double fb = face_value<FLUX, UNIT>(S, C, S.fbp, S.gfbp, S.fbpx, S.fbpy, dir, q, oi, oj, ni, nj, xf, yf);
         double hh, ff;
         face_bgk<double, MPD_MODEL_PP_D2Q9, UNIT>(C, fm, q, 0.0, fb, C.ah_pp, C.bh_pp, C.ch_pp, hh, ff);
         double xin = C.dx * (dir == 0 ? C.xi[0][q] : C.xi[1][q]);
-        ffl[(size_t)q * S.N + fidx] = ff * xin;
+        ffl[(size_t)q * fs] = ff * xin;
     }
 }
 
@@ -398,9 +453,10 @@ __global__ void __launch_bounds__(256) k_update(DevState S, Consts C)
     const double aTP = 4.0 / 3.0, bTP = 1.0 / 3.0;
     const double *hx = S.hfl, *hy = S.hfl + (size_t)Q9 * N;
     const double *fx = S.ffl, *fy = S.ffl + (size_t)Q9 * N;
-    // Allen-Cahn UW: the last column / row owns the second copy of its periodic
-    // east / north face (Face_C, Ark/Cpp/Mesh/MeshCartesian.cpp:28-60)
-    const bool copies = MODEL != MPD_MODEL_PP_D2Q9 && C.flux == MPD_FLUX_UW;
+    // UW (Allen-Cahn; pseudopotential in the reference face order): the last
+    // column / row owns the second copy of its periodic east / north face
+    // (Face_C, Ark/Cpp/Mesh/MeshCartesian.cpp:28-60)
+    const bool copies = C.flux == MPD_FLUX_UW && (MODEL != MPD_MODEL_PP_D2Q9 || (C.face_ref && S.cbpsi != nullptr));
     const bool cE = copies && i == C.nx - 1, cN = copies && !C.walls && j == C.ny - 1;
     const size_t nb = (size_t)C.nx + C.ny;
     double ht[Q9], ft[Q9];
@@ -701,8 +757,9 @@ static void phase_step_t(const DevState &S, const Consts &C, hipStream_t st)
                            (const double *)S.gfbp, S.fbpx, S.fbpy);
     }
     if (MODEL == MPD_MODEL_PP_D2Q9) {
-        hipLaunchKernelGGL((k_pp_face_psi<FLUX, UNIT>), dim3(nblk(2 * S.N)), dim3(256), 0, st, S, C);
-        hipLaunchKernelGGL((k_pp_face_flux<FLUX, UNIT>), dim3(nblk(2 * S.N)), dim3(256), 0, st, S, C);
+        const size_t nf = 2 * S.N + (FLUX == MPD_FLUX_UW ? (size_t)C.nx + C.ny : 0);
+        hipLaunchKernelGGL((k_pp_face_psi<FLUX, UNIT>), dim3(nblk(nf)), dim3(256), 0, st, S, C);
+        hipLaunchKernelGGL((k_pp_face_flux<FLUX, UNIT>), dim3(nblk(nf)), dim3(256), 0, st, S, C);
     } else {
         size_t nf = S.N + (size_t)C.nyf * C.nx + (FLUX == MPD_FLUX_UW ? uw_copies(C) : 0);
         hipLaunchKernelGGL((k_faces_ac<MODEL, FLUX, UNIT>), dim3(nblk(nf)), dim3(256), 0, st, S, C);
@@ -735,6 +792,25 @@ static void phase_step_m(const DevState &S, const Consts &C, hipStream_t st)
     case MPD_MODEL_PP_D2Q9 * 2 + MPD_FLUX_UW: phase_step_t<MPD_MODEL_PP_D2Q9, MPD_FLUX_UW, UNIT>(S, C, st); break;
     default: break;
     }
+}
+
+__global__ void __launch_bounds__(256) k_copies_from_first(DevState S, Consts C, int forces, int psi)
+{
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= C.nx + C.ny) return;
+    const size_t k = b < C.ny ? (size_t)b * C.nx : (size_t)S.N + (b - C.ny);
+    if (forces) {
+        S.cbffx[b] = S.ffx[k];
+        S.cbffy[b] = S.ffy[k];
+    }
+    if (psi) S.cbpsi_old[b] = S.fpsi_old[k];
+}
+
+void pp_copies_from_first(const DevState &S, const Consts &C, bool forces, bool psi, hipStream_t st)
+{
+    if (!S.cbpsi_old) return;
+    hipLaunchKernelGGL(k_copies_from_first, dim3(nblk((size_t)C.nx + C.ny)), dim3(256), 0, st, S, C, (int)forces,
+                       (int)psi);
 }
 
 void phase_step(const DevState &S, const Consts &C, hipStream_t st)

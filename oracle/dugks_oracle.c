@@ -22,9 +22,14 @@
  * Face convention: x-face (i,j) is the west face of cell (i,j) with normal +x,
  * owner (i-1,j), neighbour (i,j); y-face (i,j) is the south face of cell (i,j)
  * with normal +y, owner (i,j-1).  The reference stores each periodic boundary
- * face twice (zones 12 and 8) with opposite normals; both copies evaluate to
- * the same bits as the single structured face here because every operation
- * that differs is an exact sign flip (see face_recon()).
+ * face twice (zones 12 and 8) with opposite normals, each owned by its mesh
+ * cell.  With CD faces both copies evaluate to the same bits as the single
+ * structured face here, because every operation that differs is an exact sign
+ * flip (see face_recon()).  With UW faces the xi.n = 0 average adds the owner
+ * first, so the copies round apart: the structured face at index 0 is the
+ * zone-12 copy, and the zone-8 copy (make_face at i = nx / j = ny) is evaluated
+ * as well for the last column / row (cbhl, cbfl; cbpsi in the reference face
+ * order).
  *
  * The pseudopotential face force has two modes.  The reference writes each
  * face's Psi and reads its neighbour faces' Psi inside the same OpenMP loop
@@ -78,6 +83,9 @@ struct ora {
     double *cbfx, *cbfy;      /* force of the second copy (zone 8) of each periodic face:
                                  x-faces i = 0 by row [ny], then y-faces j = 0 by column [nx] */
     double *cbfl;             /* flux of those copies [Q][ny + nx] */
+    double *cbpsi, *cbpsi_old;/* Psi of those copies, this step and the previous one [ny + nx]
+                                 (with UW faces their reconstruction, and so Psi, can
+                                 differ from the first copy's in the last bits) */
     double *cbhl;             /* h flux of those copies (Allen-Cahn UW) [Q][ny + nx] */
     int one_copy;             /* test knob (ora_set_one_copy): UW without the second copies */
 };
@@ -316,6 +324,8 @@ ora *ora_create(const mpd_config *cfg)
     o->cbfy = calloc((size_t)o->nx + o->ny, sizeof(double));
     o->cbfl = calloc((size_t)o->Q * (o->nx + o->ny), sizeof(double));
     o->cbhl = calloc((size_t)o->Q * (o->nx + o->ny), sizeof(double));
+    o->cbpsi = calloc((size_t)o->nx + o->ny, sizeof(double));
+    o->cbpsi_old = calloc((size_t)o->nx + o->ny, sizeof(double));
     return o;
 }
 
@@ -324,7 +334,7 @@ void ora_destroy(ora *o)
     if (!o) return;
     double *bufs[] = {o->h, o->f, o->m, o->ffx, o->ffy, o->fpsi, o->hbp, o->fbp, o->hbpx,
                       o->hbpy, o->fbpx, o->fbpy, o->hfl, o->ffl, o->gm, o->ghbp, o->gfbp,
-                      o->fpsi_old, o->cbfx, o->cbfy, o->cbfl, o->cbhl};
+                      o->fpsi_old, o->cbfx, o->cbfy, o->cbfl, o->cbhl, o->cbpsi, o->cbpsi_old};
     for (size_t k = 0; k < sizeof(bufs) / sizeof(bufs[0]); ++k) free(bufs[k]);
     free(o);
 }
@@ -352,7 +362,7 @@ int ora_set_state(ora *o, const double *h, const double *f, const double *m,
 
 int ora_set_face_order(ora *o, int ref_order)
 {
-    if (ref_order && !(o->c.model == MPD_MODEL_PP_D2Q9 && o->c.flux == MPD_FLUX_CD && !o->walls))
+    if (ref_order && !(o->c.model == MPD_MODEL_PP_D2Q9 && !o->walls))
         return -1;
     o->ref_order = ref_order != 0;
     return 0;
@@ -368,7 +378,12 @@ int ora_set_one_copy(ora *o, int one)
 
 int ora_set_face_extra(ora *o, const double *psi_old, const double *copy_fx, const double *copy_fy)
 {
-    if (psi_old) memcpy(o->fpsi_old, psi_old, 2 * (size_t)o->N * sizeof(double));
+    if (psi_old) {
+        memcpy(o->fpsi_old, psi_old, 2 * (size_t)o->N * sizeof(double));
+        /* the second copies take the first copies' previous Psi (the same bits with CD faces) */
+        for (int j = 0; j < o->ny; ++j) o->cbpsi_old[j] = psi_old[(size_t)j * o->nx];
+        for (int i = 0; i < o->nx; ++i) o->cbpsi_old[o->ny + i] = psi_old[o->N + i];
+    }
     if (copy_fx) memcpy(o->cbfx, copy_fx, ((size_t)o->nx + o->ny) * sizeof(double));
     if (copy_fy) memcpy(o->cbfy, copy_fy, ((size_t)o->nx + o->ny) * sizeof(double));
     return 0;
@@ -737,8 +752,13 @@ static void face_obj_step(const ora *o, int dir, int *i, int *j, int di, int dj)
 static double psi_seen(const ora *o, int dir, int i, int j, int di, int dj, long r0)
 {
     face_obj_step(o, dir, &i, &j, di, dj);
+    const int seen = face_rank(o, dir, i, j) < r0;
+    if ((dir == 0 && i == o->nx) || (dir == 1 && j == o->ny)) {  /* a second copy */
+        size_t b = dir == 0 ? (size_t)j : (size_t)o->ny + i;
+        return seen ? o->cbpsi[b] : o->cbpsi_old[b];
+    }
     size_t fi = (size_t)dir * o->N + (size_t)wrap(j, o->ny) * o->nx + wrap(i, o->nx);
-    return face_rank(o, dir, i, j) < r0 ? o->fpsi[fi] : o->fpsi_old[fi];
+    return seen ? o->fpsi[fi] : o->fpsi_old[fi];
 }
 
 /* Update_PseudoForce(Face_2D&) (Ark/Cpp/PseudoP.cpp:39-46) of object (dir, i, j)
@@ -761,9 +781,10 @@ static void pp_force_ref(const ora *o, int dir, int i, int j, double scg, double
 
 /* Forces of every face object in reference order.  fms holds the macros of each
  * structured face with the force of its first copy, fmb those of the second
- * copies of the periodic faces (index as cbfx); only U, V differ between the
- * copies (they carry their own previous force, D2Q9PP.cpp:119-122), rho, scG
- * and Psi are the same bits. */
+ * copies of the periodic faces (index as cbfx).  With CD faces only U, V differ
+ * between the copies (they carry their own previous force, D2Q9PP.cpp:119-122)
+ * and rho, scG and Psi are the same bits; with UW faces the copies'
+ * reconstructions, and so all their macros, can differ in the last bits. */
 static void pp_forces_ref(ora *o, face_macro *fms, face_macro *fmb)
 {
     int nx = o->nx, ny = o->ny;
@@ -870,8 +891,16 @@ static void phase4_faces(ora *o)
                     face_macros(o, &F, hb, fb, o->ffx[fi], o->ffy[fi], fm);
                     o->fpsi[fi] = fm->psi;
                     if (ref && ((dir == 0 && i == 0) || (dir == 1 && j == 0))) {
+                        /* the second copy: its own reconstruction (with UW faces
+                         * the xi.n = 0 average adds its own owner first), macros and Psi */
                         size_t b = dir == 0 ? (size_t)j : (size_t)ny + i;
-                        face_macros(o, &F, hb, fb, o->cbfx[b], o->cbfy[b], &fmb[b]);
+                        face_t FB;
+                        double fbb[QMAX];
+                        make_face(o, dir, dir == 0 ? nx : i, dir == 0 ? j : ny, &FB);
+                        for (int q = 0; q < Q; ++q)
+                            fbb[q] = face_recon(o, o->fbp, o->gfbp, o->fbpx, o->fbpy, &FB, q);
+                        face_macros(o, &FB, hb, fbb, o->cbfx[b], o->cbfy[b], &fmb[b]);
+                        o->cbpsi[b] = fmb[b].psi;
                     }
                 }
     }
@@ -954,8 +983,11 @@ static void phase4_faces(ora *o)
                  * macros and force, so its own flux (taken by the cell that owns it) */
                 if (ref && ((dir == 0 && i == 0) || (dir == 1 && j == 0))) {
                     size_t b = dir == 0 ? (size_t)j : (size_t)ny + i;
+                    face_t FB;
+                    make_face(o, dir, dir == 0 ? nx : i, dir == 0 ? j : ny, &FB);
                     for (int q = 0; q < Q; ++q) {
-                        face_bgk(o, &fmb[b], q, 0, fb[q], ah, bh, ch, &hh[q], &ff[q]);
+                        double fbb = face_recon(o, o->fbp, o->gfbp, o->fbpx, o->fbpy, &FB, q);
+                        face_bgk(o, &fmb[b], q, 0, fbb, ah, bh, ch, &hh[q], &ff[q]);
                         double xin = o->c.dx * (dir == 0 ? o->xi[0][q] : o->xi[1][q]);
                         o->cbfl[(size_t)q * (nx + ny) + b] = ff[q] * xin;
                     }
@@ -987,6 +1019,7 @@ static void phase4_faces(ora *o)
         }
     }
     if (pp) memcpy(o->fpsi_old, o->fpsi, 2 * (size_t)N * sizeof(double));
+    if (ref) memcpy(o->cbpsi_old, o->cbpsi, ((size_t)nx + ny) * sizeof(double));
     free(fms);
     free(fmb);
 }

@@ -96,7 +96,7 @@ class Oracle:
         """Pseudopotential D2Q9: True = visit faces in the reference's
         one-thread order (Ark/Cpp/DUGKS2DSolver.cpp:219-230), False = Jacobi."""
         if lib().ora_set_face_order(self._h, 1 if reference else 0) != 0:
-            raise ValueError("reference face order: pseudopotential D2Q9, CD, periodic only")
+            raise ValueError("reference face order: pseudopotential D2Q9, periodic only")
 
     def set_one_copy(self, one):
         """Test knob: Allen-Cahn UW with one face per location (without the

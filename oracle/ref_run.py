@@ -48,6 +48,9 @@ CONFIGS = {
     # SCMP_Drop shifted by (60, 70) cells: the drop crosses both periodic lines
     "pp_shift": ("D2Q9PP", "-include " + os.path.join(CFGDIR, "pp", "flags.h"), False,
                  os.path.join(CFGDIR, "pp_shift")),
+    # pseudopotential with UW faces (SCMP_Drop, the -I dir of "pp" selects it)
+    "pp_uw": ("D2Q9PP", "-include " + os.path.join(CFGDIR, "pp_uw", "flags.h"), False,
+              os.path.join(CFGDIR, "pp")),
     # SCMP_FlatInterface: a liquid band across the periodic x line
     "pp_flat": ("D2Q9PP", "-include " + os.path.join(CFGDIR, "pp", "flags.h"), False,
                 os.path.join(CFGDIR, "pp_flat")),

@@ -44,7 +44,9 @@ CONFIGS = {"hcz": ("ac_hcz", STEPS), "acp": ("ac_p", STEPS), "hcz_wall": ("ac_hc
            # the same drop shifted cyclically by (60, 70) cells: it crosses both periodic lines
            "pp_shift": ("pp_d2q9", STEPS),
            # SCMP_FlatInterface: a liquid band across the periodic x line
-           "pp_flat": ("pp_d2q9", STEPS)}
+           "pp_flat": ("pp_d2q9", STEPS),
+           # SCMP_Drop with UW faces
+           "pp_uw": ("pp_d2q9", STEPS)}
 SHIFT = {"pp_shift": [60, 70]}  # (x, y) cells, oracle/refcfg/pp_shift/ZeroCondition.h
 
 

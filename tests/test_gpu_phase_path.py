@@ -90,7 +90,7 @@ def test_nonfinite_state_is_reported():
     s.close()
 
 
-@pytest.mark.parametrize("name", ["pp", "pp_shift", "pp_flat"])
+@pytest.mark.parametrize("name", ["pp", "pp_shift", "pp_flat", "pp_uw"])
 def test_pp_reference_face_order_bit_exact_vs_reference(name):
     """The FP64 pseudopotential step in the reference's one-thread face order
     reproduces the compiled reference's own state digests (tests/golden,
@@ -110,11 +110,14 @@ def test_pp_reference_face_order_bit_exact_vs_reference(name):
     s.close()
 
 
-def test_pp_reference_face_order_vs_oracle_nonsquare(oracle_mod):
+@pytest.mark.parametrize("flux", [abi.MPD_FLUX_CD, abi.MPD_FLUX_UW], ids=["cd", "uw"])
+def test_pp_reference_face_order_vs_oracle_nonsquare(flux, oracle_mod):
     """Reference face order on a 48x40 mesh with the drop across a corner: the
     face ranks depend on nx and ny separately; GPU state, face forces and face
-    Psi equal the oracle's reference-order mode bit for bit."""
-    cfg, st = make_case("pp_d2q9", nx=48, ny=40, radius=12.0)
+    Psi equal the oracle's reference-order mode bit for bit (with UW faces the
+    second copies of the periodic faces, which the drop crosses, have their
+    own Psi and force)."""
+    cfg, st = make_case("pp_d2q9", nx=48, ny=40, radius=12.0, flux=flux)
     sx, sy = 20, 17
     roll = lambda a: np.roll(a.reshape(a.shape[0], 40, 48), (sy, sx), axis=(1, 2)).reshape(a.shape)
     st = solver.State(cfg, None, roll(st.f), roll(st.m), st.ffx, st.ffy)
