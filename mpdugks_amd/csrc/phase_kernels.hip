@@ -301,7 +301,7 @@ __global__ void __launch_bounds__(256) k_pp_face_psi(DevState S, Consts C)
     for (int q = 0; q < Q9; ++q) sy += fb[q] * xi_of<UNIT, double>(C, 1, q);
     double rho = s;
     double U = sx / rho, V = sy / rho;
-    const size_t b = t - F2;
+    const size_t b = copy ? t - F2 : 0;  // index of a second copy
     U += 0.5 * C.hdt * (copy ? S.cbffx[b] : S.ffx[t]) / rho;
     V += 0.5 * C.hdt * (copy ? S.cbffy[b] : S.ffy[t]) / rho;
     double p = C.pp_k * carnahan_starling(rho, C.t_cs);
@@ -403,7 +403,7 @@ __global__ void __launch_bounds__(256) k_pp_face_flux(DevState S, Consts C)
     }
     double gx, gy;
     grad9(C, c, d, gx, gy);
-    const size_t b = t - F2;
+    const size_t b = copy ? t - F2 : 0;  // index of a second copy
     const double *sc = copy ? S.cbscr + b : S.fscr + t;
     const size_t ss = copy ? nb : F2;
     FaceMacro<double> fm;
