@@ -12,7 +12,11 @@ completion time minus arrival time (wall clock).
 With FP32-bulk populations the residual of a coalesced drop settles on a
 rounding floor of about 1e-6..3e-5 (the kinetic energy of the spurious
 currents, 1e-6 of the state's scale, carries FP32 rounding); the FP64 path
-goes below 1e-8.  A tolerance under that floor is reached only by chance.
+goes below 1e-8.  A tolerance under that floor is reached only by chance, so
+with `switch_tol` set a request that reaches it in the FP32 batch continues as
+an FP64 handle from its downloaded state (stepped on its own stream beside the
+batch) until `ke_tol`: the batch does the long relaxation, FP64 the last
+decades.
 
 The reference runs one case per process with a density residual stop
 (Ark/Cpp/DUGKS2DSolver.cpp:807-820); this loop and its KE residual are new.
@@ -25,7 +29,7 @@ import numpy as np
 
 class Request:
     __slots__ = ("rid", "arrival", "make", "bytes_estimate", "admitted", "done", "steps", "converged",
-                 "ke_prev", "handle", "cells", "final", "params")
+                 "ke_prev", "handle", "cells", "final", "params", "fp64_from")
 
     def __init__(self, rid, arrival, make, bytes_estimate, cells, params=None):
         self.rid = rid
@@ -41,6 +45,7 @@ class Request:
         self.handle = None
         self.final = None             # state at retirement (GpuEngine(keep_final=True))
         self.params = params or {}
+        self.fp64_from = None         # step at which the case continued in FP64
 
 
 class GpuEngine:
@@ -51,6 +56,7 @@ class GpuEngine:
         self._solver = solver
         self.batch = solver.Batch()
         self.keep_final = keep_final
+        self.fp64 = []                # FP64 handles, each stepped on its own stream
 
     def admit(self, req):
         from . import abi
@@ -67,7 +73,25 @@ class GpuEngine:
         return s.device_bytes()
 
     def step(self, n):
-        self.batch.step(n)
+        # all queued before any wait: the batch and the FP64 handles run side by side
+        if self.batch.size()[0]:
+            self.batch.step(n)
+        for h in self.fp64:
+            h.step(n)
+
+    def refine(self, req):
+        """Continue `req` as an FP64 handle from its current state."""
+        from . import abi
+        st = req.handle.download()
+        cfg = type(st.cfg).from_buffer_copy(st.cfg)
+        cfg.precision = abi.MPD_FP64
+        self.batch.remove(req.handle)
+        req.handle.close()
+        s = self._solver.Solver(cfg, path=abi.MPD_PATH_FUSED)
+        s.upload(self._solver.State(cfg, st.h, st.f, st.m))
+        self.fp64.append(s)
+        req.handle = s
+        return s.device_bytes()
 
     def kinetic_energy(self, req):
         return req.handle.reductions()["kinetic_energy"]
@@ -75,7 +99,10 @@ class GpuEngine:
     def retire(self, req):
         if self.keep_final:
             req.final = req.handle.download()
-        self.batch.remove(req.handle)
+        if req.fp64_from is not None:
+            self.fp64.remove(req.handle)
+        else:
+            self.batch.remove(req.handle)
         req.handle.close()
         req.handle = None
 
@@ -85,8 +112,9 @@ class GpuEngine:
 
 class Server:
     def __init__(self, engine, hbm_budget, check_every=1000, ke_tol=1e-8, max_steps=10 ** 6,
-                 clock=time.perf_counter, sleep=time.sleep):
+                 clock=time.perf_counter, sleep=time.sleep, switch_tol=None):
         self.engine = engine
+        self.switch_tol = switch_tol  # FP32 residual at which a case continues in FP64 (None: never)
         self.budget = hbm_budget
         self.check_every = check_every
         self.ke_tol = ke_tol
@@ -139,8 +167,15 @@ class Server:
                     self.engine.retire(r)
                     self.used -= r.bytes_estimate
                     finished.append(r)
-                else:
-                    still.append(r)
+                    continue
+                if self.switch_tol is not None and r.fp64_from is None and res < self.switch_tol:
+                    # past what FP32 resolves: the rest in FP64, residual history restarted
+                    self.used -= r.bytes_estimate
+                    r.bytes_estimate = self.engine.refine(r)
+                    self.used += r.bytes_estimate
+                    r.fp64_from = r.steps
+                    r.ke_prev = None
+                still.append(r)
             active = still
         wall = self.clock() - t0
         lat = np.array([r.done - r.arrival for r in finished]) if finished else np.array([0.0])
@@ -157,6 +192,9 @@ class Server:
             "max_active": max_active,
             "queued_for_budget": len(queued),
             "mean_steps": float(np.mean([r.steps for r in finished])) if finished else 0.0,
+            "continued_fp64": int(sum(r.fp64_from is not None for r in finished)),
+            "mean_fp64_steps": float(np.mean([r.steps - r.fp64_from for r in finished if r.fp64_from is not None]))
+            if any(r.fp64_from is not None for r in finished) else 0.0,
         }
 
 

@@ -76,3 +76,39 @@ def test_step_cap_retires_unconverged_cases():
     stats = Server(eng, hbm_budget=10 ** 6, check_every=1000, max_steps=5000,
                    clock=clk, sleep=clk.sleep).run(make_requests(4))
     assert stats["completed"] == 4 and stats["converged"] == 0
+
+
+class FloorEngine(FakeEngine):
+    """KE relaxes as 1 + exp(-steps/2000); in FP32 it carries a relative
+    rounding jitter of 5e-6 (alternating sign from check to check), so its
+    residual settles near 1e-5; a case continued in FP64 has none."""
+
+    def __init__(self, clock):
+        super().__init__(clock)
+        self.fp64 = set()
+
+    def kinetic_energy(self, r):
+        ke = 1.0 + np.exp(-r.steps / 2000.0)
+        if r.rid in self.fp64:
+            return ke
+        return ke * (1.0 + 5e-6 * (1 if (r.steps // 1000) % 2 else -1))
+
+    def refine(self, r):
+        self.fp64.add(r.rid)
+        return r.bytes_estimate
+
+
+def test_fp64_continuation_reaches_the_tolerance_fp32_cannot():
+    """Below the FP32 residual floor only the FP64 continuation converges: with
+    switch_tol every case continues in FP64 and reaches 1e-8; without it none
+    does before the step cap."""
+    runs = {}
+    for switch in (None, 1e-4):
+        clk = FakeClock()
+        eng = FloorEngine(clk)
+        srv = Server(eng, hbm_budget=10 ** 6, check_every=1000, ke_tol=1e-8, max_steps=60000,
+                     clock=clk, sleep=clk.sleep, switch_tol=switch)
+        runs[switch] = srv.run(make_requests(5))
+    assert runs[None]["converged"] == 0 and runs[None]["continued_fp64"] == 0
+    assert runs[1e-4]["converged"] == 5 and runs[1e-4]["continued_fp64"] == 5
+    assert 0 < runs[1e-4]["mean_fp64_steps"] < runs[1e-4]["mean_steps"]
