@@ -48,8 +48,8 @@ typedef enum mpd_model {
 } mpd_model;
 
 /* Face reconstruction: Ark/Constant/ZeroFlip.h:27-35.  CD with every model;
- * UW with the Allen-Cahn models (periodic or walled y, both precisions) and
- * the pseudopotential D2Q9 model (FP64), not with D2V16. */
+ * UW with the Allen-Cahn models (periodic or walled y) and the pseudopotential
+ * D2Q9 model (periodic), in both precisions; not with D2V16. */
 typedef enum mpd_flux { MPD_FLUX_CD = 0, MPD_FLUX_UW = 1 } mpd_flux;
 
 /* Boundary in y (x is always periodic): Ark/Constant/ZeroFlip.h:59-77. */

@@ -260,7 +260,7 @@ int check_cfg(const mpd_config *c)
     if (c->precision == MPD_FP32_BULK && !fused_supported(*c))
         return fail(MPD_ERR_INVALID, "no FP32-bulk kernel for this model / flux / boundary (FP32-bulk: Allen-Cahn "
                                      "with CD or UW faces and periodic, bounce-back or NEE y; pseudopotential D2Q9 "
-                                     "with CD faces, periodic)");
+                                     "with CD or UW faces, periodic)");
     if (c->precision == MPD_FP32_BULK && !is_pp(*c)) {
         // the FP32 step kernel addresses each state set through one 32-bit buffer
         // resource (all float planes) and the FP64 phase field through another
@@ -401,6 +401,8 @@ int mpd_set_path(mpd_handle *h, int32_t path)
         return fail(MPD_ERR_INVALID, "no fused kernel for this configuration");
     if (h->cfg.precision == MPD_FP32_BULK && path == MPD_PATH_PHASES && !is_pp(h->cfg))
         return fail(MPD_ERR_INVALID, "the phase path is FP64 only (and FP32 for the pseudopotential model)");
+    if (h->cfg.precision == MPD_FP32_BULK && path == MPD_PATH_PHASES && h->cfg.flux == MPD_FLUX_UW)
+        return fail(MPD_ERR_INVALID, "the FP32 pseudopotential step with UW faces has one launch per step");
     h->path = path;
     return 0;
 }
@@ -462,8 +464,8 @@ int mpd_get_tile(mpd_handle *h, int32_t *tile)
 {
     if (!h || !tile) return fail(MPD_ERR_INVALID, "null argument");
     if (h->tile) *tile = h->tile;
-    else if (is_pp(h->cfg)) *tile = pp32_tile_auto(h->cfg.nx, h->cfg.ny, h->cus);
     else if (h->cfg.flux == MPD_FLUX_UW) *tile = MPD_TILE_16X15;
+    else if (is_pp(h->cfg)) *tile = pp32_tile_auto(h->cfg.nx, h->cfg.ny, h->cus);
     else if (h->cfg.precision == MPD_FP64) *tile = fused64_tile_auto(h->cfg.nx, h->cfg.ny, h->cus);
     else *tile = fused32_tile_auto(h->cfg.nx, h->cfg.ny, h->cus, fused32_state_bytes(h->cfg));
     return 0;

@@ -684,9 +684,8 @@ int fused64_tile_auto(int nx, int ny, int cus)
 
 bool fused_supported(const mpd_config &cfg)
 {
-    // FP32-bulk pseudopotential D2Q9: the FP32 phase kernels of pp32.hip
-    if (cfg.model == MPD_MODEL_PP_D2Q9)
-        return cfg.precision == MPD_FP32_BULK && cfg.flux == MPD_FLUX_CD && cfg.bc_y == MPD_BC_PERIODIC;
+    // FP32-bulk pseudopotential D2Q9: the FP32 kernels of pp32.hip (CD and UW faces)
+    if (cfg.model == MPD_MODEL_PP_D2Q9) return cfg.precision == MPD_FP32_BULK && cfg.bc_y == MPD_BC_PERIODIC;
     const bool ac = (cfg.model == MPD_MODEL_AC_HCZ || cfg.model == MPD_MODEL_AC_P) && cfg.nx >= 4 && cfg.ny >= 4;
     // UW faces: the FP32-bulk tile kernel (the FP64 fused kernel is CD only)
     if (cfg.flux == MPD_FLUX_UW) return ac && cfg.precision == MPD_FP32_BULK;

@@ -1,4 +1,5 @@
-// FP32-bulk pseudopotential D2Q9 step (periodic mesh, CD faces).
+// FP32-bulk pseudopotential D2Q9 step (periodic mesh; CD faces, and UW faces on
+// the one-launch tile kernel).
 //
 // The phases of the reference step for the D2Q9PP model, one kernel each as in
 // the FP64 phase path (phase_kernels.hip: k_phase1, k_pp_face_psi,
@@ -50,6 +51,9 @@ struct KPP {
     float ah, bh, ch;  // face BGK factors
     float dtv;         // dt / V
     float flux;        // dx MaSpan: micro-flux per unit e_n
+    // UW face value (x_f - h xi - x_c) . grad b / dx with the gradient as 6-point
+    // sums (x 12 dx): normal (1/2 - A)/12 upwind, tangential A/12, 1/24 each side for e_n = 0
+    float uw_n, uw_t, uw_0;
 };
 
 KPP make_kpp(const Consts &C)
@@ -76,6 +80,9 @@ KPP make_kpp(const Consts &C)
     k.ch = (float)C.ch_pp;
     k.dtv = (float)C.dtv;
     k.flux = (float)(C.dx * ms);
+    k.uw_n = (float)((0.5 - A) / 12.0);
+    k.uw_t = (float)(A / 12.0);
+    k.uw_0 = (float)(1.0 / 24.0);
     return k;
 }
 
@@ -438,11 +445,14 @@ __global__ void __launch_bounds__(256) k_pp32_to64(PP32Planes P, DevState S, Con
 // kernels, from the same inputs.
 // Tile shapes: 16x15 cells on 256 threads, 32x16 on 512 (per cell about 10%
 // less halo work, and twice the LDS: 81.3 KB, two workgroups per CU).
-template <int FTX_, int FTY_>
+// H: halo of the face values' cells beyond the faces' own (1 CD; 2 UW, whose
+// 6-point gradients reach one cell further)
+template <int FTX_, int FTY_, int H_ = 1>
 struct PPTile {
-    static constexpr int FTX = FTX_, FTY = FTY_, FNT = FTX * FTY > 256 ? 512 : 256;
-    static constexpr int QX = FTX + 6, QY = FTY + 6, QN = QX * QY;           // Psi region: -3 .. +2
-    static constexpr int RX = FTX + 4, RY = FTY + 4, RN = RX * RY;           // bar-plus region: -2 .. +1
+    static constexpr int FTX = FTX_, FTY = FTY_, H = H_, FNT = FTX * FTY > 256 ? 512 : 256;
+    static constexpr int PO = 2 + H, RO = 1 + H;                             // region offsets
+    static constexpr int QX = FTX + 2 * PO, QY = FTY + 2 * PO, QN = QX * QY; // Psi region: -PO .. +PO-1
+    static constexpr int RX = FTX + 2 * RO, RY = FTY + 2 * RO, RN = RX * RY; // bar-plus region: -RO .. +RO-1
     static constexpr int XPX = FTX + 3, XPY = FTY + 2, XPN = XPX * XPY;      // x-face Psi region: fx -1..TX+1, fy -1..TY
     static constexpr int YPX = FTX + 2, YPY = FTY + 3, YPN = YPX * YPY;      // y-face Psi region: fx -1..TX, fy -1..TY+1
     static constexpr int XFX = FTX + 1, XFN = XFX * FTY;                     // flux x-faces: fx 0..TX, fy 0..TY-1
@@ -473,8 +483,8 @@ __device__ __forceinline__ int fused_tile_of_block(int b, int ntiles)
 template <class G>
 __device__ __forceinline__ float fused_face_value(const KPP &k, const float *bp, int dir, int q, int fx, int fy)
 {
-    constexpr int RX = G::RX;
-    auto B = [&](int cx, int cy) { return bp[(cy + 2) * RX + (cx + 2)]; };
+    constexpr int RX = G::RX, RO = G::RO;
+    auto B = [&](int cx, int cy) { return bp[(cy + RO) * RX + (cx + RO)]; };
     if (dir == 0) {
         const int ox = fx - 1;
         return pp_recon<0>(k, q, B(ox, fy), B(fx, fy), B(fx, fy + 1), B(fx, fy - 1), B(ox, fy + 1), B(ox, fy - 1));
@@ -483,12 +493,40 @@ __device__ __forceinline__ float fused_face_value(const KPP &k, const float *bp,
     return pp_recon<1>(k, q, B(fx, oy), B(fx, fy), B(fx + 1, fy), B(fx - 1, fy), B(fx + 1, oy), B(fx - 1, oy));
 }
 
-template <bool FORCE, int TX, int TY>
+// UW face value of population q (UW_Interior_phi_Bh + Update_phi_fBh,
+// DUGKS2DSolver.cpp:560-603,688-706; gradients Grad_VS_6points,
+// GradSchemeBasic.h:95-126) from the bar-plus region: the upwind cell's value
+// plus its 6-point gradient along x_f - h xi - x_c; for e_n = 0 the mean of
+// both cells' (owner west / south, neighbour east / north)
+template <class G>
+__device__ __forceinline__ float fused_face_value_uw(const KPP &k, const float *bp, int dir, int q, int fx, int fy)
+{
+    constexpr int RX = G::RX, RO = G::RO;
+    auto B = [&](int cx, int cy) { return bp[(cy + RO) * RX + (cx + RO)]; };
+    const int en = dir == 0 ? ex9(q) : ey9(q), et = dir == 0 ? ey9(q) : ex9(q);
+    const float ct = et == 0 ? 0.f : (et > 0 ? -k.uw_t : k.uw_t);
+    auto side = [&](int cx, int cy, float cn) {
+        const float e = B(cx + 1, cy), w = B(cx - 1, cy), n = B(cx, cy + 1), s = B(cx, cy - 1);
+        const float ne = B(cx + 1, cy + 1), nw = B(cx - 1, cy + 1), sw = B(cx - 1, cy - 1), se = B(cx + 1, cy - 1);
+        const float gx = 4.f * (e - w) + ((ne - nw) + (se - sw));
+        const float gy = 4.f * (n - s) + ((ne - se) + (nw - sw));
+        float v = fmaf(cn, dir == 0 ? gx : gy, B(cx, cy));
+        if (et != 0) v = fmaf(ct, dir == 0 ? gy : gx, v);
+        return v;
+    };
+    const int ox = dir == 0 ? fx - 1 : fx, oy = dir == 0 ? fy : fy - 1;
+    if (en > 0) return side(ox, oy, k.uw_n);
+    if (en < 0) return side(fx, fy, -k.uw_n);
+    return 0.5f * (side(ox, oy, k.uw_0) + side(fx, fy, -k.uw_0));
+}
+
+template <bool FORCE, int TX, int TY, int FLUX = MPD_FLUX_CD>
 __global__ void __launch_bounds__(TX * TY > 256 ? 512 : 256) k_pp32_tile(PP32Planes in, PP32Set out, Consts C, KPP k,
                                                                         int ntx, int ntiles)
 {
-    using G = PPTile<TX, TY>;
+    using G = PPTile<TX, TY, FLUX == MPD_FLUX_UW ? 2 : 1>;
     constexpr int FTX = G::FTX, FTY = G::FTY, FNT = G::FNT, QX = G::QX, QN = G::QN, RX = G::RX, RN = G::RN;
+    constexpr int PO = G::PO, RO = G::RO;
     constexpr int XPX = G::XPX, XPN = G::XPN, YPX = G::YPX, YPN = G::YPN, XFX = G::XFX, XFN = G::XFN, FN = G::FN;
     static_assert(FNT == (TX * TY > 256 ? 512 : 256), "launch bounds");
     __shared__ float s_bp[Q9 * RN];
@@ -507,7 +545,7 @@ __global__ void __launch_bounds__(TX * TY > 256 ? 512 : 256) k_pp32_tile(PP32Pla
 
     // S0: Psi, scG
     for (int q = tid; q < QN; q += FNT) {
-        const int cx = q % QX - 3, cy = q / QX - 3;
+        const int cx = q % QX - PO, cy = q / QX - PO;
         const size_t n = (size_t)wrapn(j0 + cy, ny) * nx + wrapn(i0 + cx, nx);
         s_psi[q] = in.psi[n];
         s_scg[q] = in.scg[n];
@@ -516,16 +554,16 @@ __global__ void __launch_bounds__(TX * TY > 256 ? 512 : 256) k_pp32_tile(PP32Pla
 
     // S1: cell force and bar-plus
     for (int r = tid; r < RN; r += FNT) {
-        const int cx = r % RX - 2, cy = r / RX - 2;
+        const int cx = r % RX - RO, cy = r / RX - RO;
         const size_t n = (size_t)wrapn(j0 + cy, ny) * nx + wrapn(i0 + cx, nx);
         float Fx, Fy;
         if (FORCE) {
-            auto S = [&](int a, int b) { return s_psi[(b + 3) * QX + (a + 3)]; };
+            auto S = [&](int a, int b) { return s_psi[(b + PO) * QX + (a + PO)]; };
             const float c[4] = {S(cx + 1, cy), S(cx, cy + 1), S(cx - 1, cy), S(cx, cy - 1)};
             const float d[4] = {S(cx + 1, cy + 1), S(cx - 1, cy + 1), S(cx - 1, cy - 1), S(cx + 1, cy - 1)};
             float gx, gy;
             pp_grad9(k, c, d, gx, gy);
-            const int qi = (cy + 3) * QX + (cx + 3);
+            const int qi = (cy + PO) * QX + (cx + PO);
             const float sp = -s_scg[qi] * s_psi[qi];
             Fx = gx * sp;
             Fy = gy * sp;
@@ -555,7 +593,10 @@ __global__ void __launch_bounds__(TX * TY > 256 ? 512 : 256) k_pp32_tile(PP32Pla
         const size_t gf = (dir ? N : 0) + (size_t)wrapn(j0 + fy, ny) * nx + wrapn(i0 + fx, nx);
         float fb[Q9];
 #pragma unroll
-        for (int q = 0; q < Q9; ++q) fb[q] = fused_face_value<G>(k, s_bp + q * RN, dir, q, fx, fy);
+        for (int q = 0; q < Q9; ++q) {
+            if constexpr (FLUX == MPD_FLUX_UW) fb[q] = fused_face_value_uw<G>(k, s_bp + q * RN, dir, q, fx, fy);
+            else fb[q] = fused_face_value<G>(k, s_bp + q * RN, dir, q, fx, fy);
+        }
         // a face of the tile keeps its values for the flux stage
         int kf = -1;
         if (dir == 0 && fx >= 0 && fx <= FTX && fy >= 0 && fy < FTY) kf = fy * XFX + fx;
@@ -621,7 +662,7 @@ __global__ void __launch_bounds__(TX * TY > 256 ? 512 : 256) k_pp32_tile(PP32Pla
     const size_t n = (size_t)(j0 + cy) * nx + (i0 + cx);
     const int kW = cy * XFX + cx, kE = kW + 1;
     const int kS = XFN + cy * FTX + cx, kN = kS + FTX;
-    const int rc = (cy + 2) * RX + (cx + 2);
+    const int rc = (cy + RO) * RX + (cx + RO);
     const float aTP = 4.f / 3.f, bTP = 1.f / 3.f, dtv = k.dtv;
     float ft[Q9];
     double rest = 0.0;
@@ -738,14 +779,16 @@ int pp32_tile_auto(int nx, int ny, int cus)
     return (int64_t)((nx + 31) / 32) * ((ny + 15) / 16) >= (cus > 0 ? cus : 256) ? MPD_TILE_32X16 : MPD_TILE_16X15;
 }
 
-template <int TX, int TY>
+template <int TX, int TY, int FLUX = MPD_FLUX_CD>
 void pp32_tile_launch(PP32Planes &P, PP32State &St, const Consts &C, const KPP &k, bool force_stale, hipStream_t st)
 {
     const int ntx = (P.nx + TX - 1) / TX, nty = (P.ny + TY - 1) / TY, ntiles = ntx * nty;
     const unsigned nblocks = (unsigned)(((ntiles + 7) / 8) * 8);
     dim3 b(PPTile<TX, TY>::FNT);
-    if (force_stale) hipLaunchKernelGGL((k_pp32_tile<true, TX, TY>), dim3(nblocks), b, 0, st, P, St.other, C, k, ntx, ntiles);
-    else hipLaunchKernelGGL((k_pp32_tile<false, TX, TY>), dim3(nblocks), b, 0, st, P, St.other, C, k, ntx, ntiles);
+    if (force_stale)
+        hipLaunchKernelGGL((k_pp32_tile<true, TX, TY, FLUX>), dim3(nblocks), b, 0, st, P, St.other, C, k, ntx, ntiles);
+    else
+        hipLaunchKernelGGL((k_pp32_tile<false, TX, TY, FLUX>), dim3(nblocks), b, 0, st, P, St.other, C, k, ntx, ntiles);
 }
 
 void pp32_step(PP32State &St, const Consts &C, bool force_stale, hipStream_t st)
@@ -762,7 +805,8 @@ void pp32_step(PP32State &St, const Consts &C, bool force_stale, hipStream_t st)
         return;
     }
     const int tile = St.tile ? St.tile : pp32_tile_auto(P.nx, P.ny, St.cus);
-    if (tile == MPD_TILE_32X16) pp32_tile_launch<32, 16>(P, St, C, k, force_stale, st);
+    if (C.flux == MPD_FLUX_UW) pp32_tile_launch<16, 15, MPD_FLUX_UW>(P, St, C, k, force_stale, st);  // 16x15 only
+    else if (tile == MPD_TILE_32X16) pp32_tile_launch<32, 16>(P, St, C, k, force_stale, st);
     else pp32_tile_launch<16, 15>(P, St, C, k, force_stale, st);
     // the written set becomes the current one
     PP32Set cur{P.rho, P.f, P.U, P.V, P.Fx, P.Fy, P.psi, P.scg, P.ffx, P.ffy};

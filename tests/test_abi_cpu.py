@@ -62,11 +62,11 @@ def test_interface_initialiser_profile():
 
 def test_fp32_bulk_without_a_kernel_is_refused_at_create():
     """FP32-bulk exists for the Allen-Cahn models (CD and UW faces; periodic,
-    bounce-back or NEE y) and the pseudopotential D2Q9 model (CD, periodic);
-    other combinations are refused by mpd_create before any device call."""
+    bounce-back or NEE y) and the pseudopotential D2Q9 model (CD and UW,
+    periodic); D2V16 has no FP32-bulk kernel, and mpd_create refuses it before
+    any device call."""
     L = solver.lib()
     bad = [
-        (abi.MPD_MODEL_PP_D2Q9, dict(flux=abi.MPD_FLUX_UW), abi.MPD_BC_PERIODIC),
         (abi.MPD_MODEL_PP_D2V16, dict(), abi.MPD_BC_PERIODIC),
     ]
     for model, over, bc in bad:
@@ -112,9 +112,9 @@ def test_fp32_mesh_beyond_buffer_range_refused():
 
 def test_fp32_uw_configurations():
     """FP32-bulk UW faces: the Allen-Cahn models with periodic, bounce-back and
-    NEE y pass the checks (on a machine without a GPU mpd_create then fails
-    with MPD_ERR_DEVICE); the pseudopotential model is refused with
-    MPD_ERR_INVALID before any device call."""
+    NEE y and the periodic pseudopotential D2Q9 model pass the checks (on a
+    machine without a GPU mpd_create then fails with MPD_ERR_DEVICE); D2V16 is
+    refused with MPD_ERR_INVALID before any device call."""
     for model in (abi.MPD_MODEL_AC_HCZ, abi.MPD_MODEL_AC_P):
         for bc in (abi.MPD_BC_PERIODIC, abi.MPD_BC_WALL_BB, abi.MPD_BC_WALL_NEE):
             cfg = default_config(model, nx=48, ny=32)
@@ -125,5 +125,12 @@ def test_fp32_uw_configurations():
                 solver.lib().mpd_destroy(h)
             assert rc in (0, -2), (model, bc, rc)
     cfg = default_config(abi.MPD_MODEL_PP_D2Q9, nx=48, ny=32)
+    cfg.precision, cfg.flux = abi.MPD_FP32_BULK, abi.MPD_FLUX_UW
+    h = C.c_void_p()
+    rc = solver.lib().mpd_create(C.byref(cfg), C.byref(h))
+    if rc == 0:
+        solver.lib().mpd_destroy(h)
+    assert rc in (0, -2), rc
+    cfg = default_config(abi.MPD_MODEL_PP_D2V16, nx=48, ny=32)
     cfg.precision, cfg.flux = abi.MPD_FP32_BULK, abi.MPD_FLUX_UW
     assert solver.lib().mpd_create(C.byref(cfg), C.byref(C.c_void_p())) == -1

@@ -1,4 +1,4 @@
-"""FP32-bulk pseudopotential D2Q9 path (pp32.hip) against the FP64 oracle.
+"""FP32-bulk pseudopotential D2Q9 path (pp32.hip), CD and UW faces, against the FP64 oracle.
 
 Populations, bar-plus, face values and fluxes are FP32; the Carnahan-Starling
 pressure and Psi are FP64; the global sums are FP64 (compensated).  Stated
@@ -151,3 +151,41 @@ def test_pp32_tile_shapes_agree_bit_for_bit():
     with pytest.raises(solver.MpdError):
         s.set_tile(abi.MPD_TILE_16X16)
     s.close()
+
+
+@pytest.mark.parametrize("nx,ny,steps", [(64, 64, 100), (64, 64, 500), (70, 47, 300), (20, 18, 200)],
+                         ids=["64_100", "64_500", "70x47_300", "20x18_200"])
+def test_pp32_uw_error_vs_fp64_oracle(nx, ny, steps, oracle_mod):
+    """UW faces on the one-launch tile kernel (16x15 tiles; the bar-plus and Psi
+    regions one cell wider for the 6-point gradients) against the FP64
+    oracle's UW step, at the bar of the CD cases; tile-multiple, ragged and
+    narrower-than-a-tile meshes.  The result is far from the oracle's CD run
+    (the face scheme takes effect)."""
+    cfg, st = make_case("pp_d2q9", nx=nx, ny=ny, radius=0.25 * min(nx, ny), flux=abi.MPD_FLUX_UW,
+                        precision=abi.MPD_FP32_BULK)
+    ref = oracle_run(cfg, st, [steps])[steps]
+    cd = cfg.__class__.from_buffer_copy(cfg)
+    cd.flux = abi.MPD_FLUX_CD
+    ref_cd = oracle_run(cd, st, [steps])[steps]
+    s = solver.Solver(cfg)
+    assert s.tile() == abi.MPD_TILE_16X15
+    with pytest.raises(solver.MpdError):
+        s.set_path(abi.MPD_PATH_PHASES)
+    s.upload(st)
+    s.step(steps)
+    g = s.download()
+    r = s.reductions()
+    s.close()
+    N = nx * ny
+    errs = {"rho": rel_l2(g.m[abi.M["rho"]], ref.m[abi.M["rho"]]), "f": rel_l2(g.f, ref.f),
+            "psi": rel_l2(g.m[abi.M["psi"]], ref.m[abi.M["psi"]])}
+    for name in ("u", "v"):
+        errs[name] = rel_l2(g.m[abi.M[name]], ref.m[abi.M[name]], floor=1e-6 * np.sqrt(N))
+    d_cd = rel_l2(g.f, ref_cd.f)
+    mass = abs(r["sum_rho"] - ref.m[abi.M["rho"]].sum()) / ref.m[abi.M["rho"]].sum()
+    print("pp32 UW %dx%d %d steps: %s mass %.2e; f vs the CD oracle %.2e" % (nx, ny, steps, errs, mass, d_cd))
+    assert r["status"] == 0 and r["nonfinite"] == 0
+    assert errs["rho"] < 1e-5 and errs["f"] < 1e-5 and errs["psi"] < 1e-5, errs
+    assert errs["u"] < 1e-3 and errs["v"] < 1e-3, errs
+    assert mass < 1e-9
+    assert d_cd > 100 * errs["f"]
