@@ -477,6 +477,10 @@ static int alloc_phase_scratch(mpd_handle *h)
     if (S.fbp) return 0;
     const size_t Q = (size_t)num_q(h->cfg), N = S.N, QN = Q * N, yfn = (size_t)h->C.nyf * h->cfg.nx;
     bool pp = is_pp(h->cfg), ac = !pp, uw = h->cfg.flux == MPD_FLUX_UW;
+    // UW: fluxes of the second copies of the periodic faces (Allen-Cahn; pseudopotential
+    // with its copies' state, allocated at create), and the pseudopotential copies' scratch
+    const bool pp_copies = S.cbpsi_old != nullptr;
+    const size_t nb = (size_t)h->cfg.nx + h->cfg.ny;
     int r;
     if ((r = dalloc(h, &S.hbp, ac ? QN : 0)) || (r = dalloc(h, &S.fbp, QN)) ||
         (r = dalloc(h, &S.hbpx, ac && uw ? QN : 0)) || (r = dalloc(h, &S.hbpy, ac && uw ? QN : 0)) ||
@@ -486,10 +490,9 @@ static int alloc_phase_scratch(mpd_handle *h)
         (r = dalloc(h, &S.gm, h->C.walls ? (size_t)MPD_M_COUNT * 2 * h->cfg.nx : 0)) ||
         (r = dalloc(h, &S.ghbp, h->C.walls ? (size_t)Q9 * 2 * h->cfg.nx : 0)) ||
         (r = dalloc(h, &S.gfbp, h->C.walls ? (size_t)Q9 * 2 * h->cfg.nx : 0)) ||
-        (r = dalloc(h, &S.cbhl, ac && uw ? (size_t)Q9 * (h->cfg.nx + h->cfg.ny) : 0)) ||
-        (r = dalloc(h, &S.cbfl, (ac && uw) || S.cbpsi_old ? (size_t)Q9 * (h->cfg.nx + h->cfg.ny) : 0)) ||
-        (r = dalloc(h, &S.cbpsi, S.cbpsi_old ? (size_t)(h->cfg.nx + h->cfg.ny) : 0)) ||
-        (r = dalloc(h, &S.cbscr, S.cbpsi_old ? (size_t)4 * (h->cfg.nx + h->cfg.ny) : 0)) ||
+        (r = dalloc(h, &S.cbhl, ac && uw ? Q9 * nb : 0)) ||
+        (r = dalloc(h, &S.cbfl, (ac && uw) || pp_copies ? Q9 * nb : 0)) ||
+        (r = dalloc(h, &S.cbpsi, pp_copies ? nb : 0)) || (r = dalloc(h, &S.cbscr, pp_copies ? 4 * nb : 0)) ||
         (r = dalloc(h, &S.fpsi, pp ? 2 * N : 0)) || (r = dalloc(h, &S.fscr, pp ? 8 * N : 0)))
         return r;
     return 0;
