@@ -101,8 +101,11 @@ def test_fp32_mesh_beyond_buffer_range_refused():
     """FP32-bulk Allen-Cahn handles address a state set through 32-bit buffer
     resources: a mesh whose float planes would span 4 GiB or more is refused
     with MPD_ERR_INVALID by mpd_create, before any device call."""
-    for model, side in ((abi.MPD_MODEL_AC_HCZ, 7000), (abi.MPD_MODEL_AC_P, 8000)):
-        cfg = default_config(model, nx=side, ny=side)
+    # HCZ: 22 planes, at most 48,806,446 cells; 8192 x 5958 is the first row
+    # over (8192 x 5957 runs: tests/test_gpu_max_mesh.py)
+    for model, nx, ny in ((abi.MPD_MODEL_AC_HCZ, 7000, 7000), (abi.MPD_MODEL_AC_P, 8000, 8000),
+                          (abi.MPD_MODEL_AC_HCZ, 8192, 5958)):
+        cfg = default_config(model, nx=nx, ny=ny)
         cfg.precision = abi.MPD_FP32_BULK
         h = C.c_void_p()
         rc = solver.lib().mpd_create(C.byref(cfg), C.byref(h))
