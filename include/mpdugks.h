@@ -241,8 +241,8 @@ int mpd_kernel_timing(mpd_handle *h, double *avg_ms, int64_t *launches);
  * cells, mpd_batch_set_tile) of one case from a work list, and the step pair
  * is captured once in a HIP graph and replayed.  Members keep their own state;
  * mpd_download_state / mpd_reductions on a member synchronise the batch; a
- * member cannot be stepped, uploaded to, re-initialised or switched to another
- * path on its own (MPD_ERR_STATE).  No counterpart in the reference, which runs
+ * member cannot be stepped, uploaded to, re-initialised, switched to another
+ * path or given a tile shape on its own (MPD_ERR_STATE).  No counterpart in the reference, which runs
  * one case per process. */
 typedef struct mpd_batch mpd_batch;
 int mpd_batch_create(mpd_batch **out);

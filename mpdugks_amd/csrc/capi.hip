@@ -99,10 +99,10 @@ int dalloc(mpd_handle *h, double **p, size_t n)
     void *q = nullptr;
     hipError_t e = hipMalloc(&q, n * sizeof(double));
     if (e != hipSuccess) return fail(MPD_ERR_NOMEM, "hipMalloc(%zu doubles): %s", n, hipGetErrorString(e));
+    h->allocs.push_back(q);  // freed by mpd_destroy, also if the clearing below fails
+    h->bytes += (int64_t)(n * sizeof(double));
     e = hipMemsetAsync(q, 0, n * sizeof(double), h->stream);
     if (e != hipSuccess) return fail(MPD_ERR_DEVICE, "hipMemsetAsync: %s", hipGetErrorString(e));
-    h->allocs.push_back(q);
-    h->bytes += (int64_t)(n * sizeof(double));
     *p = (double *)q;
     return 0;
 }
@@ -449,6 +449,7 @@ int mpd_download_face_psi(mpd_handle *h, double *psi)
 int mpd_set_tile(mpd_handle *h, int32_t tile)
 {
     if (!h) return fail(MPD_ERR_INVALID, "null handle");
+    if (h->batch) return fail(MPD_ERR_STATE, "handle is a member of a batch: set the batch's tile");
     if (tile < MPD_TILE_AUTO || tile > MPD_TILE_32X16) return fail(MPD_ERR_INVALID, "unknown tile shape %d", tile);
     if (is_pp(h->cfg) && tile == MPD_TILE_16X16)
         return fail(MPD_ERR_INVALID, "the pseudopotential step has 16x15 and 32x16 tiles");
@@ -745,13 +746,20 @@ int mpd_step(mpd_handle *h, int64_t nsteps)
     }
     h->effective_path = path;
     hipEvent_t e0 = nullptr, e1 = nullptr;
+    // an error return hands the timing pair back to the pool
+    auto bail = [&](int r) {
+        if (e0) h->ev_free.push_back(e0);
+        if (e1) h->ev_free.push_back(e1);
+        return r;
+    };
     if (!h->per_launch) {
         if (int r = take_event(h, &e0)) return r;
-        if (int r = take_event(h, &e1)) return r;
-        HIPCHK(hipEventRecord(e0, h->stream));
+        if (int r = take_event(h, &e1)) return bail(r);
+        if (hipError_t e = hipEventRecord(e0, h->stream); e != hipSuccess)
+            return bail(fail(MPD_ERR_DEVICE, "hipEventRecord: %s", hipGetErrorString(e)));
     }
     if (path == MPD_PATH_PHASES) {
-        if (int r = alloc_phase_scratch(h)) return r;
+        if (int r = alloc_phase_scratch(h)) return bail(r);
         for (int64_t s = 0; s < nsteps; ++s) {
             phase_step(h->S, h->C, h->stream);
             // this step's face Psi becomes the previous step's for the next one
@@ -761,7 +769,7 @@ int mpd_step(mpd_handle *h, int64_t nsteps)
     } else {
         if (!h->F.ready) {
             int r = fused_alloc(h->F, h->S, h->C, h->cfg);
-            if (r) return fail(MPD_ERR_NOMEM, "fused path allocation failed");
+            if (r) return bail(fail(MPD_ERR_NOMEM, "fused path allocation failed"));
         }
         h->F.pp32.phases = pp32_phases;
         if (is_pp(h->cfg)) h->F.pp32.tile = h->tile;
@@ -780,7 +788,7 @@ int mpd_step(mpd_handle *h, int64_t nsteps)
             // The derived state needs two explicit steps: the first step after an
             // upload uses the uploaded sources (not derivable from phi), and the
             // momentum update of a step uses the sources of the step before.
-            if (int r = alloc_phase_scratch(h)) return r;
+            if (int r = alloc_phase_scratch(h)) return bail(r);
             for (; s < nsteps && h->prime_stage < 2; ++s) {
                 if (h->prime_stage == 0) fused_convert(h->F, h->S, h->C, 0, h->stream);
                 phase_step(h->S, h->C, h->stream);

@@ -55,9 +55,13 @@ def test_member_cannot_step_alone():
         s.upload(st)
     with pytest.raises(solver.MpdError, match="MPD_ERR_STATE"):
         s.set_path(abi.MPD_PATH_PHASES)
+    # nor given a tile shape of its own (the batch has one)
+    with pytest.raises(solver.MpdError, match="MPD_ERR_STATE"):
+        s.set_tile(abi.MPD_TILE_16X15)
     b.remove(s)
     s.step(1)
     s.upload(st)
+    s.set_tile(abi.MPD_TILE_16X15)
     b.close()
 
 
