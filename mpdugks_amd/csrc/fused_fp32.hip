@@ -1194,14 +1194,26 @@ int batch32_plan(Batch32Plan *P, const BatchMember *m, int n, int tile, hipStrea
         const int p = (b % 8) * per + b / 8;
         if (p < (int)list.size()) work[(size_t)b] = list[(size_t)p];
     }
+    // growing a table frees the old one first; a failed allocation leaves the
+    // table empty (capacity 0), never a freed pointer
     if ((size_t)n > P->cap_cases) {
         if (P->d_cases) (void)hipFree(P->d_cases);
-        if (hipMalloc(&P->d_cases, sizeof(BatchCase) * (size_t)n) != hipSuccess) return -1;
+        P->d_cases = nullptr;
+        P->cap_cases = 0;
+        if (hipMalloc(&P->d_cases, sizeof(BatchCase) * (size_t)n) != hipSuccess) {
+            P->d_cases = nullptr;
+            return -1;
+        }
         P->cap_cases = (size_t)n;
     }
     if (work.size() > P->cap_work) {
         if (P->d_work) (void)hipFree(P->d_work);
-        if (hipMalloc(&P->d_work, sizeof(int2) * work.size()) != hipSuccess) return -1;
+        P->d_work = nullptr;
+        P->cap_work = 0;
+        if (hipMalloc(&P->d_work, sizeof(int2) * work.size()) != hipSuccess) {
+            P->d_work = nullptr;
+            return -1;
+        }
         P->cap_work = work.size();
     }
     if (n && hipMemcpyAsync(P->d_cases, cs.data(), sizeof(BatchCase) * (size_t)n, hipMemcpyHostToDevice, st) != hipSuccess)
