@@ -62,12 +62,16 @@ class GpuEngine:
         from . import abi
         cfg, st = req.make()
         s = self._solver.Solver(cfg, path=abi.MPD_PATH_FUSED)
-        if isinstance(st, list):
-            s.init_drops(st, sign=-1)   # droplets built on the device: no host state, no upload
-        else:
-            s.upload(st)
-        s.step(2)
-        self.batch.add(s)
+        try:
+            if isinstance(st, list):
+                s.init_drops(st, sign=-1)   # droplets built on the device: no host state, no upload
+            else:
+                s.upload(st)
+            s.step(2)
+            self.batch.add(s)
+        except Exception:
+            s.close()
+            raise
         req.handle = s
         req.steps = 2
         return s.device_bytes()
@@ -80,7 +84,9 @@ class GpuEngine:
             h.step(n)
 
     def refine(self, req):
-        """Continue `req` as an FP64 handle from its current state."""
+        """Continue `req` as an FP64 handle from its current state.  Returns its
+        device bytes so far: the phase scratch and the derived planes are
+        allocated at its first step."""
         from . import abi
         st = req.handle.download()
         cfg = type(st.cfg).from_buffer_copy(st.cfg)
@@ -107,6 +113,9 @@ class GpuEngine:
         req.handle = None
 
     def close(self):
+        for h in self.fp64:
+            h.close()
+        self.fp64 = []
         self.batch.close()
 
 
@@ -170,8 +179,10 @@ class Server:
                     continue
                 if self.switch_tol is not None and r.fp64_from is None and res < self.switch_tol:
                     # past what FP32 resolves: the rest in FP64, residual history restarted
+                    # the FP32 estimate covers the FP64 handle's full footprint (its
+                    # phase scratch and derived planes appear only at its first step)
                     self.used -= r.bytes_estimate
-                    r.bytes_estimate = self.engine.refine(r)
+                    r.bytes_estimate = max(r.bytes_estimate, self.engine.refine(r))
                     self.used += r.bytes_estimate
                     r.fp64_from = r.steps
                     r.ke_prev = None

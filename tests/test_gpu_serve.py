@@ -70,3 +70,19 @@ def test_server_continues_in_fp64_to_1e8():
     t.close()
     assert np.array_equal(alone.f, r.final.f) and np.array_equal(alone.h, r.final.h)
     assert np.array_equal(alone.m, r.final.m)
+
+
+@pytest.mark.parametrize("precision", [abi.MPD_FP32_BULK, abi.MPD_FP64])
+def test_admission_estimate_covers_device_bytes(precision):
+    """coalescence_requests budgets 1100 B per cell.  That covers a primed case in
+    either precision: the FP64 explicit state, the phase scratch of the priming
+    steps and the two derived plane sets (FP32, or FP64 for a continuation)."""
+    for side in (64, 256):
+        cfg, drops = cases.coalescence_config(10.0, 10.0, 4, precision=precision, sides=(side,))
+        s = solver.Solver(cfg, path=abi.MPD_PATH_FUSED)
+        s.init_drops(drops, sign=-1)
+        s.step(3)
+        b = s.device_bytes()
+        s.close()
+        print("precision %d, %d^2: %.1f B per cell" % (precision, side, b / side ** 2))
+        assert b <= 1100 * side * side
