@@ -210,11 +210,13 @@ class Solver:
 
     def set_face_order(self, order):
         """Pseudopotential face force order (abi.MPD_FACE_ORDER_*): Jacobi, or the
-        reference's one-thread order (FP64 D2Q9, CD, periodic)."""
+        reference's one-thread order (FP64 D2Q9, CD or UW, periodic)."""
         _check(lib().mpd_set_face_order(self._h, int(order)), "mpd_set_face_order")
 
     def upload_face_psi(self, psi):
         p = _f64(psi)
+        if p is None or p.size != 2 * self.cfg.nx * self.cfg.ny:
+            raise MpdError("upload_face_psi: need 2*nx*ny values")
         _check(lib().mpd_upload_face_psi(self._h, _p(p)), "mpd_upload_face_psi")
 
     def download_face_psi(self):
@@ -270,7 +272,18 @@ class Solver:
                "mpd_device_init_pp_flat")
 
     def upload(self, s):
-        arrs = [_f64(a) for a in (s.h, s.f, s.m, s.ffx, s.ffy)]
+        # the library reads this many values from each array it takes
+        N = self.cfg.nx * self.cfg.ny
+        pp = not abi.has_h(self.cfg.model)
+        need = {"h": None if pp else 9 * N, "f": abi.num_q(self.cfg.model) * N, "m": NM * N,
+                "ffx": 2 * N if pp else None, "ffy": 2 * N if pp else None}
+        arrs = []
+        for name in ("h", "f", "m", "ffx", "ffy"):
+            a = _f64(getattr(s, name))
+            if a is not None and need[name] is not None and a.size != need[name]:
+                raise MpdError("upload: %s holds %d values; this handle's %dx%d mesh needs %d"
+                               % (name, a.size, self.cfg.nx, self.cfg.ny, need[name]))
+            arrs.append(a)
         _check(lib().mpd_upload_state(self._h, *[_p(a) for a in arrs]), "mpd_upload_state")
 
     def download(self):

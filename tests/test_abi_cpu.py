@@ -6,6 +6,7 @@ import os
 import re
 
 import numpy as np
+import pytest
 
 from mpdugks_amd import abi, solver
 from mpdugks_amd.config import default_config
@@ -137,3 +138,19 @@ def test_fp32_uw_configurations():
     cfg = default_config(abi.MPD_MODEL_PP_D2V16, nx=48, ny=32)
     cfg.precision, cfg.flux = abi.MPD_FP32_BULK, abi.MPD_FLUX_UW
     assert solver.lib().mpd_create(C.byref(cfg), C.byref(C.c_void_p())) == -1
+
+
+def test_upload_checks_array_sizes_before_the_library():
+    """A host state built for another mesh is refused on the host: the library
+    would read nx*ny-sized planes past the end of the smaller arrays."""
+    cfg = default_config(abi.MPD_MODEL_AC_HCZ, nx=16, ny=12)
+    s = solver.Solver.__new__(solver.Solver)   # no device handle: the check comes first
+    s.cfg, s._h = cfg, C.c_void_p()
+    with pytest.raises(solver.MpdError, match="mesh needs"):
+        s.upload(solver.State(default_config(abi.MPD_MODEL_AC_HCZ, nx=8, ny=12)))
+    pp = default_config(abi.MPD_MODEL_PP_D2Q9, nx=16, ny=12)
+    s.cfg = pp
+    st = solver.State(pp)
+    st.ffx = st.ffx[:10]
+    with pytest.raises(solver.MpdError, match="ffx holds 10 values"):
+        s.upload(st)
