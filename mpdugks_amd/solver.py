@@ -242,14 +242,20 @@ class Solver:
                "mpd_device_init_drops")
 
     def init_interface(self, y0, sign=+1, p0=1.0):
-        y = np.ascontiguousarray(y0, dtype=np.float64)
+        y = self._heights(y0)
         _check(lib().mpd_device_init_interface(self._h, _p(y), int(sign), p0), "mpd_device_init_interface")
 
     def init_interface_noalloc(self, y0, sign=+1, p0=1.0):
         """init_interface through the entry point that allocates no device memory."""
-        y = np.ascontiguousarray(y0, dtype=np.float64)
+        y = self._heights(y0)
         _check(lib().mpd_device_init_interface_noalloc(self._h, _p(y), int(sign), p0),
                "mpd_device_init_interface_noalloc")
+
+    def _heights(self, y0):
+        y = np.ascontiguousarray(y0, dtype=np.float64)
+        if y.size != self.cfg.nx:
+            raise MpdError("interface heights: need nx = %d values, got %d" % (self.cfg.nx, y.size))
+        return y
 
     def init_pp_noise(self, rho0, amp=0.01, seed=0, mu0=0.1):
         _check(lib().mpd_device_init_pp_noise(self._h, rho0, amp, int(seed), mu0), "mpd_device_init_pp_noise")
