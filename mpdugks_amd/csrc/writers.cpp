@@ -18,6 +18,14 @@ namespace {
 
 const char *const FS = "    ";  // field separator of the reference's writers (ZeroMacros.h:21)
 
+// a fixed-size text field of a caller's struct, read up to its first NUL or its
+// end (a field that fills its array has no terminator)
+template <size_t N>
+std::string str(const char (&a)[N])
+{
+    return std::string(a, strnlen(a, N));
+}
+
 int put_name(const std::string &s, char *buf, int32_t len)
 {
     if (!buf || len <= 0) return MPD_ERR_INVALID;
@@ -30,7 +38,7 @@ std::string tag(const mpd_output_info &o)
 {
     // "Ma<Ma>_<mesh type><NL>_CFL<CFL>" (Output.cpp:319-320,487-488)
     std::ostringstream s;
-    s << "Ma" << o.ma << "_" << o.mesh_type << o.mesh_nl << "_CFL" << o.cfl;
+    s << "Ma" << o.ma << "_" << str(o.mesh_type) << o.mesh_nl << "_CFL" << o.cfl;
     return s.str();
 }
 
@@ -164,10 +172,10 @@ int mpd_output_name(int32_t kind, const mpd_output_info *o, int64_t step, double
         s << "global/" << "step" << step << tag(*o) << "_T" << t << ".dat";
         break;
     case MPD_OUT_SUMRHO:     // Output_SumRho, :258-261
-        s << "Convergence/SumRho_mu" << o->mu0 << "_Re" << o->re << "_" << o->mesh_file << ".dat";
+        s << "Convergence/SumRho_mu" << o->mu0 << "_Re" << o->re << "_" << str(o->mesh_file) << ".dat";
         break;
     case MPD_OUT_RESIDUAL:   // Output_Residual, :272-275
-        s << "Convergence/Residual_mu" << o->mu0 << "_Re" << o->re << "_" << o->mesh_file << ".dat";
+        s << "Convergence/Residual_mu" << o->mu0 << "_Re" << o->re << "_" << str(o->mesh_file) << ".dat";
         break;
     case MPD_OUT_MIDX:       // Output_MidX, :318-320
     case MPD_OUT_MIDY:       // Output_MidY, :411-413
@@ -278,14 +286,14 @@ int mpd_write_case(const char *path, const mpd_case_info *c)
     std::ofstream f(path);
     if (!f) return MPD_ERR_INVALID;
     const char *e = "    =    ";
-    f << c->case_name << e << "Case Name" << std::endl
-      << c->mesh_file << e << "Mesh File" << std::endl
-      << c->mesh_type << e << "Mesh Type" << std::endl
-      << c->flux_scheme << e << "Flux Scheme" << std::endl
-      << c->bc << e << "Boundary Condition" << std::endl
-      << c->velocity_model << e << "velocity model" << std::endl
-      << c->force_model << e << "Force model" << std::endl
-      << c->eos_model << e << "EoS model" << std::endl
+    f << str(c->case_name) << e << "Case Name" << std::endl
+      << str(c->mesh_file) << e << "Mesh File" << std::endl
+      << str(c->mesh_type) << e << "Mesh Type" << std::endl
+      << str(c->flux_scheme) << e << "Flux Scheme" << std::endl
+      << str(c->bc) << e << "Boundary Condition" << std::endl
+      << str(c->velocity_model) << e << "velocity model" << std::endl
+      << str(c->force_model) << e << "Force model" << std::endl
+      << str(c->eos_model) << e << "EoS model" << std::endl
       << c->zone_left << FS << "=" << FS << "left" << std::endl
       << c->zone_right << FS << "=" << FS << "right" << std::endl
       << c->zone_top << FS << "=" << FS << "top" << std::endl

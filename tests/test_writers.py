@@ -7,6 +7,7 @@ Residual records at step 1000.  The states fed to the library's writers come
 from the bit-exact initialiser and the bit-exact oracle (tests/test_oracle_golden.py
 pins both), so every file must match exactly.  Host code only: no GPU.
 """
+import ctypes as C
 import hashlib
 import json
 import os
@@ -98,3 +99,17 @@ def test_flowfield_cell_order(tmp_path, states):
     assert lines[0] == "(300 (101 2 1 0 0 1 16384)("
     assert float(lines[1]) == float("%.12e" % rho[order[0]])
     assert float(lines[5]) == float("%.12e" % rho[order[4]])
+
+
+def test_unterminated_text_fields_are_read_to_their_end():
+    """A text field that fills its array has no NUL: the writers read it to the
+    end of the array and no further (here mesh_file, 128 letters, and mesh_type,
+    32, followed in the struct by nothing that may be read as text)."""
+    cfg = default_config(abi.MPD_MODEL_AC_HCZ, nx=32, ny=32)
+    info = solver.output_info(cfg)
+    C.memmove(C.addressof(info) + type(info).mesh_file.offset, b"m" * 128, 128)
+    C.memmove(C.addressof(info) + type(info).mesh_type.offset, b"t" * 32, 32)
+    name = solver.output_name(abi.MPD_OUT_SUMRHO, info, 0, 0.0)
+    assert name.endswith("_" + "m" * 128 + ".dat")
+    name = solver.output_name(abi.MPD_OUT_FLOWFIELD, info, 5, 1.0)
+    assert "_" + "t" * 32 + str(info.mesh_nl) + "_CFL" in name
