@@ -15,6 +15,7 @@
 #include "device_state.h"
 #include "d2v16.h"
 #include "fused.h"
+#include "last_error.h"
 #include "physics.h"
 
 using namespace mpd;
@@ -23,14 +24,20 @@ namespace {
 
 thread_local std::string g_err;
 
-int fail(int code, const char *fmt, ...)
+int vfail(int code, const char *fmt, va_list ap)
 {
     char buf[512];
+    std::vsnprintf(buf, sizeof(buf), fmt, ap);
+    g_err = buf;
+    return code;
+}
+
+int fail(int code, const char *fmt, ...)
+{
     va_list ap;
     va_start(ap, fmt);
-    std::vsnprintf(buf, sizeof(buf), fmt, ap);
+    vfail(code, fmt, ap);
     va_end(ap);
-    g_err = buf;
     return code;
 }
 
@@ -41,6 +48,15 @@ int fail(int code, const char *fmt, ...)
     } while (0)
 
 }  // namespace
+
+int mpd::set_error(int code, const char *fmt, ...)
+{
+    va_list ap;
+    va_start(ap, fmt);
+    vfail(code, fmt, ap);
+    va_end(ap);
+    return code;
+}
 
 struct mpd_batch;
 

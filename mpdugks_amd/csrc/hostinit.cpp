@@ -8,6 +8,7 @@
 
 #include "../../include/mpdugks.h"
 #include "d2v16.h"
+#include "last_error.h"
 
 namespace {
 
@@ -52,10 +53,20 @@ void cell_eq(const mpd_config &c, double phi, double rho, double U, double V, do
     }
 }
 
-int valid(const mpd_config *c)
+// 0, or MPD_ERR_INVALID with the reason for mpd_last_error: fn's config, its
+// model family (Allen-Cahn, or pseudopotential) and its output arrays
+int check(const char *fn, const mpd_config *c, bool allen_cahn, bool arrays)
 {
-    return c && c->nx > 0 && c->ny > 0 && c->rt > 0 && c->model >= MPD_MODEL_AC_P &&
-           c->model <= MPD_MODEL_PP_D2V16;
+    if (!c) return mpd::set_error(MPD_ERR_INVALID, "%s: null config", fn);
+    if (c->model < MPD_MODEL_AC_P || c->model > MPD_MODEL_PP_D2V16)
+        return mpd::set_error(MPD_ERR_INVALID, "%s: unknown model %d", fn, c->model);
+    if (!(c->nx > 0 && c->ny > 0 && c->rt > 0))
+        return mpd::set_error(MPD_ERR_INVALID, "%s: need nx, ny and rt > 0", fn);
+    if (allen_cahn != (c->model < MPD_MODEL_PP_D2Q9))
+        return mpd::set_error(MPD_ERR_INVALID, "%s: starts the %s models", fn,
+                              allen_cahn ? "Allen-Cahn" : "pseudopotential");
+    if (!arrays) return mpd::set_error(MPD_ERR_INVALID, "%s: null input or output array", fn);
+    return 0;
 }
 
 }  // namespace
@@ -70,7 +81,7 @@ extern "C" {
 int mpd_init_ac_drop(const mpd_config *cfg, double cx, double cy, double radius, int32_t sign,
                      double p0, double mu0, double *tilde_h, double *tilde_f, double *macro)
 {
-    if (!valid(cfg) || cfg->model >= MPD_MODEL_PP_D2Q9 || !tilde_h || !tilde_f || !macro) return MPD_ERR_INVALID;
+    if (int r = check("mpd_init_ac_drop", cfg, true, tilde_h && tilde_f && macro)) return r;
     const mpd_config &c = *cfg;
     const size_t N = (size_t)c.nx * c.ny;
     const double aphi = (c.rho_l - c.rho_v) / (c.phi_l - c.phi_v), bphi = c.rho_v - aphi * c.phi_v;
@@ -106,8 +117,7 @@ int mpd_init_ac_drop(const mpd_config *cfg, double cx, double cy, double radius,
 int mpd_init_ac_interface(const mpd_config *cfg, const double *y0, int32_t sign, double p0, double *tilde_h,
                           double *tilde_f, double *macro)
 {
-    if (!valid(cfg) || cfg->model >= MPD_MODEL_PP_D2Q9 || !y0 || !tilde_h || !tilde_f || !macro)
-        return MPD_ERR_INVALID;
+    if (int r = check("mpd_init_ac_interface", cfg, true, y0 && tilde_h && tilde_f && macro)) return r;
     const mpd_config &c = *cfg;
     const size_t N = (size_t)c.nx * c.ny;
     const double aphi = (c.rho_l - c.rho_v) / (c.phi_l - c.phi_v), bphi = c.rho_v - aphi * c.phi_v;
@@ -142,7 +152,7 @@ int mpd_init_ac_interface(const mpd_config *cfg, const double *y0, int32_t sign,
 int mpd_init_pp_drop(const mpd_config *cfg, double cx, double cy, double radius, double rho_l, double rho_v,
                      double width, double mu0, double *tilde_f, double *macro, double *face_fx, double *face_fy)
 {
-    if (!valid(cfg) || cfg->model < MPD_MODEL_PP_D2Q9 || !tilde_f || !macro) return MPD_ERR_INVALID;
+    if (int r = check("mpd_init_pp_drop", cfg, false, tilde_f && macro)) return r;
     const mpd_config &c = *cfg;
     const size_t N = (size_t)c.nx * c.ny;
     std::memset(macro, 0, sizeof(double) * MPD_M_COUNT * N);
@@ -173,7 +183,7 @@ int mpd_init_pp_drop(const mpd_config *cfg, double cx, double cy, double radius,
 int mpd_init_pp_flat(const mpd_config *cfg, double y_lo, double y_hi, double slope, double rho_l, double rho_v,
                      double mu0, double *tilde_f, double *macro, double *face_fx, double *face_fy)
 {
-    if (!valid(cfg) || cfg->model < MPD_MODEL_PP_D2Q9 || !tilde_f || !macro) return MPD_ERR_INVALID;
+    if (int r = check("mpd_init_pp_flat", cfg, false, tilde_f && macro)) return r;
     const mpd_config &c = *cfg;
     const size_t N = (size_t)c.nx * c.ny;
     std::memset(macro, 0, sizeof(double) * MPD_M_COUNT * N);
@@ -202,7 +212,7 @@ int mpd_init_pp_flat(const mpd_config *cfg, double y_lo, double y_hi, double slo
 int mpd_init_pp_noise(const mpd_config *cfg, double rho0, double amp, uint64_t seed, double mu0,
                       double *tilde_f, double *macro, double *face_fx, double *face_fy)
 {
-    if (!valid(cfg) || cfg->model < MPD_MODEL_PP_D2Q9 || !tilde_f || !macro) return MPD_ERR_INVALID;
+    if (int r = check("mpd_init_pp_noise", cfg, false, tilde_f && macro)) return r;
     const mpd_config &c = *cfg;
     const size_t N = (size_t)c.nx * c.ny;
     std::memset(macro, 0, sizeof(double) * MPD_M_COUNT * N);

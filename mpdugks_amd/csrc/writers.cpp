@@ -13,6 +13,7 @@
 #include <string>
 
 #include "../../include/mpdugks.h"
+#include "last_error.h"
 
 namespace {
 
@@ -26,10 +27,23 @@ std::string str(const char (&a)[N])
     return std::string(a, strnlen(a, N));
 }
 
+// MPD_ERR_INVALID with the reason for mpd_last_error
+int bad(const char *fn, const char *what) { return mpd::set_error(MPD_ERR_INVALID, "%s: %s", fn, what); }
+int cannot_open(const char *fn, const char *path)
+{
+    return mpd::set_error(MPD_ERR_INVALID, "%s: cannot open %s for writing", fn, path);
+}
+int written(const char *fn, const char *path, const std::ofstream &f)
+{
+    return f ? MPD_OK : mpd::set_error(MPD_ERR_INVALID, "%s: writing %s failed", fn, path);
+}
+
 int put_name(const std::string &s, char *buf, int32_t len)
 {
-    if (!buf || len <= 0) return MPD_ERR_INVALID;
-    if ((int64_t)s.size() + 1 > len) return MPD_ERR_INVALID;
+    if (!buf || len <= 0) return bad("mpd_output_name", "null or empty output buffer");
+    if ((int64_t)s.size() + 1 > len)
+        return mpd::set_error(MPD_ERR_INVALID, "mpd_output_name: the name takes %zu bytes, the buffer %d",
+                              s.size() + 1, len);
     std::memcpy(buf, s.c_str(), s.size() + 1);
     return MPD_OK;
 }
@@ -73,7 +87,7 @@ extern "C" {
 
 int mpd_default_output_info(const mpd_config *cfg, mpd_output_info *o)
 {
-    if (!cfg || !o) return MPD_ERR_INVALID;
+    if (!cfg || !o) return bad("mpd_default_output_info", "null argument");
     std::memset(o, 0, sizeof(*o));
     const mpd_config &c = *cfg;
     o->ma = 0.0;                             // U0 / sqrt(R0 T0) with U0 = 0 (ZeroReference.h)
@@ -93,7 +107,7 @@ int mpd_default_output_info(const mpd_config *cfg, mpd_output_info *o)
 
 int mpd_default_case_info(const mpd_config *cfg, mpd_case_info *ci)
 {
-    if (!cfg || !ci) return MPD_ERR_INVALID;
+    if (!cfg || !ci) return bad("mpd_default_case_info", "null argument");
     std::memset(ci, 0, sizeof(*ci));
     const mpd_config &c = *cfg;
     const bool ac = c.model == MPD_MODEL_AC_P || c.model == MPD_MODEL_AC_HCZ;
@@ -165,7 +179,7 @@ int mpd_default_case_info(const mpd_config *cfg, mpd_case_info *ci)
 
 int mpd_output_name(int32_t kind, const mpd_output_info *o, int64_t step, double t, char *buf, int32_t len)
 {
-    if (!o) return MPD_ERR_INVALID;
+    if (!o) return bad("mpd_output_name", "null output info");
     std::ostringstream s;
     switch (kind) {
     case MPD_OUT_FLOWFIELD:  // Output_Flowfield, Output.cpp:486-488
@@ -184,7 +198,7 @@ int mpd_output_name(int32_t kind, const mpd_output_info *o, int64_t step, double
           << tag(*o) << "ratio_" << o->rho_l / o->rho_v << "step" << step << ".dat";
         break;
     default:
-        return MPD_ERR_INVALID;
+        return mpd::set_error(MPD_ERR_INVALID, "mpd_output_name: unknown kind %d", kind);
     }
     return put_name(s.str(), buf, len);
 }
@@ -195,9 +209,10 @@ int mpd_output_name(int32_t kind, const mpd_output_info *o, int64_t step, double
 int mpd_write_flowfield(const char *path, int32_t nx, int32_t ny, const double *macro, const int64_t *order,
                         const mpd_output_info *o)
 {
-    if (!path || !macro || !o || nx <= 0 || ny <= 0) return MPD_ERR_INVALID;
+    if (!path || !macro || !o || nx <= 0 || ny <= 0)
+        return bad("mpd_write_flowfield", "null argument or empty mesh");
     std::ofstream f(path);
-    if (!f) return MPD_ERR_INVALID;
+    if (!f) return cannot_open("mpd_write_flowfield", path);
     f << std::setiosflags(std::ios::scientific) << std::setprecision(12);
     const size_t N = (size_t)nx * ny;
     auto cell = [&](size_t k) { return order ? (size_t)order[k] : k; };
@@ -221,7 +236,7 @@ int mpd_write_flowfield(const char *path, int32_t nx, int32_t ny, const double *
     zone(702, MPD_M_FX);
     zone(703, MPD_M_FY);
     f.close();
-    return f ? MPD_OK : MPD_ERR_INVALID;
+    return written("mpd_write_flowfield", path, f);
 }
 
 // Output_MidX / Output_MidY / Output_Diag (Output.cpp:309-460): Rho, DP (p),
@@ -230,12 +245,13 @@ int mpd_write_flowfield(const char *path, int32_t nx, int32_t ny, const double *
 int mpd_write_profile(const char *path, int32_t kind, const mpd_config *cfg, const double *macro,
                       const mpd_output_info *o)
 {
-    if (!path || !cfg || !macro || !o) return MPD_ERR_INVALID;
+    if (!path || !cfg || !macro || !o) return bad("mpd_write_profile", "null argument");
     const int nx = cfg->nx, ny = cfg->ny;
-    if (kind != MPD_OUT_MIDX && kind != MPD_OUT_MIDY && kind != MPD_OUT_DIAG) return MPD_ERR_INVALID;
-    if (kind == MPD_OUT_DIAG && ny < nx) return MPD_ERR_INVALID;
+    if (kind != MPD_OUT_MIDX && kind != MPD_OUT_MIDY && kind != MPD_OUT_DIAG)
+        return mpd::set_error(MPD_ERR_INVALID, "mpd_write_profile: kind %d is not a profile", kind);
+    if (kind == MPD_OUT_DIAG && ny < nx) return bad("mpd_write_profile", "the diagonal profile needs ny >= nx");
     std::ofstream f(path);
-    if (!f) return MPD_ERR_INVALID;
+    if (!f) return cannot_open("mpd_write_profile", path);
     f << std::setiosflags(std::ios::scientific) << std::setprecision(kind == MPD_OUT_DIAG ? 8 : 20);
     const size_t N = (size_t)nx * ny;
     const int count = kind == MPD_OUT_MIDY || kind == MPD_OUT_DIAG ? nx : ny;
@@ -264,27 +280,27 @@ int mpd_write_profile(const char *path, int32_t kind, const mpd_config *cfg, con
         }
     }
     f.close();
-    return f ? MPD_OK : MPD_ERR_INVALID;
+    return written("mpd_write_profile", path, f);
 }
 
 // Output_SumRho / Output_Residual (Output.cpp:258-287): append "t    value".
 int mpd_append_convergence(const char *path, double t, double value)
 {
-    if (!path) return MPD_ERR_INVALID;
+    if (!path) return bad("mpd_append_convergence", "null path");
     std::ofstream f(path, std::ofstream::app);
-    if (!f) return MPD_ERR_INVALID;
+    if (!f) return cannot_open("mpd_append_convergence", path);
     f << std::setiosflags(std::ios::scientific) << std::setprecision(8);
     f << t << FS << value << std::endl;
     f.close();
-    return f ? MPD_OK : MPD_ERR_INVALID;
+    return written("mpd_append_convergence", path, f);
 }
 
 // OutputCase (Output.cpp:65-144): Case.ark.
 int mpd_write_case(const char *path, const mpd_case_info *c)
 {
-    if (!path || !c) return MPD_ERR_INVALID;
+    if (!path || !c) return bad("mpd_write_case", "null argument");
     std::ofstream f(path);
-    if (!f) return MPD_ERR_INVALID;
+    if (!f) return cannot_open("mpd_write_case", path);
     const char *e = "    =    ";
     f << str(c->case_name) << e << "Case Name" << std::endl
       << str(c->mesh_file) << e << "Mesh File" << std::endl
@@ -354,7 +370,7 @@ int mpd_write_case(const char *path, const mpd_case_info *c)
     }
     f << "-------------End--------------" << std::endl;
     f.close();
-    return f ? MPD_OK : MPD_ERR_INVALID;
+    return written("mpd_write_case", path, f);
 }
 
 }  // extern "C"

@@ -154,3 +154,21 @@ def test_upload_checks_array_sizes_before_the_library():
     st.ffx = st.ffx[:10]
     with pytest.raises(solver.MpdError, match="ffx holds 10 values"):
         s.upload(st)
+
+
+def test_host_entry_points_report_why_they_fail():
+    """The host initialisers and the writers set mpd_last_error as the device
+    entry points do."""
+    L = solver.lib()
+    pp = default_config(abi.MPD_MODEL_PP_D2Q9, nx=8, ny=8)
+    h = np.zeros((9, 64))
+    rc = L.mpd_init_ac_drop(C.byref(pp), 4.0, 4.0, 2.0, -1, 1.0, 0.1, h.ctypes.data_as(C.c_void_p),
+                            h.ctypes.data_as(C.c_void_p), np.zeros((abi.NM, 64)).ctypes.data_as(C.c_void_p))
+    assert rc == -1
+    assert b"mpd_init_ac_drop: starts the Allen-Cahn models" in L.mpd_last_error()
+    info = solver.output_info(default_config(abi.MPD_MODEL_AC_HCZ, nx=8, ny=8))
+    buf = C.create_string_buffer(8)
+    assert L.mpd_output_name(abi.MPD_OUT_SUMRHO, C.byref(info), 0, 0.0, buf, 8) == -1
+    assert b"the buffer 8" in L.mpd_last_error()
+    rc = L.mpd_append_convergence(b"/nonexistent-dir/x.dat", 0.0, 1.0)
+    assert rc == -1 and b"cannot open /nonexistent-dir/x.dat" in L.mpd_last_error()
