@@ -371,10 +371,12 @@ def main():
         sv = serving_bench(args.serve_requests, args.serve_max_steps, args.serve_budget_gb, glups,
                            ke_tol=args.serve_ke_tol)
         line["serving"] = sv
+        conv = ("p50 %.3f s, p99 %.3f s" % (sv["p50_latency_converged_s"], sv["p99_latency_converged_s"])
+                if sv["converged"] else "none")
         print("[bench] serving: %d done, %d converged to KE residual %.0e, %.2f solves/s, p50 %.3f s, "
-              "p99 %.3f s, busy %.2f, mean %.0f steps"
+              "p99 %.3f s (converged requests: %s), busy %.2f, mean %.0f steps"
               % (sv["completed"], sv["converged"], sv["ke_tol"], sv["solves_per_s"], sv["p50_latency_s"],
-                 sv["p99_latency_s"], sv["gpu_busy_frac"], sv["mean_steps"]), file=sys.stderr, flush=True)
+                 sv["p99_latency_s"], conv, sv["gpu_busy_frac"], sv["mean_steps"]), file=sys.stderr, flush=True)
     if not args.no_pp:
         pp = pp_bench(steps=args.pp_steps)
         line["pseudopotential"] = pp

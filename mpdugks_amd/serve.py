@@ -190,6 +190,9 @@ class Server:
             active = still
         wall = self.clock() - t0
         lat = np.array([r.done - r.arrival for r in finished]) if finished else np.array([0.0])
+        # the requests that met the tolerance (the others stopped at the step cap)
+        conv = np.array([r.done - r.arrival for r in finished if r.converged])
+        pct = lambda a, q: float(np.percentile(a, q)) if a.size else None
         return {
             "completed": len(finished),
             "rejected": len(self.rejected),
@@ -198,6 +201,8 @@ class Server:
             "solves_per_s": len(finished) / wall if wall > 0 else 0.0,
             "p50_latency_s": float(np.percentile(lat, 50)),
             "p99_latency_s": float(np.percentile(lat, 99)),
+            "p50_latency_converged_s": pct(conv, 50),
+            "p99_latency_converged_s": pct(conv, 99),
             "gpu_busy_frac": busy / wall if wall > 0 else 0.0,
             "cell_steps": int(sum(r.cells * r.steps for r in finished)),
             "max_active": max_active,
