@@ -62,7 +62,9 @@ typedef enum mpd_bc {
 /* Arithmetic of the bulk update. */
 typedef enum mpd_precision {
     MPD_FP64 = 0,            /* plain FP64 drop-in */
-    MPD_FP32_BULK = 1        /* FP32 populations/arithmetic, FP64 EOS and global sums */
+    MPD_FP32_BULK = 1        /* FP32 populations/arithmetic, FP64 EOS and global sums: Allen-Cahn
+                                models (CD, UW; periodic or walled y), pseudopotential D2Q9 (CD, UW)
+                                and D2V16 (CD), periodic */
 } mpd_precision;
 
 /* Run-time replacement for the compile-time constants of
@@ -177,7 +179,8 @@ int mpd_download_face_psi(mpd_handle *h, double *psi);
 /* Tile shape of the one-launch step kernels (cells per workgroup): the
  * FP32-bulk Allen-Cahn kernel has all three with CD faces and 16x15 with UW
  * faces (a 3-cell halo), the FP32-bulk pseudopotential
- * kernel 16x15 and 32x16, the FP64 Allen-Cahn kernel 16x15 and 16x16 (other
+ * kernel 16x15 and 32x16 (D2Q9 with CD faces; 16x15 otherwise), the FP64
+ * Allen-Cahn kernel 16x15 and 16x16 (other
  * shapes are refused with MPD_ERR_INVALID).
  * MPD_TILE_AUTO (the default) chooses per mesh; the others force one.  The
  * state does not depend on it (bit for bit with periodic and BB walls, to

@@ -2,6 +2,7 @@
 // state upload/download, stepping and global reductions.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -276,7 +277,7 @@ int check_cfg(const mpd_config *c)
     if (c->precision == MPD_FP32_BULK && !fused_supported(*c))
         return fail(MPD_ERR_INVALID, "no FP32-bulk kernel for this model / flux / boundary (FP32-bulk: Allen-Cahn "
                                      "with CD or UW faces and periodic, bounce-back or NEE y; pseudopotential D2Q9 "
-                                     "with CD or UW faces, periodic)");
+                                     "with CD or UW faces, D2V16 with CD faces, periodic)");
     if (c->precision == MPD_FP32_BULK && !is_pp(*c)) {
         // the FP32 step kernel addresses each state set through one 32-bit buffer
         // resource (all float planes) and the FP64 phase field through another
@@ -285,6 +286,16 @@ int check_cfg(const mpd_config *c)
             return fail(MPD_ERR_INVALID, "mesh %dx%d too large for the FP32-bulk Allen-Cahn kernel (%d float "
                                          "planes of 4 B per cell must span < 4 GiB: at most %lld cells)",
                         c->nx, c->ny, fused_nplanes(*c), (long long)((lim - 1) / (4 * fused_nplanes(*c))));
+    }
+    if (c->precision == MPD_FP32_BULK && is_pp(*c)) {
+        // the FP32 pseudopotential step addresses a state set's float planes, its
+        // FP64 density and the shared planes through 32-bit buffer resources
+        const int64_t n = (int64_t)c->nx * c->ny, lim = (int64_t)1 << 32;
+        const int np = std::max(pp32_set_planes(num_q(*c)), 7);
+        if (4 * n * np >= lim || 8 * n >= lim)
+            return fail(MPD_ERR_INVALID, "mesh %dx%d too large for the FP32-bulk pseudopotential kernel (%d float "
+                                         "planes of 4 B per cell must span < 4 GiB: at most %lld cells)",
+                        c->nx, c->ny, np, (long long)((lim - 1) / (4 * np)));
     }
     if (!(c->dx > 0) || !(c->dt > 0) || !(c->rt > 0)) return fail(MPD_ERR_INVALID, "dx, dt, rt must be > 0");
     if (c->phi_l == c->phi_v || !(c->width > 0)) return fail(MPD_ERR_INVALID, "phi_l == phi_v or width <= 0");
@@ -419,6 +430,8 @@ int mpd_set_path(mpd_handle *h, int32_t path)
         return fail(MPD_ERR_INVALID, "the phase path is FP64 only (and FP32 for the pseudopotential model)");
     if (h->cfg.precision == MPD_FP32_BULK && path == MPD_PATH_PHASES && h->cfg.flux == MPD_FLUX_UW)
         return fail(MPD_ERR_INVALID, "the FP32 pseudopotential step with UW faces has one launch per step");
+    if (h->cfg.precision == MPD_FP32_BULK && path == MPD_PATH_PHASES && h->cfg.model == MPD_MODEL_PP_D2V16)
+        return fail(MPD_ERR_INVALID, "the FP32 D2V16 step has one launch per step");
     h->path = path;
     return 0;
 }
@@ -473,6 +486,8 @@ int mpd_set_tile(mpd_handle *h, int32_t tile)
         return fail(MPD_ERR_INVALID, "the FP64 step has 16x15 and 16x16 tiles");
     if (h->cfg.flux == MPD_FLUX_UW && tile != MPD_TILE_AUTO && tile != MPD_TILE_16X15)
         return fail(MPD_ERR_INVALID, "the UW step has 16x15 tiles");
+    if (h->cfg.model == MPD_MODEL_PP_D2V16 && tile != MPD_TILE_AUTO && tile != MPD_TILE_16X15)
+        return fail(MPD_ERR_INVALID, "the D2V16 step has 16x15 tiles");
     h->tile = tile;
     return 0;
 }
@@ -482,7 +497,7 @@ int mpd_get_tile(mpd_handle *h, int32_t *tile)
     if (!h || !tile) return fail(MPD_ERR_INVALID, "null argument");
     if (h->tile) *tile = h->tile;
     else if (h->cfg.flux == MPD_FLUX_UW) *tile = MPD_TILE_16X15;
-    else if (is_pp(h->cfg)) *tile = pp32_tile_auto(h->cfg.nx, h->cfg.ny, h->cus);
+    else if (is_pp(h->cfg)) *tile = pp32_tile_auto(h->cfg.nx, h->cfg.ny, num_q(h->cfg), h->cus);
     else if (h->cfg.precision == MPD_FP64) *tile = fused64_tile_auto(h->cfg.nx, h->cfg.ny, h->cus);
     else *tile = fused32_tile_auto(h->cfg.nx, h->cfg.ny, h->cus, fused32_state_bytes(h->cfg));
     return 0;

@@ -10,38 +10,52 @@ namespace mpd {
 
 struct Consts;
 
-// FP32-bulk pseudopotential D2Q9 planes (pp32.hip)
+// FP32-bulk pseudopotential planes (pp32.hip), velocity set of q populations
+// (D2Q9 or D2V16).  A state set is one float block of (q + 4) planes -- f~[q],
+// U, V, and the face Psi of the step that produced it, signed by scG (x-faces,
+// then y-faces) -- and the conserved density in FP64.  The shared planes hold
+// what a step does not change (tau), the forces of an uploaded state (read by
+// the first step after an upload) or of a download, and the scratch of the
+// one-launch-per-phase kernels (D2Q9).
 struct PP32Planes {
-    int nx, ny;
+    int nx, ny, q;
     int64_t N;
-    double *rho;                                  // [N] conserved density (FP64)
-    float *f, *fbp;                               // [9][N]
-    float *U, *V, *tau, *Fx, *Fy, *psi, *scg;     // [N]
-    float *ffx, *ffy;                             // face force [2N]
-    float *fpsi, *frho, *fU, *fV, *fscg, *fmass;  // face scratch [2N]
-    float *ffl;                                   // face fluxes [9][2N]
+    double *rho;                                  // current set: conserved density (FP64) [N]
+    float *f;                                     // current set: float block [(q + 4) N]
+    float *tau, *Fx, *Fy;                         // [N]: relaxation time, cell force
+    float *ffx, *ffy;                             // [2N]: face force
+    float *fbp, *psi, *scg;                       // per-phase scratch: bar-plus [9][N], cell Psi, scG [N]
+    float *fpsi, *frho, *fU, *fV, *fscg, *fmass;  // per-phase face scratch [2N]
+    float *ffl;                                   // per-phase face values / fluxes [9][2N]
 };
-// the carried planes of one state set: the one-launch step reads the current
-// set (with its halos) and writes the other
+// planes of a set's float block
+__host__ __device__ inline float *pp32_U(float *f, int q, int64_t N) { return f + (int64_t)q * N; }
+__host__ __device__ inline float *pp32_V(float *f, int q, int64_t N) { return f + (int64_t)(q + 1) * N; }
+__host__ __device__ inline float *pp32_fv(float *f, int q, int64_t N) { return f + (int64_t)(q + 2) * N; }
+// the other state set: the one-launch step reads the current set (with its
+// halos) and writes this one
 struct PP32Set {
     double *rho;
-    float *f, *U, *V, *Fx, *Fy, *psi, *scg, *ffx, *ffy;
+    float *f;
 };
 struct PP32State {
-    PP32Planes P;         // the current set, tau, and the scratch of the per-phase kernels
+    PP32Planes P;         // the current set, the shared planes and the per-phase scratch
     PP32Set other;        // the second set
-    bool phases = false;  // step with one launch per phase instead of the one-launch tile kernel
+    bool phases = false;  // step with one launch per phase instead of the one-launch tile kernel (D2Q9)
     int cus = 256;        // compute units of the device (tile shape choice)
     int tile = MPD_TILE_AUTO;  // tile shape of the one-launch step: 16x15, 32x16, or chosen per mesh
 };
 // tile shape (MPD_TILE_16X15 or MPD_TILE_32X16) of the FP32-bulk pseudopotential step
-int pp32_tile_auto(int nx, int ny, int cus);
-int pp32_alloc(PP32State &S, void **mem, int64_t *bytes, const Consts &C, int64_t N);
+int pp32_tile_auto(int nx, int ny, int q, int cus);
+// float planes of one state set (the 32-bit range of the step kernel's buffer resources)
+int pp32_set_planes(int q);
+int pp32_alloc(PP32State &S, void **mem, int64_t *bytes, const Consts &C, int64_t N, int q);
 void pp32_from64(const PP32State &S, const DevState &D, hipStream_t st);
-// force_stale: the stored cell force is the previous step's (every step but the
-// first after a conversion); the step or the conversion recomputes it from Psi
-void pp32_to64(const PP32State &S, const DevState &D, const Consts &C, bool force_stale, hipStream_t st);
-void pp32_step(PP32State &S, const Consts &C, bool force_stale, hipStream_t st);
+// stepped: the state has been stepped since the last conversion from the
+// explicit state, so its forces derive from Psi (the face forces from the
+// stored face Psi); else they are the uploaded ones in the shared planes
+void pp32_to64(const PP32State &S, const DevState &D, const Consts &C, bool stepped, hipStream_t st);
+void pp32_step(PP32State &S, const Consts &C, bool stepped, hipStream_t st);
 
 struct FusedState {
     bool ready = false;
@@ -59,7 +73,7 @@ struct FusedState {
     double p_ref = 0.0;  // FP32: f~ stored minus the rest equilibrium at p_ref
     bool pp = false;     // FP32-bulk pseudopotential: explicit FP32 planes, no derived state
     PP32State pp32;
-    bool pp_force_stale = false;  // the stored cell force predates the last update
+    bool pp_stepped = false;  // stepped since the last conversion: the forces derive from Psi
     int tile = MPD_TILE_16X15;   // Allen-Cahn: tile shape of the step kernel (mpd_tile)
 };
 

@@ -684,15 +684,21 @@ int fused64_tile_auto(int nx, int ny, int cus)
 
 bool fused_supported(const mpd_config &cfg)
 {
-    // FP32-bulk pseudopotential D2Q9: the FP32 kernels of pp32.hip (CD and UW faces)
+    // FP32-bulk pseudopotential: the FP32 kernels of pp32.hip (D2Q9 with CD and
+    // UW faces, D2V16 with CD faces; periodic)
     if (cfg.model == MPD_MODEL_PP_D2Q9) return cfg.precision == MPD_FP32_BULK && cfg.bc_y == MPD_BC_PERIODIC;
+    if (cfg.model == MPD_MODEL_PP_D2V16)
+        return cfg.precision == MPD_FP32_BULK && cfg.bc_y == MPD_BC_PERIODIC && cfg.flux == MPD_FLUX_CD;
     const bool ac = (cfg.model == MPD_MODEL_AC_HCZ || cfg.model == MPD_MODEL_AC_P) && cfg.nx >= 4 && cfg.ny >= 4;
     // UW faces: the FP32-bulk tile kernel (the FP64 fused kernel is CD only)
     if (cfg.flux == MPD_FLUX_UW) return ac && cfg.precision == MPD_FP32_BULK;
     return ac && cfg.flux == MPD_FLUX_CD;
 }
 
-bool fused_needs_priming(const mpd_config &cfg) { return cfg.model != MPD_MODEL_PP_D2Q9; }
+bool fused_needs_priming(const mpd_config &cfg)
+{
+    return cfg.model != MPD_MODEL_PP_D2Q9 && cfg.model != MPD_MODEL_PP_D2V16;
+}
 
 int fused_nplanes(const mpd_config &cfg)
 {
@@ -711,11 +717,11 @@ int fused_alloc(FusedState &F, const DevState &S, const Consts &C, const mpd_con
     fused_free(F);
     F.ncell = S.N;
     F.fp32 = cfg.precision == MPD_FP32_BULK;
-    F.pp = cfg.model == MPD_MODEL_PP_D2Q9;
+    F.pp = cfg.model == MPD_MODEL_PP_D2Q9 || cfg.model == MPD_MODEL_PP_D2V16;
     if (F.pp) {
         void *p = nullptr;
         int64_t b = 0;
-        if (pp32_alloc(F.pp32, &p, &b, C, S.N)) return -1;
+        if (pp32_alloc(F.pp32, &p, &b, C, S.N, cfg.model == MPD_MODEL_PP_D2V16 ? 16 : 9)) return -1;
         F.mem[F.nmem++] = p;
         F.bytes = b;
         F.ready = true;
@@ -769,7 +775,7 @@ void fused_convert(FusedState &F, const DevState &S, const Consts &C, int which,
 {
     if (F.pp) {
         pp32_from64(F.pp32, S, st);
-        F.pp_force_stale = false;
+        F.pp_stepped = false;
         return;
     }
     if (F.fp32) convert_t<float>(F, S, C, which, st);
@@ -780,8 +786,8 @@ void fused_step(FusedState &F, const DevState &S, const Consts &C, const mpd_con
 {
     (void)S; (void)cfg; (void)first;
     if (F.pp) {
-        pp32_step(F.pp32, C, F.pp_force_stale, st);
-        F.pp_force_stale = true;
+        pp32_step(F.pp32, C, F.pp_stepped, st);
+        F.pp_stepped = true;
     }
     else if (F.fp32) step_t<float>(F, C, st);
     else step_t<double>(F, C, st);
@@ -790,10 +796,7 @@ void fused_step(FusedState &F, const DevState &S, const Consts &C, const mpd_con
 int fused_materialize(FusedState &F, const DevState &S, const Consts &C, const mpd_config &cfg, hipStream_t st)
 {
     (void)cfg;
-    if (F.pp) {
-        pp32_to64(F.pp32, S, C, F.pp_force_stale, st);
-        F.pp_force_stale = false;
-    }
+    if (F.pp) pp32_to64(F.pp32, S, C, F.pp_stepped, st);
     else if (F.fp32) materialize_t<float>(F, S, C, st);
     else materialize_t<double>(F, S, C, st);
     return hipGetLastError() == hipSuccess ? 0 : -1;

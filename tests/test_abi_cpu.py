@@ -61,23 +61,28 @@ def test_interface_initialiser_profile():
     assert np.allclose(st.h.sum(axis=0), st.m[abi.M["phi"]], rtol=0, atol=1e-15)
 
 
-def test_fp32_bulk_without_a_kernel_is_refused_at_create():
+def test_fp32_bulk_configurations_pass_the_checks():
     """FP32-bulk exists for the Allen-Cahn models (CD and UW faces; periodic,
-    bounce-back or NEE y) and the pseudopotential D2Q9 model (CD and UW,
-    periodic); D2V16 has no FP32-bulk kernel, and mpd_create refuses it before
-    any device call."""
+    bounce-back or NEE y), the pseudopotential D2Q9 model (CD and UW, periodic)
+    and D2V16 (CD, periodic): every such configuration passes the checks (on a
+    machine without a GPU mpd_create then fails with MPD_ERR_DEVICE, -2), and
+    D2V16 with UW faces is refused with MPD_ERR_INVALID before any device call."""
     L = solver.lib()
-    bad = [
-        (abi.MPD_MODEL_PP_D2V16, dict(), abi.MPD_BC_PERIODIC),
-    ]
-    for model, over, bc in bad:
-        cfg = default_config(model, nx=32, ny=32, **over)
-        cfg.precision = abi.MPD_FP32_BULK
-        cfg.bc_y = bc
+    ok = [(abi.MPD_MODEL_PP_D2V16, abi.MPD_FLUX_CD), (abi.MPD_MODEL_PP_D2Q9, abi.MPD_FLUX_CD),
+          (abi.MPD_MODEL_PP_D2Q9, abi.MPD_FLUX_UW)]
+    for model, flux in ok:
+        cfg = default_config(model, nx=32, ny=32)
+        cfg.precision, cfg.flux = abi.MPD_FP32_BULK, flux
         h = C.c_void_p()
-        assert L.mpd_create(C.byref(cfg), C.byref(h)) == -1, (model, over)  # MPD_ERR_INVALID
-        msg = L.mpd_last_error()
-        assert b"FP32-bulk" in msg, msg
+        rc = L.mpd_create(C.byref(cfg), C.byref(h))
+        if rc == 0:
+            L.mpd_destroy(h)
+        assert rc in (0, -2), (model, flux, rc, L.mpd_last_error())
+    cfg = default_config(abi.MPD_MODEL_PP_D2V16, nx=32, ny=32)
+    cfg.precision, cfg.flux = abi.MPD_FP32_BULK, abi.MPD_FLUX_UW
+    h = C.c_void_p()
+    assert L.mpd_create(C.byref(cfg), C.byref(h)) == -1
+    assert b"D2V16" in L.mpd_last_error()
 
 
 def test_header_compiles_as_c99_and_links(tmp_path):
@@ -112,6 +117,14 @@ def test_fp32_mesh_beyond_buffer_range_refused():
         rc = solver.lib().mpd_create(C.byref(cfg), C.byref(h))
         assert rc == -1 and not h.value
         assert b"too large for the FP32-bulk" in solver.lib().mpd_last_error()
+    # pseudopotential: a set's float block is q + 4 planes (D2Q9 13, D2V16 20)
+    for model, nx, ny in ((abi.MPD_MODEL_PP_D2V16, 8192, 6554), (abi.MPD_MODEL_PP_D2Q9, 9000, 9200)):
+        cfg = default_config(model, nx=nx, ny=ny)
+        cfg.precision = abi.MPD_FP32_BULK
+        h = C.c_void_p()
+        rc = solver.lib().mpd_create(C.byref(cfg), C.byref(h))
+        assert rc == -1 and not h.value
+        assert b"too large for the FP32-bulk pseudopotential" in solver.lib().mpd_last_error()
 
 
 def test_fp32_uw_configurations():
