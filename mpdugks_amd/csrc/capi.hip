@@ -75,6 +75,7 @@ struct mpd_handle {
     bool uploaded = false;
     bool fused_primed = false;   // fused state derived from the explicit macros
     int prime_stage = 0;         // explicit steps taken towards priming
+    double tau_u = 0.0;          // pseudopotential: the relaxation time of every cell, when the state has one
     std::vector<void *> allocs;
     int64_t bytes = 0;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
@@ -582,6 +583,15 @@ int mpd_upload_state(mpd_handle *h, const double *tilde_h, const double *tilde_f
         pp_copies_from_first(h->S, h->C, true, true, h->stream);
         HIPCHK(hipGetLastError());
     }
+    // one relaxation time for every cell (every pseudopotential initialiser
+    // writes pp_tau): the FP32 step then takes it as a constant
+    h->tau_u = 0.0;
+    if (!ac) {
+        const double *tau = macro + (size_t)MPD_M_TAU * N, t0 = tau[0];
+        bool same = t0 > 0.0;
+        for (size_t n = 1; same && n < N; ++n) same = tau[n] == t0;
+        if (same) h->tau_u = t0;
+    }
     // Rho_1k starts at 0 in the reference (Ark/Header/Macroscopic.h:44)
     HIPCHK(hipMemsetAsync(h->S.rho1k, 0, N * sizeof(double), h->stream));
     HIPCHK(hipStreamSynchronize(h->stream));
@@ -709,6 +719,7 @@ int mpd_device_init_pp_noise(mpd_handle *h, double rho0, double amp, uint64_t se
     if (h->batch) return fail(MPD_ERR_STATE, "handle is a member of a batch");
     if (!is_pp(h->cfg)) return fail(MPD_ERR_INVALID, "noise start is for the pseudopotential models");
     devinit_noise(h->S, h->C, rho0, amp, seed, mu0, h->stream);
+    h->tau_u = h->C.pp_tau;  // every cell: pp_tau
     return device_state_ready(h);
 }
 
@@ -720,6 +731,7 @@ int mpd_device_init_pp_drop(mpd_handle *h, double cx, double cy, double radius, 
     if (!is_pp(h->cfg)) return fail(MPD_ERR_INVALID, "SCMP_Drop starts the pseudopotential models");
     if (!(width > 0.0)) return fail(MPD_ERR_INVALID, "width must be positive");
     devinit_pp_drop(h->S, h->C, cx, cy, radius, rho_l, rho_v, width, mu0, h->stream);
+    h->tau_u = h->C.pp_tau;
     return device_state_ready(h);
 }
 
@@ -730,6 +742,7 @@ int mpd_device_init_pp_flat(mpd_handle *h, double y_lo, double y_hi, double slop
     if (h->batch) return fail(MPD_ERR_STATE, "handle is a member of a batch");
     if (!is_pp(h->cfg)) return fail(MPD_ERR_INVALID, "SCMP_FlatInterface starts the pseudopotential models");
     devinit_pp_flat(h->S, h->C, y_lo, y_hi, slope, rho_l, rho_v, mu0, h->stream);
+    h->tau_u = h->C.pp_tau;
     return device_state_ready(h);
 }
 
@@ -803,7 +816,10 @@ int mpd_step(mpd_handle *h, int64_t nsteps)
             if (r) return bail(fail(MPD_ERR_NOMEM, "fused path allocation failed"));
         }
         h->F.pp32.phases = pp32_phases;
-        if (is_pp(h->cfg)) h->F.pp32.tile = h->tile;
+        if (is_pp(h->cfg)) {
+            h->F.pp32.tile = h->tile;
+            h->F.pp32.tau_u = h->tau_u;
+        }
         else if (h->cfg.precision == MPD_FP32_BULK)
             h->F.tile = h->tile ? h->tile
                                 : fused32_tile_auto(h->cfg.nx, h->cfg.ny, h->cus, fused32_state_bytes(h->cfg));

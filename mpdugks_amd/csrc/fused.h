@@ -44,6 +44,7 @@ struct PP32State {
     bool phases = false;  // step with one launch per phase instead of the one-launch tile kernel (D2Q9)
     int cus = 256;        // compute units of the device (tile shape choice)
     int tile = MPD_TILE_AUTO;  // tile shape of the one-launch step: 16x15, 32x16, or chosen per mesh
+    double tau_u = 0.0;   // the relaxation time of every cell when the state has one (else 0)
 };
 // tile shape (MPD_TILE_16X15 or MPD_TILE_32X16) of the FP32-bulk pseudopotential step
 int pp32_tile_auto(int nx, int ny, int q, int cus);

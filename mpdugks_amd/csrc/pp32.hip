@@ -122,13 +122,16 @@ struct KPP {
     float rt4;             // tangential factor per unit e_t: hdt ms / (2 twodx)
     float inv_d2x6;        // 9-point gradient
     float w[3];            // D2Q9 weights: rest, axis, diagonal
+    float wirt[3];         // w / RT
+    float bbp_u, cbp_u;    // bar-plus factors of a uniform relaxation time (tau_u > 0)
+    float tau_u;           // the uniform cell relaxation time, or 0
     float bh, ch;          // face BGK: f + bh (f_eq - f) + ch f_So
     float dtv, flux;       // dt / V; dx ms (micro-flux per unit e_n)
     float uw_n, uw_t, uw_0;  // UW face value, 6-point gradient sums (x 12 dx)
     double pp_k, t_cs, drt;  // FP64 EOS: K, T_CS, RT
 };
 
-KPP make_kpp(const Consts &C, int q)
+KPP make_kpp(const Consts &C, int q, double tau_u)
 {
     KPP k;
     const double ms = q == 16 ? d2v16::C : C.xi[0][1];
@@ -151,6 +154,11 @@ KPP make_kpp(const Consts &C, int q)
     k.w[0] = (float)C.w[0];
     k.w[1] = (float)C.w[1];
     k.w[2] = (float)C.w[2];
+    for (int c = 0; c < 3; ++c) k.wirt[c] = (float)(C.w[c] / C.rt);
+    // Cell_2D.cpp:120-125: bBP = 1 - (2 tau - hdt)/(2 tau + dt) = (dt + hdt)/(2 tau + dt), cBP = tau bBP
+    k.tau_u = tau_u > 0 ? (float)tau_u : 0.f;
+    k.bbp_u = tau_u > 0 ? (float)((C.dt + C.hdt) / (2.0 * tau_u + C.dt)) : 0.f;
+    k.cbp_u = tau_u > 0 ? (float)(tau_u * (C.dt + C.hdt) / (2.0 * tau_u + C.dt)) : 0.f;
     k.bh = (float)C.bh_pp;
     k.ch = (float)C.ch_pp;
     k.dtv = (float)C.dtv;
@@ -172,13 +180,14 @@ __host__ __device__ constexpr int wcls(int q) { return q == 0 ? 0 : ((q & 1) ? 1
 // with the Carnahan-Starling pressure, Ark/Lib/nonidealEoS.h:59-73, a = 1,
 // b = 4): d = p - rho RT in FP64, Psi = sqrt(2 |d|) (dividing by scG = +-1),
 // scG = sign(d); NaN where d = 0 (the reference's 0/0).  The division by
-// (1 - rho)^3 is a hardware reciprocal refined by two Newton steps.
+// (1 - rho)^3 is the hardware reciprocal (relative error below 2^-22) refined by
+// one Newton step (below 2^-44): d keeps ~1e-13 of p, far below the FP32
+// rounding of Psi however close p and rho RT are.
 __device__ __forceinline__ float pp_vsig(const KPP &k, double rho)
 {
     const double om = 1.0 - rho;
     const double den = om * om * om;
     double r = __builtin_amdgcn_rcp(den);
-    r = fma(r, fma(-den, r, 1.0), r);
     r = fma(r, fma(-den, r, 1.0), r);
     const double num = fma(rho, fma(rho, 1.0 - rho, 1.0), 1.0);  // 1 + rho + rho^2 - rho^3
     const double p = k.pp_k * fma(rho * k.t_cs * num, r, -(rho * rho));
@@ -202,52 +211,79 @@ __device__ __forceinline__ void pp_force(const KPP &k, float v, float e, float n
     Fy = gy * -v;
 }
 
+// f(std::integral_constant<int, i>) for i = 0 .. N-1: loop indices that are
+// compile-time constants inside the body
+template <int N, class F>
+__device__ __forceinline__ void unroll(F &&f)
+{
+    if constexpr (N > 0) {
+        unroll<N - 1>(f);
+        f(std::integral_constant<int, N - 1>{});
+    }
+}
+
 // ---------------- equilibrium and source ----------------
-// f_eq and the He-Shan-Doolen source f_So = F.(xi - u) f_eq / (rho RT) of every
-// population (Update_phi_Eq / Update_DVDF_Source: D2Q9PP.cpp:51-62,137-145;
-// D2V16.cpp:157-166 with Macro16 at the reference temperature, :125-145).
-// MASK(q) selects the populations evaluated.
+// f_eq and the He-Shan-Doolen source f_So = F.(xi - u) f_eq / (rho RT) of the
+// populations of the pairs PMASK(p) selects (and the rest population, REST)
+// (Update_phi_Eq / Update_DVDF_Source: D2Q9PP.cpp:51-62,137-145; D2V16.cpp:157-166
+// with Macro16 at the reference temperature, :125-145).  Every equilibrium is
+// rho times a function of u (p = rho RT), so f_So = F.(xi - u) (f_eq / rho) / RT
+// takes no division by rho.  D2Q9 pairs (q, q') share
+// B = 1 + u1^2/2 - |u|^2 Lambda0: f_eq = w rho (B +- u1), u1 = xi_q.u / RT.
 struct Site {
-    float rho, U, V, Fx, Fy, irr;  // irr = 1 / (rho RT)
+    float rho, U, V, Fx, Fy;
 };
-template <class L, class M>
-__device__ __forceinline__ void feq_src(const KPP &k, const Site &s, float *feq, float *fso, M mask)
+template <int A, int B>
+__device__ __forceinline__ float edot(float x, float y)
+{
+    if (A == 0 && B == 0) return 0.f;
+    if (A == 0) return B > 0 ? y : -y;
+    if (B == 0) return A > 0 ? x : -x;
+    if (A > 0) return B > 0 ? x + y : x - y;
+    return B > 0 ? y - x : -(x + y);
+}
+template <class L, bool REST, class PM>
+__device__ __forceinline__ void feq_src(const KPP &k, const Site &s, float *feq, float *fso, PM pmask)
 {
     const float uF = fmaf(s.U, s.Fx, s.V * s.Fy);
     if constexpr (L::Q == 9) {
-        const float uul = fmaf(s.U, s.U, s.V * s.V) * k.lambda0;
-#pragma unroll
-        for (int q = 0; q < 9; ++q) {
-            if (!mask(q)) continue;
-            const int ex = L::ex(q), ey = L::ey(q);
-            const float eu = ex == 0 ? (ey == 0 ? 0.f : (ey > 0 ? s.V : -s.V))
-                                     : (ey == 0 ? (ex > 0 ? s.U : -s.U)
-                                                : (ex > 0 ? (ey > 0 ? s.U + s.V : s.U - s.V)
-                                                          : (ey > 0 ? s.V - s.U : -(s.U + s.V))));
-            const float eF = ex == 0 ? (ey == 0 ? 0.f : (ey > 0 ? s.Fy : -s.Fy))
-                                     : (ey == 0 ? (ex > 0 ? s.Fx : -s.Fx)
-                                                : (ex > 0 ? (ey > 0 ? s.Fx + s.Fy : s.Fx - s.Fy)
-                                                          : (ey > 0 ? s.Fy - s.Fx : -(s.Fx + s.Fy))));
-            const float u1 = k.ms_rt * eu;
-            const float wr = k.w[wcls(q)] * s.rho;
-            feq[q] = wr * (fmaf(0.5f * u1, u1, 1.f + u1) - uul);
-            fso[q] = fmaf(k.ms, eF, -uF) * feq[q] * s.irr;
+        const float c0 = fmaf(-k.lambda0, fmaf(s.U, s.U, s.V * s.V), 1.f);
+        if (REST) {
+            feq[0] = (k.w[0] * s.rho) * c0;
+            fso[0] = (-uF * k.wirt[0]) * c0;
         }
+        unroll<L::NP>([&](auto P) {
+            constexpr int p = decltype(P)::value, q = L::first(p), qp = L::opp(q);
+            constexpr int EX = L::ex(q), EY = L::ey(q), c = wcls(q);
+            if (!pmask(p)) return;
+            const float u1 = k.ms_rt * edot<EX, EY>(s.U, s.V);
+            const float eF = edot<EX, EY>(s.Fx, s.Fy);
+            const float B = fmaf(0.5f * u1, u1, c0);
+            const float bp = B + u1, bm = B - u1, wr = k.w[c] * s.rho;
+            feq[q] = wr * bp;
+            feq[qp] = wr * bm;
+            fso[q] = (fmaf(k.ms, eF, -uF) * k.wirt[c]) * bp;
+            fso[qp] = (fmaf(-k.ms, eF, -uF) * k.wirt[c]) * bm;
+        });
     } else {
         float Mo[16];
         const float rt = k.rt;
-        d2v16::macro16<float>(Mo, s.rho, s.U, s.V, rt, s.rho * rt);
-#pragma unroll
-        for (int q = 0; q < 16; ++q) {
-            if (!mask(q)) continue;
-            float a = 0.f;
-#pragma unroll
-            for (int m = 0; m < 16; ++m)
-                if (kA16.a[q][m] != 0.f) a = fmaf(kA16.a[q][m], Mo[m], a);
-            feq[q] = a;
-            const float eF = fmaf((float)L::ex(q), s.Fx, (float)L::ey(q) * s.Fy);
-            fso[q] = fmaf(k.ms, eF, -uF) * a * s.irr;
-        }
+        d2v16::macro16<float>(Mo, 1.f, s.U, s.V, rt, rt);
+        unroll<L::NP>([&](auto P) {
+            constexpr int p = decltype(P)::value;
+            if (!pmask(p)) return;
+            unroll<2>([&](auto S) {
+                constexpr int q = decltype(S)::value == 0 ? L::first(p) : L::opp(L::first(p));
+                float a = 0.f;
+                unroll<16>([&](auto MM) {
+                    constexpr int m = decltype(MM)::value;
+                    if constexpr (kA16.a[q][m] != 0.f) a = fmaf(kA16.a[q][m], Mo[m], a);
+                });
+                const float eF = fmaf((float)L::ex(q), s.Fx, (float)L::ey(q) * s.Fy);
+                feq[q] = s.rho * a;
+                fso[q] = (fmaf(k.ms, eF, -uF) * k.inv_rt) * a;
+            });
+        });
     }
 }
 
@@ -315,16 +351,6 @@ __device__ __forceinline__ int tile_of_block(int b, int ntiles)
 
 __device__ __forceinline__ float rcp(float x) { return __builtin_amdgcn_rcpf(x); }
 
-// f(std::integral_constant<int, i>) for i = 0 .. N-1: loop indices that are
-// compile-time constants inside the body
-template <int N, class F>
-__device__ __forceinline__ void unroll(F &&f)
-{
-    if constexpr (N > 0) {
-        unroll<N - 1>(f);
-        f(std::integral_constant<int, N - 1>{});
-    }
-}
 
 // ---------------- tile geometry ----------------
 // One TX x TY tile per workgroup (NT threads, one tile cell per thread in the
@@ -336,7 +362,9 @@ __device__ __forceinline__ void unroll(F &&f)
 //    faces whose Psi the force of a tile face reads (and whose previous Psi
 //    the previous force of a tile face reads);
 //  tile faces: x-faces fx 0..TX, fy 0..TY-1; y-faces fx 0..TX-1, fy 0..TY.
-// Face fx of a row is the face west of cell fx (y-faces: south).
+// Face fx of a row is the face west of cell fx (y-faces: south), so face (fx,
+// fy) has the index of cell (fx, fy): the face-Psi arrays use the Psi region's
+// layout, one per direction.
 template <class L, int TX_, int TY_, int H_>
 struct PTile {
     static constexpr int TX = TX_, TY = TY_, H = H_;
@@ -345,13 +373,12 @@ struct PTile {
     static constexpr int RO = 1 + H, PO = RO + 1;
     static constexpr int RX = TX + 2 * RO, RY = TY + 2 * RO, RN = RX * RY;
     static constexpr int QX = TX + 2 * PO, QY = TY + 2 * PO, QN = QX * QY;
-    static constexpr int XPX = TX + 3, XPY = TY + 2, XPN = XPX * XPY;
-    static constexpr int YPX = TX + 2, YPY = TY + 3, YPN = YPX * YPY;
     static constexpr int NXF = (TX + 1) * TY, NYF = TX * (TY + 1), FN = NXF + NYF;
-    static constexpr int NRF = XPN + YPN - FN;                               // Psi-only faces
+    static constexpr int NXR = 2 * (TX + 3) + 2 * TY, NYR = 2 * (TY + 3) + 2 * TX;  // Psi-only faces
     static constexpr int NXA = NXF < NT ? NXF : NT, NYA = NYF < NT ? NYF : NT;  // first two face passes
     static constexpr int EXF = NXF - NXA, EYF = NYF - NYA;                  // tile faces of the third pass
-    static constexpr int NC = EXF + EYF + NRF;                                // third pass
+    // third pass: x-faces on threads [0, EXF + NXR), y-faces from the next wave
+    static constexpr int YC0 = (EXF + NXR + 63) / 64 * 64, NC = YC0 + EYF + NYR;
     static constexpr int NM = nm<L, 0>();                                     // flux populations per face
     // LDS, floats: a union of the stages up to the faces (bar-plus pairs, rest,
     // cell Psi, cell density) and of the flux stage (NM fluxes + mass per tile face)
@@ -359,6 +386,7 @@ struct PTile {
     static constexpr int U2 = (NM + 1) * FN;
     static constexpr int NU = U1 > U2 ? U1 : U2;
     static_assert(NIN <= NT && NC <= NT, "face passes");
+    static_assert(PO >= 2, "face-Psi regions inside the Psi region");
 };
 
 // position (cx, cy) of ring item r of a region of halo HW around the tile
@@ -383,32 +411,30 @@ __device__ __forceinline__ void ring_pos(int r, int &cx, int &cy)
     }
 }
 
-// Psi-only face r (the faces of the face-Psi regions that are not tile faces):
-// x-faces: rows fy = -1 and fy = TY (fx -1..TX+1), then columns fx = -1 and
-// fx = TX+1 (fy 0..TY-1); y-faces: columns fx = -1 and fx = TX (fy -1..TY+1),
-// then rows fy = -1 and fy = TY+1 (fx 0..TX-1)
-template <class G>
-__device__ __forceinline__ void ring_face(int r, int &dir, int &fx, int &fy)
+// Psi-only face r of a direction (the faces of the face-Psi region that are
+// not tile faces).  x-faces: rows fy = -1 and fy = TY (fx -1..TX+1), then
+// columns fx = -1 and fx = TX+1 (fy 0..TY-1); y-faces: columns fx = -1 and
+// fx = TX (fy -1..TY+1), then rows fy = -1 and fy = TY+1 (fx 0..TX-1)
+template <class G, int DIR>
+__device__ __forceinline__ void ring_face(int r, int &fx, int &fy)
 {
-    constexpr int TX = G::TX, TY = G::TY, XPX = G::XPX, YPY = G::YPY;
-    constexpr int NXR = 2 * XPX + 2 * TY;
-    if (r < NXR) {
-        dir = 0;
-        if (r < 2 * XPX) { fy = r < XPX ? -1 : TY; fx = (r < XPX ? r : r - XPX) - 1; }
-        else { r -= 2 * XPX; fx = r < TY ? -1 : TX + 1; fy = r < TY ? r : r - TY; }
+    constexpr int TX = G::TX, TY = G::TY;
+    if (DIR == 0) {
+        constexpr int W = TX + 3;
+        if (r < 2 * W) { fy = r < W ? -1 : TY; fx = (r < W ? r : r - W) - 1; }
+        else { r -= 2 * W; fx = r < TY ? -1 : TX + 1; fy = r < TY ? r : r - TY; }
     } else {
-        r -= NXR;
-        dir = 1;
-        if (r < 2 * YPY) { fx = r < YPY ? -1 : TX; fy = (r < YPY ? r : r - YPY) - 1; }
-        else { r -= 2 * YPY; fy = r < TX ? -1 : TY + 1; fx = r < TX ? r : r - TX; }
+        constexpr int W = TY + 3;
+        if (r < 2 * W) { fx = r < W ? -1 : TX; fy = (r < W ? r : r - W) - 1; }
+        else { r -= 2 * W; fy = r < TX ? -1 : TY + 1; fx = r < TX ? r : r - TX; }
     }
 }
 
-// index of a face in the face-Psi regions (x-faces, then y-faces)
-template <class G, int DIR>
-__device__ __forceinline__ int fpi(int fx, int fy)
+// index of a face (or a cell) in the Psi-region layout
+template <class G>
+__device__ __forceinline__ int qidx(int cx, int cy)
 {
-    return DIR == 0 ? (fy + 1) * G::XPX + (fx + 1) : G::XPN + (fy + 1) * G::YPX + (fx + 1);
+    return (cy + G::PO) * G::QX + (cx + G::PO);
 }
 
 // ---------------- face values ----------------
@@ -437,7 +463,7 @@ __device__ __forceinline__ float2 recon(const KPP &k, const float2 *b, const Fac
     constexpr int AN = EN < 0 ? -EN : EN, AT = ET < 0 ? -ET : ET;
     const float2 bo = b[x.o], bn = b[x.n];
     float2 v;
-    if (EN == 0) {
+    if constexpr (EN == 0) {
         v.x = 0.5f * (bo.x + bn.x);
         v.y = 0.5f * (bo.y + bn.y);
     } else {
@@ -446,7 +472,7 @@ __device__ __forceinline__ float2 recon(const KPP &k, const float2 *b, const Fac
         v.x = EN > 0 ? fmaf(rb, bn.x, ra * bo.x) : fmaf(ra, bn.x, rb * bo.x);
         v.y = EN > 0 ? fmaf(ra, bn.y, rb * bo.y) : fmaf(rb, bn.y, ra * bo.y);
     }
-    if (ET != 0) {
+    if constexpr (ET != 0) {
         const float2 np = b[x.np], nm = b[x.nm], op = b[x.op], om = b[x.om];
         const float c = (ET > 0 ? -k.rt4 : k.rt4) * (float)AT;
         v.x = fmaf(c, (np.x - nm.x) + (op.x - om.x), v.x);
@@ -569,8 +595,8 @@ __device__ __forceinline__ void moment1(const float *f, float &sx, float &sy)
 }
 
 // ---------------- the one-launch tile kernel ----------------
-//  S0: density (FP64) and signed Psi (FP64 EOS) of the Psi region; the previous
-//      face Psi of the face-Psi regions;
+//  S0: density (FP64) and signed Psi (FP64 EOS) of the Psi region, and the
+//      previous face Psi of the face-Psi regions;
 //  S1: cell force and bar-plus of the bar-plus region (tile cells first: each
 //      keeps b + (b - f~)/3 and its force in registers for S4);
 //  S2: face values, density and Psi of every face of the face-Psi regions; the
@@ -579,14 +605,8 @@ __device__ __forceinline__ void moment1(const float *f, float &sx, float &sy)
 //  S3: face force, half-step BGK (Update_phi_h, DUGKS2DSolver.cpp:402-423) and
 //      micro-flux (:424-441) of the tile faces; their Psi is the new face state;
 //  S4: Update_phi_T (:748-792) with the FP64 density, Update_MacroVar.
-// FORCE: the forces derive from Psi (every step but the first after a conversion).
-template <class L, class G, int FLUX, int NFH>
-struct FaceHold {
-    float v[NFH][G::NM];
-    float rho[NFH], U[NFH], V[NFH];
-    int dir[NFH], kf[NFH], ri[NFH];
-    bool on[NFH];
-};
+// FORCE: the forces derive from Psi (every step but the first after a
+// conversion).  TAUU: every cell has the relaxation time k.tau_u.
 
 template <class L, class G, int FLUX, bool FORCE, int DIR>
 __device__ __forceinline__ void s2_face(const KPP &k, const Rs &in, const float2 *sp, const float *sr,
@@ -594,17 +614,19 @@ __device__ __forceinline__ void s2_face(const KPP &k, const Rs &in, const float2
                                         int j0, int nx, int ny, bool small, float *hv, float &hrho, float &hU,
                                         float &hV)
 {
+    constexpr int QX = G::QX;
     const FaceIdx x = face_idx<G, DIR>(fx, fy);
     float fb[L::Q];
     face_values<L, G, DIR, FLUX>(k, sp, sr, x, fb);
     const float rho = face_rho<L>(fb);
-    s_v[fpi<G, DIR>(fx, fy)] = pp_vsig(k, (double)rho);
+    const int c = qidx<G>(fx, fy);
+    s_v[c] = pp_vsig(k, (double)rho);
     if (!tile_face) return;
     float Fx, Fy;
     if (FORCE) {
-        const int c = fpi<G, DIR>(fx, fy), W = DIR == 0 ? G::XPX : G::YPX;
-        pp_force(k, s_vold[c], s_vold[c + 1], s_vold[c + W], s_vold[c - 1], s_vold[c - W], s_vold[c + W + 1],
-                 s_vold[c + W - 1], s_vold[c - W - 1], s_vold[c - W + 1], Fx, Fy);
+        const float *v = s_vold;
+        pp_force(k, v[c], v[c + 1], v[c + QX], v[c - 1], v[c - QX], v[c + QX + 1], v[c + QX - 1], v[c - QX - 1],
+                 v[c - QX + 1], Fx, Fy);
     } else {
         const uint32_t n = (uint32_t)(wrap(j0 + fy, ny, small) * nx + wrap(i0 + fx, nx, small));
         const uint32_t o = DIR == 0 ? 0u : in.pb;
@@ -625,21 +647,20 @@ __device__ __forceinline__ void s3_face(const KPP &k, const Rs &out, const float
                                         int kf, const float *hv, float rho, float U, float V, int i0, int j0, int nx,
                                         int ny)
 {
-    constexpr int FN = G::FN, NM = G::NM;
-    const int c = fpi<G, DIR>(fx, fy), W = DIR == 0 ? G::XPX : G::YPX;
+    constexpr int FN = G::FN, NM = G::NM, QX = G::QX;
+    const int c = qidx<G>(fx, fy);
     const float vc = s_v[c];
     Site s;
-    pp_force(k, vc, s_v[c + 1], s_v[c + W], s_v[c - 1], s_v[c - W], s_v[c + W + 1], s_v[c + W - 1], s_v[c - W - 1],
-             s_v[c - W + 1], s.Fx, s.Fy);
+    pp_force(k, vc, s_v[c + 1], s_v[c + QX], s_v[c - 1], s_v[c - QX], s_v[c + QX + 1], s_v[c + QX - 1],
+             s_v[c - QX - 1], s_v[c - QX + 1], s.Fx, s.Fy);
     // the tile's own faces (not the last column / row, which the next tile owns) are new state
     if ((DIR == 0 ? fx < G::TX : fy < G::TY) && i0 + fx < nx && j0 + fy < ny)
         stf(out.f, (uint32_t)((j0 + fy) * nx + (i0 + fx)), vc, (L::Q + 2 + DIR) * out.pb);
     s.rho = rho;
     s.U = U;
     s.V = V;
-    s.irr = k.inv_rt * rcp(rho);
     float feq[L::Q], fso[L::Q];
-    feq_src<L>(k, s, feq, fso, [](int q) { return en<L, DIR>(q) != 0; });
+    feq_src<L, false>(k, s, feq, fso, [](int p) { return en<L, DIR>(L::first(p)) != 0; });
     float mass = 0.f;
     unroll<NM>([&](auto Z) {
         constexpr int z = decltype(Z)::value, q = mq<L, DIR>(z);
@@ -651,17 +672,17 @@ __device__ __forceinline__ void s3_face(const KPP &k, const Rs &out, const float
     s_fl[NM * FN + kf] = mass;
 }
 
-template <class L, int TX, int TY, int FLUX, bool FORCE>
+template <class L, int TX, int TY, int FLUX, bool FORCE, bool TAUU>
 __global__ void __launch_bounds__(TX * TY > 256 ? 512 : 256)
     k_pp32_tile(PP32Planes P, PP32Set out_set, KPP k, int ntx, int ntiles)
 {
     using G = PTile<L, TX, TY, FLUX == MPD_FLUX_UW ? 2 : 1>;
     constexpr int Q = L::Q, NP = L::NP, NT = G::NT, NIN = G::NIN, RN = G::RN, RX = G::RX, RO = G::RO;
     constexpr int QN = G::QN, QX = G::QX, PO = G::PO, NM = G::NM, FN = G::FN, NXF = G::NXF;
-    constexpr int XPN = G::XPN, YPN = G::YPN, NXA = G::NXA, NYA = G::NYA, EXF = G::EXF, EYF = G::EYF;
+    constexpr int NXA = G::NXA, NYA = G::NYA, EXF = G::EXF, EYF = G::EYF, YC0 = G::YC0;
     __shared__ __align__(16) float s_u[G::NU];
-    __shared__ float s_v[XPN + YPN];
-    __shared__ float s_vold[FORCE ? XPN + YPN : 1];
+    __shared__ float s_v[2 * QN];                 // this step's face Psi, x then y (Psi-region layout)
+    __shared__ float s_vold[FORCE ? 2 * QN : 1];  // the previous step's
     float2 *const s_pair = reinterpret_cast<float2 *>(s_u);
     float *const s_rest = s_u + 2 * NP * RN;
     float *const s_psi = s_rest + (L::REST ? RN : 0);
@@ -682,26 +703,31 @@ __global__ void __launch_bounds__(TX * TY > 256 ? 512 : 256)
     const int ci = tid % TX, cj = tid / TX;
     const bool cell = tid < NIN;
     const bool inside = cell && i0 + ci < nx && j0 + cj < ny;
-    float cf[Q], cU = 0.f, cV = 0.f, ctau = 0.f, cFx = 0.f, cFy = 0.f;
+    float cf[Q], cU = 0.f, cV = 0.f, ctau = k.tau_u, cFx = 0.f, cFy = 0.f;
     double crho = 0.0;
-    uint32_t cn = 0;
     if (cell) {
-        cn = (uint32_t)(wrap(j0 + cj, ny, small) * nx + wrap(i0 + ci, nx, small));
+        const uint32_t cn = (uint32_t)(wrap(j0 + cj, ny, small) * nx + wrap(i0 + ci, nx, small));
 #pragma unroll
         for (int q = 0; q < Q; ++q) cf[q] = ldf(in.f, cn, q * in.pb);
         cU = ldf(in.f, cn, Q * in.pb);
         cV = ldf(in.f, cn, (Q + 1) * in.pb);
-        ctau = ldf(in.sh, cn, SH_TAU * in.pb);
+        if (!TAUU) ctau = ldf(in.sh, cn, SH_TAU * in.pb);
         if (!FORCE) {
             cFx = ldf(in.sh, cn, SH_FX * in.pb);
             cFy = ldf(in.sh, cn, SH_FY * in.pb);
         }
         crho = ldd(in.rho, cn);
+        // S0 for the tile cell: its previous face Psi (both directions)
+        if (FORCE) {
+            const int qi = qidx<G>(ci, cj);
+            s_vold[qi] = ldf(in.f, cn, (Q + 2) * in.pb);
+            s_vold[QN + qi] = ldf(in.f, cn, (Q + 3) * in.pb);
+        }
     }
 
     // ---- S0: density and signed Psi of the Psi region; previous face Psi ----
     if (cell) {
-        const int qi = (cj + PO) * QX + (ci + PO);
+        const int qi = qidx<G>(ci, cj);
         s_rho[qi] = (float)crho;
         if (FORCE) s_psi[qi] = pp_vsig(k, crho);
     }
@@ -710,25 +736,21 @@ __global__ void __launch_bounds__(TX * TY > 256 ? 512 : 256)
         ring_pos<TX, TY, PO>(r, cx, cy);
         const uint32_t n = (uint32_t)(wrap(j0 + cy, ny, small) * nx + wrap(i0 + cx, nx, small));
         const double rho = ldd(in.rho, n);
-        const int qi = (cy + PO) * QX + (cx + PO);
+        const int qi = qidx<G>(cx, cy);
+        if (FORCE) {
+            // x-faces fx -1..TX+1, fy -1..TY; y-faces fx -1..TX, fy -1..TY+1
+            if (cx >= -1 && cx <= TX + 1 && cy >= -1 && cy <= TY) s_vold[qi] = ldf(in.f, n, (Q + 2) * in.pb);
+            if (cx >= -1 && cx <= TX && cy >= -1 && cy <= TY + 1) s_vold[QN + qi] = ldf(in.f, n, (Q + 3) * in.pb);
+        }
         s_rho[qi] = (float)rho;
         if (FORCE) s_psi[qi] = pp_vsig(k, rho);
-    }
-    if (FORCE) {
-        for (int t = tid; t < XPN + YPN; t += NT) {
-            const int dir = t < XPN ? 0 : 1;
-            const int u = dir ? t - XPN : t, W = dir ? G::YPX : G::XPX;
-            const int fx = u % W - 1, fy = u / W - 1;
-            const uint32_t n = (uint32_t)(wrap(j0 + fy, ny, small) * nx + wrap(i0 + fx, nx, small));
-            s_vold[t] = ldf(in.f, n, (Q + 2 + dir) * in.pb);
-        }
     }
     __syncthreads();
 
     // ---- S1: cell force and bar-plus (Update_phi_BP, DUGKS2DSolver.cpp:379-401;
     // factors Cell_2D.cpp:120-125, written f~ + bBP (f_eq - f~) + cBP f_So) ----
     auto barplus = [&](int cx, int cy, const float *f, float U, float V, float tau, float &Fx, float &Fy, float *base) {
-        const int qi = (cy + PO) * QX + (cx + PO);
+        const int qi = qidx<G>(cx, cy);
         if (FORCE)
             pp_force(k, s_psi[qi], s_psi[qi + 1], s_psi[qi + QX], s_psi[qi - 1], s_psi[qi - QX], s_psi[qi + QX + 1],
                      s_psi[qi + QX - 1], s_psi[qi - QX - 1], s_psi[qi - QX + 1], Fx, Fy);
@@ -738,10 +760,16 @@ __global__ void __launch_bounds__(TX * TY > 256 ? 512 : 256)
         s.V = V;
         s.Fx = Fx;
         s.Fy = Fy;
-        s.irr = k.inv_rt * rcp(s.rho);
         float feq[Q], fso[Q];
-        feq_src<L>(k, s, feq, fso, [](int) { return true; });
-        const float bbp = k.bdt * rcp(fmaf(2.f, tau, k.dt)), cbp = tau * bbp;
+        feq_src<L, L::REST>(k, s, feq, fso, [](int) { return true; });
+        float bbp, cbp;
+        if (TAUU) {
+            bbp = k.bbp_u;
+            cbp = k.cbp_u;
+        } else {
+            bbp = k.bdt * rcp(fmaf(2.f, tau, k.dt));
+            cbp = tau * bbp;
+        }
         const int ri = (cy + RO) * RX + (cx + RO);
         float b[Q];
 #pragma unroll
@@ -761,11 +789,11 @@ __global__ void __launch_bounds__(TX * TY > 256 ? 512 : 256)
         int cx, cy;
         ring_pos<TX, TY, RO>(r, cx, cy);
         const uint32_t n = (uint32_t)(wrap(j0 + cy, ny, small) * nx + wrap(i0 + cx, nx, small));
-        float f[Q], Fx = 0.f, Fy = 0.f;
+        float f[Q], Fx = 0.f, Fy = 0.f, tau = k.tau_u;
 #pragma unroll
         for (int q = 0; q < Q; ++q) f[q] = ldf(in.f, n, q * in.pb);
         const float U = ldf(in.f, n, Q * in.pb), V = ldf(in.f, n, (Q + 1) * in.pb);
-        const float tau = ldf(in.sh, n, SH_TAU * in.pb);
+        if (!TAUU) tau = ldf(in.sh, n, SH_TAU * in.pb);
         if (!FORCE) {
             Fx = ldf(in.sh, n, SH_FX * in.pb);
             Fy = ldf(in.sh, n, SH_FY * in.pb);
@@ -775,8 +803,10 @@ __global__ void __launch_bounds__(TX * TY > 256 ? 512 : 256)
     __syncthreads();
 
     // ---- S2: faces ----
-    // pass A: tile x-face tid (fx = tid % TX, fy = tid / TX; then the last column)
-    // pass B: tile y-face tid; pass C: the remaining tile faces, then the Psi-only faces
+    // pass A: tile x-face tid (fx = tid % TX, fy = tid / TX; then the last column);
+    // pass B: tile y-face tid; pass C: the remaining tile x-faces and the Psi-only
+    // x-faces from thread 0, the remaining tile y-faces and the Psi-only y-faces
+    // from the first thread of a later wave (each wave one direction)
     float av[NM], bv[NM], cv[NM];
     float arho = 1.f, aU = 0.f, aV = 0.f, brho = 1.f, bU = 0.f, bV = 0.f, crh = 1.f, cU2 = 0.f, cV2 = 0.f;
     const bool pa = tid < NXA, pb = tid < NYA;
@@ -791,33 +821,33 @@ __global__ void __launch_bounds__(TX * TY > 256 ? 512 : 256)
     if (pb) {
         bfx = tid % TX;
         bfy = tid / TX;
-        s2_face<L, G, FLUX, FORCE, 1>(k, in, s_pair, s_rest, s_vold, s_v, bfx, bfy, true, i0, j0, nx, ny, small, bv,
-                                      brho, bU, bV);
+        s2_face<L, G, FLUX, FORCE, 1>(k, in, s_pair, s_rest, s_vold + QN, s_v + QN, bfx, bfy, true, i0, j0, nx, ny,
+                                      small, bv, brho, bU, bV);
     }
-    int cdir = -1, cfx = 0, cfy = 0;
-    bool ctile = false;
-    if (tid < G::NC) {
+    int cfx = 0, cfy = 0;
+    const bool cx_ = tid < EXF + G::NXR, cy_ = tid >= YC0 && tid < G::NC;
+    const bool ctile = cx_ ? tid < EXF : (cy_ && tid - YC0 < EYF);
+    if (cx_) {
         if (tid < EXF) {
-            const int t = NXA + tid;  // x-faces beyond the first pass
-            cdir = 0;
+            const int t = NXA + tid;  // x-faces beyond the first pass: the last column
             cfx = t < NIN ? t % TX : TX;
             cfy = t < NIN ? t / TX : t - NIN;
-            ctile = true;
-        } else if (tid < EXF + EYF) {
-            const int t = NYA + tid - EXF;
-            cdir = 1;
-            cfx = t % TX;
-            cfy = t / TX;
-            ctile = true;
         } else {
-            ring_face<G>(tid - EXF - EYF, cdir, cfx, cfy);
+            ring_face<G, 0>(tid - EXF, cfx, cfy);
         }
-        if (cdir == 0)
-            s2_face<L, G, FLUX, FORCE, 0>(k, in, s_pair, s_rest, s_vold, s_v, cfx, cfy, ctile, i0, j0, nx, ny, small,
-                                          cv, crh, cU2, cV2);
-        else
-            s2_face<L, G, FLUX, FORCE, 1>(k, in, s_pair, s_rest, s_vold, s_v, cfx, cfy, ctile, i0, j0, nx, ny, small,
-                                          cv, crh, cU2, cV2);
+        s2_face<L, G, FLUX, FORCE, 0>(k, in, s_pair, s_rest, s_vold, s_v, cfx, cfy, ctile, i0, j0, nx, ny, small, cv,
+                                      crh, cU2, cV2);
+    } else if (cy_) {
+        const int t = tid - YC0;
+        if (t < EYF) {
+            const int u = NYA + t;  // y-faces beyond the first pass: the last row
+            cfx = u % TX;
+            cfy = u / TX;
+        } else {
+            ring_face<G, 1>(t - EYF, cfx, cfy);
+        }
+        s2_face<L, G, FLUX, FORCE, 1>(k, in, s_pair, s_rest, s_vold + QN, s_v + QN, cfx, cfy, ctile, i0, j0, nx, ny,
+                                      small, cv, crh, cU2, cV2);
     }
     __syncthreads();
 
@@ -826,10 +856,10 @@ __global__ void __launch_bounds__(TX * TY > 256 ? 512 : 256)
     auto xslot = [&](int fx, int fy) { return fx < TX ? fy * TX + fx : NIN + fy; };
     if (pa) s3_face<L, G, 0>(k, out, s_v, s_fl, afx, afy, xslot(afx, afy), av, arho, aU, aV, i0, j0, nx, ny);
     __builtin_amdgcn_sched_barrier(0);
-    if (pb) s3_face<L, G, 1>(k, out, s_v, s_fl, bfx, bfy, NXF + bfy * TX + bfx, bv, brho, bU, bV, i0, j0, nx, ny);
+    if (pb) s3_face<L, G, 1>(k, out, s_v + QN, s_fl, bfx, bfy, NXF + bfy * TX + bfx, bv, brho, bU, bV, i0, j0, nx, ny);
     if (ctile) {
-        if (cdir == 0) s3_face<L, G, 0>(k, out, s_v, s_fl, cfx, cfy, xslot(cfx, cfy), cv, crh, cU2, cV2, i0, j0, nx, ny);
-        else s3_face<L, G, 1>(k, out, s_v, s_fl, cfx, cfy, NXF + cfy * TX + cfx, cv, crh, cU2, cV2, i0, j0, nx, ny);
+        if (cx_) s3_face<L, G, 0>(k, out, s_v, s_fl, cfx, cfy, xslot(cfx, cfy), cv, crh, cU2, cV2, i0, j0, nx, ny);
+        else s3_face<L, G, 1>(k, out, s_v + QN, s_fl, cfx, cfy, NXF + cfy * TX + cfx, cv, crh, cU2, cV2, i0, j0, nx, ny);
     }
     __syncthreads();
 
@@ -920,9 +950,8 @@ __global__ void __launch_bounds__(256) k_pp32_phase1(PP32Planes P, KPP k)
     s.V = pp32_V(P.f, 9, N)[n];
     s.Fx = Fx;
     s.Fy = Fy;
-    s.irr = k.inv_rt * rcp(s.rho);
     float feq[9], fso[9];
-    feq_src<L9>(k, s, feq, fso, [](int) { return true; });
+    feq_src<L9, true>(k, s, feq, fso, [](int) { return true; });
     const float tau = P.tau[n];
     const float bbp = k.bdt * rcp(fmaf(2.f, tau, k.dt)), cbp = tau * bbp;
 #pragma unroll
@@ -992,10 +1021,9 @@ __global__ void __launch_bounds__(256) k_pp32_face_flux(PP32Planes P, KPP k)
     s.rho = P.frho[t];
     s.U = P.fU[t];
     s.V = P.fV[t];
-    s.irr = k.inv_rt * rcp(s.rho);
     float feq[9], fso[9];
-    if (dir == 0) feq_src<L9>(k, s, feq, fso, [](int q) { return ex9(q) != 0; });
-    else feq_src<L9>(k, s, feq, fso, [](int q) { return ey9(q) != 0; });
+    if (dir == 0) feq_src<L9, false>(k, s, feq, fso, [](int p) { return ex9(L9::first(p)) != 0; });
+    else feq_src<L9, false>(k, s, feq, fso, [](int p) { return ey9(L9::first(p)) != 0; });
     float mass = 0.f;
 #pragma unroll
     for (int q = 0; q < 9; ++q) {
@@ -1182,18 +1210,20 @@ void pp32_from64(const PP32State &St, const DevState &S, hipStream_t st)
 
 void pp32_to64(const PP32State &St, const DevState &S, const Consts &C, bool stepped, hipStream_t st)
 {
-    const KPP k = make_kpp(C, St.P.q);
+    const KPP k = make_kpp(C, St.P.q, 0.0);
     if (stepped) hipLaunchKernelGGL(k_pp32_forces, dim3(blocks(St.P.N)), dim3(256), 0, st, St.P, k);
     hipLaunchKernelGGL(k_pp32_to64, dim3(blocks(St.P.N)), dim3(256), 0, st, St.P, S, C, k);
 }
 
 int pp32_tile_auto(int nx, int ny, int q, int cus)
 {
-    // D2V16: 16x15 tiles.  D2Q9: 32x16 tiles where every CU gets at least one,
-    // else 16x15 (at 512x512: 512 tiles of 32x16 against 1,120 of 16x15, a
-    // partly filled second round of the resident workgroups)
+    // D2V16: 16x15 tiles.  D2Q9: 32x16 tiles (per cell about 10% less halo work
+    // than 16x15) unless fewer than half the CUs would get one.  Measured on
+    // MI355X, us per step 16x15 / 32x16: 128^2 9.6 / 10.4, 256^2 9.5 / 9.1,
+    // 512^2 15.9 / 12.3, 768^2 26.5 / 24.2, 1024^2 42.4 / 37.8, 2048^2 176 / 152.
     if (q != 9) return MPD_TILE_16X15;
-    return (int64_t)((nx + 31) / 32) * ((ny + 15) / 16) >= (cus > 0 ? cus : 256) ? MPD_TILE_32X16 : MPD_TILE_16X15;
+    return 2 * (int64_t)((nx + 31) / 32) * ((ny + 15) / 16) >= (cus > 0 ? cus : 256) ? MPD_TILE_32X16
+                                                                                        : MPD_TILE_16X15;
 }
 
 template <class L, int TX, int TY, int FLUX>
@@ -1201,19 +1231,19 @@ void pp32_tile_launch(PP32State &St, const KPP &k, bool stepped, hipStream_t st)
 {
     const PP32Planes &P = St.P;
     const int ntx = (P.nx + TX - 1) / TX, nty = (P.ny + TY - 1) / TY, ntiles = ntx * nty;
-    const unsigned nblocks = (unsigned)(((ntiles + 7) / 8) * 8);
-    dim3 b(PTile<L, TX, TY, 1>::NT);
-    if (stepped)
-        hipLaunchKernelGGL((k_pp32_tile<L, TX, TY, FLUX, true>), dim3(nblocks), b, 0, st, P, St.other, k, ntx, ntiles);
-    else
-        hipLaunchKernelGGL((k_pp32_tile<L, TX, TY, FLUX, false>), dim3(nblocks), b, 0, st, P, St.other, k, ntx, ntiles);
+    const dim3 g((unsigned)(((ntiles + 7) / 8) * 8)), b(PTile<L, TX, TY, 1>::NT);
+    const bool u = k.tau_u > 0.f;
+    if (stepped && u) hipLaunchKernelGGL((k_pp32_tile<L, TX, TY, FLUX, true, true>), g, b, 0, st, P, St.other, k, ntx, ntiles);
+    else if (stepped) hipLaunchKernelGGL((k_pp32_tile<L, TX, TY, FLUX, true, false>), g, b, 0, st, P, St.other, k, ntx, ntiles);
+    else if (u) hipLaunchKernelGGL((k_pp32_tile<L, TX, TY, FLUX, false, true>), g, b, 0, st, P, St.other, k, ntx, ntiles);
+    else hipLaunchKernelGGL((k_pp32_tile<L, TX, TY, FLUX, false, false>), g, b, 0, st, P, St.other, k, ntx, ntiles);
 }
 
 void pp32_step(PP32State &St, const Consts &C, bool stepped, hipStream_t st)
 {
     PP32Planes &P = St.P;
     const size_t N = P.N;
-    const KPP k = make_kpp(C, P.q);
+    const KPP k = make_kpp(C, P.q, St.tau_u);
     if (St.phases && P.q == 9) {
         if (stepped) {
             hipLaunchKernelGGL(k_pp32_psi, dim3(blocks(N)), dim3(256), 0, st, P, k);
