@@ -816,6 +816,7 @@ int mpd_step(mpd_handle *h, int64_t nsteps)
             if (r) return bail(fail(MPD_ERR_NOMEM, "fused path allocation failed"));
         }
         h->F.pp32.phases = pp32_phases;
+        h->F.cus = h->cus;
         if (is_pp(h->cfg)) {
             h->F.pp32.tile = h->tile;
             h->F.pp32.tau_u = h->tau_u;

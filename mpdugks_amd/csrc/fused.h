@@ -76,6 +76,7 @@ struct FusedState {
     PP32State pp32;
     bool pp_stepped = false;  // stepped since the last conversion: the forces derive from Psi
     int tile = MPD_TILE_16X15;   // Allen-Cahn: tile shape of the step kernel (mpd_tile)
+    int cus = 256;               // compute units of the device
 };
 
 // FP32 planes of the derived state (see fused_kernels.hip); phd is the
@@ -84,9 +85,10 @@ struct FusedPlanes32 {
     float *h, *f, *phi, *phl, *ppu, *ppv, *gl;
     double *phd;
 };
-// one FP32-bulk step (fused_fp32.hip) with tiles of shape `tile` (mpd_tile)
+// one FP32-bulk step (fused_fp32.hip) with tiles of shape `tile` (mpd_tile) on
+// a GPU with `cus` compute units
 void fused32_step(const FusedPlanes32 &in, const FusedPlanes32 &out, const Consts &C, int64_t N, int tile,
-                  hipStream_t st);
+                  int cus, hipStream_t st);
 // tile shape (mpd_tile) of the FP32-bulk step for an nx x ny mesh on a GPU with
 // `cus` compute units, whose two state sets take `state_bytes` of device memory
 int fused32_tile_auto(int nx, int ny, int cus, int64_t state_bytes);

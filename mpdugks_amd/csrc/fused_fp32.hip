@@ -141,18 +141,29 @@ __device__ __forceinline__ float ld(rsrc_t r, uint32_t n, uint32_t plane_bytes =
 {
     return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, n * 4u, plane_bytes, 0));
 }
+// Stores take a cache policy SC: 0, or 16 (the SC1 bit, device scope), which
+// has the L2 write the state through as it is stored instead of in the
+// write-back at the end of the kernel that makes it visible to the next
+// step's other XCDs.  That pays where every tile finishes at about the same
+// time, a launch of one round of 32x16 tiles (RT 256x1024: 15.85 against
+// 16.39 us per step, 512x512: 15.5 against 15.9-16.3, same state bit for bit);
+// on launches of several rounds it measured 1-5% slower, and on the 7.8 M-cell
+// ensemble batch 18% slower, so they keep 0.  (The streaming NT bit: 17.5 us.)
+constexpr int ST_WT = 16;
+template <int SC = 0>
 __device__ __forceinline__ void st(rsrc_t r, uint32_t n, float v, uint32_t plane_bytes = 0)
 {
-    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(uint32_t, v), r, n * 4u, plane_bytes, 0);
+    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(uint32_t, v), r, n * 4u, plane_bytes, SC);
 }
 __device__ __forceinline__ double ld64(rsrc_t r, uint32_t n)
 {
     return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, n * 8u, 0, 0));
 }
+template <int SC = 0>
 __device__ __forceinline__ void st64(rsrc_t r, uint32_t n, double v)
 {
     using u2 = unsigned int __attribute__((ext_vector_type(2)));
-    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u2, v), r, n * 8u, 0, 0);
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u2, v), r, n * 8u, 0, SC);
 }
 
 // The planes of one state set are consecutive in one allocation (h~ 0-8,
@@ -327,7 +338,7 @@ __device__ __forceinline__ void cell_load(const Rs &in, int nx, int ny, bool sma
     }
 }
 
-template <int MODEL, int WALLS, class G>
+template <int MODEL, int WALLS, class G, int SC>
 __device__ __forceinline__ void cell_bp(const K32 &k, const Rs &out, int nx, int ny, int rx, int ry, const CellIn &x,
                                         float2 *S, const float *s_phi, const float *s_phl, float *base, bool interior,
                                         float2 *s_nee)
@@ -487,13 +498,13 @@ __device__ __forceinline__ void cell_bp(const K32 &k, const Rs &out, int nx, int
         for (int z = 0; z < 3; ++z) s_nee[(top ? 3 * TX : 0) + 3 * (rx - G::H) + z] = make_float2(nh[z], nf[z]);
     }
     if (interior) {
-        if (HCZ || WALLS) st(out.a, n, phi, P_PHL * out.pb);
+        if (HCZ || WALLS) st<SC>(out.a, n, phi, P_PHL * out.pb);
         if (HCZ) {
-            st(out.a, n, npu, P_PPU * out.pb);
-            st(out.a, n, npv, P_PPV * out.pb);
+            st<SC>(out.a, n, npu, P_PPU * out.pb);
+            st<SC>(out.a, n, npv, P_PPV * out.pb);
             if (WALLS && (gj == 0 || gj == ny - 1)) {
                 const int nxt = gj == 0 ? sti.c[2] : sti.c[0];
-                st(out.gl, (uint32_t)((gj == 0 ? 0 : nx) + gi), 2.f * s_phl[sti.c[1]] - s_phl[nxt]);
+                st<SC>(out.gl, (uint32_t)((gj == 0 ? 0 : nx) + gi), 2.f * s_phl[sti.c[1]] - s_phl[nxt]);
             }
         }
     }
@@ -840,7 +851,7 @@ __device__ __forceinline__ void ring_pos(int r, int &rx, int &ry)
     }
 }
 
-template <int MODEL, int WALLS, int FLUX, class G>
+template <int MODEL, int WALLS, int FLUX, class G, int SC = 0>
 __device__ __forceinline__ void tile32(const K32 &k, int nx, int ny, const FusedPlanes32 &pin,
                                        const FusedPlanes32 &pout, int64_t N, int i0, int j0)
 {
@@ -910,7 +921,7 @@ __device__ __forceinline__ void tile32(const K32 &k, int nx, int ny, const Fused
 
     // ---- stage B ----
     float base[17];
-    cell_bp<MODEL, WALLS, G>(k, out, nx, ny, rx1, ry1, x1, s_big, s_phi, s_phl, base, inside, s_nee);
+    cell_bp<MODEL, WALLS, G, SC>(k, out, nx, ny, rx1, ry1, x1, s_big, s_phi, s_phl, base, inside, s_nee);
     // 16x16 and 32x16: the waves with a second pass (ring cells here, faces in
     // stage C) are the critical path of their stage; issue them first
     constexpr bool PRIO = G::EXF > 0 || G::EYF > 0;
@@ -918,7 +929,7 @@ __device__ __forceinline__ void tile32(const K32 &k, int nx, int ny, const Fused
         if (PRIO) __builtin_amdgcn_s_setprio(2);
         if (!PRELOAD_RING) cell_load<MODEL, WALLS, G::H>(in, nx, ny, small, i0, j0, rx2, ry2, x2);
         float dummy[17];
-        cell_bp<MODEL, WALLS, G>(k, out, nx, ny, rx2, ry2, x2, s_big, s_phi, s_phl, dummy, false, s_nee);
+        cell_bp<MODEL, WALLS, G, SC>(k, out, nx, ny, rx2, ry2, x2, s_big, s_phi, s_phl, dummy, false, s_nee);
         if (PRIO) __builtin_amdgcn_s_setprio(0);
     }
     __syncthreads();
@@ -1037,17 +1048,17 @@ __device__ __forceinline__ void tile32(const K32 &k, int nx, int ny, const Fused
 #pragma unroll
     for (int q = 1; q < 9; ++q) {
         const float hn = fmaf(k.dtvf, dh[q], base[q - 1]);
-        st(out.a, n, hn, (P_H + q) * out.pb);
+        st<SC>(out.a, n, hn, (P_H + q) * out.pb);
         rest += hn;
     }
-    st(out.a, n, (float)(phd - (double)rest), P_H * out.pb);
-    st64(out.d, n, phd);
+    st<SC>(out.a, n, (float)(phd - (double)rest), P_H * out.pb);
+    st64<SC>(out.d, n, phd);
 #pragma unroll
-    for (int q = 0; q < 9; ++q) st(out.a, n, fmaf(k.dtvf, df[q], base[8 + q]), (P_F + q) * out.pb);
-    st(out.a, n, phi, P_PHI * out.pb);
+    for (int q = 0; q < 9; ++q) st<SC>(out.a, n, fmaf(k.dtvf, df[q], base[8 + q]), (P_F + q) * out.pb);
+    st<SC>(out.a, n, phi, P_PHI * out.pb);
 }
 
-template <int MODEL, int WALLS, int TX, int TY, int FLUX = MPD_FLUX_CD>
+template <int MODEL, int WALLS, int TX, int TY, int FLUX = MPD_FLUX_CD, int SC = 0>
 __global__ void __launch_bounds__(TX * TY > 256 ? 512 : 256) k_fused32(K32 k, int nx, int ny, FusedPlanes32 in,
                                                                         FusedPlanes32 out, int64_t N, int ntx,
                                                                         int ntiles)
@@ -1056,7 +1067,7 @@ __global__ void __launch_bounds__(TX * TY > 256 ? 512 : 256) k_fused32(K32 k, in
     static_assert((TX * TY > 256 ? 512 : 256) == G::NT, "launch bounds");
     const int tile = tile_of_block(blockIdx.x, ntiles);
     if (tile < 0) return;
-    tile32<MODEL, WALLS, FLUX, G>(k, nx, ny, in, out, N, (tile % ntx) * TX, (tile / ntx) * TY);
+    tile32<MODEL, WALLS, FLUX, G, SC>(k, nx, ny, in, out, N, (tile % ntx) * TX, (tile / ntx) * TY);
 }
 
 // Batched step: every workgroup takes one (case, tile) entry of a work list
@@ -1257,7 +1268,7 @@ void batch32_step(const Batch32Plan *P, int parity, hipStream_t st)
     else batch32_launch_shape<MPD_MODEL_AC_P>(P, parity, st);
 }
 
-template <int MODEL, int TX, int TY>
+template <int MODEL, int TX, int TY, int SC = 0>
 void fused32_launch(const K32 &k, const FusedPlanes32 &in, const FusedPlanes32 &out, const Consts &C, int64_t N,
                     hipStream_t st)
 {
@@ -1265,16 +1276,23 @@ void fused32_launch(const K32 &k, const FusedPlanes32 &in, const FusedPlanes32 &
     const int ntiles = ntx * nty;
     dim3 g((unsigned)(((ntiles + 7) / 8) * 8)), b(TileK<TX, TY>::NT);
     const int w = walls_code(C);
-    if (w == 2) hipLaunchKernelGGL((k_fused32<MODEL, 2, TX, TY>), g, b, 0, st, k, C.nx, C.ny, in, out, N, ntx, ntiles);
-    else if (w) hipLaunchKernelGGL((k_fused32<MODEL, 1, TX, TY>), g, b, 0, st, k, C.nx, C.ny, in, out, N, ntx, ntiles);
-    else hipLaunchKernelGGL((k_fused32<MODEL, 0, TX, TY>), g, b, 0, st, k, C.nx, C.ny, in, out, N, ntx, ntiles);
+    if (w == 2)
+        hipLaunchKernelGGL((k_fused32<MODEL, 2, TX, TY, MPD_FLUX_CD, SC>), g, b, 0, st, k, C.nx, C.ny, in, out, N, ntx,
+                           ntiles);
+    else if (w)
+        hipLaunchKernelGGL((k_fused32<MODEL, 1, TX, TY, MPD_FLUX_CD, SC>), g, b, 0, st, k, C.nx, C.ny, in, out, N, ntx,
+                           ntiles);
+    else
+        hipLaunchKernelGGL((k_fused32<MODEL, 0, TX, TY, MPD_FLUX_CD, SC>), g, b, 0, st, k, C.nx, C.ny, in, out, N, ntx,
+                           ntiles);
 }
 
 template <int MODEL>
 void fused32_launch_shape(const K32 &k, const FusedPlanes32 &in, const FusedPlanes32 &out, const Consts &C,
-                          int64_t N, int tile, hipStream_t st)
+                          int64_t N, int tile, bool one_round, hipStream_t st)
 {
-    if (tile == MPD_TILE_32X16) fused32_launch<MODEL, 32, 16>(k, in, out, C, N, st);
+    if (tile == MPD_TILE_32X16 && one_round) fused32_launch<MODEL, 32, 16, ST_WT>(k, in, out, C, N, st);
+    else if (tile == MPD_TILE_32X16) fused32_launch<MODEL, 32, 16>(k, in, out, C, N, st);
     else if (tile == MPD_TILE_16X16) fused32_launch<MODEL, 16, 16>(k, in, out, C, N, st);
     else fused32_launch<MODEL, 16, 15>(k, in, out, C, N, st);
 }
@@ -1324,8 +1342,10 @@ void fused32_launch_uw(const K32 &k, const FusedPlanes32 &in, const FusedPlanes3
 }
 
 void fused32_step(const FusedPlanes32 &in, const FusedPlanes32 &out, const Consts &C, int64_t N, int tile,
-                  hipStream_t st)
+                  int cus, hipStream_t st)
 {
+    // one round: every 32x16 tile resident at once (two 512-thread workgroups per CU)
+    const bool one_round = (int64_t)((C.nx + 31) / 32) * ((C.ny + 15) / 16) <= 2 * (int64_t)(cus > 0 ? cus : 256);
     const K32 k = make_k32(C);
     if (C.flux == MPD_FLUX_UW) {
         // UW faces: 16x15 tiles (the 3-cell halo)
@@ -1333,8 +1353,8 @@ void fused32_step(const FusedPlanes32 &in, const FusedPlanes32 &out, const Const
         else fused32_launch_uw<MPD_MODEL_AC_P>(k, in, out, C, N, st);
         return;
     }
-    if (C.model == MPD_MODEL_AC_HCZ) fused32_launch_shape<MPD_MODEL_AC_HCZ>(k, in, out, C, N, tile, st);
-    else fused32_launch_shape<MPD_MODEL_AC_P>(k, in, out, C, N, tile, st);
+    if (C.model == MPD_MODEL_AC_HCZ) fused32_launch_shape<MPD_MODEL_AC_HCZ>(k, in, out, C, N, tile, one_round, st);
+    else fused32_launch_shape<MPD_MODEL_AC_P>(k, in, out, C, N, tile, one_round, st);
 }
 
 }  // namespace mpd

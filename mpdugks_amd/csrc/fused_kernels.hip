@@ -619,7 +619,7 @@ void step_t<float>(FusedState &F, const Consts &C, hipStream_t st)
     Planes<float> a = planes_of<float>(F, F.cur), b = planes_of<float>(F, 1 - F.cur);
     FusedPlanes32 in{a.h, a.f, a.phi, a.phl, a.ppu, a.ppv, a.gl, a.phd},
         out{b.h, b.f, b.phi, b.phl, b.ppu, b.ppv, b.gl, b.phd};
-    fused32_step(in, out, C, F.ncell, F.tile, st);
+    fused32_step(in, out, C, F.ncell, F.tile, F.cus, st);
     F.cur = 1 - F.cur;
 }
 
