@@ -141,6 +141,11 @@ __device__ __forceinline__ float ld(rsrc_t r, uint32_t n, uint32_t plane_bytes =
 {
     return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, n * 4u, plane_bytes, 0));
 }
+// load at a per-lane byte offset (plane included)
+__device__ __forceinline__ float ldb(rsrc_t r, uint32_t bytes)
+{
+    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, bytes, 0u, 0));
+}
 // Stores take a cache policy SC: 0, or 16 (the SC1 bit, device scope), which
 // has the L2 write the state through as it is stored instead of in the
 // write-back at the end of the kernel that makes it visible to the next
@@ -326,15 +331,17 @@ __device__ __forceinline__ void cell_load(const Rs &in, int nx, int ny, bool sma
     x.valid = !(WALLS && (gj < 0 || gj >= ny));  // wall ghost rows: extrapolated afterwards
     x.gj = gj = WALLS ? gj : wrap(gj, ny, small);
     x.n = (uint32_t)(gj * nx + x.gi);
-    if (!x.valid) return;
+    // a ghost-row cell loads its wall neighbour's populations (unused): every
+    // thread issues the same loads, so the waits counted after them stay exact
+    const uint32_t nl = x.valid ? x.n : (uint32_t)((gj < 0 ? 0 : ny - 1) * nx + x.gi);
 #pragma unroll
     for (int q = 0; q < 9; ++q) {
-        x.h[q] = ld(in.a, x.n, (P_H + q) * in.pb);
-        x.f[q] = ld(in.a, x.n, (P_F + q) * in.pb);
+        x.h[q] = ld(in.a, nl, (P_H + q) * in.pb);
+        x.f[q] = ld(in.a, nl, (P_F + q) * in.pb);
     }
     if (MODEL == MPD_MODEL_AC_HCZ) {
-        x.pu = ld(in.a, x.n, P_PPU * in.pb);
-        x.pv = ld(in.a, x.n, P_PPV * in.pb);
+        x.pu = ld(in.a, nl, P_PPU * in.pb);
+        x.pv = ld(in.a, nl, P_PPV * in.pb);
     }
 }
 
@@ -880,14 +887,20 @@ __device__ __forceinline__ void tile32(const K32 &k, int nx, int ny, const Fused
     if (cell) { rx1 = ci + H; ry1 = cj + H; }
     else if constexpr (H == 1) { rx1 = tid - NIN; ry1 = 0; }   // the first NT - NIN < R1X ring cells: bottom row
     else ring_pos<G>(tid - NIN, rx1, ry1);
-    cell_load<MODEL, WALLS, G::H>(in, nx, ny, small, i0, j0, rx1, ry1, x1);
-    if (tid < NR2) {
-        ring_pos<G>(tid + (NT - NIN), rx2, ry2);
-        if (PRELOAD_RING) cell_load<MODEL, WALLS, G::H>(in, nx, ny, small, i0, j0, rx2, ry2, x2);
-    }
-
     // ---- stage A ----
-    for (int q = tid; q < R2N; q += NT) {
+    // phi and phi_lag of up to two cells of the phi region per thread, loaded
+    // before this thread's stage-B populations: loads return in order, so the
+    // phi staging below waits only for these, and the phi-only part of stage B
+    // (stencils, sources) can run while the populations are still arriving
+    static_assert(R2N <= 2 * NT, "two phi-region cells per thread");
+    // The same loads on every path (no branch around a load), so that the wait
+    // before the staging counts only them.  A thread without a second cell
+    // loads its first one again.  Wall ghost rows j = -1 and ny take 2 near -
+    // next of phi_lag and the stored lagged ghost row; rows beyond take 0.
+    float pA[2] = {0.f, 0.f}, plA[2] = {0.f, 0.f};
+#pragma unroll
+    for (int u = 0; u < 2; ++u) {
+        const int q = tid + u * NT < R2N ? tid + u * NT : tid;
         // LDS index q -> phi-region column and row (layout of col2)
         int rx, ry;
         if (q < TX * R2Y) {
@@ -899,23 +912,36 @@ __device__ __forceinline__ void tile32(const K32 &k, int nx, int ny, const Fused
             rx = hc < H2 ? hc : hc + TX;
         }
         const int gi = wrap(i0 - H2 + rx, nx, small);
-        int gj = j0 - H2 + ry;
-        float p = 0.f, pl = 0.f;
-        if (WALLS && (gj < 0 || gj >= ny)) {
-            if (gj == -1 || gj == ny) {
-                const int jn = gj < 0 ? 0 : ny - 1, jnn = gj < 0 ? 1 : ny - 2;
-                p = 2.f * ld(in.a, (uint32_t)(jn * nx + gi), P_PHL * in.pb) -
-                    ld(in.a, (uint32_t)(jnn * nx + gi), P_PHL * in.pb);
-                if (HCZ) pl = ld(in.gl, (uint32_t)((gj < 0 ? 0 : nx) + gi));
-            }
+        const int gj = j0 - H2 + ry;
+        if (WALLS) {
+            const bool ghost = gj < 0 || gj >= ny, edge = gj == -1 || gj == ny;
+            const int jn = gj < 0 ? 0 : ny - 1, jnn = gj < 0 ? 1 : ny - 2;
+            const int j1 = ghost ? jn : gj, j2 = ghost ? jnn : gj;
+            const uint32_t o1 = (uint32_t)(j1 * nx + gi) * 4u + (ghost ? P_PHL : P_PHI) * in.pb;
+            const uint32_t o2 = (uint32_t)(j2 * nx + gi) * 4u + P_PHL * in.pb;
+            const float a1 = ldb(in.a, o1), a2 = ldb(in.a, o2);
+            const float a3 = HCZ ? ld(in.gl, (uint32_t)((gj < 0 ? 0 : nx) + gi)) : 0.f;
+            pA[u] = ghost ? (edge ? 2.f * a1 - a2 : 0.f) : a1;
+            if (HCZ) plA[u] = ghost ? (edge ? a3 : 0.f) : a2;
         } else {
-            gj = WALLS ? gj : wrap(gj, ny, small);
-            const uint32_t n = (uint32_t)(gj * nx + gi);
-            p = ld(in.a, n, P_PHI * in.pb);
-            if (HCZ) pl = ld(in.a, n, P_PHL * in.pb);
+            const uint32_t n = (uint32_t)(wrap(gj, ny, small) * nx + gi);
+            pA[u] = ld(in.a, n, P_PHI * in.pb);
+            if (HCZ) plA[u] = ld(in.a, n, P_PHL * in.pb);
         }
-        s_phi[q] = p;
-        if (HCZ) s_phl[q] = pl;
+    }
+    __builtin_amdgcn_sched_barrier(0);
+    cell_load<MODEL, WALLS, G::H>(in, nx, ny, small, i0, j0, rx1, ry1, x1);
+    if (tid < NR2) {
+        ring_pos<G>(tid + (NT - NIN), rx2, ry2);
+        if (PRELOAD_RING) cell_load<MODEL, WALLS, G::H>(in, nx, ny, small, i0, j0, rx2, ry2, x2);
+    }
+    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+    for (int u = 0; u < 2; ++u) {
+        const int q = tid + u * NT;
+        if (q >= R2N) break;
+        s_phi[q] = pA[u];
+        if (HCZ) s_phl[q] = plA[u];
     }
     __syncthreads();
 
