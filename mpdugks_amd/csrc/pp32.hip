@@ -699,51 +699,74 @@ __global__ void __launch_bounds__(TX * TY > 256 ? 512 : 256)
     const Rs in = make_rs(P.f, P.rho, P.tau, N, Q);
     const Rs out = make_rs(out_set.f, out_set.rho, P.tau, N, Q);
 
-    // ---- the tile cell of this thread: its S1 inputs first, so their latency overlaps S0 ----
+    // ---- loads: S0's (the densities and previous face Psi of this thread's
+    // Psi-region cells) first, then the S1 inputs of its tile cell.  Loads
+    // return in order, so S0's EOS waits only for its own loads while the
+    // population loads are in flight; every thread issues the same loads (a
+    // thread beyond the tile, or without a second ring item, loads a cell it
+    // does not use), so the waits count exactly.
+    constexpr int NR0 = QN - NIN, NRI = (NR0 + NT - 1) / NT;  // ring items per thread
     const int ci = tid % TX, cj = tid / TX;
     const bool cell = tid < NIN;
     const bool inside = cell && i0 + ci < nx && j0 + cj < ny;
-    float cf[Q], cU = 0.f, cV = 0.f, ctau = k.tau_u, cFx = 0.f, cFy = 0.f;
-    double crho = 0.0;
-    if (cell) {
-        const uint32_t cn = (uint32_t)(wrap(j0 + cj, ny, small) * nx + wrap(i0 + ci, nx, small));
+    const uint32_t cn = (uint32_t)(wrap(j0 + cj, ny, small) * nx + wrap(i0 + ci, nx, small));
+    const double crho = ldd(in.rho, cn);
+    float cvx = 0.f, cvy = 0.f;
+    if (FORCE) {
+        cvx = ldf(in.f, cn, (Q + 2) * in.pb);
+        cvy = ldf(in.f, cn, (Q + 3) * in.pb);
+    }
+    // ring items: the threads beyond the tile's cells first, then the others
+    double rrho[NRI];
+    float rvx[NRI], rvy[NRI];
+    int rcx[NRI], rcy[NRI];
+    bool rok[NRI];
 #pragma unroll
-        for (int q = 0; q < Q; ++q) cf[q] = ldf(in.f, cn, q * in.pb);
-        cU = ldf(in.f, cn, Q * in.pb);
-        cV = ldf(in.f, cn, (Q + 1) * in.pb);
-        if (!TAUU) ctau = ldf(in.sh, cn, SH_TAU * in.pb);
-        if (!FORCE) {
-            cFx = ldf(in.sh, cn, SH_FX * in.pb);
-            cFy = ldf(in.sh, cn, SH_FY * in.pb);
-        }
-        crho = ldd(in.rho, cn);
-        // S0 for the tile cell: its previous face Psi (both directions)
+    for (int u = 0; u < NRI; ++u) {
+        const int r = tid - NIN + (tid < NIN ? NT : 0) + u * NT;
+        rok[u] = r < NR0;
+        ring_pos<TX, TY, PO>(rok[u] ? r : 0, rcx[u], rcy[u]);
+        const uint32_t n = (uint32_t)(wrap(j0 + rcy[u], ny, small) * nx + wrap(i0 + rcx[u], nx, small));
+        rrho[u] = ldd(in.rho, n);
         if (FORCE) {
-            const int qi = qidx<G>(ci, cj);
-            s_vold[qi] = ldf(in.f, cn, (Q + 2) * in.pb);
-            s_vold[QN + qi] = ldf(in.f, cn, (Q + 3) * in.pb);
+            rvx[u] = ldf(in.f, n, (Q + 2) * in.pb);
+            rvy[u] = ldf(in.f, n, (Q + 3) * in.pb);
         }
     }
+    __builtin_amdgcn_sched_barrier(0);
+    float cf[Q], cU, cV, ctau = k.tau_u, cFx = 0.f, cFy = 0.f;
+#pragma unroll
+    for (int q = 0; q < Q; ++q) cf[q] = ldf(in.f, cn, q * in.pb);
+    cU = ldf(in.f, cn, Q * in.pb);
+    cV = ldf(in.f, cn, (Q + 1) * in.pb);
+    if (!TAUU) ctau = ldf(in.sh, cn, SH_TAU * in.pb);
+    if (!FORCE) {
+        cFx = ldf(in.sh, cn, SH_FX * in.pb);
+        cFy = ldf(in.sh, cn, SH_FY * in.pb);
+    }
+    __builtin_amdgcn_sched_barrier(0);
 
     // ---- S0: density and signed Psi of the Psi region; previous face Psi ----
     if (cell) {
         const int qi = qidx<G>(ci, cj);
         s_rho[qi] = (float)crho;
-        if (FORCE) s_psi[qi] = pp_vsig(k, crho);
-    }
-    for (int r = tid - NIN + (tid < NIN ? NT : 0); r < QN - NIN; r += NT) {
-        int cx, cy;
-        ring_pos<TX, TY, PO>(r, cx, cy);
-        const uint32_t n = (uint32_t)(wrap(j0 + cy, ny, small) * nx + wrap(i0 + cx, nx, small));
-        const double rho = ldd(in.rho, n);
-        const int qi = qidx<G>(cx, cy);
         if (FORCE) {
-            // x-faces fx -1..TX+1, fy -1..TY; y-faces fx -1..TX, fy -1..TY+1
-            if (cx >= -1 && cx <= TX + 1 && cy >= -1 && cy <= TY) s_vold[qi] = ldf(in.f, n, (Q + 2) * in.pb);
-            if (cx >= -1 && cx <= TX && cy >= -1 && cy <= TY + 1) s_vold[QN + qi] = ldf(in.f, n, (Q + 3) * in.pb);
+            s_psi[qi] = pp_vsig(k, crho);
+            s_vold[qi] = cvx;
+            s_vold[QN + qi] = cvy;
         }
-        s_rho[qi] = (float)rho;
-        if (FORCE) s_psi[qi] = pp_vsig(k, rho);
+    }
+#pragma unroll
+    for (int u = 0; u < NRI; ++u) {
+        if (!rok[u]) continue;
+        const int cx = rcx[u], cy = rcy[u], qi = qidx<G>(cx, cy);
+        s_rho[qi] = (float)rrho[u];
+        if (FORCE) {
+            s_psi[qi] = pp_vsig(k, rrho[u]);
+            // x-faces fx -1..TX+1, fy -1..TY; y-faces fx -1..TX, fy -1..TY+1
+            if (cx >= -1 && cx <= TX + 1 && cy >= -1 && cy <= TY) s_vold[qi] = rvx[u];
+            if (cx >= -1 && cx <= TX && cy >= -1 && cy <= TY + 1) s_vold[QN + qi] = rvy[u];
+        }
     }
     __syncthreads();
 
@@ -1219,8 +1242,8 @@ int pp32_tile_auto(int nx, int ny, int q, int cus)
 {
     // D2V16: 16x15 tiles.  D2Q9: 32x16 tiles (per cell about 10% less halo work
     // than 16x15) unless fewer than half the CUs would get one.  Measured on
-    // MI355X, us per step 16x15 / 32x16: 128^2 9.6 / 10.4, 256^2 9.5 / 9.1,
-    // 512^2 15.9 / 12.3, 768^2 26.5 / 24.2, 1024^2 42.4 / 37.8, 2048^2 176 / 152.
+    // MI355X, us per step 16x15 / 32x16: 128^2 9.6 / 10.4, 256^2 8.9 / 8.6,
+    // 512^2 15.2 / 11.5, 1024^2 41 / 35, 2048^2 172 / 149.
     if (q != 9) return MPD_TILE_16X15;
     return 2 * (int64_t)((nx + 31) / 32) * ((ny + 15) / 16) >= (cus > 0 ? cus : 256) ? MPD_TILE_32X16
                                                                                         : MPD_TILE_16X15;
