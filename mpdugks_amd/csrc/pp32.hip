@@ -726,11 +726,16 @@ __global__ void __launch_bounds__(TX * TY > 256 ? 512 : 256)
         const int r = tid - NIN + (tid < NIN ? NT : 0) + u * NT;
         rok[u] = r < NR0;
         ring_pos<TX, TY, PO>(rok[u] ? r : 0, rcx[u], rcy[u]);
-        const uint32_t n = (uint32_t)(wrap(j0 + rcy[u], ny, small) * nx + wrap(i0 + rcx[u], nx, small));
+        const int cx = rcx[u], cy = rcy[u];
+        const uint32_t n = (uint32_t)(wrap(j0 + cy, ny, small) * nx + wrap(i0 + cx, nx, small));
         rrho[u] = ldd(in.rho, n);
         if (FORCE) {
-            rvx[u] = ldf(in.f, n, (Q + 2) * in.pb);
-            rvy[u] = ldf(in.f, n, (Q + 3) * in.pb);
+            // a cell outside a face-Psi region loads the tile cell's value instead
+            // (a line this thread reads anyway), not a line nobody needs
+            const bool xin = cx >= -1 && cx <= TX + 1 && cy >= -1 && cy <= TY;
+            const bool yin = cx >= -1 && cx <= TX && cy >= -1 && cy <= TY + 1;
+            rvx[u] = ldf(in.f, xin ? n : cn, (Q + 2) * in.pb);
+            rvy[u] = ldf(in.f, yin ? n : cn, (Q + 3) * in.pb);
         }
     }
     __builtin_amdgcn_sched_barrier(0);
