@@ -24,8 +24,8 @@ struct PP32Planes {
     float *f;                                     // current set: float block [(q + 4) N]
     float *tau, *Fx, *Fy;                         // [N]: relaxation time, cell force
     float *ffx, *ffy;                             // [2N]: face force
-    float *fbp, *psi, *scg;                       // per-phase scratch: bar-plus [9][N], cell Psi, scG [N]
-    float *fpsi, *frho, *fU, *fV, *fscg, *fmass;  // per-phase face scratch [2N]
+    float *fbp, *psi;                             // per-phase scratch: bar-plus [9][N], cell scG Psi [N]
+    float *fpsi, *frho, *fU, *fV, *fmass;         // per-phase face scratch: scG Psi, rho, U, V, mass [2N]
     float *ffl;                                   // per-phase face values / fluxes [9][2N]
 };
 // planes of a set's float block

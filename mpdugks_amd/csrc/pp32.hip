@@ -507,11 +507,12 @@ __device__ __forceinline__ void grad6(const float2 *b, int c, float2 &gx, float2
     gy.x = fmaf(4.f, n.x - s.x, (ne.x - se.x) + (nw.x - sw.x));
     gy.y = fmaf(4.f, n.y - s.y, (ne.y - se.y) + (nw.y - sw.y));
 }
-template <class G, int DIR>
-__device__ __forceinline__ float uw_one(const KPP &k, int en_, int et_, float bc, float gn, float gt, float cn)
+// one side's value: the cell's bar-plus plus its normal (cn) and tangential
+// (e_t) gradient terms
+__device__ __forceinline__ float uw_one(const KPP &k, int e_t, float bc, float gn, float gt, float cn)
 {
     float v = fmaf(cn, gn, bc);
-    if (et_ != 0) v = fmaf(et_ > 0 ? -k.uw_t : k.uw_t, gt, v);
+    if (e_t != 0) v = fmaf(e_t > 0 ? -k.uw_t : k.uw_t, gt, v);
     return v;
 }
 template <class L, class G, int DIR, int P>
@@ -528,16 +529,16 @@ __device__ __forceinline__ float2 recon_uw(const KPP &k, const float2 *b, const 
     const float2 bo = b[x.o], bn = b[x.n];
     float2 v;
     if (EN == 0) {
-        v.x = 0.5f * (uw_one<G, DIR>(k, 0, ET, bo.x, go_n.x, go_t.x, k.uw_0) +
-                      uw_one<G, DIR>(k, 0, ET, bn.x, gn_n.x, gn_t.x, -k.uw_0));
-        v.y = 0.5f * (uw_one<G, DIR>(k, 0, -ET, bo.y, go_n.y, go_t.y, k.uw_0) +
-                      uw_one<G, DIR>(k, 0, -ET, bn.y, gn_n.y, gn_t.y, -k.uw_0));
+        v.x = 0.5f * (uw_one(k, ET, bo.x, go_n.x, go_t.x, k.uw_0) +
+                      uw_one(k, ET, bn.x, gn_n.x, gn_t.x, -k.uw_0));
+        v.y = 0.5f * (uw_one(k, -ET, bo.y, go_n.y, go_t.y, k.uw_0) +
+                      uw_one(k, -ET, bn.y, gn_n.y, gn_t.y, -k.uw_0));
     } else {
         // q (e_n = EN) is upwinded from the owner for EN > 0; its partner from the other side
-        v.x = EN > 0 ? uw_one<G, DIR>(k, EN, ET, bo.x, go_n.x, go_t.x, k.uw_n)
-                     : uw_one<G, DIR>(k, EN, ET, bn.x, gn_n.x, gn_t.x, -k.uw_n);
-        v.y = EN > 0 ? uw_one<G, DIR>(k, -EN, -ET, bn.y, gn_n.y, gn_t.y, -k.uw_n)
-                     : uw_one<G, DIR>(k, -EN, -ET, bo.y, go_n.y, go_t.y, k.uw_n);
+        v.x = EN > 0 ? uw_one(k, ET, bo.x, go_n.x, go_t.x, k.uw_n)
+                     : uw_one(k, ET, bn.x, gn_n.x, gn_t.x, -k.uw_n);
+        v.y = EN > 0 ? uw_one(k, -ET, bn.y, gn_n.y, gn_t.y, -k.uw_n)
+                     : uw_one(k, -ET, bo.y, go_n.y, go_t.y, k.uw_n);
     }
     return v;
 }
@@ -1183,10 +1184,10 @@ int pp32_alloc(PP32State &St, void **mem, int64_t *bytes, const Consts &C, int64
 {
     // two sets of: FP64 density [N]; float block (q + 4) [N]; then the shared
     // planes tau, Fx, Fy [N], ffx, ffy [2N]; D2Q9 also the per-phase scratch:
-    // bar-plus 9 [N], cell Psi, scG [N], face Psi rho U V scG mass 6 [2N], face values 9 [2N]
+    // bar-plus 9 [N], cell scG Psi [N], face scG Psi, rho, U, V, mass 5 [2N], face values 9 [2N]
     const size_t n = (size_t)N;
     const size_t set_f = (size_t)pp32_set_planes(q) * n;
-    const size_t scratch_f = q == 9 ? 11 * n + 12 * n + 18 * n : 0;
+    const size_t scratch_f = q == 9 ? 10 * n + 10 * n + 18 * n : 0;
     const size_t bytes_total = 2 * n * sizeof(double) + (2 * set_f + SH_PLANES * n + scratch_f) * sizeof(float);
     void *base = nullptr;
     if (hipMalloc(&base, bytes_total) != hipSuccess) return -1;
@@ -1219,12 +1220,10 @@ int pp32_alloc(PP32State &St, void **mem, int64_t *bytes, const Consts &C, int64
     if (q == 9) {
         P.fbp = p; p += 9 * n;
         P.psi = p; p += n;
-        P.scg = p; p += n;
         P.fpsi = p; p += 2 * n;
         P.frho = p; p += 2 * n;
         P.fU = p; p += 2 * n;
         P.fV = p; p += 2 * n;
-        P.fscg = p; p += 2 * n;
         P.fmass = p; p += 2 * n;
         P.ffl = p;
     }
