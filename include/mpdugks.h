@@ -234,7 +234,12 @@ int mpd_set_kernel_timing(mpd_handle *h, int enable);
 
 /* Average duration (ms) per step of the step kernels since the last call:
  * with per-launch timing on, the mean of the bracketed launches; otherwise the
- * batch wall time on the stream divided by the steps.  Resets the counters. */
+ * time on the stream across each mpd_step call's launches divided by the
+ * launches counted.  On the fused path a call of two or more steps is timed
+ * from the end of its first step kernel (the first kernel's start waits for
+ * the host to enqueue it) and counts the others; priming steps after an
+ * upload are not counted.  `launches` receives the launches counted.  Resets
+ * the counters. */
 int mpd_kernel_timing(mpd_handle *h, double *avg_ms, int64_t *launches);
 
 /* ---- batched ensembles ----

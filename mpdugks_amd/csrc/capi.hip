@@ -796,14 +796,27 @@ int mpd_step(mpd_handle *h, int64_t nsteps)
         if (e1) h->ev_free.push_back(e1);
         return r;
     };
+    // Without per-launch timing one event pair brackets the timed launches.
+    // On the fused path the start event follows the first step kernel when
+    // there are two or more: an event ahead of the first kernel is stamped as
+    // soon as the idle stream reaches it, before the host has enqueued that
+    // kernel, so the pair would also time the host's enqueue latency.
+    // `timed` counts the launches inside the pair.
+    int64_t timed = 0;
     if (!h->per_launch) {
         if (int r = take_event(h, &e0)) return r;
         if (int r = take_event(h, &e1)) return bail(r);
-        if (hipError_t e = hipEventRecord(e0, h->stream); e != hipSuccess)
-            return bail(fail(MPD_ERR_DEVICE, "hipEventRecord: %s", hipGetErrorString(e)));
     }
+    auto mark = [&]() -> int {
+        if (e0)
+            if (hipError_t e = hipEventRecord(e0, h->stream); e != hipSuccess)
+                return bail(fail(MPD_ERR_DEVICE, "hipEventRecord: %s", hipGetErrorString(e)));
+        return 0;
+    };
     if (path == MPD_PATH_PHASES) {
         if (int r = alloc_phase_scratch(h)) return bail(r);
+        if (int r = mark()) return r;
+        timed = nsteps;
         for (int64_t s = 0; s < nsteps; ++s) {
             phase_step(h->S, h->C, h->stream);
             // this step's face Psi becomes the previous step's for the next one
@@ -852,7 +865,6 @@ int mpd_step(mpd_handle *h, int64_t nsteps)
                 h->evs.resize(2 * pool);
                 for (auto &e : h->evs) HIPCHK(hipEventCreate(&e));
             }
-            int64_t timed = 0;
             double ms_sum = 0.0;
             while (s < nsteps) {
                 int64_t k = 0;
@@ -874,17 +886,29 @@ int mpd_step(mpd_handle *h, int64_t nsteps)
             HIPCHK(hipGetLastError());
             return 0;
         }
+        if (nsteps - s >= 2) {
+            fused_step(h->F, h->S, h->C, h->cfg, 0, h->stream);
+            ++s;
+        }
+        if (int r = mark()) return r;
+        timed = nsteps - s;
         for (; s < nsteps; ++s) fused_step(h->F, h->S, h->C, h->cfg, 0, h->stream);
     }
     HIPCHK(hipGetLastError());
     if (e1) {
+        if (timed == 0) {
+            // every step was a priming step: nothing to time
+            h->ev_free.push_back(e0);
+            h->ev_free.push_back(e1);
+            return 0;
+        }
         HIPCHK(hipEventRecord(e1, h->stream));
         h->pending.emplace_back(e0, e1);
         // bound the pool: a caller that never synchronises waits here now and then
         if (h->pending.size() >= 256)
             if (int r = resolve_timing(h)) return r;
     }
-    h->launches += nsteps;
+    h->launches += timed;
     return 0;
 }
 
