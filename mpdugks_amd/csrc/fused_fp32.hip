@@ -5,9 +5,14 @@
 //    folds to adds and the populations with zero normal velocity skip the
 //    face BGK (their flux is zero);
 //  - divisions by per-cell/per-face scalars become one reciprocal each;
-//  - the three convex combinations are written as  x + b (y - x) + c z  so the
-//    unit coefficient is exact and FP32 rounding of a and b = 1 - a cannot make
-//    the scheme gain or lose mass;
+//  - the BGK combinations a x + b x_eq + c x_src (a = 1 - b) are written as
+//    (1 - b) x + (b x_eq) + c x_src with b folded into the equilibrium terms
+//    (one FMA per population on top of the equilibrium), and 1 - b, b the FP32
+//    pair whose sum is exactly 1 (split1): the unit coefficient carries no
+//    rounding.  Without it the rounding of a + b acts every step on every
+//    population, a bias (measured on the Rayleigh-Taylor case: kinetic energy
+//    1.2e-3 low after 2 10^4 steps, 6x the spread of FP64 runs perturbed at
+//    FP32 round-off);
 //  - the D2Q9 weights are rounded so that they sum to exactly 1 in FP32;
 //  - AC-p (U == 0 by construction, Ark/Lib/D2Q9AC.cpp:197-198) drops every
 //    velocity term and has constant relaxation factors.
@@ -78,12 +83,16 @@ struct K32 {
     float rt, inv_rt, mlambda0, hdt, hhdt, hhdt_rt, dt, bdt, aphi, bphi, beta, kappa, phi_l, phi_v;
     float tau_a, tau_b;               // tau = tau_a phi + tau_b (HCZ)
     float inv_d2x6, inv_lapden, inv_rtdt, inv_wdphi, kp, dtvf;  // dtvf = dt/V * MaSpan * area
-    float ra, rb, rt4;                // face reconstruction: (1/2 + A, 1/2 - A), tangential factor
+    float ra, rb, rt4, rt2;           // face reconstruction: (1/2 + A, 1/2 - A), tangential factor, 2 rt4
     float gx, gy, rho_ref;
-    float bbp_h, bh_h;                // Allen-Cahn factors (constant tau_h)
-    float cbpw_h[3], chw_h[3];        // c * w_q for the weight classes 0 / axis / diagonal
-    float bbp0, bh0;                  // AC-p constant factors
-    float cbpw0[3], chw0[3];
+    // BGK factors of the bar-plus (bp) and face half step as  (1 - b) x + b eq + c src,
+    // (omb, b) an FP32 pair whose sum is exactly 1 (split1)
+    float omb_bp_h, omb_h;            // Allen-Cahn (constant tau_h): 1 - b
+    float bpw_h[3], bw_h[3];          // b w_q for the weight classes 0 / axis / diagonal
+    float cbpw_h[3], chw_h[3];        // c w_q
+    float omb_bp0, omb0;              // AC-p constant factors
+    float bpw0[3], bw0[3], cbpw0[3], chw0[3];
+    float bprest0;                    // AC-p rest population: b (w_0 - 1) / RT
     float w[3];                       // weights rest / axis / diagonal, w0 = 1 - 4 w1 - 4 w2
     float hneq, fneq0, hdx;           // NEE walls: 2 tau/(2 tau + dt) of h (and of f, AC-p), dx/2
     float uw_n, uw_t, uw_0;           // UW faces: (1/2 - A)/12, A/12, 1/24 (A = hdt MaSpan / dx)
@@ -190,6 +199,16 @@ __device__ __forceinline__ Rs make_rs(const FusedPlanes32 &p, int64_t N, int nx,
     return r;
 }
 
+// From 1 - b rounded to FP32 (a), the FP32 pair (1 - b, b) whose sum is exactly
+// 1: the member >= 1/2 keeps a rounded value and the other is its exact
+// complement (Sterbenz), so (1 - b) x + b y reproduces x + b (y - x) with a
+// unit coefficient that no rounding moves off 1.
+__host__ __device__ __forceinline__ void split1(float a, float &omb, float &b)
+{
+    b = 1.f - a;
+    omb = 1.f - b;
+}
+
 // e_a x + e_b y for compile-time e in {-1, 0, 1} (no multiplications by 0 or 1)
 template <int A, int B>
 __device__ __forceinline__ float edot(float x, float y)
@@ -201,10 +220,15 @@ __device__ __forceinline__ float edot(float x, float y)
     return B > 0 ? y - x : -(x + y);
 }
 
-// LDS layout of the bar-plus region (R1X x R1Y cells): interior columns
-// 1..TX at row pitch TX, so that the two 16-lane rows of a half-wave fall on
-// disjoint banks, then the halo columns 0 and TX+1 (two per row).  The phi
-// region (R2X x R2Y) likewise: columns 2..TX+1 at pitch TX, then the four halo
+// LDS layout of the bar-plus region (R1X x R1Y cells).  The LDS serves a
+// 64-bit read 16 lanes at a time and a 32-bit read 32 lanes at a time, over 32
+// banks; lanes of one group that hit one bank at different addresses serialise.
+// 16-wide tiles: interior columns 1..TX at row pitch TX, so that the two 16-lane
+// rows of a half-wave fall on disjoint banks, then the halo columns 0 and TX+1
+// (two per row).  32-wide tiles: rows of R1X in place, halo columns included, so
+// that the 16 lanes of a group read 16 consecutive cells of one row even where
+// the first or last of them is a halo cell.  The phi region (R2X x R2Y, 32-bit
+// reads of two rows per wave): columns 2..TX+1 at pitch TX, then the four halo
 // columns of each row.  Column info of one column: index = row * pitch + off.
 struct Col {
     int pitch, off;
@@ -213,6 +237,7 @@ template <class G>
 __device__ __forceinline__ Col col1(int c)
 {
     constexpr int TX = G::TX, H = G::H;
+    if constexpr (TX == 32) return Col{G::R1X, c};
     const bool in = c >= H && c < H + TX;
     return Col{in ? TX : 2 * H, in ? c - H : TX * G::R1Y + (c < H ? c : c - TX)};
 }
@@ -271,17 +296,19 @@ __device__ __forceinline__ void momentum_source(const K32 &k, float phi, float g
     Fy = mu_c * gy - k.gy * drho;
 }
 
-// Quantities of one site (cell centre or face) shared by its populations.
+// Quantities of one site (cell centre or face) shared by its populations; the
+// weighted equilibrium terms carry the site's BGK factor b, so that pair_eq
+// yields b x_eq (the NEE wall values use a copy with factor 1).
 struct Site {
     float U, V, G0, UF, UR;            // velocity, -|u|^2 Lambda0, u.F, u.grad rho
     float mFx, mFy, mrx, mry, mAx, mAy;  // MaSpan * (F, grad rho, AC source); AC-p: F / RT
-    float wphi[3], wp[3], wr[3];       // w phi, w p (AC-p: w p / RT), w rho RT per weight class
+    float wphi[3], wp[3], wr[3];       // b w phi, b w p (AC-p: b w p / RT), b w rho RT per weight class
 };
 
 // Equilibria and sources of the opposite pair (q, q') with e_q = (EX, EY),
 // e_q' = -e_q (Update_phi_Eq + Update_DVDF_Source: D2Q9HCZ.cpp:54-92,244-257,
-// D2Q9AC.cpp:69-105,331-344).  fs, fsp and hs are the sources without the
-// weight; the caller's factor carries it.
+// D2Q9AC.cpp:69-105,331-344), the equilibria times the Site's factor b.  fs,
+// fsp and hs are the sources without the weight; the caller's factor carries it.
 template <int MODEL, int EX, int EY>
 __device__ __forceinline__ void pair_eq(const K32 &k, const Site &m, float &heq, float &heqp, float &feq, float &feqp,
                                         float &fs, float &fsp, float &hs)
@@ -361,7 +388,7 @@ __device__ __forceinline__ void cell_bp(const K32 &k, const Rs &out, int nx, int
     const float phi = s_phi[sti.c[1]];
     const float rho = k.aphi * phi + k.bphi;
     Site m;
-    float p, bbp, tau = 0.f, rtau = 0.f;
+    float p, bbp, omb, tau = 0.f, rtau = 0.f;
     m.U = m.V = 0.f;
     if (HCZ) {
         float lgx, lgy, llap, lFx, lFy, lrx, lry;
@@ -376,10 +403,11 @@ __device__ __forceinline__ void cell_bp(const K32 &k, const Rs &out, int nx, int
         p = fmaf(k.hdt * k.rt, lrx * m.U + lry * m.V, s);
         tau = fmaf(phi, k.tau_a, k.tau_b);
         rtau = rcp(2.f * tau + k.dt);
-        bbp = k.bdt * rtau;                  // 1 - (2 tau - dt/2)/(2 tau + dt)
+        split1(fmaf(-k.bdt, rtau, 1.f), omb, bbp);   // 1 - b = (2 tau - dt/2)/(2 tau + dt)
     } else {
         p = k.kp * (((f[1] + f[2]) + (f[3] + f[4])) + ((f[5] + f[6]) + (f[7] + f[8])));
-        bbp = k.bbp0;
+        omb = k.omb_bp0;
+        bbp = 0.f;
     }
     float gx, gy, lap, Fx, Fy, rgx, rgy;
     stencil(k, s_phi, sti, gx, gy, lap);
@@ -415,12 +443,12 @@ __device__ __forceinline__ void cell_bp(const K32 &k, const Rs &out, int nx, int
         m.mrx = k.ms * rgx;
         m.mry = k.ms * rgy;
         const float r = rho * k.rt;
-        const float cbp = tau * bbp;
+        const float cbp = tau * bbp, bp = bbp * p, br = bbp * r;
 #pragma unroll
         for (int t = 0; t < 3; ++t) {
-            m.wphi[t] = k.w[t] * phi;
-            m.wp[t] = k.w[t] * p;
-            m.wr[t] = k.w[t] * r;
+            m.wphi[t] = k.bpw_h[t] * phi;
+            m.wp[t] = k.w[t] * bp;
+            m.wr[t] = k.w[t] * br;
             cw[t] = cbp * k.w[t];
             chw[t] = k.cbpw_h[t];
         }
@@ -431,11 +459,22 @@ __device__ __forceinline__ void cell_bp(const K32 &k, const Rs &out, int nx, int
         const float pr = p * k.inv_rt;
 #pragma unroll
         for (int t = 0; t < 3; ++t) {
-            m.wphi[t] = k.w[t] * phi;
-            m.wp[t] = k.w[t] * pr;
+            m.wphi[t] = k.bpw_h[t] * phi;
+            m.wp[t] = k.bpw0[t] * pr;
             m.wr[t] = 0.f;
             cw[t] = k.cbpw0[t];
             chw[t] = k.cbpw_h[t];
+        }
+    }
+    // NEE walls use the equilibria themselves (factor 1)
+    Site mu = m;
+    if (WALLS == 2) {
+        const float pu = HCZ ? p : p * k.inv_rt, ru = HCZ ? rho * k.rt : 0.f;
+#pragma unroll
+        for (int t = 0; t < 3; ++t) {
+            mu.wphi[t] = k.w[t] * phi;
+            mu.wp[t] = k.w[t] * pu;
+            mu.wr[t] = k.w[t] * ru;
         }
     }
     // NEE walls (Wall_3_NEE, Boundary.cpp:140-184; WALLS == 2): the macros of a
@@ -460,23 +499,24 @@ __device__ __forceinline__ void cell_bp(const K32 &k, const Rs &out, int nx, int
             feq = fmaf(m.wr[0], m.G0, m.wp[0]);
             fs = -fmaf(m.G0, fmaf(k.rt, m.UR, m.UF), m.UF);
         } else {
-            feq = (k.w[0] - 1.f) * p * k.inv_rt;
+            feq = k.bprest0 * p;
             fs = 0.f;
         }
-        bh[0] = fmaf(k.bbp_h, heq - h[0], h[0]);
-        bf[0] = fmaf(bbp, feq - f[0], f[0]) + cw[0] * fs;
+        bh[0] = fmaf(k.omb_bp_h, h[0], heq);
+        bf[0] = fmaf(omb, f[0], feq) + cw[0] * fs;
     }
 #define MPD_CELL_PAIR(Q, EX, EY)                                                         \
     {                                                                                    \
         constexpr int QP = Q + 4, W = (EX == 0 || EY == 0) ? 1 : 2;                      \
         float heq, heqp, feq, feqp, fs, fsp, hs;                                         \
         pair_eq<MODEL, EX, EY>(k, m, heq, heqp, feq, feqp, fs, fsp, hs);                 \
-        bh[Q] = fmaf(k.bbp_h, heq - h[Q], h[Q]) + chw[W] * hs;                           \
-        bh[QP] = fmaf(k.bbp_h, heqp - h[QP], h[QP]) - chw[W] * hs;                       \
-        bf[Q] = fmaf(bbp, feq - f[Q], f[Q]) + cw[W] * fs;                                \
-        bf[QP] = fmaf(bbp, feqp - f[QP], f[QP]) + cw[W] * fsp;                           \
+        bh[Q] = fmaf(k.omb_bp_h, h[Q], heq) + chw[W] * hs;                               \
+        bh[QP] = fmaf(k.omb_bp_h, h[QP], heqp) - chw[W] * hs;                            \
+        bf[Q] = fmaf(omb, f[Q], feq) + cw[W] * fs;                                       \
+        bf[QP] = fmaf(omb, f[QP], feqp) + cw[W] * fsp;                                   \
         if (WALLS == 2 && Q >= 2) {                                                      \
             constexpr int NK = Q == 3 ? 0 : (Q == 2 ? 1 : 2);                            \
+            pair_eq<MODEL, EX, EY>(k, mu, heq, heqp, feq, feqp, fs, fsp, hs);            \
             const float hw = k.hdt * k.w[W];                                             \
             const float hx = top ? h[Q] - heq : h[QP] - heqp;                            \
             const float fx = top ? f[Q] - feq : f[QP] - feqp;                            \
@@ -518,7 +558,10 @@ __device__ __forceinline__ void cell_bp(const K32 &k, const Rs &out, int nx, int
 }
 
 // Face reconstruction of one population (CD_Interior_phi_Bh,
-// DUGKS2DSolver.cpp:604-687): 1/2 (o + n) - h e_n (n - o)/dx - h e_t (dt o + dt n)/2
+// DUGKS2DSolver.cpp:604-687): 1/2 (o + n) - h e_n (n - o)/dx - h e_t (dt o + dt n)/2.
+// A population with e_n = 0 carries no flux and enters only the face moments;
+// its value is returned doubled (o + n - h e_t (dt o + dt n)), and the moments
+// take the 1/2 once.
 struct FaceIdx {
     int o, n, op, om, np, nm;  // owner, neighbour, and their tangential neighbours (+/-)
 };
@@ -535,13 +578,14 @@ __device__ __forceinline__ float2 recon(const K32 &k, const float2 *b, const Fac
         v.x = fmaf(k.ra, bn.x, k.rb * bo.x);
         v.y = fmaf(k.ra, bn.y, k.rb * bo.y);
     } else {
-        v.x = 0.5f * (bo.x + bn.x);
-        v.y = 0.5f * (bo.y + bn.y);
+        v.x = bo.x + bn.x;
+        v.y = bo.y + bn.y;
     }
     if (ET != 0) {
         const float2 np = b[x.np], nm = b[x.nm], op = b[x.op], om = b[x.om];
         const float tx = (np.x - nm.x) + (op.x - om.x), ty = (np.y - nm.y) + (op.y - om.y);
-        const float c = ET > 0 ? -k.rt4 : k.rt4;
+        const float r = EN == 0 ? k.rt2 : k.rt4;
+        const float c = ET > 0 ? -r : r;
         v.x = fmaf(c, tx, v.x);
         v.y = fmaf(c, ty, v.y);
     }
@@ -636,8 +680,8 @@ __device__ __forceinline__ float2 recon_uw(const K32 &k, const float2 *b, const 
             vn.x = fmaf(ct, gtn.x, vn.x);
             vn.y = fmaf(ct, gtn.y, vn.y);
         }
-        v.x = 0.5f * (vo.x + vn.x);
-        v.y = 0.5f * (vo.y + vn.y);
+        v.x = vo.x + vn.x;  // doubled, as in recon
+        v.y = vo.y + vn.y;
     }
     return v;
 }
@@ -698,22 +742,28 @@ __device__ __forceinline__ void face(const K32 &k, const float2 *S, const X &x, 
     const float ax = A0.x + A1.x, ay = A0.y + A1.y;
     const float rgx = R0.x + R1.x, rgy = R0.y + R1.y;
     const float Fx = F0.x + F1.x, Fy = F0.y + F1.y;
-    const float phi = ((hv[0] + hv[1]) + (hv[2] + hv[3])) + ((hv[4] + hv[5]) + (hv[6] + hv[7])) + hv[8];
+    // hv, fv of the e_n = 0 populations (0 and the pair EA, EB along the face)
+    // are doubled; the axis pair along the normal is (NA, NB)
+    constexpr int EA = DIR == 0 ? 3 : 1, EB = DIR == 0 ? 7 : 5, NA = DIR == 0 ? 1 : 3, NB = DIR == 0 ? 5 : 7;
+    const float phi = fmaf(0.5f, (hv[0] + hv[EA]) + hv[EB], ((hv[NA] + hv[NB]) + (hv[2] + hv[6])) + (hv[4] + hv[8]));
     Site m;
     m.mAx = k.ms * ax;
     m.mAy = k.ms * ay;
-    float bh, cw[3];
+    float omb, cw[3];
     if (HCZ) {
         const float rho = k.aphi * phi + k.bphi;
         const float irr = rcp(rho * k.rt);
-        const float sx = (fv[1] + fv[2] + fv[8]) - (fv[4] + fv[5] + fv[6]);
-        const float sy = (fv[2] + fv[3] + fv[4]) - (fv[6] + fv[7] + fv[8]);
+        // the diagonal populations' part of e_x f and e_y f, then the axis pairs
+        const float dx = (fv[2] + fv[8]) - (fv[4] + fv[6]), dy = (fv[2] + fv[4]) - (fv[6] + fv[8]);
+        const float sx = DIR == 0 ? dx + (fv[1] - fv[5]) : fmaf(0.5f, fv[1] - fv[5], dx);
+        const float sy = DIR == 0 ? fmaf(0.5f, fv[3] - fv[7], dy) : dy + (fv[3] - fv[7]);
         m.U = fmaf(k.ms, sx, k.hhdt * Fx) * irr;
         m.V = fmaf(k.ms, sy, k.hhdt * Fy) * irr;
-        const float s = ((fv[0] + fv[1]) + (fv[2] + fv[3])) + ((fv[4] + fv[5]) + (fv[6] + fv[7])) + fv[8];
+        const float s = fmaf(0.5f, (fv[0] + fv[EA]) + fv[EB], ((fv[NA] + fv[NB]) + (fv[2] + fv[6])) + (fv[4] + fv[8]));
         const float p = fmaf(k.hhdt_rt, rgx * m.U + rgy * m.V, s);
         const float tau = fmaf(phi, k.tau_a, k.tau_b);
-        bh = k.hdt * rcp(2.f * tau + k.hdt);
+        float bh;
+        split1(fmaf(-k.hdt, rcp(2.f * tau + k.hdt), 1.f), omb, bh);   // 1 - b = 2 tau / (2 tau + h)
         const float ch = tau * bh;
         m.G0 = k.mlambda0 * (m.U * m.U + m.V * m.V);
         m.UF = m.U * Fx + m.V * Fy;
@@ -723,24 +773,25 @@ __device__ __forceinline__ void face(const K32 &k, const float2 *S, const X &x, 
         m.mrx = k.ms * rgx;
         m.mry = k.ms * rgy;
         const float r = rho * k.rt;
+        const float bp = bh * p, br = bh * r;
 #pragma unroll
         for (int t = 1; t < 3; ++t) {
-            m.wphi[t] = k.w[t] * phi;
-            m.wp[t] = k.w[t] * p;
-            m.wr[t] = k.w[t] * r;
+            m.wphi[t] = k.bw_h[t] * phi;
+            m.wp[t] = k.w[t] * bp;
+            m.wr[t] = k.w[t] * br;
             cw[t] = ch * k.w[t];
         }
     } else {
-        const float p = k.kp * (((fv[1] + fv[2]) + (fv[3] + fv[4])) + ((fv[5] + fv[6]) + (fv[7] + fv[8])));
-        bh = k.bh0;
+        const float p = k.kp * fmaf(0.5f, fv[EA] + fv[EB], ((fv[NA] + fv[NB]) + (fv[2] + fv[6])) + (fv[4] + fv[8]));
+        omb = k.omb0;
         m.U = m.V = m.G0 = m.UF = m.UR = m.mrx = m.mry = 0.f;
         m.mFx = k.ms * Fx * k.inv_rt;
         m.mFy = k.ms * Fy * k.inv_rt;
         const float pr = p * k.inv_rt;
 #pragma unroll
         for (int t = 1; t < 3; ++t) {
-            m.wphi[t] = k.w[t] * phi;
-            m.wp[t] = k.w[t] * pr;
+            m.wphi[t] = k.bw_h[t] * phi;
+            m.wp[t] = k.bw0[t] * pr;
             m.wr[t] = 0.f;
             cw[t] = k.chw0[t];
         }
@@ -752,10 +803,10 @@ __device__ __forceinline__ void face(const K32 &k, const float2 *S, const X &x, 
         constexpr int W = (EX == 0 || EY == 0) ? 1 : 2;                                  \
         float heq, heqp, feq, feqp, fs, fsp, hs;                                         \
         pair_eq<MODEL, EX, EY>(k, m, heq, heqp, feq, feqp, fs, fsp, hs);                 \
-        ho[Z] = fmaf(k.bh_h, heq - hv[Q], hv[Q]) + k.chw_h[W] * hs;                      \
-        ho[Z + 1] = fmaf(k.bh_h, heqp - hv[QP], hv[QP]) - k.chw_h[W] * hs;               \
-        fo[Z] = fmaf(bh, feq - fv[Q], fv[Q]) + cw[W] * fs;                               \
-        fo[Z + 1] = fmaf(bh, feqp - fv[QP], fv[QP]) + cw[W] * fsp;                       \
+        ho[Z] = fmaf(k.omb_h, hv[Q], heq) + k.chw_h[W] * hs;                             \
+        ho[Z + 1] = fmaf(k.omb_h, hv[QP], heqp) - k.chw_h[W] * hs;                       \
+        fo[Z] = fmaf(omb, fv[Q], feq) + cw[W] * fs;                                      \
+        fo[Z + 1] = fmaf(omb, fv[QP], feqp) + cw[W] * fsp;                               \
     }
     if (DIR == 0) {
         MPD_FACE_PAIR(0, 1, 5, 1, 0)
@@ -1043,22 +1094,28 @@ __device__ __forceinline__ void tile32(const K32 &k, int nx, int ny, const Fused
     // z have e_n = +1, odd z e_n = -1
     constexpr int xp[6] = {1, 5, 2, 6, 8, 4};
     constexpr int yp[6] = {3, 7, 2, 6, 4, 8};
+    // every population but 0 has an x- or a y-face term or both: the x terms
+    // (populations 1, 5 and the diagonals) and those of 3 and 7 are assigned,
+    // the diagonals' y terms added; population 0 has none
     float dh[9], df[9];
-#pragma unroll
-    for (int q = 0; q < 9; ++q) { dh[q] = 0.f; df[q] = 0.f; }
 #pragma unroll
     for (int z = 0; z < 6; ++z) {
         const int a = (z & 1) ? kE : kW, b = (z & 1) ? kW : kE;
         const float2 va = s_fx[z * FS + a], vb = s_fx[z * FS + b];
-        dh[xp[z]] += va.x - vb.x;
-        df[xp[z]] += va.y - vb.y;
+        dh[xp[z]] = va.x - vb.x;
+        df[xp[z]] = va.y - vb.y;
     }
 #pragma unroll
     for (int z = 0; z < 6; ++z) {
         const int a = (z & 1) ? kN : kS, b = (z & 1) ? kS : kN;
         const float2 va = s_fy[z * FS + a], vb = s_fy[z * FS + b];
-        dh[yp[z]] += va.x - vb.x;
-        df[yp[z]] += va.y - vb.y;
+        if (z < 2) {
+            dh[yp[z]] = va.x - vb.x;
+            df[yp[z]] = va.y - vb.y;
+        } else {
+            dh[yp[z]] += va.x - vb.x;
+            df[yp[z]] += va.y - vb.y;
+        }
     }
     // phi is the conserved variable, kept in FP64: it moves only by the
     // antisymmetric face mass fluxes (each face's sum over populations, shared
@@ -1079,8 +1136,9 @@ __device__ __forceinline__ void tile32(const K32 &k, int nx, int ny, const Fused
     }
     st<SC>(out.a, n, (float)(phd - (double)rest), P_H * out.pb);
     st64<SC>(out.d, n, phd);
+    st<SC>(out.a, n, base[8], P_F * out.pb);
 #pragma unroll
-    for (int q = 0; q < 9; ++q) st<SC>(out.a, n, fmaf(k.dtvf, df[q], base[8 + q]), (P_F + q) * out.pb);
+    for (int q = 1; q < 9; ++q) st<SC>(out.a, n, fmaf(k.dtvf, df[q], base[8 + q]), (P_F + q) * out.pb);
     st<SC>(out.a, n, phi, P_PHI * out.pb);
 }
 
@@ -1098,9 +1156,12 @@ __global__ void __launch_bounds__(TX * TY > 256 ? 512 : 256) k_fused32(K32 k, in
 
 // Batched step: every workgroup takes one (case, tile) entry of a work list
 // built on the host; the cases share the model, the y boundary and the tile height.
+// Four waves per SIMD (at most 128 VGPRs: two 512-thread or four 256-thread
+// workgroups per CU), which the register allocator would otherwise exceed on the
+// periodic 32x16 instantiation; NEE batches (16x15 tiles) are left unbounded.
 template <int MODEL, int WALLS, int TX, int TY>
-__global__ void __launch_bounds__(TX * TY > 256 ? 512 : 256) k_fused32_batch(const BatchCase *cases, const int2 *work,
-                                                                            int nwork, int parity)
+__global__ void __launch_bounds__(TX * TY > 256 ? 512 : 256, WALLS == 2 ? 1 : 4)
+    k_fused32_batch(const BatchCase *cases, const int2 *work, int nwork, int parity)
 {
     const int b = blockIdx.x;
     if (b >= nwork) return;
@@ -1147,6 +1208,7 @@ K32 make_k32(const Consts &C)
     k.ra = (float)(0.5 + A);
     k.rb = (float)(0.5 - A);
     k.rt4 = (float)(C.hdt * ms / (2.0 * C.twodx));
+    k.rt2 = 2.f * k.rt4;
     k.gx = (float)C.gx;
     k.gy = (float)C.gy;
     k.rho_ref = (float)C.rho_ref;
@@ -1164,16 +1226,23 @@ K32 make_k32(const Consts &C)
     // AC-p: tau = Mu/(rho RT) with Mu = rho RT Tau0 (D2Q9AC.cpp:34-60) -> Tau0
     const double bbp0 = 1.5 * C.dt / (2.0 * C.tau0 + C.dt), cbp0 = C.tau0 * bbp0;
     const double bh0 = C.hdt / (2.0 * C.tau0 + C.hdt), ch0 = C.tau0 * bh0;
-    k.bbp_h = (float)bbp_h;
-    k.bh_h = (float)bh_h;
-    k.bbp0 = (float)bbp0;
-    k.bh0 = (float)bh0;
+    float fb_bp_h, fb_h, fb_bp0, fb0;
+    split1((float)(1.0 - bbp_h), k.omb_bp_h, fb_bp_h);
+    split1((float)(1.0 - bh_h), k.omb_h, fb_h);
+    split1((float)(1.0 - bbp0), k.omb_bp0, fb_bp0);
+    split1((float)(1.0 - bh0), k.omb0, fb0);
+    const double b_bp_h = fb_bp_h, b_h = fb_h, b_bp0 = fb_bp0, b0 = fb0;
     for (int t = 0; t < 3; ++t) {
+        k.bpw_h[t] = (float)(b_bp_h * k.w[t]);
+        k.bw_h[t] = (float)(b_h * k.w[t]);
+        k.bpw0[t] = (float)(b_bp0 * k.w[t]);
+        k.bw0[t] = (float)(b0 * k.w[t]);
         k.cbpw_h[t] = (float)(cbp_h * wd[t]);
         k.chw_h[t] = (float)(ch_h * wd[t]);
         k.cbpw0[t] = (float)(cbp0 * wd[t]);
         k.chw0[t] = (float)(ch0 * wd[t]);
     }
+    k.bprest0 = (float)(b_bp0 * ((double)k.w[0] - 1.0) / C.rt);
     k.hneq = (float)(2.0 * C.taum / (2.0 * C.taum + C.dt));
     k.fneq0 = (float)(2.0 * C.tau0 / (2.0 * C.tau0 + C.dt));
     k.hdx = (float)(0.5 * C.dx);
