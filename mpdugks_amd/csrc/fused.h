@@ -10,6 +10,17 @@ namespace mpd {
 
 struct Consts;
 
+// BGK combinations a x + b x_eq + c x_src (a = 1 - b) with b folded into the
+// equilibrium: (1 - b) x + (b x_eq) + c x_src.  From 1 - b rounded to FP32 (a),
+// the FP32 pair (1 - b, b) whose sum is exactly 1: the member >= 1/2 keeps a
+// rounded value and the other is its exact complement (Sterbenz), so the unit
+// coefficient that x + b (x_eq - x) has exactly carries no rounding either.
+__host__ __device__ __forceinline__ void split1(float a, float &omb, float &b)
+{
+    b = 1.f - a;
+    omb = 1.f - b;
+}
+
 // FP32-bulk pseudopotential planes (pp32.hip), velocity set of q populations
 // (D2Q9 or D2V16).  A state set is one float block of (q + 4) planes -- f~[q],
 // U, V, and the face Psi of the step that produced it, signed by scG (x-faces,

@@ -199,16 +199,6 @@ __device__ __forceinline__ Rs make_rs(const FusedPlanes32 &p, int64_t N, int nx,
     return r;
 }
 
-// From 1 - b rounded to FP32 (a), the FP32 pair (1 - b, b) whose sum is exactly
-// 1: the member >= 1/2 keeps a rounded value and the other is its exact
-// complement (Sterbenz), so (1 - b) x + b y reproduces x + b (y - x) with a
-// unit coefficient that no rounding moves off 1.
-__host__ __device__ __forceinline__ void split1(float a, float &omb, float &b)
-{
-    b = 1.f - a;
-    omb = 1.f - b;
-}
-
 // e_a x + e_b y for compile-time e in {-1, 0, 1} (no multiplications by 0 or 1)
 template <int A, int B>
 __device__ __forceinline__ float edot(float x, float y)

@@ -123,9 +123,11 @@ struct KPP {
     float inv_d2x6;        // 9-point gradient
     float w[3];            // D2Q9 weights: rest, axis, diagonal
     float wirt[3];         // w / RT
-    float bbp_u, cbp_u;    // bar-plus factors of a uniform relaxation time (tau_u > 0)
+    float omb_u, bbp_u, cbp_u;  // bar-plus factors of a uniform relaxation time (tau_u > 0): 1 - b, b, c
     float tau_u;           // the uniform cell relaxation time, or 0
-    float bh, ch;          // face BGK: f + bh (f_eq - f) + ch f_So
+    float omb_h, bh, ch;   // face BGK: (1 - bh) f + bh f_eq + ch f_So
+    // (1 - b, b) pairs sum to exactly 1 (split1); the equilibria are computed
+    // for the density b rho, which yields b f_eq
     float dtv, flux;       // dt / V; dx ms (micro-flux per unit e_n)
     float uw_n, uw_t, uw_0;  // UW face value, 6-point gradient sums (x 12 dx)
     double pp_k, t_cs, drt;  // FP64 EOS: K, T_CS, RT
@@ -157,9 +159,10 @@ KPP make_kpp(const Consts &C, int q, double tau_u)
     for (int c = 0; c < 3; ++c) k.wirt[c] = (float)(C.w[c] / C.rt);
     // Cell_2D.cpp:120-125: bBP = 1 - (2 tau - hdt)/(2 tau + dt) = (dt + hdt)/(2 tau + dt), cBP = tau bBP
     k.tau_u = tau_u > 0 ? (float)tau_u : 0.f;
-    k.bbp_u = tau_u > 0 ? (float)((C.dt + C.hdt) / (2.0 * tau_u + C.dt)) : 0.f;
+    k.omb_u = k.bbp_u = 0.f;
+    if (tau_u > 0) split1((float)(1.0 - (C.dt + C.hdt) / (2.0 * tau_u + C.dt)), k.omb_u, k.bbp_u);
     k.cbp_u = tau_u > 0 ? (float)(tau_u * (C.dt + C.hdt) / (2.0 * tau_u + C.dt)) : 0.f;
-    k.bh = (float)C.bh_pp;
+    split1((float)(1.0 - C.bh_pp), k.omb_h, k.bh);
     k.ch = (float)C.ch_pp;
     k.dtv = (float)C.dtv;
     k.flux = (float)(C.dx * ms);
@@ -657,7 +660,7 @@ __device__ __forceinline__ void s3_face(const KPP &k, const Rs &out, const float
     // the tile's own faces (not the last column / row, which the next tile owns) are new state
     if ((DIR == 0 ? fx < G::TX : fy < G::TY) && i0 + fx < nx && j0 + fy < ny)
         stf(out.f, (uint32_t)((j0 + fy) * nx + (i0 + fx)), vc, (L::Q + 2 + DIR) * out.pb);
-    s.rho = rho;
+    s.rho = k.bh * rho;   // feq = bh f_eq
     s.U = U;
     s.V = V;
     float feq[L::Q], fso[L::Q];
@@ -665,7 +668,7 @@ __device__ __forceinline__ void s3_face(const KPP &k, const Rs &out, const float
     float mass = 0.f;
     unroll<NM>([&](auto Z) {
         constexpr int z = decltype(Z)::value, q = mq<L, DIR>(z);
-        const float ff = fmaf(k.ch, fso[q], fmaf(k.bh, feq[q] - hv[z], hv[z]));
+        const float ff = fmaf(k.ch, fso[q], fmaf(k.omb_h, hv[z], feq[q]));
         const float fl = ff * (k.flux * (float)en<L, DIR>(q));
         s_fl[z * FN + kf] = fl;
         mass += fl;
@@ -777,32 +780,33 @@ __global__ void __launch_bounds__(TX * TY > 256 ? 512 : 256)
     __syncthreads();
 
     // ---- S1: cell force and bar-plus (Update_phi_BP, DUGKS2DSolver.cpp:379-401;
-    // factors Cell_2D.cpp:120-125, written f~ + bBP (f_eq - f~) + cBP f_So) ----
+    // factors Cell_2D.cpp:120-125, written (1 - bBP) f~ + bBP f_eq + cBP f_So) ----
     auto barplus = [&](int cx, int cy, const float *f, float U, float V, float tau, float &Fx, float &Fy, float *base) {
         const int qi = qidx<G>(cx, cy);
         if (FORCE)
             pp_force(k, s_psi[qi], s_psi[qi + 1], s_psi[qi + QX], s_psi[qi - 1], s_psi[qi - QX], s_psi[qi + QX + 1],
                      s_psi[qi + QX - 1], s_psi[qi - QX - 1], s_psi[qi - QX + 1], Fx, Fy);
+        float omb, bbp, cbp;
+        if (TAUU) {
+            omb = k.omb_u;
+            bbp = k.bbp_u;
+            cbp = k.cbp_u;
+        } else {
+            split1(fmaf(-k.bdt, rcp(fmaf(2.f, tau, k.dt)), 1.f), omb, bbp);
+            cbp = tau * bbp;
+        }
         Site s;
-        s.rho = s_rho[qi];
+        s.rho = bbp * s_rho[qi];   // feq = bBP f_eq
         s.U = U;
         s.V = V;
         s.Fx = Fx;
         s.Fy = Fy;
         float feq[Q], fso[Q];
         feq_src<L, L::REST>(k, s, feq, fso, [](int) { return true; });
-        float bbp, cbp;
-        if (TAUU) {
-            bbp = k.bbp_u;
-            cbp = k.cbp_u;
-        } else {
-            bbp = k.bdt * rcp(fmaf(2.f, tau, k.dt));
-            cbp = tau * bbp;
-        }
         const int ri = (cy + RO) * RX + (cx + RO);
         float b[Q];
 #pragma unroll
-        for (int q = 0; q < Q; ++q) b[q] = fmaf(cbp, fso[q], fmaf(bbp, feq[q] - f[q], f[q]));
+        for (int q = 0; q < Q; ++q) b[q] = fmaf(cbp, fso[q], fmaf(omb, f[q], feq[q]));
         if constexpr (L::REST) s_rest[ri] = b[0];
 #pragma unroll
         for (int p = 0; p < NP; ++p) s_pair[p * RN + ri] = make_float2(b[L::first(p)], b[L::opp(L::first(p))]);
@@ -973,20 +977,22 @@ __global__ void __launch_bounds__(256) k_pp32_phase1(PP32Planes P, KPP k)
         Fx = P.Fx[n];
         Fy = P.Fy[n];
     }
+    const float tau = P.tau[n];
+    float omb, bbp;
+    split1(fmaf(-k.bdt, rcp(fmaf(2.f, tau, k.dt)), 1.f), omb, bbp);
+    const float cbp = tau * bbp;
     Site s;
-    s.rho = (float)P.rho[n];
+    s.rho = bbp * (float)P.rho[n];   // feq = bBP f_eq
     s.U = pp32_U(P.f, 9, N)[n];
     s.V = pp32_V(P.f, 9, N)[n];
     s.Fx = Fx;
     s.Fy = Fy;
     float feq[9], fso[9];
     feq_src<L9, true>(k, s, feq, fso, [](int) { return true; });
-    const float tau = P.tau[n];
-    const float bbp = k.bdt * rcp(fmaf(2.f, tau, k.dt)), cbp = tau * bbp;
 #pragma unroll
     for (int q = 0; q < 9; ++q) {
         const float f = P.f[q * N + n];
-        P.fbp[q * N + n] = fmaf(cbp, fso[q], fmaf(bbp, feq[q] - f, f));
+        P.fbp[q * N + n] = fmaf(cbp, fso[q], fmaf(omb, f, feq[q]));
     }
 }
 
@@ -1047,7 +1053,7 @@ __global__ void __launch_bounds__(256) k_pp32_face_flux(PP32Planes P, KPP k)
     Site s;
     plane_force(k, P.fpsi + (dir ? N : 0), nx, ny, (int)(fi % nx), (int)(fi / nx), s.Fx, s.Fy);
     pp32_fv(P.f, 9, N)[t] = P.fpsi[t];
-    s.rho = P.frho[t];
+    s.rho = k.bh * P.frho[t];   // feq = bh f_eq
     s.U = P.fU[t];
     s.V = P.fV[t];
     float feq[9], fso[9];
@@ -1060,7 +1066,7 @@ __global__ void __launch_bounds__(256) k_pp32_face_flux(PP32Planes P, KPP k)
         float fl = 0.f;
         if (e_n != 0) {
             const float v = P.ffl[(size_t)q * 2 * N + t];
-            const float ff = fmaf(k.ch, fso[q], fmaf(k.bh, feq[q] - v, v));
+            const float ff = fmaf(k.ch, fso[q], fmaf(k.omb_h, v, feq[q]));
             fl = ff * (k.flux * (float)e_n);
         }
         P.ffl[(size_t)q * 2 * N + t] = fl;
